@@ -1,0 +1,124 @@
+// Internal declarations shared by the host plan builder, the kernels and the
+// C-ABI layer.  Not part of the public interface.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "compactles_hip.h"
+
+namespace cles {
+
+// ---------------------------------------------------------------------------
+// Errors
+// ---------------------------------------------------------------------------
+void set_error(const std::string &msg);
+struct Error {
+    int code;
+    std::string msg;
+};
+
+#define CLES_HIP(call)                                                              \
+    do {                                                                            \
+        hipError_t e_ = (call);                                                     \
+        if (e_ != hipSuccess)                                                       \
+            throw ::cles::Error{CLES_ERR_HIP, std::string(#call) + ": " +           \
+                                                   hipGetErrorString(e_)};          \
+    } while (0)
+
+// ---------------------------------------------------------------------------
+// Line-operator plan in device form.
+//
+// A line of n points is split into P = 2^p chunks of balanced length
+// (m or m+1).  Each chunk is factorised on its own (Thomas for q = 1,
+// unpivoted banded LU for q = 2, the algorithms of src/tridiag.jl:31-57 and
+// src/banded.jl:38-80 applied per chunk); the coupling between chunks is the
+// spike / reduced-interface system of src/tridiag.jl:1-22 with the chunks in
+// the role of ranks.  The reduced block-tridiagonal (cyclic when periodic)
+// system in the 2q interface unknowns per chunk is solved by block parallel
+// cyclic reduction whose coefficient blocks depend only on the scheme and are
+// precomputed here.  P = 1 is the reference's own single-rank algorithm.
+// ---------------------------------------------------------------------------
+enum RowField1 { R1_L = 0, R1_DINV, R1_C, R1_V, R1_W, R1_N };
+enum RowField2 { R2_L1 = 0, R2_L2, R2_U0INV, R2_U1, R2_U2, R2_V0, R2_V1, R2_W0, R2_W1, R2_N };
+
+struct LineDev {
+    int n;          // points per line
+    int q;          // LHS half-bandwidth
+    int M;          // RHS half-width
+    int sym;        // symmetric (filter) RHS
+    int nc;         // closure rows per closed edge
+    int periodic;
+    int closed;     // both edges closed (one rank, non-periodic)
+    int P;          // chunks per line (power of two)
+    int nsteps;     // log2(P)
+    int m;          // base chunk length
+    int extra;      // first `extra` chunks have length m + 1
+    const double *coef64;  // [0..8) ci, [8] a0, [16..80) clo[8][8], [80..144) chi[8][8]
+    const float *coef32;
+    const double *row64;   // per-row constants, [field * n + g]
+    const float *row32;
+    const double *pcr;     // per step s, chunk k: alpha, beta (B*B each, row-major);
+                           // then per chunk the final inverse block
+};
+
+constexpr int COEF_CI = 0, COEF_A0 = 8, COEF_CLO = 16, COEF_CHI = 80, COEF_N = 144;
+
+struct HostPlan {
+    LineDev dev{};
+    int dim = 1;
+    int64_t nloc[3] = {1, 1, 1};
+    int dtype = CLES_F64;
+    double h = 1.0;
+    void *dbuf = nullptr;   // one device allocation holding every table
+    size_t dbytes = 0;
+};
+
+// Builds the device tables for one (scheme, n, periodic, h).  Throws Error.
+void build_line_plan(HostPlan &hp, const cles_scheme_desc &s, int n, int periodic,
+                     double h, int max_chunk);
+void free_line_plan(HostPlan &hp);
+
+// ---------------------------------------------------------------------------
+// Array views: element (i, j, k) of the interior is at base[i*s0 + j*s1 + k*s2]
+// ---------------------------------------------------------------------------
+struct View3 {
+    int64_t n[3];
+    int64_t s[3];
+};
+
+// ---------------------------------------------------------------------------
+// Line-operator epilogues: what to do with the solved value v at element
+// offset o (offsets are shared by every array of one view).
+// ---------------------------------------------------------------------------
+enum EpiKind {
+    EPI_STORE = 0,      // out[o] = v
+    EPI_STORE_NEG,      // out[o] = -v
+    EPI_SUB,            // out[o] -= v
+    EPI_LSRK,           // dq = acc[o] - v; du = A du + dt dq; Q += B du   (FP64 state)
+    EPI_ART_MUBETA,     // rs = rho max(v,0); mu = C0 rs; beta = C1 rs
+    EPI_ART_KAPPA,      // kappa = C0 (rho c / max(T, eps)) max(v,0)
+    EPI_ART_D,          // D = C0 c max(v,0)
+    EPI_NKIND
+};
+
+struct EpiArgs {
+    int kind;
+    void *out;          // typed by the operator dtype (STORE/NEG/SUB/ART_*)
+    void *out2;         // beta (ART_MUBETA)
+    const void *a0;     // rho (ART_*), acc dQ (LSRK)
+    const void *a1;     // c (ART_KAPPA, ART_D)
+    const void *a2;     // T (ART_KAPPA)
+    double *du;         // LSRK
+    double *Q;          // LSRK
+    double c0, c1;      // coefficients (art) or A, B (LSRK)
+    double dt;          // LSRK
+};
+
+// Launch one line operator along `dim` (1..3) over the view.
+void launch_line(const HostPlan &hp, int dim, const View3 &v, const void *f,
+                 const EpiArgs &epi, hipStream_t st);
+
+}  // namespace cles

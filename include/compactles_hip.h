@@ -1,0 +1,221 @@
+/*
+ * compactles_hip.h -- C ABI of the MI355X (gfx950) backend for the CompactLES
+ * per-step hot path.
+ *
+ * Plain C types only (pointers, sizes, doubles); no HIP or torch types in any
+ * signature.  Streams are passed as `void*` (a hipStream_t, or NULL for the
+ * context's own stream).  Every entry point returns an int status (CLES_OK = 0)
+ * and leaves a message for cles_last_error() on failure.
+ *
+ * Each declaration names the reference interface it replaces
+ * (stillyslalom/CompactLES.jl, path:line).
+ */
+#ifndef COMPACTLES_HIP_H
+#define COMPACTLES_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CLES_ABI_VERSION 1
+
+/* Status codes.  The solver-failure codes mirror SolverFailure reasons
+ * (src/stepcontrol.jl:138-164): nonfinite, negative_density, planck, dt_min,
+ * dt_collapse. */
+enum {
+    CLES_OK = 0,
+    CLES_ERR_INVALID = 1,        /* bad argument (error(...) in the reference) */
+    CLES_ERR_TOO_SMALL = 2,      /* local extent too small for the scheme */
+    CLES_ERR_HIP = 3,            /* HIP runtime error */
+    CLES_ERR_NONFINITE = 4,
+    CLES_ERR_NEGATIVE_DENSITY = 5,
+    CLES_ERR_PLANCK = 6,
+    CLES_ERR_DT_MIN = 7,
+    CLES_ERR_DT_COLLAPSE = 8,
+    CLES_ERR_NOMEM = 9,
+    CLES_ERR_UNSUPPORTED = 10
+};
+
+/* Element types of operator inputs/outputs. */
+enum { CLES_F64 = 0, CLES_F32 = 1 };
+
+/* Boundary-condition codes (src/boundary.jl, src/problem.jl:192, src/nscbc.jl). */
+enum {
+    CLES_BC_PERIODIC = 0,    /* PeriodicBC            boundary.jl:24 */
+    CLES_BC_SLIPWALL = 1,    /* SlipWallBC            boundary.jl:32,232-243 */
+    CLES_BC_DIRICHLET = 2,   /* DirichletBC, constant state  problem.jl:192-207 */
+    CLES_BC_NSCBC_OUT = 3,   /* NSCBCOutflowBC        nscbc.jl:45-154 */
+    CLES_BC_NSCBC_IN = 4     /* NSCBCInflowBC, constant targets  nscbc.jl:204-297 */
+};
+
+#define CLES_MAXCL 8         /* closure rows per edge */
+#define CLES_MAXRHS 16       /* closure RHS stencil length */
+#define CLES_MAXM 8          /* interior RHS half-width */
+#define CLES_MAXSP 8         /* species */
+
+typedef struct cles_ctx cles_ctx;
+typedef struct cles_plan cles_plan;
+typedef struct cles_solver cles_solver;
+
+/* One compact scheme: the fields of CompactScheme (src/kernels.jl:43-50) and
+ * BandedCompactScheme (src/kernels_banded.jl:31-39).  q = LHS half-bandwidth
+ * (1: tridiagonal, 2: pentadiagonal); lhs[s-1] is the LHS coefficient at
+ * distance s; closure_lhs[j] holds the 2q+1 centred LHS entries of low-edge
+ * closure row j (for q = 1: sub, diag, super); closure_rhs[j][0..len-1] are
+ * weights on f[1..len] counted inward from the edge.  High-edge rows are
+ * mirrored as in the reference (operators.jl:77-83,87-92). */
+typedef struct {
+    int q;
+    double lhs[4];
+    double a0;
+    int M;
+    double coeffs[CLES_MAXM];
+    int symmetric;
+    int nclosure;
+    double closure_lhs[CLES_MAXCL][9];
+    int closure_rhs_len[CLES_MAXCL];
+    double closure_rhs[CLES_MAXCL][CLES_MAXRHS];
+} cles_scheme_desc;
+
+/* ----------------------------------------------------------------------- */
+/* Context, errors, device memory                                           */
+/* ----------------------------------------------------------------------- */
+
+/* Selects the HIP device and creates the context's stream.  No reference
+ * counterpart (the reference is CPU-only, src/CompactLES.jl:1-13). */
+int cles_ctx_create(int device, cles_ctx **ctx);
+int cles_ctx_destroy(cles_ctx *ctx);
+/* Thread-local message of the last failing call on this thread. */
+const char *cles_last_error(void);
+int cles_abi_version(void);
+/* The context's stream as a void* hipStream_t. */
+void *cles_ctx_stream(cles_ctx *ctx);
+
+/* Device buffers for hosts without a GPU array package (stand-ins for the
+ * reference's `field(decomp)` / `allocate_state`, decomposition.jl:129-134). */
+int cles_malloc(cles_ctx *ctx, size_t bytes, void **dptr);
+int cles_free(cles_ctx *ctx, void *dptr);
+int cles_memcpy_h2d(cles_ctx *ctx, void *dst, const void *src, size_t bytes);
+int cles_memcpy_d2h(cles_ctx *ctx, void *dst, const void *src, size_t bytes);
+int cles_memset(cles_ctx *ctx, void *dptr, int value, size_t bytes);
+int cles_synchronize(cles_ctx *ctx);
+
+/* ----------------------------------------------------------------------- */
+/* Operator plugin: plan_direction / apply_along!                           */
+/* ----------------------------------------------------------------------- */
+
+/* plan_direction(decomp, scheme, dim, h) -> DirPlan | BandPlan
+ * (src/operators.jl:44-101, src/operators_banded.jl:29-125), one rank:
+ * whole-domain lines.  dim is 1-based (1 = x).  n_local are the interior
+ * extents; `periodic` selects the cyclic system (otherwise both edges are
+ * closed and the closure rows are used).  h is the grid spacing (derivative
+ * RHS prescaled by 1/h).  dtype is CLES_F64 or CLES_F32.  Fails with
+ * CLES_ERR_TOO_SMALL on the reference's extent check (operators.jl:50-53). */
+int cles_plan_create(cles_ctx *ctx, const cles_scheme_desc *scheme, int dim,
+                     const int64_t n_local[3], int periodic, double h, int dtype,
+                     cles_plan **plan);
+int cles_plan_destroy(cles_plan *plan);
+
+/* apply_along!(out, plan, f, decomp) (src/operators.jl:199-215): applies the
+ * compact operator along the plan's dimension to every line of f and writes
+ * the interior of out.  f and out are device pointers to column-major arrays
+ * of extents padded[3] with interior offset halo[3] (Julia's padded `field`
+ * layout; pass halo = {0,0,0} for unpadded arrays).  Periodic lines wrap in
+ * the interior (equal to reading current halos); halos are never read or
+ * written.  out may equal f (in-place filtering).  Asynchronous on `stream`
+ * (NULL: the context stream). */
+int cles_apply_along(cles_plan *plan, void *out, const void *f,
+                     const int64_t padded[3], const int halo[3], void *stream);
+
+/* Times `iters` back-to-back apply_along launches with HIP events on the
+ * launch stream and returns the mean milliseconds per launch. */
+int cles_apply_along_timed(cles_plan *plan, void *out, const void *f,
+                           const int64_t padded[3], const int halo[3],
+                           int iters, double *ms_per_launch);
+
+/* ----------------------------------------------------------------------- */
+/* Solver: setup / compute_rhs! / step! / max_rate / filter_state! / run!   */
+/* ----------------------------------------------------------------------- */
+
+/* NSCBC face parameters (src/nscbc.jl:45-54 outflow, :204-216 inflow). */
+typedef struct {
+    double pinf, sigma, Lref, beta_t;      /* outflow; Lref <= 0: domain length */
+    double u[3], T, Y[CLES_MAXSP];         /* inflow targets */
+    double eta_u, eta_T, eta_t, eta_Y;
+} cles_nscbc_desc;
+
+/* Fields of Problem + Numerics (src/problem.jl:243-306) that reach the hot
+ * path, with the EOS as an IdealMixture species table (physics.jl:104-126),
+ * Transport (physics.jl:589-593), ArtParams (artificial.jl:50-57) and
+ * StepControl's floors and predictor (stepcontrol.jl:116-136). */
+typedef struct {
+    int64_t n_global[3];
+    double L_domain[3];
+    double origin[3];
+    int n_species;
+    double R[CLES_MAXSP], gamma[CLES_MAXSP];
+    double mu0, Pr, Sc;
+    int art_enabled;
+    double C_mu, C_beta, C_kappa, C_D;
+    int beta_sensor;                       /* 0 :strain, 1 :gated_strain, 2 :dilatation */
+    cles_scheme_desc deriv, filt;
+    double cfl;
+    int filter_interval;
+    int bc[3][2];                          /* CLES_BC_* per (dim, side) */
+    double bc_state[3][2][CLES_MAXSP + 4]; /* Dirichlet conserved state */
+    cles_nscbc_desc nscbc[3][2];
+    double predict, max_growth, dt_min, dt_min_ratio;
+    int precision;                         /* CLES_F64, or CLES_F32 = mixed form */
+} cles_solver_desc;
+
+/* setup(prob, num) -> Solver (src/problem.jl:321-345, src/rhs.jl:122-311),
+ * one rank.  Allocates the device-resident state and every scratch field. */
+int cles_solver_create(cles_ctx *ctx, const cles_solver_desc *desc, cles_solver **solver);
+int cles_solver_destroy(cles_solver *solver);
+
+/* Conserved state transfer in Julia's padded layout (Q[..., c], column-major,
+ * extents padded[3], interior offset halo[3], n_cons components). */
+int cles_solver_upload(cles_solver *solver, const double *Q_host,
+                       const int64_t padded[3], const int halo[3]);
+int cles_solver_download(cles_solver *solver, double *Q_host,
+                         const int64_t padded[3], const int halo[3]);
+/* Download a named internal field's interior (as double) into an unpadded
+ * host array: "rho","u","v","w","p","T","c","cp","mu_art","beta_art",
+ * "kappa_art","dQ[c]","grad_u[dj]","grad_T[d]","Y[k]","D_art[k]" (1-based). */
+int cles_solver_get_field(cles_solver *solver, const char *name, double *host);
+
+int cles_solver_get_time(cles_solver *solver, double *t, int64_t *step);
+int cles_solver_set_time(cles_solver *solver, double t, int64_t step);
+
+/* apply_bcs!(solver, Q) (src/boundary.jl:280-286) at the current stage time. */
+int cles_apply_bcs(cles_solver *solver);
+/* compute_rhs!(solver, Q, dQ, primitives_current) (src/rhs.jl:624-694). */
+int cles_compute_rhs(cles_solver *solver, int primitives_current);
+/* max_rate(solver, Q) -> (rate, rho_min) (src/timestep.jl:125-175). */
+int cles_max_rate(cles_solver *solver, double *rate, double *rho_min);
+/* step!(solver, Q, dQ, du, dt, prepared) (src/timestep.jl:72-99). */
+int cles_step(cles_solver *solver, double dt, int prepared);
+/* filter_state!(solver, Q) (src/timestep.jl:414-426). */
+int cles_filter_state(cles_solver *solver);
+/* run!(solver, Q; tfinal, nmax) (src/timestep.jl:277-400) with a fresh
+ * Workspace, no callbacks and no retries.  Returns CLES_OK or a solver-failure
+ * code (state left at the failing step).  dts (may be NULL) receives up to
+ * ndts step sizes. */
+int cles_run(cles_solver *solver, double tfinal, int64_t nmax, double *dts, int64_t ndts);
+/* Kinetic energy ∫½|ρu|²/ρ dV (examples/taylor_green.jl:51-60) and enstrophy
+ * ∫½|ω|² dV with ω from the compact grad_u of the current state
+ * (src/io.jl:332-340), both reduced in FP64 in a fixed order
+ * (src/diagnostics.jl:61-80 volume_integral). */
+int cles_diagnostics(cles_solver *solver, double *ke, double *enstrophy);
+
+/* Device-resident state geometry: interior extents and per-field element
+ * count (the solver stores unpadded column-major fields). */
+int cles_solver_shape(cles_solver *solver, int64_t n[3], int *n_cons);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
