@@ -65,6 +65,7 @@ SIGNATURES = {
     "cles_memcpy_d2h": (C.c_int, [P, P, P, C.c_size_t]),
     "cles_memset": (C.c_int, [P, P, C.c_int, C.c_size_t]),
     "cles_synchronize": (C.c_int, [P]),
+    "cles_memcpy_d2d": (C.c_int, [P, P, P, C.c_size_t]),
     "cles_plan_create": (C.c_int, [P, C.POINTER(SchemeDesc), C.c_int, C.POINTER(C.c_int64), C.c_int,
                                    C.c_double, C.c_int, PP]),
     "cles_plan_destroy": (C.c_int, [P]),
@@ -85,6 +86,8 @@ SIGNATURES = {
     "cles_run": (C.c_int, [P, C.c_double, C.c_int64, DP, C.c_int64]),
     "cles_diagnostics": (C.c_int, [P, DP, DP]),
     "cles_solver_shape": (C.c_int, [P, C.POINTER(C.c_int64), C.POINTER(C.c_int)]),
+    "cles_solver_profile": (C.c_int, [P, C.c_int]),
+    "cles_solver_profile_read": (C.c_int, [P, C.c_int, C.c_char_p, C.c_int, DP, C.POINTER(C.c_int64), DP]),
 }
 
 

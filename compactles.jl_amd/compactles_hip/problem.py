@@ -1,0 +1,440 @@
+"""Host-side mirror of the CompactLES frontend (src/problem.jl, src/physics.jl,
+src/artificial.jl, src/stepcontrol.jl, src/boundary.jl, src/nscbc.jl,
+src/timestep.jl): the same Problem / Numerics / setup / run! vocabulary, with
+the solver state resident on the GPU behind the C ABI.
+
+Differences from the Julia frontend, all forced by running the physics on the
+device: initial-condition functions are evaluated vectorised (they receive
+numpy coordinate arrays), Dirichlet faces hold a constant state (their
+function is evaluated once, at t = 0, and must be uniform over the face), and
+NSCBC inflow targets are constants.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+from dataclasses import dataclass, field
+from typing import Callable, Sequence, Tuple
+
+import numpy as np
+
+from . import _lib as L
+from .device import Context
+from .schemes import AbstractCompactScheme, compact_filter, lele_d1_6
+
+
+# --------------------------------------------------------------------------
+# EOS and transport (physics.jl)
+# --------------------------------------------------------------------------
+@dataclass
+class IdealSpecies:
+    """physics.jl:86-90."""
+    name: str
+    R: float
+    gamma: float
+
+
+@dataclass
+class IdealMixture:
+    """physics.jl:104-115: mass-fraction averaged ideal mixture."""
+    sp: Sequence[IdealSpecies]
+
+    @property
+    def Rk(self):
+        return [s.R for s in self.sp]
+
+    @property
+    def cvk(self):
+        return [s.R / (s.gamma - 1) for s in self.sp]
+
+    def nspecies(self):
+        return len(self.sp)
+
+
+def single_species(gamma: float = 1.4, R: float = 1.0, name: str = "gas") -> IdealMixture:
+    """physics.jl:125-126."""
+    return IdealMixture([IdealSpecies(name, float(R), float(gamma))])
+
+
+@dataclass
+class Transport:
+    """physics.jl:589-593."""
+    mu0: float = 0.0
+    Pr: float = 0.7
+    Sc: float = 0.7
+
+
+@dataclass
+class ArtParams:
+    """artificial.jl:50-57."""
+    enabled: bool = True
+    C_mu: float = 0.002
+    C_beta: float = 1.0
+    C_kappa: float = 0.01
+    C_D: float = 0.01
+    beta_sensor: str = "strain"
+
+
+@dataclass
+class StepControl:
+    """stepcontrol.jl:116-136 (prediction and floors; retries are not offered)."""
+    predict: float = 0.0
+    max_growth: float = 0.0
+    landing_steps: int = 2
+    dt_min: float = 0.0
+    dt_min_ratio: float = 1e-8
+
+
+# --------------------------------------------------------------------------
+# Primitive states (problem.jl:41-90)
+# --------------------------------------------------------------------------
+class Prim:
+    """problem.jl:41-57: exactly two of p, rho, T_ion; the EOS derives the third.
+    Entries may be numpy arrays (vectorised initial conditions)."""
+
+    def __init__(self, u=(0.0, 0.0, 0.0), p=math.nan, T_ion=math.nan, rho=math.nan, Y=(1.0,)):
+        given = [not _isnan(x) for x in (p, T_ion, rho)]
+        if sum(given) != 2:
+            raise ValueError("Prim: specify exactly two of p, rho, and T_ion; the EOS derives the third")
+        Ysum = sum(np.asarray(y, dtype=float) for y in Y)
+        if np.max(np.abs(np.asarray(Ysum) - 1.0)) >= 1e-10:
+            raise ValueError("Prim: mass fractions must sum to 1")
+        self.u = tuple(u)
+        self.p, self.T_ion, self.rho = p, T_ion, rho
+        self.Y = tuple(Y)
+
+
+def _isnan(x):
+    a = np.asarray(x, dtype=float)
+    return a.size == 1 and bool(np.isnan(a).all())
+
+
+def conserved_from_prim(eos: IdealMixture, pr: Prim):
+    """problem.jl:76-90 for an ideal mixture: (rho Y_k..., rho u, rho v, rho w, rho E)."""
+    N = eos.nspecies()
+    if len(pr.Y) != N:
+        raise ValueError(f"Prim carries {len(pr.Y)} mass fractions; EOS has {N} species")
+    Rm = 0.0
+    cvm = 0.0
+    for k in range(N):
+        Rm = Rm + pr.Y[k] * eos.Rk[k]
+        cvm = cvm + pr.Y[k] * eos.cvk[k]
+    rho = pr.p / (Rm * pr.T_ion) if _isnan(pr.rho) else pr.rho
+    T = pr.p / (rho * Rm) if _isnan(pr.T_ion) else pr.T_ion
+    u1, u2, u3 = pr.u
+    ke = 0.5 * (u1 * u1 + u2 * u2 + u3 * u3)
+    out = [rho * pr.Y[k] for k in range(N)]
+    out += [rho * u1, rho * u2, rho * u3, rho * (cvm * T + ke)]
+    return out
+
+
+def tanh_blend(x, x0, delta):
+    """problem.jl:172."""
+    return 0.5 * (1 + np.tanh((x - x0) / delta))
+
+
+# --------------------------------------------------------------------------
+# Boundary conditions (boundary.jl, problem.jl:192-207, nscbc.jl)
+# --------------------------------------------------------------------------
+class BoundaryCondition:
+    code = None
+
+
+class PeriodicBC(BoundaryCondition):
+    code = L.BC_PERIODIC
+
+
+class SlipWallBC(BoundaryCondition):
+    code = L.BC_SLIPWALL
+
+
+class DirichletBC(BoundaryCondition):
+    """Hard prescription of the full state on a face from fun(x1,x2,x3,t) -> Prim,
+    held constant (evaluated at t = 0)."""
+    code = L.BC_DIRICHLET
+
+    def __init__(self, fun: Callable):
+        self.fun = fun
+
+
+@dataclass
+class NSCBCOutflowBC(BoundaryCondition):
+    """nscbc.jl:45-54."""
+    pinf: float
+    sigma: float = 0.25
+    Lref: float = 0.0
+    beta_t: float = -1.0
+    code = L.BC_NSCBC_OUT
+
+
+@dataclass
+class NSCBCInflowBC(BoundaryCondition):
+    """nscbc.jl:204-216 (constant targets)."""
+    u: Tuple[float, float, float]
+    T_ion: float
+    Y: Sequence[float] = (1.0,)
+    eta_u: float = 0.28
+    eta_T: float = 0.28
+    eta_t: float = 0.28
+    eta_Y: float = 0.28
+    Lref: float = 0.0
+    code = L.BC_NSCBC_IN
+
+
+# --------------------------------------------------------------------------
+# Problem / Numerics (problem.jl:243-306)
+# --------------------------------------------------------------------------
+@dataclass
+class Problem:
+    domain: Sequence[Tuple[float, float]]
+    bcs: Sequence[Tuple[BoundaryCondition, BoundaryCondition]]
+    ic: Callable
+    name: str = "problem"
+    eos: IdealMixture = field(default_factory=single_species)
+    transport: Transport = field(default_factory=Transport)
+    metric: str = "cartesian"
+    sources: tuple = ()
+
+
+@dataclass
+class Numerics:
+    n_global: Tuple[int, int, int]
+    deriv: AbstractCompactScheme = field(default_factory=lele_d1_6)
+    filt: AbstractCompactScheme = field(default_factory=lambda: compact_filter(0.45))
+    art: ArtParams = field(default_factory=ArtParams)
+    cfl: float = 0.5
+    control: StepControl = field(default_factory=StepControl)
+    filter_interval: int = 1
+    dims: object = None
+    n_halo: int = 4
+    stretch: object = None
+    precision: str = "fp64"     # "fp64" (the drop-in) or "fp32" (mixed form)
+
+
+_BETA = {"strain": 0, "gated_strain": 1, "dilatation": 2}
+
+
+class Solver:
+    """Device-resident solver (the reference's Solver + Workspace)."""
+
+    def __init__(self, ctx: Context, prob: Problem, num: Numerics):
+        if prob.metric != "cartesian":
+            raise ValueError("only CartesianMetric is available on the GPU backend")
+        if num.stretch is not None:
+            raise ValueError("stretched grids are not available on the GPU backend")
+        if prob.sources:
+            raise ValueError("explicit sources are not available on the GPU backend")
+        self.ctx = ctx
+        self.prob, self.num = prob, num
+        self.eos = prob.eos
+        self.n = tuple(int(x) for x in num.n_global)
+        self.origin = tuple(float(prob.domain[d][0]) for d in range(3))
+        self.L = tuple(float(prob.domain[d][1] - prob.domain[d][0]) for d in range(3))
+        if not all(x > 0 for x in self.L):
+            raise ValueError("domain extents must be positive")
+        self.n_species = self.eos.nspecies()
+        self.n_cons = self.n_species + 4
+        self.periodic = tuple(isinstance(prob.bcs[d][0], PeriodicBC) or self.n[d] == 1 for d in range(3))
+        self.h = tuple(1.0 if self.n[d] == 1 else (self.L[d] / self.n[d] if self.periodic[d]
+                                                   else self.L[d] / (self.n[d] - 1)) for d in range(3))
+        self.halo = tuple(num.n_halo if self.n[d] > 1 else 0 for d in range(3))
+        self.padded = tuple(self.n[d] + 2 * self.halo[d] for d in range(3))
+        desc = self._desc()
+        h = C.c_void_p()
+        L.check(ctx.lib.cles_solver_create(ctx.handle, C.byref(desc), C.byref(h)))
+        self.handle = h
+        self._desc_keep = desc
+
+    def _desc(self) -> L.SolverDesc:
+        prob, num = self.prob, self.num
+        d = L.SolverDesc()
+        for k in range(3):
+            d.n_global[k] = self.n[k]
+            d.L_domain[k] = self.L[k]
+            d.origin[k] = self.origin[k]
+        d.n_species = self.n_species
+        for k, sp in enumerate(self.eos.sp):
+            d.R[k] = sp.R
+            d.gamma[k] = sp.gamma
+        tr = prob.transport
+        d.mu0, d.Pr, d.Sc = tr.mu0, tr.Pr, tr.Sc
+        art = num.art
+        d.art_enabled = int(art.enabled)
+        d.C_mu, d.C_beta, d.C_kappa, d.C_D = art.C_mu, art.C_beta, art.C_kappa, art.C_D
+        if art.beta_sensor not in _BETA:
+            raise ValueError(f"art.beta_sensor must be :strain, :gated_strain or :dilatation, got :{art.beta_sensor}")
+        d.beta_sensor = _BETA[art.beta_sensor]
+        d.deriv = num.deriv.to_desc()
+        d.filt = num.filt.to_desc()
+        d.cfl = num.cfl
+        d.filter_interval = num.filter_interval
+        ctl = num.control
+        d.predict, d.max_growth, d.dt_min, d.dt_min_ratio = ctl.predict, ctl.max_growth, ctl.dt_min, ctl.dt_min_ratio
+        d.precision = L.F64 if num.precision == "fp64" else L.F32
+        for k in range(3):
+            for s in range(2):
+                bc = prob.bcs[k][s]
+                if bc.code is None:
+                    raise ValueError(f"unsupported boundary condition {type(bc).__name__}")
+                d.bc[k][s] = bc.code
+                if isinstance(bc, DirichletBC):
+                    q = self._dirichlet_state(bc, k, s)
+                    for c, v in enumerate(q):
+                        d.bc_state[k][s][c] = v
+                elif isinstance(bc, NSCBCOutflowBC):
+                    f = d.nscbc[k][s]
+                    f.pinf, f.sigma, f.Lref, f.beta_t = bc.pinf, bc.sigma, bc.Lref, bc.beta_t
+                elif isinstance(bc, NSCBCInflowBC):
+                    f = d.nscbc[k][s]
+                    for i in range(3):
+                        f.u[i] = bc.u[i]
+                    f.T = bc.T_ion
+                    if len(bc.Y) != self.n_species:
+                        raise ValueError("NSCBCInflowBC: target composition does not match the EOS")
+                    for i, y in enumerate(bc.Y):
+                        f.Y[i] = y
+                    f.eta_u, f.eta_T, f.eta_t, f.eta_Y, f.Lref = bc.eta_u, bc.eta_T, bc.eta_t, bc.eta_Y, bc.Lref
+        return d
+
+    def xcoord(self, d: int) -> np.ndarray:
+        """rhs.jl:320-335: node coordinates along dimension d (0-based here)."""
+        return self.origin[d] + np.arange(self.n[d]) * self.h[d]
+
+    def _dirichlet_state(self, bc: DirichletBC, d: int, side: int):
+        xs = [self.xcoord(k) for k in range(3)]
+        xs[d] = xs[d][:1] if side == 0 else xs[d][-1:]
+        X, Y, Z = np.meshgrid(*xs, indexing="ij")
+        q = [np.broadcast_to(np.asarray(v, dtype=float), X.shape)
+             for v in conserved_from_prim(self.eos, bc.fun(X, Y, Z, 0.0))]
+        out = []
+        for v in q:
+            if np.ptp(v) != 0.0:
+                raise ValueError("DirichletBC on the GPU backend needs a state that is uniform over the face")
+            out.append(float(v.flat[0]))
+        return out
+
+    # ---- state transfer -------------------------------------------------------
+    def allocate_state(self) -> np.ndarray:
+        """decomposition.jl:132: a zeroed padded host state Q[..., c] (Fortran order)."""
+        return np.zeros(self.padded + (self.n_cons,), order="F")
+
+    def upload(self, Q: np.ndarray):
+        Q = np.asfortranarray(Q, dtype=np.float64)
+        L.check(self.ctx.lib.cles_solver_upload(self.handle, Q.ctypes.data_as(L.DP), L.I64_3(*self.padded),
+                                                L.INT_3(*self.halo)))
+
+    def download(self, Q: np.ndarray):
+        assert Q.flags.f_contiguous and Q.dtype == np.float64 and Q.shape == self.padded + (self.n_cons,)
+        L.check(self.ctx.lib.cles_solver_download(self.handle, Q.ctypes.data_as(L.DP), L.I64_3(*self.padded),
+                                                  L.INT_3(*self.halo)))
+        return Q
+
+    def field(self, name: str) -> np.ndarray:
+        """Interior of a named device field as a float64 array of shape n."""
+        out = np.empty(self.n, order="F")
+        L.check(self.ctx.lib.cles_solver_get_field(self.handle, name.encode(), out.ctypes.data_as(L.DP)))
+        return out
+
+    # ---- time ----------------------------------------------------------------
+    @property
+    def t(self):
+        t, st = C.c_double(), C.c_int64()
+        L.check(self.ctx.lib.cles_solver_get_time(self.handle, C.byref(t), C.byref(st)))
+        return t.value
+
+    @property
+    def step_count(self):
+        t, st = C.c_double(), C.c_int64()
+        L.check(self.ctx.lib.cles_solver_get_time(self.handle, C.byref(t), C.byref(st)))
+        return st.value
+
+    def set_time(self, t: float, step: int):
+        L.check(self.ctx.lib.cles_solver_set_time(self.handle, t, step))
+
+    # ---- the per-step path -----------------------------------------------------
+    def apply_bcs(self):
+        L.check(self.ctx.lib.cles_apply_bcs(self.handle))
+
+    def compute_rhs(self, primitives_current=False):
+        L.check(self.ctx.lib.cles_compute_rhs(self.handle, int(primitives_current)))
+
+    def max_rate(self):
+        r, m = C.c_double(), C.c_double()
+        L.check(self.ctx.lib.cles_max_rate(self.handle, C.byref(r), C.byref(m)))
+        return r.value, m.value
+
+    def step(self, dt, prepared=False):
+        L.check(self.ctx.lib.cles_step(self.handle, float(dt), int(prepared)))
+
+    def filter_state(self):
+        L.check(self.ctx.lib.cles_filter_state(self.handle))
+
+    def run(self, tfinal, nmax=2**62, record=0):
+        dts = (C.c_double * max(record, 1))()
+        L.check(self.ctx.lib.cles_run(self.handle, float(tfinal), int(nmax), dts, int(record)))
+        return np.array(dts[:record])
+
+    def diagnostics(self):
+        ke, en = C.c_double(), C.c_double()
+        L.check(self.ctx.lib.cles_diagnostics(self.handle, C.byref(ke), C.byref(en)))
+        return ke.value, en.value
+
+    def profile(self, enable: bool = True):
+        """Start (fresh accumulation) or stop per-phase event timing."""
+        L.check(self.ctx.lib.cles_solver_profile(self.handle, int(enable)))
+
+    def profile_read(self):
+        """{phase: (ms_total, launches, algorithmic_bytes)} for phases that ran."""
+        out = {}
+        name = C.create_string_buffer(64)
+        ms, n, by = C.c_double(), C.c_int64(), C.c_double()
+        i = 0
+        while self.ctx.lib.cles_solver_profile_read(self.handle, i, name, 64, C.byref(ms), C.byref(n),
+                                                    C.byref(by)) == L.OK:
+            if n.value:
+                out[name.value.decode()] = (ms.value, n.value, by.value)
+            i += 1
+        return out
+
+    def close(self):
+        if getattr(self, "handle", None):
+            self.ctx.lib.cles_solver_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def initialize_(solver: Solver, Q: np.ndarray, ic: Callable) -> np.ndarray:
+    """problem.jl:146-163 initialize!: fills the interior of Q from ic(x, y, z) -> Prim
+    (vectorised over the interior nodes); halos are left unchanged."""
+    X, Y, Z = np.meshgrid(*(solver.xcoord(d) for d in range(3)), indexing="ij")
+    q = conserved_from_prim(solver.eos, ic(X, Y, Z))
+    sl = tuple(slice(solver.halo[d], solver.halo[d] + solver.n[d]) for d in range(3))
+    for c in range(solver.n_cons):
+        Q[sl + (c,)] = np.broadcast_to(q[c], X.shape)
+    return Q
+
+
+def setup(prob: Problem, num: Numerics, ctx: Context | None = None):
+    """problem.jl:321-345 setup(prob, num) -> (solver, Q); Q is also uploaded."""
+    ctx = ctx or Context(0)
+    solver = Solver(ctx, prob, num)
+    Q = solver.allocate_state()
+    initialize_(solver, Q, prob.ic)
+    solver.upload(Q)
+    return solver, Q
+
+
+def run_(solver: Solver, Q: np.ndarray, tfinal: float, nmax: int = 2**62, upload: bool = False):
+    """timestep.jl:402-405 run!(solver, Q; tfinal, nmax): advances the device
+    state and writes the result back into Q.  upload=True first copies Q to the
+    device (after a host-side modification)."""
+    if upload:
+        solver.upload(Q)
+    solver.run(tfinal, nmax)
+    solver.download(Q)
+    return Q

@@ -4,16 +4,7 @@
 #include <string>
 
 #include "cles_internal.h"
-
-struct cles_ctx {
-    int device = 0;
-    hipStream_t stream = nullptr;
-};
-
-struct cles_plan {
-    cles_ctx *ctx = nullptr;
-    cles::HostPlan hp;
-};
+#include "cles_point.h"
 
 namespace cles {
 
@@ -21,27 +12,20 @@ static thread_local std::string g_last_error;
 
 void set_error(const std::string &msg) { g_last_error = msg; }
 
-}  // namespace cles
+void ctx_retain(cles_ctx *ctx) { ctx->refs++; }
 
-#define CLES_TRY(...)                                                      \
-    try {                                                                   \
-        __VA_ARGS__;                                                        \
-        return CLES_OK;                                                     \
-    } catch (const cles::Error &e) {                                        \
-        cles::set_error(e.msg);                                             \
-        return e.code;                                                      \
-    } catch (const std::bad_alloc &) {                                      \
-        cles::set_error("host allocation failed");                          \
-        return CLES_ERR_NOMEM;                                              \
-    } catch (const std::exception &e) {                                     \
-        cles::set_error(e.what());                                          \
-        return CLES_ERR_INVALID;                                            \
+void ctx_release(cles_ctx *ctx)
+{
+    if (--ctx->refs > 0) return;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) {
+        (void)hipStreamSynchronize(ctx->stream);
+        (void)hipStreamDestroy(ctx->stream);
     }
+    delete ctx;
+}
 
-#define CLES_REQUIRE(cond, msg)                                             \
-    do {                                                                    \
-        if (!(cond)) throw cles::Error{CLES_ERR_INVALID, msg};              \
-    } while (0)
+}  // namespace cles
 
 extern "C" {
 
@@ -77,10 +61,7 @@ int cles_ctx_destroy(cles_ctx *ctx)
 {
     CLES_TRY({
         if (!ctx) return CLES_OK;
-        (void)hipSetDevice(ctx->device);
-        if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
-        if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
-        delete ctx;
+        cles::ctx_release(ctx);   // deferred while plans or solvers still use it
     })
 }
 
@@ -136,6 +117,15 @@ int cles_memset(cles_ctx *ctx, void *dptr, int value, size_t bytes)
     })
 }
 
+int cles_memcpy_d2d(cles_ctx *ctx, void *dst, const void *src, size_t bytes)
+{
+    CLES_TRY({
+        CLES_REQUIRE(ctx && dst && src, "NULL argument");
+        cles::launch_copy(dst, src, bytes, ctx->stream);
+        CLES_HIP(hipStreamSynchronize(ctx->stream));
+    })
+}
+
 int cles_synchronize(cles_ctx *ctx)
 {
     CLES_TRY({
@@ -162,11 +152,12 @@ int cles_plan_create(cles_ctx *ctx, const cles_scheme_desc *scheme, int dim, con
         p->hp.dtype = dtype;
         for (int d = 0; d < 3; d++) p->hp.nloc[d] = n_local[d];
         try {
-            cles::build_line_plan(p->hp, *scheme, (int)n_local[dim - 1], periodic, h, 32);
+            cles::build_line_plan(p->hp, *scheme, (int)n_local[dim - 1], periodic, h, dim == 1 ? 16 : 32);
         } catch (...) {
             delete p;
             throw;
         }
+        cles::ctx_retain(ctx);
         *out = p;
     })
 }
@@ -177,7 +168,9 @@ int cles_plan_destroy(cles_plan *plan)
         if (!plan) return CLES_OK;
         (void)hipStreamSynchronize(plan->ctx->stream);
         cles::free_line_plan(plan->hp);
+        cles_ctx *ctx = plan->ctx;
         delete plan;
+        cles::ctx_release(ctx);
     })
 }
 

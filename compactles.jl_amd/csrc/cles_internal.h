@@ -4,6 +4,8 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <new>
+#include <stdexcept>
 #include <string>
 #include <vector>
 
@@ -122,3 +124,39 @@ void launch_line(const HostPlan &hp, int dim, const View3 &v, const void *f,
                  const EpiArgs &epi, hipStream_t st);
 
 }  // namespace cles
+
+struct cles_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int refs = 1;            // the owner's reference plus one per live plan / solver
+};
+
+namespace cles {
+void ctx_retain(cles_ctx *ctx);
+void ctx_release(cles_ctx *ctx);   // frees the context when the last reference goes
+}  // namespace cles
+
+struct cles_plan {
+    cles_ctx *ctx = nullptr;
+    cles::HostPlan hp;
+};
+
+#define CLES_TRY(...)                                                       \
+    try {                                                                   \
+        __VA_ARGS__;                                                        \
+        return CLES_OK;                                                     \
+    } catch (const cles::Error &e) {                                        \
+        cles::set_error(e.msg);                                             \
+        return e.code;                                                      \
+    } catch (const std::bad_alloc &) {                                      \
+        cles::set_error("host allocation failed");                          \
+        return CLES_ERR_NOMEM;                                              \
+    } catch (const std::exception &e) {                                     \
+        cles::set_error(e.what());                                          \
+        return CLES_ERR_INVALID;                                            \
+    }
+
+#define CLES_REQUIRE(cond, msg)                                             \
+    do {                                                                    \
+        if (!(cond)) throw cles::Error{CLES_ERR_INVALID, msg};              \
+    } while (0)

@@ -1,0 +1,685 @@
+// Compact line-operator kernels for gfx950 (wave64).
+//
+// One thread owns one chunk (<= MC points) of one line and keeps it in
+// registers: it forms the compact RHS from a register window of f
+// (src/operators.jl:113-174 fill), runs the chunk-local Thomas / banded LU
+// sweeps (src/tridiag.jl:46-57, src/banded.jl:63-80), exchanges its 2q end
+// values with the other chunks of the same line through LDS, solves the
+// reduced interface system by block parallel cyclic reduction (coefficients
+// precomputed by the plan), and applies the spike correction
+// x = y - V tail_prev - W head_next (src/tridiag.jl:159-168,
+// src/banded.jl:371-382) before handing each value to a fused epilogue.
+//
+//  * y/z sweeps: 64 (or fewer) consecutive x-lanes per chunk row; every load
+//    and store of a row is a contiguous 512-byte segment; no transposes.
+//  * x sweep: the block stages Lt whole lines in LDS with coalesced loads,
+//    each thread reads its chunk window from LDS, results return through the
+//    same tile for coalesced stores.
+// No MFMA: the work is banded and HBM-bound.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "cles_internal.h"
+
+namespace cles {
+
+constexpr int MC = 32;     // maximum chunk length (points per thread)
+constexpr int NCMAX = 4;   // closure rows per edge handled by the kernels
+
+template <typename T> __device__ __forceinline__ const T *coefp(const LineDev &p);
+template <> __device__ __forceinline__ const double *coefp<double>(const LineDev &p) { return p.coef64; }
+template <> __device__ __forceinline__ const float *coefp<float>(const LineDev &p) { return p.coef32; }
+template <typename T> __device__ __forceinline__ const T *rowp(const LineDev &p);
+template <> __device__ __forceinline__ const double *rowp<double>(const LineDev &p) { return p.row64; }
+template <> __device__ __forceinline__ const float *rowp<float>(const LineDev &p) { return p.row32; }
+
+// ---------------------------------------------------------------------------
+// Epilogues
+// ---------------------------------------------------------------------------
+template <typename T, int EPI>
+__device__ __forceinline__ void epilogue(const EpiArgs &e, int64_t o, double v)
+{
+    if constexpr (EPI == EPI_STORE) {
+        ((T *)e.out)[o] = (T)v;
+    } else if constexpr (EPI == EPI_STORE_NEG) {
+        ((T *)e.out)[o] = (T)(-v);
+    } else if constexpr (EPI == EPI_SUB) {
+        T *out = (T *)e.out;
+        out[o] = out[o] - (T)v;
+    } else if constexpr (EPI == EPI_LSRK) {
+        // rhs.jl:665-667 dQ -= D_z F_z, then timestep.jl:85-94 on the FP64 state
+        // acc == NULL: this is the only active dimension, dQ = 0 - v
+        const T *acc = (const T *)e.a0;
+        const double dq = acc ? (double)(acc[o] - (T)v) : -v;
+        double nv = e.c0 * e.du[o] + e.dt * dq;
+        e.du[o] = nv;
+        e.Q[o] += e.c1 * nv;
+    } else if constexpr (EPI == EPI_ART_MUBETA) {
+        // artificial.jl:296-304
+        const T *rho = (const T *)e.a0;
+        T s = (T)v;
+        T rs = rho[o] * (s > T(0) ? s : T(0));
+        ((T *)e.out)[o] = (T)e.c0 * rs;
+        ((T *)e.out2)[o] = (T)e.c1 * rs;
+    } else if constexpr (EPI == EPI_ART_KAPPA) {
+        // artificial.jl:331-339 with physics.jl:155-156
+        const T *rho = (const T *)e.a0, *c = (const T *)e.a1, *Tt = (const T *)e.a2;
+        T s = (T)v;
+        T tt = Tt[o];
+        const T eps = sizeof(T) == 8 ? (T)2.220446049250313e-16 : (T)1.1920929e-07f;
+        T scale = rho[o] * c[o] / (tt > eps ? tt : eps);
+        ((T *)e.out)[o] = (T)e.c0 * scale * (s > T(0) ? s : T(0));
+    } else if constexpr (EPI == EPI_ART_D) {
+        // artificial.jl:349-355
+        const T *c = (const T *)e.a1;
+        T s = (T)v;
+        ((T *)e.out)[o] = (T)e.c0 * c[o] * (s > T(0) ? s : T(0));
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Per-thread chunk machinery
+// ---------------------------------------------------------------------------
+template <typename T, int Q, int M, int SYM, int CL = MC>
+struct Chunk {
+    static constexpr int B = 2 * Q;
+    T d[CL];
+
+    // RHS of every row of the chunk from the window fv[t + M] = f[s + t]
+    // (closure rows from fv (low edge, s = 0) and fe[kap] = f[n-1-kap]).
+    __device__ __forceinline__ void rhs(const LineDev &p, int k, int len, const T (&fv)[CL + 2 * M],
+                                        const T (&fe)[8])
+    {
+        const T *cf = coefp<T>(p);
+        T ci[M];
+#pragma unroll
+        for (int m = 0; m < M; m++) ci[m] = cf[COEF_CI + m];
+        const T a0 = cf[COEF_A0];
+        const bool lo = p.closed && k == 0;
+        const bool hi = p.closed && k == p.P - 1;
+#pragma unroll
+        for (int t = 0; t < CL; t++) {
+            T acc;
+            if constexpr (SYM) {
+                acc = a0 * fv[t + M];
+#pragma unroll
+                for (int m = 1; m <= M; m++) acc += ci[m - 1] * (fv[t + M + m] + fv[t + M - m]);
+            } else {
+                acc = T(0);
+#pragma unroll
+                for (int m = 1; m <= M; m++) acc += ci[m - 1] * (fv[t + M + m] - fv[t + M - m]);
+            }
+            if (t < NCMAX) {
+                if (lo && t < p.nc) {
+                    acc = T(0);
+#pragma unroll
+                    for (int kap = 0; kap < 8; kap++) acc += cf[COEF_CLO + t * 8 + kap] * fv[kap + M];
+                }
+            }
+            if (hi) {
+                const int jr = len - 1 - t;
+                if (jr >= 0 && jr < p.nc) {
+                    acc = T(0);
+#pragma unroll
+                    for (int kap = 0; kap < 8; kap++) acc += cf[COEF_CHI + jr * 8 + kap] * fe[kap];
+                }
+            }
+            d[t] = acc;
+        }
+    }
+
+    // chunk-local forward/backward sweeps
+    __device__ __forceinline__ void solve(const LineDev &p, int rs, int len)
+    {
+        const int n = p.n;
+        const T *rw = rowp<T>(p);
+        if constexpr (Q == 1) {
+            const T *rl = rw + (size_t)R1_L * n + rs;
+            const T *rdi = rw + (size_t)R1_DINV * n + rs;
+            const T *rc = rw + (size_t)R1_C * n + rs;
+#pragma unroll
+            for (int t = 1; t < CL; t++)
+                if (t < len) d[t] -= rl[t] * d[t - 1];
+#pragma unroll
+            for (int t = CL - 1; t >= 0; t--) {
+                if (t < len) {
+                    const T nxt = d[t + 1 < CL ? t + 1 : CL - 1];
+                    d[t] = (t == len - 1) ? d[t] * rdi[t] : (d[t] - rc[t] * nxt) * rdi[t];
+                }
+            }
+        } else {
+            const T *l1 = rw + (size_t)R2_L1 * n + rs;
+            const T *l2 = rw + (size_t)R2_L2 * n + rs;
+            const T *u0 = rw + (size_t)R2_U0INV * n + rs;
+            const T *u1 = rw + (size_t)R2_U1 * n + rs;
+            const T *u2 = rw + (size_t)R2_U2 * n + rs;
+#pragma unroll
+            for (int t = 0; t < CL - 1; t++) {
+                if (t < len - 1) {
+                    const T xt = d[t];
+                    d[t + 1] -= l1[t] * xt;
+                    if (t + 2 < CL)
+                        if (t < len - 2) d[t + 2 < CL ? t + 2 : t] -= l2[t] * xt;
+                }
+            }
+#pragma unroll
+            for (int t = CL - 1; t >= 0; t--) {
+                if (t < len) {
+                    T acc = d[t];
+                    if (t + 1 < CL)
+                        if (t + 1 < len) acc -= u1[t] * d[t + 1 < CL ? t + 1 : t];
+                    if (t + 2 < CL)
+                        if (t + 2 < len) acc -= u2[t] * d[t + 2 < CL ? t + 2 : t];
+                    d[t] = acc * u0[t];
+                }
+            }
+        }
+    }
+
+    __device__ __forceinline__ void ends(int len, double (&r)[B]) const
+    {
+#pragma unroll
+        for (int i = 0; i < Q; i++) r[i] = (double)d[i];
+#pragma unroll
+        for (int t = 0; t < CL; t++)
+#pragma unroll
+            for (int i = 0; i < Q; i++)
+                if (t == len - Q + i) r[Q + i] = (double)d[t];
+    }
+};
+
+// Reduced interface system: block PCR across the P chunks of each line
+// (slot = chunk * nl + line-in-block), then fetch the previous chunk's tail
+// and the next chunk's head.  Every thread of the block must call this.
+template <int Q>
+__device__ __forceinline__ void reduced_solve(const LineDev &p, double *buf, int NT, int nl, int k, int li,
+                                              double (&r)[2 * Q], double (&tp)[Q], double (&hn)[Q])
+{
+    constexpr int B = 2 * Q;
+    const int P = p.P;
+    const int slot = k * nl + li;
+    const double *pc = p.pcr;
+    for (int st = 0, dd = 1; st < p.nsteps; st++, dd <<= 1) {
+        double *bf = buf + (st & 1) * B * NT;
+#pragma unroll
+        for (int c = 0; c < B; c++) bf[c * NT + slot] = r[c];
+        __syncthreads();
+        int km = k - dd, kp = k + dd;
+        bool hm = true, hp = true;
+        if (p.periodic) {
+            km &= (P - 1);
+            kp &= (P - 1);
+        } else {
+            hm = km >= 0;
+            hp = kp < P;
+        }
+        double rm[B], rp[B];
+#pragma unroll
+        for (int c = 0; c < B; c++) {
+            rm[c] = hm ? bf[c * NT + km * nl + li] : 0.0;
+            rp[c] = hp ? bf[c * NT + kp * nl + li] : 0.0;
+        }
+        const double *al = pc + ((size_t)(st * P + k) * 2) * B * B;
+        const double *be = al + B * B;
+#pragma unroll
+        for (int i = 0; i < B; i++) {
+            double acc = r[i];
+#pragma unroll
+            for (int j = 0; j < B; j++) acc += al[i * B + j] * rm[j] + be[i * B + j] * rp[j];
+            r[i] = acc;
+        }
+    }
+    const double *mi = pc + (size_t)p.nsteps * P * 2 * B * B + (size_t)k * B * B;
+    double z[B];
+#pragma unroll
+    for (int i = 0; i < B; i++) {
+        double acc = 0.0;
+#pragma unroll
+        for (int j = 0; j < B; j++) acc += mi[i * B + j] * r[j];
+        z[i] = acc;
+    }
+    double *bf = buf + (p.nsteps & 1) * B * NT;
+#pragma unroll
+    for (int c = 0; c < B; c++) bf[c * NT + slot] = z[c];
+    __syncthreads();
+    int kprev = k - 1, knext = k + 1;
+    bool hasp = true, hasn = true;
+    if (p.periodic) {
+        kprev &= (P - 1);
+        knext &= (P - 1);
+    } else {
+        hasp = kprev >= 0;
+        hasn = knext < P;
+    }
+#pragma unroll
+    for (int i = 0; i < Q; i++) {
+        tp[i] = hasp ? bf[(Q + i) * NT + kprev * nl + li] : 0.0;
+        hn[i] = hasn ? bf[i * NT + knext * nl + li] : 0.0;
+    }
+}
+
+template <typename T, int Q>
+__device__ __forceinline__ double corrected(const LineDev &p, int g, T y, const double (&tp)[Q],
+                                            const double (&hn)[Q])
+{
+    const int n = p.n;
+    const double *rw = p.row64;
+    if constexpr (Q == 1) {
+        return (double)y - (rw[(size_t)R1_V * n + g] * tp[0] + rw[(size_t)R1_W * n + g] * hn[0]);
+    } else {
+        double acc = rw[(size_t)R2_V0 * n + g] * tp[0] + rw[(size_t)R2_W0 * n + g] * hn[0];
+        acc += rw[(size_t)R2_V1 * n + g] * tp[1] + rw[(size_t)R2_W1 * n + g] * hn[1];
+        return (double)y - acc;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// y / z sweeps: lanes along x, one chunk row per load instruction.
+// Line dimension stride sd, outer dimension stride so; block = Lx lanes x P
+// chunks.  UNI: Lx == 64, so each wave holds one chunk and every per-row
+// constant is wave-uniform (scalar loads).
+// ---------------------------------------------------------------------------
+template <typename T, int Q, int M, int SYM, int EPI, bool UNI>
+__global__ void __launch_bounds__(Q == 1 ? 1024 : 512) k_line_yz(LineDev p, const T *__restrict__ f, int sd, int64_t so,
+                                                  int nlane, int nxb, int Lx, EpiArgs e)
+{
+    extern __shared__ double smem[];
+    const int NT = Lx * p.P;
+    const int lane = UNI ? (int)(threadIdx.x & 63) : (int)(threadIdx.x % Lx);
+    int k = UNI ? (int)(threadIdx.x >> 6) : (int)(threadIdx.x / Lx);
+    if constexpr (UNI) k = __builtin_amdgcn_readfirstlane(k);
+    const int xb = blockIdx.x % nxb;
+    const int64_t ob = blockIdx.x / nxb;
+    const int xi = xb * Lx + lane;
+    const bool valid = xi < nlane;
+    const int64_t off = (int64_t)xi + ob * so;
+    const T *fl = f + off;
+    const int n = p.n;
+    const int s = k * p.m + min(k, p.extra);
+    const int len = p.m + (k < p.extra ? 1 : 0);
+
+    T fv[MC + 2 * M];
+#pragma unroll
+    for (int t = -M; t < MC + M; t++) {
+        T val = T(0);
+        if (valid && t < len + M) {
+            int r = s + t;
+            bool ok = true;
+            if (r < 0) {
+                if (p.periodic) r += n; else ok = false;
+            } else if (r >= n) {
+                if (p.periodic) r -= n; else ok = false;
+            }
+            if (ok) val = fl[r * sd];
+        }
+        fv[t + M] = val;
+    }
+    T fe[8];
+#pragma unroll
+    for (int kap = 0; kap < 8; kap++) fe[kap] = T(0);
+    if (p.closed && k == p.P - 1 && valid) {
+#pragma unroll
+        for (int kap = 0; kap < 8; kap++)
+            if (kap < n) fe[kap] = fl[(n - 1 - kap) * sd];
+    }
+
+    Chunk<T, Q, M, SYM> ch;
+    ch.rhs(p, k, len, fv, fe);
+    ch.solve(p, s, len);
+    double r[2 * Q], tp[Q], hn[Q];
+    ch.ends(len, r);
+    reduced_solve<Q>(p, smem, NT, Lx, k, lane, r, tp, hn);
+    if (!valid) return;
+#pragma unroll
+    for (int t = 0; t < MC; t++) {
+        if (t < len) {
+            const int g = s + t;
+            epilogue<T, EPI>(e, off + (int64_t)g * sd, corrected<T, Q>(p, g, ch.d[t], tp, hn));
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// x sweep: Lt whole lines staged in LDS (row stride n + 1), threads own
+// (line, chunk) pairs, tid = line + Lt * chunk.  Lines are indexed
+// l = j + ny*k over the y-z plane.  UNI: Lt == 64 (one chunk per wave).
+// ---------------------------------------------------------------------------
+template <typename T, int Q, int M, int SYM, int EPI, bool UNI>
+__global__ void __launch_bounds__(Q == 1 ? 1024 : 512) k_line_x(LineDev p, const T *__restrict__ f, int64_t sy, int64_t sz,
+                                                 int ny, int64_t nlines, int Lt, EpiArgs e)
+{
+    extern __shared__ double smem[];
+    const int n = p.n;
+    const int TS = n + 1;
+    const int NT = Lt * p.P;
+    T *tile = (T *)smem;
+    double *buf = smem + (((size_t)Lt * TS * sizeof(T) + 7) / 8);
+    const int tid = threadIdx.x;
+    const int64_t l0 = (int64_t)blockIdx.x * Lt;
+
+    for (int li = 0; li < Lt; li++) {
+        const int64_t l = l0 + li;
+        if (l >= nlines) break;
+        const T *src = f + (l % ny) * sy + (l / ny) * sz;
+        for (int i = tid; i < n; i += NT) tile[li * TS + i] = src[i];
+    }
+    __syncthreads();
+
+    const int li = UNI ? (tid & 63) : (tid % Lt);
+    int k = UNI ? (tid >> 6) : (tid / Lt);
+    if constexpr (UNI) k = __builtin_amdgcn_readfirstlane(k);
+    const int s = k * p.m + min(k, p.extra);
+    const int len = p.m + (k < p.extra ? 1 : 0);
+    const T *trow = tile + li * TS;
+    T fv[MC + 2 * M];
+#pragma unroll
+    for (int t = -M; t < MC + M; t++) {
+        T val = T(0);
+        if (t < len + M) {
+            int r = s + t;
+            bool ok = true;
+            if (r < 0) {
+                if (p.periodic) r += n; else ok = false;
+            } else if (r >= n) {
+                if (p.periodic) r -= n; else ok = false;
+            }
+            if (ok) val = trow[r];
+        }
+        fv[t + M] = val;
+    }
+    T fe[8];
+#pragma unroll
+    for (int kap = 0; kap < 8; kap++) fe[kap] = (p.closed && kap < n) ? trow[n - 1 - kap] : T(0);
+
+    Chunk<T, Q, M, SYM> ch;
+    ch.rhs(p, k, len, fv, fe);
+    ch.solve(p, s, len);
+    double r[2 * Q], tp[Q], hn[Q];
+    ch.ends(len, r);
+    reduced_solve<Q>(p, buf, NT, Lt, k, li, r, tp, hn);
+    // every thread has finished reading the tile before the first reduced-solve barrier
+    T *wrow = tile + li * TS;
+#pragma unroll
+    for (int t = 0; t < MC; t++)
+        if (t < len) wrow[s + t] = (T)corrected<T, Q>(p, s + t, ch.d[t], tp, hn);
+    __syncthreads();
+
+    for (int lj = 0; lj < Lt; lj++) {
+        const int64_t l = l0 + lj;
+        if (l >= nlines) break;
+        const int64_t off = (l % ny) * sy + (l / ny) * sz;
+        for (int i = tid; i < n; i += NT) epilogue<T, EPI>(e, off + i, (double)tile[lj * TS + i]);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// x sweep, wave-local: a line of n <= 64*XCL points is split into P <= 64
+// chunks of <= XCL contiguous points; lane (li, k) of a wave owns chunk k of
+// line li (64/P lines per wave).  The reduced interface system is solved by
+// block PCR through cross-lane shuffles (no LDS tile, no barrier).
+// UNIROW: periodic line with equal chunks, so every chunk has the same
+// factors, spikes and PCR blocks (those of chunk 0): all per-row constants are
+// wave-uniform.
+// ---------------------------------------------------------------------------
+constexpr int XCL = 16;
+
+template <int Q>
+__device__ __forceinline__ void reduced_solve_wave(const LineDev &p, int k, int kc, int base, double (&r)[2 * Q],
+                                                   double (&tp)[Q], double (&hn)[Q])
+{
+    constexpr int B = 2 * Q;
+    const int P = p.P;
+    const double *pc = p.pcr;
+    for (int st = 0, dd = 1; st < p.nsteps; st++, dd <<= 1) {
+        int km = k - dd, kp = k + dd;
+        bool hm = true, hp = true;
+        if (p.periodic) {
+            km &= (P - 1);
+            kp &= (P - 1);
+        } else {
+            hm = km >= 0;
+            hp = kp < P;
+            km = hm ? km : k;
+            kp = hp ? kp : k;
+        }
+        double rm[B], rp[B];
+#pragma unroll
+        for (int c = 0; c < B; c++) {
+            rm[c] = __shfl(r[c], base + km, 64);
+            rp[c] = __shfl(r[c], base + kp, 64);
+            if (!hm) rm[c] = 0.0;
+            if (!hp) rp[c] = 0.0;
+        }
+        const double *al = pc + ((size_t)(st * P + kc) * 2) * B * B;
+        const double *be = al + B * B;
+#pragma unroll
+        for (int i = 0; i < B; i++) {
+            double acc = r[i];
+#pragma unroll
+            for (int j = 0; j < B; j++) acc += al[i * B + j] * rm[j] + be[i * B + j] * rp[j];
+            r[i] = acc;
+        }
+    }
+    const double *mi = pc + (size_t)p.nsteps * P * 2 * B * B + (size_t)kc * B * B;
+    double z[B];
+#pragma unroll
+    for (int i = 0; i < B; i++) {
+        double acc = 0.0;
+#pragma unroll
+        for (int j = 0; j < B; j++) acc += mi[i * B + j] * r[j];
+        z[i] = acc;
+    }
+    int kprev = k - 1, knext = k + 1;
+    bool hasp = true, hasn = true;
+    if (p.periodic) {
+        kprev &= (P - 1);
+        knext &= (P - 1);
+    } else {
+        hasp = kprev >= 0;
+        hasn = knext < P;
+        kprev = hasp ? kprev : k;
+        knext = hasn ? knext : k;
+    }
+#pragma unroll
+    for (int i = 0; i < Q; i++) {
+        const double a = __shfl(z[Q + i], base + kprev, 64);
+        const double b = __shfl(z[i], base + knext, 64);
+        tp[i] = hasp ? a : 0.0;
+        hn[i] = hasn ? b : 0.0;
+    }
+}
+
+// One wave per block.  The wave's 64/P lines are staged in LDS (a pad slot
+// after every XCL points keeps the chunk-strided reads conflict-free), so
+// global loads and stores are contiguous 512-byte rows.
+__device__ __forceinline__ int xw_slot(int x) { return x + x / XCL; }
+
+template <typename T, int Q, int M, int SYM, int EPI, bool UNIROW>
+__global__ void __launch_bounds__(64) k_line_xw(LineDev p, const T *__restrict__ f, int64_t sy, int64_t sz, int ny,
+                                                int64_t nlines, int TS, EpiArgs e)
+{
+    extern __shared__ double smem[];
+    T *tile = (T *)smem;
+    const int P = p.P;
+    const int n = p.n;
+    const int lane = threadIdx.x;
+    const int lpw = 64 / P;
+    const int64_t l0 = (int64_t)blockIdx.x * lpw;
+    // stage the lines
+    for (int li = 0; li < lpw; li++) {
+        const int64_t l = l0 + li;
+        if (l >= nlines) break;
+        const T *src = f + (l % ny) * sy + (l / ny) * sz;
+        for (int x = lane; x < n; x += 64) tile[li * TS + xw_slot(x)] = src[x];
+    }
+    __syncthreads();
+    const int li = lane / P;
+    const int k = lane & (P - 1);
+    const int base = li * P;
+    const T *trow = tile + li * TS;
+    const int s = k * p.m + min(k, p.extra);
+    const int len = p.m + (k < p.extra ? 1 : 0);
+    const int rs = UNIROW ? 0 : s;
+    const int kc = UNIROW ? 0 : k;
+
+    T fv[XCL + 2 * M];
+#pragma unroll
+    for (int t = -M; t < XCL + M; t++) {
+        T val = T(0);
+        if (t < len + M) {
+            int r = s + t;
+            bool ok = true;
+            if (r < 0) {
+                if (p.periodic) r += n; else ok = false;
+            } else if (r >= n) {
+                if (p.periodic) r -= n; else ok = false;
+            }
+            if (ok) val = trow[xw_slot(r)];
+        }
+        fv[t + M] = val;
+    }
+    T fe[8];
+#pragma unroll
+    for (int kap = 0; kap < 8; kap++) fe[kap] = (p.closed && kap < n) ? trow[xw_slot(n - 1 - kap)] : T(0);
+    Chunk<T, Q, M, SYM, XCL> ch;
+    ch.rhs(p, k, len, fv, fe);
+    ch.solve(p, rs, len);
+    double r[2 * Q], tp[Q], hn[Q];
+    ch.ends(len, r);
+    reduced_solve_wave<Q>(p, k, kc, base, r, tp, hn);
+    __syncthreads();   // all window reads done before the tile is overwritten
+    T *wrow = tile + li * TS;
+#pragma unroll
+    for (int t = 0; t < XCL; t++)
+        if (t < len) wrow[xw_slot(s + t)] = (T)corrected<T, Q>(p, rs + t, ch.d[t], tp, hn);
+    __syncthreads();
+    for (int lj = 0; lj < lpw; lj++) {
+        const int64_t l = l0 + lj;
+        if (l >= nlines) break;
+        const int64_t off = (l % ny) * sy + (l / ny) * sz;
+        for (int x = lane; x < n; x += 64) epilogue<T, EPI>(e, off + x, (double)tile[lj * TS + xw_slot(x)]);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Dispatch
+// ---------------------------------------------------------------------------
+inline size_t lds_limit()
+{
+    static size_t lim = 0;
+    if (!lim) {
+        int dev = 0, v = 0;
+        (void)hipGetDevice(&dev);
+        if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess || v <= 0)
+            v = 64 * 1024;
+        lim = (size_t)v;
+    }
+    return lim;
+}
+
+template <typename K>
+inline void allow_lds(K kern, size_t shm)
+{
+    if (shm > 64 * 1024)
+        CLES_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
+}
+
+template <typename T, int Q, int M, int SYM, int EPI>
+void launch_variant(const HostPlan &hp, int dim, const View3 &v, const T *f, const EpiArgs &e,
+                           hipStream_t st)
+{
+    const LineDev &p = hp.dev;
+    const int P = p.P;
+    const int B = 2 * Q;
+    const size_t lim = lds_limit();
+    const int maxNT = Q == 1 ? 1024 : 512;
+    if (dim == 1 && P <= 64 && p.m + (p.extra ? 1 : 0) <= XCL) {
+        const int64_t nlines = v.n[1] * v.n[2];
+        const int lpw = 64 / P;
+        const int TS = p.n + p.n / XCL + 1;
+        const size_t shm = (size_t)lpw * TS * sizeof(T);
+        const int64_t nblk = (nlines + lpw - 1) / lpw;
+        if (p.periodic && p.extra == 0)
+            hipLaunchKernelGGL((k_line_xw<T, Q, M, SYM, EPI, true>), dim3((unsigned)nblk), dim3(64), shm, st, p, f,
+                               v.s[1], v.s[2], (int)v.n[1], nlines, TS, e);
+        else
+            hipLaunchKernelGGL((k_line_xw<T, Q, M, SYM, EPI, false>), dim3((unsigned)nblk), dim3(64), shm, st, p, f,
+                               v.s[1], v.s[2], (int)v.n[1], nlines, TS, e);
+    } else if (dim == 1) {
+        const size_t tile_row = (size_t)(p.n + 1) * sizeof(T);
+        const int64_t nlines = v.n[1] * v.n[2];
+        auto shm_of = [&](int Lt) {
+            return (((size_t)Lt * tile_row + 7) / 8) * 8 + (size_t)2 * B * Lt * P * sizeof(double);
+        };
+        if (64 * P <= maxNT && shm_of(64) <= lim && nlines >= 64) {
+            const int Lt = 64, NT = 64 * P;
+            const size_t shm = shm_of(Lt);
+            auto kern = k_line_x<T, Q, M, SYM, EPI, true>;
+            allow_lds(kern, shm);
+            const int64_t nblk = (nlines + Lt - 1) / Lt;
+            hipLaunchKernelGGL(kern, dim3((unsigned)nblk), dim3(NT), shm, st, p, f, v.s[1], v.s[2], (int)v.n[1],
+                               nlines, Lt, e);
+        } else {
+            int Lt = 1;
+            while (Lt * 2 * P <= 512 && shm_of(Lt * 2) <= 48 * 1024 && Lt < 32) Lt *= 2;
+            if (Lt * P > maxNT || shm_of(Lt) > lim)
+                throw Error{CLES_ERR_UNSUPPORTED, "line too long for the x-sweep kernel"};
+            const int NT = Lt * P;
+            const size_t shm = shm_of(Lt);
+            auto kern = k_line_x<T, Q, M, SYM, EPI, false>;
+            allow_lds(kern, shm);
+            const int64_t nblk = (nlines + Lt - 1) / Lt;
+            hipLaunchKernelGGL(kern, dim3((unsigned)nblk), dim3(NT), shm, st, p, f, v.s[1], v.s[2], (int)v.n[1],
+                               nlines, Lt, e);
+        }
+    } else {
+        const int64_t nlane = v.n[0];
+        const int64_t sd = dim == 2 ? v.s[1] : v.s[2];
+        const int64_t so = dim == 2 ? v.s[2] : v.s[1];
+        const int64_t nouter = dim == 2 ? v.n[2] : v.n[1];
+        if ((int64_t)(p.n - 1) * sd > INT32_MAX) throw Error{CLES_ERR_UNSUPPORTED, "array too large for 32-bit line offsets"};
+        if (nlane >= 48 && 64 * P <= maxNT) {
+            const int Lx = 64, NT = 64 * P;
+            const int nxb = (int)((nlane + Lx - 1) / Lx);
+            const size_t shm = (size_t)2 * B * NT * sizeof(double);
+            hipLaunchKernelGGL((k_line_yz<T, Q, M, SYM, EPI, true>), dim3((unsigned)(nxb * nouter)), dim3(NT), shm,
+                               st, p, f, (int)sd, so, (int)nlane, nxb, Lx, e);
+        } else {
+            int Lx = 64;
+            while (Lx * P > maxNT && Lx > 1) Lx /= 2;
+            while (Lx / 2 >= nlane && Lx > 1) Lx /= 2;   // narrow x extents (2-D / 1-D grids)
+            if (Lx * P > maxNT) throw Error{CLES_ERR_UNSUPPORTED, "line too long for the y/z-sweep kernel"};
+            const int NT = Lx * P;
+            const int nxb = (int)((nlane + Lx - 1) / Lx);
+            const size_t shm = (size_t)2 * B * NT * sizeof(double);
+            hipLaunchKernelGGL((k_line_yz<T, Q, M, SYM, EPI, false>), dim3((unsigned)(nxb * nouter)), dim3(NT),
+                               shm, st, p, f, (int)sd, so, (int)nlane, nxb, Lx, e);
+        }
+    }
+    CLES_HIP(hipGetLastError());
+}
+
+template <typename T, int Q, int M, int SYM>
+void launch_epi(const HostPlan &hp, int dim, const View3 &v, const T *f, const EpiArgs &e, hipStream_t st)
+{
+    if constexpr (SYM) {
+        switch (e.kind) {
+        case EPI_STORE: return launch_variant<T, Q, M, SYM, EPI_STORE>(hp, dim, v, f, e, st);
+        case EPI_ART_MUBETA: return launch_variant<T, Q, M, SYM, EPI_ART_MUBETA>(hp, dim, v, f, e, st);
+        case EPI_ART_KAPPA: return launch_variant<T, Q, M, SYM, EPI_ART_KAPPA>(hp, dim, v, f, e, st);
+        case EPI_ART_D: return launch_variant<T, Q, M, SYM, EPI_ART_D>(hp, dim, v, f, e, st);
+        default: break;
+        }
+    } else {
+        switch (e.kind) {
+        case EPI_STORE: return launch_variant<T, Q, M, SYM, EPI_STORE>(hp, dim, v, f, e, st);
+        case EPI_STORE_NEG: return launch_variant<T, Q, M, SYM, EPI_STORE_NEG>(hp, dim, v, f, e, st);
+        case EPI_SUB: return launch_variant<T, Q, M, SYM, EPI_SUB>(hp, dim, v, f, e, st);
+        case EPI_LSRK: return launch_variant<T, Q, M, SYM, EPI_LSRK>(hp, dim, v, f, e, st);
+        default: break;
+        }
+    }
+    throw Error{CLES_ERR_UNSUPPORTED, "epilogue not available for this scheme kind"};
+}
+
+}  // namespace cles

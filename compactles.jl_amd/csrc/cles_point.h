@@ -1,0 +1,102 @@
+// Argument records and launchers of the pointwise / stencil kernels.
+#pragma once
+#include "cles_internal.h"
+
+namespace cles {
+
+template <typename W>
+struct GradPtrs {
+    W *gu[9];   // grad_u[d*3 + j] = d u_j / d x_d ; NULL on collapsed d (reads as 0)
+};
+
+template <typename W>
+struct PrimArgs {
+    const double *Q;
+    int Ns;
+    double Rk[CLES_MAXSP], cvk[CLES_MAXSP];
+    W *rho, *u, *v, *w, *p, *T, *c, *cp;
+    W *Y[CLES_MAXSP];
+    W *esens;   // optional: internal energy of the kappa* sensor (artificial.jl:319-326)
+};
+
+template <typename W>
+struct D4Args {
+    const W *f;          // MODE 0
+    const double *Q;     // MODE 1
+    const W *rho;        // MODE 1
+    int im1;             // first momentum component
+    int64_t n[3];
+    int64_t N;
+    int active[3], periodic[3];
+    double wd[3];        // h_d^wpow
+};
+
+template <typename W>
+struct FluxArgs {
+    const double *Q;
+    int Ns;
+    int active[3];
+    double mu0, Pr, Sc;
+    double cpk[CLES_MAXSP];
+    const W *rho, *u, *v, *w, *p, *T, *cp;
+    const W *mu_art, *beta_art, *kappa_art;
+    const W *Y[CLES_MAXSP];
+    const W *D_art[CLES_MAXSP];
+    GradPtrs<W> g;
+    const W *gT[3];
+    const W *gY[3][CLES_MAXSP];
+    W *flux[3][CLES_MAXSP + 4];
+};
+
+template <typename W>
+struct RateArgs {
+    const double *Q;
+    int Ns;
+    int active[3];
+    double invh[3];
+    double mu0, Pr, Sc;
+    const W *rho, *u, *v, *w, *c, *cp;
+    const W *mu_art, *beta_art, *kappa_art;
+    const W *D_art[CLES_MAXSP];
+};
+
+struct PlaneArgs {
+    int dim, side, Ns;
+    int64_t n[3];
+    int64_t N, count;
+    double state[CLES_MAXSP + 4];
+};
+
+template <typename W>
+struct DiagArgs {
+    const double *Q;
+    int Ns;
+    int64_t n[3];
+    int64_t N;
+    int closed[3];
+    GradPtrs<W> g;
+};
+
+template <typename W> void launch_primitives(const PrimArgs<W> &a, int64_t N, hipStream_t st);
+template <typename W> void launch_strain(const GradPtrs<W> &g, W *out, int64_t N, hipStream_t st);
+template <typename W> void launch_delta4(const D4Args<W> &a, int mode, W *out, hipStream_t st);
+template <typename W> void launch_gate_beta(const GradPtrs<W> &g, W *beta, int64_t N, hipStream_t st);
+template <typename W>
+void launch_dilatation_prep(const GradPtrs<W> &g, W *div, W *sw, int64_t N, hipStream_t st);
+template <typename W>
+void launch_dilatation_beta(const W *rho, const W *sensor, const W *sw, W *beta, double C_beta, int64_t N,
+                            hipStream_t st);
+template <typename W> void launch_fluxes(const FluxArgs<W> &a, int64_t N, hipStream_t st);
+template <typename W>
+void launch_lsrk(double *Q, double *du, const W *dQ, double A, double B, double dt, int64_t NC, hipStream_t st);
+template <typename W>
+void launch_rate(const RateArgs<W> &a, double *part, int nb, double *out, int64_t N, hipStream_t st);
+void launch_slipwall(const PlaneArgs &a, double *Q, hipStream_t st);
+void launch_dirichlet(const PlaneArgs &a, double *Q, hipStream_t st);
+template <typename W>
+void launch_integral(const DiagArgs<W> &a, int kind, double *part, int nb, double scale, double *out,
+                     hipStream_t st);
+void launch_copy(void *dst, const void *src, size_t bytes, hipStream_t st);
+template <typename Dst, typename Src> void launch_convert(Dst *d, const Src *s, int64_t N, hipStream_t st);
+
+}  // namespace cles

@@ -1,0 +1,640 @@
+// Pointwise and stencil kernels of the per-step path (everything that is not
+// a compact line solve).  Fields are unpadded column-major arrays of the
+// interior, element (i,j,k) at i + nx*(j + ny*k).  W is the working type of
+// the primitive / gradient / flux fields (double, or float in the mixed
+// form); the conserved state Q and the LSRK register du are always double.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+#include "cles_point.h"
+
+// Pointwise arithmetic mirrors the reference operation by operation (Julia
+// does not contract a*b+c without @fastmath/@muladd), so primitives, sensor
+// inputs and fluxes reproduce the CPU path to the last bit where the
+// operation order is the same.
+#pragma clang fp contract(off)
+
+namespace cles {
+
+namespace {
+
+constexpr int TPB = 256;
+
+inline unsigned nblocks(int64_t n, int per = TPB) { return (unsigned)((n + per - 1) / per); }
+
+__device__ __forceinline__ double d_max(double a, double b) { return a > b ? a : b; }
+__device__ __forceinline__ float d_max(float a, float b) { return a > b ? a : b; }
+
+// ---------------------------------------------------------------------------
+// physics.jl:604-653 _primitives! (IdealMixture), over the interior
+// ---------------------------------------------------------------------------
+template <typename W>
+__global__ void __launch_bounds__(TPB) k_primitives(PrimArgs<W> a, int64_t N)
+{
+    const int64_t I = (int64_t)blockIdx.x * TPB + threadIdx.x;
+    if (I >= N) return;
+    const int Ns = a.Ns;
+    double rho = 0.0;
+#pragma unroll
+    for (int sp = 0; sp < CLES_MAXSP; sp++)
+        if (sp < Ns) rho += a.Q[(int64_t)sp * N + I];
+    if (rho > 0) {
+        const double ri = 1 / rho;
+        double Rm = 0.0, cvm = 0.0;
+#pragma unroll
+        for (int sp = 0; sp < CLES_MAXSP; sp++)
+            if (sp < Ns) {
+                const double q = a.Q[(int64_t)sp * N + I];
+                Rm += q * a.Rk[sp];
+                cvm += q * a.cvk[sp];
+                a.Y[sp][I] = (W)(q * ri);
+            }
+        Rm *= ri;
+        cvm *= ri;
+        const double u = a.Q[(int64_t)Ns * N + I] * ri;
+        const double v = a.Q[(int64_t)(Ns + 1) * N + I] * ri;
+        const double w = a.Q[(int64_t)(Ns + 2) * N + I] * ri;
+        const double e = a.Q[(int64_t)(Ns + 3) * N + I] * ri - 0.5 * (u * u + v * v + w * w);
+        const double T = d_max(e / cvm, 1e-300);
+        const double p = rho * Rm * T;
+        const double gm = 1 + Rm / cvm;
+        if (a.esens) {
+            // artificial.jl:321-324 with rho = max(rho, 1e-300)
+            const double r = d_max(rho, 1e-300);
+            const double m1 = a.Q[(int64_t)Ns * N + I], m2 = a.Q[(int64_t)(Ns + 1) * N + I],
+                         m3 = a.Q[(int64_t)(Ns + 2) * N + I];
+            const double ke = 0.5 * (m1 * m1 + m2 * m2 + m3 * m3) / r;
+            a.esens[I] = (W)((a.Q[(int64_t)(Ns + 3) * N + I] - ke) / r);
+        }
+        a.rho[I] = (W)rho;
+        a.u[I] = (W)u;
+        a.v[I] = (W)v;
+        a.w[I] = (W)w;
+        a.p[I] = (W)p;
+        a.T[I] = (W)T;
+        a.c[I] = (W)sqrt(gm * Rm * T);
+        a.cp[I] = (W)(cvm + Rm);
+    } else {
+        if (a.esens) {
+            const double m1 = a.Q[(int64_t)Ns * N + I], m2 = a.Q[(int64_t)(Ns + 1) * N + I],
+                         m3 = a.Q[(int64_t)(Ns + 2) * N + I];
+            const double ke = 0.5 * (m1 * m1 + m2 * m2 + m3 * m3) / 1.0;
+            a.esens[I] = (W)((a.Q[(int64_t)(Ns + 3) * N + I] - ke) / 1.0);
+        }
+        a.rho[I] = 1; a.u[I] = 0; a.v[I] = 0; a.w[I] = 0;
+        a.p[I] = 1; a.T[I] = 1; a.c[I] = 1; a.cp[I] = 1;
+#pragma unroll
+        for (int sp = 0; sp < CLES_MAXSP; sp++)
+            if (sp < Ns) a.Y[sp][I] = sp == 0 ? W(1) : W(0);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// artificial.jl:279-290 strain-rate magnitude |S|
+// ---------------------------------------------------------------------------
+template <typename W>
+__global__ void __launch_bounds__(TPB) k_strain(GradPtrs<W> g, W *__restrict__ out, int64_t N)
+{
+    const int64_t I = (int64_t)blockIdx.x * TPB + threadIdx.x;
+    if (I >= N) return;
+    W G[9];
+#pragma unroll
+    for (int m = 0; m < 9; m++) G[m] = g.gu[m] ? g.gu[m][I] : W(0);
+    W ss = 0;
+#pragma unroll
+    for (int b = 0; b < 3; b++)
+#pragma unroll
+        for (int a2 = 0; a2 < 3; a2++) {
+            const W S = W(0.5) * (G[a2 * 3 + b] + G[b * 3 + a2]);
+            ss += S * S;
+        }
+    out[I] = sqrt(ss);
+}
+
+// ---------------------------------------------------------------------------
+// artificial.jl:67-94 delta4_sum!: sum over active d of h_d^wpow |delta4_d f|,
+// indices clamped at closed edges, wrapped on periodic dimensions.
+// MODE 0: f is a field.  MODE 1: f = e = (E - ke)/rho from Q and rho
+// (artificial.jl:319-326) evaluated at each stencil point.
+// ---------------------------------------------------------------------------
+template <typename W, int MODE>
+__device__ __forceinline__ W d4_value(const D4Args<W> &a, int64_t J)
+{
+    if constexpr (MODE == 0) {
+        return a.f[J];
+    } else {
+        const int64_t N = a.N;
+        const double rho = d_max((double)a.rho[J], 1e-300);
+        const double m1 = a.Q[(int64_t)a.im1 * N + J], m2 = a.Q[(int64_t)(a.im1 + 1) * N + J],
+                     m3 = a.Q[(int64_t)(a.im1 + 2) * N + J];
+        const double ke = 0.5 * (m1 * m1 + m2 * m2 + m3 * m3) / rho;
+        return (W)((a.Q[(int64_t)(a.im1 + 3) * N + J] - ke) / rho);
+    }
+}
+
+// Blocks are dispatched round-robin over the 8 XCDs, each with its own L2;
+// renumbering them so that every XCD sweeps one contiguous slab keeps the
+// +-2 rows and planes of the stencil in that XCD's L2.
+__device__ __forceinline__ unsigned xcd_block(unsigned b, unsigned nb)
+{
+    const unsigned nfull = nb & ~7u;
+    return b < nfull ? (b & 7u) * (nfull >> 3) + (b >> 3) : b;
+}
+
+template <typename W, int MODE>
+__global__ void __launch_bounds__(TPB) k_delta4(D4Args<W> a, W *__restrict__ out)
+{
+    const unsigned blk = xcd_block(blockIdx.x, gridDim.x);
+    const int64_t I = (int64_t)blk * TPB + threadIdx.x;
+    if (I >= a.N) return;
+    const int nx = (int)a.n[0], ny = (int)a.n[1];
+    const int I32 = (int)I;
+    const int ix = I32 % nx, r = I32 / nx;
+    const int ii[3] = {ix, r % ny, r / ny};
+    const int st[3] = {1, nx, nx * ny};
+    W acc_out = 0;
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        if (!a.active[d]) continue;
+        const int nd = (int)a.n[d], il = ii[d];
+        double acc = 0.0;
+        const double D4[5] = {1.0, -4.0, 6.0, -4.0, 1.0};
+#pragma unroll
+        for (int m = -2; m <= 2; m++) {
+            int j = il + m;
+            if (a.periodic[d]) {
+                if (j < 0) j += nd;
+                else if (j >= nd) j -= nd;
+            } else {
+                j = j < 0 ? 0 : (j > nd - 1 ? nd - 1 : j);
+            }
+            acc += D4[m + 2] * (double)d4_value<W, MODE>(a, I32 + (j - il) * st[d]);
+        }
+        acc_out += (W)(a.wd[d] * fabs(acc));
+    }
+    out[I] = acc_out;
+}
+
+// ---------------------------------------------------------------------------
+// artificial.jl:136-145 compression switch, 171-183 gate_beta!, 216-242 dilatation
+// ---------------------------------------------------------------------------
+template <typename W>
+__device__ __forceinline__ W compression_switch(const GradPtrs<W> &g, int64_t I)
+{
+    auto G = [&](int d, int j) -> W { return g.gu[d * 3 + j] ? g.gu[d * 3 + j][I] : W(0); };
+    const W div = G(0, 0) + G(1, 1) + G(2, 2);
+    if (!(div < 0)) return W(0);
+    const W w1 = G(1, 2) - G(2, 1);
+    const W w2 = G(2, 0) - G(0, 2);
+    const W w3 = G(0, 1) - G(1, 0);
+    return div * div / (div * div + w1 * w1 + w2 * w2 + w3 * w3 + (W)1e-32);
+}
+
+template <typename W>
+__global__ void __launch_bounds__(TPB) k_gate_beta(GradPtrs<W> g, W *__restrict__ beta, int64_t N)
+{
+    const int64_t I = (int64_t)blockIdx.x * TPB + threadIdx.x;
+    if (I >= N) return;
+    beta[I] *= compression_switch(g, I);
+}
+
+template <typename W>
+__global__ void __launch_bounds__(TPB) k_dilatation_prep(GradPtrs<W> g, W *__restrict__ div, W *__restrict__ sw,
+                                                         int64_t N)
+{
+    const int64_t I = (int64_t)blockIdx.x * TPB + threadIdx.x;
+    if (I >= N) return;
+    auto G = [&](int d) -> W { return g.gu[d * 4] ? g.gu[d * 4][I] : W(0); };
+    div[I] = G(0) + G(1) + G(2);
+    sw[I] = compression_switch(g, I);
+}
+
+template <typename W>
+__global__ void __launch_bounds__(TPB) k_dilatation_beta(const W *__restrict__ rho, const W *__restrict__ sensor,
+                                                         const W *__restrict__ sw, W *__restrict__ beta, double C_beta,
+                                                         int64_t N)
+{
+    const int64_t I = (int64_t)blockIdx.x * TPB + threadIdx.x;
+    if (I >= N) return;
+    const W s = sensor[I];
+    beta[I] = (W)C_beta * rho[I] * (s > W(0) ? s : W(0)) * sw[I];
+}
+
+// ---------------------------------------------------------------------------
+// rhs.jl:495-556 _assemble_fluxes! for the active dimensions
+// ---------------------------------------------------------------------------
+template <typename W>
+__global__ void __launch_bounds__(TPB) k_fluxes(FluxArgs<W> a, int64_t N)
+{
+    const int64_t I = (int64_t)blockIdx.x * TPB + threadIdx.x;
+    if (I >= N) return;
+    const int Ns = a.Ns;
+    const W rho = a.rho[I];
+    const W uv[3] = {a.u[I], a.v[I], a.w[I]};
+    const W p = a.p[I], Tp = a.T[I];
+    const W E = (W)a.Q[(int64_t)(Ns + 3) * N + I];
+    const W mu0 = (W)a.mu0;
+    const W mu = mu0 + (a.mu_art ? a.mu_art[I] : W(0));
+    const W beta = a.beta_art ? a.beta_art[I] : W(0);
+    const W kappa = mu0 * a.cp[I] / (W)a.Pr + (a.kappa_art ? a.kappa_art[I] : W(0));
+    const W D0 = mu0 / ((W)a.Sc * rho);
+    W G[9];
+#pragma unroll
+    for (int m = 0; m < 9; m++) G[m] = a.g.gu[m] ? a.g.gu[m][I] : W(0);
+    const W divu = G[0] + G[4] + G[8];
+    const W t23 = W(2.0 / 3.0);
+    const W t11 = mu * (2 * G[0] - t23 * divu) + beta * divu;
+    const W t22 = mu * (2 * G[4] - t23 * divu) + beta * divu;
+    const W t33 = mu * (2 * G[8] - t23 * divu) + beta * divu;
+    const W t12 = mu * (G[1] + G[3]);
+    const W t13 = mu * (G[2] + G[6]);
+    const W t23b = mu * (G[5] + G[7]);
+    const W tau[3][3] = {{t11, t12, t13}, {t12, t22, t23b}, {t13, t23b, t33}};
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        if (!a.active[d]) continue;
+        const W ud = uv[d];
+        W **F = a.flux[d];
+        W hdiff = 0;
+        if (Ns > 1) {
+            W Vc = 0;
+#pragma unroll
+            for (int sp = 0; sp < CLES_MAXSP; sp++)
+                if (sp < Ns) Vc += (D0 + a.D_art[sp][I]) * a.gY[d][sp][I];
+#pragma unroll
+            for (int sp = 0; sp < CLES_MAXSP; sp++)
+                if (sp < Ns) {
+                    const W Dk = D0 + a.D_art[sp][I];
+                    const W J = rho * (-Dk * a.gY[d][sp][I] + a.Y[sp][I] * Vc);
+                    F[sp][I] = rho * a.Y[sp][I] * ud + J;
+                    hdiff += (W)a.cpk[sp] * Tp * J;
+                }
+        } else {
+            // one species: sum_k J_k = 0 forces J_1 = 0 (the reference's
+            // rho*D0*dY*(Y-1) with Y == 1 to rounding)
+            F[0][I] = rho * a.Y[0][I] * ud;
+        }
+        F[Ns][I] = rho * ud * uv[0] + (d == 0 ? p : W(0)) - tau[d][0];
+        F[Ns + 1][I] = rho * ud * uv[1] + (d == 1 ? p : W(0)) - tau[d][1];
+        F[Ns + 2][I] = rho * ud * uv[2] + (d == 2 ? p : W(0)) - tau[d][2];
+        F[Ns + 3][I] = (E + p) * ud - (uv[0] * tau[d][0] + uv[1] * tau[d][1] + uv[2] * tau[d][2]) -
+                       kappa * a.gT[d][I] + hdiff;
+    }
+}
+
+// timestep.jl:85-94 (standalone form, used when the RHS needs boundary corrections)
+template <typename W>
+__global__ void __launch_bounds__(TPB) k_lsrk(double *__restrict__ Q, double *__restrict__ du,
+                                              const W *__restrict__ dQ, double A, double B, double dt, int64_t NC)
+{
+    const int64_t I = (int64_t)blockIdx.x * TPB + threadIdx.x;
+    if (I >= NC) return;
+    const double v = A * du[I] + dt * (double)dQ[I];
+    du[I] = v;
+    Q[I] += B * v;
+}
+
+// ---------------------------------------------------------------------------
+// timestep.jl:125-175 max_rate: per-block max of the CFL rate and min of the
+// mixture density, then a single-block final reduction (order-independent).
+// ---------------------------------------------------------------------------
+template <typename W>
+__global__ void __launch_bounds__(TPB) k_rate(RateArgs<W> a, double *__restrict__ part, int64_t N)
+{
+    __shared__ double smax[TPB], smin[TPB];
+    double rate = 0.0, rmin = INFINITY;
+    for (int64_t I = (int64_t)blockIdx.x * TPB + threadIdx.x; I < N; I += (int64_t)gridDim.x * TPB) {
+        double rQ = 0.0;
+        for (int sp = 0; sp < a.Ns; sp++) rQ += a.Q[(int64_t)sp * N + I];
+        rmin = rQ < rmin ? rQ : (rQ != rQ ? rQ : rmin);
+        const double rho = a.rho[I];
+        const double ri = 1 / rho;
+        const double c = a.c[I], cp = a.cp[I];
+        const double uv[3] = {(double)a.u[I], (double)a.v[I], (double)a.w[I]};
+        double acc = 0.0, dsum = 0.0;
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            if (!a.active[d]) continue;
+            const double idx = a.invh[d];
+            acc += (fabs(uv[d]) + c) * idx;
+            dsum += idx * idx;
+        }
+        double Dmax = a.mu0 / (a.Sc * rho);
+        for (int sp = 0; sp < a.Ns; sp++)
+            if (a.D_art[sp]) Dmax = d_max(Dmax, a.mu0 / (a.Sc * rho) + (double)a.D_art[sp][I]);
+        const double mua = a.mu_art ? (double)a.mu_art[I] : 0.0;
+        const double bea = a.beta_art ? (double)a.beta_art[I] : 0.0;
+        const double kaa = a.kappa_art ? (double)a.kappa_art[I] : 0.0;
+        const double nu = (a.mu0 + mua + bea) * ri + (a.mu0 * cp / a.Pr + kaa) * ri / cp + Dmax;
+        acc += 2 * nu * dsum;
+        rate = acc > rate ? acc : (acc != acc ? acc : rate);
+    }
+    smax[threadIdx.x] = rate;
+    smin[threadIdx.x] = rmin;
+    __syncthreads();
+    for (int s = TPB / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < s) {
+            double x = smax[threadIdx.x + s], y = smin[threadIdx.x + s];
+            double X = smax[threadIdx.x], Y = smin[threadIdx.x];
+            smax[threadIdx.x] = (x > X || x != x) ? x : X;
+            smin[threadIdx.x] = (y < Y || y != y) ? y : Y;
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        part[2 * blockIdx.x] = smax[0];
+        part[2 * blockIdx.x + 1] = smin[0];
+    }
+}
+
+__global__ void __launch_bounds__(TPB) k_rate_final(const double *__restrict__ part, int nb, double *__restrict__ out)
+{
+    __shared__ double smax[TPB], smin[TPB];
+    double mx = 0.0, mn = INFINITY;
+    for (int b = threadIdx.x; b < nb; b += TPB) {
+        double x = part[2 * b], y = part[2 * b + 1];
+        mx = (x > mx || x != x) ? x : mx;
+        mn = (y < mn || y != y) ? y : mn;
+    }
+    smax[threadIdx.x] = mx;
+    smin[threadIdx.x] = mn;
+    __syncthreads();
+    for (int s = TPB / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < s) {
+            double x = smax[threadIdx.x + s], y = smin[threadIdx.x + s];
+            double X = smax[threadIdx.x], Y = smin[threadIdx.x];
+            smax[threadIdx.x] = (x > X || x != x) ? x : X;
+            smin[threadIdx.x] = (y < Y || y != y) ? y : Y;
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        out[0] = smax[0];
+        out[1] = smin[0];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// boundary.jl:232-243 SlipWallBC, problem.jl:196-207 DirichletBC (constant)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int64_t plane_index(const PlaneArgs &a, int64_t t)
+{
+    const int d = a.dim;
+    const int oa = d == 0 ? 1 : 0, ob = d == 2 ? 1 : 2;
+    const int64_t ia = t % a.n[oa], ib = t / a.n[oa];
+    int64_t idx[3];
+    idx[d] = a.side == 0 ? 0 : a.n[d] - 1;
+    idx[oa] = ia;
+    idx[ob] = ib;
+    return idx[0] + a.n[0] * (idx[1] + a.n[1] * idx[2]);
+}
+
+__global__ void __launch_bounds__(TPB) k_slipwall(PlaneArgs a, double *__restrict__ Q)
+{
+    const int64_t t = (int64_t)blockIdx.x * TPB + threadIdx.x;
+    if (t >= a.count) return;
+    const int64_t I = plane_index(a, t);
+    const int64_t N = a.N;
+    double rho = 0.0;
+    for (int k = 0; k < a.Ns; k++) rho += Q[(int64_t)k * N + I];
+    const int64_t mc = (int64_t)(a.Ns + a.dim) * N + I;
+    const double mn = Q[mc];
+    Q[(int64_t)(a.Ns + 3) * N + I] -= 0.5 * mn * mn / rho;
+    Q[mc] = 0;
+}
+
+__global__ void __launch_bounds__(TPB) k_dirichlet(PlaneArgs a, double *__restrict__ Q)
+{
+    const int64_t t = (int64_t)blockIdx.x * TPB + threadIdx.x;
+    if (t >= a.count) return;
+    const int64_t I = plane_index(a, t);
+    for (int c = 0; c < a.Ns + 4; c++) Q[(int64_t)c * a.N + I] = a.state[c];
+}
+
+// ---------------------------------------------------------------------------
+// Diagnostics (diagnostics.jl:41-80): deterministic FP64 block sums with the
+// node-centred edge half-weights, then a single-block fixed-order final sum.
+// KIND 0: 0.5|m|^2/rho from Q.  KIND 1: 0.5|omega|^2 from grad_u.
+// ---------------------------------------------------------------------------
+template <typename W, int KIND>
+__global__ void __launch_bounds__(TPB) k_integrand_sum(DiagArgs<W> a, double *__restrict__ part)
+{
+    __shared__ double sh[TPB];
+    const int64_t N = a.N;
+    double acc = 0.0;
+    for (int64_t I = (int64_t)blockIdx.x * TPB + threadIdx.x; I < N; I += (int64_t)gridDim.x * TPB) {
+        const int64_t nx = a.n[0], ny = a.n[1];
+        const int64_t ii[3] = {I % nx, (I / nx) % ny, I / (nx * ny)};
+        double wq = 1.0;
+#pragma unroll
+        for (int d = 0; d < 3; d++)
+            if (a.closed[d] && (ii[d] == 0 || ii[d] == a.n[d] - 1)) wq *= 0.5;
+        double f;
+        if constexpr (KIND == 0) {
+            double rho = 0.0;
+            for (int sp = 0; sp < a.Ns; sp++) rho += a.Q[(int64_t)sp * N + I];
+            const double m1 = a.Q[(int64_t)a.Ns * N + I], m2 = a.Q[(int64_t)(a.Ns + 1) * N + I],
+                         m3 = a.Q[(int64_t)(a.Ns + 2) * N + I];
+            f = 0.5 * (m1 * m1 + m2 * m2 + m3 * m3) / rho;
+        } else {
+            auto G = [&](int d, int j) -> double { return a.g.gu[d * 3 + j] ? (double)a.g.gu[d * 3 + j][I] : 0.0; };
+            const double w1 = G(1, 2) - G(2, 1), w2 = G(2, 0) - G(0, 2), w3 = G(0, 1) - G(1, 0);
+            f = 0.5 * (w1 * w1 + w2 * w2 + w3 * w3);
+        }
+        acc += wq * f;
+    }
+    sh[threadIdx.x] = acc;
+    __syncthreads();
+    for (int s = TPB / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+}
+
+__global__ void __launch_bounds__(TPB) k_sum_final(const double *__restrict__ part, int nb, double scale,
+                                                   double *__restrict__ out)
+{
+    __shared__ double sh[TPB];
+    double acc = 0.0;
+    for (int b = threadIdx.x; b < nb; b += TPB) acc += part[b];
+    sh[threadIdx.x] = acc;
+    __syncthreads();
+    for (int s = TPB / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[0] = sh[0] * scale;
+}
+
+// plain streaming copy in 16-byte words (calibration kernel for byte counters)
+__global__ void __launch_bounds__(TPB) k_copy16(uint4 *__restrict__ d, const uint4 *__restrict__ s, int64_t n16)
+{
+    for (int64_t i = (int64_t)blockIdx.x * TPB + threadIdx.x; i < n16; i += (int64_t)gridDim.x * TPB) d[i] = s[i];
+}
+
+__global__ void __launch_bounds__(TPB) k_copy1(unsigned char *__restrict__ d, const unsigned char *__restrict__ s,
+                                               int64_t n)
+{
+    const int64_t i = (int64_t)blockIdx.x * TPB + threadIdx.x;
+    if (i < n) d[i] = s[i];
+}
+
+template <typename Dst, typename Src>
+__global__ void __launch_bounds__(TPB) k_convert(Dst *__restrict__ d, const Src *__restrict__ s, int64_t N)
+{
+    const int64_t I = (int64_t)blockIdx.x * TPB + threadIdx.x;
+    if (I < N) d[I] = (Dst)s[I];
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Host launchers
+// ---------------------------------------------------------------------------
+#define CHECK_LAUNCH() CLES_HIP(hipGetLastError())
+
+template <typename W>
+void launch_primitives(const PrimArgs<W> &a, int64_t N, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_primitives<W>, dim3(nblocks(N)), dim3(TPB), 0, st, a, N);
+    CHECK_LAUNCH();
+}
+
+template <typename W>
+void launch_strain(const GradPtrs<W> &g, W *out, int64_t N, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_strain<W>, dim3(nblocks(N)), dim3(TPB), 0, st, g, out, N);
+    CHECK_LAUNCH();
+}
+
+template <typename W>
+void launch_delta4(const D4Args<W> &a, int mode, W *out, hipStream_t st)
+{
+    if (mode == 0)
+        hipLaunchKernelGGL((k_delta4<W, 0>), dim3(nblocks(a.N)), dim3(TPB), 0, st, a, out);
+    else
+        hipLaunchKernelGGL((k_delta4<W, 1>), dim3(nblocks(a.N)), dim3(TPB), 0, st, a, out);
+    CHECK_LAUNCH();
+}
+
+template <typename W>
+void launch_gate_beta(const GradPtrs<W> &g, W *beta, int64_t N, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_gate_beta<W>, dim3(nblocks(N)), dim3(TPB), 0, st, g, beta, N);
+    CHECK_LAUNCH();
+}
+
+template <typename W>
+void launch_dilatation_prep(const GradPtrs<W> &g, W *div, W *sw, int64_t N, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_dilatation_prep<W>, dim3(nblocks(N)), dim3(TPB), 0, st, g, div, sw, N);
+    CHECK_LAUNCH();
+}
+
+template <typename W>
+void launch_dilatation_beta(const W *rho, const W *sensor, const W *sw, W *beta, double C_beta, int64_t N,
+                            hipStream_t st)
+{
+    hipLaunchKernelGGL(k_dilatation_beta<W>, dim3(nblocks(N)), dim3(TPB), 0, st, rho, sensor, sw, beta, C_beta, N);
+    CHECK_LAUNCH();
+}
+
+template <typename W>
+void launch_fluxes(const FluxArgs<W> &a, int64_t N, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_fluxes<W>, dim3(nblocks(N)), dim3(TPB), 0, st, a, N);
+    CHECK_LAUNCH();
+}
+
+template <typename W>
+void launch_lsrk(double *Q, double *du, const W *dQ, double A, double B, double dt, int64_t NC, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_lsrk<W>, dim3(nblocks(NC)), dim3(TPB), 0, st, Q, du, dQ, A, B, dt, NC);
+    CHECK_LAUNCH();
+}
+
+template <typename W>
+void launch_rate(const RateArgs<W> &a, double *part, int nb, double *out, int64_t N, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_rate<W>, dim3(nb), dim3(TPB), 0, st, a, part, N);
+    CHECK_LAUNCH();
+    hipLaunchKernelGGL(k_rate_final, dim3(1), dim3(TPB), 0, st, part, nb, out);
+    CHECK_LAUNCH();
+}
+
+void launch_slipwall(const PlaneArgs &a, double *Q, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_slipwall, dim3(nblocks(a.count)), dim3(TPB), 0, st, a, Q);
+    CHECK_LAUNCH();
+}
+
+void launch_dirichlet(const PlaneArgs &a, double *Q, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_dirichlet, dim3(nblocks(a.count)), dim3(TPB), 0, st, a, Q);
+    CHECK_LAUNCH();
+}
+
+template <typename W>
+void launch_integral(const DiagArgs<W> &a, int kind, double *part, int nb, double scale, double *out, hipStream_t st)
+{
+    if (kind == 0)
+        hipLaunchKernelGGL((k_integrand_sum<W, 0>), dim3(nb), dim3(TPB), 0, st, a, part);
+    else
+        hipLaunchKernelGGL((k_integrand_sum<W, 1>), dim3(nb), dim3(TPB), 0, st, a, part);
+    CHECK_LAUNCH();
+    hipLaunchKernelGGL(k_sum_final, dim3(1), dim3(TPB), 0, st, part, nb, scale, out);
+    CHECK_LAUNCH();
+}
+
+void launch_copy(void *dst, const void *src, size_t bytes, hipStream_t st)
+{
+    const uintptr_t a = (uintptr_t)dst | (uintptr_t)src;
+    if (a % 16 == 0) {
+        const int64_t n16 = (int64_t)(bytes / 16);
+        if (n16 > 0) {
+            const unsigned nb = (unsigned)std::min<int64_t>((n16 + TPB - 1) / TPB, 256 * 64);
+            hipLaunchKernelGGL(k_copy16, dim3(nb), dim3(TPB), 0, st, (uint4 *)dst, (const uint4 *)src, n16);
+            CHECK_LAUNCH();
+        }
+        const size_t done = (size_t)n16 * 16;
+        if (bytes > done) {
+            hipLaunchKernelGGL(k_copy1, dim3(1), dim3(TPB), 0, st, (unsigned char *)dst + done,
+                               (const unsigned char *)src + done, (int64_t)(bytes - done));
+            CHECK_LAUNCH();
+        }
+    } else if (bytes) {
+        hipLaunchKernelGGL(k_copy1, dim3(nblocks((int64_t)bytes)), dim3(TPB), 0, st, (unsigned char *)dst,
+                           (const unsigned char *)src, (int64_t)bytes);
+        CHECK_LAUNCH();
+    }
+}
+
+template <typename Dst, typename Src>
+void launch_convert(Dst *d, const Src *s, int64_t N, hipStream_t st)
+{
+    hipLaunchKernelGGL((k_convert<Dst, Src>), dim3(nblocks(N)), dim3(TPB), 0, st, d, s, N);
+    CHECK_LAUNCH();
+}
+
+#define CLES_INST(W)                                                                                       \
+    template void launch_primitives<W>(const PrimArgs<W> &, int64_t, hipStream_t);                        \
+    template void launch_strain<W>(const GradPtrs<W> &, W *, int64_t, hipStream_t);                        \
+    template void launch_delta4<W>(const D4Args<W> &, int, W *, hipStream_t);                              \
+    template void launch_gate_beta<W>(const GradPtrs<W> &, W *, int64_t, hipStream_t);                     \
+    template void launch_dilatation_prep<W>(const GradPtrs<W> &, W *, W *, int64_t, hipStream_t);          \
+    template void launch_dilatation_beta<W>(const W *, const W *, const W *, W *, double, int64_t,         \
+                                            hipStream_t);                                                  \
+    template void launch_fluxes<W>(const FluxArgs<W> &, int64_t, hipStream_t);                             \
+    template void launch_lsrk<W>(double *, double *, const W *, double, double, double, int64_t,           \
+                                 hipStream_t);                                                             \
+    template void launch_rate<W>(const RateArgs<W> &, double *, int, double *, int64_t, hipStream_t);      \
+    template void launch_integral<W>(const DiagArgs<W> &, int, double *, int, double, double *, hipStream_t);
+CLES_INST(double)
+CLES_INST(float)
+template void launch_convert<double, float>(double *, const float *, int64_t, hipStream_t);
+template void launch_convert<float, double>(float *, const double *, int64_t, hipStream_t);
+template void launch_convert<double, double>(double *, const double *, int64_t, hipStream_t);
+
+}  // namespace cles

@@ -1,0 +1,934 @@
+// Device-resident solver: setup, compute_rhs!, step!, max_rate,
+// filter_state!, run! and diagnostics on one GPU (whole-domain lines).
+//
+// Dataflow per RHS follows src/rhs.jl:624-694 with these fusions, none of
+// which changes the arithmetic of a value:
+//  * the last pass of every sensor smoothing (artificial.jl:98-111) writes the
+//    coefficient fields directly (mu*/beta*, kappa*, D*_k epilogues);
+//  * the e-sensor's internal energy (artificial.jl:319-326) is evaluated inside
+//    the delta4 stencil instead of being stored;
+//  * the flux divergence accumulates in place (dQ = -D_1 F_1, dQ -= D_d F_d)
+//    and, when no boundary correction follows, the last direction feeds the
+//    LSRK update (timestep.jl:85-94) without storing the final dQ.
+// Periodic halos are index wraps inside the kernels (halo.jl:27-36).
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "cles_internal.h"
+#include "cles_point.h"
+
+namespace {
+
+// timestep.jl:5-21
+const double RKA[5] = {0.0, -567301805773.0 / 1357537059087.0, -2404267990393.0 / 2016746695238.0,
+                       -3550918686646.0 / 2091501179385.0, -1275806237668.0 / 842570457699.0};
+const double RKB[5] = {1432997174477.0 / 9575080441755.0, 5161836677717.0 / 13612068292357.0,
+                       1720146321549.0 / 2090206949498.0, 3134564353537.0 / 4481467310338.0,
+                       2277821191437.0 / 14882151754819.0};
+const double RKC[5] = {0.0, 1432997174477.0 / 9575080441755.0, 2526269341429.0 / 6820363962896.0,
+                       2006345519317.0 / 3224310063776.0, 2802321613138.0 / 2924317926251.0};
+const double PLANCK_TIME = 5.391247e-44;   // stepcontrol.jl:63
+
+inline double jmax(double a, double b) { return (a != a || b != b) ? NAN : (a > b ? a : b); }
+inline double jmin(double a, double b) { return (a != a || b != b) ? NAN : (a < b ? a : b); }
+
+}  // namespace
+
+struct cles_solver {
+    cles_ctx *ctx = nullptr;
+    cles_solver_desc d{};
+    int Ns = 1, C = 5, prec = CLES_F64;
+    int active[3] = {0, 0, 0}, periodic[3] = {1, 1, 1};
+    int64_t n[3] = {1, 1, 1}, N = 1;
+    double h[3] = {1, 1, 1};
+    double Rk[CLES_MAXSP] = {}, cvk[CLES_MAXSP] = {}, cpk[CLES_MAXSP] = {};
+    int nact = 0, adim[3] = {0, 0, 0};
+    bool nscbc = false;
+    cles::HostPlan dplan[3], fplan[3], fplanQ[3];
+    bool own_fplanQ = false;
+    std::vector<void *> allocs;
+    // device state (W-typed pointers are void* here; the Impl casts them)
+    double *Q = nullptr, *du = nullptr;
+    void *dQ = nullptr;
+    void *rho = nullptr, *u = nullptr, *v = nullptr, *w = nullptr, *p = nullptr, *T = nullptr, *c = nullptr,
+         *cp = nullptr;
+    void *Y[CLES_MAXSP] = {}, *gu[9] = {}, *gT[3] = {}, *gY[3][CLES_MAXSP] = {};
+    void *mu_art = nullptr, *beta_art = nullptr, *kappa_art = nullptr, *D_art[CLES_MAXSP] = {};
+    void *sensor = nullptr, *strain = nullptr, *tmp1 = nullptr, *tmp2 = nullptr, *esens = nullptr;
+    void *flux[3][CLES_MAXSP + 4] = {};
+    double *part = nullptr, *scal = nullptr;   // reduction partials, device scalars
+    int nred = 1024;
+    double *host_scal = nullptr;               // pinned
+    double t = 0, tstage = 0, dt_prev = 0, rate_prev = 0;
+    int64_t step = 0;
+
+    // per-phase event timing (cles_solver_profile)
+    struct Phase {
+        double ms = 0, bytes = 0;
+        int64_t launches = 0;
+    };
+    struct Pending {
+        int phase;
+        hipEvent_t a, b;
+        double bytes;
+    };
+    bool prof = false;
+    std::vector<Phase> phases;
+    std::vector<Pending> pending;
+    std::vector<hipEvent_t> evpool;
+
+    hipEvent_t event()
+    {
+        if (!evpool.empty()) {
+            hipEvent_t e = evpool.back();
+            evpool.pop_back();
+            return e;
+        }
+        hipEvent_t e;
+        CLES_HIP(hipEventCreate(&e));
+        return e;
+    }
+    template <typename F>
+    void timed(int phase, double bytes, F &&launch)
+    {
+        if (!prof) {
+            launch();
+            return;
+        }
+        hipEvent_t a = event(), b = event();
+        CLES_HIP(hipEventRecord(a, ctx->stream));
+        launch();
+        CLES_HIP(hipEventRecord(b, ctx->stream));
+        pending.push_back({phase, a, b, bytes});
+    }
+    void resolve()
+    {
+        if (pending.empty()) return;
+        CLES_HIP(hipStreamSynchronize(ctx->stream));
+        for (auto &pd : pending) {
+            float ms = 0.f;
+            CLES_HIP(hipEventElapsedTime(&ms, pd.a, pd.b));
+            Phase &ph = phases[pd.phase];
+            ph.ms += ms;
+            ph.bytes += pd.bytes;
+            ph.launches += 1;
+            evpool.push_back(pd.a);
+            evpool.push_back(pd.b);
+        }
+        pending.clear();
+    }
+
+    hipStream_t st() const { return ctx->stream; }
+    size_t wsize() const { return prec == CLES_F64 ? 8 : 4; }
+
+    void *alloc(size_t bytes)
+    {
+        void *ptr = nullptr;
+        hipError_t e = hipMalloc(&ptr, bytes ? bytes : 8);
+        if (e != hipSuccess) throw cles::Error{CLES_ERR_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e)};
+        allocs.push_back(ptr);
+        CLES_HIP(hipMemsetAsync(ptr, 0, bytes ? bytes : 8, st()));
+        return ptr;
+    }
+    void *wfield() { return alloc((size_t)N * wsize()); }
+
+    ~cles_solver()
+    {
+        if (ctx && ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+        for (auto &pd : pending) { evpool.push_back(pd.a); evpool.push_back(pd.b); }
+        for (hipEvent_t e : evpool) (void)hipEventDestroy(e);
+        for (void *ptr : allocs) (void)hipFree(ptr);
+        if (host_scal) (void)hipHostFree(host_scal);
+        for (int k = 0; k < 3; k++) {
+            cles::free_line_plan(dplan[k]);
+            cles::free_line_plan(fplan[k]);
+            if (own_fplanQ) cles::free_line_plan(fplanQ[k]);
+        }
+        if (ctx) cles::ctx_release(ctx);
+    }
+};
+
+namespace cles {
+
+// Phases of the per-step path, after bench/phases.jl:119-193.  The five line
+// phases are split by sweep direction (x, y, z).
+enum Phase {
+    PH_DERIV = 0,           // + dim: velocity and temperature gradients
+    PH_DIV = 3,             // + dim: flux divergence (store / accumulate)
+    PH_DIV_LSRK = 6,        // + dim: last divergence pass fused with the LSRK update
+    PH_SMOOTH = 9,          // + dim: sensor smoothing filters
+    PH_FILTER_STATE = 12,   // + dim: conserved-state filter
+    PH_PRIMITIVES = 15,
+    PH_STRAIN,
+    PH_DELTA4,
+    PH_FLUXES,
+    PH_RATE,
+    PH_BCS,
+    PH_LSRK,
+    PH_OTHER,
+    PH_N
+};
+static const char *const PHASE_NAMES[PH_N] = {
+    "line_deriv_x", "line_deriv_y", "line_deriv_z", "line_div_x", "line_div_y", "line_div_z",
+    "line_div_lsrk_x", "line_div_lsrk_y", "line_div_lsrk_z", "line_smooth_x", "line_smooth_y", "line_smooth_z",
+    "line_filter_state_x", "line_filter_state_y", "line_filter_state_z", "primitives", "strain", "delta4",
+    "fluxes", "max_rate", "bcs", "lsrk", "other"};
+
+template <typename W>
+struct Impl {
+    cles_solver &s;
+    explicit Impl(cles_solver &s_) : s(s_) {}
+
+    W *f(void *ptr) const { return (W *)ptr; }
+    View3 view() const
+    {
+        View3 v;
+        for (int k = 0; k < 3; k++) v.n[k] = s.n[k];
+        v.s[0] = 1;
+        v.s[1] = s.n[0];
+        v.s[2] = s.n[0] * s.n[1];
+        return v;
+    }
+    GradPtrs<W> grads() const
+    {
+        GradPtrs<W> g;
+        for (int m = 0; m < 9; m++) g.gu[m] = f(s.gu[m]);
+        return g;
+    }
+    // algorithmic bytes per point of one line operator with epilogue kind
+    double line_bytes(const HostPlan &hp, int kind) const
+    {
+        const double w = hp.dtype == CLES_F64 ? 8.0 : 4.0;
+        switch (kind) {
+        case EPI_SUB: return 3 * w;                 // read f, read-modify-write out
+        case EPI_LSRK: return 2 * w + 32;           // f, dQ; du and Q read + written
+        case EPI_ART_MUBETA: return 4 * w;          // sensor, rho -> mu, beta
+        case EPI_ART_KAPPA: return 5 * w;           // sensor, rho, c, T -> kappa
+        case EPI_ART_D: return 3 * w;               // sensor, c -> D
+        default: return 2 * w;                      // read f, write out
+        }
+    }
+    void line(const HostPlan &hp, int d, const void *in, EpiArgs e, int phase) const
+    {
+        s.timed(phase + d, line_bytes(hp, e.kind) * (double)s.N,
+                [&] { launch_line(hp, d + 1, view(), in, e, s.st()); });
+    }
+    EpiArgs store(void *out, int kind = EPI_STORE) const
+    {
+        EpiArgs e{};
+        e.kind = kind;
+        e.out = out;
+        return e;
+    }
+
+    // physics.jl:604-653
+    void primitives() const
+    {
+        PrimArgs<W> a{};
+        a.Q = s.Q;
+        a.Ns = s.Ns;
+        for (int k = 0; k < s.Ns; k++) { a.Rk[k] = s.Rk[k]; a.cvk[k] = s.cvk[k]; a.Y[k] = f(s.Y[k]); }
+        a.rho = f(s.rho); a.u = f(s.u); a.v = f(s.v); a.w = f(s.w);
+        a.p = f(s.p); a.T = f(s.T); a.c = f(s.c); a.cp = f(s.cp);
+        a.esens = f(s.esens);
+        const double w = sizeof(W);
+        s.timed(PH_PRIMITIVES, (8.0 * s.C + w * (8 + s.Ns + (s.esens ? 1 : 0))) * (double)s.N,
+                [&] { launch_primitives<W>(a, s.N, s.st()); });
+    }
+
+    // rhs.jl:591-606 (velocity gradients of the active dimensions)
+    void velocity_gradients() const
+    {
+        void *vel[3] = {s.u, s.v, s.w};
+        for (int jj = 0; jj < 3; jj++)
+            for (int k = 0; k < s.nact; k++) {
+                const int d = s.adim[k];
+                line(s.dplan[d], d, vel[jj], store(s.gu[d * 3 + jj]), PH_DERIV);
+            }
+    }
+
+    // artificial.jl:98-111 smooth! in place, the last pass with epilogue `last`
+    void smooth(void *field, EpiArgs last) const
+    {
+        for (int k = 0; k < s.nact; k++) {
+            const int d = s.adim[k];
+            if (k < s.nact - 1)
+                line(s.fplan[d], d, field, store(field), PH_SMOOTH);
+            else
+                line(s.fplan[d], d, field, last, PH_SMOOTH);
+        }
+    }
+
+    D4Args<W> d4(int wpow) const
+    {
+        D4Args<W> a{};
+        for (int k = 0; k < 3; k++) {
+            a.n[k] = s.n[k];
+            a.active[k] = s.active[k];
+            a.periodic[k] = s.periodic[k];
+            a.wd[k] = wpow == 2 ? s.h[k] * s.h[k] : s.h[k];
+        }
+        a.N = s.N;
+        a.Q = s.Q;
+        a.rho = f(s.rho);
+        a.im1 = s.Ns;
+        return a;
+    }
+
+    // artificial.jl:261-359 compute_artificial!
+    void artificial() const
+    {
+        const cles_solver_desc &d = s.d;
+        if (!d.art_enabled) return;
+        const double w = sizeof(W);
+        s.timed(PH_STRAIN, w * (3 * s.nact + 1) * (double)s.N,
+                [&] { launch_strain<W>(grads(), f(s.strain), s.N, s.st()); });
+        D4Args<W> a = d4(2);
+        a.f = f(s.strain);
+        s.timed(PH_DELTA4, 2 * w * (double)s.N, [&] { launch_delta4<W>(a, 0, f(s.sensor), s.st()); });
+        EpiArgs mb{};
+        mb.kind = EPI_ART_MUBETA;
+        mb.out = s.mu_art;
+        mb.out2 = s.beta_art;
+        mb.a0 = s.rho;
+        mb.c0 = d.C_mu;
+        mb.c1 = d.C_beta;
+        smooth(s.sensor, mb);
+        if (d.beta_sensor == 1) {
+            s.timed(PH_OTHER, w * (3 * s.nact + 2) * (double)s.N,
+                    [&] { launch_gate_beta<W>(grads(), f(s.beta_art), s.N, s.st()); });
+        } else if (d.beta_sensor == 2) {
+            s.timed(PH_OTHER, w * (3 * s.nact + 2) * (double)s.N,
+                    [&] { launch_dilatation_prep<W>(grads(), f(s.tmp1), f(s.tmp2), s.N, s.st()); });
+            D4Args<W> b = d4(2);
+            b.f = f(s.tmp1);
+            s.timed(PH_DELTA4, 2 * w * (double)s.N, [&] { launch_delta4<W>(b, 0, f(s.sensor), s.st()); });
+            smooth(s.sensor, store(s.sensor));
+            s.timed(PH_OTHER, 4 * w * (double)s.N, [&] {
+                launch_dilatation_beta<W>(f(s.rho), f(s.sensor), f(s.tmp2), f(s.beta_art), d.C_beta, s.N, s.st());
+            });
+        }
+        // kappa* from the internal-energy sensor (e stored by the primitives pass)
+        D4Args<W> ae = d4(1);
+        ae.f = f(s.esens);
+        s.timed(PH_DELTA4, 2 * w * (double)s.N, [&] { launch_delta4<W>(ae, 0, f(s.sensor), s.st()); });
+        EpiArgs kp{};
+        kp.kind = EPI_ART_KAPPA;
+        kp.out = s.kappa_art;
+        kp.a0 = s.rho;
+        kp.a1 = s.c;
+        kp.a2 = s.T;
+        kp.c0 = d.C_kappa;
+        smooth(s.sensor, kp);
+        if (s.Ns > 1) {
+            for (int sp = 0; sp < s.Ns; sp++) {
+                D4Args<W> b = d4(1);
+                b.f = f(s.Y[sp]);
+                s.timed(PH_DELTA4, 2 * w * (double)s.N, [&] { launch_delta4<W>(b, 0, f(s.sensor), s.st()); });
+                EpiArgs dd{};
+                dd.kind = EPI_ART_D;
+                dd.out = s.D_art[sp];
+                dd.a1 = s.c;
+                dd.c0 = d.C_D;
+                smooth(s.sensor, dd);
+            }
+        }
+    }
+
+    void fluxes() const
+    {
+        FluxArgs<W> a{};
+        a.Q = s.Q;
+        a.Ns = s.Ns;
+        for (int k = 0; k < 3; k++) a.active[k] = s.active[k];
+        a.mu0 = s.d.mu0;
+        a.Pr = s.d.Pr;
+        a.Sc = s.d.Sc;
+        for (int k = 0; k < s.Ns; k++) {
+            a.cpk[k] = s.cpk[k];
+            a.Y[k] = f(s.Y[k]);
+            a.D_art[k] = f(s.D_art[k]);
+        }
+        a.rho = f(s.rho); a.u = f(s.u); a.v = f(s.v); a.w = f(s.w);
+        a.p = f(s.p); a.T = f(s.T); a.cp = f(s.cp);
+        a.mu_art = f(s.mu_art);
+        a.beta_art = f(s.beta_art);
+        a.kappa_art = f(s.kappa_art);
+        a.g = grads();
+        for (int d = 0; d < 3; d++) {
+            a.gT[d] = f(s.gT[d]);
+            for (int k = 0; k < s.Ns; k++) a.gY[d][k] = f(s.gY[d][k]);
+            for (int c = 0; c < s.C; c++) a.flux[d][c] = f(s.flux[d][c]);
+        }
+        const double w = sizeof(W);
+        const double reads = 8.0 + w * (10 + 3 * s.nact + s.nact + s.Ns * (s.Ns > 1 ? 2 + s.nact : 1));
+        s.timed(PH_FLUXES, (reads + w * s.nact * s.C) * (double)s.N, [&] { launch_fluxes<W>(a, s.N, s.st()); });
+    }
+
+    W *dQc(int c) const { return f(s.dQ) + (size_t)c * s.N; }
+
+    // rhs.jl:624-694 compute_rhs!; with fuse, the last flux-divergence pass of
+    // each component performs the LSRK stage update (A, B, dt).
+    void rhs(bool primitives_current, bool fuse, double A, double B, double dt) const
+    {
+        if (!primitives_current) primitives();
+        velocity_gradients();
+        artificial();
+        for (int k = 0; k < s.nact; k++) {
+            const int d = s.adim[k];
+            line(s.dplan[d], d, s.T, store(s.gT[d]), PH_DERIV);
+            if (s.Ns > 1)
+                for (int sp = 0; sp < s.Ns; sp++) line(s.dplan[d], d, s.Y[sp], store(s.gY[d][sp]), PH_DERIV);
+        }
+        fluxes();
+        for (int c = 0; c < s.C; c++) {
+            for (int k = 0; k < s.nact; k++) {
+                const int d = s.adim[k];
+                const bool last = k == s.nact - 1;
+                EpiArgs e{};
+                if (fuse && last) {
+                    e.kind = EPI_LSRK;
+                    e.a0 = s.nact > 1 ? (const void *)dQc(c) : nullptr;
+                    e.du = s.du + (size_t)c * s.N;
+                    e.Q = s.Q + (size_t)c * s.N;
+                    e.c0 = A;
+                    e.c1 = B;
+                    e.dt = dt;
+                } else {
+                    e.kind = k == 0 ? EPI_STORE_NEG : EPI_SUB;
+                    e.out = dQc(c);
+                }
+                line(s.dplan[d], d, s.flux[d][c], e, (fuse && last) ? PH_DIV_LSRK : PH_DIV);
+            }
+        }
+    }
+
+    // timestep.jl:125-175
+    void max_rate(double *rate, double *rho_min) const
+    {
+        primitives();
+        RateArgs<W> a{};
+        a.Q = s.Q;
+        a.Ns = s.Ns;
+        for (int k = 0; k < 3; k++) {
+            a.active[k] = s.active[k];
+            a.invh[k] = 1.0 / s.h[k];
+        }
+        a.mu0 = s.d.mu0;
+        a.Pr = s.d.Pr;
+        a.Sc = s.d.Sc;
+        a.rho = f(s.rho); a.u = f(s.u); a.v = f(s.v); a.w = f(s.w); a.c = f(s.c); a.cp = f(s.cp);
+        a.mu_art = f(s.mu_art);
+        a.beta_art = f(s.beta_art);
+        a.kappa_art = f(s.kappa_art);
+        for (int k = 0; k < s.Ns; k++) a.D_art[k] = f(s.D_art[k]);
+        int nb = (int)std::min<int64_t>(s.nred, (s.N + 255) / 256);
+        const double w = sizeof(W);
+        s.timed(PH_RATE, (8.0 * s.Ns + w * (9 + (s.Ns > 1 ? s.Ns : 0))) * (double)s.N,
+                [&] { launch_rate<W>(a, s.part, nb, s.scal, s.N, s.st()); });
+        CLES_HIP(hipMemcpyAsync(s.host_scal, s.scal, 2 * sizeof(double), hipMemcpyDeviceToHost, s.st()));
+        CLES_HIP(hipStreamSynchronize(s.st()));
+        *rate = s.host_scal[0];
+        *rho_min = s.host_scal[1];
+    }
+
+    void lsrk_all(double A, double B, double dt) const
+    {
+        s.timed(PH_LSRK, (32.0 + sizeof(W)) * s.C * (double)s.N,
+                [&] { launch_lsrk<W>(s.Q, s.du, f(s.dQ), A, B, dt, (int64_t)s.C * s.N, s.st()); });
+    }
+
+    // timestep.jl:72-99 step!
+    void step(double dt, bool prepared) const
+    {
+        const bool fuse = !s.nscbc;
+        for (int stg = 0; stg < 5; stg++) {
+            s.tstage = s.t + RKC[stg] * dt;
+            const bool first_prepared = prepared && stg == 0;
+            if (!first_prepared) apply_bcs();
+            rhs(first_prepared, fuse, RKA[stg], RKB[stg], dt);
+            if (!fuse) {
+                nscbc_corrections();
+                lsrk_all(RKA[stg], RKB[stg], dt);
+            }
+        }
+        s.tstage = s.t + dt;
+        apply_bcs();
+    }
+
+    void nscbc_corrections() const
+    {
+        if (s.nscbc) throw Error{CLES_ERR_UNSUPPORTED, "NSCBC faces are not available in this build"};
+    }
+
+    // boundary.jl:280-286 apply_bcs!
+    void apply_bcs() const
+    {
+        for (int d = 0; d < 3; d++)
+            for (int side = 0; side < 2; side++) {
+                if (!s.active[d]) continue;
+                const int bc = s.d.bc[d][side];
+                if (bc != CLES_BC_SLIPWALL && bc != CLES_BC_DIRICHLET) continue;
+                PlaneArgs a{};
+                a.dim = d;
+                a.side = side;
+                a.Ns = s.Ns;
+                for (int k = 0; k < 3; k++) a.n[k] = s.n[k];
+                a.N = s.N;
+                a.count = s.N / s.n[d];
+                for (int c = 0; c < s.C; c++) a.state[c] = s.d.bc_state[d][side][c];
+                s.timed(PH_BCS, 8.0 * s.C * (double)a.count, [&] {
+                    if (bc == CLES_BC_SLIPWALL)
+                        launch_slipwall(a, s.Q, s.st());
+                    else
+                        launch_dirichlet(a, s.Q, s.st());
+                });
+            }
+    }
+
+    // timestep.jl:414-426 filter_state! (in place, dimension by dimension, FP64)
+    void filter_state() const
+    {
+        View3 v = view();
+        for (int k = 0; k < s.nact; k++) {
+            const int d = s.adim[k];
+            for (int c = 0; c < s.C; c++) {
+                EpiArgs e{};
+                e.kind = EPI_STORE;
+                e.out = s.Q + (size_t)c * s.N;
+                s.timed(PH_FILTER_STATE + d, 16.0 * (double)s.N,
+                        [&] { launch_line(s.fplanQ[d], d + 1, v, s.Q + (size_t)c * s.N, e, s.st()); });
+            }
+        }
+    }
+
+    void diagnostics(double *ke, double *ens) const
+    {
+        DiagArgs<W> a{};
+        a.Q = s.Q;
+        a.Ns = s.Ns;
+        for (int k = 0; k < 3; k++) {
+            a.n[k] = s.n[k];
+            a.closed[k] = s.active[k] && !s.periodic[k];
+        }
+        a.N = s.N;
+        double dV = 1.0;
+        for (int k = 0; k < 3; k++) dV *= s.active[k] ? s.h[k] : 1.0;
+        int nb = (int)std::min<int64_t>(s.nred, (s.N + 255) / 256);
+        launch_integral<W>(a, 0, s.part, nb, dV, s.scal, s.st());
+        primitives();
+        velocity_gradients();
+        a.g = grads();
+        launch_integral<W>(a, 1, s.part, nb, dV, s.scal + 1, s.st());
+        CLES_HIP(hipMemcpyAsync(s.host_scal, s.scal, 2 * sizeof(double), hipMemcpyDeviceToHost, s.st()));
+        CLES_HIP(hipStreamSynchronize(s.st()));
+        *ke = s.host_scal[0];
+        *ens = s.host_scal[1];
+    }
+};
+
+}  // namespace cles
+
+namespace {
+
+template <typename Fn>
+void dispatch(cles_solver *s, Fn &&fn)
+{
+    if (s->prec == CLES_F64) {
+        cles::Impl<double> im(*s);
+        fn(im);
+    } else {
+        cles::Impl<float> im(*s);
+        fn(im);
+    }
+}
+
+// stepcontrol.jl:184-204 check_step
+int check_step(const cles_solver_desc &d, double dt, double rho_min, double dt_seen, std::string &msg)
+{
+    char buf[256];
+    if (!std::isfinite(dt)) { snprintf(buf, sizeof buf, "timestep is %g; the state has gone non-finite", dt); msg = buf; return CLES_ERR_NONFINITE; }
+    if (!(rho_min > 0)) { snprintf(buf, sizeof buf, "minimum mixture density is %g; the state is unphysical", rho_min); msg = buf; return CLES_ERR_NEGATIVE_DENSITY; }
+    if (!(dt > PLANCK_TIME)) { msg = "timestep is below the Planck time"; return CLES_ERR_PLANCK; }
+    if (!(dt >= d.dt_min)) { msg = "timestep is below the configured dt_min"; return CLES_ERR_DT_MIN; }
+    if (d.dt_min_ratio > 0 && dt_seen > 0 && !(dt >= d.dt_min_ratio * dt_seen)) {
+        snprintf(buf, sizeof buf, "timestep has collapsed to %g of the largest seen this run", dt / dt_seen);
+        msg = buf;
+        return CLES_ERR_DT_COLLAPSE;
+    }
+    return CLES_OK;
+}
+
+void validate_desc(const cles_solver_desc &d)
+{
+    if (d.n_species < 1 || d.n_species > CLES_MAXSP) throw cles::Error{CLES_ERR_INVALID, "n_species must be 1..8"};
+    for (int k = 0; k < 3; k++) {
+        if (d.n_global[k] < 1) throw cles::Error{CLES_ERR_INVALID, "n_global must be >= 1"};
+        if (!(d.L_domain[k] > 0)) throw cles::Error{CLES_ERR_INVALID, "domain extents must be positive"};
+        const int lo = d.bc[k][0], hi = d.bc[k][1];
+        if ((lo == CLES_BC_PERIODIC) != (hi == CLES_BC_PERIODIC))
+            throw cles::Error{CLES_ERR_INVALID, "dimension mixes periodic and non-periodic conditions"};
+        if (d.n_global[k] == 1 && lo != CLES_BC_PERIODIC)
+            throw cles::Error{CLES_ERR_INVALID, "collapsed dimension must use (PeriodicBC(), PeriodicBC())"};
+        for (int sd = 0; sd < 2; sd++)
+            if (d.bc[k][sd] < 0 || d.bc[k][sd] > CLES_BC_NSCBC_IN)
+                throw cles::Error{CLES_ERR_INVALID, "unknown boundary-condition code"};
+    }
+    if (d.beta_sensor < 0 || d.beta_sensor > 2)
+        throw cles::Error{CLES_ERR_INVALID, "art.beta_sensor must be :strain, :gated_strain or :dilatation"};
+    if (d.precision != CLES_F64 && d.precision != CLES_F32)
+        throw cles::Error{CLES_ERR_INVALID, "precision must be CLES_F64 or CLES_F32"};
+    if (!(d.cfl > 0)) throw cles::Error{CLES_ERR_INVALID, "cfl must be positive"};
+    if ((double)d.n_global[0] * (double)d.n_global[1] * (double)d.n_global[2] >= 2147483648.0)
+        throw cles::Error{CLES_ERR_UNSUPPORTED, "grids of 2^31 points or more are not supported on one GPU"};
+}
+
+}  // namespace
+
+extern "C" {
+
+int cles_solver_create(cles_ctx *ctx, const cles_solver_desc *desc, cles_solver **out)
+{
+    CLES_TRY({
+        CLES_REQUIRE(ctx && desc && out, "NULL argument");
+        validate_desc(*desc);
+        CLES_HIP(hipSetDevice(ctx->device));
+        auto *s = new cles_solver;
+        try {
+            s->ctx = ctx;
+            cles::ctx_retain(ctx);
+            s->d = *desc;
+            s->Ns = desc->n_species;
+            s->C = s->Ns + 4;
+            s->prec = desc->precision;
+            s->N = 1;
+            for (int k = 0; k < 3; k++) {
+                s->n[k] = desc->n_global[k];
+                s->N *= s->n[k];
+                s->active[k] = s->n[k] > 1;
+                s->periodic[k] = s->active[k] ? desc->bc[k][0] == CLES_BC_PERIODIC : 1;
+                // rhs.jl:198-204
+                s->h[k] = !s->active[k] ? 1.0
+                          : s->periodic[k] ? desc->L_domain[k] / (double)desc->n_global[k]
+                                           : desc->L_domain[k] / (double)(desc->n_global[k] - 1);
+                if (s->active[k]) s->adim[s->nact++] = k;
+                for (int sd = 0; sd < 2; sd++)
+                    if (desc->bc[k][sd] == CLES_BC_NSCBC_IN || desc->bc[k][sd] == CLES_BC_NSCBC_OUT)
+                        s->nscbc = s->active[k] || s->nscbc;
+            }
+            if (s->nscbc) throw cles::Error{CLES_ERR_UNSUPPORTED, "NSCBC faces are not available in this build"};
+            for (int k = 0; k < s->Ns; k++) {   // physics.jl:111-115
+                s->Rk[k] = desc->R[k];
+                s->cvk[k] = desc->R[k] / (desc->gamma[k] - 1);
+                s->cpk[k] = s->Rk[k] + s->cvk[k];
+            }
+            const int wdt = s->prec;
+            for (int k = 0; k < 3; k++) {
+                if (!s->active[k]) continue;
+                for (cles::HostPlan *hp : {&s->dplan[k], &s->fplan[k]}) {
+                    hp->dim = k + 1;
+                    hp->dtype = wdt;
+                    for (int j = 0; j < 3; j++) hp->nloc[j] = s->n[j];
+                }
+                cles::build_line_plan(s->dplan[k], desc->deriv, (int)s->n[k], s->periodic[k], s->h[k], k == 0 ? 16 : 32);
+                cles::build_line_plan(s->fplan[k], desc->filt, (int)s->n[k], s->periodic[k], s->h[k], k == 0 ? 16 : 32);
+            }
+            if (s->prec == CLES_F64) {
+                for (int k = 0; k < 3; k++) s->fplanQ[k] = s->fplan[k];
+            } else {
+                s->own_fplanQ = true;
+                for (int k = 0; k < 3; k++) {
+                    if (!s->active[k]) continue;
+                    s->fplanQ[k].dim = k + 1;
+                    s->fplanQ[k].dtype = CLES_F64;
+                    for (int j = 0; j < 3; j++) s->fplanQ[k].nloc[j] = s->n[j];
+                    cles::build_line_plan(s->fplanQ[k], desc->filt, (int)s->n[k], s->periodic[k], s->h[k], k == 0 ? 16 : 32);
+                }
+            }
+            const size_t N = (size_t)s->N;
+            s->Q = (double *)s->alloc(N * s->C * sizeof(double));
+            s->du = (double *)s->alloc(N * s->C * sizeof(double));
+            s->dQ = s->alloc(N * s->C * s->wsize());
+            for (void **fp : {&s->rho, &s->u, &s->v, &s->w, &s->p, &s->T, &s->c, &s->cp}) *fp = s->wfield();
+            for (int k = 0; k < s->Ns; k++) s->Y[k] = s->wfield();
+            for (int dd = 0; dd < 3; dd++) {
+                if (!s->active[dd]) continue;
+                for (int j = 0; j < 3; j++) s->gu[dd * 3 + j] = s->wfield();
+                s->gT[dd] = s->wfield();
+                if (s->Ns > 1)
+                    for (int k = 0; k < s->Ns; k++) s->gY[dd][k] = s->wfield();
+                for (int c = 0; c < s->C; c++) s->flux[dd][c] = s->wfield();
+            }
+            // artificial coefficients stay zero when disabled (artificial.jl:263-265)
+            s->mu_art = s->wfield();
+            s->beta_art = s->wfield();
+            s->kappa_art = s->wfield();
+            if (s->Ns > 1)
+                for (int k = 0; k < s->Ns; k++) s->D_art[k] = s->wfield();
+            if (desc->art_enabled) {
+                s->sensor = s->wfield();
+                s->strain = s->wfield();
+                s->esens = s->wfield();
+                if (desc->beta_sensor == 2) {
+                    s->tmp1 = s->wfield();
+                    s->tmp2 = s->wfield();
+                }
+            }
+            s->part = (double *)s->alloc((size_t)s->nred * 2 * sizeof(double));
+            s->scal = (double *)s->alloc(16 * sizeof(double));
+            CLES_HIP(hipHostMalloc((void **)&s->host_scal, 16 * sizeof(double), hipHostMallocDefault));
+            CLES_HIP(hipStreamSynchronize(ctx->stream));
+        } catch (...) {
+            delete s;
+            throw;
+        }
+        *out = s;
+    })
+}
+
+int cles_solver_destroy(cles_solver *s)
+{
+    CLES_TRY({ delete s; })
+}
+
+int cles_solver_shape(cles_solver *s, int64_t n[3], int *n_cons)
+{
+    CLES_TRY({
+        CLES_REQUIRE(s && n && n_cons, "NULL argument");
+        for (int k = 0; k < 3; k++) n[k] = s->n[k];
+        *n_cons = s->C;
+    })
+}
+
+static void check_layout(const cles_solver *s, const int64_t padded[3], const int halo[3])
+{
+    for (int k = 0; k < 3; k++)
+        if (halo[k] < 0 || padded[k] < s->n[k] + 2 * (int64_t)halo[k])
+            throw cles::Error{CLES_ERR_INVALID, "padded extents smaller than n + 2*halo"};
+}
+
+int cles_solver_upload(cles_solver *s, const double *Qh, const int64_t padded[3], const int halo[3])
+{
+    CLES_TRY({
+        CLES_REQUIRE(s && Qh && padded && halo, "NULL argument");
+        check_layout(s, padded, halo);
+        const int64_t P0 = padded[0], P1 = padded[1], NP = padded[0] * padded[1] * padded[2];
+        std::vector<double> buf((size_t)s->N * s->C);
+        for (int c = 0; c < s->C; c++)
+            for (int64_t k = 0; k < s->n[2]; k++)
+                for (int64_t j = 0; j < s->n[1]; j++)
+                    std::memcpy(&buf[(size_t)c * s->N + (size_t)s->n[0] * (j + s->n[1] * k)],
+                                Qh + c * NP + halo[0] + P0 * ((j + halo[1]) + P1 * (k + halo[2])),
+                                sizeof(double) * s->n[0]);
+        CLES_HIP(hipMemcpyAsync(s->Q, buf.data(), buf.size() * sizeof(double), hipMemcpyHostToDevice, s->st()));
+        CLES_HIP(hipStreamSynchronize(s->st()));
+    })
+}
+
+int cles_solver_download(cles_solver *s, double *Qh, const int64_t padded[3], const int halo[3])
+{
+    CLES_TRY({
+        CLES_REQUIRE(s && Qh && padded && halo, "NULL argument");
+        check_layout(s, padded, halo);
+        std::vector<double> buf((size_t)s->N * s->C);
+        CLES_HIP(hipMemcpyAsync(buf.data(), s->Q, buf.size() * sizeof(double), hipMemcpyDeviceToHost, s->st()));
+        CLES_HIP(hipStreamSynchronize(s->st()));
+        const int64_t P0 = padded[0], P1 = padded[1], NP = padded[0] * padded[1] * padded[2];
+        for (int c = 0; c < s->C; c++)
+            for (int64_t k = 0; k < s->n[2]; k++)
+                for (int64_t j = 0; j < s->n[1]; j++)
+                    std::memcpy(Qh + c * NP + halo[0] + P0 * ((j + halo[1]) + P1 * (k + halo[2])),
+                                &buf[(size_t)c * s->N + (size_t)s->n[0] * (j + s->n[1] * k)],
+                                sizeof(double) * s->n[0]);
+    })
+}
+
+int cles_solver_get_field(cles_solver *s, const char *name, double *host)
+{
+    CLES_TRY({
+        CLES_REQUIRE(s && name && host, "NULL argument");
+        std::string nm(name);
+        void *src = nullptr;
+        bool is_double = false;
+        int a = 0, b = 0;
+        if (nm == "rho") src = s->rho;
+        else if (nm == "u") src = s->u;
+        else if (nm == "v") src = s->v;
+        else if (nm == "w") src = s->w;
+        else if (nm == "p") src = s->p;
+        else if (nm == "T") src = s->T;
+        else if (nm == "c") src = s->c;
+        else if (nm == "cp") src = s->cp;
+        else if (nm == "mu_art") src = s->mu_art;
+        else if (nm == "beta_art") src = s->beta_art;
+        else if (nm == "kappa_art") src = s->kappa_art;
+        else if (std::sscanf(name, "dQ[%d]", &a) == 1 && a >= 1 && a <= s->C)
+            src = (char *)s->dQ + (size_t)(a - 1) * s->N * s->wsize();
+        else if (std::sscanf(name, "du[%d]", &a) == 1 && a >= 1 && a <= s->C) {
+            src = s->du + (size_t)(a - 1) * s->N;
+            is_double = true;
+        } else if (std::sscanf(name, "grad_u[%1d%1d]", &a, &b) == 2 && a >= 1 && a <= 3 && b >= 1 && b <= 3)
+            src = s->gu[(a - 1) * 3 + (b - 1)];
+        else if (std::sscanf(name, "grad_T[%d]", &a) == 1 && a >= 1 && a <= 3)
+            src = s->gT[a - 1];
+        else if (std::sscanf(name, "Y[%d]", &a) == 1 && a >= 1 && a <= s->Ns)
+            src = s->Y[a - 1];
+        else if (std::sscanf(name, "D_art[%d]", &a) == 1 && a >= 1 && a <= s->Ns)
+            src = s->D_art[a - 1];
+        if (!src) throw cles::Error{CLES_ERR_INVALID, "unknown or unallocated field '" + nm + "'"};
+        std::vector<double> buf((size_t)s->N);
+        if (is_double || s->prec == CLES_F64) {
+            CLES_HIP(hipMemcpyAsync(buf.data(), src, s->N * sizeof(double), hipMemcpyDeviceToHost, s->st()));
+        } else {
+            std::vector<float> tmp((size_t)s->N);
+            CLES_HIP(hipMemcpyAsync(tmp.data(), src, s->N * sizeof(float), hipMemcpyDeviceToHost, s->st()));
+            CLES_HIP(hipStreamSynchronize(s->st()));
+            for (size_t i = 0; i < tmp.size(); i++) buf[i] = tmp[i];
+        }
+        CLES_HIP(hipStreamSynchronize(s->st()));
+        std::memcpy(host, buf.data(), buf.size() * sizeof(double));
+    })
+}
+
+int cles_solver_get_time(cles_solver *s, double *t, int64_t *step)
+{
+    CLES_TRY({
+        CLES_REQUIRE(s && t && step, "NULL argument");
+        *t = s->t;
+        *step = s->step;
+    })
+}
+
+int cles_solver_set_time(cles_solver *s, double t, int64_t step)
+{
+    CLES_TRY({
+        CLES_REQUIRE(s, "NULL argument");
+        s->t = t;
+        s->step = step;
+    })
+}
+
+int cles_apply_bcs(cles_solver *s)
+{
+    CLES_TRY({
+        CLES_REQUIRE(s, "NULL argument");
+        dispatch(s, [](auto &im) { im.apply_bcs(); });
+        CLES_HIP(hipStreamSynchronize(s->st()));
+    })
+}
+
+int cles_compute_rhs(cles_solver *s, int primitives_current)
+{
+    CLES_TRY({
+        CLES_REQUIRE(s, "NULL argument");
+        dispatch(s, [&](auto &im) {
+            im.rhs(primitives_current != 0, false, 0, 0, 0);
+            im.nscbc_corrections();
+        });
+        CLES_HIP(hipStreamSynchronize(s->st()));
+    })
+}
+
+int cles_max_rate(cles_solver *s, double *rate, double *rho_min)
+{
+    CLES_TRY({
+        CLES_REQUIRE(s && rate && rho_min, "NULL argument");
+        dispatch(s, [&](auto &im) { im.max_rate(rate, rho_min); });
+    })
+}
+
+int cles_step(cles_solver *s, double dt, int prepared)
+{
+    CLES_TRY({
+        CLES_REQUIRE(s, "NULL argument");
+        dispatch(s, [&](auto &im) { im.step(dt, prepared != 0); });
+        CLES_HIP(hipStreamSynchronize(s->st()));
+    })
+}
+
+int cles_filter_state(cles_solver *s)
+{
+    CLES_TRY({
+        CLES_REQUIRE(s, "NULL argument");
+        dispatch(s, [&](auto &im) { im.filter_state(); });
+        CLES_HIP(hipStreamSynchronize(s->st()));
+    })
+}
+
+// timestep.jl:277-400 run! (no callbacks, no retries)
+int cles_run(cles_solver *s, double tfinal, int64_t nmax, double *dts, int64_t ndts)
+{
+    CLES_TRY({
+        CLES_REQUIRE(s, "NULL argument");
+        int rc = CLES_OK;
+        std::string msg;
+        dispatch(s, [&](auto &im) {
+            // fresh Workspace (timestep.jl:402-405)
+            CLES_HIP(hipMemsetAsync(s->du, 0, (size_t)s->N * s->C * sizeof(double), s->st()));
+            CLES_HIP(hipMemsetAsync(s->dQ, 0, (size_t)s->N * s->C * s->wsize(), s->st()));
+            double dt_seen = 0.0;
+            int64_t taken = 0;
+            while (s->t < tfinal && s->step < nmax) {
+                s->tstage = s->t;
+                im.apply_bcs();
+                double rate = 0, rho_min = 0;
+                im.max_rate(&rate, &rho_min);
+                double r = rate;   // timestep.jl:29-42 predicted_dt
+                if (s->d.predict > 0 && s->rate_prev > 0) r = jmax(r, r + s->d.predict * (r - s->rate_prev));
+                double dt = s->d.cfl / r;
+                if (s->d.max_growth > 0 && s->dt_prev > 0) dt = jmin(dt, s->d.max_growth * s->dt_prev);
+                rc = check_step(s->d, dt, rho_min, dt_seen, msg);
+                if (rc != CLES_OK) return;
+                dt_seen = jmax(dt_seen, dt);
+                dt = jmin(dt, tfinal - s->t);
+                im.step(dt, true);
+                if (dts && taken < ndts) dts[taken] = dt;
+                taken++;
+                s->t += dt;
+                s->step += 1;
+                s->dt_prev = dt;
+                s->rate_prev = rate;
+                if (s->d.filter_interval > 0 && s->step % s->d.filter_interval == 0) im.filter_state();
+            }
+            CLES_HIP(hipStreamSynchronize(s->st()));
+        });
+        if (rc != CLES_OK) throw cles::Error{rc, msg};
+    })
+}
+
+int cles_solver_profile(cles_solver *s, int enable)
+{
+    CLES_TRY({
+        CLES_REQUIRE(s, "NULL argument");
+        s->resolve();
+        if (enable) s->phases.assign(cles::PH_N, cles_solver::Phase{});
+        s->prof = enable != 0;
+    })
+}
+
+int cles_solver_profile_read(cles_solver *s, int index, char *name, int namelen, double *ms, int64_t *launches,
+                             double *bytes)
+{
+    CLES_TRY({
+        CLES_REQUIRE(s && name && namelen > 0 && ms && launches && bytes, "NULL argument");
+        s->resolve();
+        if (index < 0 || index >= cles::PH_N || s->phases.empty())
+            throw cles::Error{CLES_ERR_INVALID, "no such phase"};
+        std::snprintf(name, (size_t)namelen, "%s", cles::PHASE_NAMES[index]);
+        *ms = s->phases[index].ms;
+        *launches = s->phases[index].launches;
+        *bytes = s->phases[index].bytes;
+    })
+}
+
+int cles_diagnostics(cles_solver *s, double *ke, double *ens)
+{
+    CLES_TRY({
+        CLES_REQUIRE(s && ke && ens, "NULL argument");
+        dispatch(s, [&](auto &im) { im.diagnostics(ke, ens); });
+    })
+}
+
+}  // extern "C"

@@ -1,0 +1,7 @@
+// Explicit instantiation: double line operators, q=1 M=4 sym=0.
+#include "cles_line.hpp"
+
+namespace cles {
+template void launch_epi<double, 1, 4, 0>(const HostPlan &, int, const View3 &, const double *, const EpiArgs &,
+                                            hipStream_t);
+}  // namespace cles

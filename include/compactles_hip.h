@@ -102,6 +102,9 @@ int cles_memcpy_h2d(cles_ctx *ctx, void *dst, const void *src, size_t bytes);
 int cles_memcpy_d2h(cles_ctx *ctx, void *dst, const void *src, size_t bytes);
 int cles_memset(cles_ctx *ctx, void *dptr, int value, size_t bytes);
 int cles_synchronize(cles_ctx *ctx);
+/* Device-to-device copy done by a streaming kernel (it also serves as the
+ * known-size calibration kernel for hardware byte counters). */
+int cles_memcpy_d2d(cles_ctx *ctx, void *dst, const void *src, size_t bytes);
 
 /* ----------------------------------------------------------------------- */
 /* Operator plugin: plan_direction / apply_along!                           */
@@ -210,6 +213,16 @@ int cles_run(cles_solver *solver, double tfinal, int64_t nmax, double *dts, int6
  * (src/io.jl:332-340), both reduced in FP64 in a fixed order
  * (src/diagnostics.jl:61-80 volume_integral). */
 int cles_diagnostics(cles_solver *solver, double *ke, double *enstrophy);
+
+/* Per-phase timing of the solver's launches with HIP events on its stream
+ * (phases modelled on bench/phases.jl:119-193; adds a few microseconds per
+ * launch, so leave it off for throughput runs).  enable = 1 starts a fresh
+ * accumulation, 0 stops.  cles_solver_profile_read returns phase `index`
+ * (name, total milliseconds, launches, algorithmic bytes moved) and
+ * CLES_ERR_INVALID past the last phase. */
+int cles_solver_profile(cles_solver *solver, int enable);
+int cles_solver_profile_read(cles_solver *solver, int index, char *name, int namelen,
+                             double *ms_total, int64_t *launches, double *bytes_total);
 
 /* Device-resident state geometry: interior extents and per-field element
  * count (the solver stores unpadded column-major fields). */
