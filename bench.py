@@ -75,6 +75,27 @@ def cpu_baseline(n, steps=1):
                       f"{steps} full run! step(s), {dt:.2f} s"}
 
 
+def kernel_only(ch, ctx, n):
+    import ctypes as C
+
+    import numpy as np
+    dec = ch.Decomp((n, n, n), (True, True, True), n_halo=0)
+    f = ch.DeviceArray.from_numpy(ctx, np.asfortranarray(np.random.default_rng(0).standard_normal((n, n, n))))
+    out = ch.DeviceArray(ctx, (n, n, n))
+    plan = ch.plan_direction(ctx, dec, ch.lele_d1_6(), 3, 2 * np.pi / n)
+    ms = ch.operators.apply_along_timed(out, plan, f, dec, 10)
+    nbytes = 1 << 30
+    a = ch.DeviceArray(ctx, (nbytes // 8,))
+    b = ch.DeviceArray(ctx, (nbytes // 8,))
+    a.zero()
+    for _ in range(3):
+        ch._lib.check(ctx.lib.cles_memcpy_d2d(ctx.handle, b.ptr, a.ptr, C.c_size_t(nbytes)))
+    ctx.synchronize()
+    print(json.dumps({"kernel": DOMINANT_KERNEL, "launches": 10, "avg_launch_us": ms * 1e3,
+                      "alg_bytes_per_launch": 16 * n ** 3, "copy_bytes_read": nbytes, "copy_bytes_written": nbytes,
+                      "copy_launches": 3}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -85,6 +106,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-n", type=int, default=256)
     ap.add_argument("--profile-steps", type=int, default=2)
+    ap.add_argument("--kernel-only", action="store_true",
+                    help="run only the dominant kernel (C6 z-derivative, 256^3 FP64) 10 times and a 1 GiB "
+                         "streaming copy 3 times (the byte-counter calibration run under rocprofv3 --pmc)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -94,6 +118,9 @@ def main():
     import compactles_hip as ch
     ctx = ch.Context(0)
     n = args.n
+    if args.kernel_only:
+        kernel_only(ch, ctx, n)
+        return
     prob, num = tgv_problem(ch, n, args.precision)
     solver, _ = ch.setup(prob, num, ctx)
     solver.run(1e9, args.warmup)

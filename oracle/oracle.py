@@ -117,6 +117,11 @@ def lib():
     return _lib
 
 
+class OwnedArray(np.ndarray):
+    """A numpy view of solver-owned memory that keeps the solver alive."""
+    _owner = None
+
+
 def _dptr(a: np.ndarray):
     assert a.dtype == np.float64 and a.flags.f_contiguous
     return a.ctypes.data_as(C.POINTER(C.c_double))
@@ -211,7 +216,9 @@ class Solver:
         shape = self.padded if ncomp is None else self.padded + (ncomp,)
         cnt = int(np.prod(shape))
         buf = np.ctypeslib.as_array(ptr, shape=(cnt,))
-        return buf.reshape(shape, order="F")
+        out = buf.reshape(shape, order="F").view(OwnedArray)
+        out._owner = self
+        return out
 
     @property
     def Q(self) -> np.ndarray:

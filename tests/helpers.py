@@ -64,6 +64,6 @@ def fourier_field(n, kmax, seed, slope=-5.0 / 3.0):
                 else:
                     F[kx % n[0], ky % n[1], 0] += c
                     F[(-kx) % n[0], (-ky) % n[1], 0] += np.conj(c)
-    f = np.fft.irfftn(F, s=n) * (n[0] * n[1] * n[2])
+    f = np.fft.irfftn(F, s=n, axes=(0, 1, 2)) * (n[0] * n[1] * n[2])
     f /= np.sqrt(np.mean(f * f))
     return np.asfortranarray(f)

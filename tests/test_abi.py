@@ -1,0 +1,113 @@
+"""The C ABI boundary, without a GPU: the library loads, exports every
+function include/compactles_hip.h declares, the ctypes records match the C
+layout, and the two independent restatements of the scheme presets (the
+product's Python mirror and the oracle) agree field by field."""
+import ctypes as C
+import os
+import re
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "compactles_hip.h")
+
+
+def header_functions():
+    src = open(HEADER).read()
+    return sorted(set(re.findall(r"^\s*(?:int|const char \*|void \*)\s*(cles_\w+)\s*\(", src, re.M)))
+
+
+def test_library_exports_every_declared_symbol():
+    import compactles_hip as ch
+    lib = ch.load()
+    declared = header_functions()
+    assert len(declared) >= 30
+    missing = [f for f in declared if not hasattr(lib, f)]
+    assert not missing, missing
+    from compactles_hip import _lib
+    assert sorted(_lib.SIGNATURES) == declared
+
+
+def test_abi_version():
+    import compactles_hip as ch
+    assert ch.load().cles_abi_version() == 1
+
+
+def test_struct_layout_matches_header():
+    """sizeof/offsetof from the C compiler against the ctypes records."""
+    from compactles_hip import _lib as L
+    prog = r'''
+#include <stdio.h>
+#include <stddef.h>
+#include "compactles_hip.h"
+int main(void) {
+  printf("%zu %zu %zu %zu\n", sizeof(cles_scheme_desc), sizeof(cles_nscbc_desc), sizeof(cles_solver_desc),
+         offsetof(cles_solver_desc, precision));
+  printf("%zu %zu %zu %zu\n", offsetof(cles_scheme_desc, closure_rhs), offsetof(cles_solver_desc, filt),
+         offsetof(cles_solver_desc, bc_state), offsetof(cles_solver_desc, nscbc));
+  return 0;
+}
+'''
+    with tempfile.TemporaryDirectory() as td:
+        c = os.path.join(td, "p.c")
+        exe = os.path.join(td, "p")
+        open(c, "w").write(prog)
+        subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), c, "-o", exe], check=True)
+        out = subprocess.run([exe], capture_output=True, text=True, check=True).stdout.split()
+    got = [int(x) for x in out]
+    want = [C.sizeof(L.SchemeDesc), C.sizeof(L.NscbcDesc), C.sizeof(L.SolverDesc), L.SolverDesc.precision.offset,
+            L.SchemeDesc.closure_rhs.offset, L.SolverDesc.filt.offset, L.SolverDesc.bc_state.offset,
+            L.SolverDesc.nscbc.offset]
+    assert got == want
+
+
+@pytest.mark.parametrize("name", ["lele_d1_6", "pade_d1_4", "lele_d1_10", "compact_filter"])
+def test_scheme_presets_agree_with_oracle(name):
+    import compactles_hip as ch
+    import oracle as O
+    py = getattr(ch, name)().to_desc()
+    orc = O.scheme(name)
+    assert py.q == orc.q and py.M == orc.M and py.symmetric == orc.symmetric and py.nclosure == orc.nclosure
+    assert py.a0 == orc.a0
+    assert list(py.lhs) == list(orc.lhs)
+    assert list(py.coeffs) == list(orc.coeffs)
+    for j in range(py.nclosure):
+        w = 2 * py.q + 1
+        assert list(py.closure_lhs[j])[:w] == list(orc.cl_lhs[j])[:w]
+        assert py.closure_rhs_len[j] == orc.cl_rhs_len[j]
+        assert list(py.closure_rhs[j]) == list(orc.cl_rhs[j])
+
+
+def test_no_device_is_a_loud_error():
+    """Without a gfx950 device the context cannot be created (no host fallback)."""
+    import compactles_hip as ch
+    code = "import sys; sys.path.insert(0, %r)\nimport compactles_hip as ch\n" \
+           "try:\n    ch.Context(0)\nexcept ch.ClesError as e:\n    print('ERR', e.code)\nelse:\n    print('OK')\n" % \
+           os.path.join(ROOT, "compactles.jl_amd")
+    r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=120)
+    has_gpu = os.path.exists("/dev/kfd")
+    if not has_gpu:
+        assert "ERR" in r.stdout, (r.stdout, r.stderr)
+
+
+def test_bench_refuses_multi_gpu():
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=60)
+    assert r.returncode == 2 and "one GPU" in r.stderr
+
+
+def test_python_frontend_validation():
+    """problem.jl:49-57 Prim validation and conserved_from_prim."""
+    import compactles_hip as ch
+    with pytest.raises(ValueError):
+        ch.Prim(p=1.0, rho=1.0, T_ion=1.0)
+    with pytest.raises(ValueError):
+        ch.Prim(p=1.0, rho=1.0, Y=(0.5, 0.4))
+    eos = ch.single_species(gamma=1.4)
+    q = ch.conserved_from_prim(eos, ch.Prim(u=(1.0, 2.0, 3.0), p=0.4, rho=2.0))
+    # rho E = p/(gamma-1) + rho |u|^2 / 2
+    assert np.allclose(q, [2.0, 2.0, 4.0, 6.0, 0.4 / 0.4 + 0.5 * 2.0 * 14.0], rtol=1e-15)
