@@ -1,6 +1,7 @@
 """ctypes binding of libcompactles_hip.so (the C ABI in include/compactles_hip.h).
 
-The library is loaded from the package directory's parent (in-tree build).
+The library is loaded from the package directory's parent (in-tree build), or
+from the path in $COMPACTLES_HIP_LIB.
 There is no fallback: if the shared library is missing or no gfx950 device is
 present, calls raise instead of computing anything on the host.
 """
@@ -10,7 +11,7 @@ import ctypes as C
 import os
 
 _PKG = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(os.path.dirname(_PKG), "libcompactles_hip.so")
+LIB_PATH = os.environ.get("COMPACTLES_HIP_LIB") or os.path.join(os.path.dirname(_PKG), "libcompactles_hip.so")
 
 MAXCL, MAXRHS, MAXM, MAXSP = 8, 16, 8, 8
 

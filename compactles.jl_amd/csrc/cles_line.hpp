@@ -77,6 +77,82 @@ __device__ __forceinline__ void epilogue(const EpiArgs &e, int64_t o, double v)
     }
 }
 
+// Batched epilogue: every input of NB points is loaded before any output is
+// stored, so the loads of a batch are in flight together (the compiler cannot
+// reorder them across stores through pointers that may alias).
+template <typename T, int EPI, int NB>
+__device__ __forceinline__ void epilogue_batch(const EpiArgs &e, const int64_t (&o)[NB], const T (&v)[NB],
+                                               const bool (&ok)[NB])
+{
+    if constexpr (EPI == EPI_STORE || EPI == EPI_STORE_NEG) {
+#pragma unroll
+        for (int i = 0; i < NB; i++)
+            if (ok[i]) epilogue<T, EPI>(e, o[i], (double)v[i]);
+    } else if constexpr (EPI == EPI_SUB) {
+        T *out = (T *)e.out;
+        T a[NB];
+#pragma unroll
+        for (int i = 0; i < NB; i++) a[i] = ok[i] ? out[o[i]] : T(0);
+#pragma unroll
+        for (int i = 0; i < NB; i++)
+            if (ok[i]) out[o[i]] = a[i] - v[i];
+    } else if constexpr (EPI == EPI_LSRK) {
+        const T *acc = (const T *)e.a0;
+        T a[NB];
+        double du[NB], q[NB];
+#pragma unroll
+        for (int i = 0; i < NB; i++) {
+            a[i] = (ok[i] && acc) ? acc[o[i]] : T(0);
+            du[i] = ok[i] ? e.du[o[i]] : 0.0;
+            q[i] = ok[i] ? e.Q[o[i]] : 0.0;
+        }
+#pragma unroll
+        for (int i = 0; i < NB; i++) {
+            if (!ok[i]) continue;
+            const double dq = acc ? (double)(a[i] - v[i]) : -(double)v[i];
+            const double nv = e.c0 * du[i] + e.dt * dq;
+            e.du[o[i]] = nv;
+            e.Q[o[i]] = q[i] + e.c1 * nv;
+        }
+    } else if constexpr (EPI == EPI_ART_MUBETA) {
+        const T *rho = (const T *)e.a0;
+        T r[NB];
+#pragma unroll
+        for (int i = 0; i < NB; i++) r[i] = ok[i] ? rho[o[i]] : T(0);
+#pragma unroll
+        for (int i = 0; i < NB; i++) {
+            if (!ok[i]) continue;
+            const T rs = r[i] * (v[i] > T(0) ? v[i] : T(0));
+            ((T *)e.out)[o[i]] = (T)e.c0 * rs;
+            ((T *)e.out2)[o[i]] = (T)e.c1 * rs;
+        }
+    } else if constexpr (EPI == EPI_ART_KAPPA) {
+        const T *rho = (const T *)e.a0, *c = (const T *)e.a1, *Tt = (const T *)e.a2;
+        T r[NB], cc[NB], tt[NB];
+#pragma unroll
+        for (int i = 0; i < NB; i++) {
+            r[i] = ok[i] ? rho[o[i]] : T(0);
+            cc[i] = ok[i] ? c[o[i]] : T(0);
+            tt[i] = ok[i] ? Tt[o[i]] : T(1);
+        }
+        const T eps = sizeof(T) == 8 ? (T)2.220446049250313e-16 : (T)1.1920929e-07f;
+#pragma unroll
+        for (int i = 0; i < NB; i++) {
+            if (!ok[i]) continue;
+            const T scale = r[i] * cc[i] / (tt[i] > eps ? tt[i] : eps);
+            ((T *)e.out)[o[i]] = (T)e.c0 * scale * (v[i] > T(0) ? v[i] : T(0));
+        }
+    } else if constexpr (EPI == EPI_ART_D) {
+        const T *c = (const T *)e.a1;
+        T cc[NB];
+#pragma unroll
+        for (int i = 0; i < NB; i++) cc[i] = ok[i] ? c[o[i]] : T(0);
+#pragma unroll
+        for (int i = 0; i < NB; i++)
+            if (ok[i]) ((T *)e.out)[o[i]] = (T)e.c0 * cc[i] * (v[i] > T(0) ? v[i] : T(0));
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Per-thread chunk machinery
 // ---------------------------------------------------------------------------
@@ -331,11 +407,20 @@ __global__ void __launch_bounds__(Q == 1 ? 1024 : 512) k_line_yz(LineDev p, cons
     reduced_solve<Q>(p, smem, NT, Lx, k, lane, r, tp, hn);
     if (!valid) return;
 #pragma unroll
-    for (int t = 0; t < MC; t++) {
-        if (t < len) {
-            const int g = s + t;
-            epilogue<T, EPI>(e, off + (int64_t)g * sd, corrected<T, Q>(p, g, ch.d[t], tp, hn));
+    for (int t = 0; t < MC; t++)
+        if (t < len) ch.d[t] = (T)corrected<T, Q>(p, s + t, ch.d[t], tp, hn);
+#pragma unroll
+    for (int b = 0; b < MC; b += 8) {
+        int64_t o[8];
+        T v[8];
+        bool ok[8];
+#pragma unroll
+        for (int i = 0; i < 8; i++) {
+            ok[i] = b + i < len;
+            o[i] = off + (int64_t)(s + b + i) * sd;
+            v[i] = ch.d[b + i];
         }
+        epilogue_batch<T, EPI, 8>(e, o, v, ok);
     }
 }
 
@@ -503,61 +588,109 @@ __global__ void __launch_bounds__(64) k_line_xw(LineDev p, const T *__restrict__
     const int P = p.P;
     const int n = p.n;
     const int lane = threadIdx.x;
-    const int lpw = 64 / P;
-    const int64_t l0 = (int64_t)blockIdx.x * lpw;
-    // stage the lines
-    for (int li = 0; li < lpw; li++) {
-        const int64_t l = l0 + li;
-        if (l >= nlines) break;
-        const T *src = f + (l % ny) * sy + (l / ny) * sz;
-        for (int x = lane; x < n; x += 64) tile[li * TS + xw_slot(x)] = src[x];
+    const int nl = 64 / P;           // lines staged by the wave
+    const int64_t l0 = (int64_t)blockIdx.x * nl;
+    // stage the lines: lane-element i of the block's nl*n points is line lj,
+    // point x (shifts when n is a power of two >= 64, else a division); every
+    // load is issued before the LDS writes.
+    const int cnt = (nl * n + 63) / 64;
+    const bool pow2 = n >= 64 && (n & (n - 1)) == 0;
+    const int sh = pow2 ? __builtin_ctz((unsigned)(n >> 6)) : 0;
+    const int j0 = (int)(l0 % ny), k0 = (int)(l0 / ny);
+    auto line_of = [&](int i, int &lj, int &x) {
+        if (pow2) {
+            lj = i >> sh;
+            x = lane + 64 * (i & ((1 << sh) - 1));
+        } else {
+            const int q = lane + 64 * i;
+            lj = q / n;
+            x = q - lj * n;
+        }
+    };
+    auto line_off = [&](int lj) -> int64_t {
+        int j = j0 + lj, k = k0;
+        while (j >= ny) {
+            j -= ny;
+            k++;
+        }
+        return (int64_t)j * sy + (int64_t)k * sz;
+    };
+    {
+        T tmp[XCL];
+#pragma unroll
+        for (int i = 0; i < XCL; i++) {
+            tmp[i] = T(0);
+            int lj, x;
+            line_of(i, lj, x);
+            if (i < cnt && lj < nl && l0 + lj < nlines) tmp[i] = f[line_off(lj) + x];
+        }
+#pragma unroll
+        for (int i = 0; i < XCL; i++) {
+            int lj, x;
+            line_of(i, lj, x);
+            if (i < cnt && lj < nl) tile[lj * TS + xw_slot(x)] = tmp[i];
+        }
     }
     __syncthreads();
-    const int li = lane / P;
     const int k = lane & (P - 1);
-    const int base = li * P;
-    const T *trow = tile + li * TS;
+    const int base = (lane / P) * P;
     const int s = k * p.m + min(k, p.extra);
     const int len = p.m + (k < p.extra ? 1 : 0);
     const int rs = UNIROW ? 0 : s;
     const int kc = UNIROW ? 0 : k;
-
-    T fv[XCL + 2 * M];
+    T res[XCL];
+    {
+        const T *trow = tile + (lane / P) * TS;
+        T fv[XCL + 2 * M];
 #pragma unroll
-    for (int t = -M; t < XCL + M; t++) {
-        T val = T(0);
-        if (t < len + M) {
-            int r = s + t;
-            bool ok = true;
-            if (r < 0) {
-                if (p.periodic) r += n; else ok = false;
-            } else if (r >= n) {
-                if (p.periodic) r -= n; else ok = false;
+        for (int t = -M; t < XCL + M; t++) {
+            T val = T(0);
+            if (t < len + M) {
+                int r = s + t;
+                bool ok = true;
+                if (r < 0) {
+                    if (p.periodic) r += n; else ok = false;
+                } else if (r >= n) {
+                    if (p.periodic) r -= n; else ok = false;
+                }
+                if (ok) val = trow[xw_slot(r)];
             }
-            if (ok) val = trow[xw_slot(r)];
+            fv[t + M] = val;
         }
-        fv[t + M] = val;
+        T fe[8];
+#pragma unroll
+        for (int kap = 0; kap < 8; kap++) fe[kap] = (p.closed && kap < n) ? trow[xw_slot(n - 1 - kap)] : T(0);
+        Chunk<T, Q, M, SYM, XCL> ch;
+        ch.rhs(p, k, len, fv, fe);
+        ch.solve(p, rs, len);
+        double r[2 * Q], tp[Q], hn[Q];
+        ch.ends(len, r);
+        reduced_solve_wave<Q>(p, k, kc, base, r, tp, hn);
+#pragma unroll
+        for (int t = 0; t < XCL; t++) res[t] = (T)corrected<T, Q>(p, rs + t, ch.d[t], tp, hn);
     }
-    T fe[8];
-#pragma unroll
-    for (int kap = 0; kap < 8; kap++) fe[kap] = (p.closed && kap < n) ? trow[xw_slot(n - 1 - kap)] : T(0);
-    Chunk<T, Q, M, SYM, XCL> ch;
-    ch.rhs(p, k, len, fv, fe);
-    ch.solve(p, rs, len);
-    double r[2 * Q], tp[Q], hn[Q];
-    ch.ends(len, r);
-    reduced_solve_wave<Q>(p, k, kc, base, r, tp, hn);
     __syncthreads();   // all window reads done before the tile is overwritten
-    T *wrow = tile + li * TS;
+    {
+        T *wrow = tile + (lane / P) * TS;
 #pragma unroll
-    for (int t = 0; t < XCL; t++)
-        if (t < len) wrow[xw_slot(s + t)] = (T)corrected<T, Q>(p, rs + t, ch.d[t], tp, hn);
+        for (int t = 0; t < XCL; t++)
+            if (t < len) wrow[xw_slot(s + t)] = res[t];
+    }
     __syncthreads();
-    for (int lj = 0; lj < lpw; lj++) {
-        const int64_t l = l0 + lj;
-        if (l >= nlines) break;
-        const int64_t off = (l % ny) * sy + (l / ny) * sz;
-        for (int x = lane; x < n; x += 64) epilogue<T, EPI>(e, off + x, (double)tile[lj * TS + xw_slot(x)]);
+#pragma unroll
+    for (int b = 0; b < XCL; b += 8) {
+        int64_t o[8];
+        T v[8];
+        bool ok[8];
+#pragma unroll
+        for (int i = 0; i < 8; i++) {
+            int lj, x;
+            line_of(b + i, lj, x);
+            ok[i] = (b + i) < cnt && lj < nl && l0 + lj < nlines;
+            o[i] = ok[i] ? line_off(lj) + x : 0;
+            v[i] = ok[i] ? tile[lj * TS + xw_slot(x)] : T(0);
+        }
+        epilogue_batch<T, EPI, 8>(e, o, v, ok);
     }
 }
 
@@ -595,16 +728,16 @@ void launch_variant(const HostPlan &hp, int dim, const View3 &v, const T *f, con
     const int maxNT = Q == 1 ? 1024 : 512;
     if (dim == 1 && P <= 64 && p.m + (p.extra ? 1 : 0) <= XCL) {
         const int64_t nlines = v.n[1] * v.n[2];
-        const int lpw = 64 / P;
+        const int nl = 64 / P;
         const int TS = p.n + p.n / XCL + 1;
-        const size_t shm = (size_t)lpw * TS * sizeof(T);
-        const int64_t nblk = (nlines + lpw - 1) / lpw;
+        const size_t shm = (size_t)nl * TS * sizeof(T);
+        const int64_t nblk = (nlines + nl - 1) / nl;
         if (p.periodic && p.extra == 0)
-            hipLaunchKernelGGL((k_line_xw<T, Q, M, SYM, EPI, true>), dim3((unsigned)nblk), dim3(64), shm, st, p, f,
-                               v.s[1], v.s[2], (int)v.n[1], nlines, TS, e);
+            hipLaunchKernelGGL((k_line_xw<T, Q, M, SYM, EPI, true>), dim3((unsigned)nblk), dim3(64), shm, st, p,
+                               f, v.s[1], v.s[2], (int)v.n[1], nlines, TS, e);
         else
-            hipLaunchKernelGGL((k_line_xw<T, Q, M, SYM, EPI, false>), dim3((unsigned)nblk), dim3(64), shm, st, p, f,
-                               v.s[1], v.s[2], (int)v.n[1], nlines, TS, e);
+            hipLaunchKernelGGL((k_line_xw<T, Q, M, SYM, EPI, false>), dim3((unsigned)nblk), dim3(64), shm, st, p,
+                               f, v.s[1], v.s[2], (int)v.n[1], nlines, TS, e);
     } else if (dim == 1) {
         const size_t tile_row = (size_t)(p.n + 1) * sizeof(T);
         const int64_t nlines = v.n[1] * v.n[2];
