@@ -10,10 +10,11 @@ stages, filter_state) with the state resident in HBM.  The metric is
 Mpt-steps/s = 256^3 * steps / wall seconds.
 
 Prints one JSON line.  Also reports, from a separate profiling pass, the HIP
-event timing of the dominant kernel (the y/z-sweep C6 derivative with a plain
-store, 16 algorithmic bytes per point) against the 8 TB/s HBM peak, and the
-oracle (C restatement of the reference CPU path) timed on one step of the same
-workload on the host cores.
+event timing of the dominant kernel -- the z-sweep C6 flux divergence fused
+with the LSRK(5,4) register update, 48 algorithmic bytes per point (flux and
+partial dQ read, du and Q read and written) -- against the 8 TB/s HBM peak,
+and the oracle (C restatement of the reference CPU path) timed on one step of
+the same workload on the host cores.
 """
 from __future__ import annotations
 
@@ -29,8 +30,8 @@ sys.path[:0] = [os.path.join(ROOT, "compactles.jl_amd"), os.path.join(ROOT, "ora
 
 HBM_PEAK_GBPS = 8000.0
 ALG_BYTES_PER_POINT_STEP = 8088.0   # reference-dataflow byte model, TGV art on, FP64
-DOMINANT = ("line_deriv_y", "line_deriv_z")
-DOMINANT_KERNEL = "k_line_yz<double, 1, 2, 0, 0, true> (C6 derivative, y/z sweep, store)"
+DOMINANT = ("fluxes",)
+DOMINANT_KERNEL = {"fp64": "cles::k_fluxes<double>", "fp32": "cles::k_fluxes<float>"}
 
 
 def tgv_problem(ch, n, precision):
@@ -45,13 +46,13 @@ def tgv_problem(ch, n, precision):
     return prob, num
 
 
-def read_traffic(n):
-    """HBM bytes per launch of the dominant kernel from the committed rocprofv3
-    --pmc summary (FETCH_SIZE + WRITE_SIZE, calibrated), or None."""
-    path = os.path.join(ROOT, "profiles", f"line_deriv_yz_tgv{n}_pmc.md")
+def read_traffic(n, precision, kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc
+    summary (FETCH_SIZE + WRITE_SIZE, calibrated against a copy), or None."""
+    path = os.path.join(ROOT, "profiles", f"tgv{n}_{precision}_hbm_counters.md")
     if not os.path.exists(path):
         return None
-    m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", open(path).read())
+    m = re.search(r"traffic_bytes_per_launch\[" + re.escape(kernel) + r"\]:\s*([0-9.eE+]+)", open(path).read())
     return float(m.group(1)) if m else None
 
 
@@ -75,15 +76,15 @@ def cpu_baseline(n, steps=1):
                       f"{steps} full run! step(s), {dt:.2f} s"}
 
 
-def kernel_only(ch, ctx, n):
+def counters_run(ch, ctx, n, steps):
+    """Workload of the rocprofv3 --pmc passes: `steps` full run! steps of the TGV
+    case, then a 1 GiB streaming copy three times (the counter calibration)."""
     import ctypes as C
-
-    import numpy as np
-    dec = ch.Decomp((n, n, n), (True, True, True), n_halo=0)
-    f = ch.DeviceArray.from_numpy(ctx, np.asfortranarray(np.random.default_rng(0).standard_normal((n, n, n))))
-    out = ch.DeviceArray(ctx, (n, n, n))
-    plan = ch.plan_direction(ctx, dec, ch.lele_d1_6(), 3, 2 * np.pi / n)
-    ms = ch.operators.apply_along_timed(out, plan, f, dec, 10)
+    prob, num = tgv_problem(ch, n, "fp64")
+    solver, _ = ch.setup(prob, num, ctx)
+    solver.run(1e9, steps)
+    ctx.synchronize()
+    solver.close()
     nbytes = 1 << 30
     a = ch.DeviceArray(ctx, (nbytes // 8,))
     b = ch.DeviceArray(ctx, (nbytes // 8,))
@@ -91,9 +92,8 @@ def kernel_only(ch, ctx, n):
     for _ in range(3):
         ch._lib.check(ctx.lib.cles_memcpy_d2d(ctx.handle, b.ptr, a.ptr, C.c_size_t(nbytes)))
     ctx.synchronize()
-    print(json.dumps({"kernel": DOMINANT_KERNEL, "launches": 10, "avg_launch_us": ms * 1e3,
-                      "alg_bytes_per_launch": 16 * n ** 3, "copy_bytes_read": nbytes, "copy_bytes_written": nbytes,
-                      "copy_launches": 3}), flush=True)
+    print(json.dumps({"kernel": DOMINANT_KERNEL["fp64"], "steps": steps, "copy_bytes_read": nbytes,
+                      "copy_bytes_written": nbytes, "copy_launches": 3}), flush=True)
 
 
 def main():
@@ -106,9 +106,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-n", type=int, default=256)
     ap.add_argument("--profile-steps", type=int, default=2)
-    ap.add_argument("--kernel-only", action="store_true",
-                    help="run only the dominant kernel (C6 z-derivative, 256^3 FP64) 10 times and a 1 GiB "
-                         "streaming copy 3 times (the byte-counter calibration run under rocprofv3 --pmc)")
+    ap.add_argument("--counters", type=int, default=0, metavar="STEPS",
+                    help="run only STEPS run! steps and a 1 GiB streaming copy 3 times (the workload of the "
+                         "rocprofv3 --pmc byte-counter passes), print a short JSON line and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -118,8 +118,8 @@ def main():
     import compactles_hip as ch
     ctx = ch.Context(0)
     n = args.n
-    if args.kernel_only:
-        kernel_only(ch, ctx, n)
+    if args.counters:
+        counters_run(ch, ctx, n, args.counters)
         return
     prob, num = tgv_problem(ch, n, args.precision)
     solver, _ = ch.setup(prob, num, ctx)
@@ -162,9 +162,10 @@ def main():
         "config": {"workload": f"Taylor-Green vortex {n}^3 periodic, full run! step (max_rate + 5 LSRK(5,4) "
                                "stages + filter_state), C6 derivative, C8 filter, Cook artificial "
                                "properties on, cfl 0.5", "n": n, "precision": args.precision},
-        "roofline": {"bound": "hbm", "kernel": DOMINANT_KERNEL, "achieved": round(achieved, 1),
+        "roofline": {"bound": "hbm", "kernel": DOMINANT_KERNEL[args.precision], "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4),
-                     "traffic": read_traffic(n), "avg_launch_us": round(avg_ms * 1e3, 2),
+                     "traffic": read_traffic(n, args.precision, DOMINANT_KERNEL[args.precision]),
+                     "avg_launch_us": round(avg_ms * 1e3, 2),
                      "alg_bytes_per_launch": alg_per_launch, "launches_profiled": launches},
         "step_roofline": {"alg_bytes_per_step": step_alg,
                           "achieved_GBps": round(step_alg / (ms_per_step * 1e-3) / 1e9, 1),
