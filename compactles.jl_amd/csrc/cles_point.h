@@ -67,6 +67,23 @@ struct PlaneArgs {
     double state[CLES_MAXSP + 4];
 };
 
+// One NSCBC face (nscbc.jl:59-154 outflow, :220-306 inflow, constant targets).
+template <typename W>
+struct NscbcArgs {
+    int dim, side, Ns, inflow;
+    int64_t n[3];
+    int64_t N, count;
+    int act1, act2;                 // transverse dimensions active (outflow)
+    const W *rho, *c, *p, *T, *cp, *vel[3];
+    const W *Y[CLES_MAXSP], *gY[CLES_MAXSP];   // gY along dim; NULL reads as 0
+    const W *gu[9];
+    const W *dpn, *dt1, *dt2;       // dp/dx_dim; outflow dp/dx_t1, dp/dx_t2; inflow drho/dx_dim in dt1
+    double Lref, pinf, sigma, beta_t;
+    double ut[3], Tt, Yt[CLES_MAXSP], eta_u, eta_T, eta_t, eta_Y;
+    double Rk[CLES_MAXSP], cvk[CLES_MAXSP];
+    W *dQ;
+};
+
 template <typename W>
 struct DiagArgs {
     const double *Q;
@@ -91,6 +108,8 @@ template <typename W>
 void launch_lsrk(double *Q, double *du, const W *dQ, double A, double B, double dt, int64_t NC, hipStream_t st);
 template <typename W>
 void launch_rate(const RateArgs<W> &a, double *part, int nb, double *out, int64_t N, hipStream_t st);
+template <typename W>
+void launch_nscbc(const NscbcArgs<W> &a, hipStream_t st);
 void launch_slipwall(const PlaneArgs &a, double *Q, hipStream_t st);
 void launch_dirichlet(const PlaneArgs &a, double *Q, hipStream_t st);
 template <typename W>

@@ -413,6 +413,108 @@ __global__ void __launch_bounds__(TPB) k_dirichlet(PlaneArgs a, double *__restri
 }
 
 // ---------------------------------------------------------------------------
+// NSCBC LODI corrections of dQ on one face plane (nscbc.jl:96-152 outflow,
+// :236-306 inflow).  The plane's normal and transverse pressure (inflow:
+// density) derivatives come from full-field line operators run beforehand.
+// Arithmetic is FP64 in the reference's operation order for every precision.
+// ---------------------------------------------------------------------------
+template <typename W>
+__global__ void __launch_bounds__(TPB) k_nscbc(NscbcArgs<W> a)
+{
+    const int64_t t = (int64_t)blockIdx.x * TPB + threadIdx.x;
+    if (t >= a.count) return;
+    const int d = a.dim;
+    const int oa = d == 0 ? 1 : 0, ob = d == 2 ? 1 : 2;
+    int64_t idx[3];
+    idx[d] = a.side == 0 ? 0 : a.n[d] - 1;
+    idx[oa] = t % a.n[oa];
+    idx[ob] = t / a.n[oa];
+    const int64_t I = idx[0] + a.n[0] * (idx[1] + a.n[1] * idx[2]);
+    const int64_t N = a.N;
+    const double rho = a.rho[I], c = a.c[I], p = a.p[I], Tp = a.T[I], cpm = a.cp[I];
+    const double uv[3] = {a.vel[0][I], a.vel[1][I], a.vel[2][I]};
+    const double un = uv[d];
+    const double Ma = fabs(un) / c;
+    if (!(Ma < 1)) return;   // supersonic point: every wave leaves
+    const int t1 = d == 0 ? 1 : 0, t2 = d == 2 ? 1 : 2;
+    const int im = a.Ns;
+    // physics.jl:142-145 eos_phi for an ideal mixture
+    const double Rm = p / (rho * Tp);
+    const double phi = cpm / Rm - 1;
+    const double ke = 0.5 * (uv[0] * uv[0] + uv[1] * uv[1] + uv[2] * uv[2]);
+    const double dpn = 1.0 * (double)a.dpn[I];
+    const double dun = a.gu[d * 3 + d][I];
+    if (!a.inflow) {
+        const double sgn = a.side == 1 ? -1.0 : 1.0;
+        const double Lcomp = a.side == 1 ? (un - c) * (dpn - rho * c * dun) : (un + c) * (dpn + rho * c * dun);
+        double tin = 0.0;
+        if (a.act1) {
+            const double ut = uv[t1];
+            tin += ut * 1.0 * (double)a.dt1[I] + rho * c * c * (double)a.gu[t1 * 3 + t1][I] +
+                   sgn * rho * c * ut * (double)a.gu[t1 * 3 + d][I];
+        }
+        if (a.act2) {
+            const double ut = uv[t2];
+            tin += ut * 1.0 * (double)a.dt2[I] + rho * c * c * (double)a.gu[t2 * 3 + t2][I] +
+                   sgn * rho * c * ut * (double)a.gu[t2 * 3 + d][I];
+        }
+        const double bt = a.beta_t < 0 ? Ma : a.beta_t;
+        const double K = a.sigma * (1 - Ma * Ma) * c / a.Lref;
+        const double dL = K * (p - a.pinf) - bt * tin - Lcomp;
+        const double dd1 = dL / (2 * c * c);
+        const double dd2 = dL / 2;
+        const double dd3 = sgn * dL / (2 * rho * c);
+        for (int k = 0; k < a.Ns; k++) {
+            W *q = a.dQ + (int64_t)k * N + I;
+            *q = (W)((double)*q - (double)a.Y[k][I] * dd1);
+        }
+#pragma unroll
+        for (int m = 0; m < 3; m++) {
+            W *q = a.dQ + (int64_t)(im + m) * N + I;
+            *q = (W)((double)*q - (uv[m] * dd1 + (m == d ? rho * dd3 : 0.0)));
+        }
+        W *qe = a.dQ + (int64_t)(im + 3) * N + I;
+        *qe = (W)((double)*qe - (ke * dd1 + phi * dd2 + rho * un * dd3));
+        return;
+    }
+    const double drn = 1.0 * (double)a.dt1[I];
+    const double K = c / a.Lref;
+    const double L1c = (un - c) * (dpn - rho * c * dun);
+    const double L5c = (un + c) * (dpn + rho * c * dun);
+    const double L2c = un * (c * c * drn - dpn);
+    const double rel_ac = a.eta_u * rho * c * c * (1 - Ma * Ma) / a.Lref * (un - a.ut[d]);
+    const bool lowface = a.side == 0;
+    const double dL1 = lowface ? 0.0 : (-rel_ac - L1c);
+    const double dL5 = lowface ? (rel_ac - L5c) : 0.0;
+    const double c3 = c * c * c;
+    const double dL2 = a.eta_T * rho * c3 * (a.Tt - Tp) / (a.Lref * Tp) - L2c;
+    const double dd1 = dL2 / (c * c) + (dL5 + dL1) / (2 * c * c);
+    const double dd2 = (dL5 + dL1) / 2;
+    const double dd3 = (dL5 - dL1) / (2 * rho * c);
+    const double dd4 = a.eta_t * K * (uv[t1] - a.ut[t1]) - un * (double)a.gu[d * 3 + t1][I];
+    const double dd5 = a.eta_t * K * (uv[t2] - a.ut[t2]) - un * (double)a.gu[d * 3 + t2][I];
+    const double cvm = cpm - Rm;
+    double sphiY = 0.0;
+    for (int k = 0; k < a.Ns; k++) {
+        const double Yk = a.Y[k][I];
+        const double gYk = a.gY[k] ? (double)a.gY[k][I] : 0.0;
+        const double dLs = a.eta_Y * K * (Yk - a.Yt[k]) - un * gYk;
+        W *q = a.dQ + (int64_t)k * N + I;
+        *q = (W)((double)*q - (Yk * dd1 + rho * dLs));
+        // physics.jl:148-152 eos_dphi_dY
+        sphiY += (a.cvk[k] * Rm - a.Rk[k] * cvm) / (Rm * Rm) * dLs;
+    }
+#pragma unroll
+    for (int m = 0; m < 3; m++) {
+        const double extra = m == d ? rho * dd3 : m == t1 ? rho * dd4 : rho * dd5;
+        W *q = a.dQ + (int64_t)(im + m) * N + I;
+        *q = (W)((double)*q - (uv[m] * dd1 + extra));
+    }
+    W *qe = a.dQ + (int64_t)(im + 3) * N + I;
+    *qe = (W)((double)*qe - (ke * dd1 + phi * dd2 + p * sphiY + rho * (un * dd3 + uv[t1] * dd4 + uv[t2] * dd5)));
+}
+
+// ---------------------------------------------------------------------------
 // Diagnostics (diagnostics.jl:41-80): deterministic FP64 block sums with the
 // node-centred edge half-weights, then a single-block fixed-order final sum.
 // KIND 0: 0.5|m|^2/rho from Q.  KIND 1: 0.5|omega|^2 from grad_u.
@@ -563,6 +665,16 @@ void launch_rate(const RateArgs<W> &a, double *part, int nb, double *out, int64_
     hipLaunchKernelGGL(k_rate_final, dim3(1), dim3(TPB), 0, st, part, nb, out);
     CHECK_LAUNCH();
 }
+
+template <typename W>
+void launch_nscbc(const NscbcArgs<W> &a, hipStream_t st)
+{
+    if (a.count <= 0) return;
+    hipLaunchKernelGGL((k_nscbc<W>), dim3(nblocks(a.count)), dim3(TPB), 0, st, a);
+    CHECK_LAUNCH();
+}
+template void launch_nscbc<double>(const NscbcArgs<double> &, hipStream_t);
+template void launch_nscbc<float>(const NscbcArgs<float> &, hipStream_t);
 
 void launch_slipwall(const PlaneArgs &a, double *Q, hipStream_t st)
 {

@@ -57,6 +57,7 @@ struct cles_solver {
     void *Y[CLES_MAXSP] = {}, *gu[9] = {}, *gT[3] = {}, *gY[3][CLES_MAXSP] = {};
     void *mu_art = nullptr, *beta_art = nullptr, *kappa_art = nullptr, *D_art[CLES_MAXSP] = {};
     void *sensor = nullptr, *strain = nullptr, *tmp1 = nullptr, *tmp2 = nullptr, *esens = nullptr;
+    void *nsd[3] = {};                         // NSCBC pressure / density derivatives
     void *flux[3][CLES_MAXSP + 4] = {};
     double *part = nullptr, *scal = nullptr;   // reduction partials, device scalars
     int nred = 1024;
@@ -167,6 +168,7 @@ enum Phase {
     PH_RATE,
     PH_BCS,
     PH_LSRK,
+    PH_NSCBC,
     PH_OTHER,
     PH_N
 };
@@ -174,7 +176,7 @@ static const char *const PHASE_NAMES[PH_N] = {
     "line_deriv_x", "line_deriv_y", "line_deriv_z", "line_div_x", "line_div_y", "line_div_z",
     "line_div_lsrk_x", "line_div_lsrk_y", "line_div_lsrk_z", "line_smooth_x", "line_smooth_y", "line_smooth_z",
     "line_filter_state_x", "line_filter_state_y", "line_filter_state_z", "primitives", "strain", "delta4",
-    "fluxes", "max_rate", "bcs", "lsrk", "other"};
+    "fluxes", "max_rate", "bcs", "lsrk", "nscbc", "other"};
 
 template <typename W>
 struct Impl {
@@ -458,9 +460,66 @@ struct Impl {
         apply_bcs();
     }
 
+    // rhs.jl:688-690: boundary corrections after the flux divergence, faces in
+    // (dim, low, high) order; nscbc.jl:59-154 outflow, :220-306 inflow.
     void nscbc_corrections() const
     {
-        if (s.nscbc) throw Error{CLES_ERR_UNSUPPORTED, "NSCBC faces are not available in this build"};
+        if (!s.nscbc) return;
+        void *vel[3] = {s.u, s.v, s.w};
+        for (int d = 0; d < 3; d++)
+            for (int side = 0; side < 2; side++) {
+                if (!s.active[d]) continue;
+                const int bc = s.d.bc[d][side];
+                if (bc != CLES_BC_NSCBC_OUT && bc != CLES_BC_NSCBC_IN) continue;
+                const cles_nscbc_desc &nd = s.d.nscbc[d][side];
+                const int t1 = d == 0 ? 1 : 0, t2 = d == 2 ? 1 : 2;
+                NscbcArgs<W> a{};
+                a.dim = d;
+                a.side = side;
+                a.Ns = s.Ns;
+                a.inflow = bc == CLES_BC_NSCBC_IN;
+                for (int k = 0; k < 3; k++) a.n[k] = s.n[k];
+                a.N = s.N;
+                a.count = s.N / s.n[d];
+                // full-field derivatives first (nscbc.jl:69-87, :223-224)
+                line(s.dplan[d], d, s.p, store(s.nsd[0]), PH_NSCBC);
+                a.dpn = f(s.nsd[0]);
+                if (a.inflow) {
+                    line(s.dplan[d], d, s.rho, store(s.nsd[1]), PH_NSCBC);
+                    a.dt1 = f(s.nsd[1]);
+                } else {
+                    a.act1 = s.active[t1];
+                    a.act2 = s.active[t2];
+                    if (a.act1) line(s.dplan[t1], t1, s.p, store(s.nsd[1]), PH_NSCBC);
+                    if (a.act2) line(s.dplan[t2], t2, s.p, store(s.nsd[2]), PH_NSCBC);
+                    a.dt1 = f(s.nsd[1]);
+                    a.dt2 = f(s.nsd[2]);
+                }
+                a.rho = f(s.rho); a.c = f(s.c); a.p = f(s.p); a.T = f(s.T); a.cp = f(s.cp);
+                for (int k = 0; k < 3; k++) a.vel[k] = f(vel[k]);
+                for (int k = 0; k < 9; k++) a.gu[k] = f(s.gu[k]);
+                for (int k = 0; k < s.Ns; k++) {
+                    a.Y[k] = f(s.Y[k]);
+                    a.gY[k] = f(s.gY[d][k]);   // NULL for one species: d(1)/dx = 0
+                    a.Rk[k] = s.Rk[k];
+                    a.cvk[k] = s.cvk[k];
+                    a.Yt[k] = nd.Y[k];
+                }
+                a.Lref = nd.Lref > 0 ? nd.Lref : s.d.L_domain[d];
+                a.pinf = nd.pinf;
+                a.sigma = nd.sigma;
+                a.beta_t = nd.beta_t;
+                for (int k = 0; k < 3; k++) a.ut[k] = nd.u[k];
+                a.Tt = nd.T;
+                a.eta_u = nd.eta_u;
+                a.eta_T = nd.eta_T;
+                a.eta_t = nd.eta_t;
+                a.eta_Y = nd.eta_Y;
+                a.dQ = f(s.dQ);
+                const double w = sizeof(W);
+                s.timed(PH_NSCBC, (w * (14 + 2 * s.Ns) + 2 * w * s.C) * (double)a.count,
+                        [&] { launch_nscbc<W>(a, s.st()); });
+            }
     }
 
     // boundary.jl:280-286 apply_bcs!
@@ -618,7 +677,6 @@ int cles_solver_create(cles_ctx *ctx, const cles_solver_desc *desc, cles_solver 
                     if (desc->bc[k][sd] == CLES_BC_NSCBC_IN || desc->bc[k][sd] == CLES_BC_NSCBC_OUT)
                         s->nscbc = s->active[k] || s->nscbc;
             }
-            if (s->nscbc) throw cles::Error{CLES_ERR_UNSUPPORTED, "NSCBC faces are not available in this build"};
             for (int k = 0; k < s->Ns; k++) {   // physics.jl:111-115
                 s->Rk[k] = desc->R[k];
                 s->cvk[k] = desc->R[k] / (desc->gamma[k] - 1);
@@ -676,6 +734,8 @@ int cles_solver_create(cles_ctx *ctx, const cles_solver_desc *desc, cles_solver 
                     s->tmp2 = s->wfield();
                 }
             }
+            if (s->nscbc)
+                for (void *&fp : s->nsd) fp = s->wfield();
             s->part = (double *)s->alloc((size_t)s->nred * 2 * sizeof(double));
             s->scal = (double *)s->alloc(16 * sizeof(double));
             CLES_HIP(hipHostMalloc((void **)&s->host_scal, 16 * sizeof(double), hipHostMallocDefault));
@@ -766,7 +826,10 @@ int cles_solver_get_field(cles_solver *s, const char *name, double *host)
         else if (nm == "kappa_art") src = s->kappa_art;
         else if (std::sscanf(name, "dQ[%d]", &a) == 1 && a >= 1 && a <= s->C)
             src = (char *)s->dQ + (size_t)(a - 1) * s->N * s->wsize();
-        else if (std::sscanf(name, "du[%d]", &a) == 1 && a >= 1 && a <= s->C) {
+        else if (std::sscanf(name, "Q[%d]", &a) == 1 && a >= 1 && a <= s->C) {
+            src = s->Q + (size_t)(a - 1) * s->N;
+            is_double = true;
+        } else if (std::sscanf(name, "du[%d]", &a) == 1 && a >= 1 && a <= s->C) {
             src = s->du + (size_t)(a - 1) * s->N;
             is_double = true;
         } else if (std::sscanf(name, "grad_u[%1d%1d]", &a, &b) == 2 && a >= 1 && a <= 3 && b >= 1 && b <= 3)

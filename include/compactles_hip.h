@@ -187,7 +187,8 @@ int cles_solver_download(cles_solver *solver, double *Q_host,
                          const int64_t padded[3], const int halo[3]);
 /* Download a named internal field's interior (as double) into an unpadded
  * host array: "rho","u","v","w","p","T","c","cp","mu_art","beta_art",
- * "kappa_art","dQ[c]","grad_u[dj]","grad_T[d]","Y[k]","D_art[k]" (1-based). */
+ * "kappa_art","Q[c]","dQ[c]","du[c]","grad_u[dj]","grad_T[d]","Y[k]","D_art[k]"
+ * (1-based). */
 int cles_solver_get_field(cles_solver *solver, const char *name, double *host);
 
 int cles_solver_get_time(cles_solver *solver, double *t, int64_t *step);

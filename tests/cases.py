@@ -46,8 +46,27 @@ def oracle_solver(n, L, q, **kw):
     return s
 
 
+def nscbc_bc(ch, par):
+    """Frontend NSCBC object from the oracle's parameter dict."""
+    if "pinf" in par:
+        return ch.NSCBCOutflowBC(pinf=par["pinf"], sigma=par.get("sigma", 0.25), Lref=par.get("Lref", 0.0),
+                                 beta_t=par.get("beta_t", -1.0))
+    return ch.NSCBCInflowBC(u=tuple(par["u"]), T_ion=par["T"], Y=tuple(par.get("Y", (1.0,))),
+                            eta_u=par.get("eta_u", 0.28), eta_T=par.get("eta_T", 0.28),
+                            eta_t=par.get("eta_t", 0.28), eta_Y=par.get("eta_Y", 0.28), Lref=par.get("Lref", 0.0))
+
+
+def nscbc_codes(bcs, nscbc):
+    """BC code table with the NSCBC faces filled in (oracle codes)."""
+    out = [list(b) for b in (bcs or ((0, 0),) * 3)]
+    for (k, s), par in (nscbc or {}).items():
+        out[k][s] = O.BC_NSCBC_OUT if "pinf" in par else O.BC_NSCBC_IN
+    return tuple(tuple(b) for b in out)
+
+
 def gpu_solver(ctx, n, L, q, species=((1.0, 1.4),), mu0=0.0, art=True, cfl=0.5, bcs=None, deriv="lele_d1_6",
-               precision="fp64", beta_sensor="strain", bc_funs=None, filter_interval=1, origin=(0.0, 0.0, 0.0)):
+               precision="fp64", beta_sensor="strain", bc_funs=None, filter_interval=1, origin=(0.0, 0.0, 0.0),
+               nscbc=None):
     import compactles_hip as ch
     eos = ch.IdealMixture([ch.IdealSpecies(f"s{k}", R, g) for k, (R, g) in enumerate(species)])
     code = {0: ch.PeriodicBC, 1: ch.SlipWallBC}
@@ -58,6 +77,8 @@ def gpu_solver(ctx, n, L, q, species=((1.0, 1.4),), mu0=0.0, art=True, cfl=0.5, 
         for s in range(2):
             if bc_funs and (k, s) in bc_funs:
                 pair.append(ch.DirichletBC(bc_funs[(k, s)]))
+            elif nscbc and (k, s) in nscbc:
+                pair.append(nscbc_bc(ch, nscbc[(k, s)]))
             else:
                 pair.append(code[bcs[k][s]]())
         bco.append(tuple(pair))
@@ -84,3 +105,43 @@ def grid(n, L, periodic=(True, True, True), origin=(0.0, 0.0, 0.0)):
             h = L[k] / (n[k] if periodic[k] else n[k] - 1)
             xs.append(origin[k] + np.arange(n[k]) * h)
     return np.meshgrid(*xs, indexing="ij")
+
+
+# --------------------------------------------------------------------------
+# Two-dimensional shock / perturbed-interface case with NSCBC faces (the shape
+# of docs/case_studies/shock_tube_2d.jl:37-131): He (light) and CO2 (heavy)
+# ideal gases, R_k = R_u / M_k with R_u = 8.314462618 J/(mol K) and M in
+# g/mol (kJ/(kg K)); gamma 5/3 and 1.289.  A Mach-1.5 shock in He from the
+# normal-shock relations, frame velocity U, inflow at low x holding the shocked
+# state, characteristic outflow at high x relaxing to the undisturbed pressure,
+# periodic y.
+# --------------------------------------------------------------------------
+HE_CO2 = ((8.314462618 / 4.002602, 5.0 / 3.0), (8.314462618 / 44.0095, 1.289))
+
+
+def shock_interface(n, Lx=2.0, Ly=1.0, MS=1.5, U=0.3):
+    (RL, g), (RH, _) = HE_CO2
+    p1, T1 = 1.0, 1.0
+    rho1 = p1 / (RL * T1)
+    c1 = np.sqrt(g * p1 / rho1)
+    p2 = p1 * (2 * g * MS ** 2 - (g - 1)) / (g + 1)
+    rho2 = rho1 * ((g + 1) * MS ** 2) / ((g - 1) * MS ** 2 + 2)
+    u2 = c1 * 2 / (g + 1) * (MS ** 2 - 1) / MS
+    T2 = p2 / (rho2 * RL)
+    L = (Lx, Ly, Ly / n[1])
+    X, Yg, Z = grid(n, L, periodic=(False, True, True))
+    hx = Lx / (n[0] - 1)
+    xs, xi0, amp = 0.15 * Lx, 0.25 * Lx, 0.05 * Ly
+    s = 0.5 * (1 + np.tanh((X - xs) / (2 * hx)))                       # 0 shocked, 1 undisturbed
+    th = 0.5 * (1 + np.tanh((X - (xi0 + amp * np.cos(2 * np.pi * Yg / Ly))) / (3 * hx)))   # 0 He, 1 CO2
+    Ys = [1 - th, th]
+    u = (1 - s) * u2 + U
+    p = (1 - s) * p2 + s * p1
+    T = (1 - s) * T2 + s * T1
+    Rm = Ys[0] * RL + Ys[1] * RH
+    rho = p / (Rm * T)
+    q = conserved((u, 0 * X, 0 * X, p, rho), species=HE_CO2, Y=Ys)
+    nscbc = {(0, 0): {"u": (u2 + U, 0.0, 0.0), "T": T2, "Y": (1.0, 0.0), "eta_u": 0.28, "eta_T": 0.28,
+                      "eta_t": 0.28, "eta_Y": 0.28},
+             (0, 1): {"pinf": p1, "sigma": 0.25, "beta_t": -1.0}}
+    return q, L, nscbc
