@@ -29,9 +29,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path[:0] = [os.path.join(ROOT, "compactles.jl_amd"), os.path.join(ROOT, "oracle"), os.path.join(ROOT, "tests")]
 
 HBM_PEAK_GBPS = 8000.0
-ALG_BYTES_PER_POINT_STEP = 8088.0   # reference-dataflow byte model, TGV art on, FP64
-DOMINANT = ("fluxes",)
-DOMINANT_KERNEL = {"fp64": "cles::k_fluxes<double>", "fp32": "cles::k_fluxes<float>"}
+# the kernel with the largest share of the step's kernel time in each form
+# (rocprofv3 --stats), with the solver phase that launches it
+DOMINANT = {"fp64": "fluxes", "fp32": "line_div_lsrk_z"}
+DOMINANT_KERNEL = {"fp64": "cles::k_fluxes<double>", "fp32": "cles::k_line_yz<float, 1, 2, 0, 3, true>"}
 
 
 def tgv_problem(ch, n, precision):
@@ -76,11 +77,11 @@ def cpu_baseline(n, steps=1):
                       f"{steps} full run! step(s), {dt:.2f} s"}
 
 
-def counters_run(ch, ctx, n, steps):
+def counters_run(ch, ctx, n, steps, precision):
     """Workload of the rocprofv3 --pmc passes: `steps` full run! steps of the TGV
     case, then a 1 GiB streaming copy three times (the counter calibration)."""
     import ctypes as C
-    prob, num = tgv_problem(ch, n, "fp64")
+    prob, num = tgv_problem(ch, n, precision)
     solver, _ = ch.setup(prob, num, ctx)
     solver.run(1e9, steps)
     ctx.synchronize()
@@ -92,8 +93,79 @@ def counters_run(ch, ctx, n, steps):
     for _ in range(3):
         ch._lib.check(ctx.lib.cles_memcpy_d2d(ctx.handle, b.ptr, a.ptr, C.c_size_t(nbytes)))
     ctx.synchronize()
-    print(json.dumps({"kernel": DOMINANT_KERNEL["fp64"], "steps": steps, "copy_bytes_read": nbytes,
+    print(json.dumps({"kernel": DOMINANT_KERNEL[precision], "steps": steps, "copy_bytes_read": nbytes,
                       "copy_bytes_written": nbytes, "copy_launches": 3}), flush=True)
+
+
+def step_passes(S, D):
+    """Per-step passes of the reference-dataflow byte model, split into
+    (FP64-stored, working-precision) passes.  Per RHS: primitives 2S+12
+    (C reads of Q), velocity gradients 6D, artificial properties
+    30+4D+[S>1]S(5+2D), scalar gradients 2D(1+S), fluxes 35+8S (one read of
+    Q), divergence C(3D-1), LSRK update 5C (du and Q read and written); stage 1
+    skips the primitives; max_rate 4S+21 (C reads of Q); filter_state 2CD on
+    Q.  In FP64 every pass is 8 bytes; in the FP32-mixed form the working
+    passes are 4 bytes."""
+    C = S + 4
+    p1 = (C, 8 + S)
+    rhs_rest = (1 + 4 * C, 6 * D + 30 + 4 * D + (S * (5 + 2 * D) if S > 1 else 0) + 2 * D * (1 + S)
+                + 34 + 8 * S + C * (3 * D - 1) + C)
+    rhs = (p1[0] + rhs_rest[0], p1[1] + rhs_rest[1])
+    rate = (C, 4 * S + 21 - C)
+    filt = (2 * C * D, 0)
+    f64 = 4 * rhs[0] + rhs_rest[0] + rate[0] + filt[0]
+    wp = 4 * rhs[1] + rhs_rest[1] + rate[1] + filt[1]
+    return f64, wp
+
+
+def alg_bytes_per_point_step(S, D, precision):
+    f64, wp = step_passes(S, D)
+    return 8.0 * f64 + (8.0 if precision == "fp64" else 4.0) * wp
+
+
+def measure(ch, ctx, n, precision, steps, warmup, profile_steps):
+    prob, num = tgv_problem(ch, n, precision)
+    solver, _ = ch.setup(prob, num, ctx)
+    solver.run(1e9, warmup)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    solver.run(1e9, warmup + steps)
+    ctx.synchronize()
+    wall = time.perf_counter() - t0
+    npts = n ** 3
+    value = npts * steps / wall / 1e6
+    ms_per_step = wall / steps * 1e3
+
+    # live per-phase HIP-event timing on the solver stream (separate pass)
+    solver.profile(True)
+    solver.run(1e9, warmup + steps + profile_steps)
+    phases = solver.profile_read()
+    solver.profile(False)
+    ke, ens = solver.diagnostics()
+    ms, launches, bytes_ = phases.get(DOMINANT[precision], (0.0, 0, 0.0))
+    avg_ms = ms / max(launches, 1)
+    alg_per_launch = bytes_ / max(launches, 1)
+    achieved = alg_per_launch / (avg_ms * 1e-3) / 1e9 if launches else 0.0
+    total_ms = sum(v[0] for v in phases.values())
+    step_alg = alg_bytes_per_point_step(1, 3, precision) * npts
+    solver.close()
+    return {
+        "value": round(value, 3), "ms_per_step": round(ms_per_step, 4), "ke": ke, "enstrophy": ens,
+        "steps_total": warmup + steps + profile_steps,
+        "roofline": {"bound": "hbm", "kernel": DOMINANT_KERNEL[precision], "achieved": round(achieved, 1),
+                     "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4),
+                     "traffic": read_traffic(n, precision, DOMINANT_KERNEL[precision]),
+                     "avg_launch_us": round(avg_ms * 1e3, 2),
+                     "alg_bytes_per_launch": alg_per_launch, "launches_profiled": launches},
+        "step_roofline": {"alg_bytes_per_point_step": alg_bytes_per_point_step(1, 3, precision),
+                          "alg_bytes_per_step": step_alg,
+                          "achieved_GBps": round(step_alg / (ms_per_step * 1e-3) / 1e9, 1),
+                          "frac": round(step_alg / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBPS, 4)},
+        "phases_ms_per_step": {k: round(v[0] / profile_steps, 3) for k, v in
+                               sorted(phases.items(), key=lambda kv: -kv[1][0])},
+        "phases_GBps": {k: round(v[2] / (v[0] * 1e-3) / 1e9, 1) for k, v in phases.items() if v[0] > 0},
+        "profiled_kernel_ms_per_step": round(total_ms / profile_steps, 3),
+    }
 
 
 def main():
@@ -104,6 +176,8 @@ def main():
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-forms", action="store_true",
+                    help="skip the FP32-mixed form that the default FP64 run also reports")
     ap.add_argument("--cpu-n", type=int, default=256)
     ap.add_argument("--profile-steps", type=int, default=2)
     ap.add_argument("--counters", type=int, default=0, metavar="STEPS",
@@ -119,41 +193,17 @@ def main():
     ctx = ch.Context(0)
     n = args.n
     if args.counters:
-        counters_run(ch, ctx, n, args.counters)
+        counters_run(ch, ctx, n, args.counters, args.precision)
         return
-    prob, num = tgv_problem(ch, n, args.precision)
-    solver, _ = ch.setup(prob, num, ctx)
-    solver.run(1e9, args.warmup)
-    ctx.synchronize()
-    t0 = time.perf_counter()
-    solver.run(1e9, args.warmup + args.steps)
-    ctx.synchronize()
-    wall = time.perf_counter() - t0
-    npts = n ** 3
-    value = npts * args.steps / wall / 1e6
-    ms_per_step = wall / args.steps * 1e3
-
-    # live per-phase HIP-event timing on the solver stream (separate pass)
-    solver.profile(True)
-    solver.run(1e9, args.warmup + args.steps + args.profile_steps)
-    phases = solver.profile_read()
-    solver.profile(False)
-    ms = sum(phases[p][0] for p in DOMINANT if p in phases)
-    launches = sum(phases[p][1] for p in DOMINANT if p in phases)
-    bytes_ = sum(phases[p][2] for p in DOMINANT if p in phases)
-    avg_ms = ms / max(launches, 1)
-    alg_per_launch = bytes_ / max(launches, 1)
-    achieved = alg_per_launch / (avg_ms * 1e-3) / 1e9 if launches else 0.0
-    total_ms = sum(v[0] for v in phases.values())
-    step_alg = ALG_BYTES_PER_POINT_STEP * npts
+    primary = measure(ch, ctx, n, args.precision, args.steps, args.warmup, args.profile_steps)
     out = {
         "metric": f"Mpt-steps/s TGV {n}^3 {args.precision.upper()}",
-        "value": round(value, 3),
+        "value": primary["value"],
         "unit": "Mpt-steps/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(ms_per_step, 4),
+        "ms_per_step": primary["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -162,20 +212,21 @@ def main():
         "config": {"workload": f"Taylor-Green vortex {n}^3 periodic, full run! step (max_rate + 5 LSRK(5,4) "
                                "stages + filter_state), C6 derivative, C8 filter, Cook artificial "
                                "properties on, cfl 0.5", "n": n, "precision": args.precision},
-        "roofline": {"bound": "hbm", "kernel": DOMINANT_KERNEL[args.precision], "achieved": round(achieved, 1),
-                     "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4),
-                     "traffic": read_traffic(n, args.precision, DOMINANT_KERNEL[args.precision]),
-                     "avg_launch_us": round(avg_ms * 1e3, 2),
-                     "alg_bytes_per_launch": alg_per_launch, "launches_profiled": launches},
-        "step_roofline": {"alg_bytes_per_step": step_alg,
-                          "achieved_GBps": round(step_alg / (ms_per_step * 1e-3) / 1e9, 1),
-                          "frac": round(step_alg / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBPS, 4)},
-        "phases_ms_per_step": {k: round(v[0] / args.profile_steps, 3) for k, v in
-                               sorted(phases.items(), key=lambda kv: -kv[1][0])},
-        "phases_GBps": {k: round(v[2] / (v[0] * 1e-3) / 1e9, 1) for k, v in phases.items() if v[0] > 0},
-        "profiled_kernel_ms_per_step": round(total_ms / args.profile_steps, 3),
     }
-    solver.close()
+    for k in ("roofline", "step_roofline", "phases_ms_per_step", "phases_GBps", "profiled_kernel_ms_per_step"):
+        out[k] = primary[k]
+    if not args.no_forms and args.precision == "fp64":
+        # the FP32-mixed form of the same workload, and its kinetic-energy
+        # difference from the FP64 run after the same number of steps
+        mixed = measure(ch, ctx, n, "fp32", args.steps, args.warmup, args.profile_steps)
+        out["forms"] = {"fp32_mixed": {
+            "value": mixed["value"], "unit": "Mpt-steps/s", "ms_per_step": mixed["ms_per_step"],
+            "step_roofline": mixed["step_roofline"],
+            "roofline": {k: mixed["roofline"][k] for k in ("kernel", "achieved", "frac", "avg_launch_us")},
+            "ke_relerr_vs_fp64_same_steps": abs(mixed["ke"] - primary["ke"]) / primary["ke"],
+            "enstrophy_relerr_vs_fp64_same_steps": abs(mixed["enstrophy"] - primary["enstrophy"]) /
+            primary["enstrophy"],
+            "steps_compared": mixed["steps_total"]}}
     ctx.close()
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(args.cpu_n)
