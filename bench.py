@@ -97,19 +97,20 @@ def counters_run(ch, ctx, n, steps, precision):
                       "copy_bytes_written": nbytes, "copy_launches": 3}), flush=True)
 
 
-def step_passes(S, D):
+def step_passes(S, D, nscbc_ops=0):
     """Per-step passes of the reference-dataflow byte model, split into
     (FP64-stored, working-precision) passes.  Per RHS: primitives 2S+12
-    (C reads of Q), velocity gradients 6D, artificial properties
-    30+4D+[S>1]S(5+2D), scalar gradients 2D(1+S), fluxes 35+8S (one read of
-    Q), divergence C(3D-1), LSRK update 5C (du and Q read and written); stage 1
-    skips the primitives; max_rate 4S+21 (C reads of Q); filter_state 2CD on
-    Q.  In FP64 every pass is 8 bytes; in the FP32-mixed form the working
-    passes are 4 bytes."""
+    (C reads of Q), velocity gradients 6D plus the reference's zero-fill of
+    the 3(3-D) collapsed ones, artificial properties 30+4D+[S>1]S(5+2D), scalar
+    gradients 2D(1+S), fluxes 35+8S (one read of Q), divergence C(3D-1), LSRK
+    update 5C (du and Q read and written), 2 per NSCBC line op; stage 1 skips
+    the primitives; max_rate 4S+21 (C reads of Q); filter_state 2CD on Q.  In
+    FP64 every pass is 8 bytes; in the FP32-mixed form the working passes are
+    4 bytes."""
     C = S + 4
     p1 = (C, 8 + S)
-    rhs_rest = (1 + 4 * C, 6 * D + 30 + 4 * D + (S * (5 + 2 * D) if S > 1 else 0) + 2 * D * (1 + S)
-                + 34 + 8 * S + C * (3 * D - 1) + C)
+    rhs_rest = (1 + 4 * C, 6 * D + 3 * (3 - D) + 30 + 4 * D + (S * (5 + 2 * D) if S > 1 else 0)
+                + 2 * D * (1 + S) + 34 + 8 * S + C * (3 * D - 1) + C + 2 * nscbc_ops)
     rhs = (p1[0] + rhs_rest[0], p1[1] + rhs_rest[1])
     rate = (C, 4 * S + 21 - C)
     filt = (2 * C * D, 0)
@@ -118,8 +119,8 @@ def step_passes(S, D):
     return f64, wp
 
 
-def alg_bytes_per_point_step(S, D, precision):
-    f64, wp = step_passes(S, D)
+def alg_bytes_per_point_step(S, D, precision, nscbc_ops=0):
+    f64, wp = step_passes(S, D, nscbc_ops)
     return 8.0 * f64 + (8.0 if precision == "fp64" else 4.0) * wp
 
 
@@ -168,6 +169,91 @@ def measure(ch, ctx, n, precision, steps, warmup, profile_steps):
     }
 
 
+def measure_tubes(ch, ctx, count, steps, warmup, precision):
+    """The batched form: `count` independent one-dimensional two-gas shock
+    tubes (N_i ~ U{128..2048}, p_L ~ U[2, 20], rho_L ~ U[0.1, 10] from one
+    seeded generator; He | CO2, Dirichlet ends, art on, cfl 0.4, filter every
+    step) advanced together, each with its own dt."""
+    import cases
+    specs = cases.tube_specs(count, seed=20261202)
+    npts = sum(sp["n"] for sp in specs)
+    t0 = time.perf_counter()
+    b = cases.tube_batch(ctx, specs, precision=precision)
+    setup_s = time.perf_counter() - t0
+    b.run(max_steps=warmup)
+    t0 = time.perf_counter()
+    b.run(max_steps=steps)
+    wall = time.perf_counter() - t0
+    _, st, status = b.state()
+    b.close()
+    ms = wall / steps * 1e3
+    alg = alg_bytes_per_point_step(2, 1, precision) * npts
+    return {"value": round(npts * steps / wall / 1e6, 3), "unit": "Mpt-steps/s", "ms_per_step": round(ms, 4),
+            "instance_steps_per_s": round(count * steps / wall, 1), "instances": count, "points": npts,
+            "all_ok": bool((status == 0).all()), "setup_s": round(setup_s, 2),
+            "step_roofline": {"alg_bytes_per_point_step": alg_bytes_per_point_step(2, 1, precision),
+                              "achieved_GBps": round(alg / (ms * 1e-3) / 1e9, 1),
+                              "frac": round(alg / (ms * 1e-3) / 1e9 / HBM_PEAK_GBPS, 4)},
+            "workload": f"{count} one-dimensional He|CO2 shock tubes, N_i ~ U{{128..2048}} (seed 20261202), "
+                        "Dirichlet ends, art on, cfl 0.4, filter every step, per-instance dt"}
+
+
+def measure_serving(ch, count=48, rate_per_s=16.0, seed=20261202, budget=64 << 30, runners=4):
+    """The serving form: `count` requests with Poisson arrivals (rate
+    `rate_per_s`, seeded) mixing Taylor-Green cubes of 32^3..128^3 to t = 0.05
+    and two-gas shock / interface grids of 256^2..1024x512 to t = 0.03, admitted
+    by device footprint under an HBM budget and run concurrently."""
+    import numpy as np
+
+    import cases
+    rng = np.random.default_rng(seed)
+    arrivals = np.cumsum(rng.exponential(1.0 / rate_per_s, count))
+    kinds = []
+    for _ in range(count):
+        if rng.random() < 0.5:
+            kinds.append(("tgv", int(rng.choice([32, 48, 64, 96, 128]))))
+        else:
+            kinds.append(("shock", [(256, 256, 1), (512, 256, 1), (1024, 512, 1)][int(rng.integers(0, 3))]))
+    reqs = []
+    for kind, n in kinds:
+        if kind == "tgv":
+            prob, num = tgv_problem(ch, n, "fp64")
+            spec = ch.SolverSpec(prob, num)
+            reqs.append((prob, num, ch.initialize_(spec, spec.allocate_state(), prob.ic), 0.05))
+        else:
+            q, L, faces = cases.shock_interface(n)
+            eos = ch.IdealMixture([ch.IdealSpecies(f"s{k}", R, g) for k, (R, g) in enumerate(cases.HE_CO2)])
+            bcs = ((cases.nscbc_bc(ch, faces[(0, 0)]), cases.nscbc_bc(ch, faces[(0, 1)])),
+                   (ch.PeriodicBC(), ch.PeriodicBC()), (ch.PeriodicBC(), ch.PeriodicBC()))
+            prob = ch.Problem(domain=tuple((0.0, L[k]) for k in range(3)), bcs=bcs, ic=lambda X, Y, Z: None,
+                              eos=eos, transport=ch.Transport(mu0=0.0))
+            num = ch.Numerics(n_global=n, cfl=0.4)
+            spec = ch.SolverSpec(prob, num)
+            Q = spec.allocate_state()
+            sl = tuple(slice(spec.halo[k], spec.halo[k] + n[k]) for k in range(3))
+            for c, qc in enumerate(q):
+                Q[sl + (c,)] = qc
+            reqs.append((prob, num, Q, 0.03))
+    sv = ch.Server(device=0, hbm_budget=budget, runners=runners)
+    t0 = time.perf_counter()
+    ids = []
+    for (prob, num, Q, tf), at in zip(reqs, arrivals):
+        delay = at - (time.perf_counter() - t0)
+        if delay > 0:
+            time.sleep(delay)
+        ids.append(sv.submit(prob, num, tfinal=tf, Q=Q))
+    ok = all(sv.wait(rid) == ch.server.DONE for rid in ids)
+    times = [sv.timing(rid) for rid in ids]
+    sv.close()
+    lat = np.array([d - s for s, a, d, fp in times])
+    span = max(d for s, a, d, fp in times) - min(s for s, a, d, fp in times)
+    return {"solves_per_s": round(count / span, 2), "p50_s": round(float(np.percentile(lat, 50)), 4),
+            "p99_s": round(float(np.percentile(lat, 99)), 4), "requests": count, "arrival_rate_per_s": rate_per_s,
+            "all_done": ok, "hbm_budget_GB": budget / 2 ** 30, "runners": runners,
+            "workload": "Poisson arrivals (seed 20261202): Taylor-Green 32^3..128^3 to t = 0.05 and He|CO2 "
+                        "shock / interface 256^2..1024x512 (NSCBC in / out) to t = 0.03, FP64"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -177,7 +263,8 @@ def main():
     ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-forms", action="store_true",
-                    help="skip the FP32-mixed form that the default FP64 run also reports")
+                    help="skip the FP32-mixed and batched forms that the default FP64 run also reports")
+    ap.add_argument("--tubes", type=int, default=4096, help="instances of the batched form")
     ap.add_argument("--cpu-n", type=int, default=256)
     ap.add_argument("--profile-steps", type=int, default=2)
     ap.add_argument("--counters", type=int, default=0, metavar="STEPS",
@@ -227,6 +314,9 @@ def main():
             "enstrophy_relerr_vs_fp64_same_steps": abs(mixed["enstrophy"] - primary["enstrophy"]) /
             primary["enstrophy"],
             "steps_compared": mixed["steps_total"]}}
+        out["forms"]["batched_tubes"] = {p: measure_tubes(ch, ctx, args.tubes, args.steps, args.warmup, p)
+                                         for p in ("fp64", "fp32")}
+        out["forms"]["serving"] = measure_serving(ch)
     ctx.close()
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(args.cpu_n)

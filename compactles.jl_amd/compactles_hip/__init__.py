@@ -7,5 +7,7 @@ from .operators import Decomp, DirPlan, apply_along_, field, filter_field_, plan
 from .schemes import (BandedClosureRow, BandedCompactScheme, ClosureRow, CompactScheme,  # noqa: F401
                       compact_filter, lele_d1_10, lele_d1_6, pade_d1_4)
 from .problem import (ArtParams, DirichletBC, IdealMixture, IdealSpecies, NSCBCInflowBC,  # noqa: F401
-                      NSCBCOutflowBC, Numerics, PeriodicBC, Prim, Problem, SlipWallBC, Solver, StepControl,
+                      NSCBCOutflowBC, Numerics, PeriodicBC, Prim, Problem, SlipWallBC, Solver, SolverSpec, StepControl,
                       Transport, conserved_from_prim, initialize_, run_, setup, single_species, tanh_blend)
+from .batch import Batch, Instance  # noqa: F401
+from .server import Server  # noqa: F401

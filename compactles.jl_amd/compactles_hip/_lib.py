@@ -49,6 +49,11 @@ class SolverDesc(C.Structure):
 
 P = C.c_void_p
 PP = C.POINTER(C.c_void_p)
+class BatchInstance(C.Structure):
+    _fields_ = [("n", C.c_int64), ("L", C.c_double), ("origin", C.c_double), ("tfinal", C.c_double),
+                ("nmax", C.c_int64), ("bc_state", (C.c_double * (MAXSP + 4)) * 2)]
+
+
 I64_3 = C.c_int64 * 3
 INT_3 = C.c_int * 3
 DP = C.POINTER(C.c_double)
@@ -89,6 +94,28 @@ SIGNATURES = {
     "cles_solver_shape": (C.c_int, [P, C.POINTER(C.c_int64), C.POINTER(C.c_int)]),
     "cles_solver_profile": (C.c_int, [P, C.c_int]),
     "cles_solver_profile_read": (C.c_int, [P, C.c_int, C.c_char_p, C.c_int, DP, C.POINTER(C.c_int64), DP]),
+    "cles_solver_footprint": (C.c_int, [C.POINTER(SolverDesc), C.POINTER(C.c_size_t)]),
+    "cles_solver_memory": (C.c_int, [P, C.POINTER(C.c_size_t)]),
+    "cles_server_create": (C.c_int, [C.c_int, C.c_size_t, C.c_int, PP]),
+    "cles_server_destroy": (C.c_int, [P]),
+    "cles_server_submit": (C.c_int, [P, C.POINTER(SolverDesc), DP, C.POINTER(C.c_int64), C.POINTER(C.c_int),
+                                     C.c_double, C.c_int64, C.POINTER(C.c_int64)]),
+    "cles_server_wait": (C.c_int, [P, C.c_int64, C.c_double, C.POINTER(C.c_int)]),
+    "cles_server_result": (C.c_int, [P, C.c_int64, DP, C.POINTER(C.c_int64), C.POINTER(C.c_int),
+                                     C.POINTER(C.c_int), DP, C.POINTER(C.c_int64)]),
+    "cles_server_timing": (C.c_int, [P, C.c_int64, DP, DP, DP, C.POINTER(C.c_size_t)]),
+    "cles_server_release": (C.c_int, [P, C.c_int64]),
+    "cles_server_stats": (C.c_int, [P, C.POINTER(C.c_size_t), C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    "cles_batch_create": (C.c_int, [P, C.POINTER(SolverDesc), C.c_int64, C.POINTER(BatchInstance), PP]),
+    "cles_batch_destroy": (C.c_int, [P]),
+    "cles_batch_shape": (C.c_int, [P, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int)]),
+    "cles_batch_upload": (C.c_int, [P, C.c_int64, DP, C.POINTER(C.c_int64), C.POINTER(C.c_int)]),
+    "cles_batch_download": (C.c_int, [P, C.c_int64, DP, C.POINTER(C.c_int64), C.POINTER(C.c_int)]),
+    "cles_batch_run": (C.c_int, [P, C.c_int64, C.c_int64, DP, C.POINTER(C.c_int64)]),
+    "cles_batch_state": (C.c_int, [P, DP, C.POINTER(C.c_int64), C.POINTER(C.c_int)]),
+    "cles_batch_compute_rhs": (C.c_int, [P, C.c_int]),
+    "cles_batch_get_field": (C.c_int, [P, C.c_int64, C.c_char_p, DP]),
+    "cles_batch_kinetic_energy": (C.c_int, [P, DP]),
 }
 
 

@@ -214,17 +214,45 @@ class Numerics:
 _BETA = {"strain": 0, "gated_strain": 1, "dilatation": 2}
 
 
-class Solver:
-    """Device-resident solver (the reference's Solver + Workspace)."""
+def fill_common_desc(d: L.SolverDesc, prob: Problem, num: Numerics):
+    """The fields of a solver descriptor that do not depend on the grid:
+    EOS, transport, artificial properties, schemes, cfl, filter cadence,
+    step control and precision."""
+    eos = prob.eos
+    d.n_species = eos.nspecies()
+    for k, sp in enumerate(eos.sp):
+        d.R[k] = sp.R
+        d.gamma[k] = sp.gamma
+    tr = prob.transport
+    d.mu0, d.Pr, d.Sc = tr.mu0, tr.Pr, tr.Sc
+    art = num.art
+    d.art_enabled = int(art.enabled)
+    d.C_mu, d.C_beta, d.C_kappa, d.C_D = art.C_mu, art.C_beta, art.C_kappa, art.C_D
+    if art.beta_sensor not in _BETA:
+        raise ValueError(f"art.beta_sensor must be :strain, :gated_strain or :dilatation, got :{art.beta_sensor}")
+    d.beta_sensor = _BETA[art.beta_sensor]
+    d.deriv = num.deriv.to_desc()
+    d.filt = num.filt.to_desc()
+    d.cfl = num.cfl
+    d.filter_interval = num.filter_interval
+    ctl = num.control
+    d.predict, d.max_growth, d.dt_min, d.dt_min_ratio = ctl.predict, ctl.max_growth, ctl.dt_min, ctl.dt_min_ratio
+    if num.precision not in ("fp64", "fp32"):
+        raise ValueError("precision must be 'fp64' or 'fp32'")
+    d.precision = L.F64 if num.precision == "fp64" else L.F32
 
-    def __init__(self, ctx: Context, prob: Problem, num: Numerics):
+
+class SolverSpec:
+    """The grid and the C-ABI descriptor of setup(prob, num), with no device
+    state (what a Solver is created from, and what the server admits)."""
+
+    def __init__(self, prob: Problem, num: Numerics):
         if prob.metric != "cartesian":
             raise ValueError("only CartesianMetric is available on the GPU backend")
         if num.stretch is not None:
             raise ValueError("stretched grids are not available on the GPU backend")
         if prob.sources:
             raise ValueError("explicit sources are not available on the GPU backend")
-        self.ctx = ctx
         self.prob, self.num = prob, num
         self.eos = prob.eos
         self.n = tuple(int(x) for x in num.n_global)
@@ -239,11 +267,7 @@ class Solver:
                                                    else self.L[d] / (self.n[d] - 1)) for d in range(3))
         self.halo = tuple(num.n_halo if self.n[d] > 1 else 0 for d in range(3))
         self.padded = tuple(self.n[d] + 2 * self.halo[d] for d in range(3))
-        desc = self._desc()
-        h = C.c_void_p()
-        L.check(ctx.lib.cles_solver_create(ctx.handle, C.byref(desc), C.byref(h)))
-        self.handle = h
-        self._desc_keep = desc
+        self.desc = self._desc()
 
     def _desc(self) -> L.SolverDesc:
         prob, num = self.prob, self.num
@@ -252,25 +276,7 @@ class Solver:
             d.n_global[k] = self.n[k]
             d.L_domain[k] = self.L[k]
             d.origin[k] = self.origin[k]
-        d.n_species = self.n_species
-        for k, sp in enumerate(self.eos.sp):
-            d.R[k] = sp.R
-            d.gamma[k] = sp.gamma
-        tr = prob.transport
-        d.mu0, d.Pr, d.Sc = tr.mu0, tr.Pr, tr.Sc
-        art = num.art
-        d.art_enabled = int(art.enabled)
-        d.C_mu, d.C_beta, d.C_kappa, d.C_D = art.C_mu, art.C_beta, art.C_kappa, art.C_D
-        if art.beta_sensor not in _BETA:
-            raise ValueError(f"art.beta_sensor must be :strain, :gated_strain or :dilatation, got :{art.beta_sensor}")
-        d.beta_sensor = _BETA[art.beta_sensor]
-        d.deriv = num.deriv.to_desc()
-        d.filt = num.filt.to_desc()
-        d.cfl = num.cfl
-        d.filter_interval = num.filter_interval
-        ctl = num.control
-        d.predict, d.max_growth, d.dt_min, d.dt_min_ratio = ctl.predict, ctl.max_growth, ctl.dt_min, ctl.dt_min_ratio
-        d.precision = L.F64 if num.precision == "fp64" else L.F32
+        fill_common_desc(d, prob, num)
         for k in range(3):
             for s in range(2):
                 bc = prob.bcs[k][s]
@@ -313,11 +319,34 @@ class Solver:
             out.append(float(v.flat[0]))
         return out
 
-    # ---- state transfer -------------------------------------------------------
+    def footprint(self) -> int:
+        """Device bytes the solver would allocate (cles_solver_footprint)."""
+        b = C.c_size_t()
+        L.check(L.load().cles_solver_footprint(C.byref(self.desc), C.byref(b)))
+        return b.value
+
     def allocate_state(self) -> np.ndarray:
         """decomposition.jl:132: a zeroed padded host state Q[..., c] (Fortran order)."""
         return np.zeros(self.padded + (self.n_cons,), order="F")
 
+
+class Solver(SolverSpec):
+    """Device-resident solver (the reference's Solver + Workspace)."""
+
+    def __init__(self, ctx: Context, prob: Problem, num: Numerics):
+        super().__init__(prob, num)
+        self.ctx = ctx
+        h = C.c_void_p()
+        L.check(ctx.lib.cles_solver_create(ctx.handle, C.byref(self.desc), C.byref(h)))
+        self.handle = h
+
+    def memory(self) -> int:
+        """Device bytes held by this solver (cles_solver_memory)."""
+        b = C.c_size_t()
+        L.check(self.ctx.lib.cles_solver_memory(self.handle, C.byref(b)))
+        return b.value
+
+    # ---- state transfer -------------------------------------------------------
     def upload(self, Q: np.ndarray):
         Q = np.asfortranarray(Q, dtype=np.float64)
         L.check(self.ctx.lib.cles_solver_upload(self.handle, Q.ctypes.data_as(L.DP), L.I64_3(*self.padded),

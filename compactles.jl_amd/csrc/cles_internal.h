@@ -78,6 +78,17 @@ struct HostPlan {
     size_t dbytes = 0;
 };
 
+// Host image of one line plan's tables; `dev` holds the shape fields and the
+// pointers are bound to a device copy of `image` by bind_line_tables.
+struct LineTables {
+    LineDev dev{};
+    std::vector<unsigned char> image;
+    size_t o_coef32 = 0, o_row64 = 0, o_row32 = 0, o_pcr = 0;
+};
+void build_line_tables(LineTables &out, const cles_scheme_desc &s, int n, int periodic, double h, int max_chunk,
+                       int dim);
+void bind_line_tables(LineDev &dev, const LineTables &t, unsigned char *base);
+
 // Builds the device tables for one (scheme, n, periodic, h).  Throws Error.
 void build_line_plan(HostPlan &hp, const cles_scheme_desc &s, int n, int periodic,
                      double h, int max_chunk);
@@ -122,6 +133,32 @@ struct EpiArgs {
 // Launch one line operator along `dim` (1..3) over the view.
 void launch_line(const HostPlan &hp, int dim, const View3 &v, const void *f,
                  const EpiArgs &epi, hipStream_t st);
+
+// timestep.jl:5-21 LSRK(5,4) coefficient tables (Carpenter & Kennedy)
+inline constexpr double LSRK_A[5] = {0.0, -567301805773.0 / 1357537059087.0, -2404267990393.0 / 2016746695238.0,
+                       -3550918686646.0 / 2091501179385.0, -1275806237668.0 / 842570457699.0};
+inline constexpr double LSRK_B[5] = {1432997174477.0 / 9575080441755.0, 5161836677717.0 / 13612068292357.0,
+                       1720146321549.0 / 2090206949498.0, 3134564353537.0 / 4481467310338.0,
+                       2277821191437.0 / 14882151754819.0};
+inline constexpr double LSRK_C[5] = {0.0, 1432997174477.0 / 9575080441755.0, 2526269341429.0 / 6820363962896.0,
+                       2006345519317.0 / 3224310063776.0, 2802321613138.0 / 2924317926251.0};
+
+// Ragged batch of independent lines (one line per instance, each with its own
+// length and plan), laid end to end: instance i occupies [off[i], off[i]+n).
+// Work items are waves: item w = (P, start, count) covers the instances
+// order[start .. start+count) whose plans all have P chunks (count <= 64/P).
+struct RagLines {
+    const LineDev *plans;    // per instance
+    const int64_t *off;      // per instance
+    const int *order;        // instances grouped by chunk count
+    const int *items;        // 3 ints per work item
+    int nitems;
+    const double *dt;        // per-instance dt for the LSRK epilogue (NULL: epi.dt)
+    const int *mask;         // per-instance: nonzero applies the epilogue (NULL: all)
+    int q, M, sym;           // scheme shape shared by every instance
+};
+constexpr int RAG_MAX_CHUNK = 32;   // points per thread in the ragged kernel
+void launch_line_rag(const RagLines &r, int dtype, const void *f, const EpiArgs &epi, hipStream_t st);
 
 }  // namespace cles
 

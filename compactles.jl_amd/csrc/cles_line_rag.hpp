@@ -1,0 +1,152 @@
+// Ragged batch of independent compact lines (one line per instance of a
+// batch of one-dimensional problems, each with its own length and plan).
+//
+// One wave per work item.  The item's 64/P lines (all with P chunks of at
+// most RAG_MAX_CHUNK points) are staged in LDS with coalesced loads; lane
+// (line, chunk) forms its chunk's RHS from the tile, runs the chunk sweeps,
+// solves the reduced interface system by block PCR over cross-lane shuffles
+// and applies the spike correction (the same machinery as the x sweep); the
+// results return through the tile for coalesced, per-instance epilogues.
+// Every line is computed from its own data and plan only, so an instance's
+// result does not depend on the other instances of the batch.
+#pragma once
+#include "cles_line.hpp"
+
+namespace cles {
+
+__device__ __forceinline__ int rag_slot(int x) { return x + (x >> 5); }
+
+template <typename T, int Q, int M, int SYM, int EPI>
+__global__ void __launch_bounds__(64) k_line_rag(RagLines r, const T *__restrict__ f, EpiArgs e)
+{
+    constexpr int CL = RAG_MAX_CHUNK;
+    extern __shared__ double smem[];
+    T *tile = (T *)smem;
+    const int lane = threadIdx.x;
+    const int *it = r.items + 3 * blockIdx.x;
+    const int P = it[0], start = it[1], count = it[2];
+    const int TS = 33 * P;   // rag_slot(32 P - 1) < 33 P
+
+    // stage the item's lines
+    for (int lj = 0; lj < count; lj++) {
+        const int inst = r.order[start + lj];
+        const int n = r.plans[inst].n;
+        const T *src = f + r.off[inst];
+        T *row = tile + lj * TS;
+        for (int x0 = 0; x0 < n; x0 += 512) {
+            T v[8];
+#pragma unroll
+            for (int i = 0; i < 8; i++) {
+                const int x = x0 + lane + 64 * i;
+                v[i] = x < n ? src[x] : T(0);
+            }
+#pragma unroll
+            for (int i = 0; i < 8; i++) {
+                const int x = x0 + lane + 64 * i;
+                if (x < n) row[rag_slot(x)] = v[i];
+            }
+        }
+    }
+    __syncthreads();
+
+    // lanes of empty line slots repeat line 0 and store nothing
+    const int li = lane / P, k = lane & (P - 1);
+    const bool live = li < count;
+    const int inst = r.order[start + (live ? li : 0)];
+    const LineDev p = r.plans[inst];
+    const int n = p.n;
+    const int s = k * p.m + min(k, p.extra);
+    const int len = p.m + (k < p.extra ? 1 : 0);
+    T *trow = tile + (live ? li : 0) * TS;
+    T fv[CL + 2 * M];
+#pragma unroll
+    for (int t = -M; t < CL + M; t++) {
+        T val = T(0);
+        if (t < len + M) {
+            int rr = s + t;
+            bool ok = true;
+            if (rr < 0) {
+                if (p.periodic) rr += n; else ok = false;
+            } else if (rr >= n) {
+                if (p.periodic) rr -= n; else ok = false;
+            }
+            if (ok) val = trow[rag_slot(rr)];
+        }
+        fv[t + M] = val;
+    }
+    T fe[8];
+#pragma unroll
+    for (int kap = 0; kap < 8; kap++) fe[kap] = (p.closed && kap < n) ? trow[rag_slot(n - 1 - kap)] : T(0);
+    Chunk<T, Q, M, SYM, CL> ch;
+    ch.rhs(p, k, len, fv, fe);
+    ch.solve(p, s, len);
+    double ends[2 * Q], tp[Q], hn[Q];
+    ch.ends(len, ends);
+    reduced_solve_wave<Q>(p, k, k, li * P, ends, tp, hn);
+#pragma unroll
+    for (int t = 0; t < CL; t++)
+        if (t < len) ch.d[t] = (T)corrected<T, Q>(p, s + t, ch.d[t], tp, hn);
+    __syncthreads();   // every window read is done before the tile is overwritten
+    if (live) {
+#pragma unroll
+        for (int t = 0; t < CL; t++)
+            if (t < len) trow[rag_slot(s + t)] = ch.d[t];
+    }
+    __syncthreads();
+
+    for (int lj = 0; lj < count; lj++) {
+        const int ij = r.order[start + lj];
+        if (r.mask && !r.mask[ij]) continue;
+        const int nj = r.plans[ij].n;
+        const int64_t oj = r.off[ij];
+        EpiArgs ej = e;
+        if (r.dt) ej.dt = r.dt[ij];
+        const T *row = tile + lj * TS;
+        for (int x0 = 0; x0 < nj; x0 += 512) {
+            int64_t o[8];
+            T v[8];
+            bool ok[8];
+#pragma unroll
+            for (int i = 0; i < 8; i++) {
+                const int x = x0 + lane + 64 * i;
+                ok[i] = x < nj;
+                o[i] = oj + (ok[i] ? x : 0);
+                v[i] = ok[i] ? row[rag_slot(x)] : T(0);
+            }
+            epilogue_batch<T, EPI, 8>(ej, o, v, ok);
+        }
+    }
+}
+
+template <typename T, int Q, int M, int SYM, int EPI>
+void launch_rag_variant(const RagLines &r, const T *f, const EpiArgs &e, hipStream_t st)
+{
+    if (r.nitems <= 0) return;
+    const size_t shm = (size_t)64 * 33 * sizeof(T);
+    hipLaunchKernelGGL((k_line_rag<T, Q, M, SYM, EPI>), dim3((unsigned)r.nitems), dim3(64), shm, st, r, f, e);
+    CLES_HIP(hipGetLastError());
+}
+
+template <typename T, int Q, int M, int SYM>
+void launch_rag_epi(const RagLines &r, const T *f, const EpiArgs &e, hipStream_t st)
+{
+    if constexpr (SYM) {
+        switch (e.kind) {
+        case EPI_STORE: return launch_rag_variant<T, Q, M, SYM, EPI_STORE>(r, f, e, st);
+        case EPI_ART_MUBETA: return launch_rag_variant<T, Q, M, SYM, EPI_ART_MUBETA>(r, f, e, st);
+        case EPI_ART_KAPPA: return launch_rag_variant<T, Q, M, SYM, EPI_ART_KAPPA>(r, f, e, st);
+        case EPI_ART_D: return launch_rag_variant<T, Q, M, SYM, EPI_ART_D>(r, f, e, st);
+        default: break;
+        }
+    } else {
+        switch (e.kind) {
+        case EPI_STORE: return launch_rag_variant<T, Q, M, SYM, EPI_STORE>(r, f, e, st);
+        case EPI_STORE_NEG: return launch_rag_variant<T, Q, M, SYM, EPI_STORE_NEG>(r, f, e, st);
+        case EPI_LSRK: return launch_rag_variant<T, Q, M, SYM, EPI_LSRK>(r, f, e, st);
+        default: break;
+        }
+    }
+    throw Error{CLES_ERR_UNSUPPORTED, "epilogue not available for the ragged line kernel"};
+}
+
+}  // namespace cles

@@ -162,8 +162,8 @@ void chunk_solve(const ChunkFactor &F, std::vector<double> &x)
 
 }  // namespace
 
-void build_line_plan(HostPlan &hp, const cles_scheme_desc &s, int n, int periodic, double h,
-                     int max_chunk)
+void build_line_tables(LineTables &out, const cles_scheme_desc &s, int n, int periodic, double h, int max_chunk,
+                       int dim)
 {
     const int q = s.q, M = s.M, nc = s.nclosure;
     if (q < 1 || q > 2) throw Error{CLES_ERR_UNSUPPORTED, "scheme LHS half-bandwidth q must be 1 or 2"};
@@ -177,7 +177,7 @@ void build_line_plan(HostPlan &hp, const cles_scheme_desc &s, int n, int periodi
     if (q > 1) nmin = std::max(nmin, 2 * q + 1);
     if (n < nmin) {
         std::ostringstream os;
-        os << "local extent " << n << " along dim " << hp.dim
+        os << "local extent " << n << " along dim " << dim
            << " too small for scheme (need >= " << nmin << "); use fewer ranks in this dimension";
         throw Error{CLES_ERR_TOO_SMALL, os.str()};
     }
@@ -216,7 +216,7 @@ void build_line_plan(HostPlan &hp, const cles_scheme_desc &s, int n, int periodi
     auto clen = [&](int k) { return m + (k < extra ? 1 : 0); };
 
     const int B = 2 * q;
-    const int nrow = q == 1 ? R1_N : R2_N;
+    const int nrow = q == 1 ? (int)R1_N : (int)R2_N;
     std::vector<double> row((size_t)nrow * n, 0.0);
     std::vector<Mat> Lb(P), Ub(P);
 
@@ -337,40 +337,54 @@ void build_line_plan(HostPlan &hp, const cles_scheme_desc &s, int n, int periodi
             coef[COEF_CHI + j * 8 + kk] = s.closure_rhs[j][kk] * (scale * sgn);
         }
 
-    // upload: coef64 | coef32 | row64 | row32 | pcr
+    // host image: coef64 | coef32 | row64 | row32 | pcr (256-byte aligned parts)
     size_t b_coef64 = COEF_N * sizeof(double), b_coef32 = COEF_N * sizeof(float);
     size_t b_row64 = row.size() * sizeof(double), b_row32 = row.size() * sizeof(float);
     size_t b_pcr = pcr.size() * sizeof(double);
     auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
-    size_t o_coef32 = al(b_coef64), o_row64 = o_coef32 + al(b_coef32), o_row32 = o_row64 + al(b_row64),
-           o_pcr = o_row32 + al(b_row32), total = o_pcr + al(std::max<size_t>(b_pcr, 8));
-    std::vector<unsigned char> host(total, 0);
-    std::memcpy(host.data(), coef.data(), b_coef64);
+    out.o_coef32 = al(b_coef64);
+    out.o_row64 = out.o_coef32 + al(b_coef32);
+    out.o_row32 = out.o_row64 + al(b_row64);
+    out.o_pcr = out.o_row32 + al(b_row32);
+    const size_t total = out.o_pcr + al(std::max<size_t>(b_pcr, 8));
+    out.image.assign(total, 0);
+    std::memcpy(out.image.data(), coef.data(), b_coef64);
     std::vector<float> coef32(coef.begin(), coef.end()), row32(row.begin(), row.end());
-    std::memcpy(host.data() + o_coef32, coef32.data(), b_coef32);
-    std::memcpy(host.data() + o_row64, row.data(), b_row64);
-    std::memcpy(host.data() + o_row32, row32.data(), b_row32);
-    if (b_pcr) std::memcpy(host.data() + o_pcr, pcr.data(), b_pcr);
+    std::memcpy(out.image.data() + out.o_coef32, coef32.data(), b_coef32);
+    std::memcpy(out.image.data() + out.o_row64, row.data(), b_row64);
+    std::memcpy(out.image.data() + out.o_row32, row32.data(), b_row32);
+    if (b_pcr) std::memcpy(out.image.data() + out.o_pcr, pcr.data(), b_pcr);
+    LineDev &L0 = out.dev;
+    L0 = LineDev{};
+    L0.n = n; L0.q = q; L0.M = M; L0.sym = s.symmetric ? 1 : 0; L0.nc = closed ? nc : 0;
+    L0.periodic = periodic ? 1 : 0; L0.closed = closed ? 1 : 0;
+    L0.P = P; L0.nsteps = nsteps; L0.m = m; L0.extra = extra;
+}
 
+void bind_line_tables(LineDev &dev, const LineTables &t, unsigned char *base)
+{
+    dev = t.dev;
+    dev.coef64 = (const double *)base;
+    dev.coef32 = (const float *)(base + t.o_coef32);
+    dev.row64 = (const double *)(base + t.o_row64);
+    dev.row32 = (const float *)(base + t.o_row32);
+    dev.pcr = (const double *)(base + t.o_pcr);
+}
+
+void build_line_plan(HostPlan &hp, const cles_scheme_desc &s, int n, int periodic, double h, int max_chunk)
+{
+    LineTables t;
+    build_line_tables(t, s, n, periodic, h, max_chunk, hp.dim);
     void *d = nullptr;
-    CLES_HIP(hipMalloc(&d, total));
-    hipError_t e = hipMemcpy(d, host.data(), total, hipMemcpyHostToDevice);
+    CLES_HIP(hipMalloc(&d, t.image.size()));
+    hipError_t e = hipMemcpy(d, t.image.data(), t.image.size(), hipMemcpyHostToDevice);
     if (e != hipSuccess) {
         (void)hipFree(d);
         throw Error{CLES_ERR_HIP, std::string("hipMemcpy(plan tables): ") + hipGetErrorString(e)};
     }
     hp.dbuf = d;
-    hp.dbytes = total;
-    unsigned char *base = (unsigned char *)d;
-    LineDev &L0 = hp.dev;
-    L0.n = n; L0.q = q; L0.M = M; L0.sym = s.symmetric ? 1 : 0; L0.nc = closed ? nc : 0;
-    L0.periodic = periodic ? 1 : 0; L0.closed = closed ? 1 : 0;
-    L0.P = P; L0.nsteps = nsteps; L0.m = m; L0.extra = extra;
-    L0.coef64 = (const double *)base;
-    L0.coef32 = (const float *)(base + o_coef32);
-    L0.row64 = (const double *)(base + o_row64);
-    L0.row32 = (const float *)(base + o_row32);
-    L0.pcr = (const double *)(base + o_pcr);
+    hp.dbytes = t.image.size();
+    bind_line_tables(hp.dev, t, (unsigned char *)d);
     hp.h = h;
 }
 

@@ -84,6 +84,40 @@ struct NscbcArgs {
     W *dQ;
 };
 
+// ---------------------------------------------------------------------------
+// Batches of independent one-dimensional instances laid end to end (instance
+// i occupies [off[i], off[i] + n[i]) of every field; N = total points).
+// ---------------------------------------------------------------------------
+struct BatchGeom {
+    int64_t count, N;
+    const int64_t *off;
+    const int *n;
+    const double *h;        // grid spacing per instance
+    int periodic;           // every instance is periodic (else closed ends)
+};
+
+// run! control per instance (timestep.jl:277-400, stepcontrol.jl:184-204)
+struct BatchCtl {
+    int64_t count;
+    const double *rate, *rhomin, *tfinal;
+    double *t, *dt, *dt_prev, *rate_prev, *dt_seen;
+    int64_t *step;
+    const int64_t *nmax;
+    int *status, *active, *stepped, *filt;
+    double cfl, predict, max_growth, dt_min, dt_min_ratio;
+    int filter_interval;
+    double *dts;            // dt history of this run [i * nrec + step - step0 - 1], or NULL
+    const int64_t *step0;   // step count of each instance when the run began
+    int64_t nrec;
+};
+
+struct BatchBC {
+    int lo, hi;             // CLES_BC_* of the two ends
+    int Ns;
+    const double *state;    // Dirichlet conserved states [i][side][Ns + 4]
+    const int *mask;        // apply only where nonzero (NULL: every instance)
+};
+
 template <typename W>
 struct DiagArgs {
     const double *Q;
@@ -110,6 +144,16 @@ template <typename W>
 void launch_rate(const RateArgs<W> &a, double *part, int nb, double *out, int64_t N, hipStream_t st);
 template <typename W>
 void launch_nscbc(const NscbcArgs<W> &a, hipStream_t st);
+template <typename W>
+void launch_rate_batch(const RateArgs<W> &a, const BatchGeom &g, const int *mask, double *rate, double *rhomin,
+                       hipStream_t st);
+template <typename W>
+void launch_delta4_batch(const W *f, W *out, const BatchGeom &g, int wpow, hipStream_t st);
+void launch_batch_dt(const BatchCtl &c, hipStream_t st);
+void launch_batch_post(const BatchCtl &c, hipStream_t st);
+void launch_batch_bcs(const BatchGeom &g, const BatchBC &b, double *Q, hipStream_t st);
+void launch_batch_count(const int *flags, int64_t count, int *out, hipStream_t st);
+void launch_ke_batch(const double *Q, int Ns, const BatchGeom &g, double *ke, hipStream_t st);
 void launch_slipwall(const PlaneArgs &a, double *Q, hipStream_t st);
 void launch_dirichlet(const PlaneArgs &a, double *Q, hipStream_t st);
 template <typename W>

@@ -176,6 +176,83 @@ __global__ void __launch_bounds__(TPB) k_delta4(D4Args<W> a, W *__restrict__ out
     out[I] = acc_out;
 }
 
+// The same sum for MODE 0 with one thread per (x, y) column marching D4_ZB
+// planes in z: the five z values sit in a register window (one new load per
+// plane), the x and y neighbours are direct loads, and no index needs a
+// division.  Blocks of 64 x 4 columns; the block index is renumbered per XCD
+// so that neighbouring y tiles, which share stencil rows, run on one XCD.
+constexpr int D4_ZB = 8;
+
+__device__ __forceinline__ int d4_index(int v, int n, int periodic)
+{
+    if (periodic) return v < 0 ? v + n : (v >= n ? v - n : v);
+    return v < 0 ? 0 : (v > n - 1 ? n - 1 : v);
+}
+
+template <typename W>
+__global__ void __launch_bounds__(TPB) k_delta4_col(D4Args<W> a, W *__restrict__ out, int gx, int gy)
+{
+    const unsigned blk = xcd_block(blockIdx.x, gridDim.x);
+    const int bx = (int)(blk % (unsigned)gx);
+    const unsigned r = blk / (unsigned)gx;
+    const int by = (int)(r % (unsigned)gy), bz = (int)(r / (unsigned)gy);
+    const int nx = (int)a.n[0], ny = (int)a.n[1], nz = (int)a.n[2];
+    const int i = bx * 64 + (int)(threadIdx.x & 63);
+    const int j = by * 4 + (int)(threadIdx.x >> 6);
+    if (i >= nx || j >= ny) return;
+    const int sy = nx, sz = nx * ny;
+    const double D4[5] = {1.0, -4.0, 6.0, -4.0, 1.0};
+    int xo[5], yo[5];
+#pragma unroll
+    for (int m = 0; m < 5; m++) {
+        xo[m] = a.active[0] ? d4_index(i + m - 2, nx, a.periodic[0]) - i : 0;
+        yo[m] = a.active[1] ? (d4_index(j + m - 2, ny, a.periodic[1]) - j) * sy : 0;
+    }
+    const W *col = a.f + i + j * sy;
+    const int k0 = bz * D4_ZB;
+    const bool zact = a.active[2];
+    auto zl = [&](int k) -> double { return (double)col[d4_index(k, nz, a.periodic[2]) * sz]; };
+    double w0 = 0, w1 = 0, w2 = (double)col[k0 * sz], w3 = 0;
+    if (zact) {
+        w0 = zl(k0 - 2);
+        w1 = zl(k0 - 1);
+        w3 = zl(k0 + 1);
+    }
+    for (int kk = 0; kk < D4_ZB; kk++) {
+        const int k = k0 + kk;
+        if (k >= nz) break;
+        const double w4 = zact ? zl(k + 2) : 0.0;
+        const W *base = col + k * sz;
+        W acc_out = 0;
+        if (a.active[0]) {
+            double acc = 0.0;
+#pragma unroll
+            for (int m = 0; m < 5; m++) acc += D4[m] * (m == 2 ? w2 : (double)base[xo[m]]);
+            acc_out += (W)(a.wd[0] * fabs(acc));
+        }
+        if (a.active[1]) {
+            double acc = 0.0;
+#pragma unroll
+            for (int m = 0; m < 5; m++) acc += D4[m] * (m == 2 ? w2 : (double)base[yo[m]]);
+            acc_out += (W)(a.wd[1] * fabs(acc));
+        }
+        if (zact) {
+            double acc = 0.0;
+            acc += D4[0] * w0;
+            acc += D4[1] * w1;
+            acc += D4[2] * w2;
+            acc += D4[3] * w3;
+            acc += D4[4] * w4;
+            acc_out += (W)(a.wd[2] * fabs(acc));
+        }
+        out[(int64_t)k * sz + (int64_t)j * sy + i] = acc_out;
+        w0 = w1;
+        w1 = w2;
+        w2 = zact ? w3 : (k + 1 < nz ? (double)col[(k + 1) * sz] : 0.0);
+        w3 = w4;
+    }
+}
+
 // ---------------------------------------------------------------------------
 // artificial.jl:136-145 compression switch, 171-183 gate_beta!, 216-242 dilatation
 // ---------------------------------------------------------------------------
@@ -299,37 +376,38 @@ __global__ void __launch_bounds__(TPB) k_lsrk(double *__restrict__ Q, double *__
 // timestep.jl:125-175 max_rate: per-block max of the CFL rate and min of the
 // mixture density, then a single-block final reduction (order-independent).
 // ---------------------------------------------------------------------------
+// timestep.jl:146-170: the rate of point I (the mixture density goes to rQ)
 template <typename W>
-__global__ void __launch_bounds__(TPB) k_rate(RateArgs<W> a, double *__restrict__ part, int64_t N)
+__device__ __forceinline__ double rate_at(const RateArgs<W> &a, int64_t I, int64_t N, double &rQ)
 {
-    __shared__ double smax[TPB], smin[TPB];
-    double rate = 0.0, rmin = INFINITY;
-    for (int64_t I = (int64_t)blockIdx.x * TPB + threadIdx.x; I < N; I += (int64_t)gridDim.x * TPB) {
-        double rQ = 0.0;
-        for (int sp = 0; sp < a.Ns; sp++) rQ += a.Q[(int64_t)sp * N + I];
-        rmin = rQ < rmin ? rQ : (rQ != rQ ? rQ : rmin);
-        const double rho = a.rho[I];
-        const double ri = 1 / rho;
-        const double c = a.c[I], cp = a.cp[I];
-        const double uv[3] = {(double)a.u[I], (double)a.v[I], (double)a.w[I]};
-        double acc = 0.0, dsum = 0.0;
+    rQ = 0.0;
+    for (int sp = 0; sp < a.Ns; sp++) rQ += a.Q[(int64_t)sp * N + I];
+    const double rho = a.rho[I];
+    const double ri = 1 / rho;
+    const double c = a.c[I], cp = a.cp[I];
+    const double uv[3] = {(double)a.u[I], (double)a.v[I], (double)a.w[I]};
+    double acc = 0.0, dsum = 0.0;
 #pragma unroll
-        for (int d = 0; d < 3; d++) {
-            if (!a.active[d]) continue;
-            const double idx = a.invh[d];
-            acc += (fabs(uv[d]) + c) * idx;
-            dsum += idx * idx;
-        }
-        double Dmax = a.mu0 / (a.Sc * rho);
-        for (int sp = 0; sp < a.Ns; sp++)
-            if (a.D_art[sp]) Dmax = d_max(Dmax, a.mu0 / (a.Sc * rho) + (double)a.D_art[sp][I]);
-        const double mua = a.mu_art ? (double)a.mu_art[I] : 0.0;
-        const double bea = a.beta_art ? (double)a.beta_art[I] : 0.0;
-        const double kaa = a.kappa_art ? (double)a.kappa_art[I] : 0.0;
-        const double nu = (a.mu0 + mua + bea) * ri + (a.mu0 * cp / a.Pr + kaa) * ri / cp + Dmax;
-        acc += 2 * nu * dsum;
-        rate = acc > rate ? acc : (acc != acc ? acc : rate);
+    for (int d = 0; d < 3; d++) {
+        if (!a.active[d]) continue;
+        const double idx = a.invh[d];
+        acc += (fabs(uv[d]) + c) * idx;
+        dsum += idx * idx;
     }
+    double Dmax = a.mu0 / (a.Sc * rho);
+    for (int sp = 0; sp < a.Ns; sp++)
+        if (a.D_art[sp]) Dmax = d_max(Dmax, a.mu0 / (a.Sc * rho) + (double)a.D_art[sp][I]);
+    const double mua = a.mu_art ? (double)a.mu_art[I] : 0.0;
+    const double bea = a.beta_art ? (double)a.beta_art[I] : 0.0;
+    const double kaa = a.kappa_art ? (double)a.kappa_art[I] : 0.0;
+    const double nu = (a.mu0 + mua + bea) * ri + (a.mu0 * cp / a.Pr + kaa) * ri / cp + Dmax;
+    acc += 2 * nu * dsum;
+    return acc;
+}
+
+// NaN-propagating block max / min of (rate, rmin) over TPB threads
+__device__ __forceinline__ void block_maxmin(double *smax, double *smin, double rate, double rmin)
+{
     smax[threadIdx.x] = rate;
     smin[threadIdx.x] = rmin;
     __syncthreads();
@@ -342,9 +420,48 @@ __global__ void __launch_bounds__(TPB) k_rate(RateArgs<W> a, double *__restrict_
         }
         __syncthreads();
     }
+}
+
+template <typename W>
+__global__ void __launch_bounds__(TPB) k_rate(RateArgs<W> a, double *__restrict__ part, int64_t N)
+{
+    __shared__ double smax[TPB], smin[TPB];
+    double rate = 0.0, rmin = INFINITY;
+    for (int64_t I = (int64_t)blockIdx.x * TPB + threadIdx.x; I < N; I += (int64_t)gridDim.x * TPB) {
+        double rQ;
+        const double acc = rate_at(a, I, N, rQ);
+        rmin = rQ < rmin ? rQ : (rQ != rQ ? rQ : rmin);
+        rate = acc > rate ? acc : (acc != acc ? acc : rate);
+    }
+    block_maxmin(smax, smin, rate, rmin);
     if (threadIdx.x == 0) {
         part[2 * blockIdx.x] = smax[0];
         part[2 * blockIdx.x + 1] = smin[0];
+    }
+}
+
+// one block per instance of a batch (invh of the instance's own grid)
+template <typename W>
+__global__ void __launch_bounds__(TPB) k_rate_batch(RateArgs<W> a, BatchGeom g, const int *__restrict__ mask,
+                                                    double *__restrict__ rate_out, double *__restrict__ rmin_out)
+{
+    __shared__ double smax[TPB], smin[TPB];
+    const int64_t i = blockIdx.x;
+    if (mask && !mask[i]) return;
+    a.invh[0] = 1.0 / g.h[i];
+    const int64_t o = g.off[i];
+    const int n = g.n[i];
+    double rate = 0.0, rmin = INFINITY;
+    for (int x = threadIdx.x; x < n; x += TPB) {
+        double rQ;
+        const double acc = rate_at(a, o + x, g.N, rQ);
+        rmin = rQ < rmin ? rQ : (rQ != rQ ? rQ : rmin);
+        rate = acc > rate ? acc : (acc != acc ? acc : rate);
+    }
+    block_maxmin(smax, smin, rate, rmin);
+    if (threadIdx.x == 0) {
+        rate_out[i] = smax[0];
+        rmin_out[i] = smin[0];
     }
 }
 
@@ -570,6 +687,169 @@ __global__ void __launch_bounds__(TPB) k_sum_final(const double *__restrict__ pa
     if (threadIdx.x == 0) out[0] = sh[0] * scale;
 }
 
+// ---------------------------------------------------------------------------
+// Batches of one-dimensional instances
+// ---------------------------------------------------------------------------
+// artificial.jl:67-94 along x of each instance (one block per instance)
+template <typename W>
+__global__ void __launch_bounds__(TPB) k_delta4_batch(const W *__restrict__ f, W *__restrict__ out, BatchGeom g,
+                                                      int wpow)
+{
+    const int64_t i = blockIdx.x;
+    const int64_t o = g.off[i];
+    const int n = g.n[i];
+    const double hd = g.h[i];
+    const double wd = wpow == 2 ? hd * hd : hd;
+    const double D4[5] = {1.0, -4.0, 6.0, -4.0, 1.0};
+    for (int x = threadIdx.x; x < n; x += TPB) {
+        double acc = 0.0;
+#pragma unroll
+        for (int m = -2; m <= 2; m++) {
+            int j = x + m;
+            if (g.periodic) {
+                if (j < 0) j += n;
+                else if (j >= n) j -= n;
+            } else {
+                j = j < 0 ? 0 : (j > n - 1 ? n - 1 : j);
+            }
+            acc += D4[m + 2] * (double)f[o + j];
+        }
+        W acc_out = 0;
+        acc_out += (W)(wd * fabs(acc));
+        out[o + x] = acc_out;
+    }
+}
+
+__device__ __forceinline__ double j_max(double a, double b) { return (a != a || b != b) ? NAN : (a > b ? a : b); }
+__device__ __forceinline__ double j_min(double a, double b) { return (a != a || b != b) ? NAN : (a < b ? a : b); }
+
+// timestep.jl:29-42 predicted_dt, stepcontrol.jl:184-204 check_step and the
+// tfinal clip of timestep.jl:352, per instance
+__global__ void __launch_bounds__(TPB) k_batch_dt(BatchCtl c)
+{
+    const int64_t i = (int64_t)blockIdx.x * TPB + threadIdx.x;
+    if (i >= c.count) return;
+    if (!c.active[i]) {
+        c.dt[i] = 0.0;
+        c.stepped[i] = 0;
+        return;
+    }
+    const double rate = c.rate[i];
+    double r = rate;
+    if (c.predict > 0 && c.rate_prev[i] > 0) r = j_max(r, r + c.predict * (r - c.rate_prev[i]));
+    double dt = c.cfl / r;
+    if (c.max_growth > 0 && c.dt_prev[i] > 0) dt = j_min(dt, c.max_growth * c.dt_prev[i]);
+    int code = CLES_OK;
+    const double seen = c.dt_seen[i];
+    if (!isfinite(dt)) code = CLES_ERR_NONFINITE;
+    else if (!(c.rhomin[i] > 0)) code = CLES_ERR_NEGATIVE_DENSITY;
+    else if (!(dt > 5.391247e-44)) code = CLES_ERR_PLANCK;
+    else if (!(dt >= c.dt_min)) code = CLES_ERR_DT_MIN;
+    else if (c.dt_min_ratio > 0 && seen > 0 && !(dt >= c.dt_min_ratio * seen)) code = CLES_ERR_DT_COLLAPSE;
+    if (code != CLES_OK) {
+        c.status[i] = code;
+        c.active[i] = 0;
+        c.stepped[i] = 0;
+        c.dt[i] = 0.0;
+        return;
+    }
+    c.dt_seen[i] = j_max(seen, dt);
+    c.dt[i] = j_min(dt, c.tfinal[i] - c.t[i]);
+    c.stepped[i] = 1;
+}
+
+__global__ void __launch_bounds__(TPB) k_batch_post(BatchCtl c)
+{
+    const int64_t i = (int64_t)blockIdx.x * TPB + threadIdx.x;
+    if (i >= c.count) return;
+    if (!c.stepped[i]) {
+        c.filt[i] = 0;
+        return;
+    }
+    const double dt = c.dt[i];
+    c.t[i] += dt;
+    const int64_t st = c.step[i] + 1;
+    c.step[i] = st;
+    c.dt_prev[i] = dt;
+    c.rate_prev[i] = c.rate[i];
+    if (c.dts) {
+        const int64_t k = st - 1 - c.step0[i];
+        if (k >= 0 && k < c.nrec) c.dts[i * c.nrec + k] = dt;
+    }
+    c.filt[i] = (c.filter_interval > 0 && st % c.filter_interval == 0) ? 1 : 0;
+    if (!(c.t[i] < c.tfinal[i]) || st >= c.nmax[i]) c.active[i] = 0;
+}
+
+// boundary.jl:232-243 SlipWallBC and problem.jl:196-207 DirichletBC (constant)
+// at both ends of every instance
+__global__ void __launch_bounds__(TPB) k_batch_bcs(BatchGeom g, BatchBC b, double *__restrict__ Q)
+{
+    const int64_t t = (int64_t)blockIdx.x * TPB + threadIdx.x;
+    if (t >= 2 * g.count) return;
+    const int64_t i = t >> 1;
+    const int side = (int)(t & 1);
+    if (b.mask && !b.mask[i]) return;
+    const int code = side ? b.hi : b.lo;
+    const int64_t I = g.off[i] + (side ? g.n[i] - 1 : 0);
+    const int64_t N = g.N;
+    const int C = b.Ns + 4;
+    if (code == CLES_BC_DIRICHLET) {
+        const double *q = b.state + (i * 2 + side) * C;
+        for (int c = 0; c < C; c++) Q[(int64_t)c * N + I] = q[c];
+    } else if (code == CLES_BC_SLIPWALL) {
+        double rho = 0.0;
+        for (int k = 0; k < b.Ns; k++) rho += Q[(int64_t)k * N + I];
+        const int64_t mc = (int64_t)b.Ns * N + I;
+        const double mn = Q[mc];
+        Q[(int64_t)(b.Ns + 3) * N + I] -= 0.5 * mn * mn / rho;
+        Q[mc] = 0;
+    }
+}
+
+// number of nonzero flags (single block, fixed order)
+__global__ void __launch_bounds__(TPB) k_batch_count(const int *__restrict__ flags, int64_t count,
+                                                     int *__restrict__ out)
+{
+    __shared__ int sh[TPB];
+    int acc = 0;
+    for (int64_t i = threadIdx.x; i < count; i += TPB) acc += flags[i] ? 1 : 0;
+    sh[threadIdx.x] = acc;
+    __syncthreads();
+    for (int s = TPB / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[0] = sh[0];
+}
+
+// diagnostics.jl:41-80 kinetic energy of each instance (one block per
+// instance, fixed summation order)
+__global__ void __launch_bounds__(TPB) k_ke_batch(const double *__restrict__ Q, int Ns, BatchGeom g,
+                                                  double *__restrict__ ke)
+{
+    __shared__ double sh[TPB];
+    const int64_t i = blockIdx.x;
+    const int64_t o = g.off[i], N = g.N;
+    const int n = g.n[i];
+    double acc = 0.0;
+    for (int x = threadIdx.x; x < n; x += TPB) {
+        const int64_t I = o + x;
+        const double wq = (!g.periodic && (x == 0 || x == n - 1)) ? 0.5 : 1.0;
+        double rho = 0.0;
+        for (int sp = 0; sp < Ns; sp++) rho += Q[(int64_t)sp * N + I];
+        const double m1 = Q[(int64_t)Ns * N + I], m2 = Q[(int64_t)(Ns + 1) * N + I],
+                     m3 = Q[(int64_t)(Ns + 2) * N + I];
+        acc += wq * (0.5 * (m1 * m1 + m2 * m2 + m3 * m3) / rho);
+    }
+    sh[threadIdx.x] = acc;
+    __syncthreads();
+    for (int s = TPB / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) ke[i] = sh[0] * g.h[i];
+}
+
 // plain streaming copy in 16-byte words (calibration kernel for byte counters)
 __global__ void __launch_bounds__(TPB) k_copy16(uint4 *__restrict__ d, const uint4 *__restrict__ s, int64_t n16)
 {
@@ -614,9 +894,12 @@ void launch_strain(const GradPtrs<W> &g, W *out, int64_t N, hipStream_t st)
 template <typename W>
 void launch_delta4(const D4Args<W> &a, int mode, W *out, hipStream_t st)
 {
-    if (mode == 0)
-        hipLaunchKernelGGL((k_delta4<W, 0>), dim3(nblocks(a.N)), dim3(TPB), 0, st, a, out);
-    else
+    if (mode == 0) {
+        const int gx = (int)((a.n[0] + 63) / 64), gy = (int)((a.n[1] + 3) / 4);
+        const int gz = (int)((a.n[2] + D4_ZB - 1) / D4_ZB);
+        const int64_t nb = (int64_t)gx * gy * gz;
+        hipLaunchKernelGGL((k_delta4_col<W>), dim3((unsigned)nb), dim3(TPB), 0, st, a, out, gx, gy);
+    } else
         hipLaunchKernelGGL((k_delta4<W, 1>), dim3(nblocks(a.N)), dim3(TPB), 0, st, a, out);
     CHECK_LAUNCH();
 }
@@ -663,6 +946,63 @@ void launch_rate(const RateArgs<W> &a, double *part, int nb, double *out, int64_
     hipLaunchKernelGGL(k_rate<W>, dim3(nb), dim3(TPB), 0, st, a, part, N);
     CHECK_LAUNCH();
     hipLaunchKernelGGL(k_rate_final, dim3(1), dim3(TPB), 0, st, part, nb, out);
+    CHECK_LAUNCH();
+}
+
+template <typename W>
+void launch_rate_batch(const RateArgs<W> &a, const BatchGeom &g, const int *mask, double *rate, double *rhomin,
+                       hipStream_t st)
+{
+    if (g.count <= 0) return;
+    hipLaunchKernelGGL(k_rate_batch<W>, dim3((unsigned)g.count), dim3(TPB), 0, st, a, g, mask, rate, rhomin);
+    CHECK_LAUNCH();
+}
+template void launch_rate_batch<double>(const RateArgs<double> &, const BatchGeom &, const int *, double *,
+                                        double *, hipStream_t);
+template void launch_rate_batch<float>(const RateArgs<float> &, const BatchGeom &, const int *, double *, double *,
+                                       hipStream_t);
+
+template <typename W>
+void launch_delta4_batch(const W *f, W *out, const BatchGeom &g, int wpow, hipStream_t st)
+{
+    if (g.count <= 0) return;
+    hipLaunchKernelGGL(k_delta4_batch<W>, dim3((unsigned)g.count), dim3(TPB), 0, st, f, out, g, wpow);
+    CHECK_LAUNCH();
+}
+template void launch_delta4_batch<double>(const double *, double *, const BatchGeom &, int, hipStream_t);
+template void launch_delta4_batch<float>(const float *, float *, const BatchGeom &, int, hipStream_t);
+
+void launch_batch_dt(const BatchCtl &c, hipStream_t st)
+{
+    if (c.count <= 0) return;
+    hipLaunchKernelGGL(k_batch_dt, dim3(nblocks(c.count)), dim3(TPB), 0, st, c);
+    CHECK_LAUNCH();
+}
+
+void launch_batch_post(const BatchCtl &c, hipStream_t st)
+{
+    if (c.count <= 0) return;
+    hipLaunchKernelGGL(k_batch_post, dim3(nblocks(c.count)), dim3(TPB), 0, st, c);
+    CHECK_LAUNCH();
+}
+
+void launch_batch_bcs(const BatchGeom &g, const BatchBC &b, double *Q, hipStream_t st)
+{
+    if (g.count <= 0) return;
+    hipLaunchKernelGGL(k_batch_bcs, dim3(nblocks(2 * g.count)), dim3(TPB), 0, st, g, b, Q);
+    CHECK_LAUNCH();
+}
+
+void launch_batch_count(const int *flags, int64_t count, int *out, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_batch_count, dim3(1), dim3(TPB), 0, st, flags, count, out);
+    CHECK_LAUNCH();
+}
+
+void launch_ke_batch(const double *Q, int Ns, const BatchGeom &g, double *ke, hipStream_t st)
+{
+    if (g.count <= 0) return;
+    hipLaunchKernelGGL(k_ke_batch, dim3((unsigned)g.count), dim3(TPB), 0, st, Q, Ns, g, ke);
     CHECK_LAUNCH();
 }
 

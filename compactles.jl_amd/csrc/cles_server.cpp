@@ -1,0 +1,302 @@
+// Serving loop: independent solve requests (a solver descriptor, an initial
+// state, tfinal and nmax, as setup + run! of the reference, src/problem.jl:321
+// and src/timestep.jl:277) admitted into HBM by their device footprint and run
+// concurrently, each on its own context and stream.
+//
+// Admission: a request is admitted when its footprint (cles_solver_footprint)
+// fits in what the budget has left and a runner is free.  Requests are
+// considered in arrival order; one that does not fit is passed over for
+// smaller ones (continuous batching into free capacity) until it has been
+// passed over kAging times, after which nothing behind it is admitted until
+// it runs (no starvation).
+//
+// Each admitted request owns its device buffers and stream, and the kernels
+// are deterministic, so its result is bitwise the same alone and in any mix.
+#include <atomic>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "cles_internal.h"
+
+namespace {
+
+constexpr int kAging = 4;
+
+enum { RQ_QUEUED = 0, RQ_RUNNING = 1, RQ_DONE = 2, RQ_FAILED = 3 };
+
+struct Request {
+    int64_t id = 0;
+    cles_solver_desc desc{};
+    std::vector<double> Q;   // padded host state: input, then result
+    int64_t padded[3] = {1, 1, 1};
+    int halo[3] = {0, 0, 0};
+    double tfinal = 0;
+    int64_t nmax = 0;
+    size_t footprint = 0;
+    int state = RQ_QUEUED;
+    int status = CLES_OK;
+    std::string msg;
+    int passed_over = 0;
+    double t_submit = 0, t_admit = 0, t_done = 0;
+    double t_sim = 0;
+    int64_t steps = 0;
+};
+
+double now_s(std::chrono::steady_clock::time_point t0)
+{
+    return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+}  // namespace
+
+struct cles_server {
+    int device = 0;
+    size_t budget = 0, in_use = 0;
+    int runners = 1;
+    std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
+    std::mutex mu;
+    std::condition_variable cv;
+    std::deque<int64_t> queue;
+    std::map<int64_t, std::unique_ptr<Request>> reqs;
+    std::vector<std::thread> workers;
+    bool stop = false;
+    int64_t next_id = 1;
+    int running = 0;
+
+    // under mu: the next admissible request (taken off the queue), or nullptr
+    Request *admit_locked()
+    {
+        if (running >= runners) return nullptr;
+        auto it = queue.begin();
+        for (; it != queue.end(); ++it) {
+            const Request *r = reqs[*it].get();
+            if (r->footprint <= budget - in_use) break;
+            if (r->passed_over >= kAging) return nullptr;   // capacity is held for it
+        }
+        if (it == queue.end()) return nullptr;
+        for (auto jt = queue.begin(); jt != it; ++jt) reqs[*jt]->passed_over++;
+        Request *r = reqs[*it].get();
+        queue.erase(it);
+        return r;
+    }
+
+    void run_one(Request *r)
+    {
+        cles_ctx *ctx = nullptr;
+        cles_solver *s = nullptr;
+        int rc = cles_ctx_create(device, &ctx);
+        if (rc == CLES_OK) rc = cles_solver_create(ctx, &r->desc, &s);
+        if (rc == CLES_OK) rc = cles_solver_upload(s, r->Q.data(), r->padded, r->halo);
+        int run_rc = CLES_OK;
+        std::string run_msg;
+        if (rc == CLES_OK) {
+            run_rc = cles_run(s, r->tfinal, r->nmax, nullptr, 0);
+            if (run_rc != CLES_OK) run_msg = cles_last_error();
+            // the state reached (at a failure: the last completed step)
+            rc = cles_solver_download(s, r->Q.data(), r->padded, r->halo);
+            if (rc == CLES_OK) rc = cles_solver_get_time(s, &r->t_sim, &r->steps);
+        }
+        const std::string msg = rc != CLES_OK ? std::string(cles_last_error()) : run_msg;
+        if (s) cles_solver_destroy(s);
+        if (ctx) cles_ctx_destroy(ctx);
+        std::lock_guard<std::mutex> lk(mu);
+        r->status = rc != CLES_OK ? rc : run_rc;
+        r->msg = msg;
+        r->state = r->status == CLES_OK ? RQ_DONE : RQ_FAILED;
+        r->t_done = now_s(t0);
+    }
+
+    void worker()
+    {
+        std::unique_lock<std::mutex> lk(mu);
+        for (;;) {
+            Request *r = nullptr;
+            cv.wait(lk, [&] { return stop || (r = admit_locked()) != nullptr; });
+            if (!r) return;   // stopping
+            r->state = RQ_RUNNING;
+            r->t_admit = now_s(t0);
+            in_use += r->footprint;
+            running++;
+            lk.unlock();
+            run_one(r);
+            lk.lock();
+            in_use -= r->footprint;
+            running--;
+            cv.notify_all();
+        }
+    }
+};
+
+extern "C" {
+
+int cles_server_create(int device, size_t hbm_budget, int runners, cles_server **out)
+{
+    CLES_TRY({
+        CLES_REQUIRE(out, "NULL argument");
+        CLES_REQUIRE(runners >= 1 && runners <= 64, "runners must be 1..64");
+        CLES_REQUIRE(hbm_budget > 0, "the HBM budget must be positive");
+        auto *sv = new cles_server;
+        sv->device = device;
+        sv->budget = hbm_budget;
+        sv->runners = runners;
+        try {
+            for (int k = 0; k < runners; k++) sv->workers.emplace_back([sv] { sv->worker(); });
+        } catch (...) {
+            {
+                std::lock_guard<std::mutex> lk(sv->mu);
+                sv->stop = true;
+            }
+            sv->cv.notify_all();
+            for (auto &t : sv->workers) t.join();
+            delete sv;
+            throw cles::Error{CLES_ERR_NOMEM, "could not start the runner threads"};
+        }
+        *out = sv;
+    })
+}
+
+int cles_server_destroy(cles_server *sv)
+{
+    CLES_TRY({
+        if (!sv) return CLES_OK;
+        {
+            std::lock_guard<std::mutex> lk(sv->mu);
+            sv->stop = true;
+            sv->queue.clear();   // queued requests are dropped; running ones finish
+        }
+        sv->cv.notify_all();
+        for (auto &t : sv->workers) t.join();
+        delete sv;
+    })
+}
+
+int cles_server_submit(cles_server *sv, const cles_solver_desc *desc, const double *Q_host, const int64_t padded[3],
+                       const int halo[3], double tfinal, int64_t nmax, int64_t *id)
+{
+    CLES_TRY({
+        CLES_REQUIRE(sv && desc && Q_host && padded && halo && id, "NULL argument");
+        size_t fp = 0;
+        const int rc = cles_solver_footprint(desc, &fp);
+        if (rc != CLES_OK) throw cles::Error{rc, cles_last_error()};
+        if (fp > sv->budget)
+            throw cles::Error{CLES_ERR_NOMEM, "request needs " + std::to_string(fp) + " device bytes; the budget is " +
+                                                  std::to_string(sv->budget)};
+        int64_t np = 1;
+        for (int k = 0; k < 3; k++) {
+            CLES_REQUIRE(halo[k] >= 0 && padded[k] >= desc->n_global[k] + 2 * (int64_t)halo[k],
+                         "padded extents smaller than n + 2*halo");
+            np *= padded[k];
+        }
+        auto r = std::make_unique<Request>();
+        r->desc = *desc;
+        const size_t ncons = (size_t)desc->n_species + 4;
+        r->Q.assign(Q_host, Q_host + (size_t)np * ncons);
+        for (int k = 0; k < 3; k++) {
+            r->padded[k] = padded[k];
+            r->halo[k] = halo[k];
+        }
+        r->tfinal = tfinal;
+        r->nmax = nmax;
+        r->footprint = fp;
+        {
+            std::lock_guard<std::mutex> lk(sv->mu);
+            CLES_REQUIRE(!sv->stop, "the server is stopping");
+            r->id = sv->next_id++;
+            r->t_submit = now_s(sv->t0);
+            *id = r->id;
+            sv->queue.push_back(r->id);
+            sv->reqs[r->id] = std::move(r);
+        }
+        sv->cv.notify_all();
+    })
+}
+
+int cles_server_wait(cles_server *sv, int64_t id, double timeout_s, int *state)
+{
+    CLES_TRY({
+        CLES_REQUIRE(sv && state, "NULL argument");
+        std::unique_lock<std::mutex> lk(sv->mu);
+        auto it = sv->reqs.find(id);
+        CLES_REQUIRE(it != sv->reqs.end(), "unknown request id");
+        Request *r = it->second.get();
+        auto done = [&] { return r->state == RQ_DONE || r->state == RQ_FAILED; };
+        if (timeout_s < 0)
+            sv->cv.wait(lk, done);
+        else
+            sv->cv.wait_for(lk, std::chrono::duration<double>(timeout_s), done);
+        *state = r->state;
+    })
+}
+
+int cles_server_result(cles_server *sv, int64_t id, double *Q_host, const int64_t padded[3], const int halo[3],
+                       int *status, double *t, int64_t *steps)
+{
+    CLES_TRY({
+        CLES_REQUIRE(sv && status, "NULL argument");
+        std::lock_guard<std::mutex> lk(sv->mu);
+        auto it = sv->reqs.find(id);
+        CLES_REQUIRE(it != sv->reqs.end(), "unknown request id");
+        Request *r = it->second.get();
+        CLES_REQUIRE(r->state == RQ_DONE || r->state == RQ_FAILED, "request has not finished");
+        *status = r->status;
+        if (t) *t = r->t_sim;
+        if (steps) *steps = r->steps;
+        if (Q_host) {
+            CLES_REQUIRE(padded && halo, "NULL argument");
+            for (int k = 0; k < 3; k++)
+                CLES_REQUIRE(padded[k] == r->padded[k] && halo[k] == r->halo[k],
+                             "result layout differs from the submitted one");
+            std::memcpy(Q_host, r->Q.data(), r->Q.size() * sizeof(double));
+        }
+        if (r->status != CLES_OK) cles::set_error(r->msg);
+    })
+}
+
+int cles_server_timing(cles_server *sv, int64_t id, double *t_submit, double *t_admit, double *t_done,
+                       size_t *footprint)
+{
+    CLES_TRY({
+        CLES_REQUIRE(sv, "NULL argument");
+        std::lock_guard<std::mutex> lk(sv->mu);
+        auto it = sv->reqs.find(id);
+        CLES_REQUIRE(it != sv->reqs.end(), "unknown request id");
+        Request *r = it->second.get();
+        if (t_submit) *t_submit = r->t_submit;
+        if (t_admit) *t_admit = r->t_admit;
+        if (t_done) *t_done = r->t_done;
+        if (footprint) *footprint = r->footprint;
+    })
+}
+
+int cles_server_release(cles_server *sv, int64_t id)
+{
+    CLES_TRY({
+        CLES_REQUIRE(sv, "NULL argument");
+        std::lock_guard<std::mutex> lk(sv->mu);
+        auto it = sv->reqs.find(id);
+        CLES_REQUIRE(it != sv->reqs.end(), "unknown request id");
+        CLES_REQUIRE(it->second->state == RQ_DONE || it->second->state == RQ_FAILED, "request has not finished");
+        sv->reqs.erase(it);
+    })
+}
+
+int cles_server_stats(cles_server *sv, size_t *in_use, int *running, int *queued)
+{
+    CLES_TRY({
+        CLES_REQUIRE(sv, "NULL argument");
+        std::lock_guard<std::mutex> lk(sv->mu);
+        if (in_use) *in_use = sv->in_use;
+        if (running) *running = sv->running;
+        if (queued) *queued = (int)sv->queue.size();
+    })
+}
+
+}  // extern "C"

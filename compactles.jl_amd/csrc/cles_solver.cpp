@@ -21,14 +21,7 @@
 
 namespace {
 
-// timestep.jl:5-21
-const double RKA[5] = {0.0, -567301805773.0 / 1357537059087.0, -2404267990393.0 / 2016746695238.0,
-                       -3550918686646.0 / 2091501179385.0, -1275806237668.0 / 842570457699.0};
-const double RKB[5] = {1432997174477.0 / 9575080441755.0, 5161836677717.0 / 13612068292357.0,
-                       1720146321549.0 / 2090206949498.0, 3134564353537.0 / 4481467310338.0,
-                       2277821191437.0 / 14882151754819.0};
-const double RKC[5] = {0.0, 1432997174477.0 / 9575080441755.0, 2526269341429.0 / 6820363962896.0,
-                       2006345519317.0 / 3224310063776.0, 2802321613138.0 / 2924317926251.0};
+const double *const RKA = cles::LSRK_A, *const RKB = cles::LSRK_B, *const RKC = cles::LSRK_C;
 const double PLANCK_TIME = 5.391247e-44;   // stepcontrol.jl:63
 
 inline double jmax(double a, double b) { return (a != a || b != b) ? NAN : (a > b ? a : b); }
@@ -49,6 +42,7 @@ struct cles_solver {
     cles::HostPlan dplan[3], fplan[3], fplanQ[3];
     bool own_fplanQ = false;
     std::vector<void *> allocs;
+    size_t bytes = 0;                          // device bytes allocated by this solver
     // device state (W-typed pointers are void* here; the Impl casts them)
     double *Q = nullptr, *du = nullptr;
     void *dQ = nullptr;
@@ -130,6 +124,7 @@ struct cles_solver {
         hipError_t e = hipMalloc(&ptr, bytes ? bytes : 8);
         if (e != hipSuccess) throw cles::Error{CLES_ERR_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e)};
         allocs.push_back(ptr);
+        this->bytes += bytes ? bytes : 8;
         CLES_HIP(hipMemsetAsync(ptr, 0, bytes ? bytes : 8, st()));
         return ptr;
     }
@@ -647,6 +642,52 @@ void validate_desc(const cles_solver_desc &d)
 }  // namespace
 
 extern "C" {
+
+int cles_solver_footprint(const cles_solver_desc *desc, size_t *bytes)
+{
+    CLES_TRY({
+        CLES_REQUIRE(desc && bytes, "NULL argument");
+        validate_desc(*desc);
+        const cles_solver_desc &d = *desc;
+        const int Ns = d.n_species, C = Ns + 4;
+        const double w = d.precision == CLES_F64 ? 8.0 : 4.0;
+        double N = 1.0, plans = 0.0;
+        int nact = 0;
+        bool nscbc = false;
+        for (int k = 0; k < 3; k++) {
+            N *= (double)d.n_global[k];
+            if (d.n_global[k] > 1) {
+                nact++;
+                // three line plans at most per dimension: row tables (<= 12 values of
+                // 8 + 4 bytes per node), PCR blocks and 256-byte-aligned parts
+                plans += 3.0 * ((double)d.n_global[k] * 12.0 * 12.0 + 64.0 * 1024.0);
+                for (int sd = 0; sd < 2; sd++)
+                    nscbc = nscbc || d.bc[k][sd] == CLES_BC_NSCBC_IN || d.bc[k][sd] == CLES_BC_NSCBC_OUT;
+            }
+        }
+        // the allocation table of cles_solver_create
+        double fields = 8 + Ns;                                   // primitives, Y
+        fields += nact * (3 + 1 + (Ns > 1 ? Ns : 0) + C);         // grad_u, grad_T, grad_Y, fluxes
+        fields += 3 + (Ns > 1 ? Ns : 0);                          // mu*, beta*, kappa*, D*
+        if (d.art_enabled) fields += 3 + (d.beta_sensor == 2 ? 2 : 0);
+        if (nscbc) fields += 3;
+        const double total = N * C * 16.0 + N * C * w + N * w * fields + 1024.0 * 2 * 8 + 16 * 8 + plans;
+        *bytes = (size_t)total;
+    })
+}
+
+int cles_solver_memory(cles_solver *s, size_t *bytes)
+{
+    CLES_TRY({
+        CLES_REQUIRE(s && bytes, "NULL argument");
+        size_t b = s->bytes;
+        for (int k = 0; k < 3; k++) {
+            b += s->dplan[k].dbytes + s->fplan[k].dbytes;
+            if (s->own_fplanQ) b += s->fplanQ[k].dbytes;
+        }
+        *bytes = b;
+    })
+}
 
 int cles_solver_create(cles_ctx *ctx, const cles_solver_desc *desc, cles_solver **out)
 {

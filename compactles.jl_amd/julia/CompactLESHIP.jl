@@ -127,4 +127,80 @@ function CompactLES.run!(s::HIPSolver, Q; tfinal, nmax::Int=typemax(Int))
     return Q
 end
 
+# ---------------------------------------------------------------------------
+# Batches of one-dimensional instances and the serving loop (no counterpart in
+# the reference, which runs one problem per process): thin ccall wrappers.
+# `desc` is the cles_solver_desc built from (Problem, Numerics) exactly as for
+# HIPSolver; instance i is (n, L, origin, tfinal, nmax, Dirichlet states).
+# ---------------------------------------------------------------------------
+struct ClesBatchInstance
+    n::Int64
+    L::Cdouble
+    origin::Cdouble
+    tfinal::Cdouble
+    nmax::Int64
+    bc_state::NTuple{24,Cdouble}        # [2][CLES_MAXSP + 4], row-major
+end
+
+mutable struct HIPBatch
+    ptr::Ptr{Cvoid}
+    ctx::Context
+end
+
+function HIPBatch(ctx::Context, desc::Ref, instances::Vector{ClesBatchInstance})
+    r = Ref{Ptr{Cvoid}}(C_NULL)
+    check(ccall((:cles_batch_create, LIB), Cint,
+                (Ptr{Cvoid}, Ptr{Cvoid}, Int64, Ptr{ClesBatchInstance}, Ptr{Ptr{Cvoid}}),
+                ctx.ptr, desc, length(instances), instances, r))
+    b = HIPBatch(r[], ctx)
+    finalizer(b -> ccall((:cles_batch_destroy, LIB), Cint, (Ptr{Cvoid},), b.ptr), b)
+    return b
+end
+
+"Advance every instance (each to its own tfinal / nmax); returns (t, steps, status) per instance."
+function run_batch!(b::HIPBatch; max_steps::Int=typemax(Int))
+    taken = Ref{Int64}(0)
+    check(ccall((:cles_batch_run, LIB), Cint, (Ptr{Cvoid}, Int64, Int64, Ptr{Cdouble}, Ptr{Int64}),
+                b.ptr, max_steps, 0, C_NULL, taken))
+    cnt = Ref{Int64}(0); npts = Ref{Int64}(0); nc = Ref{Cint}(0)
+    check(ccall((:cles_batch_shape, LIB), Cint, (Ptr{Cvoid}, Ptr{Int64}, Ptr{Int64}, Ptr{Cint}), b.ptr, cnt, npts, nc))
+    t = zeros(Cdouble, cnt[]); st = zeros(Int64, cnt[]); status = zeros(Cint, cnt[])
+    check(ccall((:cles_batch_state, LIB), Cint, (Ptr{Cvoid}, Ptr{Cdouble}, Ptr{Int64}, Ptr{Cint}),
+                b.ptr, t, st, status))
+    return t, st, status
+end
+
+mutable struct HIPServer
+    ptr::Ptr{Cvoid}
+end
+
+function HIPServer(; device::Integer=0, hbm_budget::Integer=64 * 2^30, runners::Integer=4)
+    r = Ref{Ptr{Cvoid}}(C_NULL)
+    check(ccall((:cles_server_create, LIB), Cint, (Cint, Csize_t, Cint, Ptr{Ptr{Cvoid}}),
+                device, hbm_budget, runners, r))
+    s = HIPServer(r[])
+    finalizer(s -> ccall((:cles_server_destroy, LIB), Cint, (Ptr{Cvoid},), s.ptr), s)
+    return s
+end
+
+"Queue setup + run!(...; tfinal, nmax) of the padded host state Q; returns the request id."
+function submit!(s::HIPServer, desc::Ref, Q::Array{Float64}, halo::NTuple{3,Int}; tfinal, nmax::Int=typemax(Int))
+    id = Ref{Int64}(0)
+    check(ccall((:cles_server_submit, LIB), Cint,
+                (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cdouble}, Ptr{Int64}, Ptr{Cint}, Cdouble, Int64, Ptr{Int64}),
+                s.ptr, desc, Q, Int64.(collect(size(Q)[1:3])), Cint.(collect(halo)), tfinal, nmax, id))
+    return id[]
+end
+
+"Wait for request `id` and copy its final state into Q; returns (status, t, steps)."
+function fetch!(s::HIPServer, id::Integer, Q::Array{Float64}, halo::NTuple{3,Int})
+    st = Ref{Cint}(0)
+    check(ccall((:cles_server_wait, LIB), Cint, (Ptr{Cvoid}, Int64, Cdouble, Ptr{Cint}), s.ptr, id, -1.0, st))
+    status = Ref{Cint}(0); t = Ref{Cdouble}(0); steps = Ref{Int64}(0)
+    check(ccall((:cles_server_result, LIB), Cint,
+                (Ptr{Cvoid}, Int64, Ptr{Cdouble}, Ptr{Int64}, Ptr{Cint}, Ptr{Cint}, Ptr{Cdouble}, Ptr{Int64}),
+                s.ptr, id, Q, Int64.(collect(size(Q)[1:3])), Cint.(collect(halo)), status, t, steps))
+    return status[], t[], steps[]
+end
+
 end # module

@@ -174,6 +174,10 @@ typedef struct {
     int precision;                         /* CLES_F64, or CLES_F32 = mixed form */
 } cles_solver_desc;
 
+/* Device bytes cles_solver_create would allocate for `desc` (an upper bound:
+ * every field of the allocation table plus an allowance for the line-plan
+ * tables); no device is touched.  The admission measure of cles_server. */
+int cles_solver_footprint(const cles_solver_desc *desc, size_t *bytes);
 /* setup(prob, num) -> Solver (src/problem.jl:321-345, src/rhs.jl:122-311),
  * one rank.  Allocates the device-resident state and every scratch field. */
 int cles_solver_create(cles_ctx *ctx, const cles_solver_desc *desc, cles_solver **solver);
@@ -228,6 +232,97 @@ int cles_solver_profile_read(cles_solver *solver, int index, char *name, int nam
 /* Device-resident state geometry: interior extents and per-field element
  * count (the solver stores unpadded column-major fields). */
 int cles_solver_shape(cles_solver *solver, int64_t n[3], int *n_cons);
+/* Device bytes held by a live solver (fields, reduction buffers, plans). */
+int cles_solver_memory(cles_solver *solver, size_t *bytes);
+
+/* ----------------------------------------------------------------------- */
+/* Batches of independent one-dimensional instances                        */
+/* ----------------------------------------------------------------------- */
+/* The reference runs one problem per setup/run! (src/problem.jl:321,
+ * src/timestep.jl:277); a batch holds many one-dimensional problems
+ * (n_global = (n, 1, 1)) that share the physics and numerics of one
+ * cles_solver_desc and differ in length, domain, end time, step cap and
+ * boundary states.  Each instance follows run! exactly (its own max_rate, dt,
+ * check_step floors, tfinal clip and filter cadence); a failing instance is
+ * frozen with its status while the others go on.  An instance's result does
+ * not depend on the other instances of the batch.  Limits: 2 <= n <= 2048,
+ * periodic, slip-wall or Dirichlet (constant state) ends. */
+typedef struct cles_batch cles_batch;
+
+typedef struct {
+    int64_t n;                             /* points of this instance */
+    double L, origin;                      /* domain [origin, origin + L] */
+    double tfinal;                         /* run!(...; tfinal) */
+    int64_t nmax;                          /* run!(...; nmax) */
+    double bc_state[2][CLES_MAXSP + 4];    /* Dirichlet conserved state, low / high end */
+} cles_batch_instance;
+
+/* `common`: physics, schemes, art, cfl, filter cadence, step control, x-end
+ * condition codes and precision; its n_global, L_domain, origin and bc_state
+ * are ignored (they come per instance). */
+int cles_batch_create(cles_ctx *ctx, const cles_solver_desc *common, int64_t count,
+                      const cles_batch_instance *instances, cles_batch **batch);
+int cles_batch_destroy(cles_batch *batch);
+int cles_batch_shape(cles_batch *batch, int64_t *count, int64_t *total_points, int *n_cons);
+/* Instance i's state in the reference's padded layout (padded = (n + 2H,
+ * 1, 1), halo = (H, 0, 0), n_cons components), as cles_solver_upload. */
+int cles_batch_upload(cles_batch *batch, int64_t i, const double *Q_host,
+                      const int64_t padded[3], const int halo[3]);
+int cles_batch_download(cles_batch *batch, int64_t i, double *Q_host,
+                        const int64_t padded[3], const int halo[3]);
+/* run! on every instance: at most max_steps steps of each; instances stop at
+ * their own tfinal / nmax / failure.  dts (count x nrec, row i = instance i)
+ * receives each instance's dt sequence of this call when nrec > 0.
+ * *steps_taken: batch iterations issued. */
+int cles_batch_run(cles_batch *batch, int64_t max_steps, int64_t nrec, double *dts, int64_t *steps_taken);
+/* Per-instance time, completed steps and status (CLES_OK or the check_step
+ * code 4..8 it stopped with); NULL arrays are skipped. */
+int cles_batch_state(cles_batch *batch, double *t, int64_t *steps, int *status);
+/* compute_rhs! of every instance into dQ (src/rhs.jl:624). */
+int cles_batch_compute_rhs(cles_batch *batch, int primitives_current);
+/* Instance i's interior of a named field: "rho","u","p","T","c","mu_art",
+ * "beta_art","kappa_art","Q[c]","dQ[c]","Y[k]","D_art[k]". */
+int cles_batch_get_field(cles_batch *batch, int64_t i, const char *name, double *host);
+/* Kinetic energy of every instance (src/diagnostics.jl:61-80 with the
+ * closed-edge half weights). */
+int cles_batch_kinetic_energy(cles_batch *batch, double *ke);
+
+/* ----------------------------------------------------------------------- */
+/* Serving loop                                                             */
+/* ----------------------------------------------------------------------- */
+/* Independent solve requests (setup + run! of the reference, one problem
+ * each: src/problem.jl:321, src/timestep.jl:277) admitted into device memory
+ * by cles_solver_footprint under an HBM budget and run concurrently (up to
+ * `runners` at once), each on its own context and stream.  Requests are
+ * considered in arrival order; one that does not fit is passed over for
+ * smaller ones until it has been passed over 4 times, after which nothing
+ * behind it is admitted until it runs.  A request's result is bitwise the
+ * same alone and in any mix.  No counterpart in the reference, which runs one
+ * problem per process. */
+typedef struct cles_server cles_server;
+
+enum { CLES_REQ_QUEUED = 0, CLES_REQ_RUNNING = 1, CLES_REQ_DONE = 2, CLES_REQ_FAILED = 3 };
+
+int cles_server_create(int device, size_t hbm_budget, int runners, cles_server **server);
+/* Drops queued requests, waits for running ones, frees everything. */
+int cles_server_destroy(cles_server *server);
+/* Copies the padded initial state (as cles_solver_upload); CLES_ERR_NOMEM if
+ * the request can never fit the budget. */
+int cles_server_submit(cles_server *server, const cles_solver_desc *desc, const double *Q_host,
+                       const int64_t padded[3], const int halo[3], double tfinal, int64_t nmax, int64_t *id);
+/* Blocks until the request is done or failed, or for timeout_s (< 0: no
+ * limit); *state is its CLES_REQ_* state. */
+int cles_server_wait(cles_server *server, int64_t id, double timeout_s, int *state);
+/* A finished request's state (same layout as submitted; NULL skips it), run!
+ * status (CLES_OK or 3..10, message in cles_last_error), time and steps. */
+int cles_server_result(cles_server *server, int64_t id, double *Q_host, const int64_t padded[3],
+                       const int halo[3], int *status, double *t, int64_t *steps);
+/* Seconds since the server started at submission, admission and completion,
+ * and the admitted footprint. */
+int cles_server_timing(cles_server *server, int64_t id, double *t_submit, double *t_admit, double *t_done,
+                       size_t *footprint);
+int cles_server_release(cles_server *server, int64_t id);
+int cles_server_stats(cles_server *server, size_t *in_use, int *running, int *queued);
 
 #ifdef __cplusplus
 }

@@ -145,3 +145,77 @@ def shock_interface(n, Lx=2.0, Ly=1.0, MS=1.5, U=0.3):
                       "eta_t": 0.28, "eta_Y": 0.28},
              (0, 1): {"pinf": p1, "sigma": 0.25, "beta_t": -1.0}}
     return q, L, nscbc
+
+
+# --------------------------------------------------------------------------
+# Batches of one-dimensional two-gas shock tubes (the shape of
+# test/cases.jl:84-107): He at (p_L, rho_L) on the left, CO2 at (1, 1) on the
+# right, diaphragm at x = 0.5 smeared over 2h, Dirichlet ends holding the
+# initial end states; N_i ~ U{nmin..nmax}, p_L ~ U[2, 20], rho_L ~ U[0.1, 10]
+# from one seeded generator.
+# --------------------------------------------------------------------------
+def tube_specs(count, seed=20261202, nmin=128, nmax=2048):
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(count):
+        n = int(rng.integers(nmin, nmax + 1))
+        pl = float(rng.uniform(2.0, 20.0))
+        rl = float(rng.uniform(0.1, 10.0))
+        out.append({"n": n, "pL": pl, "rhoL": rl, "pR": 1.0, "rhoR": 1.0})
+    return out
+
+
+def tube_state(spec):
+    """(q arrays over the nodes, (left, right) Dirichlet conserved states, L, h)."""
+    n = spec["n"]
+    L = 1.0
+    h = L / (n - 1)
+    x = np.arange(n) * h
+    th = 0.5 * (1 + np.tanh((x - 0.5) / (2 * h)))
+    Y = [1 - th, th]
+    p = (1 - th) * spec["pL"] + th * spec["pR"]
+    rho = (1 - th) * spec["rhoL"] + th * spec["rhoR"]
+    z = 0 * x
+    q = conserved((z, z, z, p, rho), species=HE_CO2, Y=Y)
+    one = np.ones(1)
+    qL = [float(v[0]) for v in conserved((0 * one, 0 * one, 0 * one, spec["pL"] * one, spec["rhoL"] * one),
+                                          species=HE_CO2, Y=[one, 0 * one])]
+    qR = [float(v[0]) for v in conserved((0 * one, 0 * one, 0 * one, spec["pR"] * one, spec["rhoR"] * one),
+                                          species=HE_CO2, Y=[0 * one, one])]
+    return [a.reshape(n, 1, 1) for a in q], (qL, qR), L, h
+
+
+def tube_oracle(spec, precision_art=True, cfl=0.4):
+    q, (qL, qR), L, h = tube_state(spec)
+    n = spec["n"]
+    return oracle_solver((n, 1, 1), (L, h, h), q, species=HE_CO2, art=precision_art, cfl=cfl,
+                         bcs=((O.BC_DIRICHLET, O.BC_DIRICHLET), (0, 0), (0, 0)), bc_state={(0, 0): qL, (0, 1): qR})
+
+
+def tube_batch(ctx, specs, precision="fp64", tfinal=None, nmax=2 ** 62, art=True, cfl=0.4):
+    """A GPU Batch of the tubes, uploaded with the same initial arrays."""
+    import compactles_hip as ch
+    eos = ch.IdealMixture([ch.IdealSpecies(f"s{k}", R, g) for k, (R, g) in enumerate(HE_CO2)])
+    dummy = ch.DirichletBC(lambda x, y, z, t: None)
+    prob = ch.Problem(domain=((0.0, 1.0),) * 3, bcs=((dummy, dummy), (ch.PeriodicBC(), ch.PeriodicBC()),
+                                                   (ch.PeriodicBC(), ch.PeriodicBC())),
+                      ic=lambda x, y, z: None, eos=eos, transport=ch.Transport(mu0=0.0))
+    num = ch.Numerics(n_global=(1, 1, 1), art=ch.ArtParams(enabled=art), cfl=cfl, filter_interval=1,
+                      precision=precision)
+    insts, states = [], []
+    for k, sp in enumerate(specs):
+        q, (qL, qR), L, h = tube_state(sp)
+        tf = tfinal[k] if isinstance(tfinal, (list, tuple)) else (tfinal if tfinal is not None else float("inf"))
+        nm = nmax[k] if isinstance(nmax, (list, tuple)) else nmax
+        insts.append(ch.Instance(n=sp["n"], domain=(0.0, L), ic=None, tfinal=tf, nmax=nm,
+                                 lo=ch.Prim(rho=sp["rhoL"], p=sp["pL"], Y=(1.0, 0.0)),
+                                 hi=ch.Prim(rho=sp["rhoR"], p=sp["pR"], Y=(0.0, 1.0))))
+        states.append(q)
+    b = ch.Batch(ctx, prob, num, insts)
+    for k, q in enumerate(states):
+        n = specs[k]["n"]
+        Q = np.zeros((n + 8, 1, 1, len(q)), order="F")
+        for c in range(len(q)):
+            Q[4:4 + n, 0, 0, c] = q[c][:, 0, 0]
+        b.upload(k, Q)
+    return b
