@@ -265,6 +265,8 @@ def main():
     ap.add_argument("--no-forms", action="store_true",
                     help="skip the FP32-mixed and batched forms that the default FP64 run also reports")
     ap.add_argument("--tubes", type=int, default=4096, help="instances of the batched form")
+    ap.add_argument("--only", choices=["tubes", "serving"], default=None,
+                    help="measure only the batched-tubes (FP64) or serving form and print its JSON (profiling runs)")
     ap.add_argument("--cpu-n", type=int, default=256)
     ap.add_argument("--profile-steps", type=int, default=2)
     ap.add_argument("--counters", type=int, default=0, metavar="STEPS",
@@ -281,6 +283,12 @@ def main():
     n = args.n
     if args.counters:
         counters_run(ch, ctx, n, args.counters, args.precision)
+        return
+    if args.only == "tubes":
+        print(json.dumps(measure_tubes(ch, ctx, args.tubes, args.steps, args.warmup, args.precision)), flush=True)
+        return
+    if args.only == "serving":
+        print(json.dumps(measure_serving(ch)), flush=True)
         return
     primary = measure(ch, ctx, n, args.precision, args.steps, args.warmup, args.profile_steps)
     out = {

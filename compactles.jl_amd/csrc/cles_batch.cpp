@@ -122,7 +122,7 @@ void build_plan_set(PlanSet &ps, std::vector<unsigned char> &blob, const cles_sc
         auto it = cache.find(key);
         if (it == cache.end()) {
             LineTables t;
-            build_line_tables(t, s, n[i], periodic, h[i], RAG_MAX_CHUNK, 1);
+            build_line_tables(t, s, n[i], periodic, h[i], RAG_MAX_CHUNK, 1, true);
             if (t.dev.P > 64)
                 throw Error{CLES_ERR_UNSUPPORTED, "batched instances are limited to " +
                                                       std::to_string(64 * RAG_MAX_CHUNK) + " points"};
@@ -135,6 +135,8 @@ void build_plan_set(PlanSet &ps, std::vector<unsigned char> &blob, const cles_sc
             d.row64 = (const double *)(uintptr_t)(base + t.o_row64);
             d.row32 = (const float *)(uintptr_t)(base + t.o_row32);
             d.pcr = (const double *)(uintptr_t)(base + t.o_pcr);
+            d.rowt64 = (const double *)(uintptr_t)(base + t.o_rowt64);
+            d.rowt32 = (const float *)(uintptr_t)(base + t.o_rowt32);
             it = cache.emplace(key, std::make_pair(d, base)).first;
         }
         ps.dev[i] = it->second.first;
@@ -166,6 +168,8 @@ void rebase(std::vector<LineDev> &v, unsigned char *base)
         d.row64 = (const double *)(base + (uintptr_t)d.row64);
         d.row32 = (const float *)(base + (uintptr_t)d.row32);
         d.pcr = (const double *)(base + (uintptr_t)d.pcr);
+        d.rowt64 = (const double *)(base + (uintptr_t)d.rowt64);
+        d.rowt32 = (const float *)(base + (uintptr_t)d.rowt32);
     }
 }
 
@@ -206,7 +210,17 @@ struct BImpl {
         r.q = filter ? b.fq : b.dq;
         r.M = filter ? b.fM : b.dM;
         r.sym = filter ? b.fsym : b.dsym;
+        r.nf = 1;
+        r.fstride = b.N;
         return r;
+    }
+    // nf fields laid out at stride N, one launch
+    void lines(bool filter, int nf, const void *in, const EpiArgs &e, const double *dtv = nullptr,
+               const int *mask = nullptr, int dtype = -1) const
+    {
+        RagLines r = rag(filter, dtv, mask);
+        r.nf = nf;
+        launch_line_rag(r, dtype < 0 ? b.prec : dtype, in, e, b.st());
     }
     void line(bool filter, const void *in, const EpiArgs &e, const int *mask = nullptr) const
     {
@@ -319,26 +333,21 @@ struct BImpl {
     void rhs(bool primitives_current, bool fuse, double A, double B) const
     {
         if (!primitives_current) primitives();
-        void *vel[3] = {b.u, b.v, b.w};
-        for (int j = 0; j < 3; j++) line(false, vel[j], store(b.gu[j]));
+        lines(false, 3, b.u, store(b.gu[0]));                                  // u, v, w -> grad_u
         artificial();
-        line(false, b.T, store(b.gT));
-        if (b.Ns > 1)
-            for (int sp = 0; sp < b.Ns; sp++) line(false, b.Y[sp], store(b.gY[sp]));
+        lines(false, b.Ns > 1 ? 1 + b.Ns : 1, b.T, store(b.gT));              // T, Y_k -> gradients
         fluxes();
-        for (int c = 0; c < b.C; c++) {
-            if (fuse) {
-                EpiArgs e{};
-                e.kind = EPI_LSRK;
-                e.a0 = nullptr;   // dQ = -D_x F_x
-                e.du = b.du + (size_t)c * b.N;
-                e.Q = b.Q + (size_t)c * b.N;
-                e.c0 = A;
-                e.c1 = B;
-                launch_line_rag(rag(false, b.dt, b.stepped), b.prec, b.flux[c], e, b.st());
-            } else {
-                line(false, b.flux[c], store(dQc(c), EPI_STORE_NEG));
-            }
+        if (fuse) {
+            EpiArgs e{};
+            e.kind = EPI_LSRK;
+            e.a0 = nullptr;   // dQ = -D_x F_x
+            e.du = b.du;
+            e.Q = b.Q;
+            e.c0 = A;
+            e.c1 = B;
+            lines(false, b.C, b.flux[0], e, b.dt, b.stepped);
+        } else {
+            lines(false, b.C, b.flux[0], store(dQc(0), EPI_STORE_NEG));
         }
     }
 
@@ -407,12 +416,10 @@ struct BImpl {
     // timestep.jl:414-426 on the FP64 state of the instances due a filter
     void filter_state() const
     {
-        for (int c = 0; c < b.C; c++) {
-            EpiArgs e{};
-            e.kind = EPI_STORE;
-            e.out = b.Q + (size_t)c * b.N;
-            launch_line_rag(rag(true, nullptr, b.filt), CLES_F64, b.Q + (size_t)c * b.N, e, b.st());
-        }
+        EpiArgs e{};
+        e.kind = EPI_STORE;
+        e.out = b.Q;
+        lines(true, b.C, b.Q, e, nullptr, b.filt, CLES_F64);
     }
 
     // one run! iteration of every active instance (timestep.jl:345-399)
@@ -550,13 +557,26 @@ int cles_batch_create(cles_ctx *ctx, const cles_solver_desc *desc, int64_t count
             b->Q = (double *)b->alloc(NN * b->C * sizeof(double));
             b->du = (double *)b->alloc(NN * b->C * sizeof(double));
             b->dQ = b->alloc(NN * b->C * b->wsize());
-            for (void **fp : {&b->rho, &b->u, &b->v, &b->w, &b->p, &b->T, &b->c, &b->cp}) *fp = b->wfield();
-            for (int k = 0; k < b->Ns; k++) b->Y[k] = b->wfield();
-            for (int j = 0; j < 3; j++) b->gu[j] = b->wfield();
-            b->gT = b->wfield();
+            // fields that one line launch treats together are contiguous:
+            // u | v | w, T | Y_1.. | , grad_u, grad_T | grad_Y, fluxes
+            const size_t ws = NN * b->wsize();
+            auto blockf = [&](int k) { return (char *)b->alloc(ws * (size_t)k); };
+            for (void **fp : {&b->rho, &b->p, &b->c, &b->cp}) *fp = b->wfield();
+            char *vel = blockf(3);
+            b->u = vel;
+            b->v = vel + ws;
+            b->w = vel + 2 * ws;
+            char *ty = blockf(1 + b->Ns);
+            b->T = ty;
+            for (int k = 0; k < b->Ns; k++) b->Y[k] = ty + (size_t)(1 + k) * ws;
+            char *gv = blockf(3);
+            for (int j = 0; j < 3; j++) b->gu[j] = gv + (size_t)j * ws;
+            char *gty = blockf(b->Ns > 1 ? 1 + b->Ns : 1);
+            b->gT = gty;
             if (b->Ns > 1)
-                for (int k = 0; k < b->Ns; k++) b->gY[k] = b->wfield();
-            for (int c = 0; c < b->C; c++) b->flux[c] = b->wfield();
+                for (int k = 0; k < b->Ns; k++) b->gY[k] = gty + (size_t)(1 + k) * ws;
+            char *fl = blockf(b->C);
+            for (int c = 0; c < b->C; c++) b->flux[c] = fl + (size_t)c * ws;
             b->mu_art = b->wfield();
             b->beta_art = b->wfield();
             b->kappa_art = b->wfield();

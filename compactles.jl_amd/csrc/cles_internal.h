@@ -64,8 +64,11 @@ struct LineDev {
     const float *row32;
     const double *pcr;     // per step s, chunk k: alpha, beta (B*B each, row-major);
                            // then per chunk the final inverse block
+    const double *rowt64;  // chunk-major copy of the per-row constants (ragged plans):
+    const float *rowt32;   //   [(field * RAG_MAX_CHUNK + t) * P + k] = row of point t of chunk k
 };
 
+constexpr int RAG_MAX_CHUNK = 32;   // points per thread in the ragged kernel
 constexpr int COEF_CI = 0, COEF_A0 = 8, COEF_CLO = 16, COEF_CHI = 80, COEF_N = 144;
 
 struct HostPlan {
@@ -83,10 +86,12 @@ struct HostPlan {
 struct LineTables {
     LineDev dev{};
     std::vector<unsigned char> image;
-    size_t o_coef32 = 0, o_row64 = 0, o_row32 = 0, o_pcr = 0;
+    size_t o_coef32 = 0, o_row64 = 0, o_row32 = 0, o_pcr = 0, o_rowt64 = 0, o_rowt32 = 0;
 };
+// chunk_major: also store the chunk-major row tables of the ragged kernel
+// (max_chunk must then be RAG_MAX_CHUNK).
 void build_line_tables(LineTables &out, const cles_scheme_desc &s, int n, int periodic, double h, int max_chunk,
-                       int dim);
+                       int dim, bool chunk_major = false);
 void bind_line_tables(LineDev &dev, const LineTables &t, unsigned char *base);
 
 // Builds the device tables for one (scheme, n, periodic, h).  Throws Error.
@@ -156,8 +161,9 @@ struct RagLines {
     const double *dt;        // per-instance dt for the LSRK epilogue (NULL: epi.dt)
     const int *mask;         // per-instance: nonzero applies the epilogue (NULL: all)
     int q, M, sym;           // scheme shape shared by every instance
+    int nf;                  // fields per launch: field f reads f + f*fstride and offsets
+    int64_t fstride;         //   the epilogue's out / du / Q by f*fstride (elements)
 };
-constexpr int RAG_MAX_CHUNK = 32;   // points per thread in the ragged kernel
 void launch_line_rag(const RagLines &r, int dtype, const void *f, const EpiArgs &epi, hipStream_t st);
 
 }  // namespace cles

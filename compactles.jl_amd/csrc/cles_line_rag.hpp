@@ -16,6 +16,79 @@ namespace cles {
 
 __device__ __forceinline__ int rag_slot(int x) { return x + (x >> 5); }
 
+// Blocks are dispatched round-robin over the 8 XCDs; renumbering them so that
+// each XCD takes one contiguous range keeps the nf blocks of one work item
+// (same lines and tables, different fields) on one XCD's L2.
+__device__ __forceinline__ unsigned rag_block(unsigned b, unsigned nb)
+{
+    const unsigned nfull = nb & ~7u;
+    return b < nfull ? (b & 7u) * (nfull >> 3) + (b >> 3) : b;
+}
+
+template <typename T> __device__ __forceinline__ const T *rowtp(const LineDev &p);
+template <> __device__ __forceinline__ const double *rowtp<double>(const LineDev &p) { return p.rowt64; }
+template <> __device__ __forceinline__ const float *rowtp<float>(const LineDev &p) { return p.rowt32; }
+
+// Chunk::solve with the chunk-major tables: the P lanes of a line read
+// consecutive words for each (field, t).  Same arithmetic, same order.
+template <typename T, int Q, int M, int SYM, int CL>
+__device__ __forceinline__ void rag_solve(Chunk<T, Q, M, SYM, CL> &ch, const LineDev &p, int k, int len)
+{
+    const int P = p.P;
+    const T *rw = rowtp<T>(p) + k;
+    auto R = [&](int f, int t) -> T { return rw[((size_t)f * CL + t) * P]; };
+    T *d = ch.d;
+    if constexpr (Q == 1) {
+#pragma unroll
+        for (int t = 1; t < CL; t++)
+            if (t < len) d[t] -= R(R1_L, t) * d[t - 1];
+#pragma unroll
+        for (int t = CL - 1; t >= 0; t--) {
+            if (t < len) {
+                const T nxt = d[t + 1 < CL ? t + 1 : CL - 1];
+                d[t] = (t == len - 1) ? d[t] * R(R1_DINV, t) : (d[t] - R(R1_C, t) * nxt) * R(R1_DINV, t);
+            }
+        }
+    } else {
+#pragma unroll
+        for (int t = 0; t < CL - 1; t++) {
+            if (t < len - 1) {
+                const T xt = d[t];
+                d[t + 1] -= R(R2_L1, t) * xt;
+                if (t + 2 < CL)
+                    if (t < len - 2) d[t + 2 < CL ? t + 2 : t] -= R(R2_L2, t) * xt;
+            }
+        }
+#pragma unroll
+        for (int t = CL - 1; t >= 0; t--) {
+            if (t < len) {
+                T acc = d[t];
+                if (t + 1 < CL)
+                    if (t + 1 < len) acc -= R(R2_U1, t) * d[t + 1 < CL ? t + 1 : t];
+                if (t + 2 < CL)
+                    if (t + 2 < len) acc -= R(R2_U2, t) * d[t + 2 < CL ? t + 2 : t];
+                d[t] = acc * R(R2_U0INV, t);
+            }
+        }
+    }
+}
+
+// corrected() with the chunk-major spikes
+template <typename T, int Q, int CL>
+__device__ __forceinline__ double rag_corrected(const LineDev &p, int k, int t, T y, const double (&tp)[Q],
+                                                const double (&hn)[Q])
+{
+    const double *rw = p.rowt64 + k;
+    auto R = [&](int f) -> double { return rw[((size_t)f * CL + t) * p.P]; };
+    if constexpr (Q == 1) {
+        return (double)y - (R(R1_V) * tp[0] + R(R1_W) * hn[0]);
+    } else {
+        double acc = R(R2_V0) * tp[0] + R(R2_W0) * hn[0];
+        acc += R(R2_V1) * tp[1] + R(R2_W1) * hn[1];
+        return (double)y - acc;
+    }
+}
+
 template <typename T, int Q, int M, int SYM, int EPI>
 __global__ void __launch_bounds__(64) k_line_rag(RagLines r, const T *__restrict__ f, EpiArgs e)
 {
@@ -23,7 +96,11 @@ __global__ void __launch_bounds__(64) k_line_rag(RagLines r, const T *__restrict
     extern __shared__ double smem[];
     T *tile = (T *)smem;
     const int lane = threadIdx.x;
-    const int *it = r.items + 3 * blockIdx.x;
+    const unsigned logical = rag_block(blockIdx.x, gridDim.x);
+    const int item = (int)(logical / (unsigned)r.nf), fld = (int)(logical % (unsigned)r.nf);
+    const int64_t fo = (int64_t)fld * r.fstride;
+    f += fo;
+    const int *it = r.items + 3 * item;
     const int P = it[0], start = it[1], count = it[2];
     const int TS = 33 * P;   // rag_slot(32 P - 1) < 33 P
 
@@ -79,13 +156,13 @@ __global__ void __launch_bounds__(64) k_line_rag(RagLines r, const T *__restrict
     for (int kap = 0; kap < 8; kap++) fe[kap] = (p.closed && kap < n) ? trow[rag_slot(n - 1 - kap)] : T(0);
     Chunk<T, Q, M, SYM, CL> ch;
     ch.rhs(p, k, len, fv, fe);
-    ch.solve(p, s, len);
+    rag_solve(ch, p, k, len);
     double ends[2 * Q], tp[Q], hn[Q];
     ch.ends(len, ends);
     reduced_solve_wave<Q>(p, k, k, li * P, ends, tp, hn);
 #pragma unroll
     for (int t = 0; t < CL; t++)
-        if (t < len) ch.d[t] = (T)corrected<T, Q>(p, s + t, ch.d[t], tp, hn);
+        if (t < len) ch.d[t] = (T)rag_corrected<T, Q, CL>(p, k, t, ch.d[t], tp, hn);
     __syncthreads();   // every window read is done before the tile is overwritten
     if (live) {
 #pragma unroll
@@ -100,6 +177,11 @@ __global__ void __launch_bounds__(64) k_line_rag(RagLines r, const T *__restrict
         const int nj = r.plans[ij].n;
         const int64_t oj = r.off[ij];
         EpiArgs ej = e;
+        if (fo) {
+            if (ej.out) ej.out = (T *)ej.out + fo;
+            if (ej.du) ej.du += fo;
+            if (ej.Q) ej.Q += fo;
+        }
         if (r.dt) ej.dt = r.dt[ij];
         const T *row = tile + lj * TS;
         for (int x0 = 0; x0 < nj; x0 += 512) {
@@ -121,9 +203,14 @@ __global__ void __launch_bounds__(64) k_line_rag(RagLines r, const T *__restrict
 template <typename T, int Q, int M, int SYM, int EPI>
 void launch_rag_variant(const RagLines &r, const T *f, const EpiArgs &e, hipStream_t st)
 {
-    if (r.nitems <= 0) return;
+    if (r.nitems <= 0 || r.nf <= 0) return;
+    if (r.nf > 1 && e.kind != EPI_STORE && e.kind != EPI_STORE_NEG && e.kind != EPI_LSRK)
+        throw Error{CLES_ERR_INVALID, "several fields per launch need a store or LSRK epilogue"};
+    if (e.kind == EPI_LSRK && e.a0 && r.nf > 1)
+        throw Error{CLES_ERR_INVALID, "the ragged LSRK epilogue takes no accumulator with several fields"};
     const size_t shm = (size_t)64 * 33 * sizeof(T);
-    hipLaunchKernelGGL((k_line_rag<T, Q, M, SYM, EPI>), dim3((unsigned)r.nitems), dim3(64), shm, st, r, f, e);
+    const int64_t nb = (int64_t)r.nitems * r.nf;
+    hipLaunchKernelGGL((k_line_rag<T, Q, M, SYM, EPI>), dim3((unsigned)nb), dim3(64), shm, st, r, f, e);
     CLES_HIP(hipGetLastError());
 }
 

@@ -163,7 +163,7 @@ void chunk_solve(const ChunkFactor &F, std::vector<double> &x)
 }  // namespace
 
 void build_line_tables(LineTables &out, const cles_scheme_desc &s, int n, int periodic, double h, int max_chunk,
-                       int dim)
+                       int dim, bool chunk_major)
 {
     const int q = s.q, M = s.M, nc = s.nclosure;
     if (q < 1 || q > 2) throw Error{CLES_ERR_UNSUPPORTED, "scheme LHS half-bandwidth q must be 1 or 2"};
@@ -337,17 +337,36 @@ void build_line_tables(LineTables &out, const cles_scheme_desc &s, int n, int pe
             coef[COEF_CHI + j * 8 + kk] = s.closure_rhs[j][kk] * (scale * sgn);
         }
 
-    // host image: coef64 | coef32 | row64 | row32 | pcr (256-byte aligned parts)
+    // chunk-major copy of the row constants for the ragged kernel
+    std::vector<double> rowt;
+    if (chunk_major) {
+        if (max_chunk != RAG_MAX_CHUNK) throw Error{CLES_ERR_INVALID, "chunk-major tables need 32-point chunks"};
+        rowt.assign((size_t)nrow * RAG_MAX_CHUNK * P, 0.0);
+        for (int f = 0; f < nrow; f++)
+            for (int k = 0; k < P; k++)
+                for (int t = 0; t < clen(k); t++)
+                    rowt[((size_t)f * RAG_MAX_CHUNK + t) * P + k] = row[(size_t)f * n + cstart(k) + t];
+    }
+
+    // host image: coef64 | coef32 | row64 | row32 | pcr | rowt64 | rowt32 (256-byte aligned parts)
     size_t b_coef64 = COEF_N * sizeof(double), b_coef32 = COEF_N * sizeof(float);
     size_t b_row64 = row.size() * sizeof(double), b_row32 = row.size() * sizeof(float);
     size_t b_pcr = pcr.size() * sizeof(double);
+    size_t b_rowt64 = rowt.size() * sizeof(double), b_rowt32 = rowt.size() * sizeof(float);
     auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
     out.o_coef32 = al(b_coef64);
     out.o_row64 = out.o_coef32 + al(b_coef32);
     out.o_row32 = out.o_row64 + al(b_row64);
     out.o_pcr = out.o_row32 + al(b_row32);
-    const size_t total = out.o_pcr + al(std::max<size_t>(b_pcr, 8));
+    out.o_rowt64 = out.o_pcr + al(std::max<size_t>(b_pcr, 8));
+    out.o_rowt32 = out.o_rowt64 + al(b_rowt64);
+    const size_t total = out.o_rowt32 + al(b_rowt32);
     out.image.assign(total, 0);
+    if (!rowt.empty()) {
+        std::vector<float> rowt32(rowt.begin(), rowt.end());
+        std::memcpy(out.image.data() + out.o_rowt64, rowt.data(), b_rowt64);
+        std::memcpy(out.image.data() + out.o_rowt32, rowt32.data(), b_rowt32);
+    }
     std::memcpy(out.image.data(), coef.data(), b_coef64);
     std::vector<float> coef32(coef.begin(), coef.end()), row32(row.begin(), row.end());
     std::memcpy(out.image.data() + out.o_coef32, coef32.data(), b_coef32);
@@ -369,6 +388,8 @@ void bind_line_tables(LineDev &dev, const LineTables &t, unsigned char *base)
     dev.row64 = (const double *)(base + t.o_row64);
     dev.row32 = (const float *)(base + t.o_row32);
     dev.pcr = (const double *)(base + t.o_pcr);
+    dev.rowt64 = t.o_rowt64 != t.o_rowt32 ? (const double *)(base + t.o_rowt64) : nullptr;
+    dev.rowt32 = t.o_rowt64 != t.o_rowt32 ? (const float *)(base + t.o_rowt32) : nullptr;
 }
 
 void build_line_plan(HostPlan &hp, const cles_scheme_desc &s, int n, int periodic, double h, int max_chunk)
