@@ -694,6 +694,116 @@ __global__ void __launch_bounds__(64) k_line_xw(LineDev p, const T *__restrict__
     }
 }
 
+// x sweep for lines of exactly 16 P points in equal 16-point chunks (the
+// production grids): the same algorithm as k_line_xw with the line length,
+// chunk length, lines per wave (NL = 64 / P) and the element-to-line map fixed
+// at compile time, line offsets computed once per wave, a mask for the
+// periodic wrap, and no closure code on periodic lines.
+template <typename T, int Q, int M, int SYM, int EPI, int NL, bool PER>
+__global__ void __launch_bounds__(64) k_line_xwf(LineDev p, const T *__restrict__ f, int64_t sy, int64_t sz, int ny,
+                                                 int64_t nlines, EpiArgs e)
+{
+    constexpr int P = 64 / NL;            // chunks (lanes) per line
+    constexpr int N = XCL * P;            // points per line
+    constexpr int TS = N + N / XCL + 1;   // padded tile row
+    constexpr int EPL = N / 64;           // staged elements per lane and line
+    __shared__ T tile[NL * TS];
+    const int lane = threadIdx.x;
+    const int64_t l0 = (int64_t)blockIdx.x * NL;
+    int64_t loff[NL];
+    bool lval[NL];
+#pragma unroll
+    for (int lj = 0; lj < NL; lj++) {
+        const int64_t l = l0 + lj;
+        lval[lj] = l < nlines;
+        loff[lj] = (l % ny) * sy + (l / ny) * sz;
+    }
+    {
+        T tmp[NL * EPL];
+#pragma unroll
+        for (int lj = 0; lj < NL; lj++)
+#pragma unroll
+            for (int q = 0; q < EPL; q++) tmp[lj * EPL + q] = lval[lj] ? f[loff[lj] + lane + 64 * q] : T(0);
+#pragma unroll
+        for (int lj = 0; lj < NL; lj++)
+#pragma unroll
+            for (int q = 0; q < EPL; q++) tile[lj * TS + xw_slot(lane + 64 * q)] = tmp[lj * EPL + q];
+    }
+    __syncthreads();
+    const int li = lane / P, k = lane % P;
+    const int s = k * XCL;
+    T *trow = tile + li * TS;
+    T fv[XCL + 2 * M];
+#pragma unroll
+    for (int t = -M; t < XCL + M; t++) {
+        int r = s + t;
+        if constexpr (PER) {
+            fv[t + M] = trow[xw_slot(r & (N - 1))];
+        } else {
+            fv[t + M] = (r >= 0 && r < N) ? trow[xw_slot(r < 0 ? 0 : (r >= N ? N - 1 : r))] : T(0);
+        }
+    }
+    T fe[8];
+#pragma unroll
+    for (int kap = 0; kap < 8; kap++) fe[kap] = PER ? T(0) : trow[xw_slot(N - 1 - kap)];
+    Chunk<T, Q, M, SYM, XCL> ch;
+    if constexpr (PER) {
+        const T *cf = coefp<T>(p);
+        T ci[M];
+#pragma unroll
+        for (int m = 0; m < M; m++) ci[m] = cf[COEF_CI + m];
+        const T a0 = cf[COEF_A0];
+#pragma unroll
+        for (int t = 0; t < XCL; t++) {
+            T acc;
+            if constexpr (SYM) {
+                acc = a0 * fv[t + M];
+#pragma unroll
+                for (int m = 1; m <= M; m++) acc += ci[m - 1] * (fv[t + M + m] + fv[t + M - m]);
+            } else {
+                acc = T(0);
+#pragma unroll
+                for (int m = 1; m <= M; m++) acc += ci[m - 1] * (fv[t + M + m] - fv[t + M - m]);
+            }
+            ch.d[t] = acc;
+        }
+    } else {
+        ch.rhs(p, k, XCL, fv, fe);
+    }
+    const int rs = PER ? 0 : s;
+    const int kc = PER ? 0 : k;
+    ch.solve(p, rs, XCL);
+    double r[2 * Q], tp[Q], hn[Q];
+    ch.ends(XCL, r);
+    reduced_solve_wave<Q>(p, k, kc, li * P, r, tp, hn);
+    T res[XCL];
+#pragma unroll
+    for (int t = 0; t < XCL; t++) res[t] = (T)corrected<T, Q>(p, rs + t, ch.d[t], tp, hn);
+    __syncthreads();   // all window reads done before the tile is overwritten
+#pragma unroll
+    for (int t = 0; t < XCL; t++) trow[xw_slot(s + t)] = res[t];
+    __syncthreads();
+#pragma unroll
+    for (int lj = 0; lj < NL; lj++) {
+#pragma unroll
+        for (int q0 = 0; q0 < EPL; q0 += 8) {
+            int64_t o[8];
+            T v[8];
+            bool ok[8];
+#pragma unroll
+            for (int i = 0; i < 8; i++) {
+                const int q = q0 + i;
+                const bool in = q < EPL && lval[lj];
+                const int x = lane + 64 * (q < EPL ? q : 0);
+                ok[i] = in;
+                o[i] = loff[lj] + x;
+                v[i] = tile[lj * TS + xw_slot(x)];
+            }
+            epilogue_batch<T, EPI, 8>(e, o, v, ok);
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Dispatch
 // ---------------------------------------------------------------------------
@@ -726,7 +836,17 @@ void launch_variant(const HostPlan &hp, int dim, const View3 &v, const T *f, con
     const int B = 2 * Q;
     const size_t lim = lds_limit();
     const int maxNT = Q == 1 ? 1024 : 512;
-    if (dim == 1 && P <= 64 && p.m + (p.extra ? 1 : 0) <= XCL) {
+    if (dim == 1 && p.extra == 0 && p.m == XCL && P == 16 && (p.periodic || p.closed)) {
+        // 256-point lines: four per wave
+        const int64_t nlines = v.n[1] * v.n[2];
+        const int64_t nblk = (nlines + 3) / 4;
+        if (p.periodic)
+            hipLaunchKernelGGL((k_line_xwf<T, Q, M, SYM, EPI, 4, true>), dim3((unsigned)nblk), dim3(64), 0, st, p, f,
+                               v.s[1], v.s[2], (int)v.n[1], nlines, e);
+        else
+            hipLaunchKernelGGL((k_line_xwf<T, Q, M, SYM, EPI, 4, false>), dim3((unsigned)nblk), dim3(64), 0, st, p, f,
+                               v.s[1], v.s[2], (int)v.n[1], nlines, e);
+    } else if (dim == 1 && P <= 64 && p.m + (p.extra ? 1 : 0) <= XCL) {
         const int64_t nlines = v.n[1] * v.n[2];
         const int nl = 64 / P;
         const int TS = p.n + p.n / XCL + 1;

@@ -174,3 +174,38 @@ def test_sod_gpu_matches_reference_measurement(ctx):
     assert 2.8e-3 < er < 3.0e-3 and 4.5e-3 < eu < 4.7e-3 and 2.4e-3 < ep < 2.6e-3
     for nm in ("rho", "u", "p"):
         assert Hh.relerr(gs.field(nm), os_.interior(os_.field(nm))) <= 1e-9
+
+
+@pytest.mark.parametrize("walls", [False, True])
+def test_rhs_step_filter_256_point_x_lines(ctx, walls):
+    """x lines of 256 points take the specialised x-sweep kernel; every
+    epilogue kind runs on it (store, negated store, subtract, the mu*/beta*,
+    kappa* coefficients, the fused LSRK update, the in-place state filter)."""
+    n, L = (256, 16, 12), (TWO_PI, 0.5, 0.4)
+    per = (not walls, True, True)
+    X, Y, Z = cases.grid(n, L, periodic=per)
+    ky, kz = 2 * np.pi / L[1], 2 * np.pi / L[2]
+    prim = (0.2 * np.sin(X) * np.cos(ky * Y) + (0.1 * np.sin(0.5 * X) ** 2 if walls else 0),
+            0.05 * np.cos(2 * X + ky * Y), 0.04 * np.sin(X + kz * Z),
+            1.0 + 0.1 * np.cos(X) * np.cos(kz * Z), 1.0 + 0.05 * np.sin(ky * Y + X))
+    q = cases.conserved(prim)
+    bcs = ((1, 1), (0, 0), (0, 0)) if walls else None
+    os_ = cases.oracle_solver(n, L, q, mu0=1e-3, art=True, bcs=bcs)
+    gs = cases.gpu_solver(ctx, n, L, q, mu0=1e-3, art=True, bcs=bcs)
+    if walls:
+        os_.apply_bcs()
+        gs.apply_bcs()
+    os_.compute_rhs(False)
+    gs.compute_rhs(False)
+    _compare_rhs(gs, os_, n)
+    r1, _ = os_.max_rate()
+    gs.max_rate()
+    dt = 0.5 / r1
+    os_.step(dt, True)
+    gs.step(dt, True)
+    os_.filter_state()
+    gs.filter_state()
+    Qg = gs.allocate_state()
+    gs.download(Qg)
+    for c in range(5):
+        assert Hh.relerr(Hh.interior(Qg[..., c], n), os_.interior(os_.Q[..., c])) <= 1e-12, c
