@@ -355,7 +355,7 @@ __device__ __forceinline__ double corrected(const LineDev &p, int g, T y, const 
 // chunks.  UNI: Lx == 64, so each wave holds one chunk and every per-row
 // constant is wave-uniform (scalar loads).
 // ---------------------------------------------------------------------------
-template <typename T, int Q, int M, int SYM, int EPI, bool UNI>
+template <typename T, int Q, int M, int SYM, int EPI, bool UNI, int CL = MC>
 __global__ void __launch_bounds__(Q == 1 ? 1024 : 512) k_line_yz(LineDev p, const T *__restrict__ f, int sd, int64_t so,
                                                   int nlane, int nxb, int Lx, EpiArgs e)
 {
@@ -374,9 +374,9 @@ __global__ void __launch_bounds__(Q == 1 ? 1024 : 512) k_line_yz(LineDev p, cons
     const int s = k * p.m + min(k, p.extra);
     const int len = p.m + (k < p.extra ? 1 : 0);
 
-    T fv[MC + 2 * M];
+    T fv[CL + 2 * M];
 #pragma unroll
-    for (int t = -M; t < MC + M; t++) {
+    for (int t = -M; t < CL + M; t++) {
         T val = T(0);
         if (valid && t < len + M) {
             int r = s + t;
@@ -399,7 +399,7 @@ __global__ void __launch_bounds__(Q == 1 ? 1024 : 512) k_line_yz(LineDev p, cons
             if (kap < n) fe[kap] = fl[(n - 1 - kap) * sd];
     }
 
-    Chunk<T, Q, M, SYM> ch;
+    Chunk<T, Q, M, SYM, CL> ch;
     ch.rhs(p, k, len, fv, fe);
     ch.solve(p, s, len);
     double r[2 * Q], tp[Q], hn[Q];
@@ -407,10 +407,10 @@ __global__ void __launch_bounds__(Q == 1 ? 1024 : 512) k_line_yz(LineDev p, cons
     reduced_solve<Q>(p, smem, NT, Lx, k, lane, r, tp, hn);
     if (!valid) return;
 #pragma unroll
-    for (int t = 0; t < MC; t++)
+    for (int t = 0; t < CL; t++)
         if (t < len) ch.d[t] = (T)corrected<T, Q>(p, s + t, ch.d[t], tp, hn);
 #pragma unroll
-    for (int b = 0; b < MC; b += 8) {
+    for (int b = 0; b < CL; b += 8) {
         int64_t o[8];
         T v[8];
         bool ok[8];
@@ -895,8 +895,12 @@ void launch_variant(const HostPlan &hp, int dim, const View3 &v, const T *f, con
             const int Lx = 64, NT = 64 * P;
             const int nxb = (int)((nlane + Lx - 1) / Lx);
             const size_t shm = (size_t)2 * B * NT * sizeof(double);
-            hipLaunchKernelGGL((k_line_yz<T, Q, M, SYM, EPI, true>), dim3((unsigned)(nxb * nouter)), dim3(NT), shm,
-                               st, p, f, (int)sd, so, (int)nlane, nxb, Lx, e);
+            if (p.m + (p.extra ? 1 : 0) <= 16)
+                hipLaunchKernelGGL((k_line_yz<T, Q, M, SYM, EPI, true, 16>), dim3((unsigned)(nxb * nouter)), dim3(NT),
+                                   shm, st, p, f, (int)sd, so, (int)nlane, nxb, Lx, e);
+            else
+                hipLaunchKernelGGL((k_line_yz<T, Q, M, SYM, EPI, true>), dim3((unsigned)(nxb * nouter)), dim3(NT),
+                                   shm, st, p, f, (int)sd, so, (int)nlane, nxb, Lx, e);
         } else {
             int Lx = 64;
             while (Lx * P > maxNT && Lx > 1) Lx /= 2;

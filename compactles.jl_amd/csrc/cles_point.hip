@@ -211,45 +211,41 @@ __global__ void __launch_bounds__(TPB) k_delta4_col(D4Args<W> a, W *__restrict__
     const W *col = a.f + i + j * sy;
     const int k0 = bz * D4_ZB;
     const bool zact = a.active[2];
-    auto zl = [&](int k) -> double { return (double)col[d4_index(k, nz, a.periodic[2]) * sz]; };
-    double w0 = 0, w1 = 0, w2 = (double)col[k0 * sz], w3 = 0;
-    if (zact) {
-        w0 = zl(k0 - 2);
-        w1 = zl(k0 - 1);
-        w3 = zl(k0 + 1);
+    // the z window of every plane of the column, loaded up front
+    double w[D4_ZB + 4];
+#pragma unroll
+    for (int m = 0; m < D4_ZB + 4; m++) {
+        const int kz = k0 - 2 + m;
+        // z inactive: nz = 1, k0 = 0, and only the centre (m = 2, plane 0) is used
+        w[m] = (zact || m == 2) ? (double)col[d4_index(kz, nz, a.periodic[2]) * sz] : 0.0;
     }
+#pragma unroll
     for (int kk = 0; kk < D4_ZB; kk++) {
         const int k = k0 + kk;
-        if (k >= nz) break;
-        const double w4 = zact ? zl(k + 2) : 0.0;
-        const W *base = col + k * sz;
-        W acc_out = 0;
-        if (a.active[0]) {
-            double acc = 0.0;
+        if (k < nz) {
+            const double wc = w[kk + 2];
+            const W *base = col + k * sz;
+            W acc_out = 0;
+            if (a.active[0]) {
+                double acc = 0.0;
 #pragma unroll
-            for (int m = 0; m < 5; m++) acc += D4[m] * (m == 2 ? w2 : (double)base[xo[m]]);
-            acc_out += (W)(a.wd[0] * fabs(acc));
-        }
-        if (a.active[1]) {
-            double acc = 0.0;
+                for (int m = 0; m < 5; m++) acc += D4[m] * (m == 2 ? wc : (double)base[xo[m]]);
+                acc_out += (W)(a.wd[0] * fabs(acc));
+            }
+            if (a.active[1]) {
+                double acc = 0.0;
 #pragma unroll
-            for (int m = 0; m < 5; m++) acc += D4[m] * (m == 2 ? w2 : (double)base[yo[m]]);
-            acc_out += (W)(a.wd[1] * fabs(acc));
+                for (int m = 0; m < 5; m++) acc += D4[m] * (m == 2 ? wc : (double)base[yo[m]]);
+                acc_out += (W)(a.wd[1] * fabs(acc));
+            }
+            if (zact) {
+                double acc = 0.0;
+#pragma unroll
+                for (int m = 0; m < 5; m++) acc += D4[m] * w[kk + m];
+                acc_out += (W)(a.wd[2] * fabs(acc));
+            }
+            out[(int64_t)k * sz + (int64_t)j * sy + i] = acc_out;
         }
-        if (zact) {
-            double acc = 0.0;
-            acc += D4[0] * w0;
-            acc += D4[1] * w1;
-            acc += D4[2] * w2;
-            acc += D4[3] * w3;
-            acc += D4[4] * w4;
-            acc_out += (W)(a.wd[2] * fabs(acc));
-        }
-        out[(int64_t)k * sz + (int64_t)j * sy + i] = acc_out;
-        w0 = w1;
-        w1 = w2;
-        w2 = zact ? w3 : (k + 1 < nz ? (double)col[(k + 1) * sz] : 0.0);
-        w3 = w4;
     }
 }
 
