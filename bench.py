@@ -124,9 +124,10 @@ def alg_bytes_per_point_step(S, D, precision, nscbc_ops=0):
     return 8.0 * f64 + (8.0 if precision == "fp64" else 4.0) * wp
 
 
-def measure(ch, ctx, n, precision, steps, warmup, profile_steps):
+def measure(ch, ctx, n, precision, steps, warmup, profile_steps, run_mode="host_dt"):
     prob, num = tgv_problem(ch, n, precision)
     solver, _ = ch.setup(prob, num, ctx)
+    solver.set_run_mode(run_mode)
     solver.run(1e9, warmup)
     ctx.synchronize()
     t0 = time.perf_counter()
@@ -265,6 +266,8 @@ def main():
     ap.add_argument("--no-forms", action="store_true",
                     help="skip the FP32-mixed and batched forms that the default FP64 run also reports")
     ap.add_argument("--tubes", type=int, default=4096, help="instances of the batched form")
+    ap.add_argument("--run-mode", choices=["step_graph", "device_dt", "host_dt"], default="host_dt",
+                    help="how run! issues its steps (identical results)")
     ap.add_argument("--only", choices=["tubes", "serving"], default=None,
                     help="measure only the batched-tubes (FP64) or serving form and print its JSON (profiling runs)")
     ap.add_argument("--cpu-n", type=int, default=256)
@@ -290,7 +293,7 @@ def main():
     if args.only == "serving":
         print(json.dumps(measure_serving(ch)), flush=True)
         return
-    primary = measure(ch, ctx, n, args.precision, args.steps, args.warmup, args.profile_steps)
+    primary = measure(ch, ctx, n, args.precision, args.steps, args.warmup, args.profile_steps, args.run_mode)
     out = {
         "metric": f"Mpt-steps/s TGV {n}^3 {args.precision.upper()}",
         "value": primary["value"],
@@ -306,14 +309,15 @@ def main():
         "data": "synthetic (Taylor-Green initial state, no checkpoint)",
         "config": {"workload": f"Taylor-Green vortex {n}^3 periodic, full run! step (max_rate + 5 LSRK(5,4) "
                                "stages + filter_state), C6 derivative, C8 filter, Cook artificial "
-                               "properties on, cfl 0.5", "n": n, "precision": args.precision},
+                               "properties on, cfl 0.5", "n": n, "precision": args.precision,
+                   "run_mode": args.run_mode},
     }
     for k in ("roofline", "step_roofline", "phases_ms_per_step", "phases_GBps", "profiled_kernel_ms_per_step"):
         out[k] = primary[k]
     if not args.no_forms and args.precision == "fp64":
         # the FP32-mixed form of the same workload, and its kinetic-energy
         # difference from the FP64 run after the same number of steps
-        mixed = measure(ch, ctx, n, "fp32", args.steps, args.warmup, args.profile_steps)
+        mixed = measure(ch, ctx, n, "fp32", args.steps, args.warmup, args.profile_steps, args.run_mode)
         out["forms"] = {"fp32_mixed": {
             "value": mixed["value"], "unit": "Mpt-steps/s", "ms_per_step": mixed["ms_per_step"],
             "step_roofline": mixed["step_roofline"],

@@ -340,6 +340,15 @@ class Solver(SolverSpec):
         L.check(ctx.lib.cles_solver_create(ctx.handle, C.byref(self.desc), C.byref(h)))
         self.handle = h
 
+    RUN_MODES = {"host_dt": 0, "device_dt": 1, "step_graph": 2}
+
+    def set_run_mode(self, mode: str):
+        """How run! issues its steps: "host_dt" (the dt decision on the host
+        every step), "device_dt" (decided on the device, read back every few
+        steps) or "step_graph" (as device_dt, each step one replay of a
+        captured graph).  Results are identical."""
+        L.check(self.ctx.lib.cles_solver_set_run_mode(self.handle, self.RUN_MODES[mode]))
+
     def memory(self) -> int:
         """Device bytes held by this solver (cles_solver_memory)."""
         b = C.c_size_t()

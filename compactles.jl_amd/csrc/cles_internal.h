@@ -119,6 +119,8 @@ enum EpiKind {
     EPI_ART_MUBETA,     // rs = rho max(v,0); mu = C0 rs; beta = C1 rs
     EPI_ART_KAPPA,      // kappa = C0 (rho c / max(T, eps)) max(v,0)
     EPI_ART_D,          // D = C0 c max(v,0)
+    EPI_LSRK_DEV,       // EPI_LSRK with dt = *dtp, and no store where *gate == 0
+    EPI_STORE_GATE,     // EPI_STORE, and no store where *gate == 0
     EPI_NKIND
 };
 
@@ -133,6 +135,8 @@ struct EpiArgs {
     double *Q;          // LSRK
     double c0, c1;      // coefficients (art) or A, B (LSRK)
     double dt;          // LSRK
+    const double *dtp;  // EPI_LSRK_DEV: dt in device memory
+    const int *gate;    // EPI_LSRK_DEV, EPI_STORE_GATE: store only when *gate != 0
 };
 
 // Launch one line operator along `dim` (1..3) over the view.

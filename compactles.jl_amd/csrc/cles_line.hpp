@@ -39,19 +39,22 @@ template <> __device__ __forceinline__ const float *rowp<float>(const LineDev &p
 template <typename T, int EPI>
 __device__ __forceinline__ void epilogue(const EpiArgs &e, int64_t o, double v)
 {
-    if constexpr (EPI == EPI_STORE) {
+    if constexpr (EPI == EPI_STORE_GATE || EPI == EPI_LSRK_DEV)
+        if (!*e.gate) return;
+    if constexpr (EPI == EPI_STORE || EPI == EPI_STORE_GATE) {
         ((T *)e.out)[o] = (T)v;
     } else if constexpr (EPI == EPI_STORE_NEG) {
         ((T *)e.out)[o] = (T)(-v);
     } else if constexpr (EPI == EPI_SUB) {
         T *out = (T *)e.out;
         out[o] = out[o] - (T)v;
-    } else if constexpr (EPI == EPI_LSRK) {
+    } else if constexpr (EPI == EPI_LSRK || EPI == EPI_LSRK_DEV) {
         // rhs.jl:665-667 dQ -= D_z F_z, then timestep.jl:85-94 on the FP64 state
         // acc == NULL: this is the only active dimension, dQ = 0 - v
         const T *acc = (const T *)e.a0;
         const double dq = acc ? (double)(acc[o] - (T)v) : -v;
-        double nv = e.c0 * e.du[o] + e.dt * dq;
+        const double dt = EPI == EPI_LSRK_DEV ? *e.dtp : e.dt;
+        double nv = e.c0 * e.du[o] + dt * dq;
         e.du[o] = nv;
         e.Q[o] += e.c1 * nv;
     } else if constexpr (EPI == EPI_ART_MUBETA) {
@@ -84,7 +87,9 @@ template <typename T, int EPI, int NB>
 __device__ __forceinline__ void epilogue_batch(const EpiArgs &e, const int64_t (&o)[NB], const T (&v)[NB],
                                                const bool (&ok)[NB])
 {
-    if constexpr (EPI == EPI_STORE || EPI == EPI_STORE_NEG) {
+    if constexpr (EPI == EPI_STORE_GATE || EPI == EPI_LSRK_DEV)
+        if (!*e.gate) return;
+    if constexpr (EPI == EPI_STORE || EPI == EPI_STORE_NEG || EPI == EPI_STORE_GATE) {
 #pragma unroll
         for (int i = 0; i < NB; i++)
             if (ok[i]) epilogue<T, EPI>(e, o[i], (double)v[i]);
@@ -96,7 +101,7 @@ __device__ __forceinline__ void epilogue_batch(const EpiArgs &e, const int64_t (
 #pragma unroll
         for (int i = 0; i < NB; i++)
             if (ok[i]) out[o[i]] = a[i] - v[i];
-    } else if constexpr (EPI == EPI_LSRK) {
+    } else if constexpr (EPI == EPI_LSRK || EPI == EPI_LSRK_DEV) {
         const T *acc = (const T *)e.a0;
         T a[NB];
         double du[NB], q[NB];
@@ -106,11 +111,12 @@ __device__ __forceinline__ void epilogue_batch(const EpiArgs &e, const int64_t (
             du[i] = ok[i] ? e.du[o[i]] : 0.0;
             q[i] = ok[i] ? e.Q[o[i]] : 0.0;
         }
+        const double dt = EPI == EPI_LSRK_DEV ? *e.dtp : e.dt;
 #pragma unroll
         for (int i = 0; i < NB; i++) {
             if (!ok[i]) continue;
             const double dq = acc ? (double)(a[i] - v[i]) : -(double)v[i];
-            const double nv = e.c0 * du[i] + e.dt * dq;
+            const double nv = e.c0 * du[i] + dt * dq;
             e.du[o[i]] = nv;
             e.Q[o[i]] = q[i] + e.c1 * nv;
         }
@@ -922,6 +928,7 @@ void launch_epi(const HostPlan &hp, int dim, const View3 &v, const T *f, const E
     if constexpr (SYM) {
         switch (e.kind) {
         case EPI_STORE: return launch_variant<T, Q, M, SYM, EPI_STORE>(hp, dim, v, f, e, st);
+        case EPI_STORE_GATE: return launch_variant<T, Q, M, SYM, EPI_STORE_GATE>(hp, dim, v, f, e, st);
         case EPI_ART_MUBETA: return launch_variant<T, Q, M, SYM, EPI_ART_MUBETA>(hp, dim, v, f, e, st);
         case EPI_ART_KAPPA: return launch_variant<T, Q, M, SYM, EPI_ART_KAPPA>(hp, dim, v, f, e, st);
         case EPI_ART_D: return launch_variant<T, Q, M, SYM, EPI_ART_D>(hp, dim, v, f, e, st);
@@ -933,6 +940,7 @@ void launch_epi(const HostPlan &hp, int dim, const View3 &v, const T *f, const E
         case EPI_STORE_NEG: return launch_variant<T, Q, M, SYM, EPI_STORE_NEG>(hp, dim, v, f, e, st);
         case EPI_SUB: return launch_variant<T, Q, M, SYM, EPI_SUB>(hp, dim, v, f, e, st);
         case EPI_LSRK: return launch_variant<T, Q, M, SYM, EPI_LSRK>(hp, dim, v, f, e, st);
+        case EPI_LSRK_DEV: return launch_variant<T, Q, M, SYM, EPI_LSRK_DEV>(hp, dim, v, f, e, st);
         default: break;
         }
     }

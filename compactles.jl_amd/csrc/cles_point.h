@@ -139,7 +139,8 @@ void launch_dilatation_beta(const W *rho, const W *sensor, const W *sw, W *beta,
                             hipStream_t st);
 template <typename W> void launch_fluxes(const FluxArgs<W> &a, int64_t N, hipStream_t st);
 template <typename W>
-void launch_lsrk(double *Q, double *du, const W *dQ, double A, double B, double dt, int64_t NC, hipStream_t st);
+void launch_lsrk(double *Q, double *du, const W *dQ, double A, double B, double dt, const double *dtp,
+                 const int *gate, int64_t NC, hipStream_t st);
 template <typename W>
 void launch_rate(const RateArgs<W> &a, double *part, int nb, double *out, int64_t N, hipStream_t st);
 template <typename W>

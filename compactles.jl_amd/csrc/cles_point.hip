@@ -359,10 +359,13 @@ __global__ void __launch_bounds__(TPB) k_fluxes(FluxArgs<W> a, int64_t N)
 // timestep.jl:85-94 (standalone form, used when the RHS needs boundary corrections)
 template <typename W>
 __global__ void __launch_bounds__(TPB) k_lsrk(double *__restrict__ Q, double *__restrict__ du,
-                                              const W *__restrict__ dQ, double A, double B, double dt, int64_t NC)
+                                              const W *__restrict__ dQ, double A, double B, double dt,
+                                              const double *__restrict__ dtp, const int *__restrict__ gate, int64_t NC)
 {
     const int64_t I = (int64_t)blockIdx.x * TPB + threadIdx.x;
     if (I >= NC) return;
+    if (gate && !*gate) return;
+    if (dtp) dt = *dtp;
     const double v = A * du[I] + dt * (double)dQ[I];
     du[I] = v;
     Q[I] += B * v;
@@ -725,8 +728,7 @@ __global__ void __launch_bounds__(TPB) k_batch_dt(BatchCtl c)
 {
     const int64_t i = (int64_t)blockIdx.x * TPB + threadIdx.x;
     if (i >= c.count) return;
-    if (!c.active[i]) {
-        c.dt[i] = 0.0;
+    if (!c.active[i]) {   // dt keeps its last value (a rejected candidate stays readable)
         c.stepped[i] = 0;
         return;
     }
@@ -746,7 +748,7 @@ __global__ void __launch_bounds__(TPB) k_batch_dt(BatchCtl c)
         c.status[i] = code;
         c.active[i] = 0;
         c.stepped[i] = 0;
-        c.dt[i] = 0.0;
+        c.dt[i] = dt;   // the rejected candidate (nothing steps with it)
         return;
     }
     c.dt_seen[i] = j_max(seen, dt);
@@ -930,9 +932,10 @@ void launch_fluxes(const FluxArgs<W> &a, int64_t N, hipStream_t st)
 }
 
 template <typename W>
-void launch_lsrk(double *Q, double *du, const W *dQ, double A, double B, double dt, int64_t NC, hipStream_t st)
+void launch_lsrk(double *Q, double *du, const W *dQ, double A, double B, double dt, const double *dtp,
+                 const int *gate, int64_t NC, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_lsrk<W>, dim3(nblocks(NC)), dim3(TPB), 0, st, Q, du, dQ, A, B, dt, NC);
+    hipLaunchKernelGGL(k_lsrk<W>, dim3(nblocks(NC)), dim3(TPB), 0, st, Q, du, dQ, A, B, dt, dtp, gate, NC);
     CHECK_LAUNCH();
 }
 
@@ -1075,8 +1078,8 @@ void launch_convert(Dst *d, const Src *s, int64_t N, hipStream_t st)
     template void launch_dilatation_beta<W>(const W *, const W *, const W *, W *, double, int64_t,         \
                                             hipStream_t);                                                  \
     template void launch_fluxes<W>(const FluxArgs<W> &, int64_t, hipStream_t);                             \
-    template void launch_lsrk<W>(double *, double *, const W *, double, double, double, int64_t,           \
-                                 hipStream_t);                                                             \
+    template void launch_lsrk<W>(double *, double *, const W *, double, double, double, const double *,   \
+                                 const int *, int64_t, hipStream_t);                                       \
     template void launch_rate<W>(const RateArgs<W> &, double *, int, double *, int64_t, hipStream_t);      \
     template void launch_integral<W>(const DiagArgs<W> &, int, double *, int, double, double *, hipStream_t);
 CLES_INST(double)

@@ -59,6 +59,23 @@ struct cles_solver {
     double t = 0, tstage = 0, dt_prev = 0, rate_prev = 0;
     int64_t step = 0;
 
+    // run! with the dt decision in device memory and each iteration replayed
+    // from a captured graph (timestep.jl:345-399 without a host round trip)
+    struct Ctl {   // device control block of one instance
+        double t, dt, dt_prev, rate_prev, dt_seen, tfinal, pad[2];
+        int64_t step, nmax, step0, pad64;
+        int status, active, stepped, filt, pad32[4];
+    };
+    Ctl *ctl = nullptr, *host_ctl = nullptr;   // device / pinned mirror
+    double *dts_buf = nullptr;                 // dt history of a run (capacity dts_cap)
+    int64_t dts_cap = 0;
+    int run_mode = 0;                          // CLES_RUN_HOST_DT, _DEVICE_DT or _STEP_GRAPH
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    // set while the iteration is captured: LSRK dt and gates in device memory
+    const double *dev_dt = nullptr;
+    const int *dev_gate = nullptr, *dev_filt = nullptr;
+
     // per-phase event timing (cles_solver_profile)
     struct Phase {
         double ms = 0, bytes = 0;
@@ -135,8 +152,12 @@ struct cles_solver {
         if (ctx && ctx->stream) (void)hipStreamSynchronize(ctx->stream);
         for (auto &pd : pending) { evpool.push_back(pd.a); evpool.push_back(pd.b); }
         for (hipEvent_t e : evpool) (void)hipEventDestroy(e);
+        if (gexec) (void)hipGraphExecDestroy(gexec);
+        if (graph) (void)hipGraphDestroy(graph);
+        if (dts_buf) (void)hipFree(dts_buf);
         for (void *ptr : allocs) (void)hipFree(ptr);
         if (host_scal) (void)hipHostFree(host_scal);
+        if (host_ctl) (void)hipHostFree(host_ctl);
         for (int k = 0; k < 3; k++) {
             cles::free_line_plan(dplan[k]);
             cles::free_line_plan(fplan[k]);
@@ -200,7 +221,8 @@ struct Impl {
         const double w = hp.dtype == CLES_F64 ? 8.0 : 4.0;
         switch (kind) {
         case EPI_SUB: return 3 * w;                 // read f, read-modify-write out
-        case EPI_LSRK: return 2 * w + 32;           // f, dQ; du and Q read + written
+        case EPI_LSRK:
+        case EPI_LSRK_DEV: return 2 * w + 32;       // f, dQ; du and Q read + written
         case EPI_ART_MUBETA: return 4 * w;          // sensor, rho -> mu, beta
         case EPI_ART_KAPPA: return 5 * w;           // sensor, rho, c, T -> kappa
         case EPI_ART_D: return 3 * w;               // sensor, c -> D
@@ -386,13 +408,15 @@ struct Impl {
                 const bool last = k == s.nact - 1;
                 EpiArgs e{};
                 if (fuse && last) {
-                    e.kind = EPI_LSRK;
+                    e.kind = s.dev_dt ? EPI_LSRK_DEV : EPI_LSRK;
                     e.a0 = s.nact > 1 ? (const void *)dQc(c) : nullptr;
                     e.du = s.du + (size_t)c * s.N;
                     e.Q = s.Q + (size_t)c * s.N;
                     e.c0 = A;
                     e.c1 = B;
                     e.dt = dt;
+                    e.dtp = s.dev_dt;
+                    e.gate = s.dev_gate;
                 } else {
                     e.kind = k == 0 ? EPI_STORE_NEG : EPI_SUB;
                     e.out = dQc(c);
@@ -434,7 +458,43 @@ struct Impl {
     void lsrk_all(double A, double B, double dt) const
     {
         s.timed(PH_LSRK, (32.0 + sizeof(W)) * s.C * (double)s.N,
-                [&] { launch_lsrk<W>(s.Q, s.du, f(s.dQ), A, B, dt, (int64_t)s.C * s.N, s.st()); });
+                [&] { launch_lsrk<W>(s.Q, s.du, f(s.dQ), A, B, dt, s.dev_dt, s.dev_gate, (int64_t)s.C * s.N,
+                                     s.st()); });
+    }
+
+    // max_rate's kernels only: the rate and the minimum density stay in
+    // s.scal[0..1] on the device
+    void rate_device() const
+    {
+        primitives();
+        RateArgs<W> a{};
+        a.Q = s.Q;
+        a.Ns = s.Ns;
+        for (int k = 0; k < 3; k++) {
+            a.active[k] = s.active[k];
+            a.invh[k] = 1.0 / s.h[k];
+        }
+        a.mu0 = s.d.mu0;
+        a.Pr = s.d.Pr;
+        a.Sc = s.d.Sc;
+        a.rho = f(s.rho); a.u = f(s.u); a.v = f(s.v); a.w = f(s.w); a.c = f(s.c); a.cp = f(s.cp);
+        a.mu_art = f(s.mu_art);
+        a.beta_art = f(s.beta_art);
+        a.kappa_art = f(s.kappa_art);
+        for (int k = 0; k < s.Ns; k++) a.D_art[k] = f(s.D_art[k]);
+        int nb = (int)std::min<int64_t>(s.nred, (s.N + 255) / 256);
+        launch_rate<W>(a, s.part, nb, s.scal, s.N, s.st());
+    }
+
+    // one run! iteration with every decision on the device (the captured graph)
+    void iteration_device(const BatchCtl &c) const
+    {
+        apply_bcs();
+        rate_device();
+        launch_batch_dt(c, s.st());
+        step(0.0, true);   // dt and the update gate come from the control block
+        launch_batch_post(c, s.st());
+        if (s.d.filter_interval > 0) filter_state();
     }
 
     // timestep.jl:72-99 step!
@@ -550,8 +610,9 @@ struct Impl {
             const int d = s.adim[k];
             for (int c = 0; c < s.C; c++) {
                 EpiArgs e{};
-                e.kind = EPI_STORE;
+                e.kind = s.dev_filt ? EPI_STORE_GATE : EPI_STORE;
                 e.out = s.Q + (size_t)c * s.N;
+                e.gate = s.dev_filt;
                 s.timed(PH_FILTER_STATE + d, 16.0 * (double)s.N,
                         [&] { launch_line(s.fplanQ[d], d + 1, v, s.Q + (size_t)c * s.N, e, s.st()); });
             }
@@ -780,6 +841,8 @@ int cles_solver_create(cles_ctx *ctx, const cles_solver_desc *desc, cles_solver 
             s->part = (double *)s->alloc((size_t)s->nred * 2 * sizeof(double));
             s->scal = (double *)s->alloc(16 * sizeof(double));
             CLES_HIP(hipHostMalloc((void **)&s->host_scal, 16 * sizeof(double), hipHostMallocDefault));
+            s->ctl = (cles_solver::Ctl *)s->alloc(sizeof(cles_solver::Ctl));
+            CLES_HIP(hipHostMalloc((void **)&s->host_ctl, sizeof(cles_solver::Ctl), hipHostMallocDefault));
             CLES_HIP(hipStreamSynchronize(ctx->stream));
         } catch (...) {
             delete s;
@@ -962,12 +1025,184 @@ int cles_filter_state(cles_solver *s)
 }
 
 // timestep.jl:277-400 run! (no callbacks, no retries)
+namespace {
+
+cles::BatchCtl device_ctl(cles_solver *s)
+{
+    cles::BatchCtl c{};
+    cles_solver::Ctl *k = s->ctl;
+    c.count = 1;
+    c.rate = s->scal;
+    c.rhomin = s->scal + 1;
+    c.tfinal = &k->tfinal;
+    c.t = &k->t;
+    c.dt = &k->dt;
+    c.dt_prev = &k->dt_prev;
+    c.rate_prev = &k->rate_prev;
+    c.dt_seen = &k->dt_seen;
+    c.step = &k->step;
+    c.nmax = &k->nmax;
+    c.status = &k->status;
+    c.active = &k->active;
+    c.stepped = &k->stepped;
+    c.filt = &k->filt;
+    c.cfl = s->d.cfl;
+    c.predict = s->d.predict;
+    c.max_growth = s->d.max_growth;
+    c.dt_min = s->d.dt_min;
+    c.dt_min_ratio = s->d.dt_min_ratio;
+    c.filter_interval = s->d.filter_interval;
+    c.dts = s->dts_buf;
+    c.step0 = &k->step0;
+    c.nrec = s->dts_cap;
+    return c;
+}
+
+void drop_graph(cles_solver *s)
+{
+    if (s->gexec) (void)hipGraphExecDestroy(s->gexec);
+    if (s->graph) (void)hipGraphDestroy(s->graph);
+    s->gexec = nullptr;
+    s->graph = nullptr;
+}
+
+// capture one device-decided run! iteration into s->gexec
+void capture_iteration(cles_solver *s)
+{
+    drop_graph(s);
+    const cles::BatchCtl c = device_ctl(s);
+    s->dev_dt = &s->ctl->dt;
+    s->dev_gate = &s->ctl->stepped;
+    s->dev_filt = &s->ctl->filt;
+    hipError_t cap = hipStreamBeginCapture(s->st(), hipStreamCaptureModeThreadLocal);
+    if (cap != hipSuccess) {
+        s->dev_dt = nullptr;
+        s->dev_gate = s->dev_filt = nullptr;
+        throw cles::Error{CLES_ERR_HIP, std::string("hipStreamBeginCapture: ") + hipGetErrorString(cap)};
+    }
+    std::string err;
+    try {
+        dispatch(s, [&](auto &im) { im.iteration_device(c); });
+    } catch (const cles::Error &e) {
+        err = e.msg;
+    }
+    hipGraph_t g = nullptr;
+    hipError_t end = hipStreamEndCapture(s->st(), &g);
+    s->dev_dt = nullptr;
+    s->dev_gate = s->dev_filt = nullptr;
+    if (!err.empty() || end != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        throw cles::Error{CLES_ERR_HIP, err.empty() ? std::string("hipStreamEndCapture: ") + hipGetErrorString(end) : err};
+    }
+    s->graph = g;
+    hipError_t inst = hipGraphInstantiate(&s->gexec, g, nullptr, nullptr, 0);
+    if (inst != hipSuccess) {
+        drop_graph(s);
+        throw cles::Error{CLES_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(inst)};
+    }
+}
+
+// timestep.jl:277-400 with every per-step decision on the device: the host
+// replays the captured iteration and reads the control block back every few
+// steps (fewer near tfinal or nmax); iterations after the run has stopped
+// change nothing (their updates are gated off).
+int run_graph(cles_solver *s, double tfinal, int64_t nmax, double *dts, int64_t ndts, std::string &msg, bool graphs)
+{
+    if (!(s->t < tfinal && s->step < nmax)) return CLES_OK;
+    if (ndts > s->dts_cap) {
+        drop_graph(s);
+        if (s->dts_buf) CLES_HIP(hipFree(s->dts_buf));
+        s->dts_buf = nullptr;
+        s->dts_cap = 0;
+        CLES_HIP(hipMalloc((void **)&s->dts_buf, (size_t)ndts * sizeof(double)));
+        s->dts_cap = ndts;
+    }
+    if (graphs && !s->gexec) capture_iteration(s);
+    cles_solver::Ctl h{};
+    h.t = s->t;
+    h.dt_prev = s->dt_prev;
+    h.rate_prev = s->rate_prev;
+    h.tfinal = tfinal;
+    h.step = s->step;
+    h.nmax = nmax;
+    h.step0 = s->step;
+    h.active = 1;
+    *s->host_ctl = h;
+    // fresh workspace (timestep.jl:402-405)
+    CLES_HIP(hipMemsetAsync(s->du, 0, (size_t)s->N * s->C * sizeof(double), s->st()));
+    CLES_HIP(hipMemsetAsync(s->dQ, 0, (size_t)s->N * s->C * s->wsize(), s->st()));
+    CLES_HIP(hipMemcpyAsync(s->ctl, s->host_ctl, sizeof(h), hipMemcpyHostToDevice, s->st()));
+    CLES_HIP(hipStreamSynchronize(s->st()));
+    for (;;) {
+        const cles_solver::Ctl &c = *s->host_ctl;
+        int64_t k = std::min<int64_t>(16, nmax - c.step);
+        if (c.dt_prev > 0) {
+            const double est = std::ceil((tfinal - c.t) / c.dt_prev);
+            if (est >= 1 && est < (double)k) k = (int64_t)est;
+        }
+        if (k < 1) k = 1;
+        if (graphs) {
+            for (int64_t i = 0; i < k; i++) CLES_HIP(hipGraphLaunch(s->gexec, s->st()));
+        } else {
+            const cles::BatchCtl bc = device_ctl(s);
+            s->dev_dt = &s->ctl->dt;
+            s->dev_gate = &s->ctl->stepped;
+            s->dev_filt = &s->ctl->filt;
+            try {
+                for (int64_t i = 0; i < k; i++) dispatch(s, [&](auto &im) { im.iteration_device(bc); });
+            } catch (...) {
+                s->dev_dt = nullptr;
+                s->dev_gate = s->dev_filt = nullptr;
+                throw;
+            }
+            s->dev_dt = nullptr;
+            s->dev_gate = s->dev_filt = nullptr;
+        }
+        CLES_HIP(hipMemcpyAsync(s->host_ctl, s->ctl, sizeof(cles_solver::Ctl), hipMemcpyDeviceToHost, s->st()));
+        CLES_HIP(hipStreamSynchronize(s->st()));
+        if (!s->host_ctl->active) break;
+    }
+    const cles_solver::Ctl &c = *s->host_ctl;
+    const int64_t taken = c.step - h.step0;
+    if (ndts > 0 && taken > 0)
+        CLES_HIP(hipMemcpy(dts, s->dts_buf, (size_t)std::min(taken, ndts) * sizeof(double), hipMemcpyDeviceToHost));
+    s->t = c.t;
+    s->step = c.step;
+    s->dt_prev = c.dt_prev;
+    s->rate_prev = c.rate_prev;
+    if (c.status != CLES_OK) {
+        // the message of check_step (stepcontrol.jl:184-204) for the rejected dt
+        CLES_HIP(hipMemcpy(s->host_scal, s->scal, 2 * sizeof(double), hipMemcpyDeviceToHost));
+        std::string m;
+        const int rc = check_step(s->d, c.dt, s->host_scal[1], c.dt_seen, m);
+        msg = rc == c.status ? m : std::string("run! stopped by check_step (status ") + std::to_string(c.status) + ")";
+        return c.status;
+    }
+    return CLES_OK;
+}
+
+}  // namespace
+
+int cles_solver_set_run_mode(cles_solver *s, int mode)
+{
+    CLES_TRY({
+        CLES_REQUIRE(s, "NULL argument");
+        CLES_REQUIRE(mode >= CLES_RUN_HOST_DT && mode <= CLES_RUN_STEP_GRAPH, "unknown run mode");
+        s->run_mode = mode;
+    })
+}
+
 int cles_run(cles_solver *s, double tfinal, int64_t nmax, double *dts, int64_t ndts)
 {
     CLES_TRY({
         CLES_REQUIRE(s, "NULL argument");
         int rc = CLES_OK;
         std::string msg;
+        if (s->run_mode != CLES_RUN_HOST_DT && !s->prof) {
+            rc = run_graph(s, tfinal, nmax, dts, ndts, msg, s->run_mode == CLES_RUN_STEP_GRAPH);
+            if (rc != CLES_OK) throw cles::Error{rc, msg};
+            return CLES_OK;
+        }
         dispatch(s, [&](auto &im) {
             // fresh Workspace (timestep.jl:402-405)
             CLES_HIP(hipMemsetAsync(s->du, 0, (size_t)s->N * s->C * sizeof(double), s->st()));

@@ -229,6 +229,17 @@ int cles_solver_profile(cles_solver *solver, int enable);
 int cles_solver_profile_read(cles_solver *solver, int index, char *name, int namelen,
                              double *ms_total, int64_t *launches, double *bytes_total);
 
+/* How run! issues its steps.  CLES_RUN_HOST_DT: every kernel launched from
+ * the host and the dt decision of predicted_dt / check_step / the tfinal clip
+ * taken on the host each step (one host round trip per step).
+ * CLES_RUN_DEVICE_DT: the same kernels, with that decision taken on the
+ * device; the host reads it back every few steps.  CLES_RUN_STEP_GRAPH: as
+ * CLES_RUN_DEVICE_DT, each step issued as one replay of a captured graph of
+ * the whole iteration.  The results are identical in every mode; the default
+ * is CLES_RUN_HOST_DT, which profiling (cles_solver_profile) always uses. */
+enum { CLES_RUN_HOST_DT = 0, CLES_RUN_DEVICE_DT = 1, CLES_RUN_STEP_GRAPH = 2 };
+int cles_solver_set_run_mode(cles_solver *solver, int mode);
+
 /* Device-resident state geometry: interior extents and per-field element
  * count (the solver stores unpadded column-major fields). */
 int cles_solver_shape(cles_solver *solver, int64_t n[3], int *n_cons);
