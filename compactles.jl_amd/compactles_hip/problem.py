@@ -340,6 +340,15 @@ class Solver(SolverSpec):
         L.check(ctx.lib.cles_solver_create(ctx.handle, C.byref(self.desc), C.byref(h)))
         self.handle = h
 
+    def save_checkpoint(self, prefix: str, rank: int = 0) -> str:
+        """io.jl:15-41 save_checkpoint: writes prefix.rNNNN.ckpt."""
+        L.check(self.ctx.lib.cles_solver_save_checkpoint(self.handle, prefix.encode(), int(rank)))
+        return prefix
+
+    def load_checkpoint(self, prefix: str, rank: int = 0):
+        """io.jl:49-75 load_checkpoint!: restores Q, t and step."""
+        L.check(self.ctx.lib.cles_solver_load_checkpoint(self.handle, prefix.encode(), int(rank)))
+
     RUN_MODES = {"host_dt": 0, "device_dt": 1, "step_graph": 2}
 
     def set_run_mode(self, mode: str):

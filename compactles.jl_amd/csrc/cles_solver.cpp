@@ -12,6 +12,7 @@
 //    LSRK update (timestep.jl:85-94) without storing the final dQ.
 // Periodic halos are index wraps inside the kernels (halo.jl:27-36).
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -703,6 +704,79 @@ void validate_desc(const cles_solver_desc &d)
 }  // namespace
 
 extern "C" {
+
+// io.jl:6-75 save_checkpoint / load_checkpoint! byte format, one rank:
+// magic, n_cons, n_global[3], n_local[3], coords[3] (Int64), t (Float64),
+// step (Int64), then the interior of Q as Float64 (x fastest, components
+// last) -- the device layout of Q.
+namespace {
+constexpr uint64_t CKPT_MAGIC = 0x434c4553434b5054ULL;   // "CLES CKPT"
+
+std::string ckpt_name(const char *prefix, int rank)
+{
+    char r[16];
+    std::snprintf(r, sizeof r, ".r%04d.ckpt", rank);
+    return std::string(prefix) + r;
+}
+}  // namespace
+
+int cles_solver_save_checkpoint(cles_solver *s, const char *prefix, int rank)
+{
+    CLES_TRY({
+        CLES_REQUIRE(s && prefix, "NULL argument");
+        CLES_REQUIRE(rank >= 0, "rank must be >= 0");
+        std::vector<double> buf((size_t)s->N * s->C);
+        CLES_HIP(hipMemcpyAsync(buf.data(), s->Q, buf.size() * sizeof(double), hipMemcpyDeviceToHost, s->st()));
+        CLES_HIP(hipStreamSynchronize(s->st()));
+        const std::string name = ckpt_name(prefix, rank);
+        FILE *fp = std::fopen(name.c_str(), "wb");
+        if (!fp) throw cles::Error{CLES_ERR_INVALID, "cannot open " + name + " for writing"};
+        int64_t hdr[10] = {s->C, s->n[0], s->n[1], s->n[2], s->n[0], s->n[1], s->n[2], 0, 0, 0};
+        const int64_t step = s->step;
+        bool ok = std::fwrite(&CKPT_MAGIC, 8, 1, fp) == 1 && std::fwrite(hdr, 8, 10, fp) == 10 &&
+                  std::fwrite(&s->t, 8, 1, fp) == 1 && std::fwrite(&step, 8, 1, fp) == 1 &&
+                  std::fwrite(buf.data(), sizeof(double), buf.size(), fp) == buf.size();
+        ok = (std::fclose(fp) == 0) && ok;
+        if (!ok) throw cles::Error{CLES_ERR_INVALID, "short write to " + name};
+    })
+}
+
+int cles_solver_load_checkpoint(cles_solver *s, const char *prefix, int rank)
+{
+    CLES_TRY({
+        CLES_REQUIRE(s && prefix, "NULL argument");
+        const std::string name = ckpt_name(prefix, rank);
+        FILE *fp = std::fopen(name.c_str(), "rb");
+        if (!fp) throw cles::Error{CLES_ERR_INVALID, "cannot open " + name};
+        struct Closer {
+            FILE *f;
+            ~Closer() { std::fclose(f); }
+        } closer{fp};
+        uint64_t magic = 0;
+        int64_t hdr[10];
+        double t = 0;
+        int64_t step = 0;
+        if (std::fread(&magic, 8, 1, fp) != 1 || magic != CKPT_MAGIC)
+            throw cles::Error{CLES_ERR_INVALID, "not a CompactLES checkpoint"};
+        if (std::fread(hdr, 8, 10, fp) != 10) throw cles::Error{CLES_ERR_INVALID, "truncated checkpoint header"};
+        if (hdr[0] != s->C) throw cles::Error{CLES_ERR_INVALID, "conserved layout mismatch"};
+        for (int d = 0; d < 3; d++)
+            if (hdr[1 + d] != s->n[d]) throw cles::Error{CLES_ERR_INVALID, "global grid mismatch"};
+        for (int d = 0; d < 3; d++)
+            if (hdr[4 + d] != s->n[d]) throw cles::Error{CLES_ERR_INVALID, "decomposition mismatch"};
+        for (int d = 0; d < 3; d++)
+            if (hdr[7 + d] != 0) throw cles::Error{CLES_ERR_INVALID, "rank layout mismatch"};
+        if (std::fread(&t, 8, 1, fp) != 1 || std::fread(&step, 8, 1, fp) != 1)
+            throw cles::Error{CLES_ERR_INVALID, "truncated checkpoint header"};
+        std::vector<double> buf((size_t)s->N * s->C);
+        if (std::fread(buf.data(), sizeof(double), buf.size(), fp) != buf.size())
+            throw cles::Error{CLES_ERR_INVALID, "truncated checkpoint data"};
+        CLES_HIP(hipMemcpyAsync(s->Q, buf.data(), buf.size() * sizeof(double), hipMemcpyHostToDevice, s->st()));
+        CLES_HIP(hipStreamSynchronize(s->st()));
+        s->t = t;
+        s->step = step;
+    })
+}
 
 int cles_solver_footprint(const cles_solver_desc *desc, size_t *bytes)
 {

@@ -174,6 +174,17 @@ typedef struct {
     int precision;                         /* CLES_F64, or CLES_F32 = mixed form */
 } cles_solver_desc;
 
+/* save_checkpoint / load_checkpoint! (src/io.jl:6-75): `prefix`.rNNNN.ckpt
+ * with the reference's header (magic, n_cons, n_global, n_local, rank
+ * coordinates, t, step; Int64 / Float64 little-endian) and the interior of Q
+ * (Float64, x fastest, components last).  One rank: n_local = n_global and
+ * coordinates (0, 0, 0).  Loading checks the header as the reference does
+ * ("not a CompactLES checkpoint", "conserved layout mismatch", "global grid
+ * mismatch", "decomposition mismatch", "rank layout mismatch") and restores
+ * Q, t and step. */
+int cles_solver_save_checkpoint(cles_solver *solver, const char *prefix, int rank);
+int cles_solver_load_checkpoint(cles_solver *solver, const char *prefix, int rank);
+
 /* Device bytes cles_solver_create would allocate for `desc` (an upper bound:
  * every field of the allocation table plus an allowance for the line-plan
  * tables); no device is touched.  The admission measure of cles_server. */
