@@ -199,7 +199,7 @@ def measure_tubes(ch, ctx, count, steps, warmup, precision):
                         "Dirichlet ends, art on, cfl 0.4, filter every step, per-instance dt"}
 
 
-def measure_serving(ch, count=48, rate_per_s=16.0, seed=20261202, budget=64 << 30, runners=4):
+def measure_serving(ch, count=32, rate_per_s=1.5, seed=20261202, budget=64 << 30, runners=4):
     """The serving form: `count` requests with Poisson arrivals (rate
     `rate_per_s`, seeded) mixing Taylor-Green cubes of 32^3..128^3 to t = 0.05
     and two-gas shock / interface grids of 256^2..1024x512 to t = 0.03, admitted
