@@ -72,3 +72,25 @@ def test_fp32_shock_interface(ctx):
     print(f"\nFP32-mixed shock/interface 96x32 40 steps vs FP64 oracle: KE {e_ke:.2e}  Q {e_q:.2e}")
     assert e_ke <= 1e-5
     assert e_q <= 1e-4
+
+
+@pytest.mark.parametrize("walls", [False, True])
+def test_fp32_256_point_x_lines(ctx, walls):
+    """The FP32 instances of the 256-point x-sweep kernel inside RHS, step
+    and filter_state, against the FP64 oracle."""
+    n, L = (256, 16, 12), (TWO_PI, 0.5, 0.4)
+    per = (not walls, True, True)
+    X, Y, Z = cases.grid(n, L, periodic=per)
+    ky, kz = 2 * np.pi / L[1], 2 * np.pi / L[2]
+    prim = (0.2 * np.sin(X) * np.cos(ky * Y) + (0.1 * np.sin(0.5 * X) ** 2 if walls else 0),
+            0.05 * np.cos(2 * X + ky * Y), 0.04 * np.sin(X + kz * Z),
+            1.0 + 0.1 * np.cos(X) * np.cos(kz * Z), 1.0 + 0.05 * np.sin(ky * Y + X))
+    q = cases.conserved(prim)
+    bcs = ((1, 1), (0, 0), (0, 0)) if walls else None
+    os_ = cases.oracle_solver(n, L, q, mu0=1e-3, art=True, bcs=bcs)
+    gs = cases.gpu_solver(ctx, n, L, q, mu0=1e-3, art=True, bcs=bcs, precision="fp32")
+    os_.run(1e9, 5)
+    gs.run(1e9, 5)
+    e_q = _state_err(gs, os_)
+    print(f"\nFP32-mixed 256x16x12 {'walls' if walls else 'periodic'}, 5 steps vs FP64 oracle: Q {e_q:.2e}")
+    assert e_q <= 1e-5

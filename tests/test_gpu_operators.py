@@ -74,8 +74,8 @@ def test_unpadded_layout_equals_padded(ctx, name):
 
 @pytest.mark.parametrize("name", SCHEMES)
 @pytest.mark.parametrize("periodic", [True, False])
-def test_fp32_against_fp64_oracle(ctx, name, periodic):
-    n = (64, 48, 40)
+@pytest.mark.parametrize("n", [(64, 48, 40), (256, 12, 10), (24, 20, 16)])
+def test_fp32_against_fp64_oracle(ctx, name, periodic, n):
     f = Hh.random_field(n, seed=5)
     f32 = f.astype(np.float32).astype(np.float64)
     for dim in (1, 2, 3):
