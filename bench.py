@@ -199,6 +199,42 @@ def measure_tubes(ch, ctx, count, steps, warmup, precision):
                         "Dirichlet ends, art on, cfl 0.4, filter every step, per-instance dt"}
 
 
+def measure_shock_2d(ch, ctx, precision, steps, warmup, n=(1024, 512, 1)):
+    """The two-dimensional form: He|CO2 Mach-1.5 shock / perturbed interface on
+    1024 x 512 (two species, NSCBC inflow and outflow in x, periodic y, art on,
+    cfl 0.4, filter every step)."""
+    import cases
+    q, L, faces = cases.shock_interface(n)
+    eos = ch.IdealMixture([ch.IdealSpecies(f"s{k}", R, g) for k, (R, g) in enumerate(cases.HE_CO2)])
+    bcs = ((cases.nscbc_bc(ch, faces[(0, 0)]), cases.nscbc_bc(ch, faces[(0, 1)])),
+           (ch.PeriodicBC(), ch.PeriodicBC()), (ch.PeriodicBC(), ch.PeriodicBC()))
+    prob = ch.Problem(domain=tuple((0.0, L[k]) for k in range(3)), bcs=bcs, ic=lambda X, Y, Z: None, eos=eos,
+                      transport=ch.Transport(mu0=0.0))
+    num = ch.Numerics(n_global=n, cfl=0.4, precision=precision)
+    s = ch.Solver(ctx, prob, num)
+    Q = s.allocate_state()
+    sl = tuple(slice(s.halo[k], s.halo[k] + n[k]) for k in range(3))
+    for c, qc in enumerate(q):
+        Q[sl + (c,)] = qc
+    s.upload(Q)
+    s.run(1e9, warmup)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    s.run(1e9, warmup + steps)
+    ctx.synchronize()
+    wall = time.perf_counter() - t0
+    s.close()
+    npts = n[0] * n[1] * n[2]
+    ms = wall / steps * 1e3
+    # Appendix B: S = 2, D = 2, NSCBC inflow (p, rho) and outflow (p normal and transverse) line ops
+    model = alg_bytes_per_point_step(2, 2, precision, nscbc_ops=4)
+    return {"value": round(npts * steps / wall / 1e6, 3), "unit": "Mpt-steps/s", "ms_per_step": round(ms, 4),
+            "step_roofline": {"alg_bytes_per_point_step": model,
+                              "achieved_GBps": round(model * npts / (ms * 1e-3) / 1e9, 1),
+                              "frac": round(model * npts / (ms * 1e-3) / 1e9 / HBM_PEAK_GBPS, 4)},
+            "workload": f"He|CO2 shock / interface {n[0]}x{n[1]}, NSCBC in / out, art on, cfl 0.4"}
+
+
 def measure_serving(ch, count=32, rate_per_s=1.5, seed=20261202, budget=64 << 30, runners=4):
     """The serving form: `count` requests with Poisson arrivals (rate
     `rate_per_s`, seeded) mixing Taylor-Green cubes of 32^3..128^3 to t = 0.05
@@ -268,7 +304,7 @@ def main():
     ap.add_argument("--tubes", type=int, default=4096, help="instances of the batched form")
     ap.add_argument("--run-mode", choices=["step_graph", "device_dt", "host_dt"], default="host_dt",
                     help="how run! issues its steps (identical results)")
-    ap.add_argument("--only", choices=["tubes", "serving"], default=None,
+    ap.add_argument("--only", choices=["tubes", "serving", "shock2d"], default=None,
                     help="measure only the batched-tubes (FP64) or serving form and print its JSON (profiling runs)")
     ap.add_argument("--cpu-n", type=int, default=256)
     ap.add_argument("--profile-steps", type=int, default=2)
@@ -289,6 +325,9 @@ def main():
         return
     if args.only == "tubes":
         print(json.dumps(measure_tubes(ch, ctx, args.tubes, args.steps, args.warmup, args.precision)), flush=True)
+        return
+    if args.only == "shock2d":
+        print(json.dumps(measure_shock_2d(ch, ctx, args.precision, args.steps, args.warmup)), flush=True)
         return
     if args.only == "serving":
         print(json.dumps(measure_serving(ch)), flush=True)
@@ -328,6 +367,8 @@ def main():
             "steps_compared": mixed["steps_total"]}}
         out["forms"]["batched_tubes"] = {p: measure_tubes(ch, ctx, args.tubes, args.steps, args.warmup, p)
                                          for p in ("fp64", "fp32")}
+        out["forms"]["shock_interface_2d"] = {p: measure_shock_2d(ch, ctx, p, args.steps, args.warmup)
+                                              for p in ("fp64", "fp32")}
         out["forms"]["serving"] = measure_serving(ch)
     ctx.close()
     if not args.no_cpu_baseline:

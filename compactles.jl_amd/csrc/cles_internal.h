@@ -139,9 +139,23 @@ struct EpiArgs {
     const int *gate;    // EPI_LSRK_DEV, EPI_STORE_GATE: store only when *gate != 0
 };
 
-// Launch one line operator along `dim` (1..3) over the view.
+// Several independent fields through one line operator in one launch (grid
+// dimension y = field).  Field 0 is the launch's own input and epilogue; field
+// i >= 1 reads in[i] and stores through out[i] (STORE, STORE_NEG, SUB,
+// STORE_GATE), or, for the LSRK kinds, offsets the accumulator, du and Q by
+// eoff[i] elements.  Small grids gain waves per launch and every grid loses
+// the per-launch gaps.
+constexpr int LINE_MF = 12;
+struct LineFields {
+    int nf = 1;
+    const void *in[LINE_MF] = {};
+    void *out[LINE_MF] = {};
+    int64_t eoff[LINE_MF] = {};
+};
+
+// Launch one line operator along `dim` (1..3) over the view (mf: more fields).
 void launch_line(const HostPlan &hp, int dim, const View3 &v, const void *f,
-                 const EpiArgs &epi, hipStream_t st);
+                 const EpiArgs &epi, hipStream_t st, const LineFields *mf = nullptr);
 
 // timestep.jl:5-21 LSRK(5,4) coefficient tables (Carpenter & Kennedy)
 inline constexpr double LSRK_A[5] = {0.0, -567301805773.0 / 1357537059087.0, -2404267990393.0 / 2016746695238.0,

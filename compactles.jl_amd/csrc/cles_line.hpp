@@ -159,6 +159,24 @@ __device__ __forceinline__ void epilogue_batch(const EpiArgs &e, const int64_t (
     }
 }
 
+// Field of a multi-field launch (grid y): field 0 keeps the launch's own input
+// and epilogue, field i >= 1 takes its input and output from the table.
+template <typename T, int EPI>
+__device__ __forceinline__ void select_field(const LineFields &mf, const T *__restrict__ &f, EpiArgs &e)
+{
+    const int i = blockIdx.y;
+    if (i == 0) return;
+    f = (const T *)mf.in[i];
+    if constexpr (EPI == EPI_LSRK || EPI == EPI_LSRK_DEV) {
+        const int64_t o = mf.eoff[i];
+        if (e.a0) e.a0 = (const T *)e.a0 + o;
+        e.du += o;
+        e.Q += o;
+    } else {
+        e.out = mf.out[i];
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Per-thread chunk machinery
 // ---------------------------------------------------------------------------
@@ -363,8 +381,9 @@ __device__ __forceinline__ double corrected(const LineDev &p, int g, T y, const 
 // ---------------------------------------------------------------------------
 template <typename T, int Q, int M, int SYM, int EPI, bool UNI, int CL = MC>
 __global__ void __launch_bounds__(Q == 1 ? 1024 : 512) k_line_yz(LineDev p, const T *__restrict__ f, int sd, int64_t so,
-                                                  int nlane, int nxb, int Lx, EpiArgs e)
+                                                  int nlane, int nxb, int Lx, EpiArgs e, LineFields mf)
 {
+    select_field<T, EPI>(mf, f, e);
     extern __shared__ double smem[];
     const int NT = Lx * p.P;
     const int lane = UNI ? (int)(threadIdx.x & 63) : (int)(threadIdx.x % Lx);
@@ -437,8 +456,9 @@ __global__ void __launch_bounds__(Q == 1 ? 1024 : 512) k_line_yz(LineDev p, cons
 // ---------------------------------------------------------------------------
 template <typename T, int Q, int M, int SYM, int EPI, bool UNI>
 __global__ void __launch_bounds__(Q == 1 ? 1024 : 512) k_line_x(LineDev p, const T *__restrict__ f, int64_t sy, int64_t sz,
-                                                 int ny, int64_t nlines, int Lt, EpiArgs e)
+                                                 int ny, int64_t nlines, int Lt, EpiArgs e, LineFields mf)
 {
+    select_field<T, EPI>(mf, f, e);
     extern __shared__ double smem[];
     const int n = p.n;
     const int TS = n + 1;
@@ -587,8 +607,9 @@ __device__ __forceinline__ int xw_slot(int x) { return x + x / XCL; }
 
 template <typename T, int Q, int M, int SYM, int EPI, bool UNIROW>
 __global__ void __launch_bounds__(64) k_line_xw(LineDev p, const T *__restrict__ f, int64_t sy, int64_t sz, int ny,
-                                                int64_t nlines, int TS, EpiArgs e)
+                                                int64_t nlines, int TS, EpiArgs e, LineFields mf)
 {
+    select_field<T, EPI>(mf, f, e);
     extern __shared__ double smem[];
     T *tile = (T *)smem;
     const int P = p.P;
@@ -707,8 +728,9 @@ __global__ void __launch_bounds__(64) k_line_xw(LineDev p, const T *__restrict__
 // periodic wrap, and no closure code on periodic lines.
 template <typename T, int Q, int M, int SYM, int EPI, int NL, bool PER>
 __global__ void __launch_bounds__(64) k_line_xwf(LineDev p, const T *__restrict__ f, int64_t sy, int64_t sz, int ny,
-                                                 int64_t nlines, EpiArgs e)
+                                                 int64_t nlines, EpiArgs e, LineFields mf)
 {
+    select_field<T, EPI>(mf, f, e);
     constexpr int P = 64 / NL;            // chunks (lanes) per line
     constexpr int N = XCL * P;            // points per line
     constexpr int TS = N + N / XCL + 1;   // padded tile row
@@ -835,8 +857,9 @@ inline void allow_lds(K kern, size_t shm)
 
 template <typename T, int Q, int M, int SYM, int EPI>
 void launch_variant(const HostPlan &hp, int dim, const View3 &v, const T *f, const EpiArgs &e,
-                           hipStream_t st)
+                           hipStream_t st, const LineFields &mf)
 {
+    const unsigned nf = (unsigned)mf.nf;
     const LineDev &p = hp.dev;
     const int P = p.P;
     const int B = 2 * Q;
@@ -847,11 +870,11 @@ void launch_variant(const HostPlan &hp, int dim, const View3 &v, const T *f, con
         const int64_t nlines = v.n[1] * v.n[2];
         const int64_t nblk = (nlines + 3) / 4;
         if (p.periodic)
-            hipLaunchKernelGGL((k_line_xwf<T, Q, M, SYM, EPI, 4, true>), dim3((unsigned)nblk), dim3(64), 0, st, p, f,
-                               v.s[1], v.s[2], (int)v.n[1], nlines, e);
+            hipLaunchKernelGGL((k_line_xwf<T, Q, M, SYM, EPI, 4, true>), dim3((unsigned)nblk, nf), dim3(64), 0, st, p, f,
+                               v.s[1], v.s[2], (int)v.n[1], nlines, e, mf);
         else
-            hipLaunchKernelGGL((k_line_xwf<T, Q, M, SYM, EPI, 4, false>), dim3((unsigned)nblk), dim3(64), 0, st, p, f,
-                               v.s[1], v.s[2], (int)v.n[1], nlines, e);
+            hipLaunchKernelGGL((k_line_xwf<T, Q, M, SYM, EPI, 4, false>), dim3((unsigned)nblk, nf), dim3(64), 0, st, p, f,
+                               v.s[1], v.s[2], (int)v.n[1], nlines, e, mf);
     } else if (dim == 1 && P <= 64 && p.m + (p.extra ? 1 : 0) <= XCL) {
         const int64_t nlines = v.n[1] * v.n[2];
         const int nl = 64 / P;
@@ -859,11 +882,11 @@ void launch_variant(const HostPlan &hp, int dim, const View3 &v, const T *f, con
         const size_t shm = (size_t)nl * TS * sizeof(T);
         const int64_t nblk = (nlines + nl - 1) / nl;
         if (p.periodic && p.extra == 0)
-            hipLaunchKernelGGL((k_line_xw<T, Q, M, SYM, EPI, true>), dim3((unsigned)nblk), dim3(64), shm, st, p,
-                               f, v.s[1], v.s[2], (int)v.n[1], nlines, TS, e);
+            hipLaunchKernelGGL((k_line_xw<T, Q, M, SYM, EPI, true>), dim3((unsigned)nblk, nf), dim3(64), shm, st, p,
+                               f, v.s[1], v.s[2], (int)v.n[1], nlines, TS, e, mf);
         else
-            hipLaunchKernelGGL((k_line_xw<T, Q, M, SYM, EPI, false>), dim3((unsigned)nblk), dim3(64), shm, st, p,
-                               f, v.s[1], v.s[2], (int)v.n[1], nlines, TS, e);
+            hipLaunchKernelGGL((k_line_xw<T, Q, M, SYM, EPI, false>), dim3((unsigned)nblk, nf), dim3(64), shm, st, p,
+                               f, v.s[1], v.s[2], (int)v.n[1], nlines, TS, e, mf);
     } else if (dim == 1) {
         const size_t tile_row = (size_t)(p.n + 1) * sizeof(T);
         const int64_t nlines = v.n[1] * v.n[2];
@@ -876,8 +899,8 @@ void launch_variant(const HostPlan &hp, int dim, const View3 &v, const T *f, con
             auto kern = k_line_x<T, Q, M, SYM, EPI, true>;
             allow_lds(kern, shm);
             const int64_t nblk = (nlines + Lt - 1) / Lt;
-            hipLaunchKernelGGL(kern, dim3((unsigned)nblk), dim3(NT), shm, st, p, f, v.s[1], v.s[2], (int)v.n[1],
-                               nlines, Lt, e);
+            hipLaunchKernelGGL(kern, dim3((unsigned)nblk, nf), dim3(NT), shm, st, p, f, v.s[1], v.s[2], (int)v.n[1],
+                               nlines, Lt, e, mf);
         } else {
             int Lt = 1;
             while (Lt * 2 * P <= 512 && shm_of(Lt * 2) <= 48 * 1024 && Lt < 32) Lt *= 2;
@@ -888,8 +911,8 @@ void launch_variant(const HostPlan &hp, int dim, const View3 &v, const T *f, con
             auto kern = k_line_x<T, Q, M, SYM, EPI, false>;
             allow_lds(kern, shm);
             const int64_t nblk = (nlines + Lt - 1) / Lt;
-            hipLaunchKernelGGL(kern, dim3((unsigned)nblk), dim3(NT), shm, st, p, f, v.s[1], v.s[2], (int)v.n[1],
-                               nlines, Lt, e);
+            hipLaunchKernelGGL(kern, dim3((unsigned)nblk, nf), dim3(NT), shm, st, p, f, v.s[1], v.s[2], (int)v.n[1],
+                               nlines, Lt, e, mf);
         }
     } else {
         const int64_t nlane = v.n[0];
@@ -902,11 +925,11 @@ void launch_variant(const HostPlan &hp, int dim, const View3 &v, const T *f, con
             const int nxb = (int)((nlane + Lx - 1) / Lx);
             const size_t shm = (size_t)2 * B * NT * sizeof(double);
             if (p.m + (p.extra ? 1 : 0) <= 16)
-                hipLaunchKernelGGL((k_line_yz<T, Q, M, SYM, EPI, true, 16>), dim3((unsigned)(nxb * nouter)), dim3(NT),
-                                   shm, st, p, f, (int)sd, so, (int)nlane, nxb, Lx, e);
+                hipLaunchKernelGGL((k_line_yz<T, Q, M, SYM, EPI, true, 16>), dim3((unsigned)(nxb * nouter), nf), dim3(NT),
+                                   shm, st, p, f, (int)sd, so, (int)nlane, nxb, Lx, e, mf);
             else
-                hipLaunchKernelGGL((k_line_yz<T, Q, M, SYM, EPI, true>), dim3((unsigned)(nxb * nouter)), dim3(NT),
-                                   shm, st, p, f, (int)sd, so, (int)nlane, nxb, Lx, e);
+                hipLaunchKernelGGL((k_line_yz<T, Q, M, SYM, EPI, true>), dim3((unsigned)(nxb * nouter), nf), dim3(NT),
+                                   shm, st, p, f, (int)sd, so, (int)nlane, nxb, Lx, e, mf);
         } else {
             int Lx = 64;
             while (Lx * P > maxNT && Lx > 1) Lx /= 2;
@@ -915,32 +938,33 @@ void launch_variant(const HostPlan &hp, int dim, const View3 &v, const T *f, con
             const int NT = Lx * P;
             const int nxb = (int)((nlane + Lx - 1) / Lx);
             const size_t shm = (size_t)2 * B * NT * sizeof(double);
-            hipLaunchKernelGGL((k_line_yz<T, Q, M, SYM, EPI, false>), dim3((unsigned)(nxb * nouter)), dim3(NT),
-                               shm, st, p, f, (int)sd, so, (int)nlane, nxb, Lx, e);
+            hipLaunchKernelGGL((k_line_yz<T, Q, M, SYM, EPI, false>), dim3((unsigned)(nxb * nouter), nf), dim3(NT),
+                               shm, st, p, f, (int)sd, so, (int)nlane, nxb, Lx, e, mf);
         }
     }
     CLES_HIP(hipGetLastError());
 }
 
 template <typename T, int Q, int M, int SYM>
-void launch_epi(const HostPlan &hp, int dim, const View3 &v, const T *f, const EpiArgs &e, hipStream_t st)
+void launch_epi(const HostPlan &hp, int dim, const View3 &v, const T *f, const EpiArgs &e, hipStream_t st,
+                const LineFields &mf)
 {
     if constexpr (SYM) {
         switch (e.kind) {
-        case EPI_STORE: return launch_variant<T, Q, M, SYM, EPI_STORE>(hp, dim, v, f, e, st);
-        case EPI_STORE_GATE: return launch_variant<T, Q, M, SYM, EPI_STORE_GATE>(hp, dim, v, f, e, st);
-        case EPI_ART_MUBETA: return launch_variant<T, Q, M, SYM, EPI_ART_MUBETA>(hp, dim, v, f, e, st);
-        case EPI_ART_KAPPA: return launch_variant<T, Q, M, SYM, EPI_ART_KAPPA>(hp, dim, v, f, e, st);
-        case EPI_ART_D: return launch_variant<T, Q, M, SYM, EPI_ART_D>(hp, dim, v, f, e, st);
+        case EPI_STORE: return launch_variant<T, Q, M, SYM, EPI_STORE>(hp, dim, v, f, e, st, mf);
+        case EPI_STORE_GATE: return launch_variant<T, Q, M, SYM, EPI_STORE_GATE>(hp, dim, v, f, e, st, mf);
+        case EPI_ART_MUBETA: return launch_variant<T, Q, M, SYM, EPI_ART_MUBETA>(hp, dim, v, f, e, st, mf);
+        case EPI_ART_KAPPA: return launch_variant<T, Q, M, SYM, EPI_ART_KAPPA>(hp, dim, v, f, e, st, mf);
+        case EPI_ART_D: return launch_variant<T, Q, M, SYM, EPI_ART_D>(hp, dim, v, f, e, st, mf);
         default: break;
         }
     } else {
         switch (e.kind) {
-        case EPI_STORE: return launch_variant<T, Q, M, SYM, EPI_STORE>(hp, dim, v, f, e, st);
-        case EPI_STORE_NEG: return launch_variant<T, Q, M, SYM, EPI_STORE_NEG>(hp, dim, v, f, e, st);
-        case EPI_SUB: return launch_variant<T, Q, M, SYM, EPI_SUB>(hp, dim, v, f, e, st);
-        case EPI_LSRK: return launch_variant<T, Q, M, SYM, EPI_LSRK>(hp, dim, v, f, e, st);
-        case EPI_LSRK_DEV: return launch_variant<T, Q, M, SYM, EPI_LSRK_DEV>(hp, dim, v, f, e, st);
+        case EPI_STORE: return launch_variant<T, Q, M, SYM, EPI_STORE>(hp, dim, v, f, e, st, mf);
+        case EPI_STORE_NEG: return launch_variant<T, Q, M, SYM, EPI_STORE_NEG>(hp, dim, v, f, e, st, mf);
+        case EPI_SUB: return launch_variant<T, Q, M, SYM, EPI_SUB>(hp, dim, v, f, e, st, mf);
+        case EPI_LSRK: return launch_variant<T, Q, M, SYM, EPI_LSRK>(hp, dim, v, f, e, st, mf);
+        case EPI_LSRK_DEV: return launch_variant<T, Q, M, SYM, EPI_LSRK_DEV>(hp, dim, v, f, e, st, mf);
         default: break;
         }
     }

@@ -235,6 +235,22 @@ struct Impl {
         s.timed(phase + d, line_bytes(hp, e.kind) * (double)s.N,
                 [&] { launch_line(hp, d + 1, view(), in, e, s.st()); });
     }
+    // one launch over nf independent fields: field i reads in[i] and writes
+    // out[i] (store kinds) or offsets the LSRK arrays of e by eoff[i] elements
+    void lines(const HostPlan &hp, int d, int nf, const void *const *in, void *const *out, const int64_t *eoff,
+               EpiArgs e, int phase) const
+    {
+        LineFields mf;
+        mf.nf = nf;
+        for (int i = 0; i < nf; i++) {
+            mf.in[i] = in[i];
+            if (out) mf.out[i] = out[i];
+            if (eoff) mf.eoff[i] = eoff[i];
+        }
+        if (out) e.out = out[0];
+        s.timed(phase + d, line_bytes(hp, e.kind) * (double)s.N * nf,
+                [&] { launch_line(hp, d + 1, view(), in[0], e, s.st(), &mf); });
+    }
     EpiArgs store(void *out, int kind = EPI_STORE) const
     {
         EpiArgs e{};
@@ -258,15 +274,27 @@ struct Impl {
                 [&] { launch_primitives<W>(a, s.N, s.st()); });
     }
 
-    // rhs.jl:591-606 (velocity gradients of the active dimensions)
-    void velocity_gradients() const
+    // rhs.jl:591-606 velocity gradients of the active dimensions; with
+    // scalars, also rhs.jl:608-622 the temperature and species gradients,
+    // every field of one direction in one launch
+    void velocity_gradients(bool scalars = false) const
     {
-        void *vel[3] = {s.u, s.v, s.w};
-        for (int jj = 0; jj < 3; jj++)
-            for (int k = 0; k < s.nact; k++) {
-                const int d = s.adim[k];
-                line(s.dplan[d], d, vel[jj], store(s.gu[d * 3 + jj]), PH_DERIV);
+        for (int k = 0; k < s.nact; k++) {
+            const int d = s.adim[k];
+            const void *in[LINE_MF] = {s.u, s.v, s.w};
+            void *out[LINE_MF] = {s.gu[d * 3], s.gu[d * 3 + 1], s.gu[d * 3 + 2]};
+            int nf = 3;
+            if (scalars) {
+                in[nf] = s.T;
+                out[nf++] = s.gT[d];
+                if (s.Ns > 1)
+                    for (int sp = 0; sp < s.Ns; sp++) {
+                        in[nf] = s.Y[sp];
+                        out[nf++] = s.gY[d][sp];
+                    }
             }
+            lines(s.dplan[d], d, nf, in, out, nullptr, store(out[0]), PH_DERIV);
+        }
     }
 
     // artificial.jl:98-111 smooth! in place, the last pass with epilogue `last`
@@ -394,35 +422,38 @@ struct Impl {
     void rhs(bool primitives_current, bool fuse, double A, double B, double dt) const
     {
         if (!primitives_current) primitives();
-        velocity_gradients();
+        // the temperature and species gradients (rhs.jl:608-622) are
+        // independent of the artificial terms and go with the velocity ones
+        velocity_gradients(true);
         artificial();
+        fluxes();
+        // rhs.jl:624-667 per direction, every component in one launch
         for (int k = 0; k < s.nact; k++) {
             const int d = s.adim[k];
-            line(s.dplan[d], d, s.T, store(s.gT[d]), PH_DERIV);
-            if (s.Ns > 1)
-                for (int sp = 0; sp < s.Ns; sp++) line(s.dplan[d], d, s.Y[sp], store(s.gY[d][sp]), PH_DERIV);
-        }
-        fluxes();
-        for (int c = 0; c < s.C; c++) {
-            for (int k = 0; k < s.nact; k++) {
-                const int d = s.adim[k];
-                const bool last = k == s.nact - 1;
-                EpiArgs e{};
-                if (fuse && last) {
-                    e.kind = s.dev_dt ? EPI_LSRK_DEV : EPI_LSRK;
-                    e.a0 = s.nact > 1 ? (const void *)dQc(c) : nullptr;
-                    e.du = s.du + (size_t)c * s.N;
-                    e.Q = s.Q + (size_t)c * s.N;
-                    e.c0 = A;
-                    e.c1 = B;
-                    e.dt = dt;
-                    e.dtp = s.dev_dt;
-                    e.gate = s.dev_gate;
-                } else {
-                    e.kind = k == 0 ? EPI_STORE_NEG : EPI_SUB;
-                    e.out = dQc(c);
-                }
-                line(s.dplan[d], d, s.flux[d][c], e, (fuse && last) ? PH_DIV_LSRK : PH_DIV);
+            const bool last = k == s.nact - 1;
+            const void *in[LINE_MF];
+            void *out[LINE_MF];
+            int64_t eoff[LINE_MF];
+            for (int c = 0; c < s.C; c++) {
+                in[c] = s.flux[d][c];
+                out[c] = dQc(c);
+                eoff[c] = (int64_t)c * s.N;
+            }
+            EpiArgs e{};
+            if (fuse && last) {
+                e.kind = s.dev_dt ? EPI_LSRK_DEV : EPI_LSRK;
+                e.a0 = s.nact > 1 ? (const void *)dQc(0) : nullptr;
+                e.du = s.du;
+                e.Q = s.Q;
+                e.c0 = A;
+                e.c1 = B;
+                e.dt = dt;
+                e.dtp = s.dev_dt;
+                e.gate = s.dev_gate;
+                lines(s.dplan[d], d, s.C, in, nullptr, eoff, e, PH_DIV_LSRK);
+            } else {
+                e.kind = k == 0 ? EPI_STORE_NEG : EPI_SUB;
+                lines(s.dplan[d], d, s.C, in, out, nullptr, e, PH_DIV);
             }
         }
     }
@@ -606,17 +637,18 @@ struct Impl {
     // timestep.jl:414-426 filter_state! (in place, dimension by dimension, FP64)
     void filter_state() const
     {
-        View3 v = view();
         for (int k = 0; k < s.nact; k++) {
             const int d = s.adim[k];
+            const void *in[LINE_MF];
+            void *out[LINE_MF];
             for (int c = 0; c < s.C; c++) {
-                EpiArgs e{};
-                e.kind = s.dev_filt ? EPI_STORE_GATE : EPI_STORE;
-                e.out = s.Q + (size_t)c * s.N;
-                e.gate = s.dev_filt;
-                s.timed(PH_FILTER_STATE + d, 16.0 * (double)s.N,
-                        [&] { launch_line(s.fplanQ[d], d + 1, v, s.Q + (size_t)c * s.N, e, s.st()); });
+                in[c] = s.Q + (size_t)c * s.N;
+                out[c] = s.Q + (size_t)c * s.N;
             }
+            EpiArgs e{};
+            e.kind = s.dev_filt ? EPI_STORE_GATE : EPI_STORE;
+            e.gate = s.dev_filt;
+            lines(s.fplanQ[d], d, s.C, in, out, nullptr, e, PH_FILTER_STATE);
         }
     }
 

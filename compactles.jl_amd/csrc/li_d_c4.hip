@@ -3,5 +3,5 @@
 
 namespace cles {
 template void launch_epi<double, 1, 1, 0>(const HostPlan &, int, const View3 &, const double *, const EpiArgs &,
-                                            hipStream_t);
+                                            hipStream_t, const LineFields &);
 }  // namespace cles

@@ -3,5 +3,5 @@
 
 namespace cles {
 template void launch_epi<float, 2, 3, 0>(const HostPlan &, int, const View3 &, const float *, const EpiArgs &,
-                                            hipStream_t);
+                                            hipStream_t, const LineFields &);
 }  // namespace cles

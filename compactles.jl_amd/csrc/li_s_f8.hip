@@ -3,5 +3,5 @@
 
 namespace cles {
 template void launch_epi<float, 1, 4, 1>(const HostPlan &, int, const View3 &, const float *, const EpiArgs &,
-                                            hipStream_t);
+                                            hipStream_t, const LineFields &);
 }  // namespace cles
