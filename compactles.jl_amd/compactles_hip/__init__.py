@@ -11,3 +11,4 @@ from .problem import (ArtParams, DirichletBC, IdealMixture, IdealSpecies, NSCBCI
                       Transport, conserved_from_prim, initialize_, run_, setup, single_species, tanh_blend)
 from .batch import Batch, Instance  # noqa: F401
 from .server import Server  # noqa: F401
+from . import tune  # noqa: F401

@@ -119,6 +119,12 @@ SIGNATURES = {
     "cles_batch_compute_rhs": (C.c_int, [P, C.c_int]),
     "cles_batch_get_field": (C.c_int, [P, C.c_int64, C.c_char_p, DP]),
     "cles_batch_kinetic_energy": (C.c_int, [P, DP]),
+    "cles_tune_solver": (C.c_int, [P, C.POINTER(SolverDesc), C.c_int, C.POINTER(C.c_int), DP]),
+    "cles_tune_lookup": (C.c_int, [C.POINTER(SolverDesc), C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    "cles_tune_set": (C.c_int, [C.POINTER(SolverDesc), C.POINTER(C.c_int)]),
+    "cles_tune_clear": (C.c_int, []),
+    "cles_tune_save": (C.c_int, [C.c_char_p]),
+    "cles_tune_load": (C.c_int, [C.c_char_p]),
 }
 
 

@@ -166,6 +166,14 @@ inline constexpr double LSRK_B[5] = {1432997174477.0 / 9575080441755.0, 51618366
 inline constexpr double LSRK_C[5] = {0.0, 1432997174477.0 / 9575080441755.0, 2526269341429.0 / 6820363962896.0,
                        2006345519317.0 / 3224310063776.0, 2802321613138.0 / 2924317926251.0};
 
+// Chunk length (points per thread) of the solver's line plans along dim
+// (1..3): the line-tile table entry for this descriptor when one exists
+// (*tuned = true), otherwise default_line_chunk.
+int line_chunk(int dim, const cles_solver_desc &d, bool *tuned = nullptr);
+// The largest chunk length (x: <= 16, y/z: <= 32, >= 8) that still gives a
+// launch of at least 131072 threads.
+int default_line_chunk(int dim, int64_t n_line, int64_t lines);
+
 // Ragged batch of independent lines (one line per instance, each with its own
 // length and plan), laid end to end: instance i occupies [off[i], off[i]+n).
 // Work items are waves: item w = (P, start, count) covers the instances

@@ -891,6 +891,7 @@ int cles_solver_create(cles_ctx *ctx, const cles_solver_desc *desc, cles_solver 
                 s->cpk[k] = s->Rk[k] + s->cvk[k];
             }
             const int wdt = s->prec;
+            auto chunk_of = [&](int k) { return cles::line_chunk(k + 1, *desc); };
             for (int k = 0; k < 3; k++) {
                 if (!s->active[k]) continue;
                 for (cles::HostPlan *hp : {&s->dplan[k], &s->fplan[k]}) {
@@ -898,8 +899,8 @@ int cles_solver_create(cles_ctx *ctx, const cles_solver_desc *desc, cles_solver 
                     hp->dtype = wdt;
                     for (int j = 0; j < 3; j++) hp->nloc[j] = s->n[j];
                 }
-                cles::build_line_plan(s->dplan[k], desc->deriv, (int)s->n[k], s->periodic[k], s->h[k], k == 0 ? 16 : 32);
-                cles::build_line_plan(s->fplan[k], desc->filt, (int)s->n[k], s->periodic[k], s->h[k], k == 0 ? 16 : 32);
+                cles::build_line_plan(s->dplan[k], desc->deriv, (int)s->n[k], s->periodic[k], s->h[k], chunk_of(k));
+                cles::build_line_plan(s->fplan[k], desc->filt, (int)s->n[k], s->periodic[k], s->h[k], chunk_of(k));
             }
             if (s->prec == CLES_F64) {
                 for (int k = 0; k < 3; k++) s->fplanQ[k] = s->fplan[k];
@@ -910,7 +911,7 @@ int cles_solver_create(cles_ctx *ctx, const cles_solver_desc *desc, cles_solver 
                     s->fplanQ[k].dim = k + 1;
                     s->fplanQ[k].dtype = CLES_F64;
                     for (int j = 0; j < 3; j++) s->fplanQ[k].nloc[j] = s->n[j];
-                    cles::build_line_plan(s->fplanQ[k], desc->filt, (int)s->n[k], s->periodic[k], s->h[k], k == 0 ? 16 : 32);
+                    cles::build_line_plan(s->fplanQ[k], desc->filt, (int)s->n[k], s->periodic[k], s->h[k], chunk_of(k));
                 }
             }
             const size_t N = (size_t)s->N;

@@ -346,6 +346,38 @@ int cles_server_timing(cles_server *server, int64_t id, double *t_submit, double
 int cles_server_release(cles_server *server, int64_t id);
 int cles_server_stats(cles_server *server, size_t *in_use, int *running, int *queued);
 
+/* ----------------------------------------------------------------------- */
+/* Line-tile autotuner (no reference counterpart: the reference solves its   */
+/* lines serially per rank, src/tridiag.jl:31-57, src/banded.jl:38-80)       */
+/* ----------------------------------------------------------------------- */
+
+/* The chunk length of a direction's line plans (points per thread; it fixes
+ * the chunks per line and with them the line kernel and its block shape).
+ * cles_solver_create takes it from a process-wide table keyed by the grid,
+ * precision, periodicity and scheme shapes; shapes the table does not hold
+ * get the largest length (x <= 16, y/z <= 32, >= 8) that still launches
+ * 131072 threads.  Nothing is tuned implicitly, so a table always gives the
+ * same plans and the same results.
+ *
+ * cles_tune_solver times every candidate length (x: 16, 8, 4; y/z: 32, 16,
+ * 8, 4) on the descriptor's grid (one four-field derivative launch and two
+ * filter passes per repetition, `reps` repetitions) and stores the fastest
+ * in the table; chunks[k] / ms_best[k] (either may be NULL) receive the
+ * choice and its milliseconds per repetition, 0 for inactive directions.
+ * Solvers created before the call keep their plans. */
+int cles_tune_solver(cles_ctx *ctx, const cles_solver_desc *desc, int reps, int chunks[3], double ms_best[3]);
+/* The lengths cles_solver_create would use for desc; tuned[k] (may be NULL)
+ * is 1 where the table holds the entry.  Touches no device. */
+int cles_tune_lookup(const cles_solver_desc *desc, int chunks[3], int tuned[3]);
+/* Pins the lengths for desc's grid: chunks[k] > 0 sets direction k (at most
+ * 32), 0 removes its entry (back to the rule), < 0 leaves it. */
+int cles_tune_set(const cles_solver_desc *desc, const int chunks[3]);
+int cles_tune_clear(void);
+/* Text table, one entry per line: dim nx ny nz precision periodic deriv_q
+ * deriv_M filt_M chunk.  Loading adds to (and overrides) the table. */
+int cles_tune_save(const char *path);
+int cles_tune_load(const char *path);
+
 #ifdef __cplusplus
 }
 #endif

@@ -66,7 +66,9 @@ def nscbc_codes(bcs, nscbc):
 
 def gpu_solver(ctx, n, L, q, species=((1.0, 1.4),), mu0=0.0, art=True, cfl=0.5, bcs=None, deriv="lele_d1_6",
                precision="fp64", beta_sensor="strain", bc_funs=None, filter_interval=1, origin=(0.0, 0.0, 0.0),
-               nscbc=None):
+               nscbc=None, chunks=None):
+    """chunks: line-tile lengths (x, y, z) for this solver's plans, set in the
+    table for its creation only (None: the table / rule as it stands)."""
     import compactles_hip as ch
     eos = ch.IdealMixture([ch.IdealSpecies(f"s{k}", R, g) for k, (R, g) in enumerate(species)])
     code = {0: ch.PeriodicBC, 1: ch.SlipWallBC}
@@ -87,7 +89,15 @@ def gpu_solver(ctx, n, L, q, species=((1.0, 1.4),), mu0=0.0, art=True, cfl=0.5, 
     sch = {"lele_d1_6": ch.lele_d1_6, "lele_d1_10": ch.lele_d1_10, "pade_d1_4": ch.pade_d1_4}[deriv]()
     num = ch.Numerics(n_global=n, deriv=sch, art=ch.ArtParams(enabled=art, beta_sensor=beta_sensor), cfl=cfl,
                       precision=precision, filter_interval=filter_interval)
-    s = ch.Solver(ctx, prob, num)
+    if chunks is None:
+        s = ch.Solver(ctx, prob, num)
+    else:
+        spec = ch.SolverSpec(prob, num)
+        ch.tune.pin(spec, chunks)
+        try:
+            s = ch.Solver(ctx, prob, num)
+        finally:
+            ch.tune.unpin(spec)
     Q = s.allocate_state()
     sl = tuple(slice(s.halo[k], s.halo[k] + n[k]) for k in range(3))
     for c, qc in enumerate(q):

@@ -88,7 +88,7 @@ def test_fp32_256_point_x_lines(ctx, walls):
     q = cases.conserved(prim)
     bcs = ((1, 1), (0, 0), (0, 0)) if walls else None
     os_ = cases.oracle_solver(n, L, q, mu0=1e-3, art=True, bcs=bcs)
-    gs = cases.gpu_solver(ctx, n, L, q, mu0=1e-3, art=True, bcs=bcs, precision="fp32")
+    gs = cases.gpu_solver(ctx, n, L, q, mu0=1e-3, art=True, bcs=bcs, precision="fp32", chunks=(16, 32, 32))
     os_.run(1e9, 5)
     gs.run(1e9, 5)
     e_q = _state_err(gs, os_)

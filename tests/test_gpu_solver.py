@@ -191,7 +191,8 @@ def test_rhs_step_filter_256_point_x_lines(ctx, walls):
     q = cases.conserved(prim)
     bcs = ((1, 1), (0, 0), (0, 0)) if walls else None
     os_ = cases.oracle_solver(n, L, q, mu0=1e-3, art=True, bcs=bcs)
-    gs = cases.gpu_solver(ctx, n, L, q, mu0=1e-3, art=True, bcs=bcs)
+    # 16-point x chunks: the tiles of the 256-point x kernel (the thread-count rule would halve them here)
+    gs = cases.gpu_solver(ctx, n, L, q, mu0=1e-3, art=True, bcs=bcs, chunks=(16, 32, 32))
     if walls:
         os_.apply_bcs()
         gs.apply_bcs()
