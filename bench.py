@@ -199,10 +199,11 @@ def measure_tubes(ch, ctx, count, steps, warmup, precision):
                         "Dirichlet ends, art on, cfl 0.4, filter every step, per-instance dt"}
 
 
-def measure_shock_2d(ch, ctx, precision, steps, warmup, n=(1024, 512, 1)):
+def measure_shock_2d(ch, ctx, precision, steps, warmup, n=(1024, 512, 1), tune=True):
     """The two-dimensional form: He|CO2 Mach-1.5 shock / perturbed interface on
     1024 x 512 (two species, NSCBC inflow and outflow in x, periodic y, art on,
-    cfl 0.4, filter every step)."""
+    cfl 0.4, filter every step).  With tune, the line tiles are tuned for the
+    grid first (outside the timed region)."""
     import cases
     q, L, faces = cases.shock_interface(n)
     eos = ch.IdealMixture([ch.IdealSpecies(f"s{k}", R, g) for k, (R, g) in enumerate(cases.HE_CO2)])
@@ -211,6 +212,9 @@ def measure_shock_2d(ch, ctx, precision, steps, warmup, n=(1024, 512, 1)):
     prob = ch.Problem(domain=tuple((0.0, L[k]) for k in range(3)), bcs=bcs, ic=lambda X, Y, Z: None, eos=eos,
                       transport=ch.Transport(mu0=0.0))
     num = ch.Numerics(n_global=n, cfl=0.4, precision=precision)
+    if tune:
+        ch.tune.tune(ctx, ch.SolverSpec(prob, num), reps=5)
+    tiles = ch.tune.lookup(ch.SolverSpec(prob, num))
     s = ch.Solver(ctx, prob, num)
     Q = s.allocate_state()
     sl = tuple(slice(s.halo[k], s.halo[k] + n[k]) for k in range(3))
@@ -232,14 +236,16 @@ def measure_shock_2d(ch, ctx, precision, steps, warmup, n=(1024, 512, 1)):
             "step_roofline": {"alg_bytes_per_point_step": model,
                               "achieved_GBps": round(model * npts / (ms * 1e-3) / 1e9, 1),
                               "frac": round(model * npts / (ms * 1e-3) / 1e9 / HBM_PEAK_GBPS, 4)},
+            "line_tiles": {"chunks": list(tiles[0]), "tuned": list(tiles[1])},
             "workload": f"He|CO2 shock / interface {n[0]}x{n[1]}, NSCBC in / out, art on, cfl 0.4"}
 
 
-def measure_serving(ch, count=32, rate_per_s=1.5, seed=20261202, budget=64 << 30, runners=4):
+def measure_serving(ch, ctx=None, count=32, rate_per_s=1.5, seed=20261202, budget=64 << 30, runners=4):
     """The serving form: `count` requests with Poisson arrivals (rate
     `rate_per_s`, seeded) mixing Taylor-Green cubes of 32^3..128^3 to t = 0.05
     and two-gas shock / interface grids of 256^2..1024x512 to t = 0.03, admitted
-    by device footprint under an HBM budget and run concurrently."""
+    by device footprint under an HBM budget and run concurrently.  With a
+    context, the line tiles of every distinct shape are tuned first."""
     import numpy as np
 
     import cases
@@ -271,6 +277,16 @@ def measure_serving(ch, count=32, rate_per_s=1.5, seed=20261202, budget=64 << 30
             for c, qc in enumerate(q):
                 Q[sl + (c,)] = qc
             reqs.append((prob, num, Q, 0.03))
+    tuned = 0
+    if ctx is not None:
+        seen = set()
+        for prob, num, _, _ in reqs:
+            spec = ch.SolverSpec(prob, num)
+            key = (spec.n, num.precision)
+            if key not in seen:
+                seen.add(key)
+                ch.tune.tune(ctx, spec, reps=5)
+        tuned = len(seen)
     sv = ch.Server(device=0, hbm_budget=budget, runners=runners)
     t0 = time.perf_counter()
     ids = []
@@ -286,7 +302,7 @@ def measure_serving(ch, count=32, rate_per_s=1.5, seed=20261202, budget=64 << 30
     span = max(d for s, a, d, fp in times) - min(s for s, a, d, fp in times)
     return {"solves_per_s": round(count / span, 2), "p50_s": round(float(np.percentile(lat, 50)), 4),
             "p99_s": round(float(np.percentile(lat, 99)), 4), "requests": count, "arrival_rate_per_s": rate_per_s,
-            "all_done": ok, "hbm_budget_GB": budget / 2 ** 30, "runners": runners,
+            "all_done": ok, "hbm_budget_GB": budget / 2 ** 30, "runners": runners, "tuned_shapes": tuned,
             "workload": "Poisson arrivals (seed 20261202): Taylor-Green 32^3..128^3 to t = 0.05 and He|CO2 "
                         "shock / interface 256^2..1024x512 (NSCBC in / out) to t = 0.03, FP64"}
 
@@ -307,11 +323,13 @@ def main():
     ap.add_argument("--only", choices=["tubes", "serving", "shock2d"], default=None,
                     help="measure only the batched-tubes (FP64) or serving form and print its JSON (profiling runs)")
     ap.add_argument("--cpu-n", type=int, default=256)
-    ap.add_argument("--profile-steps", type=int, default=2)
+    ap.add_argument("--profile-steps", type=int, default=2, help="steps of the per-phase event pass (>= 1)")
     ap.add_argument("--counters", type=int, default=0, metavar="STEPS",
                     help="run only STEPS run! steps and a 1 GiB streaming copy 3 times (the workload of the "
                          "rocprofv3 --pmc byte-counter passes), print a short JSON line and exit")
     args = ap.parse_args()
+    if args.profile_steps < 1:
+        ap.error("--profile-steps must be at least 1 (the roofline figures come from that pass)")
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               "(the multi-GPU sharding is a design in DESIGN.md, not a measured path)", file=sys.stderr)
@@ -330,7 +348,7 @@ def main():
         print(json.dumps(measure_shock_2d(ch, ctx, args.precision, args.steps, args.warmup)), flush=True)
         return
     if args.only == "serving":
-        print(json.dumps(measure_serving(ch)), flush=True)
+        print(json.dumps(measure_serving(ch, ctx)), flush=True)
         return
     primary = measure(ch, ctx, n, args.precision, args.steps, args.warmup, args.profile_steps, args.run_mode)
     out = {
@@ -369,7 +387,7 @@ def main():
                                          for p in ("fp64", "fp32")}
         out["forms"]["shock_interface_2d"] = {p: measure_shock_2d(ch, ctx, p, args.steps, args.warmup)
                                               for p in ("fp64", "fp32")}
-        out["forms"]["serving"] = measure_serving(ch)
+        out["forms"]["serving"] = measure_serving(ch, ctx)
     ctx.close()
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(args.cpu_n)

@@ -203,4 +203,18 @@ function fetch!(s::HIPServer, id::Integer, Q::Array{Float64}, halo::NTuple{3,Int
     return status[], t[], steps[]
 end
 
+# ---------------------------------------------------------------------------
+# Line-tile table (chunk lengths of the solver's line plans; no counterpart in
+# the reference).  Tune before setup(...; backend = :hip) so the solver uses it.
+# ---------------------------------------------------------------------------
+"Time the candidate chunk lengths on desc's grid; returns (chunks, ms per repetition) per direction."
+function tune_line_tiles!(ctx::Context, desc::Ref; reps::Integer=5)
+    chunks = zeros(Cint, 3); ms = zeros(Cdouble, 3)
+    check(ccall((:cles_tune_solver, LIB), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Cint, Ptr{Cint}, Ptr{Cdouble}),
+                ctx.ptr, desc, reps, chunks, ms))
+    return Tuple(chunks), Tuple(ms)
+end
+save_line_tiles(path::AbstractString) = check(ccall((:cles_tune_save, LIB), Cint, (Cstring,), path))
+load_line_tiles(path::AbstractString) = check(ccall((:cles_tune_load, LIB), Cint, (Cstring,), path))
+
 end # module
