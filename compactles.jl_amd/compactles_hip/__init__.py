@@ -12,3 +12,4 @@ from .problem import (ArtParams, DirichletBC, IdealMixture, IdealSpecies, NSCBCI
 from .batch import Batch, Instance  # noqa: F401
 from .server import Server  # noqa: F401
 from . import tune  # noqa: F401
+from .callbacks import AtTime, Callback, EveryStep, EveryTime, ProgressLog, WhenState  # noqa: F401

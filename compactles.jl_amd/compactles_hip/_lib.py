@@ -44,7 +44,8 @@ class SolverDesc(C.Structure):
                 ("filter_interval", C.c_int), ("bc", (C.c_int * 2) * 3),
                 ("bc_state", ((C.c_double * (MAXSP + 4)) * 2) * 3), ("nscbc", (NscbcDesc * 2) * 3),
                 ("predict", C.c_double), ("max_growth", C.c_double), ("dt_min", C.c_double),
-                ("dt_min_ratio", C.c_double), ("precision", C.c_int)]
+                ("dt_min_ratio", C.c_double), ("precision", C.c_int), ("retries", C.c_int),
+                ("cfl_backoff", C.c_double), ("savepoint_interval", C.c_int), ("landing_steps", C.c_int)]
 
 
 P = C.c_void_p
@@ -90,6 +91,9 @@ SIGNATURES = {
     "cles_step": (C.c_int, [P, C.c_double, C.c_int]),
     "cles_filter_state": (C.c_int, [P]),
     "cles_run": (C.c_int, [P, C.c_double, C.c_int64, DP, C.c_int64]),
+    "cles_run_ctl": (C.c_int, [P, C.c_double, C.c_int64, P, DP, C.c_int64, C.POINTER(C.c_int)]),
+    "cles_solver_get_cfl": (C.c_int, [P, DP]),
+    "cles_solver_set_cfl": (C.c_int, [P, C.c_double]),
     "cles_diagnostics": (C.c_int, [P, DP, DP]),
     "cles_solver_shape": (C.c_int, [P, C.POINTER(C.c_int64), C.POINTER(C.c_int)]),
     "cles_solver_profile": (C.c_int, [P, C.c_int]),

@@ -77,12 +77,21 @@ class ArtParams:
 
 @dataclass
 class StepControl:
-    """stepcontrol.jl:116-136 (prediction and floors; retries are not offered)."""
+    """stepcontrol.jl:116-136: prediction, growth limit, soft landing, floors,
+    and recovery (retries rollbacks to the last savepoint, each multiplying the
+    CFL by cfl_backoff; a savepoint every savepoint_interval steps)."""
     predict: float = 0.0
     max_growth: float = 0.0
     landing_steps: int = 2
     dt_min: float = 0.0
     dt_min_ratio: float = 1e-8
+    retries: int = 0
+    cfl_backoff: float = 0.5
+    savepoint_interval: int = 25
+
+    def __post_init__(self):
+        if self.landing_steps < 1:
+            raise ValueError("StepControl: landing_steps must be >= 1 (1 is a hard clip onto the scheduled time)")
 
 
 # --------------------------------------------------------------------------
@@ -237,6 +246,8 @@ def fill_common_desc(d: L.SolverDesc, prob: Problem, num: Numerics):
     d.filter_interval = num.filter_interval
     ctl = num.control
     d.predict, d.max_growth, d.dt_min, d.dt_min_ratio = ctl.predict, ctl.max_growth, ctl.dt_min, ctl.dt_min_ratio
+    d.retries, d.cfl_backoff = int(ctl.retries), float(ctl.cfl_backoff)
+    d.savepoint_interval, d.landing_steps = int(ctl.savepoint_interval), int(ctl.landing_steps)
     if num.precision not in ("fp64", "fp32"):
         raise ValueError("precision must be 'fp64' or 'fp32'")
     d.precision = L.F64 if num.precision == "fp64" else L.F32
@@ -416,10 +427,42 @@ class Solver(SolverSpec):
     def filter_state(self):
         L.check(self.ctx.lib.cles_filter_state(self.handle))
 
-    def run(self, tfinal, nmax=2**62, record=0):
+    def run(self, tfinal, nmax=2**62, record=0, callback=None):
+        """timestep.jl:277-400 run! on the device state; returns the first
+        `record` step sizes.  callback: a Callback, a tuple / list of them or
+        a bare callable (callbacks.jl), run between steps; effects receive
+        (solver, Q) with Q a lazy view of the device state.  The number of
+        StepControl rollbacks is left in self.retries_used."""
+        from .callbacks import HookAdapter, LazyState
         dts = (C.c_double * max(record, 1))()
-        L.check(self.ctx.lib.cles_run(self.handle, float(tfinal), int(nmax), dts, int(record)))
+        used = C.c_int(0)
+        adapter = None
+        hooks = None
+        if callback is not None:
+            def fetch():
+                Q = self.allocate_state()
+                self.download(Q)
+                return Q
+            adapter = HookAdapter(callback, self, LazyState(fetch), self.t, self.step_count)
+            hooks = C.byref(adapter.hooks)
+        rc = self.ctx.lib.cles_run_ctl(self.handle, float(tfinal), int(nmax), hooks, dts, int(record), C.byref(used))
+        self.retries_used = used.value
+        if adapter is not None:
+            self.rewinds = adapter.rewinds
+            adapter.check()
+        L.check(rc)
         return np.array(dts[:record])
+
+    @property
+    def cfl(self) -> float:
+        """The CFL in effect (lowered in place by StepControl rollbacks)."""
+        v = C.c_double()
+        L.check(self.ctx.lib.cles_solver_get_cfl(self.handle, C.byref(v)))
+        return v.value
+
+    @cfl.setter
+    def cfl(self, value: float):
+        L.check(self.ctx.lib.cles_solver_set_cfl(self.handle, float(value)))
 
     def diagnostics(self):
         ke, en = C.c_double(), C.c_double()
@@ -476,12 +519,12 @@ def setup(prob: Problem, num: Numerics, ctx: Context | None = None):
     return solver, Q
 
 
-def run_(solver: Solver, Q: np.ndarray, tfinal: float, nmax: int = 2**62, upload: bool = False):
-    """timestep.jl:402-405 run!(solver, Q; tfinal, nmax): advances the device
-    state and writes the result back into Q.  upload=True first copies Q to the
-    device (after a host-side modification)."""
+def run_(solver: Solver, Q: np.ndarray, tfinal: float, nmax: int = 2**62, upload: bool = False, callback=None):
+    """timestep.jl:402-405 run!(solver, Q; tfinal, nmax, callback): advances
+    the device state and writes the result back into Q.  upload=True first
+    copies Q to the device (after a host-side modification)."""
     if upload:
         solver.upload(Q)
-    solver.run(tfinal, nmax)
+    solver.run(tfinal, nmax, callback=callback)
     solver.download(Q)
     return Q

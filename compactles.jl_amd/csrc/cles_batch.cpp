@@ -482,6 +482,9 @@ int cles_batch_create(cles_ctx *ctx, const cles_solver_desc *desc, int64_t count
         for (int k = 1; k < 3; k++)
             CLES_REQUIRE(d.bc[k][0] == CLES_BC_PERIODIC && d.bc[k][1] == CLES_BC_PERIODIC,
                          "batched instances are one-dimensional: dimensions 2 and 3 are collapsed and periodic");
+        if (d.retries != 0)
+            throw cles::Error{CLES_ERR_UNSUPPORTED, "StepControl retries are not available for batched instances "
+                                                    "(each instance stops at its own failure)"};
         if (!cles::rag_shape_ok(d.deriv) || !cles::rag_shape_ok(d.filt))
             throw cles::Error{CLES_ERR_UNSUPPORTED, "scheme shape not available for batched instances"};
         CLES_HIP(hipSetDevice(ctx->device));

@@ -729,6 +729,14 @@ void validate_desc(const cles_solver_desc &d)
     if (d.precision != CLES_F64 && d.precision != CLES_F32)
         throw cles::Error{CLES_ERR_INVALID, "precision must be CLES_F64 or CLES_F32"};
     if (!(d.cfl > 0)) throw cles::Error{CLES_ERR_INVALID, "cfl must be positive"};
+    // stepcontrol.jl:116-136
+    if (d.retries < 0) throw cles::Error{CLES_ERR_INVALID, "StepControl: retries must be >= 0"};
+    if (d.retries > 0 && !(d.cfl_backoff > 0))
+        throw cles::Error{CLES_ERR_INVALID, "StepControl: cfl_backoff must be positive"};
+    if (d.savepoint_interval < 0) throw cles::Error{CLES_ERR_INVALID, "StepControl: savepoint_interval must be >= 0"};
+    if (d.landing_steps < 0)
+        throw cles::Error{CLES_ERR_INVALID, "StepControl: landing_steps must be >= 1 (1 is a hard clip onto the "
+                                            "scheduled time)"};
     if ((double)d.n_global[0] * (double)d.n_global[1] * (double)d.n_global[2] >= 2147483648.0)
         throw cles::Error{CLES_ERR_UNSUPPORTED, "grids of 2^31 points or more are not supported on one GPU"};
 }
@@ -1131,7 +1139,7 @@ int cles_filter_state(cles_solver *s)
     })
 }
 
-// timestep.jl:277-400 run! (no callbacks, no retries)
+// timestep.jl:277-400 run! with the per-step decisions on the device
 namespace {
 
 cles::BatchCtl device_ctl(cles_solver *s)
@@ -1301,46 +1309,141 @@ int cles_solver_set_run_mode(cles_solver *s, int mode)
 
 int cles_run(cles_solver *s, double tfinal, int64_t nmax, double *dts, int64_t ndts)
 {
+    return cles_run_ctl(s, tfinal, nmax, nullptr, dts, ndts, nullptr);
+}
+
+int cles_run_ctl(cles_solver *s, double tfinal, int64_t nmax, const cles_run_hooks *hooks, double *dts,
+                 int64_t ndts, int *retries_used)
+{
     CLES_TRY({
         CLES_REQUIRE(s, "NULL argument");
+        if (retries_used) *retries_used = 0;
+        const bool hooked = hooks && (hooks->next_time || hooks->after_step || hooks->rewind);
+        const bool recover = s->d.retries > 0;
         int rc = CLES_OK;
         std::string msg;
-        if (s->run_mode != CLES_RUN_HOST_DT && !s->prof) {
+        if (s->run_mode != CLES_RUN_HOST_DT && !s->prof && !hooked && !recover) {
             rc = run_graph(s, tfinal, nmax, dts, ndts, msg, s->run_mode == CLES_RUN_STEP_GRAPH);
             if (rc != CLES_OK) throw cles::Error{rc, msg};
             return CLES_OK;
         }
-        dispatch(s, [&](auto &im) {
-            // fresh Workspace (timestep.jl:402-405)
-            CLES_HIP(hipMemsetAsync(s->du, 0, (size_t)s->N * s->C * sizeof(double), s->st()));
-            CLES_HIP(hipMemsetAsync(s->dQ, 0, (size_t)s->N * s->C * s->wsize(), s->st()));
-            double dt_seen = 0.0;
-            int64_t taken = 0;
-            while (s->t < tfinal && s->step < nmax) {
-                s->tstage = s->t;
-                im.apply_bcs();
-                double rate = 0, rho_min = 0;
-                im.max_rate(&rate, &rho_min);
-                double r = rate;   // timestep.jl:29-42 predicted_dt
-                if (s->d.predict > 0 && s->rate_prev > 0) r = jmax(r, r + s->d.predict * (r - s->rate_prev));
-                double dt = s->d.cfl / r;
-                if (s->d.max_growth > 0 && s->dt_prev > 0) dt = jmin(dt, s->d.max_growth * s->dt_prev);
-                rc = check_step(s->d, dt, rho_min, dt_seen, msg);
-                if (rc != CLES_OK) return;
-                dt_seen = jmax(dt_seen, dt);
-                dt = jmin(dt, tfinal - s->t);
-                im.step(dt, true);
-                if (dts && taken < ndts) dts[taken] = dt;
-                taken++;
-                s->t += dt;
-                s->step += 1;
-                s->dt_prev = dt;
-                s->rate_prev = rate;
-                if (s->d.filter_interval > 0 && s->step % s->d.filter_interval == 0) im.filter_state();
+        // timestep.jl:286: the savepoint (an FP64 copy of Q, t, step)
+        double *save_q = nullptr;
+        double save_t = 0.0;
+        int64_t save_step = 0;
+        const size_t qbytes = (size_t)s->N * s->C * sizeof(double);
+        if (recover) {
+            CLES_HIP(hipMalloc((void **)&save_q, qbytes));
+            save_t = s->t;
+            save_step = s->step;
+        }
+        int attempts = 0;
+        try {
+            CLES_HIP(hipSetDevice(s->ctx->device));
+            if (recover) CLES_HIP(hipMemcpyAsync(save_q, s->Q, qbytes, hipMemcpyDeviceToDevice, s->st()));
+            const int landing = s->d.landing_steps > 0 ? s->d.landing_steps : 2;
+            dispatch(s, [&](auto &im) {
+                // fresh Workspace (timestep.jl:402-405)
+                CLES_HIP(hipMemsetAsync(s->du, 0, (size_t)s->N * s->C * sizeof(double), s->st()));
+                CLES_HIP(hipMemsetAsync(s->dQ, 0, (size_t)s->N * s->C * s->wsize(), s->st()));
+                double dt_seen = 0.0;
+                int64_t taken = 0;
+                int64_t guard_step = -1;
+                while (s->t < tfinal && s->step < nmax) {
+                    s->tstage = s->t;
+                    im.apply_bcs();
+                    double rate = 0, rho_min = 0;
+                    im.max_rate(&rate, &rho_min);
+                    double r = rate;   // timestep.jl:29-42 predicted_dt
+                    if (s->d.predict > 0 && s->rate_prev > 0) r = jmax(r, r + s->d.predict * (r - s->rate_prev));
+                    double dt = s->d.cfl / r;
+                    if (s->d.max_growth > 0 && s->dt_prev > 0) dt = jmin(dt, s->d.max_growth * s->dt_prev);
+                    std::string m;
+                    const int fail = check_step(s->d, dt, rho_min, dt_seen, m);
+                    if (fail != CLES_OK) {
+                        if (!recover || attempts >= s->d.retries) {
+                            rc = fail;
+                            msg = m;
+                            return;
+                        }
+                        // timestep.jl:318-344: roll back, lower the CFL (compounding)
+                        attempts++;
+                        CLES_HIP(hipMemcpyAsync(s->Q, save_q, qbytes, hipMemcpyDeviceToDevice, s->st()));
+                        guard_step = s->step;
+                        s->t = save_t;
+                        s->step = save_step;
+                        s->d.cfl *= s->d.cfl_backoff;
+                        s->dt_prev = 0.0;
+                        s->rate_prev = 0.0;
+                        dt_seen = 0.0;
+                        if (hooks && hooks->rewind) {
+                            CLES_HIP(hipStreamSynchronize(s->st()));
+                            hooks->rewind(hooks->user, save_t, save_step);
+                        }
+                        continue;
+                    }
+                    // timestep.jl:345-351: bank the state that has just passed its check
+                    if (recover && s->d.savepoint_interval > 0 && s->step > guard_step &&
+                        s->step % s->d.savepoint_interval == 0) {
+                        CLES_HIP(hipMemcpyAsync(save_q, s->Q, qbytes, hipMemcpyDeviceToDevice, s->st()));
+                        save_t = s->t;
+                        save_step = s->step;
+                    }
+                    dt_seen = jmax(dt_seen, dt);
+                    dt = jmin(dt, tfinal - s->t);
+                    // timestep.jl:367-383 soft landing on the next scheduled instant
+                    if (hooks && hooks->next_time) {
+                        const double next = hooks->next_time(hooks->user, s->t, s->step);
+                        const double gap = next - s->t;
+                        if (next <= tfinal && gap > 0 && gap < dt * landing) dt = gap / std::ceil(gap / dt);
+                    }
+                    im.step(dt, true);
+                    if (dts && taken < ndts) dts[taken] = dt;
+                    taken++;
+                    s->t += dt;
+                    s->step += 1;
+                    s->dt_prev = dt;
+                    s->rate_prev = rate;
+                    if (s->d.filter_interval > 0 && s->step % s->d.filter_interval == 0) im.filter_state();
+                    if (hooks && hooks->after_step) {
+                        CLES_HIP(hipStreamSynchronize(s->st()));
+                        if (hooks->after_step(hooks->user, s->t, s->step, dt)) break;
+                    }
+                }
+                CLES_HIP(hipStreamSynchronize(s->st()));
+            });
+        } catch (...) {
+            if (save_q) {
+                (void)hipStreamSynchronize(s->st());
+                (void)hipFree(save_q);
             }
+            throw;
+        }
+        if (save_q) {
             CLES_HIP(hipStreamSynchronize(s->st()));
-        });
+            CLES_HIP(hipFree(save_q));
+        }
+        if (retries_used) *retries_used = attempts;
+        if (attempts > 0) drop_graph(s);   // the CFL was lowered
         if (rc != CLES_OK) throw cles::Error{rc, msg};
+    })
+}
+
+int cles_solver_get_cfl(cles_solver *s, double *cfl)
+{
+    CLES_TRY({
+        CLES_REQUIRE(s && cfl, "NULL argument");
+        *cfl = s->d.cfl;
+    })
+}
+
+int cles_solver_set_cfl(cles_solver *s, double cfl)
+{
+    CLES_TRY({
+        CLES_REQUIRE(s, "NULL argument");
+        CLES_REQUIRE(cfl > 0, "cfl must be positive");
+        s->d.cfl = cfl;
+        drop_graph(s);   // a captured iteration holds the CFL as a kernel argument
     })
 }
 

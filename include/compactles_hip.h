@@ -20,7 +20,7 @@
 extern "C" {
 #endif
 
-#define CLES_ABI_VERSION 1
+#define CLES_ABI_VERSION 2
 
 /* Status codes.  The solver-failure codes mirror SolverFailure reasons
  * (src/stepcontrol.jl:138-164): nonfinite, negative_density, planck, dt_min,
@@ -172,6 +172,12 @@ typedef struct {
     cles_nscbc_desc nscbc[3][2];
     double predict, max_growth, dt_min, dt_min_ratio;
     int precision;                         /* CLES_F64, or CLES_F32 = mixed form */
+    /* StepControl recovery (stepcontrol.jl:100-114, timestep.jl:288-344) and
+     * soft landing (timestep.jl:367-383) */
+    int retries;                           /* rollbacks to the last savepoint (0: off) */
+    double cfl_backoff;                    /* CFL multiplier per rollback */
+    int savepoint_interval;                /* steps between savepoints */
+    int landing_steps;                     /* >= 1; 0 selects the default 2 */
 } cles_solver_desc;
 
 /* save_checkpoint / load_checkpoint! (src/io.jl:6-75): `prefix`.rNNNN.ckpt
@@ -220,10 +226,46 @@ int cles_step(cles_solver *solver, double dt, int prepared);
 /* filter_state!(solver, Q) (src/timestep.jl:414-426). */
 int cles_filter_state(cles_solver *solver);
 /* run!(solver, Q; tfinal, nmax) (src/timestep.jl:277-400) with a fresh
- * Workspace, no callbacks and no retries.  Returns CLES_OK or a solver-failure
- * code (state left at the failing step).  dts (may be NULL) receives up to
- * ndts step sizes. */
+ * Workspace and no callbacks; the descriptor's StepControl recovery applies.
+ * Returns CLES_OK or a solver-failure code (state left at the failing step).
+ * dts (may be NULL) receives up to ndts step sizes, in the order the steps
+ * were taken (a rollback's abandoned steps included). */
 int cles_run(cles_solver *solver, double tfinal, int64_t nmax, double *dts, int64_t ndts);
+
+/* Step-boundary hooks of run! (src/callbacks.jl), called on the host thread
+ * that runs cles_run_ctl, between completed steps (the solver's device state
+ * is then consistent and other cles_* calls on it are allowed):
+ *  next_time(user, t, step): the next scheduled instant (callback_next_time,
+ *    callbacks.jl:215-222), +INFINITY for none; run! clips the step to land on
+ *    it softly, in ceil(gap/dt) equal steps when gap < landing_steps * dt
+ *    (timestep.jl:357-383);
+ *  after_step(user, t, step, dt): after each completed step and its filter
+ *    (run_callbacks!, callbacks.jl:233-250); nonzero stops run!;
+ *  rewind(user, t, step): after a StepControl rollback to (t, step)
+ *    (rewind_callbacks!, callbacks.jl:224-231).
+ * Any of the three may be NULL. */
+typedef struct {
+    void *user;
+    double (*next_time)(void *user, double t, int64_t step);
+    int (*after_step)(void *user, double t, int64_t step, double dt);
+    void (*rewind)(void *user, double t, int64_t step);
+} cles_run_hooks;
+
+/* run!(solver, Q; tfinal, nmax, callback, control) (src/timestep.jl:277-400):
+ * cles_run with hooks (NULL: none).  With retries > 0 a failing check_step
+ * restores the last savepoint (Q, t, step; an extra FP64 state on the device,
+ * taken every savepoint_interval steps on a state that has just passed its
+ * check, never at or before a step that failed), multiplies the solver's CFL
+ * by cfl_backoff (it stays lowered), clears the rate history and dt_seen,
+ * calls rewind, and continues; with no retries left the failure is returned.
+ * Hooks or retries run every step from the host (the device run modes are
+ * used only without them). *retries_used (may be NULL) receives the number of
+ * rollbacks. */
+int cles_run_ctl(cles_solver *solver, double tfinal, int64_t nmax, const cles_run_hooks *hooks, double *dts,
+                 int64_t ndts, int *retries_used);
+/* The CFL in effect (Numerics' cfl, lowered by each rollback), and setting it. */
+int cles_solver_get_cfl(cles_solver *solver, double *cfl);
+int cles_solver_set_cfl(cles_solver *solver, double cfl);
 /* Kinetic energy ∫½|ρu|²/ρ dV (examples/taylor_green.jl:51-60) and enstrophy
  * ∫½|ω|² dV with ω from the compact grad_u of the current state
  * (src/io.jl:332-340), both reduced in FP64 in a fixed order

@@ -1220,11 +1220,27 @@ static int check_step(const or_desc *d, double dt, double rho_min, double dt_see
     return 0;
 }
 
-/* timestep.jl:277-400 run! (no callbacks, no retries); fresh Workspace */
-int or_run(or_solver *s, double tfinal, int64_t nmax, double *dts, int64_t ndts)
+/* timestep.jl:277-400 run!: fresh Workspace; StepControl recovery
+ * (timestep.jl:286-351: savepoint of Q, t, step on a state that has just
+ * passed its check, never at or before a failed step; a failure rolls back,
+ * multiplies cfl by cfl_backoff, clears the rate history and dt_seen) and the
+ * soft landing on the next scheduled instant (timestep.jl:352-383) */
+int or_run_ctl(or_solver *s, double tfinal, int64_t nmax, const or_run_hooks *hooks, double *dts, int64_t ndts,
+               int *retries_used)
 {
     memset(s->dQ, 0, sizeof(double) * s->NP * s->C);
     memset(s->du, 0, sizeof(double) * s->NP * s->C);
+    const int recover = s->d.retries > 0;
+    const size_t qn = (size_t)s->NP * s->C;
+    double *save_q = NULL, save_t = s->t;
+    int64_t save_step = s->step;
+    if (recover) {
+        save_q = malloc(qn * sizeof(double));
+        memcpy(save_q, s->Q, qn * sizeof(double));
+    }
+    const int landing = s->d.landing_steps > 0 ? s->d.landing_steps : 2;
+    int attempts = 0, rc = 0;
+    int64_t guard_step = -1;
     double dt_seen = 0.0;
     int64_t taken = 0;
     while (s->t < tfinal && s->step < nmax) {
@@ -1238,9 +1254,36 @@ int or_run(or_solver *s, double tfinal, int64_t nmax, double *dts, int64_t ndts)
         double dt = s->d.cfl / r;
         if (s->d.max_growth > 0 && s->dt_prev > 0) dt = jmin(dt, s->d.max_growth * s->dt_prev);
         int fail = check_step(&s->d, dt, rho_min, dt_seen);
-        if (fail) return fail;
+        if (fail) {
+            if (!recover || attempts >= s->d.retries) {
+                rc = fail;
+                break;
+            }
+            attempts++;
+            memcpy(s->Q, save_q, qn * sizeof(double));
+            guard_step = s->step;
+            s->t = save_t;
+            s->step = save_step;
+            s->d.cfl *= s->d.cfl_backoff;
+            s->dt_prev = 0.0;
+            s->rate_prev = 0.0;
+            dt_seen = 0.0;
+            if (hooks && hooks->rewind) hooks->rewind(hooks->user, save_t, save_step);
+            continue;
+        }
+        if (recover && s->d.savepoint_interval > 0 && s->step > guard_step &&
+            s->step % s->d.savepoint_interval == 0) {
+            memcpy(save_q, s->Q, qn * sizeof(double));
+            save_t = s->t;
+            save_step = s->step;
+        }
         dt_seen = jmax(dt_seen, dt);
         dt = jmin(dt, tfinal - s->t);
+        if (hooks && hooks->next_time) {
+            double next = hooks->next_time(hooks->user, s->t, s->step);
+            double gap = next - s->t;
+            if (next <= tfinal && gap > 0 && gap < dt * landing) dt = gap / ceil(gap / dt);
+        }
         or_step(s, dt, 1);
         if (dts && taken < ndts) dts[taken] = dt;
         taken++;
@@ -1249,9 +1292,20 @@ int or_run(or_solver *s, double tfinal, int64_t nmax, double *dts, int64_t ndts)
         s->dt_prev = dt;
         s->rate_prev = rate;
         if (s->d.filter_interval > 0 && s->step % s->d.filter_interval == 0) or_filter_state(s);
+        if (hooks && hooks->after_step && hooks->after_step(hooks->user, s->t, s->step, dt)) break;
     }
-    return 0;
+    free(save_q);
+    if (retries_used) *retries_used = attempts;
+    return rc;
 }
+
+int or_run(or_solver *s, double tfinal, int64_t nmax, double *dts, int64_t ndts)
+{
+    return or_run_ctl(s, tfinal, nmax, NULL, dts, ndts, NULL);
+}
+
+double or_solver_cfl(const or_solver *s) { return s->d.cfl; }
+void or_solver_set_cfl(or_solver *s, double cfl) { s->d.cfl = cfl; }
 
 /* diagnostics.jl:41-80 quad_weight / volume_integral (Cartesian, inv_J == 1) */
 static inline double quad_weight(const or_solver *s, int d, int64_t i)

@@ -88,7 +88,20 @@ typedef struct {
     or_nscbc nscbc[3][2];
     /* StepControl (src/stepcontrol.jl:116-136) */
     double predict, max_growth, dt_min, dt_min_ratio;
+    int retries;
+    double cfl_backoff;
+    int savepoint_interval;
+    int landing_steps;                   /* 0: the default 2 */
 } or_desc;
+
+/* Step-boundary hooks of run! (src/callbacks.jl): next scheduled instant,
+ * after each completed step (nonzero stops), after a rollback. */
+typedef struct {
+    void *user;
+    double (*next_time)(void *user, double t, int64_t step);
+    int (*after_step)(void *user, double t, int64_t step, double dt);
+    void (*rewind)(void *user, double t, int64_t step);
+} or_run_hooks;
 
 typedef struct or_solver or_solver;
 
@@ -118,6 +131,13 @@ void or_filter_state(or_solver *s);
  * or a SolverFailure code (1 nonfinite, 2 negative density, 3 planck,
  * 4 dt_min, 5 dt collapse).  dts (may be NULL) receives up to ndts dt values. */
 int or_run(or_solver *s, double tfinal, int64_t nmax, double *dts, int64_t ndts);
+/* run!(solver, Q; tfinal, nmax, callback, control): or_run with hooks (may be
+ * NULL) and StepControl recovery (timestep.jl:286-344); *retries_used (may be
+ * NULL) receives the number of rollbacks. */
+int or_run_ctl(or_solver *s, double tfinal, int64_t nmax, const or_run_hooks *hooks, double *dts, int64_t ndts,
+               int *retries_used);
+double or_solver_cfl(const or_solver *s);
+void or_solver_set_cfl(or_solver *s, double cfl);
 /* Kinetic energy and enstrophy (volume integrals; enstrophy from compact grad_u
  * of the current state). */
 void or_diagnostics(or_solver *s, double *ke, double *enstrophy);

@@ -49,7 +49,9 @@ class Desc(C.Structure):
                 ("bc_state", ((C.c_double * (MAXSP + 4)) * 2) * 3),
                 ("nscbc", (Nscbc * 2) * 3),
                 ("predict", C.c_double), ("max_growth", C.c_double),
-                ("dt_min", C.c_double), ("dt_min_ratio", C.c_double)]
+                ("dt_min", C.c_double), ("dt_min_ratio", C.c_double),
+                ("retries", C.c_int), ("cfl_backoff", C.c_double), ("savepoint_interval", C.c_int),
+                ("landing_steps", C.c_int)]
 
 
 def _cpu_has_avx2() -> bool:
@@ -97,6 +99,9 @@ def _load():
         "or_step": (None, [P, C.c_double, C.c_int]),
         "or_filter_state": (None, [P]),
         "or_run": (C.c_int, [P, C.c_double, C.c_int64, D, C.c_int64]),
+        "or_run_ctl": (C.c_int, [P, C.c_double, C.c_int64, P, D, C.c_int64, C.POINTER(C.c_int)]),
+        "or_solver_cfl": (C.c_double, [P]),
+        "or_solver_set_cfl": (None, [P, C.c_double]),
         "or_diagnostics": (None, [P, D, D]),
         "or_volume_integral": (C.c_double, [P, D]),
     }
@@ -290,12 +295,24 @@ class Solver:
     def filter_state(self):
         lib().or_filter_state(self._s)
 
-    def run(self, tfinal, nmax, record=0):
+    def run(self, tfinal, nmax, record=0, hooks=None):
+        """run! (timestep.jl:277-400); hooks: a pointer to a hook table with
+        the layout of or_run_hooks, or None.  Rollbacks in self.retries_used."""
         dts = (C.c_double * max(record, 1))()
-        rc = lib().or_run(self._s, tfinal, nmax, dts, record)
+        used = C.c_int(0)
+        rc = lib().or_run_ctl(self._s, tfinal, nmax, hooks, dts, record, C.byref(used))
+        self.retries_used = used.value
         if rc:
             raise RuntimeError(f"SolverFailure code {rc}")
         return np.array(dts[:record])
+
+    @property
+    def cfl(self):
+        return lib().or_solver_cfl(self._s)
+
+    @cfl.setter
+    def cfl(self, value):
+        lib().or_solver_set_cfl(self._s, float(value))
 
     def diagnostics(self):
         ke, en = C.c_double(), C.c_double()
@@ -307,7 +324,8 @@ def make_desc(n_global, L_domain, origin=(0.0, 0.0, 0.0), species=((1.0, 1.4),),
               mu0=0.0, Pr=0.7, Sc=0.7, art=True, C_mu=0.002, C_beta=1.0, C_kappa=0.01,
               C_D=0.01, beta_sensor=0, deriv="lele_d1_6", alphaf=0.45, cfl=0.5,
               filter_interval=1, bcs=None, bc_state=None, nscbc=None,
-              predict=0.0, max_growth=0.0, dt_min=0.0, dt_min_ratio=1e-8) -> Desc:
+              predict=0.0, max_growth=0.0, dt_min=0.0, dt_min_ratio=1e-8, retries=0, cfl_backoff=0.5,
+              savepoint_interval=25, landing_steps=2) -> Desc:
     d = Desc()
     for k in range(3):
         d.n_global[k] = int(n_global[k])
@@ -344,4 +362,6 @@ def make_desc(n_global, L_domain, origin=(0.0, 0.0, 0.0), species=((1.0, 1.4),),
                 else:
                     setattr(f, key, val)
     d.predict, d.max_growth, d.dt_min, d.dt_min_ratio = predict, max_growth, dt_min, dt_min_ratio
+    d.retries, d.cfl_backoff, d.savepoint_interval, d.landing_steps = retries, cfl_backoff, savepoint_interval, \
+        landing_steps
     return d

@@ -66,7 +66,7 @@ def nscbc_codes(bcs, nscbc):
 
 def gpu_solver(ctx, n, L, q, species=((1.0, 1.4),), mu0=0.0, art=True, cfl=0.5, bcs=None, deriv="lele_d1_6",
                precision="fp64", beta_sensor="strain", bc_funs=None, filter_interval=1, origin=(0.0, 0.0, 0.0),
-               nscbc=None, chunks=None):
+               nscbc=None, chunks=None, control=None):
     """chunks: line-tile lengths (x, y, z) for this solver's plans, set in the
     table for its creation only (None: the table / rule as it stands)."""
     import compactles_hip as ch
@@ -88,7 +88,7 @@ def gpu_solver(ctx, n, L, q, species=((1.0, 1.4),), mu0=0.0, art=True, cfl=0.5, 
                       ic=lambda X, Y, Z: None, eos=eos, transport=ch.Transport(mu0=mu0))
     sch = {"lele_d1_6": ch.lele_d1_6, "lele_d1_10": ch.lele_d1_10, "pade_d1_4": ch.pade_d1_4}[deriv]()
     num = ch.Numerics(n_global=n, deriv=sch, art=ch.ArtParams(enabled=art, beta_sensor=beta_sensor), cfl=cfl,
-                      precision=precision, filter_interval=filter_interval)
+                      precision=precision, filter_interval=filter_interval, control=control or ch.StepControl())
     if chunks is None:
         s = ch.Solver(ctx, prob, num)
     else:

@@ -34,12 +34,13 @@ def test_library_exports_every_declared_symbol():
 
 def test_abi_version():
     import compactles_hip as ch
-    assert ch.load().cles_abi_version() == 1
+    assert ch.load().cles_abi_version() == 2
 
 
 def test_struct_layout_matches_header():
     """sizeof/offsetof from the C compiler against the ctypes records."""
     from compactles_hip import _lib as L
+    from compactles_hip.callbacks import RunHooks
     prog = r'''
 #include <stdio.h>
 #include <stddef.h>
@@ -49,6 +50,9 @@ int main(void) {
          offsetof(cles_solver_desc, precision));
   printf("%zu %zu %zu %zu\n", offsetof(cles_scheme_desc, closure_rhs), offsetof(cles_solver_desc, filt),
          offsetof(cles_solver_desc, bc_state), offsetof(cles_solver_desc, nscbc));
+  printf("%zu %zu %zu %zu\n", offsetof(cles_solver_desc, retries), offsetof(cles_solver_desc, cfl_backoff),
+         offsetof(cles_solver_desc, landing_steps), sizeof(cles_run_hooks));
+  printf("%zu %zu\n", offsetof(cles_run_hooks, after_step), offsetof(cles_run_hooks, rewind));
   return 0;
 }
 '''
@@ -61,7 +65,8 @@ int main(void) {
     got = [int(x) for x in out]
     want = [C.sizeof(L.SchemeDesc), C.sizeof(L.NscbcDesc), C.sizeof(L.SolverDesc), L.SolverDesc.precision.offset,
             L.SchemeDesc.closure_rhs.offset, L.SolverDesc.filt.offset, L.SolverDesc.bc_state.offset,
-            L.SolverDesc.nscbc.offset]
+            L.SolverDesc.nscbc.offset, L.SolverDesc.retries.offset, L.SolverDesc.cfl_backoff.offset,
+            L.SolverDesc.landing_steps.offset, C.sizeof(RunHooks), RunHooks.after_step.offset, RunHooks.rewind.offset]
     assert got == want
 
 
