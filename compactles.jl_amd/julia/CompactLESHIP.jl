@@ -115,16 +115,100 @@ mutable struct HIPSolver
     halo::NTuple{3,Int}
 end
 
-"run!(solver, Q; tfinal, nmax): Q (host, padded) is uploaded, advanced on the GPU and written back."
-function CompactLES.run!(s::HIPSolver, Q; tfinal, nmax::Int=typemax(Int))
+"run!(solver, Q; tfinal, nmax, callback): Q (host, padded) is uploaded, advanced on the GPU and written back."
+function CompactLES.run!(s::HIPSolver, Q; tfinal, nmax::Int=typemax(Int), callback=nothing)
     pd = Int64.(collect(s.padded)); hl = Cint.(collect(s.halo))
     check(ccall((:cles_solver_upload, LIB), Cint, (Ptr{Cvoid}, Ptr{Cdouble}, Ptr{Int64}, Ptr{Cint}),
                 s.ptr, parent(Q), pd, hl))
-    check(ccall((:cles_run, LIB), Cint, (Ptr{Cvoid}, Cdouble, Int64, Ptr{Cdouble}, Int64),
-                s.ptr, tfinal, nmax, C_NULL, 0))
+    if callback === nothing
+        check(ccall((:cles_run, LIB), Cint, (Ptr{Cvoid}, Cdouble, Int64, Ptr{Cdouble}, Int64),
+                    s.ptr, tfinal, nmax, C_NULL, 0))
+    else
+        _run_hooked!(s, Q, tfinal, nmax, callback, pd, hl)
+    end
     check(ccall((:cles_solver_download, LIB), Cint, (Ptr{Cvoid}, Ptr{Cdouble}, Ptr{Int64}, Ptr{Cint}),
                 s.ptr, parent(Q), pd, hl))
     return Q
+end
+
+# The reference's own callback machinery (callbacks.jl: callback_next_time,
+# run_callbacks!, rewind_callbacks!) drives the GPU run through cles_run_hooks.
+# Triggers read `solver.t` / `solver.step` from a clock; effects get that clock
+# and Q refreshed from the device before they run.
+mutable struct HIPClock
+    t::Float64
+    step::Int
+    dt_prev::Float64
+    wall_step::Float64
+    wall_total::Float64
+end
+
+mutable struct _Hooked
+    cb::Any
+    clock::HIPClock
+    s::HIPSolver
+    Q::Any
+    pd::Vector{Int64}
+    hl::Vector{Cint}
+    err::Any
+end
+
+struct ClesRunHooks
+    user::Ptr{Cvoid}
+    next_time::Ptr{Cvoid}
+    after_step::Ptr{Cvoid}
+    rewind::Ptr{Cvoid}
+end
+
+function _hook_next(user::Ptr{Cvoid}, t::Cdouble, step::Int64)::Cdouble
+    h = unsafe_pointer_to_objref(user)::_Hooked
+    h.err === nothing || return Inf
+    h.clock.t = t; h.clock.step = step
+    try
+        return CompactLES.callback_next_time(h.cb, h.clock)
+    catch e
+        h.err = e
+        return Inf
+    end
+end
+
+function _hook_after(user::Ptr{Cvoid}, t::Cdouble, step::Int64, dt::Cdouble)::Cint
+    h = unsafe_pointer_to_objref(user)::_Hooked
+    h.err === nothing || return Cint(1)
+    h.clock.t = t; h.clock.step = step; h.clock.dt_prev = dt
+    try
+        check(ccall((:cles_solver_download, LIB), Cint, (Ptr{Cvoid}, Ptr{Cdouble}, Ptr{Int64}, Ptr{Cint}),
+                    h.s.ptr, parent(h.Q), h.pd, h.hl))
+        return CompactLES.run_callbacks!(h.cb, h.clock, h.Q) ? Cint(1) : Cint(0)
+    catch e
+        h.err = e
+        return Cint(1)
+    end
+end
+
+function _hook_rewind(user::Ptr{Cvoid}, t::Cdouble, step::Int64)::Cvoid
+    h = unsafe_pointer_to_objref(user)::_Hooked
+    CompactLES.rewind_callbacks!(h.cb, t, step)
+    return nothing
+end
+
+function _run_hooked!(s::HIPSolver, Q, tfinal, nmax, callback, pd, hl)
+    t = Ref{Cdouble}(0); st = Ref{Int64}(0)
+    check(ccall((:cles_solver_get_time, LIB), Cint, (Ptr{Cvoid}, Ptr{Cdouble}, Ptr{Int64}), s.ptr, t, st))
+    h = _Hooked(callback, HIPClock(t[], st[], 0.0, 0.0, 0.0), s, Q, pd, hl, nothing)
+    used = Ref{Cint}(0)
+    GC.@preserve h begin
+        hooks = Ref(ClesRunHooks(pointer_from_objref(h),
+                                 @cfunction(_hook_next, Cdouble, (Ptr{Cvoid}, Cdouble, Int64)),
+                                 @cfunction(_hook_after, Cint, (Ptr{Cvoid}, Cdouble, Int64, Cdouble)),
+                                 @cfunction(_hook_rewind, Cvoid, (Ptr{Cvoid}, Cdouble, Int64))))
+        rc = ccall((:cles_run_ctl, LIB), Cint,
+                   (Ptr{Cvoid}, Cdouble, Int64, Ref{ClesRunHooks}, Ptr{Cdouble}, Int64, Ref{Cint}),
+                   s.ptr, tfinal, nmax, hooks, C_NULL, 0, used)
+    end
+    h.err === nothing || throw(h.err)
+    check(rc)
+    return used[]
 end
 
 # ---------------------------------------------------------------------------
