@@ -170,7 +170,7 @@ def measure(ch, ctx, n, precision, steps, warmup, profile_steps, run_mode="host_
     }
 
 
-def measure_tubes(ch, ctx, count, steps, warmup, precision):
+def measure_tubes(ch, ctx, count, steps, warmup, precision, plain_sample=32):
     """The batched form: `count` independent one-dimensional two-gas shock
     tubes (N_i ~ U{128..2048}, p_L ~ U[2, 20], rho_L ~ U[0.1, 10] from one
     seeded generator; He | CO2, Dirichlet ends, art on, cfl 0.4, filter every
@@ -189,8 +189,26 @@ def measure_tubes(ch, ctx, count, steps, warmup, precision):
     b.close()
     ms = wall / steps * 1e3
     alg = alg_bytes_per_point_step(2, 1, precision) * npts
+    # the plain path on the same instances: a seeded sample of the tubes, each
+    # through its own single-domain solver, one after another
+    sample = specs[:plain_sample]
+    t_plain, pts = 0.0, 0
+    for sp in sample:
+        s1 = cases.tube_solver(ctx, sp, precision)
+        s1.run(1e9, warmup)
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        s1.run(1e9, warmup + steps)
+        ctx.synchronize()
+        t_plain += time.perf_counter() - t0
+        pts += sp["n"]
+        s1.close()
+    plain = pts * steps / t_plain / 1e6
     return {"value": round(npts * steps / wall / 1e6, 3), "unit": "Mpt-steps/s", "ms_per_step": round(ms, 4),
             "instance_steps_per_s": round(count * steps / wall, 1), "instances": count, "points": npts,
+            "plain_path_same_instances": {"value": round(plain, 3), "unit": "Mpt-steps/s",
+                                          "instances_sampled": len(sample),
+                                          "batched_over_plain": round(npts * steps / wall / 1e6 / plain, 1)},
             "all_ok": bool((status == 0).all()), "setup_s": round(setup_s, 2),
             "step_roofline": {"alg_bytes_per_point_step": alg_bytes_per_point_step(2, 1, precision),
                               "achieved_GBps": round(alg / (ms * 1e-3) / 1e9, 1),
@@ -300,9 +318,29 @@ def measure_serving(ch, ctx=None, count=32, rate_per_s=1.5, seed=20261202, budge
     sv.close()
     lat = np.array([d - s for s, a, d, fp in times])
     span = max(d for s, a, d, fp in times) - min(s for s, a, d, fp in times)
+    # throughput at saturation (every request submitted at once) against the
+    # plain path solving the same requests one after another
+    sv = ch.Server(device=0, hbm_budget=budget, runners=runners)
+    t0 = time.perf_counter()
+    ids2 = [sv.submit(prob, num, tfinal=tf, Q=Q) for prob, num, Q, tf in reqs]
+    ok = all(sv.wait(rid) == ch.server.DONE for rid in ids2) and ok
+    sat = time.perf_counter() - t0
+    sv.close()
+    seq = 0.0
+    if ctx is not None:
+        for prob, num, Q, tf in reqs:
+            t1 = time.perf_counter()
+            s1 = ch.Solver(ctx, prob, num)
+            s1.upload(Q)
+            s1.run(tf)
+            ctx.synchronize()
+            s1.close()
+            seq += time.perf_counter() - t1
     return {"solves_per_s": round(count / span, 2), "p50_s": round(float(np.percentile(lat, 50)), 4),
             "p99_s": round(float(np.percentile(lat, 99)), 4), "requests": count, "arrival_rate_per_s": rate_per_s,
             "all_done": ok, "hbm_budget_GB": budget / 2 ** 30, "runners": runners, "tuned_shapes": tuned,
+            "saturated_solves_per_s": round(count / sat, 2),
+            "plain_path_sequential_solves_per_s": round(count / seq, 2) if seq > 0 else None,
             "workload": "Poisson arrivals (seed 20261202): Taylor-Green 32^3..128^3 to t = 0.05 and He|CO2 "
                         "shock / interface 256^2..1024x512 (NSCBC in / out) to t = 0.03, FP64"}
 

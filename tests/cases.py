@@ -202,6 +202,19 @@ def tube_oracle(spec, precision_art=True, cfl=0.4):
                          bcs=((O.BC_DIRICHLET, O.BC_DIRICHLET), (0, 0), (0, 0)), bc_state={(0, 0): qL, (0, 1): qR})
 
 
+def tube_solver(ctx, spec, precision="fp64", art=True, cfl=0.4):
+    """One tube through the plain single-domain solver (Dirichlet ends holding
+    the same end states as the batch's instance)."""
+    import compactles_hip as ch
+    q, _, L, h = tube_state(spec)
+
+    def end(rho, p, y0):
+        return lambda X, Y, Z, t: ch.Prim(rho=rho + 0 * X, p=p + 0 * X, Y=(y0 + 0 * X, 1.0 - y0 + 0 * X))
+    return gpu_solver(ctx, (spec["n"], 1, 1), (L, h, h), q, species=HE_CO2, art=art, cfl=cfl,
+                      bcs=((1, 1), (0, 0), (0, 0)), precision=precision,
+                      bc_funs={(0, 0): end(spec["rhoL"], spec["pL"], 1.0), (0, 1): end(spec["rhoR"], spec["pR"], 0.0)})
+
+
 def tube_batch(ctx, specs, precision="fp64", tfinal=None, nmax=2 ** 62, art=True, cfl=0.4):
     """A GPU Batch of the tubes, uploaded with the same initial arrays."""
     import compactles_hip as ch
