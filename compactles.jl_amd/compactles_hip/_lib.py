@@ -102,6 +102,7 @@ SIGNATURES = {
     "cles_solver_save_checkpoint": (C.c_int, [P, C.c_char_p, C.c_int]),
     "cles_solver_load_checkpoint": (C.c_int, [P, C.c_char_p, C.c_int]),
     "cles_solver_footprint": (C.c_int, [C.POINTER(SolverDesc), C.POINTER(C.c_size_t)]),
+    "cles_solver_create_ex": (C.c_int, [P, C.POINTER(SolverDesc), C.POINTER(DP), PP]),
     "cles_solver_memory": (C.c_int, [P, C.POINTER(C.c_size_t)]),
     "cles_server_create": (C.c_int, [C.c_int, C.c_size_t, C.c_int, PP]),
     "cles_server_destroy": (C.c_int, [P]),

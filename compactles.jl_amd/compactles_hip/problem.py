@@ -75,6 +75,25 @@ class ArtParams:
     beta_sensor: str = "strain"
 
 
+class Stretch:
+    """metric.jl:45-58: a monotone map x(xi) of the computational coordinate
+    xi in [0, 1] onto a non-periodic dimension, with dxdxi its derivative."""
+
+    def __init__(self, x, dxdxi):
+        self.x, self.dxdxi = x, dxdxi
+
+
+def sine_cluster(lo, hi, xic, a):
+    """metric.jl:60-75: x(xi) = lo + L(xi - (a/2pi)[sin 2pi(xi - xic) + sin 2pi xic]),
+    dx/dxi = L(1 - a cos 2pi(xi - xic)); 0 <= a < 1."""
+    if not 0 <= a < 1:
+        raise ValueError("sine_cluster: need 0 ≤ a < 1")
+    L = hi - lo
+    return Stretch(lambda xi: lo + L * (xi - (a / (2 * math.pi)) * (np.sin(2 * math.pi * (xi - xic)) +
+                                                                 math.sin(2 * math.pi * xic))),
+                   lambda xi: L * (1 - a * np.cos(2 * math.pi * (xi - xic))))
+
+
 @dataclass
 class StepControl:
     """stepcontrol.jl:116-136: prediction, growth limit, soft landing, floors,
@@ -260,8 +279,9 @@ class SolverSpec:
     def __init__(self, prob: Problem, num: Numerics):
         if prob.metric != "cartesian":
             raise ValueError("only CartesianMetric is available on the GPU backend")
-        if num.stretch is not None:
-            raise ValueError("stretched grids are not available on the GPU backend")
+        stretch = tuple(num.stretch) if num.stretch is not None else (None, None, None)
+        if len(stretch) != 3:
+            raise ValueError("Numerics.stretch: one entry per dimension")
         if prob.sources:
             raise ValueError("explicit sources are not available on the GPU backend")
         self.prob, self.num = prob, num
@@ -274,8 +294,28 @@ class SolverSpec:
         self.n_species = self.eos.nspecies()
         self.n_cons = self.n_species + 4
         self.periodic = tuple(isinstance(prob.bcs[d][0], PeriodicBC) or self.n[d] == 1 for d in range(3))
-        self.h = tuple(1.0 if self.n[d] == 1 else (self.L[d] / self.n[d] if self.periodic[d]
-                                                   else self.L[d] / (self.n[d] - 1)) for d in range(3))
+        self.stretch = stretch
+        for d, st in enumerate(stretch):
+            if st is None:
+                continue
+            lo, hi = prob.domain[d]
+            # problem.jl:324-332
+            if not (abs(st.x(0.0) - lo) <= 1e-10 * self.L[d] and abs(st.x(1.0) - hi) <= 1e-10 * self.L[d]):
+                raise ValueError(f"stretch mapping for dim {d + 1} does not span the domain: "
+                                 f"x(0) = {st.x(0.0)}, x(1) = {st.x(1.0)}, domain = {prob.domain[d]}")
+            if self.n[d] == 1 or self.periodic[d]:
+                raise ValueError(f"dimension {d + 1}: stretched dimensions must be non-periodic")
+        # rhs.jl:191-201: stretched dimensions are uniform in xi on [0, 1]
+        self.h = tuple(1.0 if self.n[d] == 1 else
+                       (1.0 / (self.n[d] - 1) if stretch[d] is not None else
+                        (self.L[d] / self.n[d] if self.periodic[d] else self.L[d] / (self.n[d] - 1)))
+                       for d in range(3))
+        # the mapping Jacobian at the nodes (metric.jl:100-107), handed to the device
+        self.jac = tuple(None if stretch[d] is None else
+                         np.ascontiguousarray(np.broadcast_to(
+                             np.asarray(stretch[d].dxdxi(np.clip(np.arange(self.n[d]) * self.h[d], 0.0, 1.0)),
+                                        dtype=np.float64), (self.n[d],)))
+                         for d in range(3))
         self.halo = tuple(num.n_halo if self.n[d] > 1 else 0 for d in range(3))
         self.padded = tuple(self.n[d] + 2 * self.halo[d] for d in range(3))
         self.desc = self._desc()
@@ -314,7 +354,10 @@ class SolverSpec:
         return d
 
     def xcoord(self, d: int) -> np.ndarray:
-        """rhs.jl:320-335: node coordinates along dimension d (0-based here)."""
+        """rhs.jl:320-335: node coordinates along dimension d (0-based here),
+        mapped through the dimension's Stretch when it has one."""
+        if self.stretch[d] is not None:
+            return np.asarray(self.stretch[d].x(np.arange(self.n[d]) * self.h[d]), dtype=np.float64)
         return self.origin[d] + np.arange(self.n[d]) * self.h[d]
 
     def _dirichlet_state(self, bc: DirichletBC, d: int, side: int):
@@ -348,7 +391,11 @@ class Solver(SolverSpec):
         super().__init__(prob, num)
         self.ctx = ctx
         h = C.c_void_p()
-        L.check(ctx.lib.cles_solver_create(ctx.handle, C.byref(self.desc), C.byref(h)))
+        if any(j is not None for j in self.jac):
+            ptrs = (L.DP * 3)(*[C.cast(None, L.DP) if j is None else j.ctypes.data_as(L.DP) for j in self.jac])
+            L.check(ctx.lib.cles_solver_create_ex(ctx.handle, C.byref(self.desc), ptrs, C.byref(h)))
+        else:
+            L.check(ctx.lib.cles_solver_create(ctx.handle, C.byref(self.desc), C.byref(h)))
         self.handle = h
 
     def save_checkpoint(self, prefix: str, rank: int = 0) -> str:

@@ -26,6 +26,9 @@ class Server:
         """Queue setup(prob, num) + run!(...; tfinal, nmax); Q (padded host
         state) defaults to prob.ic on the grid.  Returns the request id."""
         spec = SolverSpec(prob, num)
+        if any(j is not None for j in spec.jac):
+            raise ValueError("stretched grids are not available through the server (cles_server_submit takes "
+                             "a descriptor only); run them with a Solver")
         if Q is None:
             Q = initialize_(spec, spec.allocate_state(), prob.ic)
         Q = np.asfortranarray(Q, dtype=np.float64)

@@ -48,12 +48,22 @@ struct FluxArgs {
     W *flux[3][CLES_MAXSP + 4];
 };
 
+// Cartesian stretch (metric.jl:109-150 with the Cartesian metric): mapping
+// Jacobians m_d (dx/dxi) at the interior nodes of each stretched dimension
+// (NULL: uniform).  Scale factor h_d = max(m_d, tiny), J = h1 h2 h3.
+struct Metric1D {
+    const double *m[3];
+    int64_t n[3];
+};
+
 template <typename W>
 struct RateArgs {
     const double *Q;
     int Ns;
     int active[3];
     double invh[3];
+    double h[3];             // spacing of each dimension (stretched: computational)
+    Metric1D met;            // stretched dimensions use inv_h[d] / h[d] (timestep.jl:151)
     double mu0, Pr, Sc;
     const W *rho, *u, *v, *w, *c, *cp;
     const W *mu_art, *beta_art, *kappa_art;
@@ -126,7 +136,20 @@ struct DiagArgs {
     int64_t N;
     int closed[3];
     GradPtrs<W> g;
+    Metric1D met;            // stretched: integrand / inv_J (diagnostics.jl:73-75)
 };
+
+// Metric scalings over nf fields of N points (interior, x fastest):
+// mode 0: f *= inv_h[dim] (rhs.jl:477-483 _scale_grad!); mode 1: f *= J / h_dim
+// (the area factor A_dim, rhs.jl:672-677); mode 2: f *= inv_J (rhs.jl:680-682).
+constexpr int METRIC_MAXF = 12;
+template <typename W>
+struct MetricFields {
+    W *f[METRIC_MAXF];
+    int nf;
+};
+template <typename W>
+void launch_metric_scale(const MetricFields<W> &fs, const Metric1D &g, int mode, int dim, int64_t N, hipStream_t st);
 
 template <typename W> void launch_primitives(const PrimArgs<W> &a, int64_t N, hipStream_t st);
 template <typename W> void launch_strain(const GradPtrs<W> &g, W *out, int64_t N, hipStream_t st);

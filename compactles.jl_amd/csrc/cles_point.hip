@@ -376,6 +376,34 @@ __global__ void __launch_bounds__(TPB) k_lsrk(double *__restrict__ Q, double *__
 // mixture density, then a single-block final reduction (order-independent).
 // ---------------------------------------------------------------------------
 // timestep.jl:146-170: the rate of point I (the mixture density goes to rQ)
+__device__ __forceinline__ double metric_h(const Metric1D &g, int d, int64_t i)
+{
+    return g.m[d] ? fmax(1.0 * g.m[d][i], 1e-300) : 1.0;
+}
+__device__ __forceinline__ void interior_ijk(const int64_t (&n)[3], int64_t I, int64_t (&ii)[3])
+{
+    ii[0] = I % n[0];
+    ii[1] = (I / n[0]) % n[1];
+    ii[2] = I / (n[0] * n[1]);
+}
+
+template <typename W, int MODE>
+__global__ void __launch_bounds__(TPB) k_metric_scale(MetricFields<W> fs, Metric1D g, int dim, int64_t N)
+{
+    for (int64_t I = (int64_t)blockIdx.x * TPB + threadIdx.x; I < N; I += (int64_t)gridDim.x * TPB) {
+        int64_t ii[3];
+        interior_ijk(g.n, I, ii);
+        double sc;
+        if constexpr (MODE == 0) {
+            sc = 1 / metric_h(g, dim, ii[dim]);
+        } else {
+            const double J = metric_h(g, 0, ii[0]) * metric_h(g, 1, ii[1]) * metric_h(g, 2, ii[2]);
+            sc = MODE == 1 ? J / metric_h(g, dim, ii[dim]) : 1 / J;
+        }
+        for (int f = 0; f < fs.nf; f++) fs.f[f][I] = (W)((double)fs.f[f][I] * sc);
+    }
+}
+
 template <typename W>
 __device__ __forceinline__ double rate_at(const RateArgs<W> &a, int64_t I, int64_t N, double &rQ)
 {
@@ -389,7 +417,12 @@ __device__ __forceinline__ double rate_at(const RateArgs<W> &a, int64_t I, int64
 #pragma unroll
     for (int d = 0; d < 3; d++) {
         if (!a.active[d]) continue;
-        const double idx = a.invh[d];
+        double idx = a.invh[d];
+        if (a.met.m[d]) {   // timestep.jl:151 inv_h[d][I] / h[d]
+            int64_t ii[3];
+            interior_ijk(a.met.n, I, ii);
+            idx = (1 / metric_h(a.met, d, ii[d])) / a.h[d];
+        }
         acc += (fabs(uv[d]) + c) * idx;
         dsum += idx * idx;
     }
@@ -660,7 +693,13 @@ __global__ void __launch_bounds__(TPB) k_integrand_sum(DiagArgs<W> a, double *__
             const double w1 = G(1, 2) - G(2, 1), w2 = G(2, 0) - G(0, 2), w3 = G(0, 1) - G(1, 0);
             f = 0.5 * (w1 * w1 + w2 * w2 + w3 * w3);
         }
-        acc += wq * f;
+        if (a.met.m[0] || a.met.m[1] || a.met.m[2]) {   // diagnostics.jl:73-75: f / inv_J
+            const double inv_J = 1 / (metric_h(a.met, 0, ii[0]) * metric_h(a.met, 1, ii[1]) *
+                                      metric_h(a.met, 2, ii[2]));
+            acc += wq * f / inv_J;
+        } else {
+            acc += wq * f;
+        }
     }
     sh[threadIdx.x] = acc;
     __syncthreads();
@@ -1087,5 +1126,24 @@ CLES_INST(float)
 template void launch_convert<double, float>(double *, const float *, int64_t, hipStream_t);
 template void launch_convert<float, double>(float *, const double *, int64_t, hipStream_t);
 template void launch_convert<double, double>(double *, const double *, int64_t, hipStream_t);
+
+template <typename W>
+void launch_metric_scale(const MetricFields<W> &fs, const Metric1D &g, int mode, int dim, int64_t N, hipStream_t st)
+{
+    if (fs.nf <= 0 || N <= 0) return;
+    const unsigned nb = nblocks(N);
+    if (mode == 0)
+        hipLaunchKernelGGL((k_metric_scale<W, 0>), dim3(nb), dim3(TPB), 0, st, fs, g, dim, N);
+    else if (mode == 1)
+        hipLaunchKernelGGL((k_metric_scale<W, 1>), dim3(nb), dim3(TPB), 0, st, fs, g, dim, N);
+    else
+        hipLaunchKernelGGL((k_metric_scale<W, 2>), dim3(nb), dim3(TPB), 0, st, fs, g, dim, N);
+    CHECK_LAUNCH();
+}
+template void launch_metric_scale<double>(const MetricFields<double> &, const Metric1D &, int, int, int64_t,
+                                          hipStream_t);
+template void launch_metric_scale<float>(const MetricFields<float> &, const Metric1D &, int, int, int64_t,
+                                         hipStream_t);
+
 
 }  // namespace cles

@@ -42,6 +42,10 @@ struct cles_solver {
     bool nscbc = false;
     cles::HostPlan dplan[3], fplan[3], fplanQ[3];
     bool own_fplanQ = false;
+    // Stretch (metric.jl:45-58): mapping Jacobians at the nodes of stretched
+    // dimensions, on the device (NULL: uniform)
+    bool stretched = false;
+    cles::Metric1D met{};
     std::vector<void *> allocs;
     size_t bytes = 0;                          // device bytes allocated by this solver
     // device state (W-typed pointers are void* here; the Impl casts them)
@@ -294,7 +298,18 @@ struct Impl {
                     }
             }
             lines(s.dplan[d], d, nf, in, out, nullptr, store(out[0]), PH_DERIV);
+            if (s.met.m[d]) metric(out, nf, 0, d);   // rhs.jl:599,631,634 _scale_grad!
         }
+    }
+
+    // metric scalings of nf fields (launch_metric_scale modes)
+    void metric(void *const *fields, int nf, int mode, int d) const
+    {
+        MetricFields<W> fs{};
+        fs.nf = nf;
+        for (int i = 0; i < nf; i++) fs.f[i] = f(fields[i]);
+        s.timed(PH_OTHER, 2.0 * sizeof(W) * nf * (double)s.N,
+                [&] { launch_metric_scale<W>(fs, s.met, mode, d, s.N, s.st()); });
     }
 
     // artificial.jl:98-111 smooth! in place, the last pass with epilogue `last`
@@ -439,6 +454,8 @@ struct Impl {
                 out[c] = dQc(c);
                 eoff[c] = (int64_t)c * s.N;
             }
+            // rhs.jl:672-677: the divergence of A_d F_d (the fluxes are scaled in place)
+            if (s.stretched) metric(s.flux[d], s.C, 1, d);
             EpiArgs e{};
             if (fuse && last) {
                 e.kind = s.dev_dt ? EPI_LSRK_DEV : EPI_LSRK;
@@ -456,6 +473,11 @@ struct Impl {
                 lines(s.dplan[d], d, s.C, in, out, nullptr, e, PH_DIV);
             }
         }
+        if (s.stretched) {   // rhs.jl:680-682: dQ -= inv_J D_d(A_d F_d)
+            void *dq[LINE_MF];
+            for (int c = 0; c < s.C; c++) dq[c] = dQc(c);
+            metric(dq, s.C, 2, 0);
+        }
     }
 
     // timestep.jl:125-175
@@ -468,7 +490,9 @@ struct Impl {
         for (int k = 0; k < 3; k++) {
             a.active[k] = s.active[k];
             a.invh[k] = 1.0 / s.h[k];
+            a.h[k] = s.h[k];
         }
+        a.met = s.met;
         a.mu0 = s.d.mu0;
         a.Pr = s.d.Pr;
         a.Sc = s.d.Sc;
@@ -505,7 +529,9 @@ struct Impl {
         for (int k = 0; k < 3; k++) {
             a.active[k] = s.active[k];
             a.invh[k] = 1.0 / s.h[k];
+            a.h[k] = s.h[k];
         }
+        a.met = s.met;
         a.mu0 = s.d.mu0;
         a.Pr = s.d.Pr;
         a.Sc = s.d.Sc;
@@ -532,7 +558,7 @@ struct Impl {
     // timestep.jl:72-99 step!
     void step(double dt, bool prepared) const
     {
-        const bool fuse = !s.nscbc;
+        const bool fuse = !s.nscbc && !s.stretched;
         for (int stg = 0; stg < 5; stg++) {
             s.tstage = s.t + RKC[stg] * dt;
             const bool first_prepared = prepared && stg == 0;
@@ -662,6 +688,7 @@ struct Impl {
             a.closed[k] = s.active[k] && !s.periodic[k];
         }
         a.N = s.N;
+        a.met = s.met;
         double dV = 1.0;
         for (int k = 0; k < 3; k++) dV *= s.active[k] ? s.h[k] : 1.0;
         int nb = (int)std::min<int64_t>(s.nred, (s.N + 255) / 256);
@@ -866,9 +893,31 @@ int cles_solver_memory(cles_solver *s, size_t *bytes)
 
 int cles_solver_create(cles_ctx *ctx, const cles_solver_desc *desc, cles_solver **out)
 {
+    return cles_solver_create_ex(ctx, desc, nullptr, out);
+}
+
+int cles_solver_create_ex(cles_ctx *ctx, const cles_solver_desc *desc, const double *const dxdxi[3],
+                          cles_solver **out)
+{
     CLES_TRY({
         CLES_REQUIRE(ctx && desc && out, "NULL argument");
         validate_desc(*desc);
+        bool any_stretch = false, any_nscbc = false;
+        for (int k = 0; k < 3; k++) {
+            for (int sd = 0; sd < 2; sd++)
+                any_nscbc = any_nscbc || ((desc->bc[k][sd] == CLES_BC_NSCBC_IN || desc->bc[k][sd] == CLES_BC_NSCBC_OUT) &&
+                                          desc->n_global[k] > 1);
+            if (!dxdxi || !dxdxi[k]) continue;
+            any_stretch = true;
+            // rhs.jl:186-187
+            CLES_REQUIRE(desc->n_global[k] > 1 && desc->bc[k][0] != CLES_BC_PERIODIC,
+                         "dimension " + std::to_string(k + 1) + ": stretched dimensions must be non-periodic");
+            for (int64_t i = 0; i < desc->n_global[k]; i++)
+                CLES_REQUIRE(dxdxi[k][i] > 0 && std::isfinite(dxdxi[k][i]),
+                             "stretch mapping for dim " + std::to_string(k + 1) + " is not monotone increasing");
+        }
+        if (any_stretch && any_nscbc)
+            throw cles::Error{CLES_ERR_UNSUPPORTED, "NSCBC faces on stretched grids are not available on the GPU backend"};
         CLES_HIP(hipSetDevice(ctx->device));
         auto *s = new cles_solver;
         try {
@@ -892,6 +941,10 @@ int cles_solver_create(cles_ctx *ctx, const cles_solver_desc *desc, cles_solver 
                 for (int sd = 0; sd < 2; sd++)
                     if (desc->bc[k][sd] == CLES_BC_NSCBC_IN || desc->bc[k][sd] == CLES_BC_NSCBC_OUT)
                         s->nscbc = s->active[k] || s->nscbc;
+                if (dxdxi && dxdxi[k]) {   // rhs.jl:191-201: computational spacing on [0, 1]
+                    s->h[k] = 1.0 / (double)(desc->n_global[k] - 1);
+                    s->stretched = true;
+                }
             }
             for (int k = 0; k < s->Ns; k++) {   // physics.jl:111-115
                 s->Rk[k] = desc->R[k];
@@ -953,6 +1006,13 @@ int cles_solver_create(cles_ctx *ctx, const cles_solver_desc *desc, cles_solver 
             }
             if (s->nscbc)
                 for (void *&fp : s->nsd) fp = s->wfield();
+            for (int k = 0; k < 3; k++) {
+                s->met.n[k] = s->n[k];
+                if (!(dxdxi && dxdxi[k])) continue;
+                double *m = (double *)s->alloc((size_t)s->n[k] * sizeof(double));
+                CLES_HIP(hipMemcpy(m, dxdxi[k], (size_t)s->n[k] * sizeof(double), hipMemcpyHostToDevice));
+                s->met.m[k] = m;
+            }
             s->part = (double *)s->alloc((size_t)s->nred * 2 * sizeof(double));
             s->scal = (double *)s->alloc(16 * sizeof(double));
             CLES_HIP(hipHostMalloc((void **)&s->host_scal, 16 * sizeof(double), hipHostMallocDefault));

@@ -198,6 +198,17 @@ int cles_solver_footprint(const cles_solver_desc *desc, size_t *bytes);
 /* setup(prob, num) -> Solver (src/problem.jl:321-345, src/rhs.jl:122-311),
  * one rank.  Allocates the device-resident state and every scratch field. */
 int cles_solver_create(cles_ctx *ctx, const cles_solver_desc *desc, cles_solver **solver);
+/* setup with Stretch mappings (src/metric.jl:45-58 with CartesianMetric):
+ * dxdxi[d] (NULL, or dxdxi itself NULL, for a uniform dimension) holds dx/dxi
+ * at the n_global[d] nodes xi = i / (n_global[d] - 1) of a stretched,
+ * non-periodic dimension.  That dimension's spacing is then 1 / (n - 1)
+ * (rhs.jl:191-201); gradients are scaled by inv_h (rhs.jl:477-483), the flux
+ * divergence takes the general path dQ -= inv_J D_d(A_d F_d) (rhs.jl:665-684),
+ * max_rate uses inv_h[d] / h[d] (timestep.jl:151) and the volume integrals
+ * divide by inv_J (diagnostics.jl:73-75).  NSCBC faces are refused on
+ * stretched grids (CLES_ERR_UNSUPPORTED). */
+int cles_solver_create_ex(cles_ctx *ctx, const cles_solver_desc *desc, const double *const dxdxi[3],
+                          cles_solver **solver);
 int cles_solver_destroy(cles_solver *solver);
 
 /* Conserved state transfer in Julia's padded layout (Q[..., c], column-major,

@@ -550,6 +550,8 @@ struct or_solver {
     double *flux[3][OR_MAXSP + 4];
     double t, tstage, dt_prev, rate_prev;
     int64_t step;
+    double *mjac[3];                     /* Stretch dx/dxi at the interior nodes (NULL: uniform) */
+    int stretched;
 };
 
 #define FOR_INTERIOR(s, I)                                                        \
@@ -564,7 +566,52 @@ static inline int64_t pidx(const or_solver *s, int64_t i, int64_t j, int64_t k)
 
 static double *fld(or_solver *s) { return calloc((size_t)s->NP, sizeof(double)); }
 
+/* metric.jl:109-150 (Cartesian): the scale factor of dimension d at padded
+ * index ip along d is max(1 * m_d, tiny), m_d the mapping Jacobian at the
+ * node (clamped into the map's domain for halo layers) or 1 when uniform. */
+static inline double hmet(const or_solver *s, int d, int64_t ip)
+{
+    if (!s->mjac[d]) return 1.0;
+    int64_t i = ip - s->H[d];
+    if (i < 0) i = 0;
+    if (i > s->n[d] - 1) i = s->n[d] - 1;
+    return fmax(1.0 * s->mjac[d][i], 1e-300);
+}
+static inline void padded_ijk(const or_solver *s, int64_t I, int64_t *ip)
+{
+    ip[0] = I % s->pad[0];
+    ip[1] = (I / s->pad[0]) % s->pad[1];
+    ip[2] = I / (s->pad[0] * s->pad[1]);
+}
+static inline double jac_at(const or_solver *s, int64_t I)
+{
+    int64_t ip[3];
+    padded_ijk(s, I, ip);
+    return hmet(s, 0, ip[0]) * hmet(s, 1, ip[1]) * hmet(s, 2, ip[2]);
+}
+
+/* rhs.jl:477-483 _scale_grad!: g *= inv_h[d] over the full array */
+static void scale_grad(or_solver *s, double *g, int d)
+{
+    if (!s->mjac[d]) return;   /* inv_h == 1 */
+#pragma omp parallel for schedule(static)
+    for (int64_t I = 0; I < s->NP; I++) {
+        int64_t ip[3];
+        padded_ijk(s, I, ip);
+        const double ih = 1 / hmet(s, d, ip[d]);
+        g[I] *= ih;
+    }
+}
+
 or_solver *or_solver_create(const or_desc *d, char *msg, int msglen)
+{
+    return or_solver_create_ex(d, NULL, msg, msglen);
+}
+
+/* setup with Stretch mappings (metric.jl:45-58): jac[d] holds dx/dxi at the
+ * n_global[d] nodes xi = i / (n - 1) of a stretched (non-periodic) dimension;
+ * its computational spacing is h = 1 / (n - 1) (rhs.jl:191-201). */
+or_solver *or_solver_create_ex(const or_desc *d, const double *const jac[3], char *msg, int msglen)
 {
     or_solver *s = calloc(1, sizeof(or_solver));
     s->d = *d;
@@ -584,6 +631,17 @@ or_solver *or_solver_create(const or_desc *d, char *msg, int msglen)
         if (!s->active[k]) s->h[k] = 1.0;
         else s->h[k] = s->periodic[k] ? d->L_domain[k] / (double)d->n_global[k]
                                       : d->L_domain[k] / (double)(d->n_global[k] - 1);
+        if (jac && jac[k]) {
+            if (!s->active[k] || s->periodic[k]) {
+                snprintf(msg, (size_t)msglen, "dimension %d: stretched dimensions must be non-periodic", k + 1);
+                free(s);
+                return NULL;
+            }
+            s->h[k] = 1.0 / (double)(d->n_global[k] - 1);
+            s->mjac[k] = malloc(sizeof(double) * (size_t)s->n[k]);
+            memcpy(s->mjac[k], jac[k], sizeof(double) * (size_t)s->n[k]);
+            s->stretched = 1;
+        }
     }
     /* physics.jl:111-115 */
     for (int k = 0; k < s->Ns; k++) {
@@ -631,6 +689,7 @@ void or_solver_destroy(or_solver *s)
     }
     free(s->mu_art); free(s->beta_art); free(s->kappa_art);
     free(s->strain_mag); free(s->sensor); free(s->sensor_sp); free(s->tmp_a); free(s->tmp_b);
+    for (int k = 0; k < 3; k++) free(s->mjac[k]);
     free(s);
 }
 
@@ -768,8 +827,12 @@ static void primitives_and_gradients(or_solver *s, int primitives_current)
     double *vel[3] = {s->u, s->v, s->w};
     for (int jj = 0; jj < 3; jj++)
         for (int d = 0; d < 3; d++) {
-            if (s->active[d]) deriv_along(s, s->gu[d * 3 + jj], vel[jj], d);
-            else memset(s->gu[d * 3 + jj], 0, sizeof(double) * s->NP);
+            if (s->active[d]) {
+                deriv_along(s, s->gu[d * 3 + jj], vel[jj], d);
+                scale_grad(s, s->gu[d * 3 + jj], d);
+            } else {
+                memset(s->gu[d * 3 + jj], 0, sizeof(double) * s->NP);
+            }
         }
     /* metric_correct_gradients! is a no-op for CartesianMetric */
 }
@@ -1082,7 +1145,11 @@ void or_compute_rhs(or_solver *s, int primitives_current)
     for (int d = 0; d < 3; d++) {
         if (!s->active[d]) continue;
         deriv_along(s, s->gT[d], s->T, d);
-        for (int sp = 0; sp < s->Ns; sp++) deriv_along(s, s->gY[d][sp], s->Y[sp], d);
+        scale_grad(s, s->gT[d], d);
+        for (int sp = 0; sp < s->Ns; sp++) {
+            deriv_along(s, s->gY[d][sp], s->Y[sp], d);
+            scale_grad(s, s->gY[d][sp], d);
+        }
     }
     assemble_fluxes(s);
     for (int d = 0; d < 3; d++)
@@ -1092,13 +1159,35 @@ void or_compute_rhs(or_solver *s, int primitives_current)
         FOR_INTERIOR(s, I) dq[I] = 0;
         for (int d = 0; d < 3; d++) {
             if (!s->active[d]) continue;
-            deriv_along(s, s->tmp_a, s->flux[d][c], d);
+            if (!s->stretched) {
+                deriv_along(s, s->tmp_a, s->flux[d][c], d);
+#pragma omp parallel for schedule(static)
+                for (int64_t k = 0; k < s->n[2]; k++)
+                    for (int64_t j = 0; j < s->n[1]; j++)
+                        for (int64_t i = 0; i < s->n[0]; i++) {
+                            int64_t I = pidx(s, i, j, k);
+                            dq[I] -= s->tmp_a[I];
+                        }
+                continue;
+            }
+            /* rhs.jl:672-684: tmp_b = A_d F_d (A_d = J / h_d), dQ -= inv_J D_d tmp_b */
+            const double *F = s->flux[d][c];
+#pragma omp parallel for schedule(static)
+            for (int64_t I = 0; I < s->NP; I++) {
+                int64_t ip[3];
+                padded_ijk(s, I, ip);
+                const double J = hmet(s, 0, ip[0]) * hmet(s, 1, ip[1]) * hmet(s, 2, ip[2]);
+                const double Ad = J / hmet(s, d, ip[d]);
+                s->tmp_b[I] = Ad * F[I];
+            }
+            deriv_along(s, s->tmp_a, s->tmp_b, d);
 #pragma omp parallel for schedule(static)
             for (int64_t k = 0; k < s->n[2]; k++)
                 for (int64_t j = 0; j < s->n[1]; j++)
                     for (int64_t i = 0; i < s->n[0]; i++) {
                         int64_t I = pidx(s, i, j, k);
-                        dq[I] -= s->tmp_a[I];
+                        const double inv_J = 1 / jac_at(s, I);
+                        dq[I] -= inv_J * s->tmp_a[I];
                     }
         }
     }
@@ -1153,7 +1242,14 @@ void or_max_rate(or_solver *s, double *rate_out, double *rho_min_out)
         double acc = 0.0, dsum = 0.0;
         for (int dd = 0; dd < 3; dd++) {
             if (!s->active[dd]) continue;
-            double idx = 1.0 / s->h[dd];       /* inv_h[d] / h[d] with inv_h == 1 */
+            double idx;                        /* inv_h[d] / h[d] */
+            if (s->mjac[dd]) {
+                int64_t ip[3];
+                padded_ijk(s, I, ip);
+                idx = (1 / hmet(s, dd, ip[dd])) / s->h[dd];
+            } else {
+                idx = 1.0 / s->h[dd];
+            }
             acc += (fabs(uv[dd]) + c) * idx;
             dsum += idx * idx;
         }
@@ -1327,7 +1423,8 @@ double or_volume_integral(const or_solver *s, const double *f)
             double wj = wk * quad_weight(s, 1, j);
             for (int64_t i = 0; i < s->n[0]; i++) {
                 int64_t I = pidx(s, i, j, k);
-                acc += wj * quad_weight(s, 0, i) * f[I] / 1.0;
+                const double inv_J = s->stretched ? 1 / jac_at(s, I) : 1.0;
+                acc += wj * quad_weight(s, 0, i) * f[I] / inv_J;
             }
         }
     }

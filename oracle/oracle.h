@@ -106,6 +106,9 @@ typedef struct {
 typedef struct or_solver or_solver;
 
 or_solver *or_solver_create(const or_desc *d, char *msg, int msglen);
+/* With Stretch mappings: jac[d] (NULL: uniform) holds dx/dxi at the
+ * n_global[d] nodes of a stretched, non-periodic dimension. */
+or_solver *or_solver_create_ex(const or_desc *d, const double *const jac[3], char *msg, int msglen);
 void or_solver_destroy(or_solver *s);
 void or_solver_dims(const or_solver *s, int64_t padded[3], int halo[3], int *n_cons);
 /* Q in the padded layout (padded[0]*padded[1]*padded[2]*n_cons doubles). */

@@ -85,6 +85,7 @@ def _load():
         "or_apply_along": (None, [P, D, D]),
         "or_wrap_halos": (None, [D, I64, I, I]),
         "or_solver_create": (P, [C.POINTER(Desc), C.c_char_p, C.c_int]),
+        "or_solver_create_ex": (P, [C.POINTER(Desc), C.POINTER(D), C.c_char_p, C.c_int]),
         "or_solver_destroy": (None, [P]),
         "or_solver_dims": (None, [P, I64, I, I]),
         "or_solver_Q": (D, [P]),
@@ -197,9 +198,16 @@ class Plan:
 class Solver:
     """The reference Solver + Workspace restated on padded arrays."""
 
-    def __init__(self, desc: Desc):
+    def __init__(self, desc: Desc, jac=None):
+        """jac: per dimension None or dx/dxi at the nodes of a stretched dimension."""
         msg = C.create_string_buffer(256)
-        self._s = lib().or_solver_create(C.byref(desc), msg, 256)
+        if jac is None or all(j is None for j in jac):
+            self._s = lib().or_solver_create(C.byref(desc), msg, 256)
+        else:
+            self._jac = [None if j is None else np.ascontiguousarray(j, dtype=np.float64) for j in jac]
+            DP = C.POINTER(C.c_double)
+            ptrs = (DP * 3)(*[C.cast(None, DP) if j is None else j.ctypes.data_as(DP) for j in self._jac])
+            self._s = lib().or_solver_create_ex(C.byref(desc), ptrs, msg, 256)
         if not self._s:
             raise ValueError(msg.value.decode())
         self.desc = desc
