@@ -14,3 +14,4 @@ from .batch import Batch, Instance  # noqa: F401
 from .server import Server  # noqa: F401
 from . import tune  # noqa: F401
 from .callbacks import AtTime, Callback, EveryStep, EveryTime, ProgressLog, WhenState  # noqa: F401
+from .nasa9 import Nasa9Interval, Nasa9Mixture, Nasa9Species, nasa9_constant_cp, read_nasa9  # noqa: F401

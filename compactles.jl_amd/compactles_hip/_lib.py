@@ -34,6 +34,17 @@ class NscbcDesc(C.Structure):
                 ("eta_u", C.c_double), ("eta_T", C.c_double), ("eta_t", C.c_double), ("eta_Y", C.c_double)]
 
 
+class Nasa9IntervalDesc(C.Structure):
+    _fields_ = [("Tmin", C.c_double), ("Tmax", C.c_double), ("a", C.c_double * 7), ("b1", C.c_double)]
+
+
+class Nasa9SpeciesDesc(C.Structure):
+    _fields_ = [("n_intervals", C.c_int), ("iv", Nasa9IntervalDesc * 3)]
+
+
+EOS_IDEAL, EOS_NASA9 = 0, 1
+
+
 class SolverDesc(C.Structure):
     _fields_ = [("n_global", C.c_int64 * 3), ("L_domain", C.c_double * 3), ("origin", C.c_double * 3),
                 ("n_species", C.c_int), ("R", C.c_double * MAXSP), ("gamma", C.c_double * MAXSP),
@@ -45,7 +56,8 @@ class SolverDesc(C.Structure):
                 ("bc_state", ((C.c_double * (MAXSP + 4)) * 2) * 3), ("nscbc", (NscbcDesc * 2) * 3),
                 ("predict", C.c_double), ("max_growth", C.c_double), ("dt_min", C.c_double),
                 ("dt_min_ratio", C.c_double), ("precision", C.c_int), ("retries", C.c_int),
-                ("cfl_backoff", C.c_double), ("savepoint_interval", C.c_int), ("landing_steps", C.c_int)]
+                ("cfl_backoff", C.c_double), ("savepoint_interval", C.c_int), ("landing_steps", C.c_int),
+                ("eos_kind", C.c_int), ("T_guess", C.c_double), ("nasa9", Nasa9SpeciesDesc * MAXSP)]
 
 
 P = C.c_void_p

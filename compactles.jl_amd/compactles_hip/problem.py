@@ -137,8 +137,12 @@ def _isnan(x):
     return a.size == 1 and bool(np.isnan(a).all())
 
 
-def conserved_from_prim(eos: IdealMixture, pr: Prim):
-    """problem.jl:76-90 for an ideal mixture: (rho Y_k..., rho u, rho v, rho w, rho E)."""
+def conserved_from_prim(eos, pr: Prim):
+    """problem.jl:76-90: (rho Y_k..., rho u, rho v, rho w, rho E) for an ideal
+    or a NASA-9 mixture."""
+    from .nasa9 import Nasa9Mixture, nasa9_conserved
+    if isinstance(eos, Nasa9Mixture):
+        return nasa9_conserved(eos, pr)
     N = eos.nspecies()
     if len(pr.Y) != N:
         raise ValueError(f"Prim carries {len(pr.Y)} mass fractions; EOS has {N} species")
@@ -246,11 +250,16 @@ def fill_common_desc(d: L.SolverDesc, prob: Problem, num: Numerics):
     """The fields of a solver descriptor that do not depend on the grid:
     EOS, transport, artificial properties, schemes, cfl, filter cadence,
     step control and precision."""
+    from .nasa9 import Nasa9Mixture
     eos = prob.eos
     d.n_species = eos.nspecies()
-    for k, sp in enumerate(eos.sp):
-        d.R[k] = sp.R
-        d.gamma[k] = sp.gamma
+    if isinstance(eos, Nasa9Mixture):
+        eos.to_desc(d)
+    else:
+        d.eos_kind = L.EOS_IDEAL
+        for k, sp in enumerate(eos.sp):
+            d.R[k] = sp.R
+            d.gamma[k] = sp.gamma
     tr = prob.transport
     d.mu0, d.Pr, d.Sc = tr.mu0, tr.Pr, tr.Sc
     art = num.art

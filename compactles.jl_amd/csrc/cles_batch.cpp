@@ -38,6 +38,7 @@ struct cles_batch {
     std::vector<double> h, L, origin;
     std::vector<void *> allocs;
     double Rk[CLES_MAXSP] = {}, cvk[CLES_MAXSP] = {}, cpk[CLES_MAXSP] = {};
+    const cles::N9Table *n9 = nullptr;   // Nasa9Mixture tables on the device (NULL: IdealMixture)
 
     // line plans (one device buffer of tables) and wave work lists
     void *tables = nullptr;
@@ -247,6 +248,7 @@ struct BImpl {
         a.Q = b.Q;
         a.Ns = b.Ns;
         for (int k = 0; k < b.Ns; k++) { a.Rk[k] = b.Rk[k]; a.cvk[k] = b.cvk[k]; a.Y[k] = f(b.Y[k]); }
+        a.n9 = b.n9;
         a.rho = f(b.rho); a.u = f(b.u); a.v = f(b.v); a.w = f(b.w);
         a.p = f(b.p); a.T = f(b.T); a.c = f(b.c); a.cp = f(b.cp);
         a.esens = f(b.esens);
@@ -314,6 +316,7 @@ struct BImpl {
             a.Y[k] = f(b.Y[k]);
             a.D_art[k] = f(b.D_art[k]);
         }
+        a.n9 = b.n9;
         a.rho = f(b.rho); a.u = f(b.u); a.v = f(b.v); a.w = f(b.w);
         a.p = f(b.p); a.T = f(b.T); a.cp = f(b.cp);
         a.mu_art = f(b.mu_art);
@@ -482,6 +485,7 @@ int cles_batch_create(cles_ctx *ctx, const cles_solver_desc *desc, int64_t count
         for (int k = 1; k < 3; k++)
             CLES_REQUIRE(d.bc[k][0] == CLES_BC_PERIODIC && d.bc[k][1] == CLES_BC_PERIODIC,
                          "batched instances are one-dimensional: dimensions 2 and 3 are collapsed and periodic");
+        cles::validate_eos(d);
         if (d.retries != 0)
             throw cles::Error{CLES_ERR_UNSUPPORTED, "StepControl retries are not available for batched instances "
                                                     "(each instance stops at its own failure)"};
@@ -500,8 +504,14 @@ int cles_batch_create(cles_ctx *ctx, const cles_solver_desc *desc, int64_t count
             b->periodic = lo == CLES_BC_PERIODIC;
             for (int k = 0; k < b->Ns; k++) {   // physics.jl:111-115
                 b->Rk[k] = d.R[k];
-                b->cvk[k] = d.R[k] / (d.gamma[k] - 1);
+                b->cvk[k] = d.eos_kind == CLES_EOS_NASA9 ? 0.0 : d.R[k] / (d.gamma[k] - 1);
                 b->cpk[k] = b->Rk[k] + b->cvk[k];
+            }
+            if (d.eos_kind == CLES_EOS_NASA9) {
+                const cles::N9Table t = cles::nasa9_table(d);
+                auto *dt = (cles::N9Table *)b->alloc(sizeof(cles::N9Table));
+                CLES_HIP(hipMemcpy(dt, &t, sizeof t, hipMemcpyHostToDevice));
+                b->n9 = dt;
             }
             b->off.resize(count);
             b->n.resize(count);

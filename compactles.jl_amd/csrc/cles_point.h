@@ -9,11 +9,25 @@ struct GradPtrs {
     W *gu[9];   // grad_u[d*3 + j] = d u_j / d x_d ; NULL on collapsed d (reads as 0)
 };
 
+// Nasa9Mixture tables in device memory (physics.jl:402-571)
+struct N9Table {
+    int ns;
+    double T_guess;
+    double R[CLES_MAXSP];
+    cles_nasa9_species sp[CLES_MAXSP];
+};
+
+// Host side: the EOS checks of a descriptor (physics.jl:312-338 for a
+// Nasa9Mixture) and the device table of its Nasa9Mixture.
+void validate_eos(const cles_solver_desc &d);
+N9Table nasa9_table(const cles_solver_desc &d);
+
 template <typename W>
 struct PrimArgs {
     const double *Q;
     int Ns;
     double Rk[CLES_MAXSP], cvk[CLES_MAXSP];
+    const N9Table *n9;   // Nasa9Mixture (NULL: IdealMixture)
     W *rho, *u, *v, *w, *p, *T, *c, *cp;
     W *Y[CLES_MAXSP];
     W *esens;   // optional: internal energy of the kappa* sensor (artificial.jl:319-326)
@@ -38,6 +52,7 @@ struct FluxArgs {
     int active[3];
     double mu0, Pr, Sc;
     double cpk[CLES_MAXSP];
+    const N9Table *n9;   // Nasa9Mixture: species enthalpies h_k(T) (NULL: cp_k T)
     const W *rho, *u, *v, *w, *p, *T, *cp;
     const W *mu_art, *beta_art, *kappa_art;
     const W *Y[CLES_MAXSP];
@@ -91,6 +106,7 @@ struct NscbcArgs {
     double Lref, pinf, sigma, beta_t;
     double ut[3], Tt, Yt[CLES_MAXSP], eta_u, eta_T, eta_t, eta_Y;
     double Rk[CLES_MAXSP], cvk[CLES_MAXSP];
+    const N9Table *n9;              // Nasa9Mixture: cv_k(T) in the inflow's eos_dphi_dY (NULL: cvk)
     W *dQ;
 };
 

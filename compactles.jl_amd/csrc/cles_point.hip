@@ -27,6 +27,68 @@ __device__ __forceinline__ double d_max(double a, double b) { return a > b ? a :
 __device__ __forceinline__ float d_max(float a, float b) { return a > b ? a : b; }
 
 // ---------------------------------------------------------------------------
+// Nasa9Mixture thermodynamics (physics.jl:290-310, 432-498).  Powers are
+// formed by multiplication (within 2 ulp of Julia's integer powers).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double jl_max(double a, double b) { return (a != a || b != b) ? a + b : (a > b ? a : b); }
+
+__device__ __forceinline__ const cles_nasa9_interval &n9_interval(const cles_nasa9_species &s, double T)
+{
+    for (int i = 0; i < s.n_intervals - 1; i++)
+        if (T <= s.iv[i].Tmax) return s.iv[i];
+    return s.iv[s.n_intervals - 1];
+}
+__device__ __forceinline__ double n9_cp_over_R(const cles_nasa9_interval &v, double T)
+{
+    const double *a = v.a;
+    const double T2 = T * T, T3 = T2 * T, T4 = T2 * T2;
+    return a[0] / T2 + a[1] / T + a[2] + a[3] * T + a[4] * T2 + a[5] * T3 + a[6] * T4;
+}
+__device__ __forceinline__ double n9_h_over_R(const cles_nasa9_interval &v, double T)
+{
+    const double *a = v.a;
+    const double T2 = T * T, T3 = T2 * T, T4 = T2 * T2, T5 = T4 * T;
+    return -a[0] / T + a[1] * log(T) + a[2] * T + a[3] * T2 / 2 + a[4] * T3 / 3 + a[5] * T4 / 4 + a[6] * T5 / 5 +
+           v.b1;
+}
+__device__ __forceinline__ double n9_cp(const N9Table &t, int k, double T)
+{
+    return t.R[k] * n9_cp_over_R(n9_interval(t.sp[k], T), T);
+}
+__device__ __forceinline__ double n9_h(const N9Table &t, int k, double T)
+{
+    return t.R[k] * n9_h_over_R(n9_interval(t.sp[k], T), T);
+}
+__device__ __forceinline__ double n9_e(const N9Table &t, int k, double T) { return n9_h(t, k, T) - t.R[k] * T; }
+
+// physics.jl:459-482 mixture_temperature: bounded Newton on sum_k Y_k e_k(T) = e
+__device__ double n9_temperature(const N9Table &t, double e, const double *Y)
+{
+    double e0 = 0.0, cv0 = 0.0;
+    for (int k = 0; k < t.ns; k++) {
+        e0 += Y[k] * n9_e(t, k, t.T_guess);
+        cv0 += Y[k] * (n9_cp(t, k, t.T_guess) - t.R[k]);
+    }
+    double T = t.T_guess;
+    if (cv0 > 0) {
+        const double x = t.T_guess + (e - e0) / cv0;
+        T = x > 1e9 ? 1e9 : (x < 1e-3 ? 1e-3 : x);
+    }
+    for (int it = 0; it < 30; it++) {
+        double f = -e, cvm = 0.0;
+        for (int k = 0; k < t.ns; k++) {
+            f += Y[k] * n9_e(t, k, T);
+            cvm += Y[k] * (n9_cp(t, k, T) - t.R[k]);
+        }
+        if (!(cvm > 0)) break;
+        const double dl = f / cvm;
+        T = jl_max(T - dl, 0.25 * T);
+        if (fabs(dl) <= 1e-14 * T) break;
+    }
+    return T;
+}
+
+// ---------------------------------------------------------------------------
 // physics.jl:604-653 _primitives! (IdealMixture), over the interior
 // ---------------------------------------------------------------------------
 template <typename W>
@@ -39,7 +101,42 @@ __global__ void __launch_bounds__(TPB) k_primitives(PrimArgs<W> a, int64_t N)
 #pragma unroll
     for (int sp = 0; sp < CLES_MAXSP; sp++)
         if (sp < Ns) rho += a.Q[(int64_t)sp * N + I];
-    if (rho > 0) {
+    if (a.n9 && rho > 0) {
+        // physics.jl:515-552 _primitives! (Nasa9Mixture)
+        const N9Table &t = *a.n9;
+        const double ri = 1 / rho;
+        double Rm = 0.0, Y[CLES_MAXSP];
+        for (int sp = 0; sp < Ns; sp++) {
+            const double q = a.Q[(int64_t)sp * N + I];
+            Y[sp] = q * ri;
+            a.Y[sp][I] = (W)Y[sp];
+            Rm += q * ri * t.R[sp];
+        }
+        const double u = a.Q[(int64_t)Ns * N + I] * ri;
+        const double v = a.Q[(int64_t)(Ns + 1) * N + I] * ri;
+        const double w = a.Q[(int64_t)(Ns + 2) * N + I] * ri;
+        const double e = a.Q[(int64_t)(Ns + 3) * N + I] * ri - 0.5 * (u * u + v * v + w * w);
+        const double T = n9_temperature(t, e, Y);
+        double cpm = 0.0;
+        for (int sp = 0; sp < Ns; sp++) cpm += Y[sp] * n9_cp(t, sp, T);
+        const double cvm = cpm - Rm;
+        if (a.esens) {
+            const double r = d_max(rho, 1e-300);
+            const double m1 = a.Q[(int64_t)Ns * N + I], m2 = a.Q[(int64_t)(Ns + 1) * N + I],
+                         m3 = a.Q[(int64_t)(Ns + 2) * N + I];
+            const double ke = 0.5 * (m1 * m1 + m2 * m2 + m3 * m3) / r;
+            a.esens[I] = (W)((a.Q[(int64_t)(Ns + 3) * N + I] - ke) / r);
+        }
+        const double cc = (cpm / cvm) * Rm * T;
+        a.rho[I] = (W)rho;
+        a.u[I] = (W)u;
+        a.v[I] = (W)v;
+        a.w[I] = (W)w;
+        a.p[I] = (W)(rho * Rm * T);
+        a.T[I] = (W)T;
+        a.c[I] = (W)sqrt(cc > 0.0 ? cc : 0.0);
+        a.cp[I] = (W)cpm;
+    } else if (rho > 0) {
         const double ri = 1 / rho;
         double Rm = 0.0, cvm = 0.0;
 #pragma unroll
@@ -331,6 +428,11 @@ __global__ void __launch_bounds__(TPB) k_fluxes(FluxArgs<W> a, int64_t N)
         W **F = a.flux[d];
         W hdiff = 0;
         if (Ns > 1) {
+            // rhs.jl:544 species_enthalpy: cp_k T (IdealMixture) or h_k(T) (Nasa9Mixture)
+            W hk[CLES_MAXSP];
+#pragma unroll
+            for (int sp = 0; sp < CLES_MAXSP; sp++)
+                if (sp < Ns) hk[sp] = a.n9 ? (W)n9_h(*a.n9, sp, (double)Tp) : (W)a.cpk[sp] * Tp;
             W Vc = 0;
 #pragma unroll
             for (int sp = 0; sp < CLES_MAXSP; sp++)
@@ -341,7 +443,7 @@ __global__ void __launch_bounds__(TPB) k_fluxes(FluxArgs<W> a, int64_t N)
                     const W Dk = D0 + a.D_art[sp][I];
                     const W J = rho * (-Dk * a.gY[d][sp][I] + a.Y[sp][I] * Vc);
                     F[sp][I] = rho * a.Y[sp][I] * ud + J;
-                    hdiff += (W)a.cpk[sp] * Tp * J;
+                    hdiff += hk[sp] * J;
                 }
         } else {
             // one species: sum_k J_k = 0 forces J_1 = 0 (the reference's
@@ -650,8 +752,9 @@ __global__ void __launch_bounds__(TPB) k_nscbc(NscbcArgs<W> a)
         const double dLs = a.eta_Y * K * (Yk - a.Yt[k]) - un * gYk;
         W *q = a.dQ + (int64_t)k * N + I;
         *q = (W)((double)*q - (Yk * dd1 + rho * dLs));
-        // physics.jl:148-152 eos_dphi_dY
-        sphiY += (a.cvk[k] * Rm - a.Rk[k] * cvm) / (Rm * Rm) * dLs;
+        // physics.jl:148-152 eos_dphi_dY; a Nasa9Mixture takes cv_k(T) (physics.jl:504-510)
+        const double cvk = a.n9 ? n9_cp(*a.n9, k, Tp) - a.Rk[k] : a.cvk[k];
+        sphiY += (cvk * Rm - a.Rk[k] * cvm) / (Rm * Rm) * dLs;
     }
 #pragma unroll
     for (int m = 0; m < 3; m++) {

@@ -46,6 +46,7 @@ struct cles_solver {
     // dimensions, on the device (NULL: uniform)
     bool stretched = false;
     cles::Metric1D met{};
+    const cles::N9Table *n9 = nullptr;   // Nasa9Mixture tables on the device (NULL: IdealMixture)
     std::vector<void *> allocs;
     size_t bytes = 0;                          // device bytes allocated by this solver
     // device state (W-typed pointers are void* here; the Impl casts them)
@@ -269,6 +270,7 @@ struct Impl {
         PrimArgs<W> a{};
         a.Q = s.Q;
         a.Ns = s.Ns;
+        a.n9 = s.n9;
         for (int k = 0; k < s.Ns; k++) { a.Rk[k] = s.Rk[k]; a.cvk[k] = s.cvk[k]; a.Y[k] = f(s.Y[k]); }
         a.rho = f(s.rho); a.u = f(s.u); a.v = f(s.v); a.w = f(s.w);
         a.p = f(s.p); a.T = f(s.T); a.c = f(s.c); a.cp = f(s.cp);
@@ -405,6 +407,7 @@ struct Impl {
         FluxArgs<W> a{};
         a.Q = s.Q;
         a.Ns = s.Ns;
+        a.n9 = s.n9;
         for (int k = 0; k < 3; k++) a.active[k] = s.active[k];
         a.mu0 = s.d.mu0;
         a.Pr = s.d.Pr;
@@ -628,6 +631,7 @@ struct Impl {
                 a.eta_T = nd.eta_T;
                 a.eta_t = nd.eta_t;
                 a.eta_Y = nd.eta_Y;
+                a.n9 = s.n9;
                 a.dQ = f(s.dQ);
                 const double w = sizeof(W);
                 s.timed(PH_NSCBC, (w * (14 + 2 * s.Ns) + 2 * w * s.C) * (double)a.count,
@@ -736,6 +740,45 @@ int check_step(const cles_solver_desc &d, double dt, double rho_min, double dt_s
     return CLES_OK;
 }
 
+}  // namespace
+
+namespace cles {
+
+void validate_eos(const cles_solver_desc &d)
+{
+    if (d.eos_kind != CLES_EOS_IDEAL && d.eos_kind != CLES_EOS_NASA9)
+        throw cles::Error{CLES_ERR_INVALID, "eos_kind must be CLES_EOS_IDEAL or CLES_EOS_NASA9"};
+    if (d.eos_kind == CLES_EOS_NASA9) {   // physics.jl:312-338 _validate_nasa9_intervals
+        if (!(d.T_guess > 0)) throw cles::Error{CLES_ERR_INVALID, "Nasa9Mixture: T_guess must be positive"};
+        for (int k = 0; k < d.n_species; k++) {
+            const cles_nasa9_species &sp = d.nasa9[k];
+            if (sp.n_intervals < 1 || sp.n_intervals > CLES_NASA9_MAXINT)
+                throw cles::Error{CLES_ERR_INVALID, "NASA-9 species " + std::to_string(k + 1) +
+                                                        ": 1..3 intervals are supported"};
+            if (!(d.R[k] > 0)) throw cles::Error{CLES_ERR_INVALID, "NASA-9 species has non-positive R"};
+            for (int i = 0; i < sp.n_intervals; i++)
+                if (!(sp.iv[i].Tmin > 0 && sp.iv[i].Tmin < sp.iv[i].Tmax))
+                    throw cles::Error{CLES_ERR_INVALID, "invalid NASA-9 interval"};
+        }
+    }
+}
+
+N9Table nasa9_table(const cles_solver_desc &d)
+{
+    N9Table t{};
+    t.ns = d.n_species;
+    t.T_guess = d.T_guess;
+    for (int k = 0; k < d.n_species; k++) {
+        t.R[k] = d.R[k];
+        t.sp[k] = d.nasa9[k];
+    }
+    return t;
+}
+
+}  // namespace cles
+
+namespace {
+
 void validate_desc(const cles_solver_desc &d)
 {
     if (d.n_species < 1 || d.n_species > CLES_MAXSP) throw cles::Error{CLES_ERR_INVALID, "n_species must be 1..8"};
@@ -761,6 +804,7 @@ void validate_desc(const cles_solver_desc &d)
     if (d.retries > 0 && !(d.cfl_backoff > 0))
         throw cles::Error{CLES_ERR_INVALID, "StepControl: cfl_backoff must be positive"};
     if (d.savepoint_interval < 0) throw cles::Error{CLES_ERR_INVALID, "StepControl: savepoint_interval must be >= 0"};
+    cles::validate_eos(d);
     if (d.landing_steps < 0)
         throw cles::Error{CLES_ERR_INVALID, "StepControl: landing_steps must be >= 1 (1 is a hard clip onto the "
                                             "scheduled time)"};
@@ -948,7 +992,7 @@ int cles_solver_create_ex(cles_ctx *ctx, const cles_solver_desc *desc, const dou
             }
             for (int k = 0; k < s->Ns; k++) {   // physics.jl:111-115
                 s->Rk[k] = desc->R[k];
-                s->cvk[k] = desc->R[k] / (desc->gamma[k] - 1);
+                s->cvk[k] = desc->eos_kind == CLES_EOS_NASA9 ? 0.0 : desc->R[k] / (desc->gamma[k] - 1);
                 s->cpk[k] = s->Rk[k] + s->cvk[k];
             }
             const int wdt = s->prec;
@@ -1006,6 +1050,12 @@ int cles_solver_create_ex(cles_ctx *ctx, const cles_solver_desc *desc, const dou
             }
             if (s->nscbc)
                 for (void *&fp : s->nsd) fp = s->wfield();
+            if (desc->eos_kind == CLES_EOS_NASA9) {
+                const cles::N9Table t = cles::nasa9_table(*desc);
+                auto *dt = (cles::N9Table *)s->alloc(sizeof(cles::N9Table));
+                CLES_HIP(hipMemcpy(dt, &t, sizeof t, hipMemcpyHostToDevice));
+                s->n9 = dt;
+            }
             for (int k = 0; k < 3; k++) {
                 s->met.n[k] = s->n[k];
                 if (!(dxdxi && dxdxi[k])) continue;

@@ -20,7 +20,7 @@
 extern "C" {
 #endif
 
-#define CLES_ABI_VERSION 2
+#define CLES_ABI_VERSION 3
 
 /* Status codes.  The solver-failure codes mirror SolverFailure reasons
  * (src/stepcontrol.jl:138-164): nonfinite, negative_density, planck, dt_min,
@@ -143,6 +143,23 @@ int cles_apply_along_timed(cles_plan *plan, void *out, const void *f,
 /* Solver: setup / compute_rhs! / step! / max_rate / filter_state! / run!   */
 /* ----------------------------------------------------------------------- */
 
+/* One NASA-9 fit interval (Nasa9Interval, src/physics.jl:280-300): cp/R =
+ * a1/T^2 + a2/T + a3 + a4 T + a5 T^2 + a6 T^3 + a7 T^4 on [Tmin, Tmax], and
+ * h/R = -a1/T + a2 ln T + a3 T + a4 T^2/2 + a5 T^3/3 + a6 T^4/4 + a7 T^5/5 + b1. */
+#define CLES_NASA9_MAXINT 3
+typedef struct {
+    double Tmin, Tmax;
+    double a[7];
+    double b1;
+} cles_nasa9_interval;
+/* Nasa9Species (physics.jl:340-375): ordered, contiguous intervals; a
+ * temperature above interval i's Tmax uses the next one, the last one
+ * beyond its range. */
+typedef struct {
+    int n_intervals;
+    cles_nasa9_interval iv[CLES_NASA9_MAXINT];
+} cles_nasa9_species;
+
 /* NSCBC face parameters (src/nscbc.jl:45-54 outflow, :204-216 inflow). */
 typedef struct {
     double pinf, sigma, Lref, beta_t;      /* outflow; Lref <= 0: domain length */
@@ -178,7 +195,14 @@ typedef struct {
     double cfl_backoff;                    /* CFL multiplier per rollback */
     int savepoint_interval;                /* steps between savepoints */
     int landing_steps;                     /* >= 1; 0 selects the default 2 */
+    /* EOS: CLES_EOS_IDEAL IdealMixture (R, gamma; physics.jl:104-126) or
+     * CLES_EOS_NASA9 Nasa9Mixture (R, nasa9, T_guess; physics.jl:402-571) */
+    int eos_kind;
+    double T_guess;
+    cles_nasa9_species nasa9[CLES_MAXSP];
 } cles_solver_desc;
+
+enum { CLES_EOS_IDEAL = 0, CLES_EOS_NASA9 = 1 };
 
 /* save_checkpoint / load_checkpoint! (src/io.jl:6-75): `prefix`.rNNNN.ckpt
  * with the reference's header (magic, n_cons, n_global, n_local, rank

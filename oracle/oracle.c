@@ -773,6 +773,89 @@ static void copy_interior(or_solver *s, double *dst, const double *src)
 }
 
 /* physics.jl:604-653 _primitives! over the full padded arrays */
+/* Nasa9Mixture (physics.jl:290-310 fits, 432-456 species functions,
+ * 459-482 mixture_temperature).  Powers by multiplication. */
+static const or_nasa9_interval *n9_interval(const or_nasa9_species *sp, double T)
+{
+    for (int i = 0; i < sp->n_intervals - 1; i++)
+        if (T <= sp->iv[i].Tmax) return &sp->iv[i];
+    return &sp->iv[sp->n_intervals - 1];
+}
+static double n9_cp_over_R(const or_nasa9_interval *v, double T)
+{
+    const double *a = v->a;
+    const double T2 = T * T, T3 = T2 * T, T4 = T2 * T2;
+    return a[0] / T2 + a[1] / T + a[2] + a[3] * T + a[4] * T2 + a[5] * T3 + a[6] * T4;
+}
+static double n9_h_over_R(const or_nasa9_interval *v, double T)
+{
+    const double *a = v->a;
+    const double T2 = T * T, T3 = T2 * T, T4 = T2 * T2, T5 = T4 * T;
+    return -a[0] / T + a[1] * log(T) + a[2] * T + a[3] * T2 / 2 + a[4] * T3 / 3 + a[5] * T4 / 4 + a[6] * T5 / 5 +
+           v->b1;
+}
+static double n9_cp(const or_solver *s, int k, double T)
+{
+    return s->d.R[k] * n9_cp_over_R(n9_interval(&s->d.nasa9[k], T), T);
+}
+static double n9_h(const or_solver *s, int k, double T)
+{
+    return s->d.R[k] * n9_h_over_R(n9_interval(&s->d.nasa9[k], T), T);
+}
+static double n9_e(const or_solver *s, int k, double T) { return n9_h(s, k, T) - s->d.R[k] * T; }
+
+static double n9_temperature(const or_solver *s, double e, const double *Y)
+{
+    double e0 = 0.0, cv0 = 0.0, Tg = s->d.T_guess;
+    for (int k = 0; k < s->Ns; k++) {
+        e0 += Y[k] * n9_e(s, k, Tg);
+        cv0 += Y[k] * (n9_cp(s, k, Tg) - s->d.R[k]);
+    }
+    double T = Tg;
+    if (cv0 > 0) {
+        double x = Tg + (e - e0) / cv0;
+        T = x > 1e9 ? 1e9 : (x < 1e-3 ? 1e-3 : x);
+    }
+    for (int it = 0; it < 30; it++) {
+        double f = -e, cvm = 0.0;
+        for (int k = 0; k < s->Ns; k++) {
+            f += Y[k] * n9_e(s, k, T);
+            cvm += Y[k] * (n9_cp(s, k, T) - s->d.R[k]);
+        }
+        if (!(cvm > 0)) break;
+        double dl = f / cvm;
+        T = jmax(T - dl, 0.25 * T);
+        if (fabs(dl) <= 1e-14 * T) break;
+    }
+    return T;
+}
+
+/* physics.jl:515-552 _primitives! (Nasa9Mixture) at padded index I */
+static void primitives_n9(or_solver *s, int64_t I, double rho)
+{
+    int Ns = s->Ns;
+    double ri = 1 / rho, Rm = 0.0, Y[OR_MAXSP];
+    for (int sp = 0; sp < Ns; sp++) {
+        Y[sp] = Qc(s, sp)[I] * ri;
+        s->Y[sp][I] = Y[sp];
+        Rm += Qc(s, sp)[I] * ri * s->d.R[sp];
+    }
+    double u = Qc(s, s->im1)[I] * ri;
+    double v = Qc(s, s->im2)[I] * ri;
+    double w = Qc(s, s->im3)[I] * ri;
+    double e = Qc(s, s->ie)[I] * ri - 0.5 * (u * u + v * v + w * w);
+    double T = n9_temperature(s, e, Y);
+    double cpm = 0.0;
+    for (int sp = 0; sp < Ns; sp++) cpm += s->Y[sp][I] * n9_cp(s, sp, T);
+    double cvm = cpm - Rm;
+    double cc = (cpm / cvm) * Rm * T;
+    s->rho[I] = rho; s->u[I] = u; s->v[I] = v; s->w[I] = w;
+    s->p[I] = rho * Rm * T;
+    s->T[I] = T;
+    s->c[I] = sqrt(cc > 0.0 ? cc : 0.0);
+    s->cp[I] = cpm;
+}
+
 static void primitives(or_solver *s)
 {
     int Ns = s->Ns;
@@ -780,7 +863,9 @@ static void primitives(or_solver *s)
     for (int64_t I = 0; I < s->NP; I++) {
         double rho = 0.0;
         for (int sp = 0; sp < Ns; sp++) rho += Qc(s, sp)[I];
-        if (rho > 0) {
+        if (rho > 0 && s->d.eos_kind == 1) {
+            primitives_n9(s, I, rho);
+        } else if (rho > 0) {
             double ri = 1 / rho;
             double Rm = 0.0, cvm = 0.0;
             for (int sp = 0; sp < Ns; sp++) {
@@ -812,8 +897,8 @@ static void primitives(or_solver *s)
 static void deriv_along(or_solver *s, double *out, const double *f, int d)
 {
     or_apply_along(s->dplan[d], out, f);
-    /* _scale_grad! multiplies by inv_h == 1 on an unstretched Cartesian grid
-     * (rhs.jl:477-483, metric.jl init_geometry!): exact no-op, omitted. */
+    /* _scale_grad! (rhs.jl:477-483) follows as scale_grad() for the gradients;
+     * it is the identity on unstretched dimensions. */
 }
 static void filt_along(or_solver *s, double *out, const double *f, int d)
 {
@@ -998,7 +1083,9 @@ static void assemble_fluxes(or_solver *s)
                         double Dk = D0 + s->D_art[sp][I];
                         double Jkd = rho * (-Dk * s->gY[dd][sp][I] + s->Y[sp][I] * Vc);
                         s->flux[dd][sp][I] = rho * s->Y[sp][I] * ud + Jkd;
-                        hdiff += s->cpk[sp] * Tp * Jkd;   /* species_enthalpy physics.jl:137 */
+                        /* species_enthalpy: cp_k T (physics.jl:137) or h_k(T) (physics.jl:446-450) */
+                        const double hk = s->d.eos_kind == 1 ? n9_h(s, sp, Tp) : s->cpk[sp] * Tp;
+                        hdiff += hk * Jkd;
                     }
                     s->flux[dd][s->im1][I] = rho * ud * uv[0] + (dd == 0 ? p : 0.0) - tau[dd][0];
                     s->flux[dd][s->im2][I] = rho * ud * uv[1] + (dd == 1 ? p : 0.0) - tau[dd][1];
@@ -1019,7 +1106,9 @@ static inline double eos_dphi_dY(const or_solver *s, int k, double rho, double p
 {
     double Rm = p / (rho * T);
     double cvm = cp - Rm;
-    return (s->cvk[k] * Rm - s->Rk[k] * cvm) / (Rm * Rm);
+    /* physics.jl:504-510: a Nasa9Mixture takes cv_k at the local temperature */
+    double cvk = s->d.eos_kind == 1 ? n9_cp(s, k, T) - s->Rk[k] : s->cvk[k];
+    return (cvk * Rm - s->Rk[k] * cvm) / (Rm * Rm);
 }
 
 /* wall plane of (dim d 0-based, side 0 low / 1 high): boundary.jl:150-170 */

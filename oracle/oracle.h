@@ -69,6 +69,17 @@ typedef struct {
     double eta_u, eta_T, eta_t, eta_Y;
 } or_nscbc;
 
+/* One NASA-9 interval and species (physics.jl:280-375) */
+typedef struct {
+    double Tmin, Tmax;
+    double a[7];
+    double b1;
+} or_nasa9_interval;
+typedef struct {
+    int n_intervals;
+    or_nasa9_interval iv[3];
+} or_nasa9_species;
+
 typedef struct {
     int64_t n_global[3];
     double L_domain[3];
@@ -92,6 +103,10 @@ typedef struct {
     double cfl_backoff;
     int savepoint_interval;
     int landing_steps;                   /* 0: the default 2 */
+    /* EOS: 0 IdealMixture, 1 Nasa9Mixture (physics.jl:280-571) */
+    int eos_kind;
+    double T_guess;
+    or_nasa9_species nasa9[OR_MAXSP];
 } or_desc;
 
 /* Step-boundary hooks of run! (src/callbacks.jl): next scheduled instant,

@@ -37,6 +37,14 @@ class Nscbc(C.Structure):
                 ("eta_t", C.c_double), ("eta_Y", C.c_double)]
 
 
+class Nasa9Iv(C.Structure):
+    _fields_ = [("Tmin", C.c_double), ("Tmax", C.c_double), ("a", C.c_double * 7), ("b1", C.c_double)]
+
+
+class Nasa9Sp(C.Structure):
+    _fields_ = [("n_intervals", C.c_int), ("iv", Nasa9Iv * 3)]
+
+
 class Desc(C.Structure):
     _fields_ = [("n_global", C.c_int64 * 3), ("L_domain", C.c_double * 3),
                 ("origin", C.c_double * 3), ("n_species", C.c_int),
@@ -51,7 +59,8 @@ class Desc(C.Structure):
                 ("predict", C.c_double), ("max_growth", C.c_double),
                 ("dt_min", C.c_double), ("dt_min_ratio", C.c_double),
                 ("retries", C.c_int), ("cfl_backoff", C.c_double), ("savepoint_interval", C.c_int),
-                ("landing_steps", C.c_int)]
+                ("landing_steps", C.c_int), ("eos_kind", C.c_int), ("T_guess", C.c_double),
+                ("nasa9", Nasa9Sp * MAXSP)]
 
 
 def _cpu_has_avx2() -> bool:
@@ -333,7 +342,9 @@ def make_desc(n_global, L_domain, origin=(0.0, 0.0, 0.0), species=((1.0, 1.4),),
               C_D=0.01, beta_sensor=0, deriv="lele_d1_6", alphaf=0.45, cfl=0.5,
               filter_interval=1, bcs=None, bc_state=None, nscbc=None,
               predict=0.0, max_growth=0.0, dt_min=0.0, dt_min_ratio=1e-8, retries=0, cfl_backoff=0.5,
-              savepoint_interval=25, landing_steps=2) -> Desc:
+              savepoint_interval=25, landing_steps=2, nasa9=None) -> Desc:
+    """nasa9: (T_guess, [(R, [(Tmin, Tmax, (a1..a7), b1), ...]), ...]) selects
+    the Nasa9Mixture (species then ignores gamma)."""
     d = Desc()
     for k in range(3):
         d.n_global[k] = int(n_global[k])
@@ -372,4 +383,17 @@ def make_desc(n_global, L_domain, origin=(0.0, 0.0, 0.0), species=((1.0, 1.4),),
     d.predict, d.max_growth, d.dt_min, d.dt_min_ratio = predict, max_growth, dt_min, dt_min_ratio
     d.retries, d.cfl_backoff, d.savepoint_interval, d.landing_steps = retries, cfl_backoff, savepoint_interval, \
         landing_steps
+    if nasa9 is not None:
+        d.eos_kind = 1
+        d.T_guess, table = nasa9
+        d.n_species = len(table)
+        for k, (R, ivs) in enumerate(table):
+            d.R[k] = R
+            d.gamma[k] = 0.0
+            d.nasa9[k].n_intervals = len(ivs)
+            for i, (Tmin, Tmax, a, b1) in enumerate(ivs):
+                iv = d.nasa9[k].iv[i]
+                iv.Tmin, iv.Tmax, iv.b1 = Tmin, Tmax, b1
+                for j in range(7):
+                    iv.a[j] = a[j]
     return d

@@ -46,6 +46,16 @@ def oracle_solver(n, L, q, jac=None, **kw):
     return s
 
 
+def nasa9_table(eos):
+    """Oracle make_desc(nasa9=...) table of a frontend Nasa9Mixture."""
+    return (eos.T_guess, [(s.R, [(iv.Tmin, iv.Tmax, iv.a, iv.b1) for iv in s.intervals]) for s in eos.sp])
+
+
+def nasa9_oracle_kw(eos):
+    """make_desc keywords of a Nasa9Mixture (species carry R; gamma unused)."""
+    return {"nasa9": nasa9_table(eos), "species": tuple((s.R, 0.0) for s in eos.sp)}
+
+
 def nscbc_bc(ch, par):
     """Frontend NSCBC object from the oracle's parameter dict."""
     if "pinf" in par:
@@ -66,11 +76,13 @@ def nscbc_codes(bcs, nscbc):
 
 def gpu_solver(ctx, n, L, q, species=((1.0, 1.4),), mu0=0.0, art=True, cfl=0.5, bcs=None, deriv="lele_d1_6",
                precision="fp64", beta_sensor="strain", bc_funs=None, filter_interval=1, origin=(0.0, 0.0, 0.0),
-               nscbc=None, chunks=None, control=None, stretch=None):
+               nscbc=None, chunks=None, control=None, stretch=None, eos=None):
     """chunks: line-tile lengths (x, y, z) for this solver's plans, set in the
-    table for its creation only (None: the table / rule as it stands)."""
+    table for its creation only (None: the table / rule as it stands).
+    eos: a frontend EOS object (None: the IdealMixture of `species`)."""
     import compactles_hip as ch
-    eos = ch.IdealMixture([ch.IdealSpecies(f"s{k}", R, g) for k, (R, g) in enumerate(species)])
+    if eos is None:
+        eos = ch.IdealMixture([ch.IdealSpecies(f"s{k}", R, g) for k, (R, g) in enumerate(species)])
     code = {0: ch.PeriodicBC, 1: ch.SlipWallBC}
     bcs = bcs or ((0, 0),) * 3
     bco = []
@@ -176,8 +188,9 @@ def tube_specs(count, seed=20261202, nmin=128, nmax=2048):
     return out
 
 
-def tube_state(spec):
-    """(q arrays over the nodes, (left, right) Dirichlet conserved states, L, h)."""
+def tube_state(spec, eos=None):
+    """(q arrays over the nodes, (left, right) Dirichlet conserved states, L, h).
+    eos: a frontend Nasa9Mixture of two species (None: the HE_CO2 ideal gases)."""
     n = spec["n"]
     L = 1.0
     h = L / (n - 1)
@@ -187,6 +200,12 @@ def tube_state(spec):
     p = (1 - th) * spec["pL"] + th * spec["pR"]
     rho = (1 - th) * spec["rhoL"] + th * spec["rhoR"]
     z = 0 * x
+    if eos is not None:
+        import compactles_hip as ch
+        q = ch.conserved_from_prim(eos, ch.Prim(u=(z, z, z), p=p, rho=rho, Y=tuple(Y)))
+        qL = [float(v) for v in ch.conserved_from_prim(eos, ch.Prim(p=spec["pL"], rho=spec["rhoL"], Y=(1.0, 0.0)))]
+        qR = [float(v) for v in ch.conserved_from_prim(eos, ch.Prim(p=spec["pR"], rho=spec["rhoR"], Y=(0.0, 1.0)))]
+        return [np.asarray(a).reshape(n, 1, 1) for a in q], (qL, qR), L, h
     q = conserved((z, z, z, p, rho), species=HE_CO2, Y=Y)
     one = np.ones(1)
     qL = [float(v[0]) for v in conserved((0 * one, 0 * one, 0 * one, spec["pL"] * one, spec["rhoL"] * one),
@@ -196,11 +215,13 @@ def tube_state(spec):
     return [a.reshape(n, 1, 1) for a in q], (qL, qR), L, h
 
 
-def tube_oracle(spec, precision_art=True, cfl=0.4):
-    q, (qL, qR), L, h = tube_state(spec)
+def tube_oracle(spec, precision_art=True, cfl=0.4, eos=None):
+    q, (qL, qR), L, h = tube_state(spec, eos)
     n = spec["n"]
-    return oracle_solver((n, 1, 1), (L, h, h), q, species=HE_CO2, art=precision_art, cfl=cfl,
-                         bcs=((O.BC_DIRICHLET, O.BC_DIRICHLET), (0, 0), (0, 0)), bc_state={(0, 0): qL, (0, 1): qR})
+    kw = {"species": HE_CO2} if eos is None else nasa9_oracle_kw(eos)
+    return oracle_solver((n, 1, 1), (L, h, h), q, art=precision_art, cfl=cfl,
+                         bcs=((O.BC_DIRICHLET, O.BC_DIRICHLET), (0, 0), (0, 0)), bc_state={(0, 0): qL, (0, 1): qR},
+                         **kw)
 
 
 def tube_solver(ctx, spec, precision="fp64", art=True, cfl=0.4):
@@ -216,10 +237,12 @@ def tube_solver(ctx, spec, precision="fp64", art=True, cfl=0.4):
                       bc_funs={(0, 0): end(spec["rhoL"], spec["pL"], 1.0), (0, 1): end(spec["rhoR"], spec["pR"], 0.0)})
 
 
-def tube_batch(ctx, specs, precision="fp64", tfinal=None, nmax=2 ** 62, art=True, cfl=0.4):
+def tube_batch(ctx, specs, precision="fp64", tfinal=None, nmax=2 ** 62, art=True, cfl=0.4, eos=None):
     """A GPU Batch of the tubes, uploaded with the same initial arrays."""
     import compactles_hip as ch
-    eos = ch.IdealMixture([ch.IdealSpecies(f"s{k}", R, g) for k, (R, g) in enumerate(HE_CO2)])
+    gas = eos
+    if eos is None:
+        eos = ch.IdealMixture([ch.IdealSpecies(f"s{k}", R, g) for k, (R, g) in enumerate(HE_CO2)])
     dummy = ch.DirichletBC(lambda x, y, z, t: None)
     prob = ch.Problem(domain=((0.0, 1.0),) * 3, bcs=((dummy, dummy), (ch.PeriodicBC(), ch.PeriodicBC()),
                                                    (ch.PeriodicBC(), ch.PeriodicBC())),
@@ -228,7 +251,7 @@ def tube_batch(ctx, specs, precision="fp64", tfinal=None, nmax=2 ** 62, art=True
                       precision=precision)
     insts, states = [], []
     for k, sp in enumerate(specs):
-        q, (qL, qR), L, h = tube_state(sp)
+        q, (qL, qR), L, h = tube_state(sp, gas)
         tf = tfinal[k] if isinstance(tfinal, (list, tuple)) else (tfinal if tfinal is not None else float("inf"))
         nm = nmax[k] if isinstance(nmax, (list, tuple)) else nmax
         insts.append(ch.Instance(n=sp["n"], domain=(0.0, L), ic=None, tfinal=tf, nmax=nm,

@@ -34,7 +34,7 @@ def test_library_exports_every_declared_symbol():
 
 def test_abi_version():
     import compactles_hip as ch
-    assert ch.load().cles_abi_version() == 2
+    assert ch.load().cles_abi_version() == 3
 
 
 def test_struct_layout_matches_header():
@@ -53,6 +53,8 @@ int main(void) {
   printf("%zu %zu %zu %zu\n", offsetof(cles_solver_desc, retries), offsetof(cles_solver_desc, cfl_backoff),
          offsetof(cles_solver_desc, landing_steps), sizeof(cles_run_hooks));
   printf("%zu %zu\n", offsetof(cles_run_hooks, after_step), offsetof(cles_run_hooks, rewind));
+  printf("%zu %zu %zu %zu\n", offsetof(cles_solver_desc, eos_kind), offsetof(cles_solver_desc, T_guess),
+         offsetof(cles_solver_desc, nasa9), sizeof(cles_nasa9_species));
   return 0;
 }
 '''
@@ -66,7 +68,9 @@ int main(void) {
     want = [C.sizeof(L.SchemeDesc), C.sizeof(L.NscbcDesc), C.sizeof(L.SolverDesc), L.SolverDesc.precision.offset,
             L.SchemeDesc.closure_rhs.offset, L.SolverDesc.filt.offset, L.SolverDesc.bc_state.offset,
             L.SolverDesc.nscbc.offset, L.SolverDesc.retries.offset, L.SolverDesc.cfl_backoff.offset,
-            L.SolverDesc.landing_steps.offset, C.sizeof(RunHooks), RunHooks.after_step.offset, RunHooks.rewind.offset]
+            L.SolverDesc.landing_steps.offset, C.sizeof(RunHooks), RunHooks.after_step.offset, RunHooks.rewind.offset,
+            L.SolverDesc.eos_kind.offset, L.SolverDesc.T_guess.offset, L.SolverDesc.nasa9.offset,
+            C.sizeof(L.Nasa9SpeciesDesc)]
     assert got == want
 
 
