@@ -8,7 +8,7 @@ from .schemes import (BandedClosureRow, BandedCompactScheme, ClosureRow, Compact
                       compact_filter, lele_d1_10, lele_d1_6, pade_d1_4)
 from .problem import (ArtParams, DirichletBC, IdealMixture, IdealSpecies, NSCBCInflowBC,  # noqa: F401
                       NSCBCOutflowBC, Numerics, PeriodicBC, Prim, Problem, SlipWallBC, Solver, SolverSpec, StepControl,
-                      Stretch, sine_cluster,
+                      Stretch, sine_cluster, CartesianMetric, CylindricalMetric, SphericalMetric,
                       Transport, conserved_from_prim, initialize_, run_, setup, single_species, tanh_blend)
 from .batch import Batch, Instance  # noqa: F401
 from .server import Server  # noqa: F401

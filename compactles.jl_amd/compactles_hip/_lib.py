@@ -57,7 +57,8 @@ class SolverDesc(C.Structure):
                 ("predict", C.c_double), ("max_growth", C.c_double), ("dt_min", C.c_double),
                 ("dt_min_ratio", C.c_double), ("precision", C.c_int), ("retries", C.c_int),
                 ("cfl_backoff", C.c_double), ("savepoint_interval", C.c_int), ("landing_steps", C.c_int),
-                ("eos_kind", C.c_int), ("T_guess", C.c_double), ("nasa9", Nasa9SpeciesDesc * MAXSP)]
+                ("eos_kind", C.c_int), ("T_guess", C.c_double), ("nasa9", Nasa9SpeciesDesc * MAXSP),
+                ("metric", C.c_int)]
 
 
 P = C.c_void_p
@@ -115,6 +116,7 @@ SIGNATURES = {
     "cles_solver_load_checkpoint": (C.c_int, [P, C.c_char_p, C.c_int]),
     "cles_solver_footprint": (C.c_int, [C.POINTER(SolverDesc), C.POINTER(C.c_size_t)]),
     "cles_solver_create_ex": (C.c_int, [P, C.POINTER(SolverDesc), C.POINTER(DP), PP]),
+    "cles_solver_create_geom": (C.c_int, [P, C.POINTER(SolverDesc), C.POINTER(DP), C.POINTER(DP), PP]),
     "cles_solver_memory": (C.c_int, [P, C.POINTER(C.c_size_t)]),
     "cles_server_create": (C.c_int, [C.c_int, C.c_size_t, C.c_int, PP]),
     "cles_server_destroy": (C.c_int, [P]),

@@ -214,6 +214,37 @@ class NSCBCInflowBC(BoundaryCondition):
 
 
 # --------------------------------------------------------------------------
+# Metrics (metric.jl:20-45): coordinates (x, y, z), (r, theta, z), (r, theta,
+# phi) with theta the polar angle; velocities are physical components.
+# --------------------------------------------------------------------------
+class CartesianMetric:
+    code = 0
+
+
+class CylindricalMetric:
+    code = 1
+
+
+class SphericalMetric:
+    code = 2
+
+
+_METRICS = {"cartesian": 0, "cylindrical": 1, "spherical": 2}
+
+
+def metric_code(metric) -> int:
+    if isinstance(metric, str):
+        if metric not in _METRICS:
+            raise ValueError(f"unknown metric {metric!r}")
+        return _METRICS[metric]
+    if isinstance(metric, type):
+        metric = metric()
+    if not isinstance(metric, (CartesianMetric, CylindricalMetric, SphericalMetric)):
+        raise ValueError(f"unknown metric {metric!r}")
+    return metric.code
+
+
+# --------------------------------------------------------------------------
 # Problem / Numerics (problem.jl:243-306)
 # --------------------------------------------------------------------------
 @dataclass
@@ -224,7 +255,7 @@ class Problem:
     name: str = "problem"
     eos: IdealMixture = field(default_factory=single_species)
     transport: Transport = field(default_factory=Transport)
-    metric: str = "cartesian"
+    metric: object = "cartesian"   # a *Metric() or its name
     sources: tuple = ()
 
 
@@ -249,7 +280,7 @@ _BETA = {"strain": 0, "gated_strain": 1, "dilatation": 2}
 def fill_common_desc(d: L.SolverDesc, prob: Problem, num: Numerics):
     """The fields of a solver descriptor that do not depend on the grid:
     EOS, transport, artificial properties, schemes, cfl, filter cadence,
-    step control and precision."""
+    step control, precision and metric."""
     from .nasa9 import Nasa9Mixture
     eos = prob.eos
     d.n_species = eos.nspecies()
@@ -279,6 +310,7 @@ def fill_common_desc(d: L.SolverDesc, prob: Problem, num: Numerics):
     if num.precision not in ("fp64", "fp32"):
         raise ValueError("precision must be 'fp64' or 'fp32'")
     d.precision = L.F64 if num.precision == "fp64" else L.F32
+    d.metric = metric_code(prob.metric)
 
 
 class SolverSpec:
@@ -286,8 +318,7 @@ class SolverSpec:
     state (what a Solver is created from, and what the server admits)."""
 
     def __init__(self, prob: Problem, num: Numerics):
-        if prob.metric != "cartesian":
-            raise ValueError("only CartesianMetric is available on the GPU backend")
+        self.metric = metric_code(prob.metric)
         stretch = tuple(num.stretch) if num.stretch is not None else (None, None, None)
         if len(stretch) != 3:
             raise ValueError("Numerics.stretch: one entry per dimension")
@@ -325,6 +356,9 @@ class SolverSpec:
                              np.asarray(stretch[d].dxdxi(np.clip(np.arange(self.n[d]) * self.h[d], 0.0, 1.0)),
                                         dtype=np.float64), (self.n[d],)))
                          for d in range(3))
+        # physical node coordinates of the stretched dimensions (the metric reads r, theta)
+        self.xnodes = tuple(None if stretch[d] is None else np.ascontiguousarray(self.xcoord(d), dtype=np.float64)
+                            for d in range(3))
         self.halo = tuple(num.n_halo if self.n[d] > 1 else 0 for d in range(3))
         self.padded = tuple(self.n[d] + 2 * self.halo[d] for d in range(3))
         self.desc = self._desc()
@@ -401,8 +435,10 @@ class Solver(SolverSpec):
         self.ctx = ctx
         h = C.c_void_p()
         if any(j is not None for j in self.jac):
-            ptrs = (L.DP * 3)(*[C.cast(None, L.DP) if j is None else j.ctypes.data_as(L.DP) for j in self.jac])
-            L.check(ctx.lib.cles_solver_create_ex(ctx.handle, C.byref(self.desc), ptrs, C.byref(h)))
+            def ptrs(arrs):
+                return (L.DP * 3)(*[C.cast(None, L.DP) if a is None else a.ctypes.data_as(L.DP) for a in arrs])
+            L.check(ctx.lib.cles_solver_create_geom(ctx.handle, C.byref(self.desc), ptrs(self.xnodes),
+                                                    ptrs(self.jac), C.byref(h)))
         else:
             L.check(ctx.lib.cles_solver_create(ctx.handle, C.byref(self.desc), C.byref(h)))
         self.handle = h

@@ -486,6 +486,8 @@ int cles_batch_create(cles_ctx *ctx, const cles_solver_desc *desc, int64_t count
             CLES_REQUIRE(d.bc[k][0] == CLES_BC_PERIODIC && d.bc[k][1] == CLES_BC_PERIODIC,
                          "batched instances are one-dimensional: dimensions 2 and 3 are collapsed and periodic");
         cles::validate_eos(d);
+        if (d.metric != CLES_METRIC_CARTESIAN)
+            throw cles::Error{CLES_ERR_UNSUPPORTED, "batched instances take the Cartesian metric"};
         if (d.retries != 0)
             throw cles::Error{CLES_ERR_UNSUPPORTED, "StepControl retries are not available for batched instances "
                                                     "(each instance stops at its own failure)"};

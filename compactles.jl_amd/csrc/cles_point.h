@@ -63,13 +63,39 @@ struct FluxArgs {
     W *flux[3][CLES_MAXSP + 4];
 };
 
-// Cartesian stretch (metric.jl:109-150 with the Cartesian metric): mapping
-// Jacobians m_d (dx/dxi) at the interior nodes of each stretched dimension
-// (NULL: uniform).  Scale factor h_d = max(m_d, tiny), J = h1 h2 h3.
+// Orthogonal metric of the grid (metric.jl:80-137) from node tables: kind
+// CLES_METRIC_*; m[d] the mapping Jacobian dx/dxi at the interior nodes of a
+// stretched dimension (NULL: 1); r the radius at the nodes of dimension 1
+// (curvilinear); sth, cth sin and cos of theta at the nodes of dimension 2
+// (spherical); cotr cot(theta)/r over the interior in its discrete-GCL form
+// (spherical with theta resolved, metric.jl:150-168; NULL: cos / (r sin)).
+// Scale factors h = (1, r, 1) cylindrical, (1, r, r sin theta) spherical,
+// times m_d, clamped to tiny; J = h1 h2 h3, A_d = J / h_d.
 struct Metric1D {
+    int kind;
     const double *m[3];
+    const double *r, *sth, *cth;
+    const double *cotr;
     int64_t n[3];
 };
+
+// Curvature terms of a curvilinear metric (metric.jl:181-269): corrections
+// added to the scaled velocity gradients, and the momentum sources of the
+// divergence of Pi.  zero[d]: dimension d is collapsed, its grad_u row holds
+// no derivative and takes the correction alone.
+template <typename W>
+struct CurvArgs {
+    Metric1D g;
+    int zero[3];
+    W *gu[9];
+    const W *u, *v, *w, *rho, *p, *mu_art, *beta_art;
+    double mu0;
+    int im1;     // first momentum component of dQ
+    int64_t N;
+    W *dQ;
+};
+template <typename W> void launch_metric_grad(const CurvArgs<W> &a, hipStream_t st);
+template <typename W> void launch_metric_sources(const CurvArgs<W> &a, hipStream_t st);
 
 template <typename W>
 struct RateArgs {
@@ -157,7 +183,8 @@ struct DiagArgs {
 
 // Metric scalings over nf fields of N points (interior, x fastest):
 // mode 0: f *= inv_h[dim] (rhs.jl:477-483 _scale_grad!); mode 1: f *= J / h_dim
-// (the area factor A_dim, rhs.jl:672-677); mode 2: f *= inv_J (rhs.jl:680-682).
+// (the area factor A_dim, rhs.jl:672-677); mode 2: f *= inv_J (rhs.jl:680-682);
+// mode 3: f = J / h_dim (metric.jl:124-126 area_d).
 constexpr int METRIC_MAXF = 12;
 template <typename W>
 struct MetricFields {

@@ -478,15 +478,49 @@ __global__ void __launch_bounds__(TPB) k_lsrk(double *__restrict__ Q, double *__
 // mixture density, then a single-block final reduction (order-independent).
 // ---------------------------------------------------------------------------
 // timestep.jl:146-170: the rate of point I (the mixture density goes to rQ)
-__device__ __forceinline__ double metric_h(const Metric1D &g, int d, int64_t i)
-{
-    return g.m[d] ? fmax(1.0 * g.m[d][i], 1e-300) : 1.0;
-}
 __device__ __forceinline__ void interior_ijk(const int64_t (&n)[3], int64_t I, int64_t (&ii)[3])
 {
     ii[0] = I % n[0];
     ii[1] = (I / n[0]) % n[1];
     ii[2] = I / (n[0] * n[1]);
+}
+// metric.jl:114-120: the clamped scale factors at interior node ii
+__device__ __forceinline__ void metric_hs(const Metric1D &g, const int64_t (&ii)[3], double (&h)[3])
+{
+    double s2 = 1.0, s3 = 1.0;
+    if (g.kind != CLES_METRIC_CARTESIAN) {
+        const double r = g.r[ii[0]];
+        s2 = r;
+        if (g.kind == CLES_METRIC_SPHERICAL) s3 = r * g.sth[ii[1]];
+    }
+    h[0] = fmax(1.0 * (g.m[0] ? g.m[0][ii[0]] : 1.0), 1e-300);
+    h[1] = fmax(s2 * (g.m[1] ? g.m[1][ii[1]] : 1.0), 1e-300);
+    h[2] = fmax(s3 * (g.m[2] ? g.m[2][ii[2]] : 1.0), 1e-300);
+}
+__device__ __forceinline__ double metric_h(const Metric1D &g, int d, const int64_t (&ii)[3])
+{
+    double h[3];
+    metric_hs(g, ii, h);
+    return h[d];
+}
+__device__ __forceinline__ double metric_J(const Metric1D &g, const int64_t (&ii)[3])
+{
+    double h[3];
+    metric_hs(g, ii, h);
+    return h[0] * h[1] * h[2];
+}
+// metric.jl:129-133: 1/r and cot(theta)/r
+__device__ __forceinline__ double metric_inv_r(const Metric1D &g, const int64_t (&ii)[3])
+{
+    return 1 / fmax(g.r[ii[0]], 1e-300);
+}
+__device__ __forceinline__ double metric_cotr(const Metric1D &g, const int64_t (&ii)[3], int64_t I)
+{
+    return g.cotr ? g.cotr[I] : g.cth[ii[1]] / fmax(g.r[ii[0]] * g.sth[ii[1]], 1e-300);
+}
+__device__ __forceinline__ bool metric_on(const Metric1D &g, int d)
+{
+    return g.kind != CLES_METRIC_CARTESIAN || g.m[d];
 }
 
 template <typename W, int MODE>
@@ -495,14 +529,77 @@ __global__ void __launch_bounds__(TPB) k_metric_scale(MetricFields<W> fs, Metric
     for (int64_t I = (int64_t)blockIdx.x * TPB + threadIdx.x; I < N; I += (int64_t)gridDim.x * TPB) {
         int64_t ii[3];
         interior_ijk(g.n, I, ii);
+        double h[3];
+        metric_hs(g, ii, h);
         double sc;
         if constexpr (MODE == 0) {
-            sc = 1 / metric_h(g, dim, ii[dim]);
+            sc = 1 / h[dim];
         } else {
-            const double J = metric_h(g, 0, ii[0]) * metric_h(g, 1, ii[1]) * metric_h(g, 2, ii[2]);
-            sc = MODE == 1 ? J / metric_h(g, dim, ii[dim]) : 1 / J;
+            const double J = h[0] * h[1] * h[2];
+            sc = MODE == 2 ? 1 / J : J / h[dim];
         }
-        for (int f = 0; f < fs.nf; f++) fs.f[f][I] = (W)((double)fs.f[f][I] * sc);
+        if constexpr (MODE == 3)
+            for (int f = 0; f < fs.nf; f++) fs.f[f][I] = (W)sc;
+        else
+            for (int f = 0; f < fs.nf; f++) fs.f[f][I] = (W)((double)fs.f[f][I] * sc);
+    }
+}
+
+// metric.jl:181-219 metric_correct_gradients! (grad_u[d][j] at gu[d*3 + j])
+template <typename W>
+__global__ void __launch_bounds__(TPB) k_metric_grad(CurvArgs<W> a)
+{
+    const int64_t I = (int64_t)blockIdx.x * TPB + threadIdx.x;
+    if (I >= a.N) return;
+    int64_t ii[3];
+    interior_ijk(a.g.n, I, ii);
+    const double ir = metric_inv_r(a.g, ii);
+    const double u = a.u[I], v = a.v[I], w = a.w[I];
+    auto G = [&](int k) -> double { return a.zero[k / 3] ? 0.0 : (double)a.gu[k][I]; };
+    if (a.g.kind == CLES_METRIC_CYLINDRICAL) {
+        a.gu[3][I] = (W)(G(3) - v * ir);
+        a.gu[4][I] = (W)(G(4) + u * ir);
+        if (a.zero[1]) a.gu[5][I] = (W)0;
+    } else {
+        const double ctr = metric_cotr(a.g, ii, I);
+        a.gu[3][I] = (W)(G(3) - v * ir);
+        a.gu[4][I] = (W)(G(4) + u * ir);
+        if (a.zero[1]) a.gu[5][I] = (W)0;
+        a.gu[6][I] = (W)(G(6) - w * ir);
+        a.gu[7][I] = (W)(G(7) - w * ctr);
+        a.gu[8][I] = (W)(G(8) + (u * ir + v * ctr));
+    }
+}
+
+// metric.jl:228-269 add_metric_sources! with _Pi (a, b zero-based)
+template <typename W>
+__global__ void __launch_bounds__(TPB) k_metric_sources(CurvArgs<W> a)
+{
+    const int64_t I = (int64_t)blockIdx.x * TPB + threadIdx.x;
+    if (I >= a.N) return;
+    int64_t ii[3];
+    interior_ijk(a.g.n, I, ii);
+    auto G = [&](int k) -> double { return a.gu[k] ? (double)a.gu[k][I] : 0.0; };
+    const double mu = a.mu0 + (a.mu_art ? (double)a.mu_art[I] : 0.0);
+    const double be = a.beta_art ? (double)a.beta_art[I] : 0.0;
+    const double divu = G(0) + G(4) + G(8);
+    const double rho = a.rho[I], p = a.p[I];
+    const double uv[3] = {(double)a.u[I], (double)a.v[I], (double)a.w[I]};
+    auto Pi = [&](int x, int y) -> double {
+        const double tau = mu * (G(x * 3 + y) + G(y * 3 + x)) + (x == y ? (be - 2 * mu / 3) * divu : 0.0);
+        return rho * uv[x] * uv[y] + (x == y ? p : 0.0) - tau;
+    };
+    const double ir = metric_inv_r(a.g, ii);
+    W *m1 = a.dQ + (int64_t)a.im1 * a.N + I, *m2 = m1 + a.N, *m3 = m2 + a.N;
+    if (a.g.kind == CLES_METRIC_CYLINDRICAL) {
+        *m1 = (W)((double)*m1 + ir * Pi(1, 1));
+        *m2 = (W)((double)*m2 - ir * Pi(1, 0));
+    } else {
+        const double ctr = metric_cotr(a.g, ii, I);
+        const double Ptt = Pi(1, 1), Ppp = Pi(2, 2);
+        *m1 = (W)((double)*m1 + ir * (Ptt + Ppp));
+        *m2 = (W)((double)*m2 + (-ir * Pi(1, 0) + ctr * Ppp));
+        *m3 = (W)((double)*m3 + (-ir * Pi(2, 0) - ctr * Pi(2, 1)));
     }
 }
 
@@ -520,13 +617,26 @@ __device__ __forceinline__ double rate_at(const RateArgs<W> &a, int64_t I, int64
     for (int d = 0; d < 3; d++) {
         if (!a.active[d]) continue;
         double idx = a.invh[d];
-        if (a.met.m[d]) {   // timestep.jl:151 inv_h[d][I] / h[d]
+        if (metric_on(a.met, d)) {   // timestep.jl:151 inv_h[d][I] / h[d]
             int64_t ii[3];
             interior_ijk(a.met.n, I, ii);
-            idx = (1 / metric_h(a.met, d, ii[d])) / a.h[d];
+            idx = (1 / metric_h(a.met, d, ii)) / a.h[d];
         }
         acc += (fabs(uv[d]) + c) * idx;
         dsum += idx * idx;
+    }
+    if (a.met.kind != CLES_METRIC_CARTESIAN) {   // timestep.jl:186-197 curvature_rate
+        int64_t ii[3];
+        interior_ijk(a.met.n, I, ii);
+        const double ir = metric_inv_r(a.met, ii);
+        if (a.met.kind == CLES_METRIC_CYLINDRICAL) {
+            if (!a.active[1]) acc += fabs(uv[1]) * ir;
+        } else {
+            double cr = 0.0;
+            if (!a.active[1]) cr += fabs(uv[1]) * ir;
+            if (!a.active[2]) cr += fabs(uv[2]) * (ir + fabs(metric_cotr(a.met, ii, I)));
+            acc += cr;
+        }
     }
     double Dmax = a.mu0 / (a.Sc * rho);
     for (int sp = 0; sp < a.Ns; sp++)
@@ -796,9 +906,8 @@ __global__ void __launch_bounds__(TPB) k_integrand_sum(DiagArgs<W> a, double *__
             const double w1 = G(1, 2) - G(2, 1), w2 = G(2, 0) - G(0, 2), w3 = G(0, 1) - G(1, 0);
             f = 0.5 * (w1 * w1 + w2 * w2 + w3 * w3);
         }
-        if (a.met.m[0] || a.met.m[1] || a.met.m[2]) {   // diagnostics.jl:73-75: f / inv_J
-            const double inv_J = 1 / (metric_h(a.met, 0, ii[0]) * metric_h(a.met, 1, ii[1]) *
-                                      metric_h(a.met, 2, ii[2]));
+        if (metric_on(a.met, 0) || a.met.m[1] || a.met.m[2]) {   // diagnostics.jl:73-75: f / inv_J
+            const double inv_J = 1 / metric_J(a.met, ii);
             acc += wq * f / inv_J;
         } else {
             acc += wq * f;
@@ -1239,10 +1348,30 @@ void launch_metric_scale(const MetricFields<W> &fs, const Metric1D &g, int mode,
         hipLaunchKernelGGL((k_metric_scale<W, 0>), dim3(nb), dim3(TPB), 0, st, fs, g, dim, N);
     else if (mode == 1)
         hipLaunchKernelGGL((k_metric_scale<W, 1>), dim3(nb), dim3(TPB), 0, st, fs, g, dim, N);
+    else if (mode == 3)
+        hipLaunchKernelGGL((k_metric_scale<W, 3>), dim3(nb), dim3(TPB), 0, st, fs, g, dim, N);
     else
         hipLaunchKernelGGL((k_metric_scale<W, 2>), dim3(nb), dim3(TPB), 0, st, fs, g, dim, N);
     CHECK_LAUNCH();
 }
+template <typename W>
+void launch_metric_grad(const CurvArgs<W> &a, hipStream_t st)
+{
+    if (a.N <= 0) return;
+    hipLaunchKernelGGL((k_metric_grad<W>), dim3(nblocks(a.N)), dim3(TPB), 0, st, a);
+    CHECK_LAUNCH();
+}
+template <typename W>
+void launch_metric_sources(const CurvArgs<W> &a, hipStream_t st)
+{
+    if (a.N <= 0) return;
+    hipLaunchKernelGGL((k_metric_sources<W>), dim3(nblocks(a.N)), dim3(TPB), 0, st, a);
+    CHECK_LAUNCH();
+}
+template void launch_metric_grad<double>(const CurvArgs<double> &, hipStream_t);
+template void launch_metric_grad<float>(const CurvArgs<float> &, hipStream_t);
+template void launch_metric_sources<double>(const CurvArgs<double> &, hipStream_t);
+template void launch_metric_sources<float>(const CurvArgs<float> &, hipStream_t);
 template void launch_metric_scale<double>(const MetricFields<double> &, const Metric1D &, int, int, int64_t,
                                           hipStream_t);
 template void launch_metric_scale<float>(const MetricFields<float> &, const Metric1D &, int, int, int64_t,

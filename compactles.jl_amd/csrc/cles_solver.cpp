@@ -42,10 +42,11 @@ struct cles_solver {
     bool nscbc = false;
     cles::HostPlan dplan[3], fplan[3], fplanQ[3];
     bool own_fplanQ = false;
-    // Stretch (metric.jl:45-58): mapping Jacobians at the nodes of stretched
-    // dimensions, on the device (NULL: uniform)
-    bool stretched = false;
+    // Metric and Stretch (metric.jl:45-137): node tables on the device;
+    // stretched: some dimension has a mapping, curv: a curvilinear metric
+    bool stretched = false, curv = false;
     cles::Metric1D met{};
+    bool geom() const { return stretched || curv; }
     const cles::N9Table *n9 = nullptr;   // Nasa9Mixture tables on the device (NULL: IdealMixture)
     std::vector<void *> allocs;
     size_t bytes = 0;                          // device bytes allocated by this solver
@@ -300,8 +301,40 @@ struct Impl {
                     }
             }
             lines(s.dplan[d], d, nf, in, out, nullptr, store(out[0]), PH_DERIV);
-            if (s.met.m[d]) metric(out, nf, 0, d);   // rhs.jl:599,631,634 _scale_grad!
+            if (s.met.m[d] || s.curv) metric(out, nf, 0, d);   // rhs.jl:599,631,634 _scale_grad!
         }
+        if (s.curv) {   // rhs.jl:604 metric_correct_gradients!
+            const CurvArgs<W> a = curv_args();
+            s.timed(PH_OTHER, sizeof(W) * 10.0 * (double)s.N, [&] { launch_metric_grad<W>(a, s.st()); });
+        }
+    }
+
+    CurvArgs<W> curv_args() const
+    {
+        CurvArgs<W> a{};
+        a.g = s.met;
+        for (int k = 0; k < 3; k++) a.zero[k] = !s.active[k];
+        for (int k = 0; k < 9; k++) a.gu[k] = f(s.gu[k]);
+        a.u = f(s.u); a.v = f(s.v); a.w = f(s.w); a.rho = f(s.rho); a.p = f(s.p);
+        a.mu_art = f(s.mu_art);
+        a.beta_art = f(s.beta_art);
+        a.mu0 = s.d.mu0;
+        a.im1 = s.Ns;
+        a.N = s.N;
+        a.dQ = f(s.dQ);
+        return a;
+    }
+
+    // metric.jl:150-168 gcl_cotr!: cot(theta)/r := inv_J D_xi2(A_2), with
+    // A_2 = J / h_2 filled over the grid and differentiated along theta
+    void gcl_cotr(double *cotr) const
+    {
+        void *a2 = s.flux[s.adim[0]][0];   // flux scratch, free outside the RHS
+        void *da = s.flux[s.adim[0]][1];
+        metric(&a2, 1, 3, 1);
+        line(s.dplan[1], 1, a2, store(da), PH_OTHER);
+        metric(&da, 1, 2, 0);
+        launch_convert<double, W>(cotr, f(da), s.N, s.st());
     }
 
     // metric scalings of nf fields (launch_metric_scale modes)
@@ -458,7 +491,7 @@ struct Impl {
                 eoff[c] = (int64_t)c * s.N;
             }
             // rhs.jl:672-677: the divergence of A_d F_d (the fluxes are scaled in place)
-            if (s.stretched) metric(s.flux[d], s.C, 1, d);
+            if (s.geom()) metric(s.flux[d], s.C, 1, d);
             EpiArgs e{};
             if (fuse && last) {
                 e.kind = s.dev_dt ? EPI_LSRK_DEV : EPI_LSRK;
@@ -476,10 +509,14 @@ struct Impl {
                 lines(s.dplan[d], d, s.C, in, out, nullptr, e, PH_DIV);
             }
         }
-        if (s.stretched) {   // rhs.jl:680-682: dQ -= inv_J D_d(A_d F_d)
+        if (s.geom()) {   // rhs.jl:680-682: dQ -= inv_J D_d(A_d F_d)
             void *dq[LINE_MF];
             for (int c = 0; c < s.C; c++) dq[c] = dQc(c);
             metric(dq, s.C, 2, 0);
+        }
+        if (s.curv) {   // rhs.jl:687 add_metric_sources!
+            const CurvArgs<W> a = curv_args();
+            s.timed(PH_OTHER, sizeof(W) * 20.0 * (double)s.N, [&] { launch_metric_sources<W>(a, s.st()); });
         }
     }
 
@@ -561,7 +598,7 @@ struct Impl {
     // timestep.jl:72-99 step!
     void step(double dt, bool prepared) const
     {
-        const bool fuse = !s.nscbc && !s.stretched;
+        const bool fuse = !s.nscbc && !s.geom();
         for (int stg = 0; stg < 5; stg++) {
             s.tstage = s.t + RKC[stg] * dt;
             const bool first_prepared = prepared && stg == 0;
@@ -805,6 +842,8 @@ void validate_desc(const cles_solver_desc &d)
         throw cles::Error{CLES_ERR_INVALID, "StepControl: cfl_backoff must be positive"};
     if (d.savepoint_interval < 0) throw cles::Error{CLES_ERR_INVALID, "StepControl: savepoint_interval must be >= 0"};
     cles::validate_eos(d);
+    if (d.metric < CLES_METRIC_CARTESIAN || d.metric > CLES_METRIC_SPHERICAL)
+        throw cles::Error{CLES_ERR_INVALID, "metric must be CLES_METRIC_CARTESIAN, _CYLINDRICAL or _SPHERICAL"};
     if (d.landing_steps < 0)
         throw cles::Error{CLES_ERR_INVALID, "StepControl: landing_steps must be >= 1 (1 is a hard clip onto the "
                                             "scheduled time)"};
@@ -917,6 +956,10 @@ int cles_solver_footprint(const cles_solver_desc *desc, size_t *bytes)
         fields += 3 + (Ns > 1 ? Ns : 0);                          // mu*, beta*, kappa*, D*
         if (d.art_enabled) fields += 3 + (d.beta_sensor == 2 ? 2 : 0);
         if (nscbc) fields += 3;
+        if (d.metric != CLES_METRIC_CARTESIAN) {   // grad_u rows of collapsed angular dims, GCL cot/r
+            fields += 3.0 * ((d.n_global[1] == 1) + (d.metric == CLES_METRIC_SPHERICAL && d.n_global[2] == 1));
+            if (d.metric == CLES_METRIC_SPHERICAL && d.n_global[1] > 1) fields += 8.0 / w;
+        }
         const double total = N * C * 16.0 + N * C * w + N * w * fields + 1024.0 * 2 * 8 + 16 * 8 + plans;
         *bytes = (size_t)total;
     })
@@ -943,6 +986,12 @@ int cles_solver_create(cles_ctx *ctx, const cles_solver_desc *desc, cles_solver 
 int cles_solver_create_ex(cles_ctx *ctx, const cles_solver_desc *desc, const double *const dxdxi[3],
                           cles_solver **out)
 {
+    return cles_solver_create_geom(ctx, desc, nullptr, dxdxi, out);
+}
+
+int cles_solver_create_geom(cles_ctx *ctx, const cles_solver_desc *desc, const double *const x[3],
+                            const double *const dxdxi[3], cles_solver **out)
+{
     CLES_TRY({
         CLES_REQUIRE(ctx && desc && out, "NULL argument");
         validate_desc(*desc);
@@ -962,6 +1011,19 @@ int cles_solver_create_ex(cles_ctx *ctx, const cles_solver_desc *desc, const dou
         }
         if (any_stretch && any_nscbc)
             throw cles::Error{CLES_ERR_UNSUPPORTED, "NSCBC faces on stretched grids are not available on the GPU backend"};
+        const int kind = desc->metric;
+        if (kind != CLES_METRIC_CARTESIAN && any_nscbc)
+            throw cles::Error{CLES_ERR_UNSUPPORTED,
+                              "NSCBC faces under a curvilinear metric are not available on the GPU backend"};
+        // the metric reads r (dimension 1) and, spherical, theta (dimension 2)
+        auto reads = [&](int k) {
+            return (kind != CLES_METRIC_CARTESIAN && k == 0) || (kind == CLES_METRIC_SPHERICAL && k == 1);
+        };
+        for (int k = 0; k < 3; k++)
+            if (reads(k) && dxdxi && dxdxi[k] && !(x && x[k]))
+                throw cles::Error{CLES_ERR_INVALID, "dimension " + std::to_string(k + 1) +
+                                                        ": a stretched curvilinear coordinate needs its node "
+                                                        "coordinates x"};
         CLES_HIP(hipSetDevice(ctx->device));
         auto *s = new cles_solver;
         try {
@@ -990,6 +1052,7 @@ int cles_solver_create_ex(cles_ctx *ctx, const cles_solver_desc *desc, const dou
                     s->stretched = true;
                 }
             }
+            s->curv = kind != CLES_METRIC_CARTESIAN;
             for (int k = 0; k < s->Ns; k++) {   // physics.jl:111-115
                 s->Rk[k] = desc->R[k];
                 s->cvk[k] = desc->eos_kind == CLES_EOS_NASA9 ? 0.0 : desc->R[k] / (desc->gamma[k] - 1);
@@ -1026,6 +1089,10 @@ int cles_solver_create_ex(cles_ctx *ctx, const cles_solver_desc *desc, const dou
             for (void **fp : {&s->rho, &s->u, &s->v, &s->w, &s->p, &s->T, &s->c, &s->cp}) *fp = s->wfield();
             for (int k = 0; k < s->Ns; k++) s->Y[k] = s->wfield();
             for (int dd = 0; dd < 3; dd++) {
+                // a collapsed angular dimension still carries the curvature
+                // terms of grad_u (metric.jl:181-219)
+                if (!s->active[dd] && s->curv && (dd == 1 || (dd == 2 && kind == CLES_METRIC_SPHERICAL)))
+                    for (int j = 0; j < 3; j++) s->gu[dd * 3 + j] = s->wfield();
                 if (!s->active[dd]) continue;
                 for (int j = 0; j < 3; j++) s->gu[dd * 3 + j] = s->wfield();
                 s->gT[dd] = s->wfield();
@@ -1063,11 +1130,48 @@ int cles_solver_create_ex(cles_ctx *ctx, const cles_solver_desc *desc, const dou
                 CLES_HIP(hipMemcpy(m, dxdxi[k], (size_t)s->n[k] * sizeof(double), hipMemcpyHostToDevice));
                 s->met.m[k] = m;
             }
+            s->met.kind = kind;
+            if (s->curv) {
+                // metric.jl:94-103: node coordinates origin + i h, or the mapping's
+                auto node = [&](int k, int64_t i) {
+                    return (x && x[k]) ? x[k][i] : desc->origin[k] + (double)i * s->h[k];
+                };
+                std::vector<double> r((size_t)s->n[0]);
+                for (int64_t i = 0; i < s->n[0]; i++) {
+                    r[i] = node(0, i);
+                    CLES_REQUIRE(r[i] > 0 && std::isfinite(r[i]),
+                                 "curvilinear metric: every node needs r > 0 (coordinate singularities are not "
+                                 "treated on the GPU backend)");
+                }
+                double *rd = (double *)s->alloc(r.size() * sizeof(double));
+                CLES_HIP(hipMemcpy(rd, r.data(), r.size() * sizeof(double), hipMemcpyHostToDevice));
+                s->met.r = rd;
+                if (kind == CLES_METRIC_SPHERICAL) {
+                    std::vector<double> sc(2 * (size_t)s->n[1]);
+                    for (int64_t j = 0; j < s->n[1]; j++) {
+                        const double th = node(1, j);
+                        CLES_REQUIRE(th > 0 && th < M_PI,
+                                     "SphericalMetric: every node needs 0 < theta < pi (the poles are not treated "
+                                     "on the GPU backend)");
+                        sc[j] = std::sin(th);
+                        sc[s->n[1] + j] = std::cos(th);
+                    }
+                    double *sd = (double *)s->alloc(sc.size() * sizeof(double));
+                    CLES_HIP(hipMemcpy(sd, sc.data(), sc.size() * sizeof(double), hipMemcpyHostToDevice));
+                    s->met.sth = sd;
+                    s->met.cth = sd + s->n[1];
+                }
+            }
             s->part = (double *)s->alloc((size_t)s->nred * 2 * sizeof(double));
             s->scal = (double *)s->alloc(16 * sizeof(double));
             CLES_HIP(hipHostMalloc((void **)&s->host_scal, 16 * sizeof(double), hipHostMallocDefault));
             s->ctl = (cles_solver::Ctl *)s->alloc(sizeof(cles_solver::Ctl));
             CLES_HIP(hipHostMalloc((void **)&s->host_ctl, sizeof(cles_solver::Ctl), hipHostMallocDefault));
+            if (kind == CLES_METRIC_SPHERICAL && s->active[1]) {
+                double *cotr = (double *)s->alloc((size_t)s->N * sizeof(double));
+                dispatch(s, [&](auto &im) { im.gcl_cotr(cotr); });
+                s->met.cotr = cotr;
+            }
             CLES_HIP(hipStreamSynchronize(ctx->stream));
         } catch (...) {
             delete s;

@@ -20,7 +20,7 @@
 extern "C" {
 #endif
 
-#define CLES_ABI_VERSION 3
+#define CLES_ABI_VERSION 4
 
 /* Status codes.  The solver-failure codes mirror SolverFailure reasons
  * (src/stepcontrol.jl:138-164): nonfinite, negative_density, planck, dt_min,
@@ -200,9 +200,14 @@ typedef struct {
     int eos_kind;
     double T_guess;
     cles_nasa9_species nasa9[CLES_MAXSP];
+    /* Metric (metric.jl:20-45): coordinates (x, y, z), (r, theta, z) or
+     * (r, theta, phi) with theta the polar angle; velocity components are
+     * physical components in the local orthonormal basis. */
+    int metric;                            /* CLES_METRIC_* */
 } cles_solver_desc;
 
 enum { CLES_EOS_IDEAL = 0, CLES_EOS_NASA9 = 1 };
+enum { CLES_METRIC_CARTESIAN = 0, CLES_METRIC_CYLINDRICAL = 1, CLES_METRIC_SPHERICAL = 2 };
 
 /* save_checkpoint / load_checkpoint! (src/io.jl:6-75): `prefix`.rNNNN.ckpt
  * with the reference's header (magic, n_cons, n_global, n_local, rank
@@ -233,6 +238,21 @@ int cles_solver_create(cles_ctx *ctx, const cles_solver_desc *desc, cles_solver 
  * stretched grids (CLES_ERR_UNSUPPORTED). */
 int cles_solver_create_ex(cles_ctx *ctx, const cles_solver_desc *desc, const double *const dxdxi[3],
                           cles_solver **solver);
+/* setup with a metric and Stretch mappings (src/metric.jl:80-269, rhs.jl:
+ * 122-311): as cles_solver_create_ex, and x[d] (NULL, or x itself NULL, for
+ * a dimension with uniform nodes origin[d] + i h[d]) holds the physical
+ * coordinate at the nodes of a stretched dimension whose coordinate the
+ * metric reads: r (dimension 1) under CLES_METRIC_CYLINDRICAL and
+ * CLES_METRIC_SPHERICAL, theta (dimension 2) under CLES_METRIC_SPHERICAL.
+ * A curvilinear metric adds the curvature corrections of the velocity
+ * gradients (metric.jl:181-219), the momentum sources (metric.jl:228-269),
+ * the discrete-GCL cot(theta)/r of a resolved theta (metric.jl:150-168) and
+ * the curvature rate of collapsed angular dimensions (timestep.jl:186-197).
+ * Coordinate singularities are not treated: every node needs r > 0 and, for
+ * the spherical metric, 0 < theta < pi (CLES_ERR_INVALID otherwise); NSCBC
+ * faces are refused under a curvilinear metric (CLES_ERR_UNSUPPORTED). */
+int cles_solver_create_geom(cles_ctx *ctx, const cles_solver_desc *desc, const double *const x[3],
+                            const double *const dxdxi[3], cles_solver **solver);
 int cles_solver_destroy(cles_solver *solver);
 
 /* Conserved state transfer in Julia's padded layout (Q[..., c], column-major,

@@ -550,8 +550,10 @@ struct or_solver {
     double *flux[3][OR_MAXSP + 4];
     double t, tstage, dt_prev, rate_prev;
     int64_t step;
-    double *mjac[3];                     /* Stretch dx/dxi at the interior nodes (NULL: uniform) */
-    int stretched;
+    /* metric.jl:109-137 geometry over the padded extent, allocated when a
+     * Stretch or a curvilinear metric makes it non-unit (geom = 1) */
+    int geom;
+    double *inv_J, *area[3], *inv_h[3], *inv_r, *cotr;
 };
 
 #define FOR_INTERIOR(s, I)                                                        \
@@ -566,53 +568,103 @@ static inline int64_t pidx(const or_solver *s, int64_t i, int64_t j, int64_t k)
 
 static double *fld(or_solver *s) { return calloc((size_t)s->NP, sizeof(double)); }
 
-/* metric.jl:109-150 (Cartesian): the scale factor of dimension d at padded
- * index ip along d is max(1 * m_d, tiny), m_d the mapping Jacobian at the
- * node (clamped into the map's domain for halo layers) or 1 when uniform. */
-static inline double hmet(const or_solver *s, int d, int64_t ip)
-{
-    if (!s->mjac[d]) return 1.0;
-    int64_t i = ip - s->H[d];
-    if (i < 0) i = 0;
-    if (i > s->n[d] - 1) i = s->n[d] - 1;
-    return fmax(1.0 * s->mjac[d][i], 1e-300);
-}
-static inline void padded_ijk(const or_solver *s, int64_t I, int64_t *ip)
-{
-    ip[0] = I % s->pad[0];
-    ip[1] = (I / s->pad[0]) % s->pad[1];
-    ip[2] = I / (s->pad[0] * s->pad[1]);
-}
-static inline double jac_at(const or_solver *s, int64_t I)
-{
-    int64_t ip[3];
-    padded_ijk(s, I, ip);
-    return hmet(s, 0, ip[0]) * hmet(s, 1, ip[1]) * hmet(s, 2, ip[2]);
-}
-
 /* rhs.jl:477-483 _scale_grad!: g *= inv_h[d] over the full array */
 static void scale_grad(or_solver *s, double *g, int d)
 {
-    if (!s->mjac[d]) return;   /* inv_h == 1 */
+    if (!s->geom) return;   /* inv_h == 1 */
+    const double *ih = s->inv_h[d];
 #pragma omp parallel for schedule(static)
-    for (int64_t I = 0; I < s->NP; I++) {
-        int64_t ip[3];
-        padded_ijk(s, I, ip);
-        const double ih = 1 / hmet(s, d, ip[d]);
-        g[I] *= ih;
+    for (int64_t I = 0; I < s->NP; I++) g[I] *= ih[I];
+}
+
+static void deriv_along(or_solver *s, double *out, const double *f, int d);
+
+/* metric.jl:94-103 _phys_and_jac: the physical coordinate and mapping
+ * Jacobian at padded index ip along d.  A stretched dimension takes its
+ * values at the nearest interior node (halo layers beyond closed edges are
+ * never read). */
+static void phys_and_jac(const or_solver *s, int d, int64_t ip, const double *const *x, const double *const *jac,
+                         double *xo, double *mo)
+{
+    const double xi = s->d.origin[d] + (double)(ip - s->H[d]) * s->h[d];
+    if (!jac || !jac[d]) {
+        *xo = xi;
+        *mo = 1.0;
+        return;
+    }
+    int64_t i = ip - s->H[d];
+    if (i < 0) i = 0;
+    if (i > s->n[d] - 1) i = s->n[d] - 1;
+    *xo = (x && x[d]) ? x[d][i] : xi;
+    *mo = jac[d][i];
+}
+
+/* metric.jl:109-137 init_geometry! and :150-168 gcl_cotr! */
+static void init_geometry(or_solver *s, const double *const *x, const double *const *jac)
+{
+    const double tiny = 1e-300;
+    s->inv_J = fld(s);
+    for (int d = 0; d < 3; d++) { s->area[d] = fld(s); s->inv_h[d] = fld(s); }
+    s->inv_r = fld(s);
+    s->cotr = fld(s);
+    const int kind = s->d.metric;
+    for (int64_t k = 0; k < s->pad[2]; k++)
+        for (int64_t j = 0; j < s->pad[1]; j++)
+            for (int64_t i = 0; i < s->pad[0]; i++) {
+                double x1, m1, x2, m2, x3, m3;
+                phys_and_jac(s, 0, i, x, jac, &x1, &m1);
+                phys_and_jac(s, 1, j, x, jac, &x2, &m2);
+                phys_and_jac(s, 2, k, x, jac, &x3, &m3);
+                /* metric.jl:80-82 scalefactors */
+                double h1 = 1.0, h2 = 1.0, h3 = 1.0;
+                if (kind == 1) h2 = x1;
+                else if (kind == 2) { h2 = x1; h3 = x1 * sin(x2); }
+                h1 *= m1; h2 *= m2; h3 *= m3;
+                h1 = fmax(h1, tiny); h2 = fmax(h2, tiny); h3 = fmax(h3, tiny);
+                const double J = h1 * h2 * h3;
+                const int64_t I = i + s->pad[0] * (j + s->pad[1] * k);
+                s->inv_J[I] = 1 / J;
+                s->area[0][I] = J / h1;
+                s->area[1][I] = J / h2;
+                s->area[2][I] = J / h3;
+                s->inv_h[0][I] = 1 / h1;
+                s->inv_h[1][I] = 1 / h2;
+                s->inv_h[2][I] = 1 / h3;
+                if (kind >= 1) s->inv_r[I] = 1 / fmax(x1, tiny);
+                if (kind == 2) s->cotr[I] = cos(x2) / fmax(x1 * sin(x2), tiny);
+            }
+    if (kind == 2 && s->active[1]) {
+        /* gcl_cotr!: cot(theta)/r := inv_J D_xi2(A_2) at the interior nodes */
+        deriv_along(s, s->tmp_a, s->area[1], 1);
+        for (int64_t k = 0; k < s->n[2]; k++)
+            for (int64_t j = 0; j < s->n[1]; j++)
+                for (int64_t i = 0; i < s->n[0]; i++) {
+                    const int64_t I = pidx(s, i, j, k);
+                    s->cotr[I] = s->inv_J[I] * s->tmp_a[I];
+                }
     }
 }
 
 or_solver *or_solver_create(const or_desc *d, char *msg, int msglen)
 {
-    return or_solver_create_ex(d, NULL, msg, msglen);
+    return or_solver_create_geom(d, NULL, NULL, msg, msglen);
+}
+
+or_solver *or_solver_create_ex(const or_desc *d, const double *const jac[3], char *msg, int msglen)
+{
+    return or_solver_create_geom(d, NULL, jac, msg, msglen);
 }
 
 /* setup with Stretch mappings (metric.jl:45-58): jac[d] holds dx/dxi at the
  * n_global[d] nodes xi = i / (n - 1) of a stretched (non-periodic) dimension;
  * its computational spacing is h = 1 / (n - 1) (rhs.jl:191-201). */
-or_solver *or_solver_create_ex(const or_desc *d, const double *const jac[3], char *msg, int msglen)
+or_solver *or_solver_create_geom(const or_desc *d, const double *const x[3], const double *const jac[3], char *msg,
+                                 int msglen)
 {
+    if (d->metric < 0 || d->metric > 2) {
+        snprintf(msg, (size_t)msglen, "metric must be Cartesian, cylindrical or spherical");
+        return NULL;
+    }
     or_solver *s = calloc(1, sizeof(or_solver));
     s->d = *d;
     s->Ns = d->n_species;
@@ -638,11 +690,17 @@ or_solver *or_solver_create_ex(const or_desc *d, const double *const jac[3], cha
                 return NULL;
             }
             s->h[k] = 1.0 / (double)(d->n_global[k] - 1);
-            s->mjac[k] = malloc(sizeof(double) * (size_t)s->n[k]);
-            memcpy(s->mjac[k], jac[k], sizeof(double) * (size_t)s->n[k]);
-            s->stretched = 1;
+            s->geom = 1;
+            /* the metric reads r (dim 1) and, spherical, theta (dim 2) */
+            if (((d->metric >= 1 && k == 0) || (d->metric == 2 && k == 1)) && !(x && x[k])) {
+                snprintf(msg, (size_t)msglen, "dimension %d: a stretched curvilinear coordinate needs its node "
+                                              "coordinates", k + 1);
+                free(s);
+                return NULL;
+            }
         }
     }
+    if (d->metric != 0) s->geom = 1;
     /* physics.jl:111-115 */
     for (int k = 0; k < s->Ns; k++) {
         s->Rk[k] = d->R[k];
@@ -671,6 +729,7 @@ or_solver *or_solver_create_ex(const or_desc *d, const double *const jac[3], cha
     s->mu_art = fld(s); s->beta_art = fld(s); s->kappa_art = fld(s);
     s->strain_mag = fld(s); s->sensor = fld(s); s->sensor_sp = fld(s);
     s->tmp_a = fld(s); s->tmp_b = fld(s);
+    if (s->geom) init_geometry(s, x, jac);
     return s;
 }
 
@@ -689,7 +748,8 @@ void or_solver_destroy(or_solver *s)
     }
     free(s->mu_art); free(s->beta_art); free(s->kappa_art);
     free(s->strain_mag); free(s->sensor); free(s->sensor_sp); free(s->tmp_a); free(s->tmp_b);
-    for (int k = 0; k < 3; k++) free(s->mjac[k]);
+    free(s->inv_J); free(s->inv_r); free(s->cotr);
+    for (int k = 0; k < 3; k++) { free(s->area[k]); free(s->inv_h[k]); }
     free(s);
 }
 
@@ -905,6 +965,62 @@ static void filt_along(or_solver *s, double *out, const double *f, int d)
     or_apply_along(s->fplan[d], out, f);
 }
 
+/* metric.jl:177-219: curvature corrections added to the scaled physical
+ * velocity gradients (grad_u[d][j] at gu[d*3 + j]) */
+static void metric_correct_gradients(or_solver *s)
+{
+    if (s->d.metric == 0) return;
+    double **g = s->gu;
+    FOR_INTERIOR(s, I) {
+        const double ir = s->inv_r[I];
+        if (s->d.metric == 1) {
+            g[3][I] -= s->v[I] * ir;
+            g[4][I] += s->u[I] * ir;
+        } else {
+            const double ctr = s->cotr[I];
+            const double ur = s->u[I], ut = s->v[I], up = s->w[I];
+            g[3][I] -= ut * ir;
+            g[4][I] += ur * ir;
+            g[6][I] -= up * ir;
+            g[7][I] -= up * ctr;
+            g[8][I] += ur * ir + ut * ctr;
+        }
+    }
+}
+
+/* metric.jl:228-237 _Pi: the momentum flux tensor from the stored
+ * primitives and gradients (a, b zero-based) */
+static double Pi_ab(const or_solver *s, int64_t I, int a, int b)
+{
+    double *const *g = s->gu;
+    const double mu = s->d.mu0 + s->mu_art[I];
+    const double be = s->beta_art[I];
+    const double divu = g[0][I] + g[4][I] + g[8][I];
+    const double tau = mu * (g[a * 3 + b][I] + g[b * 3 + a][I]) + (a == b ? (be - 2 * mu / 3) * divu : 0.0);
+    const double uv[3] = {s->u[I], s->v[I], s->w[I]};
+    return s->rho[I] * uv[a] * uv[b] + (a == b ? s->p[I] : 0.0) - tau;
+}
+
+/* metric.jl:239-269 add_metric_sources! */
+static void add_metric_sources(or_solver *s)
+{
+    if (s->d.metric == 0) return;
+    double *m1 = dQc(s, s->im1), *m2 = dQc(s, s->im2), *m3 = dQc(s, s->im3);
+    FOR_INTERIOR(s, I) {
+        const double ir = s->inv_r[I];
+        if (s->d.metric == 1) {
+            m1[I] += ir * Pi_ab(s, I, 1, 1);
+            m2[I] -= ir * Pi_ab(s, I, 1, 0);
+        } else {
+            const double ctr = s->cotr[I];
+            const double Ptt = Pi_ab(s, I, 1, 1), Ppp = Pi_ab(s, I, 2, 2);
+            m1[I] += ir * (Ptt + Ppp);
+            m2[I] += -ir * Pi_ab(s, I, 1, 0) + ctr * Ppp;
+            m3[I] += -ir * Pi_ab(s, I, 2, 0) - ctr * Pi_ab(s, I, 2, 1);
+        }
+    }
+}
+
 /* rhs.jl:591-606 */
 static void primitives_and_gradients(or_solver *s, int primitives_current)
 {
@@ -919,7 +1035,7 @@ static void primitives_and_gradients(or_solver *s, int primitives_current)
                 memset(s->gu[d * 3 + jj], 0, sizeof(double) * s->NP);
             }
         }
-    /* metric_correct_gradients! is a no-op for CartesianMetric */
+    metric_correct_gradients(s);
 }
 
 /* artificial.jl:67-94 delta4_sum! */
@@ -1248,7 +1364,7 @@ void or_compute_rhs(or_solver *s, int primitives_current)
         FOR_INTERIOR(s, I) dq[I] = 0;
         for (int d = 0; d < 3; d++) {
             if (!s->active[d]) continue;
-            if (!s->stretched) {
+            if (!s->geom) {   /* rhs.jl:649-663: unit geometry, A_d = inv_J = 1 */
                 deriv_along(s, s->tmp_a, s->flux[d][c], d);
 #pragma omp parallel for schedule(static)
                 for (int64_t k = 0; k < s->n[2]; k++)
@@ -1260,26 +1376,20 @@ void or_compute_rhs(or_solver *s, int primitives_current)
                 continue;
             }
             /* rhs.jl:672-684: tmp_b = A_d F_d (A_d = J / h_d), dQ -= inv_J D_d tmp_b */
-            const double *F = s->flux[d][c];
+            const double *F = s->flux[d][c], *Ad = s->area[d];
 #pragma omp parallel for schedule(static)
-            for (int64_t I = 0; I < s->NP; I++) {
-                int64_t ip[3];
-                padded_ijk(s, I, ip);
-                const double J = hmet(s, 0, ip[0]) * hmet(s, 1, ip[1]) * hmet(s, 2, ip[2]);
-                const double Ad = J / hmet(s, d, ip[d]);
-                s->tmp_b[I] = Ad * F[I];
-            }
+            for (int64_t I = 0; I < s->NP; I++) s->tmp_b[I] = Ad[I] * F[I];
             deriv_along(s, s->tmp_a, s->tmp_b, d);
 #pragma omp parallel for schedule(static)
             for (int64_t k = 0; k < s->n[2]; k++)
                 for (int64_t j = 0; j < s->n[1]; j++)
                     for (int64_t i = 0; i < s->n[0]; i++) {
                         int64_t I = pidx(s, i, j, k);
-                        const double inv_J = 1 / jac_at(s, I);
-                        dq[I] -= inv_J * s->tmp_a[I];
+                        dq[I] -= s->inv_J[I] * s->tmp_a[I];
                     }
         }
     }
+    add_metric_sources(s);   /* rhs.jl:687 */
     for (int d = 0; d < 3; d++)
         for (int side = 0; side < 2; side++) {
             if (!s->active[d]) continue;
@@ -1331,18 +1441,19 @@ void or_max_rate(or_solver *s, double *rate_out, double *rho_min_out)
         double acc = 0.0, dsum = 0.0;
         for (int dd = 0; dd < 3; dd++) {
             if (!s->active[dd]) continue;
-            double idx;                        /* inv_h[d] / h[d] */
-            if (s->mjac[dd]) {
-                int64_t ip[3];
-                padded_ijk(s, I, ip);
-                idx = (1 / hmet(s, dd, ip[dd])) / s->h[dd];
-            } else {
-                idx = 1.0 / s->h[dd];
-            }
+            /* inv_h[d] / h[d] */
+            const double idx = s->geom ? s->inv_h[dd][I] / s->h[dd] : 1.0 / s->h[dd];
             acc += (fabs(uv[dd]) + c) * idx;
             dsum += idx * idx;
         }
-        acc += 0.0;                            /* curvature_rate, Cartesian */
+        /* timestep.jl:186-197 curvature_rate: collapsed angular dimensions */
+        if (s->d.metric == 1 && !s->active[1]) acc += fabs(uv[1]) * s->inv_r[I];
+        if (s->d.metric == 2) {
+            double a = 0.0;
+            if (!s->active[1]) a += fabs(uv[1]) * s->inv_r[I];
+            if (!s->active[2]) a += fabs(uv[2]) * (s->inv_r[I] + fabs(s->cotr[I]));
+            acc += a;
+        }
         double Dmax = d->mu0 / (d->Sc * rho);
         for (int sp = 0; sp < s->Ns; sp++) Dmax = jmax(Dmax, d->mu0 / (d->Sc * rho) + s->D_art[sp][I]);
         double nu = (d->mu0 + s->mu_art[I] + s->beta_art[I]) * ri +
@@ -1512,7 +1623,7 @@ double or_volume_integral(const or_solver *s, const double *f)
             double wj = wk * quad_weight(s, 1, j);
             for (int64_t i = 0; i < s->n[0]; i++) {
                 int64_t I = pidx(s, i, j, k);
-                const double inv_J = s->stretched ? 1 / jac_at(s, I) : 1.0;
+                const double inv_J = s->geom ? s->inv_J[I] : 1.0;
                 acc += wj * quad_weight(s, 0, i) * f[I] / inv_J;
             }
         }

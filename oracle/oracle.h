@@ -107,6 +107,7 @@ typedef struct {
     int eos_kind;
     double T_guess;
     or_nasa9_species nasa9[OR_MAXSP];
+    int metric;                       /* 0 CartesianMetric, 1 CylindricalMetric, 2 SphericalMetric */
 } or_desc;
 
 /* Step-boundary hooks of run! (src/callbacks.jl): next scheduled instant,
@@ -124,6 +125,10 @@ or_solver *or_solver_create(const or_desc *d, char *msg, int msglen);
 /* With Stretch mappings: jac[d] (NULL: uniform) holds dx/dxi at the
  * n_global[d] nodes of a stretched, non-periodic dimension. */
 or_solver *or_solver_create_ex(const or_desc *d, const double *const jac[3], char *msg, int msglen);
+/* x[d]: physical coordinate at the interior nodes of a stretched dimension
+ * whose coordinate the metric reads (r; spherical theta); NULL elsewhere. */
+or_solver *or_solver_create_geom(const or_desc *d, const double *const x[3], const double *const jac[3], char *msg,
+                                 int msglen);
 void or_solver_destroy(or_solver *s);
 void or_solver_dims(const or_solver *s, int64_t padded[3], int halo[3], int *n_cons);
 /* Q in the padded layout (padded[0]*padded[1]*padded[2]*n_cons doubles). */

@@ -60,7 +60,7 @@ class Desc(C.Structure):
                 ("dt_min", C.c_double), ("dt_min_ratio", C.c_double),
                 ("retries", C.c_int), ("cfl_backoff", C.c_double), ("savepoint_interval", C.c_int),
                 ("landing_steps", C.c_int), ("eos_kind", C.c_int), ("T_guess", C.c_double),
-                ("nasa9", Nasa9Sp * MAXSP)]
+                ("nasa9", Nasa9Sp * MAXSP), ("metric", C.c_int)]
 
 
 def _cpu_has_avx2() -> bool:
@@ -95,6 +95,7 @@ def _load():
         "or_wrap_halos": (None, [D, I64, I, I]),
         "or_solver_create": (P, [C.POINTER(Desc), C.c_char_p, C.c_int]),
         "or_solver_create_ex": (P, [C.POINTER(Desc), C.POINTER(D), C.c_char_p, C.c_int]),
+        "or_solver_create_geom": (P, [C.POINTER(Desc), C.POINTER(D), C.POINTER(D), C.c_char_p, C.c_int]),
         "or_solver_destroy": (None, [P]),
         "or_solver_dims": (None, [P, I64, I, I]),
         "or_solver_Q": (D, [P]),
@@ -207,16 +208,21 @@ class Plan:
 class Solver:
     """The reference Solver + Workspace restated on padded arrays."""
 
-    def __init__(self, desc: Desc, jac=None):
-        """jac: per dimension None or dx/dxi at the nodes of a stretched dimension."""
+    def __init__(self, desc: Desc, jac=None, x=None):
+        """jac: per dimension None or dx/dxi at the nodes of a stretched dimension;
+        x: per dimension None or the physical node coordinates of a stretched
+        dimension that the metric reads (r; spherical theta)."""
         msg = C.create_string_buffer(256)
-        if jac is None or all(j is None for j in jac):
-            self._s = lib().or_solver_create(C.byref(desc), msg, 256)
-        else:
-            self._jac = [None if j is None else np.ascontiguousarray(j, dtype=np.float64) for j in jac]
-            DP = C.POINTER(C.c_double)
-            ptrs = (DP * 3)(*[C.cast(None, DP) if j is None else j.ctypes.data_as(DP) for j in self._jac])
-            self._s = lib().or_solver_create_ex(C.byref(desc), ptrs, msg, 256)
+        DP = C.POINTER(C.c_double)
+
+        def ptrs(arrs):
+            if arrs is None or all(a is None for a in arrs):
+                return None, None
+            keep = [None if a is None else np.ascontiguousarray(a, dtype=np.float64) for a in arrs]
+            return keep, (DP * 3)(*[C.cast(None, DP) if a is None else a.ctypes.data_as(DP) for a in keep])
+        self._jac, pj = ptrs(jac)
+        self._x, px = ptrs(x)
+        self._s = lib().or_solver_create_geom(C.byref(desc), px, pj, msg, 256)
         if not self._s:
             raise ValueError(msg.value.decode())
         self.desc = desc
@@ -255,6 +261,12 @@ class Solver:
         if not p:
             raise KeyError(name)
         return self._view(p)
+
+    def volume_integral(self, f: np.ndarray) -> float:
+        """diagnostics.jl:60-80 volume_integral of a padded field."""
+        f = np.asfortranarray(f, dtype=np.float64)
+        assert f.shape == tuple(self.padded)
+        return lib().or_volume_integral(self._s, f.ctypes.data_as(C.POINTER(C.c_double)))
 
     def interior(self, a: np.ndarray) -> np.ndarray:
         sl = tuple(slice(self.halo[d], self.halo[d] + self.n[d]) for d in range(3))
@@ -342,7 +354,7 @@ def make_desc(n_global, L_domain, origin=(0.0, 0.0, 0.0), species=((1.0, 1.4),),
               C_D=0.01, beta_sensor=0, deriv="lele_d1_6", alphaf=0.45, cfl=0.5,
               filter_interval=1, bcs=None, bc_state=None, nscbc=None,
               predict=0.0, max_growth=0.0, dt_min=0.0, dt_min_ratio=1e-8, retries=0, cfl_backoff=0.5,
-              savepoint_interval=25, landing_steps=2, nasa9=None) -> Desc:
+              savepoint_interval=25, landing_steps=2, nasa9=None, metric=0) -> Desc:
     """nasa9: (T_guess, [(R, [(Tmin, Tmax, (a1..a7), b1), ...]), ...]) selects
     the Nasa9Mixture (species then ignores gamma)."""
     d = Desc()
@@ -383,6 +395,7 @@ def make_desc(n_global, L_domain, origin=(0.0, 0.0, 0.0), species=((1.0, 1.4),),
     d.predict, d.max_growth, d.dt_min, d.dt_min_ratio = predict, max_growth, dt_min, dt_min_ratio
     d.retries, d.cfl_backoff, d.savepoint_interval, d.landing_steps = retries, cfl_backoff, savepoint_interval, \
         landing_steps
+    d.metric = {"cartesian": 0, "cylindrical": 1, "spherical": 2}.get(metric, metric)
     if nasa9 is not None:
         d.eos_kind = 1
         d.T_guess, table = nasa9

@@ -36,9 +36,9 @@ def conserved(prim, species=((1.0, 1.4),), Y=None):
     return [rho * y for y in Y] + [rho * u, rho * v, rho * w, rho * (cvm * T + ke)]
 
 
-def oracle_solver(n, L, q, jac=None, **kw):
+def oracle_solver(n, L, q, jac=None, x=None, **kw):
     d = O.make_desc(n, L, **kw)
-    s = O.Solver(d, jac)
+    s = O.Solver(d, jac, x)
     Q = s.Q
     sl = tuple(slice(s.halo[k], s.halo[k] + n[k]) for k in range(3))
     for c, qc in enumerate(q):
@@ -76,7 +76,7 @@ def nscbc_codes(bcs, nscbc):
 
 def gpu_solver(ctx, n, L, q, species=((1.0, 1.4),), mu0=0.0, art=True, cfl=0.5, bcs=None, deriv="lele_d1_6",
                precision="fp64", beta_sensor="strain", bc_funs=None, filter_interval=1, origin=(0.0, 0.0, 0.0),
-               nscbc=None, chunks=None, control=None, stretch=None, eos=None):
+               nscbc=None, chunks=None, control=None, stretch=None, eos=None, metric="cartesian"):
     """chunks: line-tile lengths (x, y, z) for this solver's plans, set in the
     table for its creation only (None: the table / rule as it stands).
     eos: a frontend EOS object (None: the IdealMixture of `species`)."""
@@ -97,7 +97,7 @@ def gpu_solver(ctx, n, L, q, species=((1.0, 1.4),), mu0=0.0, art=True, cfl=0.5, 
                 pair.append(code[bcs[k][s]]())
         bco.append(tuple(pair))
     prob = ch.Problem(domain=tuple((origin[k], origin[k] + L[k]) for k in range(3)), bcs=tuple(bco),
-                      ic=lambda X, Y, Z: None, eos=eos, transport=ch.Transport(mu0=mu0))
+                      ic=lambda X, Y, Z: None, eos=eos, transport=ch.Transport(mu0=mu0), metric=metric)
     sch = {"lele_d1_6": ch.lele_d1_6, "lele_d1_10": ch.lele_d1_10, "pade_d1_4": ch.pade_d1_4}[deriv]()
     num = ch.Numerics(n_global=n, deriv=sch, art=ch.ArtParams(enabled=art, beta_sensor=beta_sensor), cfl=cfl,
                       precision=precision, filter_interval=filter_interval, control=control or ch.StepControl(),

@@ -34,7 +34,7 @@ def test_library_exports_every_declared_symbol():
 
 def test_abi_version():
     import compactles_hip as ch
-    assert ch.load().cles_abi_version() == 3
+    assert ch.load().cles_abi_version() == 4
 
 
 def test_struct_layout_matches_header():
@@ -55,6 +55,7 @@ int main(void) {
   printf("%zu %zu\n", offsetof(cles_run_hooks, after_step), offsetof(cles_run_hooks, rewind));
   printf("%zu %zu %zu %zu\n", offsetof(cles_solver_desc, eos_kind), offsetof(cles_solver_desc, T_guess),
          offsetof(cles_solver_desc, nasa9), sizeof(cles_nasa9_species));
+  printf("%zu\n", offsetof(cles_solver_desc, metric));
   return 0;
 }
 '''
@@ -70,7 +71,7 @@ int main(void) {
             L.SolverDesc.nscbc.offset, L.SolverDesc.retries.offset, L.SolverDesc.cfl_backoff.offset,
             L.SolverDesc.landing_steps.offset, C.sizeof(RunHooks), RunHooks.after_step.offset, RunHooks.rewind.offset,
             L.SolverDesc.eos_kind.offset, L.SolverDesc.T_guess.offset, L.SolverDesc.nasa9.offset,
-            C.sizeof(L.Nasa9SpeciesDesc)]
+            C.sizeof(L.Nasa9SpeciesDesc), L.SolverDesc.metric.offset]
     assert got == want
 
 
