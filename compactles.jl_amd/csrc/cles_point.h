@@ -133,6 +133,7 @@ struct NscbcArgs {
     double ut[3], Tt, Yt[CLES_MAXSP], eta_u, eta_T, eta_t, eta_Y;
     double Rk[CLES_MAXSP], cvk[CLES_MAXSP];
     const N9Table *n9;              // Nasa9Mixture: cv_k(T) in the inflow's eos_dphi_dY (NULL: cvk)
+    Metric1D met;                   // inv_h of the derivatives (stretched or curvilinear grids)
     W *dQ;
 };
 

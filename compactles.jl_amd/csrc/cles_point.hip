@@ -799,11 +799,18 @@ __global__ void __launch_bounds__(TPB) k_nscbc(NscbcArgs<W> a)
     if (!(Ma < 1)) return;   // supersonic point: every wave leaves
     const int t1 = d == 0 ? 1 : 0, t2 = d == 2 ? 1 : 2;
     const int im = a.Ns;
+    // inv_h of the face's dimensions (nscbc.jl:108,125,130,257); 1 on unit geometry
+    double ih[3] = {1.0, 1.0, 1.0};
+    if (a.met.kind != CLES_METRIC_CARTESIAN || a.met.m[0] || a.met.m[1] || a.met.m[2]) {
+        double hs[3];
+        metric_hs(a.met, idx, hs);
+        for (int k = 0; k < 3; k++) ih[k] = 1 / hs[k];
+    }
     // physics.jl:142-145 eos_phi for an ideal mixture
     const double Rm = p / (rho * Tp);
     const double phi = cpm / Rm - 1;
     const double ke = 0.5 * (uv[0] * uv[0] + uv[1] * uv[1] + uv[2] * uv[2]);
-    const double dpn = 1.0 * (double)a.dpn[I];
+    const double dpn = ih[d] * (double)a.dpn[I];
     const double dun = a.gu[d * 3 + d][I];
     if (!a.inflow) {
         const double sgn = a.side == 1 ? -1.0 : 1.0;
@@ -811,12 +818,12 @@ __global__ void __launch_bounds__(TPB) k_nscbc(NscbcArgs<W> a)
         double tin = 0.0;
         if (a.act1) {
             const double ut = uv[t1];
-            tin += ut * 1.0 * (double)a.dt1[I] + rho * c * c * (double)a.gu[t1 * 3 + t1][I] +
+            tin += ut * ih[t1] * (double)a.dt1[I] + rho * c * c * (double)a.gu[t1 * 3 + t1][I] +
                    sgn * rho * c * ut * (double)a.gu[t1 * 3 + d][I];
         }
         if (a.act2) {
             const double ut = uv[t2];
-            tin += ut * 1.0 * (double)a.dt2[I] + rho * c * c * (double)a.gu[t2 * 3 + t2][I] +
+            tin += ut * ih[t2] * (double)a.dt2[I] + rho * c * c * (double)a.gu[t2 * 3 + t2][I] +
                    sgn * rho * c * ut * (double)a.gu[t2 * 3 + d][I];
         }
         const double bt = a.beta_t < 0 ? Ma : a.beta_t;
@@ -838,7 +845,7 @@ __global__ void __launch_bounds__(TPB) k_nscbc(NscbcArgs<W> a)
         *qe = (W)((double)*qe - (ke * dd1 + phi * dd2 + rho * un * dd3));
         return;
     }
-    const double drn = 1.0 * (double)a.dt1[I];
+    const double drn = ih[d] * (double)a.dt1[I];
     const double K = c / a.Lref;
     const double L1c = (un - c) * (dpn - rho * c * dun);
     const double L5c = (un + c) * (dpn + rho * c * dun);

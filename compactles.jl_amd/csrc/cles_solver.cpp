@@ -669,6 +669,7 @@ struct Impl {
                 a.eta_t = nd.eta_t;
                 a.eta_Y = nd.eta_Y;
                 a.n9 = s.n9;
+                a.met = s.met;
                 a.dQ = f(s.dQ);
                 const double w = sizeof(W);
                 s.timed(PH_NSCBC, (w * (14 + 2 * s.Ns) + 2 * w * s.C) * (double)a.count,
@@ -995,13 +996,8 @@ int cles_solver_create_geom(cles_ctx *ctx, const cles_solver_desc *desc, const d
     CLES_TRY({
         CLES_REQUIRE(ctx && desc && out, "NULL argument");
         validate_desc(*desc);
-        bool any_stretch = false, any_nscbc = false;
         for (int k = 0; k < 3; k++) {
-            for (int sd = 0; sd < 2; sd++)
-                any_nscbc = any_nscbc || ((desc->bc[k][sd] == CLES_BC_NSCBC_IN || desc->bc[k][sd] == CLES_BC_NSCBC_OUT) &&
-                                          desc->n_global[k] > 1);
             if (!dxdxi || !dxdxi[k]) continue;
-            any_stretch = true;
             // rhs.jl:186-187
             CLES_REQUIRE(desc->n_global[k] > 1 && desc->bc[k][0] != CLES_BC_PERIODIC,
                          "dimension " + std::to_string(k + 1) + ": stretched dimensions must be non-periodic");
@@ -1009,12 +1005,23 @@ int cles_solver_create_geom(cles_ctx *ctx, const cles_solver_desc *desc, const d
                 CLES_REQUIRE(dxdxi[k][i] > 0 && std::isfinite(dxdxi[k][i]),
                              "stretch mapping for dim " + std::to_string(k + 1) + " is not monotone increasing");
         }
-        if (any_stretch && any_nscbc)
-            throw cles::Error{CLES_ERR_UNSUPPORTED, "NSCBC faces on stretched grids are not available on the GPU backend"};
         const int kind = desc->metric;
-        if (kind != CLES_METRIC_CARTESIAN && any_nscbc)
-            throw cles::Error{CLES_ERR_UNSUPPORTED,
-                              "NSCBC faces under a curvilinear metric are not available on the GPU backend"};
+        // nscbc.jl:309-316 _validate_nscbc_face: the LODI analysis needs a face
+        // whose normal scale factor is one (metric.jl:93-95 unit_scalefactor)
+        for (int k = 0; k < 3; k++) {
+            const bool unit = kind == CLES_METRIC_CARTESIAN || (kind == CLES_METRIC_CYLINDRICAL && k != 1) ||
+                              (kind == CLES_METRIC_SPHERICAL && k == 0);
+            for (int sd = 0; sd < 2; sd++) {
+                const int bc = desc->bc[k][sd];
+                if ((bc == CLES_BC_NSCBC_IN || bc == CLES_BC_NSCBC_OUT) && desc->n_global[k] > 1 && !unit)
+                    throw cles::Error{CLES_ERR_INVALID,
+                                      std::string(bc == CLES_BC_NSCBC_IN ? "NSCBCInflowBC" : "NSCBCOutflowBC") +
+                                          " on dimension " + std::to_string(k + 1) +
+                                          ": the LODI formulation requires a face whose normal metric scale "
+                                          "factor is one (a Cartesian face, or a cylindrical r/z or spherical r "
+                                          "face)"};
+            }
+        }
         // the metric reads r (dimension 1) and, spherical, theta (dimension 2)
         auto reads = [&](int k) {
             return (kind != CLES_METRIC_CARTESIAN && k == 0) || (kind == CLES_METRIC_SPHERICAL && k == 1);

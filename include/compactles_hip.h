@@ -233,9 +233,9 @@ int cles_solver_create(cles_ctx *ctx, const cles_solver_desc *desc, cles_solver 
  * non-periodic dimension.  That dimension's spacing is then 1 / (n - 1)
  * (rhs.jl:191-201); gradients are scaled by inv_h (rhs.jl:477-483), the flux
  * divergence takes the general path dQ -= inv_J D_d(A_d F_d) (rhs.jl:665-684),
- * max_rate uses inv_h[d] / h[d] (timestep.jl:151) and the volume integrals
- * divide by inv_J (diagnostics.jl:73-75).  NSCBC faces are refused on
- * stretched grids (CLES_ERR_UNSUPPORTED). */
+ * max_rate uses inv_h[d] / h[d] (timestep.jl:151), the volume integrals
+ * divide by inv_J (diagnostics.jl:73-75) and the NSCBC derivatives take
+ * inv_h (nscbc.jl:108-130, :257). */
 int cles_solver_create_ex(cles_ctx *ctx, const cles_solver_desc *desc, const double *const dxdxi[3],
                           cles_solver **solver);
 /* setup with a metric and Stretch mappings (src/metric.jl:80-269, rhs.jl:
@@ -249,8 +249,9 @@ int cles_solver_create_ex(cles_ctx *ctx, const cles_solver_desc *desc, const dou
  * the discrete-GCL cot(theta)/r of a resolved theta (metric.jl:150-168) and
  * the curvature rate of collapsed angular dimensions (timestep.jl:186-197).
  * Coordinate singularities are not treated: every node needs r > 0 and, for
- * the spherical metric, 0 < theta < pi (CLES_ERR_INVALID otherwise); NSCBC
- * faces are refused under a curvilinear metric (CLES_ERR_UNSUPPORTED). */
+ * the spherical metric, 0 < theta < pi (CLES_ERR_INVALID otherwise).  NSCBC
+ * faces must have a normal scale factor of one: cylindrical r and z faces,
+ * spherical r faces (nscbc.jl:309-316; CLES_ERR_INVALID otherwise). */
 int cles_solver_create_geom(cles_ctx *ctx, const cles_solver_desc *desc, const double *const x[3],
                             const double *const dxdxi[3], cles_solver **solver);
 int cles_solver_destroy(cles_solver *solver);

@@ -701,6 +701,18 @@ or_solver *or_solver_create_geom(const or_desc *d, const double *const x[3], con
         }
     }
     if (d->metric != 0) s->geom = 1;
+    /* nscbc.jl:309-316: NSCBC faces need a normal scale factor of one */
+    for (int k = 0; k < 3; k++) {
+        const int unit = d->metric == 0 || (d->metric == 1 && k != 1) || (d->metric == 2 && k == 0);
+        for (int sd = 0; sd < 2; sd++)
+            if ((d->bc[k][sd] == OR_BC_NSCBC_IN || d->bc[k][sd] == OR_BC_NSCBC_OUT) && s->active[k] && !unit) {
+                snprintf(msg, (size_t)msglen, "%s on dimension %d: the LODI formulation requires a face whose normal "
+                         "metric scale factor is one", d->bc[k][sd] == OR_BC_NSCBC_IN ? "NSCBCInflowBC" :
+                         "NSCBCOutflowBC", k + 1);
+                free(s);
+                return NULL;
+            }
+    }
     /* physics.jl:111-115 */
     for (int k = 0; k < s->Ns; k++) {
         s->Rk[k] = d->R[k];
@@ -1260,17 +1272,17 @@ static void nscbc_outflow(or_solver *s, int d, int side)
         double un = vel[d][I];
         double Ma = fabs(un) / c;
         if (!(Ma < 1)) continue;
-        double dpn = 1.0 * s->tmp_a[I];
+        double dpn = (s->geom ? s->inv_h[d][I] : 1.0) * s->tmp_a[I];
         double dun = Gdd[I];
         double Lcomp = side == 1 ? (un - c) * (dpn - rho * c * dun) : (un + c) * (dpn + rho * c * dun);
         double tin = 0.0;
         if (act1) {
             double ut = vel[t1][I];
-            tin += ut * 1.0 * s->tmp_b[I] + rho * c * c * s->gu[t1 * 3 + t1][I] + sgn * rho * c * ut * s->gu[t1 * 3 + d][I];
+            tin += ut * (s->geom ? s->inv_h[t1][I] : 1.0) * s->tmp_b[I] + rho * c * c * s->gu[t1 * 3 + t1][I] + sgn * rho * c * ut * s->gu[t1 * 3 + d][I];
         }
         if (act2) {
             double ut = vel[t2][I];
-            tin += ut * 1.0 * s->sensor_sp[I] + rho * c * c * s->gu[t2 * 3 + t2][I] + sgn * rho * c * ut * s->gu[t2 * 3 + d][I];
+            tin += ut * (s->geom ? s->inv_h[t2][I] : 1.0) * s->sensor_sp[I] + rho * c * c * s->gu[t2 * 3 + t2][I] + sgn * rho * c * ut * s->gu[t2 * 3 + d][I];
         }
         double bt = bc->beta_t < 0 ? Ma : bc->beta_t;
         double K = bc->sigma * (1 - Ma * Ma) * c / Lref;
@@ -1305,7 +1317,7 @@ static void nscbc_inflow(or_solver *s, int d, int side)
         double un = vel[d][I];
         double Ma = fabs(un) / c;
         if (!(Ma < 1)) continue;
-        double ih = 1.0;
+        double ih = s->geom ? s->inv_h[d][I] : 1.0;
         double dpn = ih * s->tmp_a[I];
         double drn = ih * s->tmp_b[I];
         double dun = Gdd[I];

@@ -5,7 +5,8 @@ grad_u 1e-12; sensor-fed coefficients 1e-9), a step and a filter (state
 1e-10), for cylindrical annuli (theta resolved and collapsed with swirl),
 spherical shells (phi resolved and collapsed) and a stretched radius;
 freestream on the device; the FP32-mixed form against the FP64 oracle;
-refusals (NSCBC faces, batches, r <= 0, theta outside (0, pi))."""
+NSCBC on radial faces; refusals (NSCBC on an angular face, batches, r <= 0,
+theta outside (0, pi))."""
 import numpy as np
 import pytest
 
@@ -178,9 +179,10 @@ def test_refusals(ctx):
     with pytest.raises(ch.ClesError, match="theta"):
         cases.gpu_solver(ctx, n, (1.0, 3.2, TWO_PI), q, bcs=(SLIP, SLIP, PER), origin=(0.5, 0.1, 0.0),
                          metric="spherical")
-    faces = {(0, 1): {"pinf": 1.0, "sigma": 0.25, "Lref": 1.0, "beta_t": 0.0}}
-    with pytest.raises(ch.ClesError, match="curvilinear"):
-        cases.gpu_solver(ctx, n, L, q, bcs=bcs, origin=origin, metric="cylindrical", nscbc=faces)
+    faces = {(1, 1): {"pinf": 1.0, "sigma": 0.25, "Lref": 1.0, "beta_t": 0.0}}
+    with pytest.raises(ch.ClesError, match="normal metric scale factor is one"):
+        cases.gpu_solver(ctx, n, (1.0, 1.0, 0.5), q, bcs=(SLIP, SLIP, PER), origin=origin, metric="cylindrical",
+                         nscbc=faces)
     dummy = ch.DirichletBC(lambda x, y, z, t: None)
     prob = ch.Problem(domain=((0.5, 1.5),) * 3, bcs=((dummy, dummy), (ch.PeriodicBC(), ch.PeriodicBC()),
                                                      (ch.PeriodicBC(), ch.PeriodicBC())),
@@ -188,3 +190,39 @@ def test_refusals(ctx):
     inst = [ch.Instance(n=64, domain=(0.5, 1.5), ic=None, lo=ch.Prim(rho=1.0, p=1.0), hi=ch.Prim(rho=1.0, p=1.0))]
     with pytest.raises(ch.ClesError, match="Cartesian"):
         ch.Batch(ctx, prob, ch.Numerics(n_global=(1, 1, 1)), inst)
+
+
+@pytest.mark.parametrize("metric", ["cylindrical", "spherical"])
+def test_nscbc_radial_faces_match_oracle(ctx, metric):
+    """NSCBC on r faces (normal scale factor one): a radial outflow through an
+    annulus / shell with inflow at the inner radius (nscbc.jl:59-306 with the
+    curvature-corrected grad_u and inv_h).  Artificial terms off: with them on,
+    the oracle alone turns a 1e-13 change of the state into 1e-9 (spherical)
+    of the dt history within 15 steps (the sensors' |.| and switches at the
+    faces); the sensors under these metrics are compared above."""
+    if metric == "cylindrical":
+        n, L, origin, bcs = (28, 16, 12), (1.0, TWO_PI, 0.5), (0.5, 0.0, 0.0), (SLIP, PER, PER)
+    else:
+        n, L, origin, bcs = (28, 12, 16), (1.0, np.pi / 2, TWO_PI), (0.5, np.pi / 4, 0.0), (SLIP, SLIP, PER)
+    X, Y, Z = cases.grid(n, L, periodic=tuple(b == PER for b in bcs), origin=origin)
+    k3 = 2 * np.pi / L[2]
+    st = np.sin(np.pi * (Y - origin[1]) / L[1]) if metric == "spherical" else np.cos(Y)
+    ur = 0.25 * origin[0] / X + 0.03 * np.cos(k3 * Z) * st
+    ut = 0.04 * np.sin(np.pi * (X - origin[0])) * st
+    uz = 0.05 * np.cos(k3 * Z)
+    p = 1.0 + 0.03 * np.sin(np.pi * (X - origin[0])) * np.cos(k3 * Z)
+    rho = 1.0 + 0.02 * np.cos(k3 * Z)
+    q = cases.conserved((ur, ut, uz, p, rho))
+    faces = {(0, 0): {"u": (0.26, 0.0, 0.0), "T": 1.0, "Y": (1.0,), "eta_u": 0.28, "eta_T": 0.35, "eta_t": 0.22,
+                      "eta_Y": 0.3},
+             (0, 1): {"pinf": 0.98, "sigma": 0.3, "beta_t": -1.0}}
+    os_ = cases.oracle_solver(n, L, q, bcs=cases.nscbc_codes(bcs, faces), nscbc=faces, origin=origin, metric=metric,
+                              mu0=1e-3, art=False)
+    gs = cases.gpu_solver(ctx, n, L, q, bcs=bcs, nscbc=faces, origin=origin, metric=metric, mu0=1e-3, art=False)
+    os_.compute_rhs(False)
+    gs.compute_rhs(False)
+    _compare_rhs(gs, os_, n)
+    d1 = os_.run(1e9, 15, record=15)
+    d2 = gs.run(1e9, 15, record=15)
+    assert np.abs(d1 - d2).max() <= 1e-12 * d1.max()
+    assert _q_err(gs, os_, n) <= 1e-10

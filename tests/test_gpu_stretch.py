@@ -2,7 +2,8 @@
 oracle on the same inputs: one RHS (dQ 1e-12; sensor-fed coefficients 1e-9),
 one step and filter (state 1e-12), 20 run! steps (dt history 1e-12, kinetic
 energy 1e-12, enstrophy 1e-11), with the sine clustering along x and along y;
-exact freestream; the FP32-mixed form against the FP64 oracle; NSCBC refused."""
+exact freestream; the FP32-mixed form against the FP64 oracle; NSCBC faces
+on a stretched dimension (1e-12 / 1e-10)."""
 import numpy as np
 import pytest
 
@@ -119,12 +120,35 @@ def test_fp32_mixed_stretched(ctx):
     assert err <= 1e-5
 
 
-def test_nscbc_on_stretched_grid_refused(ctx):
+def _through(X, Y, Z):
+    k = 2 * np.pi
+    return (0.3 + 0.05 * np.sin(np.pi * X) * np.cos(k * Y), 0.04 * np.cos(k * Z) * np.sin(np.pi * X),
+            0.03 * np.sin(k * Y + k * Z), 1.0 + 0.05 * np.cos(np.pi * X) * np.sin(k * Y),
+            1.0 + 0.05 * np.sin(k * Y + k * Z))
+
+
+@pytest.mark.parametrize("kind", ["outflow", "inflow_outflow"])
+def test_nscbc_on_stretched_grid_matches_oracle(ctx, kind):
+    """nscbc.jl:108-130, :257: the face derivatives take inv_h of the stretch."""
     import compactles_hip as ch
-    n = (24, 12, 12)
-    q, jac, _ = _case(n, 0, _flow)
-    faces = {(0, 1): {"pinf": 1.0, "sigma": 0.25, "Lref": 1.0, "beta_t": 0.0}}
-    with pytest.raises(ch.ClesError) as e:
-        cases.gpu_solver(ctx, n, (1.0, 1.0, 1.0), q, bcs=((1, 1), (0, 0), (0, 0)), nscbc=faces,
-                         stretch=(ch.sine_cluster(0.0, 1.0, XC, A), None, None))
-    assert "stretched" in str(e.value)
+    n = (28, 16, 12)
+    q, jac, bcs = _case(n, 0, _through)
+    out = {"pinf": 0.97, "sigma": 0.3, "beta_t": -1.0}
+    faces = {(0, 1): out}
+    if kind == "inflow_outflow":
+        faces[(0, 0)] = {"u": (0.32, 0.01, 0.0), "T": 1.0, "Y": (1.0,), "eta_u": 0.28, "eta_T": 0.35,
+                         "eta_t": 0.22, "eta_Y": 0.3}
+    L = (1.0, 1.0, 1.0)
+    os_ = cases.oracle_solver(n, L, q, jac=jac, bcs=cases.nscbc_codes(bcs, faces), nscbc=faces, mu0=1e-3)
+    gs = cases.gpu_solver(ctx, n, L, q, bcs=bcs, nscbc=faces, mu0=1e-3,
+                          stretch=(ch.sine_cluster(0.0, 1.0, XC, A), None, None))
+    os_.compute_rhs(False)
+    gs.compute_rhs(False)
+    _compare_rhs(gs, os_, n)
+    d1 = os_.run(1e9, 15, record=15)
+    d2 = gs.run(1e9, 15, record=15)
+    assert np.abs(d1 - d2).max() <= 1e-12 * d1.max()
+    Qg = gs.allocate_state()
+    gs.download(Qg)
+    for c in range(5):
+        assert Hh.relerr(Hh.interior(Qg[..., c], n), os_.interior(os_.Q[..., c])) <= 1e-10, c

@@ -202,3 +202,21 @@ def test_viscous_couette_is_steady_in_momentum():
     # second order: the viscous term differentiates twice through third-order closures
     assert errs[1] < errs[0] / 3, errs
     assert errs[1] < 0.05 * (2 * mu * B / 0.5 ** 3), errs   # a dropped source leaves 2 mu B / r^3
+
+
+def test_nscbc_faces_need_unit_scale_factor():
+    """runtests.jl:514-532: r is a length under CylindricalMetric, theta is
+    not; an NSCBC face on theta is a setup error (inflow and outflow)."""
+    n, L, origin = (12, 12, 12), (1.0, 1.0, 1.0), (1.0, 0.0, 0.0)
+    out = {"pinf": 1.0, "sigma": 0.25, "beta_t": -1.0}
+    inflow = {"u": (0.0, 0.1, 0.0), "T": 1.0, "Y": (1.0,)}
+    _solver(n, L, origin, "cylindrical", _uniform, (SLIP, SLIP, PER))
+    for faces in ({(1, 0): out}, {(1, 1): inflow}):
+        with pytest.raises(ValueError, match="normal metric scale factor is one"):
+            _solver(n, L, origin, "cylindrical", _uniform, cases.nscbc_codes((SLIP, SLIP, PER), faces), nscbc=faces)
+    for faces in ({(0, 1): out}, {(2, 0): out}):   # r and z faces are fine (z made closed here)
+        bcs = (SLIP, SLIP, SLIP) if (2, 0) in faces else (SLIP, SLIP, PER)
+        _solver(n, L, origin, "cylindrical", _uniform, cases.nscbc_codes(bcs, faces), nscbc=faces)
+    with pytest.raises(ValueError, match="normal metric scale factor is one"):
+        _solver(n, (1.0, 1.0, 1.0), (1.0, 0.5, 0.0), "spherical", _uniform,
+                cases.nscbc_codes((SLIP, SLIP, SLIP), {(2, 1): out}), nscbc={(2, 1): out})
