@@ -189,6 +189,12 @@ class DirichletBC(BoundaryCondition):
         self.fun = fun
 
 
+class AxisBC(BoundaryCondition):
+    """boundary.jl:55-66: regularized cylindrical axis at the low end of r
+    (half-offset grid, parity mirror conditions; no state enforcement)."""
+    code = L.BC_AXIS
+
+
 @dataclass
 class NSCBCOutflowBC(BoundaryCondition):
     """nscbc.jl:45-54."""
@@ -345,11 +351,15 @@ class SolverSpec:
                                  f"x(0) = {st.x(0.0)}, x(1) = {st.x(1.0)}, domain = {prob.domain[d]}")
             if self.n[d] == 1 or self.periodic[d]:
                 raise ValueError(f"dimension {d + 1}: stretched dimensions must be non-periodic")
-        # rhs.jl:191-201: stretched dimensions are uniform in xi on [0, 1]
+        # rhs.jl:191-205: stretched dimensions are uniform in xi on [0, 1]; the
+        # axis folds r onto a half-offset grid h = L / (n - 1/2), r_1 = h / 2
+        self.axis = isinstance(prob.bcs[0][0], AxisBC)
         self.h = tuple(1.0 if self.n[d] == 1 else
                        (1.0 / (self.n[d] - 1) if stretch[d] is not None else
-                        (self.L[d] / self.n[d] if self.periodic[d] else self.L[d] / (self.n[d] - 1)))
+                        (self.L[d] / (self.n[d] - 0.5) if (d == 0 and self.axis) else
+                         (self.L[d] / self.n[d] if self.periodic[d] else self.L[d] / (self.n[d] - 1))))
                        for d in range(3))
+        self.coord_shift = tuple(self.h[d] / 2 if (d == 0 and self.axis) else 0.0 for d in range(3))
         # the mapping Jacobian at the nodes (metric.jl:100-107), handed to the device
         self.jac = tuple(None if stretch[d] is None else
                          np.ascontiguousarray(np.broadcast_to(
@@ -401,7 +411,7 @@ class SolverSpec:
         mapped through the dimension's Stretch when it has one."""
         if self.stretch[d] is not None:
             return np.asarray(self.stretch[d].x(np.arange(self.n[d]) * self.h[d]), dtype=np.float64)
-        return self.origin[d] + np.arange(self.n[d]) * self.h[d]
+        return self.origin[d] + self.coord_shift[d] + np.arange(self.n[d]) * self.h[d]
 
     def _dirichlet_state(self, bc: DirichletBC, d: int, side: int):
         xs = [self.xcoord(k) for k in range(3)]

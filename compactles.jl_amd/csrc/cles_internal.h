@@ -90,13 +90,19 @@ struct LineTables {
 };
 // chunk_major: also store the chunk-major row tables of the ragged kernel
 // (max_chunk must then be RAG_MAX_CHUNK).
+// lo_fold (0: none, +1 / -1: the solution parity sigma_g) replaces the low
+// edge's closure by a parity fold on a half-offset grid (operators.jl:59-69,
+// operators_banded.jl:55-64): the LHS coupling to ghost unknown j < 0 moves to
+// interior column -j-1 with sign sigma_g, and the low-edge RHS rows read the
+// mirror f(-j-1) sigma_f of each ghost (sigma_f = sigma_g for filters, -sigma_g
+// for derivatives), carried as synthesised closure rows.
 void build_line_tables(LineTables &out, const cles_scheme_desc &s, int n, int periodic, double h, int max_chunk,
-                       int dim, bool chunk_major = false);
+                       int dim, bool chunk_major = false, int lo_fold = 0);
 void bind_line_tables(LineDev &dev, const LineTables &t, unsigned char *base);
 
 // Builds the device tables for one (scheme, n, periodic, h).  Throws Error.
 void build_line_plan(HostPlan &hp, const cles_scheme_desc &s, int n, int periodic,
-                     double h, int max_chunk);
+                     double h, int max_chunk, int lo_fold = 0);
 void free_line_plan(HostPlan &hp);
 
 // ---------------------------------------------------------------------------

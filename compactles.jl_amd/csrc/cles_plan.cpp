@@ -163,7 +163,7 @@ void chunk_solve(const ChunkFactor &F, std::vector<double> &x)
 }  // namespace
 
 void build_line_tables(LineTables &out, const cles_scheme_desc &s, int n, int periodic, double h, int max_chunk,
-                       int dim, bool chunk_major)
+                       int dim, bool chunk_major, int lo_fold)
 {
     const int q = s.q, M = s.M, nc = s.nclosure;
     if (q < 1 || q > 2) throw Error{CLES_ERR_UNSUPPORTED, "scheme LHS half-bandwidth q must be 1 or 2"};
@@ -189,9 +189,19 @@ void build_line_tables(LineTables &out, const cles_scheme_desc &s, int n, int pe
         A.at(i, 0) = 1.0;
         for (int sd = 1; sd <= q; sd++) { A.at(i, sd) = s.lhs[sd - 1]; A.at(i, -sd) = s.lhs[sd - 1]; }
     }
+    if (lo_fold && (periodic || nc < M || nc + M > 8))
+        throw Error{CLES_ERR_UNSUPPORTED, "a parity fold needs a closed line and nclosure >= M with nclosure + M <= 8"};
     if (closed) {
-        for (int j = 0; j < nc; j++)
-            for (int sd = -q; sd <= q; sd++) A.at(j, sd) = s.closure_lhs[j][q + sd];
+        if (lo_fold) {
+            // operators_banded.jl:55-64: ghost column r - m < 0 of row r maps onto
+            // interior column m - r - 1 with sign sigma_g (q = 1: b[1] += sigma_g alpha)
+            for (int r = 0; r < q; r++)
+                for (int mm = 1; mm <= q; mm++)
+                    if (r - mm < 0) A.at(r, (mm - r - 1) - r) += (double)lo_fold * s.lhs[mm - 1];
+        } else {
+            for (int j = 0; j < nc; j++)
+                for (int sd = -q; sd <= q; sd++) A.at(j, sd) = s.closure_lhs[j][q + sd];
+        }
         for (int j = 0; j < nc; j++) {
             int r = n - 1 - j;
             for (int sd = -q; sd <= q; sd++) A.at(r, sd) = s.closure_lhs[j][q - sd];
@@ -336,6 +346,21 @@ void build_line_tables(LineTables &out, const cles_scheme_desc &s, int n, int pe
             coef[COEF_CLO + j * 8 + kk] = s.closure_rhs[j][kk] * scale;
             coef[COEF_CHI + j * 8 + kk] = s.closure_rhs[j][kk] * (scale * sgn);
         }
+    if (lo_fold) {
+        // the interior stencil of rows 0..nc-1 with each ghost f(j < 0) read as
+        // sigma_f f(-j-1) (folds.jl:79-105 fold_fill! on a half-offset grid)
+        const double sf = s.symmetric ? (double)lo_fold : -(double)lo_fold;
+        for (int j = 0; j < nc; j++) {
+            double *w = &coef[COEF_CLO + j * 8];
+            for (int kk = 0; kk < 8; kk++) w[kk] = 0.0;
+            auto add = [&](int col, double v) { w[col < 0 ? -col - 1 : col] += col < 0 ? sf * v : v; };
+            if (s.symmetric) add(j, s.a0);
+            for (int mm = 1; mm <= M; mm++) {
+                add(j + mm, s.coeffs[mm - 1] * scale);
+                add(j - mm, sgn * s.coeffs[mm - 1] * scale);
+            }
+        }
+    }
 
     // chunk-major copy of the row constants for the ragged kernel
     std::vector<double> rowt;
@@ -392,10 +417,11 @@ void bind_line_tables(LineDev &dev, const LineTables &t, unsigned char *base)
     dev.rowt32 = t.o_rowt64 != t.o_rowt32 ? (const float *)(base + t.o_rowt32) : nullptr;
 }
 
-void build_line_plan(HostPlan &hp, const cles_scheme_desc &s, int n, int periodic, double h, int max_chunk)
+void build_line_plan(HostPlan &hp, const cles_scheme_desc &s, int n, int periodic, double h, int max_chunk,
+                     int lo_fold)
 {
     LineTables t;
-    build_line_tables(t, s, n, periodic, h, max_chunk, hp.dim);
+    build_line_tables(t, s, n, periodic, h, max_chunk, hp.dim, false, lo_fold);
     void *d = nullptr;
     CLES_HIP(hipMalloc(&d, t.image.size()));
     hipError_t e = hipMemcpy(d, t.image.data(), t.image.size(), hipMemcpyHostToDevice);

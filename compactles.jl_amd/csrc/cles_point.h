@@ -178,6 +178,7 @@ struct DiagArgs {
     int64_t n[3];
     int64_t N;
     int closed[3];
+    int fold_lo[3];          // a folded (half-offset) low edge takes a full weight (diagnostics.jl:41-47)
     GradPtrs<W> g;
     Metric1D met;            // stretched: integrand / inv_J (diagnostics.jl:73-75)
 };

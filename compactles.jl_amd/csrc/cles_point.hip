@@ -900,7 +900,7 @@ __global__ void __launch_bounds__(TPB) k_integrand_sum(DiagArgs<W> a, double *__
         double wq = 1.0;
 #pragma unroll
         for (int d = 0; d < 3; d++)
-            if (a.closed[d] && (ii[d] == 0 || ii[d] == a.n[d] - 1)) wq *= 0.5;
+            if (a.closed[d] && ((ii[d] == 0 && !a.fold_lo[d]) || ii[d] == a.n[d] - 1)) wq *= 0.5;
         double f;
         if constexpr (KIND == 0) {
             double rho = 0.0;

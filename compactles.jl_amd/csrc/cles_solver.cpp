@@ -42,6 +42,10 @@ struct cles_solver {
     bool nscbc = false;
     cles::HostPlan dplan[3], fplan[3], fplanQ[3];
     bool own_fplanQ = false;
+    // AxisBC (rhs.jl:155-205, folds.jl): parity-folded plans of dimension 1,
+    // index 0: sigma_g = +1, 1: sigma_g = -1
+    bool fold0 = false;
+    cles::HostPlan dplanF[2], fplanF[2], fplanQF[2];
     // Metric and Stretch (metric.jl:45-137): node tables on the device;
     // stretched: some dimension has a mapping, curv: a curvilinear metric
     bool stretched = false, curv = false;
@@ -170,6 +174,11 @@ struct cles_solver {
             cles::free_line_plan(fplan[k]);
             if (own_fplanQ) cles::free_line_plan(fplanQ[k]);
         }
+        for (int g = 0; g < 2; g++) {
+            cles::free_line_plan(dplanF[g]);
+            cles::free_line_plan(fplanF[g]);
+            if (own_fplanQ) cles::free_line_plan(fplanQF[g]);
+        }
         if (ctx) cles::ctx_release(ctx);
     }
 };
@@ -257,6 +266,47 @@ struct Impl {
         s.timed(phase + d, line_bytes(hp, e.kind) * (double)s.N * nf,
                 [&] { launch_line(hp, d + 1, view(), in[0], e, s.st(), &mf); });
     }
+    // equations.jl:56-66 parities across the fold of dimension d (the axis:
+    // u_r and u_theta odd); flux parity = conserved x normal velocity x area
+    // (A_1 = r is odd); plans by field parity, a derivative flipping it and a
+    // filter keeping it (folds.jl:271-274)
+    int vel_parity(int d, int j) const { return (d == 0 && s.fold0 && j < 2) ? -1 : 1; }
+    int cons_parity(int d, int c) const { return (c >= s.Ns && c < s.Ns + 3) ? vel_parity(d, c - s.Ns) : 1; }
+    int flux_parity(int d, int c) const
+    {
+        return cons_parity(d, c) * vel_parity(d, d) * ((d == 0 && s.fold0) ? -1 : 1);
+    }
+    enum PlanKind { DERIV, FILT, FILT_Q };
+    const HostPlan &plan(PlanKind k, int d, int sf) const
+    {
+        if (d == 0 && s.fold0) {
+            const int g = (k == DERIV ? -sf : sf) > 0 ? 0 : 1;
+            return k == DERIV ? s.dplanF[g] : k == FILT ? s.fplanF[g] : s.fplanQF[g];
+        }
+        return k == DERIV ? s.dplan[d] : k == FILT ? s.fplan[d] : s.fplanQ[d];
+    }
+    // lines() over fields of parities par[]: one launch per plan when a fold
+    // separates them
+    void lines_par(PlanKind k, int d, int nf, const void *const *in, void *const *out, const int *par, EpiArgs e,
+                   int phase) const
+    {
+        if (!(d == 0 && s.fold0)) {
+            lines(plan(k, d, 1), d, nf, in, out, nullptr, e, phase);
+            return;
+        }
+        for (int sg : {1, -1}) {
+            const void *pin[LINE_MF];
+            void *pout[LINE_MF];
+            int m = 0;
+            for (int i = 0; i < nf; i++)
+                if (par[i] == sg) {
+                    pin[m] = in[i];
+                    pout[m++] = out ? out[i] : nullptr;
+                }
+            if (m) lines(plan(k, d, sg), d, m, pin, out ? pout : nullptr, nullptr, e, phase);
+        }
+    }
+
     EpiArgs store(void *out, int kind = EPI_STORE) const
     {
         EpiArgs e{};
@@ -290,17 +340,20 @@ struct Impl {
             const int d = s.adim[k];
             const void *in[LINE_MF] = {s.u, s.v, s.w};
             void *out[LINE_MF] = {s.gu[d * 3], s.gu[d * 3 + 1], s.gu[d * 3 + 2]};
+            int par[LINE_MF] = {vel_parity(d, 0), vel_parity(d, 1), vel_parity(d, 2)};
             int nf = 3;
             if (scalars) {
+                par[nf] = 1;
                 in[nf] = s.T;
                 out[nf++] = s.gT[d];
                 if (s.Ns > 1)
                     for (int sp = 0; sp < s.Ns; sp++) {
+                        par[nf] = 1;
                         in[nf] = s.Y[sp];
                         out[nf++] = s.gY[d][sp];
                     }
             }
-            lines(s.dplan[d], d, nf, in, out, nullptr, store(out[0]), PH_DERIV);
+            lines_par(DERIV, d, nf, in, out, par, store(out[0]), PH_DERIV);
             if (s.met.m[d] || s.curv) metric(out, nf, 0, d);   // rhs.jl:599,631,634 _scale_grad!
         }
         if (s.curv) {   // rhs.jl:604 metric_correct_gradients!
@@ -353,9 +406,9 @@ struct Impl {
         for (int k = 0; k < s.nact; k++) {
             const int d = s.adim[k];
             if (k < s.nact - 1)
-                line(s.fplan[d], d, field, store(field), PH_SMOOTH);
+                line(plan(FILT, d, 1), d, field, store(field), PH_SMOOTH);
             else
-                line(s.fplan[d], d, field, last, PH_SMOOTH);
+                line(plan(FILT, d, 1), d, field, last, PH_SMOOTH);
         }
     }
 
@@ -506,7 +559,9 @@ struct Impl {
                 lines(s.dplan[d], d, s.C, in, nullptr, eoff, e, PH_DIV_LSRK);
             } else {
                 e.kind = k == 0 ? EPI_STORE_NEG : EPI_SUB;
-                lines(s.dplan[d], d, s.C, in, out, nullptr, e, PH_DIV);
+                int par[LINE_MF];
+                for (int c = 0; c < s.C; c++) par[c] = flux_parity(d, c);
+                lines_par(DERIV, d, s.C, in, out, par, e, PH_DIV);
             }
         }
         if (s.geom()) {   // rhs.jl:680-682: dQ -= inv_J D_d(A_d F_d)
@@ -635,16 +690,16 @@ struct Impl {
                 a.N = s.N;
                 a.count = s.N / s.n[d];
                 // full-field derivatives first (nscbc.jl:69-87, :223-224)
-                line(s.dplan[d], d, s.p, store(s.nsd[0]), PH_NSCBC);
+                line(plan(DERIV, d, 1), d, s.p, store(s.nsd[0]), PH_NSCBC);
                 a.dpn = f(s.nsd[0]);
                 if (a.inflow) {
-                    line(s.dplan[d], d, s.rho, store(s.nsd[1]), PH_NSCBC);
+                    line(plan(DERIV, d, 1), d, s.rho, store(s.nsd[1]), PH_NSCBC);
                     a.dt1 = f(s.nsd[1]);
                 } else {
                     a.act1 = s.active[t1];
                     a.act2 = s.active[t2];
-                    if (a.act1) line(s.dplan[t1], t1, s.p, store(s.nsd[1]), PH_NSCBC);
-                    if (a.act2) line(s.dplan[t2], t2, s.p, store(s.nsd[2]), PH_NSCBC);
+                    if (a.act1) line(plan(DERIV, t1, 1), t1, s.p, store(s.nsd[1]), PH_NSCBC);
+                    if (a.act2) line(plan(DERIV, t2, 1), t2, s.p, store(s.nsd[2]), PH_NSCBC);
                     a.dt1 = f(s.nsd[1]);
                     a.dt2 = f(s.nsd[2]);
                 }
@@ -716,7 +771,9 @@ struct Impl {
             EpiArgs e{};
             e.kind = s.dev_filt ? EPI_STORE_GATE : EPI_STORE;
             e.gate = s.dev_filt;
-            lines(s.fplanQ[d], d, s.C, in, out, nullptr, e, PH_FILTER_STATE);
+            int par[LINE_MF];
+            for (int c = 0; c < s.C; c++) par[c] = cons_parity(d, c);
+            lines_par(FILT_Q, d, s.C, in, out, par, e, PH_FILTER_STATE);
         }
     }
 
@@ -728,6 +785,7 @@ struct Impl {
         for (int k = 0; k < 3; k++) {
             a.n[k] = s.n[k];
             a.closed[k] = s.active[k] && !s.periodic[k];
+            a.fold_lo[k] = k == 0 && s.fold0;
         }
         a.N = s.N;
         a.met = s.met;
@@ -829,7 +887,7 @@ void validate_desc(const cles_solver_desc &d)
         if (d.n_global[k] == 1 && lo != CLES_BC_PERIODIC)
             throw cles::Error{CLES_ERR_INVALID, "collapsed dimension must use (PeriodicBC(), PeriodicBC())"};
         for (int sd = 0; sd < 2; sd++)
-            if (d.bc[k][sd] < 0 || d.bc[k][sd] > CLES_BC_NSCBC_IN)
+            if (d.bc[k][sd] < 0 || d.bc[k][sd] > CLES_BC_AXIS)
                 throw cles::Error{CLES_ERR_INVALID, "unknown boundary-condition code"};
     }
     if (d.beta_sensor < 0 || d.beta_sensor > 2)
@@ -975,6 +1033,10 @@ int cles_solver_memory(cles_solver *s, size_t *bytes)
             b += s->dplan[k].dbytes + s->fplan[k].dbytes;
             if (s->own_fplanQ) b += s->fplanQ[k].dbytes;
         }
+        for (int g = 0; g < 2; g++) {
+            b += s->dplanF[g].dbytes + s->fplanF[g].dbytes;
+            if (s->own_fplanQ) b += s->fplanQF[g].dbytes;
+        }
         *bytes = b;
     })
 }
@@ -1006,6 +1068,23 @@ int cles_solver_create_geom(cles_ctx *ctx, const cles_solver_desc *desc, const d
                              "stretch mapping for dim " + std::to_string(k + 1) + " is not monotone increasing");
         }
         const int kind = desc->metric;
+        // rhs.jl:155-163 AxisBC: the low end of r of a cylindrical metric, from
+        // origin 0, unstretched; a resolved theta pairs antipodes (folds.jl),
+        // which this backend does not run
+        for (int k = 0; k < 3; k++)
+            for (int sd = 0; sd < 2; sd++)
+                if (desc->bc[k][sd] == CLES_BC_AXIS && !(k == 0 && sd == 0))
+                    throw cles::Error{CLES_ERR_INVALID, "AxisBC goes at the low end of dimension 1"};
+        const bool axis = desc->bc[0][0] == CLES_BC_AXIS;
+        if (axis) {
+            CLES_REQUIRE(kind == CLES_METRIC_CYLINDRICAL, "AxisBC requires CylindricalMetric");
+            CLES_REQUIRE(!(dxdxi && dxdxi[0]), "folded dimensions cannot be stretched");
+            CLES_REQUIRE(std::fabs(desc->origin[0]) < 1e-14, "AxisBC requires origin[1] = 0");
+            CLES_REQUIRE(desc->n_global[0] > 1, "AxisBC needs an active r dimension");
+            if (desc->n_global[1] != 1)
+                throw cles::Error{CLES_ERR_UNSUPPORTED, "a resolved theta with AxisBC (antipodal pairing) is not "
+                                                        "available on the GPU backend"};
+        }
         // nscbc.jl:309-316 _validate_nscbc_face: the LODI analysis needs a face
         // whose normal scale factor is one (metric.jl:93-95 unit_scalefactor)
         for (int k = 0; k < 3; k++) {
@@ -1059,6 +1138,10 @@ int cles_solver_create_geom(cles_ctx *ctx, const cles_solver_desc *desc, const d
                     s->stretched = true;
                 }
             }
+            if (axis) {   // rhs.jl:196-205: half-offset r, h = L / (n - 1/2), r_1 = h / 2
+                s->fold0 = true;
+                s->h[0] = desc->L_domain[0] / ((double)desc->n_global[0] - 0.5);
+            }
             s->curv = kind != CLES_METRIC_CARTESIAN;
             for (int k = 0; k < s->Ns; k++) {   // physics.jl:111-115
                 s->Rk[k] = desc->R[k];
@@ -1077,6 +1160,23 @@ int cles_solver_create_geom(cles_ctx *ctx, const cles_solver_desc *desc, const d
                 cles::build_line_plan(s->dplan[k], desc->deriv, (int)s->n[k], s->periodic[k], s->h[k], chunk_of(k));
                 cles::build_line_plan(s->fplan[k], desc->filt, (int)s->n[k], s->periodic[k], s->h[k], chunk_of(k));
             }
+            if (s->fold0)   // rhs.jl:256-261 foldspec: derivative and filter plans for sigma_g = +1, -1
+                for (int g = 0; g < 2; g++) {
+                    for (cles::HostPlan *hp : {&s->dplanF[g], &s->fplanF[g], &s->fplanQF[g]}) {
+                        hp->dim = 1;
+                        hp->dtype = wdt;
+                        for (int j = 0; j < 3; j++) hp->nloc[j] = s->n[j];
+                    }
+                    const int sg = g == 0 ? 1 : -1;
+                    cles::build_line_plan(s->dplanF[g], desc->deriv, (int)s->n[0], 0, s->h[0], chunk_of(0), sg);
+                    cles::build_line_plan(s->fplanF[g], desc->filt, (int)s->n[0], 0, s->h[0], chunk_of(0), sg);
+                    if (s->prec == CLES_F64) {
+                        s->fplanQF[g] = s->fplanF[g];
+                    } else {
+                        s->fplanQF[g].dtype = CLES_F64;
+                        cles::build_line_plan(s->fplanQF[g], desc->filt, (int)s->n[0], 0, s->h[0], chunk_of(0), sg);
+                    }
+                }
             if (s->prec == CLES_F64) {
                 for (int k = 0; k < 3; k++) s->fplanQ[k] = s->fplan[k];
             } else {
@@ -1141,7 +1241,8 @@ int cles_solver_create_geom(cles_ctx *ctx, const cles_solver_desc *desc, const d
             if (s->curv) {
                 // metric.jl:94-103: node coordinates origin + i h, or the mapping's
                 auto node = [&](int k, int64_t i) {
-                    return (x && x[k]) ? x[k][i] : desc->origin[k] + (double)i * s->h[k];
+                    const double shift = (k == 0 && s->fold0) ? s->h[0] / 2 : 0.0;   // rhs.jl:205 coord_shift
+                    return (x && x[k]) ? x[k][i] : desc->origin[k] + shift + (double)i * s->h[k];
                 };
                 std::vector<double> r((size_t)s->n[0]);
                 for (int64_t i = 0; i < s->n[0]; i++) {

@@ -48,7 +48,8 @@ enum {
     CLES_BC_SLIPWALL = 1,    /* SlipWallBC            boundary.jl:32,232-243 */
     CLES_BC_DIRICHLET = 2,   /* DirichletBC, constant state  problem.jl:192-207 */
     CLES_BC_NSCBC_OUT = 3,   /* NSCBCOutflowBC        nscbc.jl:45-154 */
-    CLES_BC_NSCBC_IN = 4     /* NSCBCInflowBC, constant targets  nscbc.jl:204-297 */
+    CLES_BC_NSCBC_IN = 4,    /* NSCBCInflowBC, constant targets  nscbc.jl:204-297 */
+    CLES_BC_AXIS = 5         /* AxisBC, low end of r     boundary.jl:55-66, rhs.jl:155-163 */
 };
 
 #define CLES_MAXCL 8         /* closure rows per edge */
@@ -248,8 +249,12 @@ int cles_solver_create_ex(cles_ctx *ctx, const cles_solver_desc *desc, const dou
  * gradients (metric.jl:181-219), the momentum sources (metric.jl:228-269),
  * the discrete-GCL cot(theta)/r of a resolved theta (metric.jl:150-168) and
  * the curvature rate of collapsed angular dimensions (timestep.jl:186-197).
- * Coordinate singularities are not treated: every node needs r > 0 and, for
- * the spherical metric, 0 < theta < pi (CLES_ERR_INVALID otherwise).  NSCBC
+ * The cylindrical axis takes CLES_BC_AXIS at the low end of r (origin[0] =
+ * 0, r unstretched, theta collapsed): a half-offset grid r_i = (i + 1/2) h
+ * with h = L / (n - 1/2) and parity-folded r operators (rhs.jl:155-205,
+ * folds.jl).  Otherwise every node needs r > 0 and, for the spherical
+ * metric, 0 < theta < pi (CLES_ERR_INVALID).  A resolved theta with the axis,
+ * and the spherical origin and poles, are CLES_ERR_UNSUPPORTED.  NSCBC
  * faces must have a normal scale factor of one: cylindrical r and z faces,
  * spherical r faces (nscbc.jl:309-316; CLES_ERR_INVALID otherwise). */
 int cles_solver_create_geom(cles_ctx *ctx, const cles_solver_desc *desc, const double *const x[3],

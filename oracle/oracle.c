@@ -161,6 +161,7 @@ struct or_plan {
     int halo[3];
     int64_t pad[3];
     int periodic, lo_closed, hi_closed;
+    int lo_fold;            /* parity fold at the low edge: sigma_g (0: none) */
     double a0, ci[OR_MAXM];
     int cl_len[OR_MAXCL];
     double clo[OR_MAXCL][OR_MAXRHS], chi[OR_MAXCL][OR_MAXRHS];
@@ -238,8 +239,21 @@ static void band_solve_col(const or_plan *p, double *x, int64_t st)
     }
 }
 
+static or_plan *plan_create_fold(const or_scheme *s, int dim, const int64_t n_local[3], const int halo[3],
+                                 int periodic, double h, int lo_fold, char *msg, int msglen);
+
 or_plan *or_plan_create(const or_scheme *s, int dim, const int64_t n_local[3],
                         const int halo[3], int periodic, double h, char *msg, int msglen)
+{
+    return plan_create_fold(s, dim, n_local, halo, periodic, h, 0, msg, msglen);
+}
+
+/* operators.jl:44-101 / operators_banded.jl:29-125 plan_direction with
+ * lo_fold = sigma_g: the parity fold of a half-offset low edge moves the LHS
+ * coupling to the ghost unknowns onto the mirrored interior columns, and the
+ * RHS keeps the interior stencil reading mirror-filled halos (fold_fill). */
+static or_plan *plan_create_fold(const or_scheme *s, int dim, const int64_t n_local[3], const int halo[3],
+                                 int periodic, double h, int lo_fold, char *msg, int msglen)
 {
     int n = (int)n_local[dim - 1];
     int nc = s->nclosure, M = s->M, q = s->q;
@@ -258,7 +272,8 @@ or_plan *or_plan_create(const or_scheme *s, int dim, const int64_t n_local[3],
     p->dim = dim; p->n = n; p->q = q; p->M = M; p->sym = s->symmetric; p->nc = nc;
     for (int d = 0; d < 3; d++) { p->nloc[d] = n_local[d]; p->halo[d] = halo[d]; p->pad[d] = n_local[d] + 2 * halo[d]; }
     p->periodic = periodic;
-    p->lo_closed = !periodic;   /* at_lo_edge on one rank (decomposition.jl:137-140) */
+    p->lo_fold = periodic ? 0 : lo_fold;
+    p->lo_closed = !periodic && !p->lo_fold;   /* at_lo_edge on one rank (decomposition.jl:137-140) */
     p->hi_closed = !periodic;
 
     /* operators.jl:87-92: prescale by 1/h (derivatives) or 1 (filters) */
@@ -280,6 +295,7 @@ or_plan *or_plan_create(const or_scheme *s, int dim, const int64_t n_local[3],
         double alpha = s->lhs[0];
         double *a = malloc(sizeof(double) * n), *b = malloc(sizeof(double) * n), *c = malloc(sizeof(double) * n);
         for (int i = 0; i < n; i++) { a[i] = alpha; b[i] = 1.0; c[i] = alpha; }
+        if (p->lo_fold) b[0] += (double)p->lo_fold * alpha;   /* operators.jl:61-66 */
         if (p->lo_closed)
             for (int j = 0; j < nc; j++) { a[j] = s->cl_lhs[j][0]; b[j] = s->cl_lhs[j][1]; c[j] = s->cl_lhs[j][2]; }
         if (p->hi_closed)
@@ -287,7 +303,7 @@ or_plan *or_plan_create(const or_scheme *s, int dim, const int64_t n_local[3],
                 int r = n - 1 - j;
                 a[r] = s->cl_lhs[j][2]; b[r] = s->cl_lhs[j][1]; c[r] = s->cl_lhs[j][0];
             }
-        double aL = p->lo_closed ? 0.0 : alpha;
+        double aL = (p->lo_closed || p->lo_fold) ? 0.0 : alpha;
         double cR = p->hi_closed ? 0.0 : alpha;
         p->l = calloc(n, sizeof(double)); p->dinv = calloc(n, sizeof(double)); p->c = calloc(n, sizeof(double));
         p->v = calloc(n, sizeof(double)); p->w = calloc(n, sizeof(double));
@@ -316,6 +332,13 @@ or_plan *or_plan_create(const or_scheme *s, int dim, const int64_t n_local[3],
         for (int i = 0; i < n; i++) AB(q, i) = 1.0;
         for (int sd = 1; sd <= q; sd++)
             for (int i = 0; i < n; i++) { AB(q + sd, i) = s->lhs[sd - 1]; AB(q - sd, i) = s->lhs[sd - 1]; }
+        if (p->lo_fold)   /* operators_banded.jl:55-64 (1-based i, m: jcol = m - i + 1) */
+            for (int i = 1; i <= q; i++)
+                for (int m = 1; m <= q; m++) {
+                    if (i - m >= 1) continue;
+                    int jcol = m - i + 1;
+                    AB(q + (jcol - i), i - 1) += (double)p->lo_fold * s->lhs[m - 1];
+                }
         if (p->lo_closed)
             for (int j = 0; j < nc; j++)
                 for (int sd = -q; sd <= q; sd++) AB(q + sd, j) = s->cl_lhs[j][q + sd];
@@ -327,7 +350,7 @@ or_plan *or_plan_create(const or_scheme *s, int dim, const int64_t n_local[3],
 #undef AB
         double AL[4][4] = {{0}}, CR[4][4] = {{0}};
         int anyL = 0, anyR = 0;
-        if (!p->lo_closed)
+        if (!p->lo_closed && !p->lo_fold)
             for (int r = 1; r <= q; r++)
                 for (int t = r; t <= q; t++) { AL[r - 1][t - 1] = s->lhs[q + r - t - 1]; anyL |= AL[r - 1][t - 1] != 0; }
         if (!p->hi_closed)
@@ -554,6 +577,11 @@ struct or_solver {
      * Stretch or a curvilinear metric makes it non-unit (geom = 1) */
     int geom;
     double *inv_J, *area[3], *inv_h[3], *inv_r, *cotr;
+    /* AxisBC (rhs.jl:155-205, folds.jl): half-offset r with parity-folded r
+     * plans, index 0: sigma_g = +1, 1: sigma_g = -1 */
+    int fold0;
+    double coord_shift0;
+    or_plan *dplanF[2], *fplanF[2];
 };
 
 #define FOR_INTERIOR(s, I)                                                        \
@@ -577,7 +605,8 @@ static void scale_grad(or_solver *s, double *g, int d)
     for (int64_t I = 0; I < s->NP; I++) g[I] *= ih[I];
 }
 
-static void deriv_along(or_solver *s, double *out, const double *f, int d);
+static void deriv_along_p(or_solver *s, double *out, double *f, int d, int sf);
+#define deriv_along(s, out, f, d) deriv_along_p((s), (out), (f), (d), 1)
 
 /* metric.jl:94-103 _phys_and_jac: the physical coordinate and mapping
  * Jacobian at padded index ip along d.  A stretched dimension takes its
@@ -586,7 +615,7 @@ static void deriv_along(or_solver *s, double *out, const double *f, int d);
 static void phys_and_jac(const or_solver *s, int d, int64_t ip, const double *const *x, const double *const *jac,
                          double *xo, double *mo)
 {
-    const double xi = s->d.origin[d] + (double)(ip - s->H[d]) * s->h[d];
+    const double xi = s->d.origin[d] + (d == 0 ? s->coord_shift0 : 0.0) + (double)(ip - s->H[d]) * s->h[d];
     if (!jac || !jac[d]) {
         *xo = xi;
         *mo = 1.0;
@@ -701,6 +730,32 @@ or_solver *or_solver_create_geom(const or_desc *d, const double *const x[3], con
         }
     }
     if (d->metric != 0) s->geom = 1;
+    /* rhs.jl:155-205 AxisBC: cylindrical, unstretched r from origin 0, theta
+     * collapsed here (a resolved theta pairs antipodes: not restated); the
+     * half-offset grid h = L / (n - 1/2), r_1 = h / 2 */
+    for (int k = 0; k < 3; k++)
+        for (int sd = 0; sd < 2; sd++)
+            if (d->bc[k][sd] == OR_BC_AXIS && !(k == 0 && sd == 0)) {
+                snprintf(msg, (size_t)msglen, "AxisBC goes at the low end of dimension 1");
+                free(s);
+                return NULL;
+            }
+    if (d->bc[0][0] == OR_BC_AXIS) {
+        const char *err = NULL;
+        if (d->metric != 1) err = "AxisBC requires CylindricalMetric";
+        else if (jac && jac[0]) err = "folded dimensions cannot be stretched";
+        else if (fabs(d->origin[0]) >= 1e-14) err = "AxisBC requires origin[1] = 0";
+        else if (d->n_global[1] != 1) err = "a resolved theta with AxisBC (antipodal pairing) is not restated";
+        else if (!s->active[0]) err = "AxisBC needs an active r dimension";
+        if (err) {
+            snprintf(msg, (size_t)msglen, "%s", err);
+            free(s);
+            return NULL;
+        }
+        s->fold0 = 1;
+        s->h[0] = d->L_domain[0] / ((double)d->n_global[0] - 0.5);
+        s->coord_shift0 = s->h[0] / 2;
+    }
     /* nscbc.jl:309-316: NSCBC faces need a normal scale factor of one */
     for (int k = 0; k < 3; k++) {
         const int unit = d->metric == 0 || (d->metric == 1 && k != 1) || (d->metric == 2 && k == 0);
@@ -726,6 +781,12 @@ or_solver *or_solver_create_geom(const or_desc *d, const double *const x[3], con
         s->fplan[k] = or_plan_create(&d->filt, k + 1, s->n, halo, s->periodic[k], s->h[k], msg, msglen);
         if (!s->dplan[k] || !s->fplan[k]) { or_solver_destroy(s); return NULL; }
     }
+    if (s->fold0)   /* rhs.jl:256-261 foldspec: plans for sigma_g = +1, -1 */
+        for (int g = 0; g < 2; g++) {
+            s->dplanF[g] = plan_create_fold(&d->deriv, 1, s->n, halo, 0, s->h[0], g == 0 ? 1 : -1, msg, msglen);
+            s->fplanF[g] = plan_create_fold(&d->filt, 1, s->n, halo, 0, s->h[0], g == 0 ? 1 : -1, msg, msglen);
+            if (!s->dplanF[g] || !s->fplanF[g]) { or_solver_destroy(s); return NULL; }
+        }
     s->Q = calloc((size_t)s->NP * s->C, sizeof(double));
     s->dQ = calloc((size_t)s->NP * s->C, sizeof(double));
     s->du = calloc((size_t)s->NP * s->C, sizeof(double));
@@ -749,6 +810,7 @@ void or_solver_destroy(or_solver *s)
 {
     if (!s) return;
     for (int k = 0; k < 3; k++) { or_plan_destroy(s->dplan[k]); or_plan_destroy(s->fplan[k]); }
+    for (int g = 0; g < 2; g++) { or_plan_destroy(s->dplanF[g]); or_plan_destroy(s->fplanF[g]); }
     free(s->Q); free(s->dQ); free(s->du);
     free(s->rho); free(s->u); free(s->v); free(s->w); free(s->p); free(s->T); free(s->c); free(s->cp);
     for (int k = 0; k < s->Ns; k++) { free(s->Y[k]); free(s->D_art[k]); }
@@ -966,15 +1028,55 @@ static void primitives(or_solver *s)
     }
 }
 
-static void deriv_along(or_solver *s, double *out, const double *f, int d)
+/* folds.jl:79-105 fold_fill! at the low edge of dimension 1: halo layer q
+ * mirrors interior layer q with sign sf (half-offset grid) */
+static void fold_fill0(or_solver *s, double *f, int sf)
 {
+    const int H = s->H[0];
+    for (int64_t k = 0; k < s->pad[2]; k++)
+        for (int64_t j = 0; j < s->pad[1]; j++) {
+            double *row = f + s->pad[0] * (j + s->pad[1] * k);
+            for (int q = 1; q <= H; q++) row[H - q] = (double)sf * row[H + q - 1];
+        }
+}
+
+/* rhs.jl:452-475 deriv_along! / filt_along!: sf is the field's parity across
+ * the fold of dimension d (ignored where there is none); a derivative flips it
+ * (sigma_g = -sf), a filter keeps it (folds.jl:271-274) */
+static void deriv_along_p(or_solver *s, double *out, double *f, int d, int sf)
+{
+    if (d == 0 && s->fold0) {
+        fold_fill0(s, f, sf);
+        or_apply_along(s->dplanF[-sf > 0 ? 0 : 1], out, f);
+        return;
+    }
     or_apply_along(s->dplan[d], out, f);
     /* _scale_grad! (rhs.jl:477-483) follows as scale_grad() for the gradients;
      * it is the identity on unstretched dimensions. */
 }
-static void filt_along(or_solver *s, double *out, const double *f, int d)
+static void filt_along_p(or_solver *s, double *out, double *f, int d, int sf)
 {
+    if (d == 0 && s->fold0) {
+        fold_fill0(s, f, sf);
+        or_apply_along(s->fplanF[sf > 0 ? 0 : 1], out, f);
+        return;
+    }
     or_apply_along(s->fplan[d], out, f);
+}
+#define filt_along(s, out, f, d) filt_along_p((s), (out), (f), (d), 1)
+
+/* equations.jl:56-66: parity of velocity component j and of conserved c
+ * across the fold of dimension d (axis: u_r, u_theta odd) */
+static int vel_parity(const or_solver *s, int d, int j)
+{
+    return (d == 0 && s->fold0 && j < 2) ? -1 : 1;
+}
+static int cons_parity(const or_solver *s, int d, int c)
+{
+    if (c == s->im1) return vel_parity(s, d, 0);
+    if (c == s->im2) return vel_parity(s, d, 1);
+    if (c == s->im3) return vel_parity(s, d, 2);
+    return 1;
 }
 
 /* metric.jl:177-219: curvature corrections added to the scaled physical
@@ -1041,7 +1143,7 @@ static void primitives_and_gradients(or_solver *s, int primitives_current)
     for (int jj = 0; jj < 3; jj++)
         for (int d = 0; d < 3; d++) {
             if (s->active[d]) {
-                deriv_along(s, s->gu[d * 3 + jj], vel[jj], d);
+                deriv_along_p(s, s->gu[d * 3 + jj], vel[jj], d, vel_parity(s, d, jj));
                 scale_grad(s, s->gu[d * 3 + jj], d);
             } else {
                 memset(s->gu[d * 3 + jj], 0, sizeof(double) * s->NP);
@@ -1391,7 +1493,9 @@ void or_compute_rhs(or_solver *s, int primitives_current)
             const double *F = s->flux[d][c], *Ad = s->area[d];
 #pragma omp parallel for schedule(static)
             for (int64_t I = 0; I < s->NP; I++) s->tmp_b[I] = Ad[I] * F[I];
-            deriv_along(s, s->tmp_a, s->tmp_b, d);
+            /* equations.jl:62-66 flux_parities: conserved x normal-velocity x area parity */
+            deriv_along_p(s, s->tmp_a, s->tmp_b, d,
+                          cons_parity(s, d, c) * vel_parity(s, d, d) * ((d == 0 && s->fold0) ? -1 : 1));
 #pragma omp parallel for schedule(static)
             for (int64_t k = 0; k < s->n[2]; k++)
                 for (int64_t j = 0; j < s->n[1]; j++)
@@ -1510,7 +1614,7 @@ void or_filter_state(or_solver *s)
         if (!s->active[d]) continue;
         for (int c = 0; c < s->C; c++) exchange_dim(s, Qc(s, c), d);
         for (int c = 0; c < s->C; c++) {
-            filt_along(s, s->tmp_a, Qc(s, c), d);
+            filt_along_p(s, s->tmp_a, Qc(s, c), d, cons_parity(s, d, c));
             copy_interior(s, Qc(s, c), s->tmp_a);
         }
     }
@@ -1619,7 +1723,7 @@ void or_solver_set_cfl(or_solver *s, double cfl) { s->d.cfl = cfl; }
 static inline double quad_weight(const or_solver *s, int d, int64_t i)
 {
     if (!(s->active[d] && !s->periodic[d])) return 1.0;
-    if (i == 0) return 0.5;
+    if (i == 0) return (d == 0 && s->fold0) ? 1.0 : 0.5;   /* a folded edge takes a full weight */
     if (i == s->n[d] - 1) return 0.5;
     return 1.0;
 }
