@@ -18,7 +18,7 @@ MAXCL, MAXRHS, MAXM, MAXSP = 8, 16, 8, 8
 OK, ERR_INVALID, ERR_TOO_SMALL, ERR_HIP, ERR_NONFINITE, ERR_NEGATIVE_DENSITY, ERR_PLANCK, \
     ERR_DT_MIN, ERR_DT_COLLAPSE, ERR_NOMEM, ERR_UNSUPPORTED = range(11)
 F64, F32 = 0, 1
-BC_PERIODIC, BC_SLIPWALL, BC_DIRICHLET, BC_NSCBC_OUT, BC_NSCBC_IN, BC_AXIS = range(6)
+BC_PERIODIC, BC_SLIPWALL, BC_DIRICHLET, BC_NSCBC_OUT, BC_NSCBC_IN, BC_AXIS, BC_ORIGIN, BC_POLE = range(8)
 
 
 class SchemeDesc(C.Structure):

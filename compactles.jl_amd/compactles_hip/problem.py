@@ -195,6 +195,16 @@ class AxisBC(BoundaryCondition):
     code = L.BC_AXIS
 
 
+class OriginBC(BoundaryCondition):
+    """boundary.jl:68-80: regularized spherical origin at the low end of r."""
+    code = L.BC_ORIGIN
+
+
+class PoleBC(BoundaryCondition):
+    """boundary.jl:82-92: regularized spherical poles, at both ends of theta."""
+    code = L.BC_POLE
+
+
 @dataclass
 class NSCBCOutflowBC(BoundaryCondition):
     """nscbc.jl:45-54."""
@@ -351,15 +361,17 @@ class SolverSpec:
                                  f"x(0) = {st.x(0.0)}, x(1) = {st.x(1.0)}, domain = {prob.domain[d]}")
             if self.n[d] == 1 or self.periodic[d]:
                 raise ValueError(f"dimension {d + 1}: stretched dimensions must be non-periodic")
-        # rhs.jl:191-205: stretched dimensions are uniform in xi on [0, 1]; the
-        # axis folds r onto a half-offset grid h = L / (n - 1/2), r_1 = h / 2
-        self.axis = isinstance(prob.bcs[0][0], AxisBC)
+        # rhs.jl:191-205: stretched dimensions are uniform in xi on [0, 1];
+        # folded dimensions (axis, origin: low end of r; poles: both ends of
+        # theta) sit on half-offset grids, h = L / (n - 1/2) or L / n, first node h / 2
+        fold_lo = (isinstance(prob.bcs[0][0], (AxisBC, OriginBC)), isinstance(prob.bcs[1][0], PoleBC), False)
+        fold_hi = (False, isinstance(prob.bcs[1][1], PoleBC), False)
         self.h = tuple(1.0 if self.n[d] == 1 else
                        (1.0 / (self.n[d] - 1) if stretch[d] is not None else
-                        (self.L[d] / (self.n[d] - 0.5) if (d == 0 and self.axis) else
+                        ((self.L[d] / self.n[d] if fold_hi[d] else self.L[d] / (self.n[d] - 0.5)) if fold_lo[d] else
                          (self.L[d] / self.n[d] if self.periodic[d] else self.L[d] / (self.n[d] - 1))))
                        for d in range(3))
-        self.coord_shift = tuple(self.h[d] / 2 if (d == 0 and self.axis) else 0.0 for d in range(3))
+        self.coord_shift = tuple(self.h[d] / 2 if fold_lo[d] and self.n[d] > 1 else 0.0 for d in range(3))
         # the mapping Jacobian at the nodes (metric.jl:100-107), handed to the device
         self.jac = tuple(None if stretch[d] is None else
                          np.ascontiguousarray(np.broadcast_to(

@@ -96,13 +96,14 @@ struct LineTables {
 // interior column -j-1 with sign sigma_g, and the low-edge RHS rows read the
 // mirror f(-j-1) sigma_f of each ghost (sigma_f = sigma_g for filters, -sigma_g
 // for derivatives), carried as synthesised closure rows.
+// hi_fold does the same at the high edge (ghost n + j mirrors n - 1 - j).
 void build_line_tables(LineTables &out, const cles_scheme_desc &s, int n, int periodic, double h, int max_chunk,
-                       int dim, bool chunk_major = false, int lo_fold = 0);
+                       int dim, bool chunk_major = false, int lo_fold = 0, int hi_fold = 0);
 void bind_line_tables(LineDev &dev, const LineTables &t, unsigned char *base);
 
 // Builds the device tables for one (scheme, n, periodic, h).  Throws Error.
 void build_line_plan(HostPlan &hp, const cles_scheme_desc &s, int n, int periodic,
-                     double h, int max_chunk, int lo_fold = 0);
+                     double h, int max_chunk, int lo_fold = 0, int hi_fold = 0);
 void free_line_plan(HostPlan &hp);
 
 // ---------------------------------------------------------------------------

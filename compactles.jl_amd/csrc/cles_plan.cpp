@@ -163,7 +163,7 @@ void chunk_solve(const ChunkFactor &F, std::vector<double> &x)
 }  // namespace
 
 void build_line_tables(LineTables &out, const cles_scheme_desc &s, int n, int periodic, double h, int max_chunk,
-                       int dim, bool chunk_major, int lo_fold)
+                       int dim, bool chunk_major, int lo_fold, int hi_fold)
 {
     const int q = s.q, M = s.M, nc = s.nclosure;
     if (q < 1 || q > 2) throw Error{CLES_ERR_UNSUPPORTED, "scheme LHS half-bandwidth q must be 1 or 2"};
@@ -189,7 +189,7 @@ void build_line_tables(LineTables &out, const cles_scheme_desc &s, int n, int pe
         A.at(i, 0) = 1.0;
         for (int sd = 1; sd <= q; sd++) { A.at(i, sd) = s.lhs[sd - 1]; A.at(i, -sd) = s.lhs[sd - 1]; }
     }
-    if (lo_fold && (periodic || nc < M || nc + M > 8))
+    if ((lo_fold || hi_fold) && (periodic || nc < M || nc + M > 8))
         throw Error{CLES_ERR_UNSUPPORTED, "a parity fold needs a closed line and nclosure >= M with nclosure + M <= 8"};
     if (closed) {
         if (lo_fold) {
@@ -202,9 +202,16 @@ void build_line_tables(LineTables &out, const cles_scheme_desc &s, int n, int pe
             for (int j = 0; j < nc; j++)
                 for (int sd = -q; sd <= q; sd++) A.at(j, sd) = s.closure_lhs[j][q + sd];
         }
-        for (int j = 0; j < nc; j++) {
-            int r = n - 1 - j;
-            for (int sd = -q; sd <= q; sd++) A.at(r, sd) = s.closure_lhs[j][q - sd];
+        if (hi_fold) {
+            // operators_banded.jl:65-73: ghost column r + m >= n maps onto 2n - 1 - (r + m)
+            for (int r = n - q; r < n; r++)
+                for (int mm = 1; mm <= q; mm++)
+                    if (r + mm >= n) A.at(r, (2 * n - 1 - (r + mm)) - r) += (double)hi_fold * s.lhs[mm - 1];
+        } else {
+            for (int j = 0; j < nc; j++) {
+                int r = n - 1 - j;
+                for (int sd = -q; sd <= q; sd++) A.at(r, sd) = s.closure_lhs[j][q - sd];
+            }
         }
         // columns outside the line do not exist at a closed edge
         for (int i = 0; i < q; i++)
@@ -346,6 +353,23 @@ void build_line_tables(LineTables &out, const cles_scheme_desc &s, int n, int pe
             coef[COEF_CLO + j * 8 + kk] = s.closure_rhs[j][kk] * scale;
             coef[COEF_CHI + j * 8 + kk] = s.closure_rhs[j][kk] * (scale * sgn);
         }
+    if (hi_fold) {
+        // rows n-1-jr (jr < nc) from fe[kap] = f[n-1-kap]: the interior stencil
+        // with each ghost f(n + g) read as sigma_f f(n - 1 - g)
+        const double sf = s.symmetric ? (double)hi_fold : -(double)hi_fold;
+        for (int jr = 0; jr < nc; jr++) {
+            double *w = &coef[COEF_CHI + jr * 8];
+            for (int kk = 0; kk < 8; kk++) w[kk] = 0.0;
+            // offset o from row n-1-jr reaches f[n-1-jr+o], i.e. kap = jr - o; kap < 0
+            // is the ghost n + (-kap-1), which mirrors kap' = -kap - 1
+            auto add = [&](int kap, double v) { w[kap < 0 ? -kap - 1 : kap] += kap < 0 ? sf * v : v; };
+            if (s.symmetric) add(jr, s.a0);
+            for (int mm = 1; mm <= M; mm++) {
+                add(jr - mm, s.coeffs[mm - 1] * scale);
+                add(jr + mm, sgn * s.coeffs[mm - 1] * scale);
+            }
+        }
+    }
     if (lo_fold) {
         // the interior stencil of rows 0..nc-1 with each ghost f(j < 0) read as
         // sigma_f f(-j-1) (folds.jl:79-105 fold_fill! on a half-offset grid)
@@ -418,10 +442,10 @@ void bind_line_tables(LineDev &dev, const LineTables &t, unsigned char *base)
 }
 
 void build_line_plan(HostPlan &hp, const cles_scheme_desc &s, int n, int periodic, double h, int max_chunk,
-                     int lo_fold)
+                     int lo_fold, int hi_fold)
 {
     LineTables t;
-    build_line_tables(t, s, n, periodic, h, max_chunk, hp.dim, false, lo_fold);
+    build_line_tables(t, s, n, periodic, h, max_chunk, hp.dim, false, lo_fold, hi_fold);
     void *d = nullptr;
     CLES_HIP(hipMalloc(&d, t.image.size()));
     hipError_t e = hipMemcpy(d, t.image.data(), t.image.size(), hipMemcpyHostToDevice);

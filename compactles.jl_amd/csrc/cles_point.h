@@ -94,6 +94,22 @@ struct CurvArgs {
     int64_t N;
     W *dQ;
 };
+// Antipodal pairing of a fold (folds.jl:118-246): the partner of interior
+// point (i, j, k) shifts pdim by half its extent and reverses revdim (-1:
+// none); canonical points sit in the lower half of sd.
+struct PairMap {
+    int64_t n[3];
+    int pdim, revdim, sd;
+};
+// e / o butterfly of f into w (sf: the field's antipodal sign)
+template <typename T> void launch_pair_forward(const T *f, T *w, const PairMap &pm, double sf, hipStream_t st);
+// inverse butterfly of the even-plan result re (canonical slots) and the
+// odd-plan result ro (partner slots), applied to dst with epilogue kind epi
+// (EPI_STORE, EPI_STORE_NEG, EPI_SUB, EPI_STORE_GATE with gate)
+template <typename T>
+void launch_pair_backward(const T *re, const T *ro, T *dst, const PairMap &pm, double sf, int epi, const int *gate,
+                          hipStream_t st);
+
 template <typename W> void launch_metric_grad(const CurvArgs<W> &a, hipStream_t st);
 template <typename W> void launch_metric_sources(const CurvArgs<W> &a, hipStream_t st);
 
@@ -178,7 +194,7 @@ struct DiagArgs {
     int64_t n[3];
     int64_t N;
     int closed[3];
-    int fold_lo[3];          // a folded (half-offset) low edge takes a full weight (diagnostics.jl:41-47)
+    int fold_lo[3], fold_hi[3];   // a folded (half-offset) edge takes a full weight (diagnostics.jl:41-47)
     GradPtrs<W> g;
     Metric1D met;            // stretched: integrand / inv_J (diagnostics.jl:73-75)
 };

@@ -545,6 +545,57 @@ __global__ void __launch_bounds__(TPB) k_metric_scale(MetricFields<W> fs, Metric
     }
 }
 
+// folds.jl:118-151 _pair_index on one rank
+__device__ __forceinline__ int64_t pair_partner(const PairMap &pm, int64_t (&ii)[3])
+{
+    int64_t m[3] = {ii[0], ii[1], ii[2]};
+    if (pm.pdim >= 0) {
+        const int64_t half = pm.n[pm.pdim] / 2;
+        m[pm.pdim] = ii[pm.pdim] < half ? ii[pm.pdim] + half : ii[pm.pdim] - half;
+    }
+    if (pm.revdim >= 0) m[pm.revdim] = pm.n[pm.revdim] - 1 - ii[pm.revdim];
+    return m[0] + pm.n[0] * (m[1] + pm.n[1] * m[2]);
+}
+
+// folds.jl:160-196 pair_forward!: e = (f(x) + sf f(Mx)) / 2 at the canonical
+// slot, o = (f(x) - sf f(Mx)) / 2 at its partner
+template <typename T>
+__global__ void __launch_bounds__(TPB) k_pair_forward(const T *f, T *w, PairMap pm, double sf, int64_t N)
+{
+    const int64_t I = (int64_t)blockIdx.x * TPB + threadIdx.x;
+    if (I >= N) return;
+    int64_t ii[3];
+    interior_ijk(pm.n, I, ii);
+    if (ii[pm.sd] >= pm.n[pm.sd] / 2) return;
+    const int64_t Mx = pair_partner(pm, ii);
+    const double a = f[I], b = sf * (double)f[Mx];
+    w[I] = (T)(0.5 * (a + b));
+    w[Mx] = (T)(0.5 * (a - b));
+}
+
+// folds.jl:206-246 pair_backward! with the result's epilogue
+template <typename T>
+__global__ void __launch_bounds__(TPB) k_pair_backward(const T *re, const T *ro, T *dst, PairMap pm, double sf,
+                                                       int epi, const int *gate, int64_t N)
+{
+    const int64_t I = (int64_t)blockIdx.x * TPB + threadIdx.x;
+    if (I >= N) return;
+    if (epi == EPI_STORE_GATE && gate && !*gate) return;
+    int64_t ii[3];
+    interior_ijk(pm.n, I, ii);
+    if (ii[pm.sd] >= pm.n[pm.sd] / 2) return;
+    const int64_t Mx = pair_partner(pm, ii);
+    const double Re = re[I], Ro = ro[Mx];
+    const double v[2] = {Re + Ro, sf * (Re - Ro)};
+    const int64_t at[2] = {I, Mx};
+    for (int t = 0; t < 2; t++) {
+        T *q = dst + at[t];
+        if (epi == EPI_STORE_NEG) *q = (T)(-v[t]);
+        else if (epi == EPI_SUB) *q = (T)((double)*q - v[t]);
+        else *q = (T)v[t];
+    }
+}
+
 // metric.jl:181-219 metric_correct_gradients! (grad_u[d][j] at gu[d*3 + j])
 template <typename W>
 __global__ void __launch_bounds__(TPB) k_metric_grad(CurvArgs<W> a)
@@ -900,7 +951,7 @@ __global__ void __launch_bounds__(TPB) k_integrand_sum(DiagArgs<W> a, double *__
         double wq = 1.0;
 #pragma unroll
         for (int d = 0; d < 3; d++)
-            if (a.closed[d] && ((ii[d] == 0 && !a.fold_lo[d]) || ii[d] == a.n[d] - 1)) wq *= 0.5;
+            if (a.closed[d] && ((ii[d] == 0 && !a.fold_lo[d]) || (ii[d] == a.n[d] - 1 && !a.fold_hi[d]))) wq *= 0.5;
         double f;
         if constexpr (KIND == 0) {
             double rho = 0.0;
@@ -1375,6 +1426,29 @@ void launch_metric_sources(const CurvArgs<W> &a, hipStream_t st)
     hipLaunchKernelGGL((k_metric_sources<W>), dim3(nblocks(a.N)), dim3(TPB), 0, st, a);
     CHECK_LAUNCH();
 }
+template <typename T>
+void launch_pair_forward(const T *f, T *w, const PairMap &pm, double sf, hipStream_t st)
+{
+    const int64_t N = pm.n[0] * pm.n[1] * pm.n[2];
+    if (N <= 0) return;
+    hipLaunchKernelGGL((k_pair_forward<T>), dim3(nblocks(N)), dim3(TPB), 0, st, f, w, pm, sf, N);
+    CHECK_LAUNCH();
+}
+template <typename T>
+void launch_pair_backward(const T *re, const T *ro, T *dst, const PairMap &pm, double sf, int epi, const int *gate,
+                          hipStream_t st)
+{
+    const int64_t N = pm.n[0] * pm.n[1] * pm.n[2];
+    if (N <= 0) return;
+    hipLaunchKernelGGL((k_pair_backward<T>), dim3(nblocks(N)), dim3(TPB), 0, st, re, ro, dst, pm, sf, epi, gate, N);
+    CHECK_LAUNCH();
+}
+template void launch_pair_forward<double>(const double *, double *, const PairMap &, double, hipStream_t);
+template void launch_pair_forward<float>(const float *, float *, const PairMap &, double, hipStream_t);
+template void launch_pair_backward<double>(const double *, const double *, double *, const PairMap &, double, int,
+                                           const int *, hipStream_t);
+template void launch_pair_backward<float>(const float *, const float *, float *, const PairMap &, double, int,
+                                          const int *, hipStream_t);
 template void launch_metric_grad<double>(const CurvArgs<double> &, hipStream_t);
 template void launch_metric_grad<float>(const CurvArgs<float> &, hipStream_t);
 template void launch_metric_sources<double>(const CurvArgs<double> &, hipStream_t);

@@ -42,10 +42,19 @@ struct cles_solver {
     bool nscbc = false;
     cles::HostPlan dplan[3], fplan[3], fplanQ[3];
     bool own_fplanQ = false;
-    // AxisBC (rhs.jl:155-205, folds.jl): parity-folded plans of dimension 1,
-    // index 0: sigma_g = +1, 1: sigma_g = -1
-    bool fold0 = false;
-    cles::HostPlan dplanF[2], fplanF[2], fplanQF[2];
+    // coordinate-singularity folds (rhs.jl:143-279, folds.jl:56-73) per
+    // dimension: folded ends, antipodal pairing (pdim shifted by half, revdim
+    // reversed; -1: none, self-paired), antipodal signs of (u, v, w), the area
+    // factor's parity, and parity-folded plans (index 0: sigma_g = +1, 1: -1)
+    struct Fold {
+        bool on = false, lo = false, hi = false;
+        int pdim = -1, revdim = -1;
+        int sv[3] = {1, 1, 1};
+        int area = 1;
+        cles::HostPlan dp[2], fp[2], fpQ[2];
+        bool paired() const { return pdim >= 0 || revdim >= 0; }
+    } fold[3];
+    void *pairw = nullptr, *paira = nullptr, *pairb = nullptr;   // pairing scratch, 8 bytes per point
     // Metric and Stretch (metric.jl:45-137): node tables on the device;
     // stretched: some dimension has a mapping, curv: a curvilinear metric
     bool stretched = false, curv = false;
@@ -174,10 +183,11 @@ struct cles_solver {
             cles::free_line_plan(fplan[k]);
             if (own_fplanQ) cles::free_line_plan(fplanQ[k]);
         }
-        for (int g = 0; g < 2; g++) {
-            cles::free_line_plan(dplanF[g]);
-            cles::free_line_plan(fplanF[g]);
-            if (own_fplanQ) cles::free_line_plan(fplanQF[g]);
+        for (int g = 0; g < 6; g++) {
+            Fold &fo = fold[g / 2];
+            cles::free_line_plan(fo.dp[g % 2]);
+            cles::free_line_plan(fo.fp[g % 2]);
+            if (own_fplanQ) cles::free_line_plan(fo.fpQ[g % 2]);
         }
         if (ctx) cles::ctx_release(ctx);
     }
@@ -270,28 +280,74 @@ struct Impl {
     // u_r and u_theta odd); flux parity = conserved x normal velocity x area
     // (A_1 = r is odd); plans by field parity, a derivative flipping it and a
     // filter keeping it (folds.jl:271-274)
-    int vel_parity(int d, int j) const { return (d == 0 && s.fold0 && j < 2) ? -1 : 1; }
+    int vel_parity(int d, int j) const { return s.fold[d].on ? s.fold[d].sv[j] : 1; }
     int cons_parity(int d, int c) const { return (c >= s.Ns && c < s.Ns + 3) ? vel_parity(d, c - s.Ns) : 1; }
     int flux_parity(int d, int c) const
     {
-        return cons_parity(d, c) * vel_parity(d, d) * ((d == 0 && s.fold0) ? -1 : 1);
+        return cons_parity(d, c) * vel_parity(d, d) * (s.fold[d].on ? s.fold[d].area : 1);
     }
     enum PlanKind { DERIV, FILT, FILT_Q };
     const HostPlan &plan(PlanKind k, int d, int sf) const
     {
-        if (d == 0 && s.fold0) {
+        if (s.fold[d].on) {
             const int g = (k == DERIV ? -sf : sf) > 0 ? 0 : 1;
-            return k == DERIV ? s.dplanF[g] : k == FILT ? s.fplanF[g] : s.fplanQF[g];
+            return k == DERIV ? s.fold[d].dp[g] : k == FILT ? s.fold[d].fp[g] : s.fold[d].fpQ[g];
         }
         return k == DERIV ? s.dplan[d] : k == FILT ? s.fplan[d] : s.fplanQ[d];
     }
+    PairMap pair_map(int d) const
+    {
+        PairMap pm{};
+        for (int k = 0; k < 3; k++) pm.n[k] = s.n[k];
+        pm.pdim = s.fold[d].pdim;
+        pm.revdim = s.fold[d].revdim;
+        pm.sd = pm.pdim >= 0 ? pm.pdim : pm.revdim;
+        return pm;
+    }
+    // folds.jl:276-330 fold_apply! for a paired fold, one field: the e / o
+    // butterfly, both parity plans over every line (the e half folds with sf,
+    // the o half as odd), and the inverse butterfly into out with e's
+    // epilogue kind (store, negated store, subtraction, gated store)
+    template <typename T>
+    void pair_apply(PlanKind k, int d, const void *in, void *out, int sf, EpiArgs e, int phase) const
+    {
+        const PairMap pm = pair_map(d);
+        T *w = (T *)s.pairw, *a = (T *)s.paira, *b = (T *)s.pairb;
+        s.timed(PH_OTHER, 2.0 * 2 * sizeof(T) * (double)s.N,
+                [&] { launch_pair_forward<T>((const T *)in, w, pm, (double)sf, s.st()); });
+        line(plan(k, d, sf), d, w, store(a), phase);
+        line(plan(k, d, -1), d, w, store(b), phase);
+        if (!(e.kind == EPI_STORE || e.kind == EPI_STORE_NEG || e.kind == EPI_SUB || e.kind == EPI_STORE_GATE))
+            throw Error{CLES_ERR_INVALID, "epilogue not available through a paired fold"};
+        s.timed(PH_OTHER, 2.0 * 3 * sizeof(T) * (double)s.N, [&] {
+            launch_pair_backward<T>(a, b, (T *)out, pm, (double)sf, e.kind, e.gate, s.st());
+        });
+    }
+    // line() of one field of parity sf along d, through the fold if any
+    void line_par(PlanKind k, int d, const void *in, int sf, EpiArgs e, int phase) const
+    {
+        if (s.fold[d].on && s.fold[d].paired()) {
+            if (k == FILT_Q) pair_apply<double>(k, d, in, e.out, sf, e, phase);
+            else pair_apply<W>(k, d, in, e.out, sf, e, phase);
+            return;
+        }
+        line(plan(k, d, sf), d, in, e, phase);
+    }
     // lines() over fields of parities par[]: one launch per plan when a fold
-    // separates them
+    // separates them, one pass per field through a paired fold
     void lines_par(PlanKind k, int d, int nf, const void *const *in, void *const *out, const int *par, EpiArgs e,
                    int phase) const
     {
-        if (!(d == 0 && s.fold0)) {
+        if (!s.fold[d].on) {
             lines(plan(k, d, 1), d, nf, in, out, nullptr, e, phase);
+            return;
+        }
+        if (s.fold[d].paired()) {
+            for (int i = 0; i < nf; i++) {
+                EpiArgs ei = e;
+                ei.out = out[i];
+                line_par(k, d, in[i], par[i], ei, phase);
+            }
             return;
         }
         for (int sg : {1, -1}) {
@@ -385,7 +441,8 @@ struct Impl {
         void *a2 = s.flux[s.adim[0]][0];   // flux scratch, free outside the RHS
         void *da = s.flux[s.adim[0]][1];
         metric(&a2, 1, 3, 1);
-        line(s.dplan[1], 1, a2, store(da), PH_OTHER);
+        // with the sign of the theta-momentum pressure flux across a pole fold
+        line_par(DERIV, 1, a2, flux_parity(1, s.Ns + 1), store(da), PH_OTHER);
         metric(&da, 1, 2, 0);
         launch_convert<double, W>(cotr, f(da), s.N, s.st());
     }
@@ -405,8 +462,10 @@ struct Impl {
     {
         for (int k = 0; k < s.nact; k++) {
             const int d = s.adim[k];
+            // a paired fold never folds the last active dimension (its pairing
+            // dimension comes after it), so the fused last pass stays plain
             if (k < s.nact - 1)
-                line(plan(FILT, d, 1), d, field, store(field), PH_SMOOTH);
+                line_par(FILT, d, field, 1, store(field), PH_SMOOTH);
             else
                 line(plan(FILT, d, 1), d, field, last, PH_SMOOTH);
         }
@@ -690,16 +749,16 @@ struct Impl {
                 a.N = s.N;
                 a.count = s.N / s.n[d];
                 // full-field derivatives first (nscbc.jl:69-87, :223-224)
-                line(plan(DERIV, d, 1), d, s.p, store(s.nsd[0]), PH_NSCBC);
+                line_par(DERIV, d, s.p, 1, store(s.nsd[0]), PH_NSCBC);
                 a.dpn = f(s.nsd[0]);
                 if (a.inflow) {
-                    line(plan(DERIV, d, 1), d, s.rho, store(s.nsd[1]), PH_NSCBC);
+                    line_par(DERIV, d, s.rho, 1, store(s.nsd[1]), PH_NSCBC);
                     a.dt1 = f(s.nsd[1]);
                 } else {
                     a.act1 = s.active[t1];
                     a.act2 = s.active[t2];
-                    if (a.act1) line(plan(DERIV, t1, 1), t1, s.p, store(s.nsd[1]), PH_NSCBC);
-                    if (a.act2) line(plan(DERIV, t2, 1), t2, s.p, store(s.nsd[2]), PH_NSCBC);
+                    if (a.act1) line_par(DERIV, t1, s.p, 1, store(s.nsd[1]), PH_NSCBC);
+                    if (a.act2) line_par(DERIV, t2, s.p, 1, store(s.nsd[2]), PH_NSCBC);
                     a.dt1 = f(s.nsd[1]);
                     a.dt2 = f(s.nsd[2]);
                 }
@@ -785,7 +844,8 @@ struct Impl {
         for (int k = 0; k < 3; k++) {
             a.n[k] = s.n[k];
             a.closed[k] = s.active[k] && !s.periodic[k];
-            a.fold_lo[k] = k == 0 && s.fold0;
+            a.fold_lo[k] = s.fold[k].on && s.fold[k].lo;
+            a.fold_hi[k] = s.fold[k].on && s.fold[k].hi;
         }
         a.N = s.N;
         a.met = s.met;
@@ -887,7 +947,7 @@ void validate_desc(const cles_solver_desc &d)
         if (d.n_global[k] == 1 && lo != CLES_BC_PERIODIC)
             throw cles::Error{CLES_ERR_INVALID, "collapsed dimension must use (PeriodicBC(), PeriodicBC())"};
         for (int sd = 0; sd < 2; sd++)
-            if (d.bc[k][sd] < 0 || d.bc[k][sd] > CLES_BC_AXIS)
+            if (d.bc[k][sd] < 0 || d.bc[k][sd] > CLES_BC_POLE)
                 throw cles::Error{CLES_ERR_INVALID, "unknown boundary-condition code"};
     }
     if (d.beta_sensor < 0 || d.beta_sensor > 2)
@@ -1015,6 +1075,12 @@ int cles_solver_footprint(const cles_solver_desc *desc, size_t *bytes)
         fields += 3 + (Ns > 1 ? Ns : 0);                          // mu*, beta*, kappa*, D*
         if (d.art_enabled) fields += 3 + (d.beta_sensor == 2 ? 2 : 0);
         if (nscbc) fields += 3;
+        {   // pairing scratch of a paired fold (folds.jl:276-330): three 8-byte fields
+            const bool paired = (d.bc[0][0] == CLES_BC_AXIS && d.n_global[1] > 1) ||
+                                (d.bc[0][0] == CLES_BC_ORIGIN && (d.n_global[1] > 1 || d.n_global[2] > 1)) ||
+                                (d.bc[1][0] == CLES_BC_POLE && d.n_global[2] > 1);
+            if (paired) fields += 3.0 * 8.0 / w;
+        }
         if (d.metric != CLES_METRIC_CARTESIAN) {   // grad_u rows of collapsed angular dims, GCL cot/r
             fields += 3.0 * ((d.n_global[1] == 1) + (d.metric == CLES_METRIC_SPHERICAL && d.n_global[2] == 1));
             if (d.metric == CLES_METRIC_SPHERICAL && d.n_global[1] > 1) fields += 8.0 / w;
@@ -1033,9 +1099,10 @@ int cles_solver_memory(cles_solver *s, size_t *bytes)
             b += s->dplan[k].dbytes + s->fplan[k].dbytes;
             if (s->own_fplanQ) b += s->fplanQ[k].dbytes;
         }
-        for (int g = 0; g < 2; g++) {
-            b += s->dplanF[g].dbytes + s->fplanF[g].dbytes;
-            if (s->own_fplanQ) b += s->fplanQF[g].dbytes;
+        for (int g = 0; g < 6; g++) {
+            const cles_solver::Fold &fo = s->fold[g / 2];
+            b += fo.dp[g % 2].dbytes + fo.fp[g % 2].dbytes;
+            if (s->own_fplanQ) b += fo.fpQ[g % 2].dbytes;
         }
         *bytes = b;
     })
@@ -1068,22 +1135,45 @@ int cles_solver_create_geom(cles_ctx *ctx, const cles_solver_desc *desc, const d
                              "stretch mapping for dim " + std::to_string(k + 1) + " is not monotone increasing");
         }
         const int kind = desc->metric;
-        // rhs.jl:155-163 AxisBC: the low end of r of a cylindrical metric, from
-        // origin 0, unstretched; a resolved theta pairs antipodes (folds.jl),
-        // which this backend does not run
+        // rhs.jl:143-190: coordinate-singularity folds recognised from the BCs
         for (int k = 0; k < 3; k++)
-            for (int sd = 0; sd < 2; sd++)
-                if (desc->bc[k][sd] == CLES_BC_AXIS && !(k == 0 && sd == 0))
+            for (int sd = 0; sd < 2; sd++) {
+                const int bc = desc->bc[k][sd];
+                if (bc == CLES_BC_AXIS && !(k == 0 && sd == 0))
                     throw cles::Error{CLES_ERR_INVALID, "AxisBC goes at the low end of dimension 1"};
-        const bool axis = desc->bc[0][0] == CLES_BC_AXIS;
+                if (bc == CLES_BC_ORIGIN && !(k == 0 && sd == 0))
+                    throw cles::Error{CLES_ERR_INVALID, "OriginBC goes at the low end of dimension 1"};
+                if (bc == CLES_BC_POLE && k != 1)
+                    throw cles::Error{CLES_ERR_INVALID, "PoleBC goes on dimension 2 (theta)"};
+            }
+        const bool axis = desc->bc[0][0] == CLES_BC_AXIS, orig = desc->bc[0][0] == CLES_BC_ORIGIN;
+        const bool pole_l = desc->bc[1][0] == CLES_BC_POLE, pole_h = desc->bc[1][1] == CLES_BC_POLE;
+        CLES_REQUIRE(pole_l == pole_h, "PoleBC must be applied at both ends of theta");
+        const int64_t *ng = desc->n_global;
+        const double *Ld = desc->L_domain;
+        auto even_period = [&](int k) { return ng[k] == 1 || (ng[k] % 2 == 0 && std::fabs(Ld[k] - 2 * M_PI) <= 1e-10); };
         if (axis) {
             CLES_REQUIRE(kind == CLES_METRIC_CYLINDRICAL, "AxisBC requires CylindricalMetric");
             CLES_REQUIRE(!(dxdxi && dxdxi[0]), "folded dimensions cannot be stretched");
             CLES_REQUIRE(std::fabs(desc->origin[0]) < 1e-14, "AxisBC requires origin[1] = 0");
-            CLES_REQUIRE(desc->n_global[0] > 1, "AxisBC needs an active r dimension");
-            if (desc->n_global[1] != 1)
-                throw cles::Error{CLES_ERR_UNSUPPORTED, "a resolved theta with AxisBC (antipodal pairing) is not "
-                                                        "available on the GPU backend"};
+            CLES_REQUIRE(even_period(1), "resolved-theta AxisBC requires an even theta point count over 2 pi");
+        }
+        if (orig) {
+            CLES_REQUIRE(kind == CLES_METRIC_SPHERICAL, "OriginBC requires SphericalMetric");
+            CLES_REQUIRE(!(dxdxi && dxdxi[0]), "folded dimensions cannot be stretched");
+            CLES_REQUIRE(std::fabs(desc->origin[0]) < 1e-14, "OriginBC requires origin[1] = 0");
+            CLES_REQUIRE(even_period(2), "resolved-phi OriginBC requires an even phi point count over 2 pi");
+            CLES_REQUIRE(ng[1] == 1 || std::fabs(2 * desc->origin[1] + Ld[1] - M_PI) <= 1e-10,
+                         "OriginBC requires a theta range symmetric about pi/2");
+            CLES_REQUIRE(!(ng[1] > 1 && ng[2] == 1 && ng[1] % 2),
+                         "OriginBC with a collapsed phi needs an even theta point count");
+        }
+        if (pole_l) {
+            CLES_REQUIRE(kind == CLES_METRIC_SPHERICAL, "PoleBC requires SphericalMetric");
+            CLES_REQUIRE(!(dxdxi && dxdxi[1]), "folded dimensions cannot be stretched");
+            CLES_REQUIRE(std::fabs(desc->origin[1]) < 1e-14 && std::fabs(Ld[1] - M_PI) <= 1e-10,
+                         "PoleBC requires the theta domain (0, pi)");
+            CLES_REQUIRE(even_period(2), "resolved-phi PoleBC requires an even phi point count over 2 pi");
         }
         // nscbc.jl:309-316 _validate_nscbc_face: the LODI analysis needs a face
         // whose normal scale factor is one (metric.jl:93-95 unit_scalefactor)
@@ -1138,10 +1228,23 @@ int cles_solver_create_geom(cles_ctx *ctx, const cles_solver_desc *desc, const d
                     s->stretched = true;
                 }
             }
-            if (axis) {   // rhs.jl:196-205: half-offset r, h = L / (n - 1/2), r_1 = h / 2
-                s->fold0 = true;
-                s->h[0] = desc->L_domain[0] / ((double)desc->n_global[0] - 0.5);
-            }
+            // rhs.jl:256-279 foldspec: folded ends, pairing, sigvel, area parity
+            auto set_fold = [&](int k, bool hi, int pdim, int revdim, int s0, int s1, int s2, int area) {
+                cles_solver::Fold &fo = s->fold[k];
+                fo.on = true;
+                fo.lo = true;
+                fo.hi = hi;
+                fo.pdim = pdim >= 0 && s->active[pdim] ? pdim : -1;
+                fo.revdim = revdim >= 0 && s->active[revdim] ? revdim : -1;
+                fo.sv[0] = s0; fo.sv[1] = s1; fo.sv[2] = s2;
+                fo.area = area;
+            };
+            if (axis && s->active[0]) set_fold(0, false, 1, -1, -1, -1, 1, -1);   // A_1 = r
+            if (orig && s->active[0]) set_fold(0, false, 2, 1, -1, 1, -1, 1);    // A_1 = r^2 sin(theta)
+            if (pole_l && s->active[1]) set_fold(1, true, 2, -1, 1, -1, -1, -1); // A_2 = r sin(theta)
+            for (int k = 0; k < 3; k++)   // rhs.jl:196-204: half-offset grids
+                if (s->fold[k].on)
+                    s->h[k] = s->fold[k].hi ? Ld[k] / (double)ng[k] : Ld[k] / ((double)ng[k] - 0.5);
             s->curv = kind != CLES_METRIC_CARTESIAN;
             for (int k = 0; k < s->Ns; k++) {   // physics.jl:111-115
                 s->Rk[k] = desc->R[k];
@@ -1160,23 +1263,26 @@ int cles_solver_create_geom(cles_ctx *ctx, const cles_solver_desc *desc, const d
                 cles::build_line_plan(s->dplan[k], desc->deriv, (int)s->n[k], s->periodic[k], s->h[k], chunk_of(k));
                 cles::build_line_plan(s->fplan[k], desc->filt, (int)s->n[k], s->periodic[k], s->h[k], chunk_of(k));
             }
-            if (s->fold0)   // rhs.jl:256-261 foldspec: derivative and filter plans for sigma_g = +1, -1
+            for (int k = 0; k < 3; k++) {   // rhs.jl:256-261: plans for sigma_g = +1, -1
+                cles_solver::Fold &fo = s->fold[k];
+                if (!fo.on) continue;
                 for (int g = 0; g < 2; g++) {
-                    for (cles::HostPlan *hp : {&s->dplanF[g], &s->fplanF[g], &s->fplanQF[g]}) {
-                        hp->dim = 1;
+                    for (cles::HostPlan *hp : {&fo.dp[g], &fo.fp[g], &fo.fpQ[g]}) {
+                        hp->dim = k + 1;
                         hp->dtype = wdt;
                         for (int j = 0; j < 3; j++) hp->nloc[j] = s->n[j];
                     }
-                    const int sg = g == 0 ? 1 : -1;
-                    cles::build_line_plan(s->dplanF[g], desc->deriv, (int)s->n[0], 0, s->h[0], chunk_of(0), sg);
-                    cles::build_line_plan(s->fplanF[g], desc->filt, (int)s->n[0], 0, s->h[0], chunk_of(0), sg);
+                    const int sg = g == 0 ? 1 : -1, lo = sg, hi = fo.hi ? sg : 0;
+                    cles::build_line_plan(fo.dp[g], desc->deriv, (int)s->n[k], 0, s->h[k], chunk_of(k), lo, hi);
+                    cles::build_line_plan(fo.fp[g], desc->filt, (int)s->n[k], 0, s->h[k], chunk_of(k), lo, hi);
                     if (s->prec == CLES_F64) {
-                        s->fplanQF[g] = s->fplanF[g];
+                        fo.fpQ[g] = fo.fp[g];
                     } else {
-                        s->fplanQF[g].dtype = CLES_F64;
-                        cles::build_line_plan(s->fplanQF[g], desc->filt, (int)s->n[0], 0, s->h[0], chunk_of(0), sg);
+                        fo.fpQ[g].dtype = CLES_F64;
+                        cles::build_line_plan(fo.fpQ[g], desc->filt, (int)s->n[k], 0, s->h[k], chunk_of(k), lo, hi);
                     }
                 }
+            }
             if (s->prec == CLES_F64) {
                 for (int k = 0; k < 3; k++) s->fplanQ[k] = s->fplan[k];
             } else {
@@ -1193,6 +1299,8 @@ int cles_solver_create_geom(cles_ctx *ctx, const cles_solver_desc *desc, const d
             s->Q = (double *)s->alloc(N * s->C * sizeof(double));
             s->du = (double *)s->alloc(N * s->C * sizeof(double));
             s->dQ = s->alloc(N * s->C * s->wsize());
+            if (s->fold[0].paired() || s->fold[1].paired() || s->fold[2].paired())
+                for (void **fp : {&s->pairw, &s->paira, &s->pairb}) *fp = s->alloc(N * sizeof(double));
             for (void **fp : {&s->rho, &s->u, &s->v, &s->w, &s->p, &s->T, &s->c, &s->cp}) *fp = s->wfield();
             for (int k = 0; k < s->Ns; k++) s->Y[k] = s->wfield();
             for (int dd = 0; dd < 3; dd++) {
@@ -1241,7 +1349,7 @@ int cles_solver_create_geom(cles_ctx *ctx, const cles_solver_desc *desc, const d
             if (s->curv) {
                 // metric.jl:94-103: node coordinates origin + i h, or the mapping's
                 auto node = [&](int k, int64_t i) {
-                    const double shift = (k == 0 && s->fold0) ? s->h[0] / 2 : 0.0;   // rhs.jl:205 coord_shift
+                    const double shift = s->fold[k].on ? s->h[k] / 2 : 0.0;   // rhs.jl:205 coord_shift
                     return (x && x[k]) ? x[k][i] : desc->origin[k] + shift + (double)i * s->h[k];
                 };
                 std::vector<double> r((size_t)s->n[0]);

@@ -49,7 +49,9 @@ enum {
     CLES_BC_DIRICHLET = 2,   /* DirichletBC, constant state  problem.jl:192-207 */
     CLES_BC_NSCBC_OUT = 3,   /* NSCBCOutflowBC        nscbc.jl:45-154 */
     CLES_BC_NSCBC_IN = 4,    /* NSCBCInflowBC, constant targets  nscbc.jl:204-297 */
-    CLES_BC_AXIS = 5         /* AxisBC, low end of r     boundary.jl:55-66, rhs.jl:155-163 */
+    CLES_BC_AXIS = 5,        /* AxisBC, low end of r     boundary.jl:55-66, rhs.jl:155-163 */
+    CLES_BC_ORIGIN = 6,      /* OriginBC, low end of r   boundary.jl:68-80, rhs.jl:164-173 */
+    CLES_BC_POLE = 7         /* PoleBC, both ends of theta  boundary.jl:82-92, rhs.jl:174-180 */
 };
 
 #define CLES_MAXCL 8         /* closure rows per edge */
@@ -249,12 +251,16 @@ int cles_solver_create_ex(cles_ctx *ctx, const cles_solver_desc *desc, const dou
  * gradients (metric.jl:181-219), the momentum sources (metric.jl:228-269),
  * the discrete-GCL cot(theta)/r of a resolved theta (metric.jl:150-168) and
  * the curvature rate of collapsed angular dimensions (timestep.jl:186-197).
- * The cylindrical axis takes CLES_BC_AXIS at the low end of r (origin[0] =
- * 0, r unstretched, theta collapsed): a half-offset grid r_i = (i + 1/2) h
- * with h = L / (n - 1/2) and parity-folded r operators (rhs.jl:155-205,
- * folds.jl).  Otherwise every node needs r > 0 and, for the spherical
- * metric, 0 < theta < pi (CLES_ERR_INVALID).  A resolved theta with the axis,
- * and the spherical origin and poles, are CLES_ERR_UNSUPPORTED.  NSCBC
+ * Coordinate singularities fold (rhs.jl:143-205, folds.jl): CLES_BC_AXIS
+ * at the low end of r (cylindrical; theta collapsed, or resolved over 2 pi
+ * with an even count), CLES_BC_ORIGIN at the low end of r (spherical; theta
+ * symmetric about pi/2, phi collapsed or even over 2 pi), CLES_BC_POLE at
+ * both ends of theta (spherical, theta over (0, pi)).  Folded dimensions are
+ * unstretched, from origin 0, on half-offset grids (h = L / (n - 1/2) with
+ * one folded end, L / n with two), with parity-folded operators and, where
+ * the antipode is another line, the even / odd pairing.  Otherwise every
+ * node needs r > 0 and, for the spherical metric, 0 < theta < pi
+ * (CLES_ERR_INVALID).  NSCBC
  * faces must have a normal scale factor of one: cylindrical r and z faces,
  * spherical r faces (nscbc.jl:309-316; CLES_ERR_INVALID otherwise). */
 int cles_solver_create_geom(cles_ctx *ctx, const cles_solver_desc *desc, const double *const x[3],

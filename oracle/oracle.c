@@ -161,7 +161,7 @@ struct or_plan {
     int halo[3];
     int64_t pad[3];
     int periodic, lo_closed, hi_closed;
-    int lo_fold;            /* parity fold at the low edge: sigma_g (0: none) */
+    int lo_fold, hi_fold;   /* parity folds at the edges: sigma_g (0: none) */
     double a0, ci[OR_MAXM];
     int cl_len[OR_MAXCL];
     double clo[OR_MAXCL][OR_MAXRHS], chi[OR_MAXCL][OR_MAXRHS];
@@ -240,12 +240,12 @@ static void band_solve_col(const or_plan *p, double *x, int64_t st)
 }
 
 static or_plan *plan_create_fold(const or_scheme *s, int dim, const int64_t n_local[3], const int halo[3],
-                                 int periodic, double h, int lo_fold, char *msg, int msglen);
+                                 int periodic, double h, int lo_fold, int hi_fold, char *msg, int msglen);
 
 or_plan *or_plan_create(const or_scheme *s, int dim, const int64_t n_local[3],
                         const int halo[3], int periodic, double h, char *msg, int msglen)
 {
-    return plan_create_fold(s, dim, n_local, halo, periodic, h, 0, msg, msglen);
+    return plan_create_fold(s, dim, n_local, halo, periodic, h, 0, 0, msg, msglen);
 }
 
 /* operators.jl:44-101 / operators_banded.jl:29-125 plan_direction with
@@ -253,7 +253,7 @@ or_plan *or_plan_create(const or_scheme *s, int dim, const int64_t n_local[3],
  * coupling to the ghost unknowns onto the mirrored interior columns, and the
  * RHS keeps the interior stencil reading mirror-filled halos (fold_fill). */
 static or_plan *plan_create_fold(const or_scheme *s, int dim, const int64_t n_local[3], const int halo[3],
-                                 int periodic, double h, int lo_fold, char *msg, int msglen)
+                                 int periodic, double h, int lo_fold, int hi_fold, char *msg, int msglen)
 {
     int n = (int)n_local[dim - 1];
     int nc = s->nclosure, M = s->M, q = s->q;
@@ -273,8 +273,9 @@ static or_plan *plan_create_fold(const or_scheme *s, int dim, const int64_t n_lo
     for (int d = 0; d < 3; d++) { p->nloc[d] = n_local[d]; p->halo[d] = halo[d]; p->pad[d] = n_local[d] + 2 * halo[d]; }
     p->periodic = periodic;
     p->lo_fold = periodic ? 0 : lo_fold;
+    p->hi_fold = periodic ? 0 : hi_fold;
     p->lo_closed = !periodic && !p->lo_fold;   /* at_lo_edge on one rank (decomposition.jl:137-140) */
-    p->hi_closed = !periodic;
+    p->hi_closed = !periodic && !p->hi_fold;
 
     /* operators.jl:87-92: prescale by 1/h (derivatives) or 1 (filters) */
     double hinv = 1.0 / h;
@@ -295,7 +296,8 @@ static or_plan *plan_create_fold(const or_scheme *s, int dim, const int64_t n_lo
         double alpha = s->lhs[0];
         double *a = malloc(sizeof(double) * n), *b = malloc(sizeof(double) * n), *c = malloc(sizeof(double) * n);
         for (int i = 0; i < n; i++) { a[i] = alpha; b[i] = 1.0; c[i] = alpha; }
-        if (p->lo_fold) b[0] += (double)p->lo_fold * alpha;   /* operators.jl:61-66 */
+        if (p->lo_fold) b[0] += (double)p->lo_fold * alpha;       /* operators.jl:61-66 */
+        if (p->hi_fold) b[n - 1] += (double)p->hi_fold * alpha;   /* operators.jl:67-70 */
         if (p->lo_closed)
             for (int j = 0; j < nc; j++) { a[j] = s->cl_lhs[j][0]; b[j] = s->cl_lhs[j][1]; c[j] = s->cl_lhs[j][2]; }
         if (p->hi_closed)
@@ -304,7 +306,7 @@ static or_plan *plan_create_fold(const or_scheme *s, int dim, const int64_t n_lo
                 a[r] = s->cl_lhs[j][2]; b[r] = s->cl_lhs[j][1]; c[r] = s->cl_lhs[j][0];
             }
         double aL = (p->lo_closed || p->lo_fold) ? 0.0 : alpha;
-        double cR = p->hi_closed ? 0.0 : alpha;
+        double cR = (p->hi_closed || p->hi_fold) ? 0.0 : alpha;
         p->l = calloc(n, sizeof(double)); p->dinv = calloc(n, sizeof(double)); p->c = calloc(n, sizeof(double));
         p->v = calloc(n, sizeof(double)); p->w = calloc(n, sizeof(double));
         trifactor(p, a, b, c);
@@ -339,6 +341,13 @@ static or_plan *plan_create_fold(const or_scheme *s, int dim, const int64_t n_lo
                     int jcol = m - i + 1;
                     AB(q + (jcol - i), i - 1) += (double)p->lo_fold * s->lhs[m - 1];
                 }
+        if (p->hi_fold)   /* operators_banded.jl:65-73: ghost column i + m > n maps onto 2n + 1 - (i + m) */
+            for (int i = n - q + 1; i <= n; i++)
+                for (int m = 1; m <= q; m++) {
+                    if (i + m <= n) continue;
+                    int jcol = 2 * n + 1 - (i + m);
+                    AB(q + (jcol - i), i - 1) += (double)p->hi_fold * s->lhs[m - 1];
+                }
         if (p->lo_closed)
             for (int j = 0; j < nc; j++)
                 for (int sd = -q; sd <= q; sd++) AB(q + sd, j) = s->cl_lhs[j][q + sd];
@@ -353,7 +362,7 @@ static or_plan *plan_create_fold(const or_scheme *s, int dim, const int64_t n_lo
         if (!p->lo_closed && !p->lo_fold)
             for (int r = 1; r <= q; r++)
                 for (int t = r; t <= q; t++) { AL[r - 1][t - 1] = s->lhs[q + r - t - 1]; anyL |= AL[r - 1][t - 1] != 0; }
-        if (!p->hi_closed)
+        if (!p->hi_closed && !p->hi_fold)
             for (int r = 1; r <= q; r++)
                 for (int t = 1; t <= r; t++) { CR[r - 1][t - 1] = s->lhs[t + q - r - 1]; anyR |= CR[r - 1][t - 1] != 0; }
         p->BL = calloc((size_t)q * n, sizeof(double));
@@ -577,11 +586,17 @@ struct or_solver {
      * Stretch or a curvilinear metric makes it non-unit (geom = 1) */
     int geom;
     double *inv_J, *area[3], *inv_h[3], *inv_r, *cotr;
-    /* AxisBC (rhs.jl:155-205, folds.jl): half-offset r with parity-folded r
-     * plans, index 0: sigma_g = +1, 1: sigma_g = -1 */
-    int fold0;
-    double coord_shift0;
-    or_plan *dplanF[2], *fplanF[2];
+    /* coordinate-singularity folds (rhs.jl:155-279, folds.jl:56-73): which
+     * ends fold, the antipodal pairing (pdim shifted by half a period, revdim
+     * reversed; -1: none, then the fold is self-paired), the antipodal signs
+     * of (u, v, w), the area factor's parity, and the parity-folded plans
+     * (index 0: sigma_g = +1, 1: sigma_g = -1) */
+    struct {
+        int on, lo, hi, pdim, revdim, sv[3], area;
+        or_plan *dp[2], *fp[2];
+    } fold[3];
+    double coord_shift[3];
+    double *pairbuf, *pairout;
 };
 
 #define FOR_INTERIOR(s, I)                                                        \
@@ -607,6 +622,8 @@ static void scale_grad(or_solver *s, double *g, int d)
 
 static void deriv_along_p(or_solver *s, double *out, double *f, int d, int sf);
 #define deriv_along(s, out, f, d) deriv_along_p((s), (out), (f), (d), 1)
+static int vel_parity(const or_solver *s, int d, int j);
+static int cons_parity(const or_solver *s, int d, int c);
 
 /* metric.jl:94-103 _phys_and_jac: the physical coordinate and mapping
  * Jacobian at padded index ip along d.  A stretched dimension takes its
@@ -615,7 +632,7 @@ static void deriv_along_p(or_solver *s, double *out, double *f, int d, int sf);
 static void phys_and_jac(const or_solver *s, int d, int64_t ip, const double *const *x, const double *const *jac,
                          double *xo, double *mo)
 {
-    const double xi = s->d.origin[d] + (d == 0 ? s->coord_shift0 : 0.0) + (double)(ip - s->H[d]) * s->h[d];
+    const double xi = s->d.origin[d] + s->coord_shift[d] + (double)(ip - s->H[d]) * s->h[d];
     if (!jac || !jac[d]) {
         *xo = xi;
         *mo = 1.0;
@@ -664,7 +681,9 @@ static void init_geometry(or_solver *s, const double *const *x, const double *co
             }
     if (kind == 2 && s->active[1]) {
         /* gcl_cotr!: cot(theta)/r := inv_J D_xi2(A_2) at the interior nodes */
-        deriv_along(s, s->tmp_a, s->area[1], 1);
+        /* with the sign of the theta-momentum pressure flux across a pole fold */
+        deriv_along_p(s, s->tmp_a, s->area[1], 1,
+                      cons_parity(s, 1, s->im2) * vel_parity(s, 1, 1) * (s->fold[1].on ? s->fold[1].area : 1));
         for (int64_t k = 0; k < s->n[2]; k++)
             for (int64_t j = 0; j < s->n[1]; j++)
                 for (int64_t i = 0; i < s->n[0]; i++) {
@@ -730,31 +749,78 @@ or_solver *or_solver_create_geom(const or_desc *d, const double *const x[3], con
         }
     }
     if (d->metric != 0) s->geom = 1;
-    /* rhs.jl:155-205 AxisBC: cylindrical, unstretched r from origin 0, theta
-     * collapsed here (a resolved theta pairs antipodes: not restated); the
-     * half-offset grid h = L / (n - 1/2), r_1 = h / 2 */
-    for (int k = 0; k < 3; k++)
-        for (int sd = 0; sd < 2; sd++)
-            if (d->bc[k][sd] == OR_BC_AXIS && !(k == 0 && sd == 0)) {
-                snprintf(msg, (size_t)msglen, "AxisBC goes at the low end of dimension 1");
-                free(s);
-                return NULL;
-            }
-    if (d->bc[0][0] == OR_BC_AXIS) {
+    /* rhs.jl:143-205: coordinate-singularity folds recognised from the BCs */
+    {
         const char *err = NULL;
-        if (d->metric != 1) err = "AxisBC requires CylindricalMetric";
-        else if (jac && jac[0]) err = "folded dimensions cannot be stretched";
-        else if (fabs(d->origin[0]) >= 1e-14) err = "AxisBC requires origin[1] = 0";
-        else if (d->n_global[1] != 1) err = "a resolved theta with AxisBC (antipodal pairing) is not restated";
-        else if (!s->active[0]) err = "AxisBC needs an active r dimension";
+        for (int k = 0; k < 3 && !err; k++)
+            for (int sd = 0; sd < 2; sd++) {
+                const int bc = d->bc[k][sd];
+                if ((bc == OR_BC_AXIS || bc == OR_BC_ORIGIN) && !(k == 0 && sd == 0))
+                    err = bc == OR_BC_AXIS ? "AxisBC goes at the low end of dimension 1"
+                                           : "OriginBC goes at the low end of dimension 1";
+                if (bc == OR_BC_POLE && k != 1) err = "PoleBC goes on dimension 2 (theta)";
+            }
+        const int axis = d->bc[0][0] == OR_BC_AXIS, orig = d->bc[0][0] == OR_BC_ORIGIN;
+        const int pl = d->bc[1][0] == OR_BC_POLE, ph = d->bc[1][1] == OR_BC_POLE;
+        const double pi = 3.14159265358979323846;
+        if (!err && pl != ph) err = "PoleBC must be applied at both ends of theta";
+        if (!err && axis) {
+            if (d->metric != 1) err = "AxisBC requires CylindricalMetric";
+            else if (jac && jac[0]) err = "folded dimensions cannot be stretched";
+            else if (fabs(d->origin[0]) >= 1e-14) err = "AxisBC requires origin[1] = 0";
+            else if (s->n[1] > 1 && (s->n[1] % 2 || fabs(d->L_domain[1] - 2 * pi) > 1e-10))
+                err = "resolved-theta AxisBC requires an even theta point count over 2 pi";
+        }
+        if (!err && orig) {
+            if (d->metric != 2) err = "OriginBC requires SphericalMetric";
+            else if (jac && jac[0]) err = "folded dimensions cannot be stretched";
+            else if (fabs(d->origin[0]) >= 1e-14) err = "OriginBC requires origin[1] = 0";
+            else if (s->n[2] > 1 && (s->n[2] % 2 || fabs(d->L_domain[2] - 2 * pi) > 1e-10))
+                err = "resolved-phi OriginBC requires an even phi point count over 2 pi";
+            else if (s->n[1] > 1 && fabs(2 * d->origin[1] + d->L_domain[1] - pi) > 1e-10)
+                err = "OriginBC requires a theta range symmetric about pi/2";
+            else if (s->n[1] > 1 && s->n[2] == 1 && s->n[1] % 2)
+                err = "OriginBC with a collapsed phi needs an even theta point count";
+        }
+        if (!err && pl) {
+            if (d->metric != 2) err = "PoleBC requires SphericalMetric";
+            else if (jac && jac[1]) err = "folded dimensions cannot be stretched";
+            else if (!(fabs(d->origin[1]) < 1e-14 && fabs(d->L_domain[1] - pi) <= 1e-10))
+                err = "PoleBC requires the theta domain (0, pi)";
+            else if (s->n[2] > 1 && (s->n[2] % 2 || fabs(d->L_domain[2] - 2 * pi) > 1e-10))
+                err = "resolved-phi PoleBC requires an even phi point count over 2 pi";
+        }
         if (err) {
             snprintf(msg, (size_t)msglen, "%s", err);
             free(s);
             return NULL;
         }
-        s->fold0 = 1;
-        s->h[0] = d->L_domain[0] / ((double)d->n_global[0] - 0.5);
-        s->coord_shift0 = s->h[0] / 2;
+        /* rhs.jl:256-279 foldspec: sigvel and the area factor's parity */
+        if (axis && s->active[0]) {
+            s->fold[0].on = 1; s->fold[0].lo = 1;
+            s->fold[0].pdim = s->active[1] ? 1 : -1; s->fold[0].revdim = -1;
+            s->fold[0].sv[0] = -1; s->fold[0].sv[1] = -1; s->fold[0].sv[2] = 1;
+            s->fold[0].area = -1;                                   /* A_1 = r */
+        }
+        if (orig && s->active[0]) {
+            s->fold[0].on = 1; s->fold[0].lo = 1;
+            s->fold[0].pdim = s->active[2] ? 2 : -1; s->fold[0].revdim = s->active[1] ? 1 : -1;
+            s->fold[0].sv[0] = -1; s->fold[0].sv[1] = 1; s->fold[0].sv[2] = -1;
+            s->fold[0].area = 1;                                    /* A_1 = r^2 sin(theta) */
+        }
+        if (pl && s->active[1]) {
+            s->fold[1].on = 1; s->fold[1].lo = 1; s->fold[1].hi = 1;
+            s->fold[1].pdim = s->active[2] ? 2 : -1; s->fold[1].revdim = -1;
+            s->fold[1].sv[0] = 1; s->fold[1].sv[1] = -1; s->fold[1].sv[2] = -1;
+            s->fold[1].area = -1;                                   /* A_2 = r sin(theta) */
+        }
+        /* rhs.jl:196-205: half-offset grids on folded dimensions */
+        for (int k = 0; k < 3; k++) {
+            if (!s->fold[k].on) continue;
+            s->h[k] = s->fold[k].hi ? d->L_domain[k] / (double)s->n[k]
+                                    : d->L_domain[k] / ((double)s->n[k] - 0.5);
+            s->coord_shift[k] = s->h[k] / 2;
+        }
     }
     /* nscbc.jl:309-316: NSCBC faces need a normal scale factor of one */
     for (int k = 0; k < 3; k++) {
@@ -781,12 +847,16 @@ or_solver *or_solver_create_geom(const or_desc *d, const double *const x[3], con
         s->fplan[k] = or_plan_create(&d->filt, k + 1, s->n, halo, s->periodic[k], s->h[k], msg, msglen);
         if (!s->dplan[k] || !s->fplan[k]) { or_solver_destroy(s); return NULL; }
     }
-    if (s->fold0)   /* rhs.jl:256-261 foldspec: plans for sigma_g = +1, -1 */
+    for (int k = 0; k < 3; k++) {   /* rhs.jl:256-261 foldspec: plans for sigma_g = +1, -1 */
+        if (!s->fold[k].on) continue;
         for (int g = 0; g < 2; g++) {
-            s->dplanF[g] = plan_create_fold(&d->deriv, 1, s->n, halo, 0, s->h[0], g == 0 ? 1 : -1, msg, msglen);
-            s->fplanF[g] = plan_create_fold(&d->filt, 1, s->n, halo, 0, s->h[0], g == 0 ? 1 : -1, msg, msglen);
-            if (!s->dplanF[g] || !s->fplanF[g]) { or_solver_destroy(s); return NULL; }
+            const int sg = g == 0 ? 1 : -1;
+            const int lo = s->fold[k].lo ? sg : 0, hi = s->fold[k].hi ? sg : 0;
+            s->fold[k].dp[g] = plan_create_fold(&d->deriv, k + 1, s->n, halo, 0, s->h[k], lo, hi, msg, msglen);
+            s->fold[k].fp[g] = plan_create_fold(&d->filt, k + 1, s->n, halo, 0, s->h[k], lo, hi, msg, msglen);
+            if (!s->fold[k].dp[g] || !s->fold[k].fp[g]) { or_solver_destroy(s); return NULL; }
         }
+    }
     s->Q = calloc((size_t)s->NP * s->C, sizeof(double));
     s->dQ = calloc((size_t)s->NP * s->C, sizeof(double));
     s->du = calloc((size_t)s->NP * s->C, sizeof(double));
@@ -802,6 +872,7 @@ or_solver *or_solver_create_geom(const or_desc *d, const double *const x[3], con
     s->mu_art = fld(s); s->beta_art = fld(s); s->kappa_art = fld(s);
     s->strain_mag = fld(s); s->sensor = fld(s); s->sensor_sp = fld(s);
     s->tmp_a = fld(s); s->tmp_b = fld(s);
+    s->pairbuf = fld(s); s->pairout = fld(s);
     if (s->geom) init_geometry(s, x, jac);
     return s;
 }
@@ -810,7 +881,9 @@ void or_solver_destroy(or_solver *s)
 {
     if (!s) return;
     for (int k = 0; k < 3; k++) { or_plan_destroy(s->dplan[k]); or_plan_destroy(s->fplan[k]); }
-    for (int g = 0; g < 2; g++) { or_plan_destroy(s->dplanF[g]); or_plan_destroy(s->fplanF[g]); }
+    for (int k = 0; k < 3; k++)
+        for (int g = 0; g < 2; g++) { or_plan_destroy(s->fold[k].dp[g]); or_plan_destroy(s->fold[k].fp[g]); }
+    free(s->pairbuf); free(s->pairout);
     free(s->Q); free(s->dQ); free(s->du);
     free(s->rho); free(s->u); free(s->v); free(s->w); free(s->p); free(s->T); free(s->c); free(s->cp);
     for (int k = 0; k < s->Ns; k++) { free(s->Y[k]); free(s->D_art[k]); }
@@ -1028,26 +1101,119 @@ static void primitives(or_solver *s)
     }
 }
 
-/* folds.jl:79-105 fold_fill! at the low edge of dimension 1: halo layer q
- * mirrors interior layer q with sign sf (half-offset grid) */
-static void fold_fill0(or_solver *s, double *f, int sf)
+/* folds.jl:79-105 fold_fill!: halo layers beyond a folded end of dimension
+ * d mirror the interior layers with sign sf (half-offset: ghost q <-> interior q) */
+static void fold_fill(or_solver *s, double *f, int d, int sf)
 {
-    const int H = s->H[0];
-    for (int64_t k = 0; k < s->pad[2]; k++)
-        for (int64_t j = 0; j < s->pad[1]; j++) {
-            double *row = f + s->pad[0] * (j + s->pad[1] * k);
-            for (int q = 1; q <= H; q++) row[H - q] = (double)sf * row[H + q - 1];
+    const int H = s->H[d];
+    const int64_t n = s->n[d];
+    const int64_t st = d == 0 ? 1 : d == 1 ? s->pad[0] : s->pad[0] * s->pad[1];
+    const int o1 = d == 0 ? 1 : 0, o2 = d == 2 ? 1 : 2;
+    for (int64_t b = 0; b < s->pad[o2]; b++)
+        for (int64_t a = 0; a < s->pad[o1]; a++) {
+            int64_t ii[3] = {0, 0, 0};
+            ii[o1] = a;
+            ii[o2] = b;
+            double *base = f + ii[0] + s->pad[0] * (ii[1] + s->pad[1] * ii[2]);
+            for (int q = 1; q <= H; q++) {
+                if (s->fold[d].lo) base[(H - q) * st] = (double)sf * base[(H + q - 1) * st];
+                if (s->fold[d].hi) base[(H + n + q - 1) * st] = (double)sf * base[(H + n - q) * st];
+            }
         }
 }
 
-/* rhs.jl:452-475 deriv_along! / filt_along!: sf is the field's parity across
- * the fold of dimension d (ignored where there is none); a derivative flips it
- * (sigma_g = -sf), a filter keeps it (folds.jl:271-274) */
+/* folds.jl:118-151 _pair_index: the antipodal partner of interior point
+ * (i, j, k) (pdim shifted by half its extent, revdim reversed) */
+static void pair_ijk(const or_solver *s, int d, const int64_t *ii, int64_t *out)
+{
+    out[0] = ii[0]; out[1] = ii[1]; out[2] = ii[2];
+    const int pd = s->fold[d].pdim, rd = s->fold[d].revdim;
+    if (pd >= 0) {
+        const int64_t half = s->n[pd] / 2;
+        out[pd] = ii[pd] < half ? ii[pd] + half : ii[pd] - half;
+    }
+    if (rd >= 0) out[rd] = s->n[rd] - 1 - ii[rd];
+}
+static int fold_paired(const or_solver *s, int d) { return s->fold[d].pdim >= 0 || s->fold[d].revdim >= 0; }
+static int pair_sd(const or_solver *s, int d) { return s->fold[d].pdim >= 0 ? s->fold[d].pdim : s->fold[d].revdim; }
+
+/* folds.jl:160-196 pair_forward! (one rank): e = (f(x) + sf f(Mx)) / 2 at the
+ * canonical slot x (lower half of the mapping dimension), o = (f(x) - sf
+ * f(Mx)) / 2 at the partner slot */
+static void pair_forward(or_solver *s, int d, double *w, const double *f, int sf)
+{
+    const int sd = pair_sd(s, d);
+    const int64_t half = s->n[sd] / 2;
+    for (int64_t k = 0; k < s->n[2]; k++)
+        for (int64_t j = 0; j < s->n[1]; j++)
+            for (int64_t i = 0; i < s->n[0]; i++) {
+                const int64_t ii[3] = {i, j, k};
+                if (ii[sd] >= half) continue;
+                int64_t mm[3];
+                pair_ijk(s, d, ii, mm);
+                const int64_t I = pidx(s, i, j, k), Mx = pidx(s, mm[0], mm[1], mm[2]);
+                const double a = f[I], b = (double)sf * f[Mx];
+                w[I] = 0.5 * (a + b);
+                w[Mx] = 0.5 * (a - b);
+            }
+}
+
+/* folds.jl:206-246 pair_backward! (one rank): out(x) = Re + Ro,
+ * out(Mx) = sf (Re - Ro) */
+static void pair_backward(or_solver *s, int d, double *out, int sf)
+{
+    const int sd = pair_sd(s, d);
+    const int64_t half = s->n[sd] / 2;
+    for (int64_t k = 0; k < s->n[2]; k++)
+        for (int64_t j = 0; j < s->n[1]; j++)
+            for (int64_t i = 0; i < s->n[0]; i++) {
+                const int64_t ii[3] = {i, j, k};
+                if (ii[sd] >= half) continue;
+                int64_t mm[3];
+                pair_ijk(s, d, ii, mm);
+                const int64_t I = pidx(s, i, j, k), Mx = pidx(s, mm[0], mm[1], mm[2]);
+                const double Re = out[I], Ro = out[Mx];
+                out[I] = Re + Ro;
+                out[Mx] = (double)sf * (Re - Ro);
+            }
+}
+
+/* folds.jl:276-330 fold_apply! (one rank).  Self-paired: mirror fill and one
+ * parity plan.  Paired: the e combo folds with sf and the o combo always as
+ * odd; both plans run and the mapping dimension's upper half takes the odd
+ * result, then the inverse butterfly. */
+static void fold_apply(or_solver *s, double *out, double *f, int d, int sf, int isfilter)
+{
+#define FPLAN(sig) (isfilter ? s->fold[d].fp[(sig) > 0 ? 0 : 1] : s->fold[d].dp[-(sig) > 0 ? 0 : 1])
+    if (!fold_paired(s, d)) {
+        fold_fill(s, f, d, sf);
+        or_apply_along(FPLAN(sf), out, f);
+        return;
+    }
+    double *w = s->pairbuf;
+    pair_forward(s, d, w, f, sf);
+    fold_fill(s, w, d, sf);
+    or_apply_along(FPLAN(sf), out, w);
+    fold_fill(s, w, d, -1);
+    or_apply_along(FPLAN(-1), s->pairout, w);
+    const int sd = pair_sd(s, d);
+    const int64_t half = s->n[sd] / 2;
+    for (int64_t k = 0; k < s->n[2]; k++)
+        for (int64_t j = 0; j < s->n[1]; j++)
+            for (int64_t i = 0; i < s->n[0]; i++) {
+                const int64_t ii[3] = {i, j, k};
+                if (ii[sd] >= half) out[pidx(s, i, j, k)] = s->pairout[pidx(s, i, j, k)];
+            }
+    pair_backward(s, d, out, sf);
+#undef FPLAN
+}
+
+/* rhs.jl:452-475 deriv_along! / filt_along!: sf is the field's antipodal sign
+ * for the fold of dimension d (ignored where there is none) */
 static void deriv_along_p(or_solver *s, double *out, double *f, int d, int sf)
 {
-    if (d == 0 && s->fold0) {
-        fold_fill0(s, f, sf);
-        or_apply_along(s->dplanF[-sf > 0 ? 0 : 1], out, f);
+    if (s->fold[d].on) {
+        fold_apply(s, out, f, d, sf, 0);
         return;
     }
     or_apply_along(s->dplan[d], out, f);
@@ -1056,9 +1222,8 @@ static void deriv_along_p(or_solver *s, double *out, double *f, int d, int sf)
 }
 static void filt_along_p(or_solver *s, double *out, double *f, int d, int sf)
 {
-    if (d == 0 && s->fold0) {
-        fold_fill0(s, f, sf);
-        or_apply_along(s->fplanF[sf > 0 ? 0 : 1], out, f);
+    if (s->fold[d].on) {
+        fold_apply(s, out, f, d, sf, 1);
         return;
     }
     or_apply_along(s->fplan[d], out, f);
@@ -1069,7 +1234,7 @@ static void filt_along_p(or_solver *s, double *out, double *f, int d, int sf)
  * across the fold of dimension d (axis: u_r, u_theta odd) */
 static int vel_parity(const or_solver *s, int d, int j)
 {
-    return (d == 0 && s->fold0 && j < 2) ? -1 : 1;
+    return s->fold[d].on ? s->fold[d].sv[j] : 1;
 }
 static int cons_parity(const or_solver *s, int d, int c)
 {
@@ -1495,7 +1660,7 @@ void or_compute_rhs(or_solver *s, int primitives_current)
             for (int64_t I = 0; I < s->NP; I++) s->tmp_b[I] = Ad[I] * F[I];
             /* equations.jl:62-66 flux_parities: conserved x normal-velocity x area parity */
             deriv_along_p(s, s->tmp_a, s->tmp_b, d,
-                          cons_parity(s, d, c) * vel_parity(s, d, d) * ((d == 0 && s->fold0) ? -1 : 1));
+                          cons_parity(s, d, c) * vel_parity(s, d, d) * (s->fold[d].on ? s->fold[d].area : 1));
 #pragma omp parallel for schedule(static)
             for (int64_t k = 0; k < s->n[2]; k++)
                 for (int64_t j = 0; j < s->n[1]; j++)
@@ -1723,8 +1888,8 @@ void or_solver_set_cfl(or_solver *s, double cfl) { s->d.cfl = cfl; }
 static inline double quad_weight(const or_solver *s, int d, int64_t i)
 {
     if (!(s->active[d] && !s->periodic[d])) return 1.0;
-    if (i == 0) return (d == 0 && s->fold0) ? 1.0 : 0.5;   /* a folded edge takes a full weight */
-    if (i == s->n[d] - 1) return 0.5;
+    if (i == 0) return s->fold[d].lo ? 1.0 : 0.5;   /* a folded edge takes a full weight */
+    if (i == s->n[d] - 1) return s->fold[d].hi ? 1.0 : 0.5;
     return 1.0;
 }
 

@@ -61,7 +61,7 @@ void or_wrap_halos(double *f, const int64_t n_local[3], const int halo[3],
 
 /* --------------------------- solver ------------------------------------- */
 enum { OR_BC_PERIODIC = 0, OR_BC_SLIPWALL = 1, OR_BC_DIRICHLET = 2,
-       OR_BC_NSCBC_OUT = 3, OR_BC_NSCBC_IN = 4, OR_BC_AXIS = 5 };
+       OR_BC_NSCBC_OUT = 3, OR_BC_NSCBC_IN = 4, OR_BC_AXIS = 5, OR_BC_ORIGIN = 6, OR_BC_POLE = 7 };
 
 typedef struct {
     double pinf, sigma, Lref, beta_t;            /* outflow; Lref <= 0: domain length */
