@@ -18,7 +18,8 @@ MAXCL, MAXRHS, MAXM, MAXSP = 8, 16, 8, 8
 OK, ERR_INVALID, ERR_TOO_SMALL, ERR_HIP, ERR_NONFINITE, ERR_NEGATIVE_DENSITY, ERR_PLANCK, \
     ERR_DT_MIN, ERR_DT_COLLAPSE, ERR_NOMEM, ERR_UNSUPPORTED = range(11)
 F64, F32 = 0, 1
-BC_PERIODIC, BC_SLIPWALL, BC_DIRICHLET, BC_NSCBC_OUT, BC_NSCBC_IN, BC_AXIS, BC_ORIGIN, BC_POLE = range(8)
+(BC_PERIODIC, BC_SLIPWALL, BC_DIRICHLET, BC_NSCBC_OUT, BC_NSCBC_IN, BC_AXIS, BC_ORIGIN, BC_POLE, BC_NOSLIP,
+ BC_EXTRAP) = range(10)
 
 
 class SchemeDesc(C.Structure):
@@ -58,7 +59,8 @@ class SolverDesc(C.Structure):
                 ("dt_min_ratio", C.c_double), ("precision", C.c_int), ("retries", C.c_int),
                 ("cfl_backoff", C.c_double), ("savepoint_interval", C.c_int), ("landing_steps", C.c_int),
                 ("eos_kind", C.c_int), ("T_guess", C.c_double), ("nasa9", Nasa9SpeciesDesc * MAXSP),
-                ("metric", C.c_int)]
+                ("metric", C.c_int), ("Twall", (C.c_double * 2) * 3), ("switchable", (C.c_int * 2) * 3),
+                ("bc_after", (C.c_int * 2) * 3)]
 
 
 P = C.c_void_p
@@ -117,6 +119,8 @@ SIGNATURES = {
     "cles_solver_footprint": (C.c_int, [C.POINTER(SolverDesc), C.POINTER(C.c_size_t)]),
     "cles_solver_create_ex": (C.c_int, [P, C.POINTER(SolverDesc), C.POINTER(DP), PP]),
     "cles_solver_create_geom": (C.c_int, [P, C.POINTER(SolverDesc), C.POINTER(DP), C.POINTER(DP), PP]),
+    "cles_solver_switch_bc": (C.c_int, [P, C.c_int, C.c_int]),
+    "cles_solver_bc_switched": (C.c_int, [P, C.c_int, C.c_int, C.POINTER(C.c_int)]),
     "cles_solver_memory": (C.c_int, [P, C.POINTER(C.c_size_t)]),
     "cles_server_create": (C.c_int, [C.c_int, C.c_size_t, C.c_int, PP]),
     "cles_server_destroy": (C.c_int, [P]),

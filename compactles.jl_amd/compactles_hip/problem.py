@@ -189,6 +189,20 @@ class DirichletBC(BoundaryCondition):
         self.fun = fun
 
 
+class NoSlipWallBC(BoundaryCondition):
+    """boundary.jl:34-43: impermeable viscous wall with zero velocity; a
+    finite Twall makes it isothermal (the EOS sets the wall energy)."""
+    code = L.BC_NOSLIP
+
+    def __init__(self, Twall: float = math.nan):
+        self.Twall = float(Twall)
+
+
+class ExtrapolationBC(BoundaryCondition):
+    """boundary.jl:45-52: copy the adjacent interior plane onto the boundary plane."""
+    code = L.BC_EXTRAP
+
+
 class AxisBC(BoundaryCondition):
     """boundary.jl:55-66: regularized cylindrical axis at the low end of r
     (half-offset grid, parity mirror conditions; no state enforcement)."""
@@ -203,6 +217,48 @@ class OriginBC(BoundaryCondition):
 class PoleBC(BoundaryCondition):
     """boundary.jl:82-92: regularized spherical poles, at both ends of theta."""
     code = L.BC_POLE
+
+
+_FOLDS = (AxisBC, OriginBC, PoleBC)
+
+
+class SwitchableBC(BoundaryCondition):
+    """boundary.jl:94-145: runs `before` until switch_(bc), then `after`.
+    Both arms agree on periodicity; fold conditions cannot be wrapped."""
+
+    def __init__(self, before: BoundaryCondition, after: BoundaryCondition):
+        if isinstance(before, PeriodicBC) != isinstance(after, PeriodicBC):
+            raise ValueError("SwitchableBC: both conditions must agree on periodicity")
+        if isinstance(before, _FOLDS) or isinstance(after, _FOLDS):
+            raise ValueError("SwitchableBC cannot wrap a fold condition (AxisBC, OriginBC, PoleBC); "
+                             "setup detects those by type")
+        if isinstance(before, SwitchableBC) or isinstance(after, SwitchableBC):
+            raise ValueError("SwitchableBC cannot wrap a SwitchableBC")
+        self.before, self.after = before, after
+        self.code = before.code
+        self._switched = False
+        self._faces = []          # (solver, dim, side) that run this condition
+
+    def switched(self) -> bool:
+        return self._switched
+
+
+def _is_periodic(bc) -> bool:
+    """boundary.jl:127 isperiodic (a SwitchableBC answers for both arms)."""
+    return isinstance(bc.before if isinstance(bc, SwitchableBC) else bc, PeriodicBC)
+
+
+def switch_(bc: SwitchableBC) -> SwitchableBC:
+    """boundary.jl:131-137 switch!: later steps of every solver running this
+    condition take `after`; repeated calls have no further effect."""
+    bc._switched = True
+    for solver, d, side in bc._faces:
+        L.check(solver.ctx.lib.cles_solver_switch_bc(solver.handle, d, side))
+    return bc
+
+
+def switched(bc: SwitchableBC) -> bool:
+    return bc.switched()
 
 
 @dataclass
@@ -349,7 +405,7 @@ class SolverSpec:
             raise ValueError("domain extents must be positive")
         self.n_species = self.eos.nspecies()
         self.n_cons = self.n_species + 4
-        self.periodic = tuple(isinstance(prob.bcs[d][0], PeriodicBC) or self.n[d] == 1 for d in range(3))
+        self.periodic = tuple(_is_periodic(prob.bcs[d][0]) or self.n[d] == 1 for d in range(3))
         self.stretch = stretch
         for d, st in enumerate(stretch):
             if st is None:
@@ -396,27 +452,42 @@ class SolverSpec:
         for k in range(3):
             for s in range(2):
                 bc = prob.bcs[k][s]
-                if bc.code is None:
+                arms = [bc]
+                if isinstance(bc, SwitchableBC):
+                    # both arms share the face's parameter slots (one per kind)
+                    if type(bc.before) is type(bc.after) and not isinstance(bc.before, (SlipWallBC, PeriodicBC,
+                                                                                           ExtrapolationBC)):
+                        raise ValueError("SwitchableBC on the GPU backend: the two arms must be of different kinds")
+                    arms = [bc.before, bc.after]
+                    d.switchable[k][s] = 1
+                    d.bc_after[k][s] = bc.after.code
+                if arms[0].code is None or arms[-1].code is None:
                     raise ValueError(f"unsupported boundary condition {type(bc).__name__}")
-                d.bc[k][s] = bc.code
-                if isinstance(bc, DirichletBC):
-                    q = self._dirichlet_state(bc, k, s)
-                    for c, v in enumerate(q):
-                        d.bc_state[k][s][c] = v
-                elif isinstance(bc, NSCBCOutflowBC):
-                    f = d.nscbc[k][s]
-                    f.pinf, f.sigma, f.Lref, f.beta_t = bc.pinf, bc.sigma, bc.Lref, bc.beta_t
-                elif isinstance(bc, NSCBCInflowBC):
-                    f = d.nscbc[k][s]
-                    for i in range(3):
-                        f.u[i] = bc.u[i]
-                    f.T = bc.T_ion
-                    if len(bc.Y) != self.n_species:
-                        raise ValueError("NSCBCInflowBC: target composition does not match the EOS")
-                    for i, y in enumerate(bc.Y):
-                        f.Y[i] = y
-                    f.eta_u, f.eta_T, f.eta_t, f.eta_Y, f.Lref = bc.eta_u, bc.eta_T, bc.eta_t, bc.eta_Y, bc.Lref
+                d.bc[k][s] = arms[0].code
+                for arm in arms:
+                    self._face_params(d, arm, k, s)
         return d
+
+    def _face_params(self, d: L.SolverDesc, bc, k: int, s: int):
+        if isinstance(bc, DirichletBC):
+            q = self._dirichlet_state(bc, k, s)
+            for c, v in enumerate(q):
+                d.bc_state[k][s][c] = v
+        elif isinstance(bc, NoSlipWallBC):
+            d.Twall[k][s] = bc.Twall if bc.Twall > 0 else 0.0   # NaN / non-positive: adiabatic
+        elif isinstance(bc, NSCBCOutflowBC):
+            f = d.nscbc[k][s]
+            f.pinf, f.sigma, f.Lref, f.beta_t = bc.pinf, bc.sigma, bc.Lref, bc.beta_t
+        elif isinstance(bc, NSCBCInflowBC):
+            f = d.nscbc[k][s]
+            for i in range(3):
+                f.u[i] = bc.u[i]
+            f.T = bc.T_ion
+            if len(bc.Y) != self.n_species:
+                raise ValueError("NSCBCInflowBC: target composition does not match the EOS")
+            for i, y in enumerate(bc.Y):
+                f.Y[i] = y
+            f.eta_u, f.eta_T, f.eta_t, f.eta_Y, f.Lref = bc.eta_u, bc.eta_T, bc.eta_t, bc.eta_Y, bc.Lref
 
     def xcoord(self, d: int) -> np.ndarray:
         """rhs.jl:320-335: node coordinates along dimension d (0-based here),
@@ -464,6 +535,13 @@ class Solver(SolverSpec):
         else:
             L.check(ctx.lib.cles_solver_create(ctx.handle, C.byref(self.desc), C.byref(h)))
         self.handle = h
+        for k in range(3):
+            for side in range(2):
+                bc = prob.bcs[k][side]
+                if isinstance(bc, SwitchableBC):
+                    bc._faces.append((self, k, side))
+                    if bc.switched():
+                        L.check(ctx.lib.cles_solver_switch_bc(self.handle, k, side))
 
     def save_checkpoint(self, prefix: str, rank: int = 0) -> str:
         """io.jl:15-41 save_checkpoint: writes prefix.rNNNN.ckpt."""

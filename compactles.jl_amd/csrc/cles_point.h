@@ -132,6 +132,9 @@ struct PlaneArgs {
     int64_t n[3];
     int64_t N, count;
     double state[CLES_MAXSP + 4];
+    double Twall;                   // NoSlipWallBC: > 0 isothermal
+    double cvk[CLES_MAXSP];         // IdealMixture cv_k (wall energy)
+    const N9Table *n9;              // Nasa9Mixture: e_k(Twall) (NULL: cv_k Twall)
 };
 
 // One NSCBC face (nscbc.jl:59-154 outflow, :220-306 inflow, constant targets).
@@ -241,6 +244,8 @@ void launch_batch_count(const int *flags, int64_t count, int *out, hipStream_t s
 void launch_ke_batch(const double *Q, int Ns, const BatchGeom &g, double *ke, hipStream_t st);
 void launch_slipwall(const PlaneArgs &a, double *Q, hipStream_t st);
 void launch_dirichlet(const PlaneArgs &a, double *Q, hipStream_t st);
+void launch_noslip(const PlaneArgs &a, double *Q, hipStream_t st);
+void launch_extrapolate(const PlaneArgs &a, double *Q, hipStream_t st);
 template <typename W>
 void launch_integral(const DiagArgs<W> &a, int kind, double *part, int nb, double scale, double *out,
                      hipStream_t st);

@@ -824,6 +824,46 @@ __global__ void __launch_bounds__(TPB) k_dirichlet(PlaneArgs a, double *__restri
     for (int c = 0; c < a.Ns + 4; c++) Q[(int64_t)c * a.N + I] = a.state[c];
 }
 
+// boundary.jl:245-262 NoSlipWallBC enforce!: remove the kinetic energy, zero
+// the momentum, and at an isothermal wall set rho e from the EOS at Twall
+// (boundary.jl:203-221 wall_internal_energy)
+__global__ void __launch_bounds__(TPB) k_noslip(PlaneArgs a, double *__restrict__ Q)
+{
+    const int64_t t = (int64_t)blockIdx.x * TPB + threadIdx.x;
+    if (t >= a.count) return;
+    const int64_t I = plane_index(a, t);
+    const int64_t N = a.N;
+    double rho = 0.0;
+    for (int k = 0; k < a.Ns; k++) rho += Q[(int64_t)k * N + I];
+    double *m1 = Q + (int64_t)a.Ns * N + I, *m2 = m1 + N, *m3 = m2 + N, *E = m3 + N;
+    const double ke = 0.5 * (*m1 * *m1 + *m2 * *m2 + *m3 * *m3) / rho;
+    *E -= ke;
+    *m1 = 0;
+    *m2 = 0;
+    *m3 = 0;
+    if (a.Twall > 0) {
+        double re = 0.0;
+        if (a.n9) {
+            for (int k = 0; k < a.Ns; k++) re += Q[(int64_t)k * N + I] * n9_e(*a.n9, k, a.Twall);
+        } else {
+            for (int k = 0; k < a.Ns; k++) re += Q[(int64_t)k * N + I] * a.cvk[k];
+            re *= a.Twall;
+        }
+        *E = re;
+    }
+}
+
+// boundary.jl:264-276 ExtrapolationBC enforce!: copy the adjacent interior plane
+__global__ void __launch_bounds__(TPB) k_extrapolate(PlaneArgs a, double *__restrict__ Q)
+{
+    const int64_t t = (int64_t)blockIdx.x * TPB + threadIdx.x;
+    if (t >= a.count) return;
+    const int64_t I = plane_index(a, t);
+    const int64_t st = a.dim == 0 ? 1 : a.dim == 1 ? a.n[0] : a.n[0] * a.n[1];
+    const int64_t J = a.side == 0 ? I + st : I - st;
+    for (int c = 0; c < a.Ns + 4; c++) Q[(int64_t)c * a.N + I] = Q[(int64_t)c * a.N + J];
+}
+
 // ---------------------------------------------------------------------------
 // NSCBC LODI corrections of dQ on one face plane (nscbc.jl:96-152 outflow,
 // :236-306 inflow).  The plane's normal and transverse pressure (inflow:
@@ -1333,6 +1373,19 @@ void launch_slipwall(const PlaneArgs &a, double *Q, hipStream_t st)
 void launch_dirichlet(const PlaneArgs &a, double *Q, hipStream_t st)
 {
     hipLaunchKernelGGL(k_dirichlet, dim3(nblocks(a.count)), dim3(TPB), 0, st, a, Q);
+    CHECK_LAUNCH();
+}
+
+void launch_noslip(const PlaneArgs &a, double *Q, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_noslip, dim3(nblocks(a.count)), dim3(TPB), 0, st, a, Q);
+    CHECK_LAUNCH();
+}
+
+void launch_extrapolate(const PlaneArgs &a, double *Q, hipStream_t st)
+{
+    if (a.n[a.dim] < 2) throw Error{CLES_ERR_INVALID, "ExtrapolationBC needs two planes"};
+    hipLaunchKernelGGL(k_extrapolate, dim3(nblocks(a.count)), dim3(TPB), 0, st, a, Q);
     CHECK_LAUNCH();
 }
 

@@ -39,7 +39,9 @@ struct cles_solver {
     double h[3] = {1, 1, 1};
     double Rk[CLES_MAXSP] = {}, cvk[CLES_MAXSP] = {}, cpk[CLES_MAXSP] = {};
     int nact = 0, adim[3] = {0, 0, 0};
-    bool nscbc = false;
+    bool nscbc = false;              // an active face currently runs NSCBC
+    bool nscbc_any = false;          // ... or may after a SwitchableBC switch (scratch held)
+    bool switched[3][2] = {};
     cles::HostPlan dplan[3], fplan[3], fplanQ[3];
     bool own_fplanQ = false;
     // coordinate-singularity folds (rhs.jl:143-279, folds.jl:56-73) per
@@ -798,7 +800,8 @@ struct Impl {
             for (int side = 0; side < 2; side++) {
                 if (!s.active[d]) continue;
                 const int bc = s.d.bc[d][side];
-                if (bc != CLES_BC_SLIPWALL && bc != CLES_BC_DIRICHLET) continue;
+                if (bc != CLES_BC_SLIPWALL && bc != CLES_BC_DIRICHLET && bc != CLES_BC_NOSLIP && bc != CLES_BC_EXTRAP)
+                    continue;
                 PlaneArgs a{};
                 a.dim = d;
                 a.side = side;
@@ -807,11 +810,18 @@ struct Impl {
                 a.N = s.N;
                 a.count = s.N / s.n[d];
                 for (int c = 0; c < s.C; c++) a.state[c] = s.d.bc_state[d][side][c];
+                a.Twall = s.d.Twall[d][side];
+                for (int k = 0; k < s.Ns; k++) a.cvk[k] = s.cvk[k];
+                a.n9 = s.n9;
                 s.timed(PH_BCS, 8.0 * s.C * (double)a.count, [&] {
                     if (bc == CLES_BC_SLIPWALL)
                         launch_slipwall(a, s.Q, s.st());
-                    else
+                    else if (bc == CLES_BC_DIRICHLET)
                         launch_dirichlet(a, s.Q, s.st());
+                    else if (bc == CLES_BC_NOSLIP)
+                        launch_noslip(a, s.Q, s.st());
+                    else
+                        launch_extrapolate(a, s.Q, s.st());
                 });
             }
     }
@@ -947,8 +957,21 @@ void validate_desc(const cles_solver_desc &d)
         if (d.n_global[k] == 1 && lo != CLES_BC_PERIODIC)
             throw cles::Error{CLES_ERR_INVALID, "collapsed dimension must use (PeriodicBC(), PeriodicBC())"};
         for (int sd = 0; sd < 2; sd++)
-            if (d.bc[k][sd] < 0 || d.bc[k][sd] > CLES_BC_POLE)
+            if (d.bc[k][sd] < 0 || d.bc[k][sd] > CLES_BC_EXTRAP)
                 throw cles::Error{CLES_ERR_INVALID, "unknown boundary-condition code"};
+        // boundary.jl:111-121 SwitchableBC: both arms agree on periodicity and
+        // neither is a fold
+        for (int sd = 0; sd < 2; sd++) {
+            if (!d.switchable[k][sd]) continue;
+            const int b0 = d.bc[k][sd], b1 = d.bc_after[k][sd];
+            if (b1 < 0 || b1 > CLES_BC_EXTRAP) throw cles::Error{CLES_ERR_INVALID, "unknown boundary-condition code"};
+            if ((b0 == CLES_BC_PERIODIC) != (b1 == CLES_BC_PERIODIC))
+                throw cles::Error{CLES_ERR_INVALID, "SwitchableBC: both conditions must agree on periodicity"};
+            for (int b : {b0, b1})
+                if (b == CLES_BC_AXIS || b == CLES_BC_ORIGIN || b == CLES_BC_POLE)
+                    throw cles::Error{CLES_ERR_INVALID, "SwitchableBC cannot wrap a fold condition (AxisBC, OriginBC, "
+                                                        "PoleBC); setup detects those by type"};
+        }
     }
     if (d.beta_sensor < 0 || d.beta_sensor > 2)
         throw cles::Error{CLES_ERR_INVALID, "art.beta_sensor must be :strain, :gated_strain or :dilatation"};
@@ -1074,6 +1097,10 @@ int cles_solver_footprint(const cles_solver_desc *desc, size_t *bytes)
         fields += nact * (3 + 1 + (Ns > 1 ? Ns : 0) + C);         // grad_u, grad_T, grad_Y, fluxes
         fields += 3 + (Ns > 1 ? Ns : 0);                          // mu*, beta*, kappa*, D*
         if (d.art_enabled) fields += 3 + (d.beta_sensor == 2 ? 2 : 0);
+        for (int k = 0; k < 3; k++)   // a SwitchableBC arm that may run NSCBC holds the scratch too
+            for (int sd = 0; sd < 2; sd++)
+                nscbc = nscbc || (d.switchable[k][sd] && d.n_global[k] > 1 &&
+                                  (d.bc_after[k][sd] == CLES_BC_NSCBC_IN || d.bc_after[k][sd] == CLES_BC_NSCBC_OUT));
         if (nscbc) fields += 3;
         {   // pairing scratch of a paired fold (folds.jl:276-330): three 8-byte fields
             const bool paired = (d.bc[0][0] == CLES_BC_AXIS && d.n_global[1] > 1) ||
@@ -1180,8 +1207,9 @@ int cles_solver_create_geom(cles_ctx *ctx, const cles_solver_desc *desc, const d
         for (int k = 0; k < 3; k++) {
             const bool unit = kind == CLES_METRIC_CARTESIAN || (kind == CLES_METRIC_CYLINDRICAL && k != 1) ||
                               (kind == CLES_METRIC_SPHERICAL && k == 0);
-            for (int sd = 0; sd < 2; sd++) {
-                const int bc = desc->bc[k][sd];
+            for (int sd = 0; sd < 2 * 2; sd++) {   // both arms of a SwitchableBC (boundary.jl:199-201)
+                if (sd >= 2 && !desc->switchable[k][sd - 2]) continue;
+                const int bc = sd < 2 ? desc->bc[k][sd] : desc->bc_after[k][sd - 2];
                 if ((bc == CLES_BC_NSCBC_IN || bc == CLES_BC_NSCBC_OUT) && desc->n_global[k] > 1 && !unit)
                     throw cles::Error{CLES_ERR_INVALID,
                                       std::string(bc == CLES_BC_NSCBC_IN ? "NSCBCInflowBC" : "NSCBCOutflowBC") +
@@ -1220,9 +1248,12 @@ int cles_solver_create_geom(cles_ctx *ctx, const cles_solver_desc *desc, const d
                           : s->periodic[k] ? desc->L_domain[k] / (double)desc->n_global[k]
                                            : desc->L_domain[k] / (double)(desc->n_global[k] - 1);
                 if (s->active[k]) s->adim[s->nact++] = k;
-                for (int sd = 0; sd < 2; sd++)
-                    if (desc->bc[k][sd] == CLES_BC_NSCBC_IN || desc->bc[k][sd] == CLES_BC_NSCBC_OUT)
-                        s->nscbc = s->active[k] || s->nscbc;
+                for (int sd = 0; sd < 2; sd++) {
+                    auto is_ns = [](int b) { return b == CLES_BC_NSCBC_IN || b == CLES_BC_NSCBC_OUT; };
+                    if (is_ns(desc->bc[k][sd])) s->nscbc = s->active[k] || s->nscbc;
+                    if (desc->switchable[k][sd] && is_ns(desc->bc_after[k][sd]))
+                        s->nscbc_any = s->active[k] || s->nscbc_any;
+                }
                 if (dxdxi && dxdxi[k]) {   // rhs.jl:191-201: computational spacing on [0, 1]
                     s->h[k] = 1.0 / (double)(desc->n_global[k] - 1);
                     s->stretched = true;
@@ -1330,7 +1361,8 @@ int cles_solver_create_geom(cles_ctx *ctx, const cles_solver_desc *desc, const d
                     s->tmp2 = s->wfield();
                 }
             }
-            if (s->nscbc)
+            s->nscbc_any = s->nscbc_any || s->nscbc;
+            if (s->nscbc_any)
                 for (void *&fp : s->nsd) fp = s->wfield();
             if (desc->eos_kind == CLES_EOS_NASA9) {
                 const cles::N9Table t = cles::nasa9_table(*desc);
@@ -1864,6 +1896,33 @@ int cles_solver_get_cfl(cles_solver *s, double *cfl)
     CLES_TRY({
         CLES_REQUIRE(s && cfl, "NULL argument");
         *cfl = s->d.cfl;
+    })
+}
+
+int cles_solver_switch_bc(cles_solver *s, int dim, int side)
+{
+    CLES_TRY({
+        CLES_REQUIRE(s, "NULL argument");
+        CLES_REQUIRE(dim >= 0 && dim < 3 && side >= 0 && side < 2, "dim must be 0..2 and side 0..1");
+        CLES_REQUIRE(s->d.switchable[dim][side], "this face is not a SwitchableBC");
+        if (s->switched[dim][side]) return CLES_OK;   // boundary.jl:131-137: repeated calls do nothing more
+        s->switched[dim][side] = true;
+        s->d.bc[dim][side] = s->d.bc_after[dim][side];
+        s->nscbc = false;
+        for (int k = 0; k < 3; k++)
+            for (int sd = 0; sd < 2; sd++)
+                if (s->d.bc[k][sd] == CLES_BC_NSCBC_IN || s->d.bc[k][sd] == CLES_BC_NSCBC_OUT)
+                    s->nscbc = s->active[k] || s->nscbc;
+        drop_graph(s);   // a captured step holds the face's condition
+    })
+}
+
+int cles_solver_bc_switched(cles_solver *s, int dim, int side, int *out)
+{
+    CLES_TRY({
+        CLES_REQUIRE(s && out, "NULL argument");
+        CLES_REQUIRE(dim >= 0 && dim < 3 && side >= 0 && side < 2, "dim must be 0..2 and side 0..1");
+        *out = s->switched[dim][side] ? 1 : 0;
     })
 }
 

@@ -51,7 +51,9 @@ enum {
     CLES_BC_NSCBC_IN = 4,    /* NSCBCInflowBC, constant targets  nscbc.jl:204-297 */
     CLES_BC_AXIS = 5,        /* AxisBC, low end of r     boundary.jl:55-66, rhs.jl:155-163 */
     CLES_BC_ORIGIN = 6,      /* OriginBC, low end of r   boundary.jl:68-80, rhs.jl:164-173 */
-    CLES_BC_POLE = 7         /* PoleBC, both ends of theta  boundary.jl:82-92, rhs.jl:174-180 */
+    CLES_BC_POLE = 7,        /* PoleBC, both ends of theta  boundary.jl:82-92, rhs.jl:174-180 */
+    CLES_BC_NOSLIP = 8,      /* NoSlipWallBC (Twall)  boundary.jl:34-43,245-262 */
+    CLES_BC_EXTRAP = 9       /* ExtrapolationBC       boundary.jl:45-52,264-276 */
 };
 
 #define CLES_MAXCL 8         /* closure rows per edge */
@@ -207,6 +209,14 @@ typedef struct {
      * (r, theta, phi) with theta the polar angle; velocity components are
      * physical components in the local orthonormal basis. */
     int metric;                            /* CLES_METRIC_* */
+    /* NoSlipWallBC wall temperature per face: > 0 isothermal (the EOS sets the
+     * wall's energy, boundary.jl:203-221), otherwise adiabatic */
+    double Twall[3][2];
+    /* SwitchableBC (boundary.jl:94-145): a face with switchable != 0 runs
+     * bc[d][s] until cles_solver_switch_bc selects bc_after[d][s]; both share
+     * the face's parameters (bc_state, nscbc, Twall) */
+    int switchable[3][2];
+    int bc_after[3][2];
 } cles_solver_desc;
 
 enum { CLES_EOS_IDEAL = 0, CLES_EOS_NASA9 = 1 };
@@ -266,6 +276,12 @@ int cles_solver_create_ex(cles_ctx *ctx, const cles_solver_desc *desc, const dou
 int cles_solver_create_geom(cles_ctx *ctx, const cles_solver_desc *desc, const double *const x[3],
                             const double *const dxdxi[3], cles_solver **solver);
 int cles_solver_destroy(cles_solver *solver);
+/* switch!(bc) (boundary.jl:131-137) on face (dim 0..2, side 0 low / 1 high)
+ * of a SwitchableBC: later steps enforce and correct with bc_after.  A second
+ * call has no further effect; CLES_ERR_INVALID on a face that is not
+ * switchable.  cles_solver_bc_switched reads the state (switched(bc)). */
+int cles_solver_switch_bc(cles_solver *solver, int dim, int side);
+int cles_solver_bc_switched(cles_solver *solver, int dim, int side, int *switched);
 
 /* Conserved state transfer in Julia's padded layout (Q[..., c], column-major,
  * extents padded[3], interior offset halo[3], n_cons components). */

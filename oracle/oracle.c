@@ -822,17 +822,35 @@ or_solver *or_solver_create_geom(const or_desc *d, const double *const x[3], con
             s->coord_shift[k] = s->h[k] / 2;
         }
     }
-    /* nscbc.jl:309-316: NSCBC faces need a normal scale factor of one */
-    for (int k = 0; k < 3; k++) {
-        const int unit = d->metric == 0 || (d->metric == 1 && k != 1) || (d->metric == 2 && k == 0);
-        for (int sd = 0; sd < 2; sd++)
-            if ((d->bc[k][sd] == OR_BC_NSCBC_IN || d->bc[k][sd] == OR_BC_NSCBC_OUT) && s->active[k] && !unit) {
-                snprintf(msg, (size_t)msglen, "%s on dimension %d: the LODI formulation requires a face whose normal "
-                         "metric scale factor is one", d->bc[k][sd] == OR_BC_NSCBC_IN ? "NSCBCInflowBC" :
-                         "NSCBCOutflowBC", k + 1);
+    /* boundary.jl:111-121 SwitchableBC: arms agree on periodicity, no folds */
+    for (int k = 0; k < 3; k++)
+        for (int sd = 0; sd < 2; sd++) {
+            if (!d->switchable[k][sd]) continue;
+            const int b0 = d->bc[k][sd], b1 = d->bc_after[k][sd];
+            const char *err = NULL;
+            if ((b0 == OR_BC_PERIODIC) != (b1 == OR_BC_PERIODIC)) err = "SwitchableBC: both conditions must agree on periodicity";
+            if (b0 == OR_BC_AXIS || b0 == OR_BC_ORIGIN || b0 == OR_BC_POLE || b1 == OR_BC_AXIS || b1 == OR_BC_ORIGIN ||
+                b1 == OR_BC_POLE)
+                err = "SwitchableBC cannot wrap a fold condition";
+            if (err) {
+                snprintf(msg, (size_t)msglen, "%s", err);
                 free(s);
                 return NULL;
             }
+        }
+    /* nscbc.jl:309-316: NSCBC faces (either arm of a SwitchableBC) need a normal scale factor of one */
+    for (int k = 0; k < 3; k++) {
+        const int unit = d->metric == 0 || (d->metric == 1 && k != 1) || (d->metric == 2 && k == 0);
+        for (int sd = 0; sd < 4; sd++) {
+            if (sd >= 2 && !d->switchable[k][sd - 2]) continue;
+            const int bc = sd < 2 ? d->bc[k][sd] : d->bc_after[k][sd - 2];
+            if ((bc == OR_BC_NSCBC_IN || bc == OR_BC_NSCBC_OUT) && s->active[k] && !unit) {
+                snprintf(msg, (size_t)msglen, "%s on dimension %d: the LODI formulation requires a face whose normal "
+                         "metric scale factor is one", bc == OR_BC_NSCBC_IN ? "NSCBCInflowBC" : "NSCBCOutflowBC", k + 1);
+                free(s);
+                return NULL;
+            }
+        }
     }
     /* physics.jl:111-115 */
     for (int k = 0; k < s->Ns; k++) {
@@ -1700,8 +1718,43 @@ void or_apply_bcs(or_solver *s)
                 FOR_PLANE(s, d, side, I) {
                     for (int c = 0; c < s->C; c++) Qc(s, c)[I] = s->d.bc_state[d][side][c];
                 }
+            } else if (bc == OR_BC_NOSLIP) {   /* boundary.jl:245-262 */
+                const double Tw = s->d.Twall[d][side];
+                FOR_PLANE(s, d, side, I) {
+                    double rho = 0.0;
+                    for (int k = 0; k < s->Ns; k++) rho += Qc(s, k)[I];
+                    double m1 = Qc(s, s->im1)[I], m2 = Qc(s, s->im2)[I], m3 = Qc(s, s->im3)[I];
+                    double ke = 0.5 * (m1 * m1 + m2 * m2 + m3 * m3) / rho;
+                    Qc(s, s->ie)[I] -= ke;
+                    Qc(s, s->im1)[I] = 0;
+                    Qc(s, s->im2)[I] = 0;
+                    Qc(s, s->im3)[I] = 0;
+                    if (Tw > 0) {   /* boundary.jl:203-221 wall_internal_energy */
+                        double re = 0.0;
+                        if (s->d.eos_kind == 1) {
+                            for (int k = 0; k < s->Ns; k++) re += Qc(s, k)[I] * n9_e(s, k, Tw);
+                        } else {
+                            for (int k = 0; k < s->Ns; k++) re += Qc(s, k)[I] * s->cvk[k];
+                            re *= Tw;
+                        }
+                        Qc(s, s->ie)[I] = re;
+                    }
+                }
+            } else if (bc == OR_BC_EXTRAP) {   /* boundary.jl:264-276 */
+                const int64_t st = d == 0 ? 1 : d == 1 ? s->pad[0] : s->pad[0] * s->pad[1];
+                FOR_PLANE(s, d, side, I) {
+                    const int64_t J = side == 0 ? I + st : I - st;
+                    for (int c = 0; c < s->C; c++) Qc(s, c)[I] = Qc(s, c)[J];
+                }
             }
         }
+}
+
+int or_solver_switch_bc(or_solver *s, int dim, int side)
+{
+    if (dim < 0 || dim > 2 || side < 0 || side > 1 || !s->d.switchable[dim][side]) return -1;
+    s->d.bc[dim][side] = s->d.bc_after[dim][side];   /* idempotent */
+    return 0;
 }
 
 /* timestep.jl:125-175 max_rate */

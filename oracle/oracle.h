@@ -61,7 +61,8 @@ void or_wrap_halos(double *f, const int64_t n_local[3], const int halo[3],
 
 /* --------------------------- solver ------------------------------------- */
 enum { OR_BC_PERIODIC = 0, OR_BC_SLIPWALL = 1, OR_BC_DIRICHLET = 2,
-       OR_BC_NSCBC_OUT = 3, OR_BC_NSCBC_IN = 4, OR_BC_AXIS = 5, OR_BC_ORIGIN = 6, OR_BC_POLE = 7 };
+       OR_BC_NSCBC_OUT = 3, OR_BC_NSCBC_IN = 4, OR_BC_AXIS = 5, OR_BC_ORIGIN = 6, OR_BC_POLE = 7,
+       OR_BC_NOSLIP = 8, OR_BC_EXTRAP = 9 };
 
 typedef struct {
     double pinf, sigma, Lref, beta_t;            /* outflow; Lref <= 0: domain length */
@@ -108,6 +109,9 @@ typedef struct {
     double T_guess;
     or_nasa9_species nasa9[OR_MAXSP];
     int metric;                       /* 0 CartesianMetric, 1 CylindricalMetric, 2 SphericalMetric */
+    double Twall[3][2];               /* NoSlipWallBC: > 0 isothermal */
+    int switchable[3][2];             /* SwitchableBC: bc until the switch, then bc_after */
+    int bc_after[3][2];
 } or_desc;
 
 /* Step-boundary hooks of run! (src/callbacks.jl): next scheduled instant,
@@ -125,6 +129,8 @@ or_solver *or_solver_create(const or_desc *d, char *msg, int msglen);
 /* With Stretch mappings: jac[d] (NULL: uniform) holds dx/dxi at the
  * n_global[d] nodes of a stretched, non-periodic dimension. */
 or_solver *or_solver_create_ex(const or_desc *d, const double *const jac[3], char *msg, int msglen);
+/* boundary.jl:131-137 switch!: face (dim 0..2, side 0/1) takes bc_after; 0 ok, -1 not switchable */
+int or_solver_switch_bc(or_solver *s, int dim, int side);
 /* x[d]: physical coordinate at the interior nodes of a stretched dimension
  * whose coordinate the metric reads (r; spherical theta); NULL elsewhere. */
 or_solver *or_solver_create_geom(const or_desc *d, const double *const x[3], const double *const jac[3], char *msg,

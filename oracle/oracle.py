@@ -20,7 +20,8 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 
 MAXCL, MAXRHS, MAXM, MAXSP = 8, 16, 8, 8
-BC_PERIODIC, BC_SLIPWALL, BC_DIRICHLET, BC_NSCBC_OUT, BC_NSCBC_IN, BC_AXIS, BC_ORIGIN, BC_POLE = range(8)
+(BC_PERIODIC, BC_SLIPWALL, BC_DIRICHLET, BC_NSCBC_OUT, BC_NSCBC_IN, BC_AXIS, BC_ORIGIN, BC_POLE, BC_NOSLIP,
+ BC_EXTRAP) = range(10)
 
 
 class Scheme(C.Structure):
@@ -60,7 +61,8 @@ class Desc(C.Structure):
                 ("dt_min", C.c_double), ("dt_min_ratio", C.c_double),
                 ("retries", C.c_int), ("cfl_backoff", C.c_double), ("savepoint_interval", C.c_int),
                 ("landing_steps", C.c_int), ("eos_kind", C.c_int), ("T_guess", C.c_double),
-                ("nasa9", Nasa9Sp * MAXSP), ("metric", C.c_int)]
+                ("nasa9", Nasa9Sp * MAXSP), ("metric", C.c_int), ("Twall", (C.c_double * 2) * 3),
+                ("switchable", (C.c_int * 2) * 3), ("bc_after", (C.c_int * 2) * 3)]
 
 
 def _cpu_has_avx2() -> bool:
@@ -115,6 +117,7 @@ def _load():
         "or_solver_set_cfl": (None, [P, C.c_double]),
         "or_diagnostics": (None, [P, D, D]),
         "or_volume_integral": (C.c_double, [P, D]),
+        "or_solver_switch_bc": (C.c_int, [P, C.c_int, C.c_int]),
     }
     for fn, (res, args) in sig.items():
         f = getattr(lib, fn)
@@ -262,6 +265,11 @@ class Solver:
             raise KeyError(name)
         return self._view(p)
 
+    def switch_bc(self, dim: int, side: int):
+        """boundary.jl:131-137 switch! on face (dim, side), 0-based."""
+        if lib().or_solver_switch_bc(self._s, dim, side) != 0:
+            raise ValueError("this face is not a SwitchableBC")
+
     def volume_integral(self, f: np.ndarray) -> float:
         """diagnostics.jl:60-80 volume_integral of a padded field."""
         f = np.asfortranarray(f, dtype=np.float64)
@@ -354,7 +362,7 @@ def make_desc(n_global, L_domain, origin=(0.0, 0.0, 0.0), species=((1.0, 1.4),),
               C_D=0.01, beta_sensor=0, deriv="lele_d1_6", alphaf=0.45, cfl=0.5,
               filter_interval=1, bcs=None, bc_state=None, nscbc=None,
               predict=0.0, max_growth=0.0, dt_min=0.0, dt_min_ratio=1e-8, retries=0, cfl_backoff=0.5,
-              savepoint_interval=25, landing_steps=2, nasa9=None, metric=0) -> Desc:
+              savepoint_interval=25, landing_steps=2, nasa9=None, metric=0, Twall=None, switch=None) -> Desc:
     """nasa9: (T_guess, [(R, [(Tmin, Tmax, (a1..a7), b1), ...]), ...]) selects
     the Nasa9Mixture (species then ignores gamma)."""
     d = Desc()
@@ -396,6 +404,11 @@ def make_desc(n_global, L_domain, origin=(0.0, 0.0, 0.0), species=((1.0, 1.4),),
     d.retries, d.cfl_backoff, d.savepoint_interval, d.landing_steps = retries, cfl_backoff, savepoint_interval, \
         landing_steps
     d.metric = {"cartesian": 0, "cylindrical": 1, "spherical": 2}.get(metric, metric)
+    for (k, sd), T in (Twall or {}).items():           # NoSlipWallBC faces: {(dim, side): Twall}
+        d.Twall[k][sd] = T
+    for (k, sd), after in (switch or {}).items():      # SwitchableBC faces: {(dim, side): bc_after}
+        d.switchable[k][sd] = 1
+        d.bc_after[k][sd] = after
     if nasa9 is not None:
         d.eos_kind = 1
         d.T_guess, table = nasa9

@@ -55,7 +55,8 @@ int main(void) {
   printf("%zu %zu\n", offsetof(cles_run_hooks, after_step), offsetof(cles_run_hooks, rewind));
   printf("%zu %zu %zu %zu\n", offsetof(cles_solver_desc, eos_kind), offsetof(cles_solver_desc, T_guess),
          offsetof(cles_solver_desc, nasa9), sizeof(cles_nasa9_species));
-  printf("%zu\n", offsetof(cles_solver_desc, metric));
+  printf("%zu %zu %zu\n", offsetof(cles_solver_desc, metric), offsetof(cles_solver_desc, Twall),
+         offsetof(cles_solver_desc, bc_after));
   return 0;
 }
 '''
@@ -71,7 +72,8 @@ int main(void) {
             L.SolverDesc.nscbc.offset, L.SolverDesc.retries.offset, L.SolverDesc.cfl_backoff.offset,
             L.SolverDesc.landing_steps.offset, C.sizeof(RunHooks), RunHooks.after_step.offset, RunHooks.rewind.offset,
             L.SolverDesc.eos_kind.offset, L.SolverDesc.T_guess.offset, L.SolverDesc.nasa9.offset,
-            C.sizeof(L.Nasa9SpeciesDesc), L.SolverDesc.metric.offset]
+            C.sizeof(L.Nasa9SpeciesDesc), L.SolverDesc.metric.offset, L.SolverDesc.Twall.offset,
+            L.SolverDesc.bc_after.offset]
     assert got == want
 
 
