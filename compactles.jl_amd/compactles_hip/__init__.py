@@ -6,10 +6,10 @@ from .device import Context, DeviceArray  # noqa: F401
 from .operators import Decomp, DirPlan, apply_along_, field, filter_field_, plan_direction  # noqa: F401
 from .schemes import (BandedClosureRow, BandedCompactScheme, ClosureRow, CompactScheme,  # noqa: F401
                       compact_filter, lele_d1_10, lele_d1_6, pade_d1_4)
-from .problem import (ArtParams, AxisBC, DirichletBC, ExtrapolationBC, NoSlipWallBC, OriginBC, PoleBC, SwitchableBC,
+from .problem import (ArtParams, AxisBC, ConstantBodyForce, DirichletBC, ExtrapolationBC, NoSlipWallBC, OriginBC, PoleBC, SwitchableBC,
                       switch_, switched, IdealMixture, IdealSpecies, NSCBCInflowBC,  # noqa: F401
                       NSCBCOutflowBC, Numerics, PeriodicBC, Prim, Problem, SlipWallBC, Solver, SolverSpec, StepControl,
-                      Stretch, sine_cluster, CartesianMetric, CylindricalMetric, SphericalMetric,
+                      StiffenedGas, Stretch, sine_cluster, CartesianMetric, CylindricalMetric, SphericalMetric,
                       Transport, conserved_from_prim, initialize_, run_, setup, single_species, tanh_blend)
 from .batch import Batch, Instance  # noqa: F401
 from .server import Server  # noqa: F401

@@ -43,7 +43,8 @@ class Nasa9SpeciesDesc(C.Structure):
     _fields_ = [("n_intervals", C.c_int), ("iv", Nasa9IntervalDesc * 3)]
 
 
-EOS_IDEAL, EOS_NASA9 = 0, 1
+EOS_IDEAL, EOS_NASA9, EOS_STIFFENED = 0, 1, 2
+MAX_FORCES = 4
 
 
 class SolverDesc(C.Structure):
@@ -60,7 +61,8 @@ class SolverDesc(C.Structure):
                 ("cfl_backoff", C.c_double), ("savepoint_interval", C.c_int), ("landing_steps", C.c_int),
                 ("eos_kind", C.c_int), ("T_guess", C.c_double), ("nasa9", Nasa9SpeciesDesc * MAXSP),
                 ("metric", C.c_int), ("Twall", (C.c_double * 2) * 3), ("switchable", (C.c_int * 2) * 3),
-                ("bc_after", (C.c_int * 2) * 3)]
+                ("bc_after", (C.c_int * 2) * 3), ("n_body_forces", C.c_int), ("body_force", (C.c_double * 3) * 4),
+                ("p_inf", C.c_double)]
 
 
 P = C.c_void_p

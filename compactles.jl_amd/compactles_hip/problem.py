@@ -143,6 +143,16 @@ def conserved_from_prim(eos, pr: Prim):
     from .nasa9 import Nasa9Mixture, nasa9_conserved
     if isinstance(eos, Nasa9Mixture):
         return nasa9_conserved(eos, pr)
+    if isinstance(eos, StiffenedGas):
+        # problem.jl:92-102: p + p_inf = rho R T_ion; rho e = rho cv T_ion + p_inf
+        if len(pr.Y) != 1:
+            raise ValueError(f"StiffenedGas is single-component; Prim carries {len(pr.Y)} mass fractions")
+        R = eos.gas_constant()
+        rho = (pr.p + eos.p_inf) / (R * pr.T_ion) if _isnan(pr.rho) else pr.rho
+        T = (pr.p + eos.p_inf) / (rho * R) if _isnan(pr.T_ion) else pr.T_ion
+        u1, u2, u3 = pr.u
+        ke = 0.5 * (u1 * u1 + u2 * u2 + u3 * u3)
+        return [rho, rho * u1, rho * u2, rho * u3, rho * eos.cv * T + eos.p_inf + rho * ke]
     N = eos.nspecies()
     if len(pr.Y) != N:
         raise ValueError(f"Prim carries {len(pr.Y)} mass fractions; EOS has {N} species")
@@ -285,6 +295,40 @@ class NSCBCInflowBC(BoundaryCondition):
     code = L.BC_NSCBC_IN
 
 
+class StiffenedGas:
+    """physics.jl:180-205: single-component p = (gamma - 1) rho e - gamma p_inf
+    with p + p_inf = rho R T_ion, R = (gamma - 1) cv."""
+
+    def __init__(self, gamma=4.4, p_inf=6.0e8, cv=1816.0, name="liquid"):
+        self.gamma, self.p_inf, self.cv, self.name = float(gamma), float(p_inf), float(cv), str(name)
+
+    def nspecies(self):
+        return 1
+
+    def gas_constant(self):
+        return (self.gamma - 1) * self.cv
+
+    @property
+    def Rk(self):
+        return [self.gas_constant()]
+
+    def species_names(self):
+        return [self.name]
+
+
+# --------------------------------------------------------------------------
+# Sources (sources.jl): explicit terms added to the assembled RHS
+# --------------------------------------------------------------------------
+class ConstantBodyForce:
+    """sources.jl:4-30: uniform acceleration; adds rho g to momentum and
+    (rho u) . g to total energy."""
+
+    def __init__(self, acceleration=(0.0, -9.81, 0.0)):
+        if len(acceleration) != 3:
+            raise ValueError("ConstantBodyForce: three acceleration components")
+        self.acceleration = tuple(float(a) for a in acceleration)
+
+
 # --------------------------------------------------------------------------
 # Metrics (metric.jl:20-45): coordinates (x, y, z), (r, theta, z), (r, theta,
 # phi) with theta the polar angle; velocities are physical components.
@@ -358,6 +402,9 @@ def fill_common_desc(d: L.SolverDesc, prob: Problem, num: Numerics):
     d.n_species = eos.nspecies()
     if isinstance(eos, Nasa9Mixture):
         eos.to_desc(d)
+    elif isinstance(eos, StiffenedGas):
+        d.eos_kind = L.EOS_STIFFENED
+        d.R[0], d.gamma[0], d.p_inf = eos.gas_constant(), eos.gamma, eos.p_inf
     else:
         d.eos_kind = L.EOS_IDEAL
         for k, sp in enumerate(eos.sp):
@@ -383,6 +430,16 @@ def fill_common_desc(d: L.SolverDesc, prob: Problem, num: Numerics):
         raise ValueError("precision must be 'fp64' or 'fp32'")
     d.precision = L.F64 if num.precision == "fp64" else L.F32
     d.metric = metric_code(prob.metric)
+    srcs = tuple(prob.sources)
+    for src in srcs:
+        if not isinstance(src, ConstantBodyForce):
+            raise ValueError(f"source {type(src).__name__} is not available on the GPU backend (ConstantBodyForce is)")
+    if len(srcs) > L.MAX_FORCES:
+        raise ValueError(f"at most {L.MAX_FORCES} sources")
+    d.n_body_forces = len(srcs)
+    for i, src in enumerate(srcs):
+        for k in range(3):
+            d.body_force[i][k] = src.acceleration[k]
 
 
 class SolverSpec:
@@ -394,8 +451,6 @@ class SolverSpec:
         stretch = tuple(num.stretch) if num.stretch is not None else (None, None, None)
         if len(stretch) != 3:
             raise ValueError("Numerics.stretch: one entry per dimension")
-        if prob.sources:
-            raise ValueError("explicit sources are not available on the GPU backend")
         self.prob, self.num = prob, num
         self.eos = prob.eos
         self.n = tuple(int(x) for x in num.n_global)

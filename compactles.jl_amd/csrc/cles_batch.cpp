@@ -249,6 +249,10 @@ struct BImpl {
         a.Ns = b.Ns;
         for (int k = 0; k < b.Ns; k++) { a.Rk[k] = b.Rk[k]; a.cvk[k] = b.cvk[k]; a.Y[k] = f(b.Y[k]); }
         a.n9 = b.n9;
+        a.sg = b.d.eos_kind == CLES_EOS_STIFFENED;
+        a.sg_gamma = b.d.gamma[0];
+        a.sg_pinf = b.d.p_inf;
+        a.sg_cv = b.cvk[0];
         a.rho = f(b.rho); a.u = f(b.u); a.v = f(b.v); a.w = f(b.w);
         a.p = f(b.p); a.T = f(b.T); a.c = f(b.c); a.cp = f(b.cp);
         a.esens = f(b.esens);
@@ -488,6 +492,8 @@ int cles_batch_create(cles_ctx *ctx, const cles_solver_desc *desc, int64_t count
         cles::validate_eos(d);
         if (d.metric != CLES_METRIC_CARTESIAN)
             throw cles::Error{CLES_ERR_UNSUPPORTED, "batched instances take the Cartesian metric"};
+        if (d.n_body_forces != 0)
+            throw cles::Error{CLES_ERR_UNSUPPORTED, "batched instances take no sources"};
         if (d.retries != 0)
             throw cles::Error{CLES_ERR_UNSUPPORTED, "StepControl retries are not available for batched instances "
                                                     "(each instance stops at its own failure)"};

@@ -28,6 +28,8 @@ struct PrimArgs {
     int Ns;
     double Rk[CLES_MAXSP], cvk[CLES_MAXSP];
     const N9Table *n9;   // Nasa9Mixture (NULL: IdealMixture)
+    int sg;              // StiffenedGas (one species): gamma, p_inf, cv
+    double sg_gamma, sg_pinf, sg_cv;
     W *rho, *u, *v, *w, *p, *T, *c, *cp;
     W *Y[CLES_MAXSP];
     W *esens;   // optional: internal energy of the kappa* sensor (artificial.jl:319-326)
@@ -111,6 +113,9 @@ void launch_pair_backward(const T *re, const T *ro, T *dst, const PairMap &pm, d
                           hipStream_t st);
 
 template <typename W> void launch_metric_grad(const CurvArgs<W> &a, hipStream_t st);
+// sources.jl:44-65 ConstantBodyForce: dQ_m += rho g, dQ_E += (rho u) . g
+template <typename W>
+void launch_body_force(const double *Q, W *dQ, int Ns, int64_t N, const double (&g)[3], hipStream_t st);
 template <typename W> void launch_metric_sources(const CurvArgs<W> &a, hipStream_t st);
 
 template <typename W>
@@ -135,6 +140,8 @@ struct PlaneArgs {
     double Twall;                   // NoSlipWallBC: > 0 isothermal
     double cvk[CLES_MAXSP];         // IdealMixture cv_k (wall energy)
     const N9Table *n9;              // Nasa9Mixture: e_k(Twall) (NULL: cv_k Twall)
+    int sg;                         // StiffenedGas: rho cv Twall + p_inf (cvk[0] = cv)
+    double sg_pinf;
 };
 
 // One NSCBC face (nscbc.jl:59-154 outflow, :220-306 inflow, constant targets).
@@ -152,6 +159,7 @@ struct NscbcArgs {
     double ut[3], Tt, Yt[CLES_MAXSP], eta_u, eta_T, eta_t, eta_Y;
     double Rk[CLES_MAXSP], cvk[CLES_MAXSP];
     const N9Table *n9;              // Nasa9Mixture: cv_k(T) in the inflow's eos_dphi_dY (NULL: cvk)
+    double sg_phi;                  // StiffenedGas: phi = 1 / (gamma - 1), dphi/dY = 0 (0: not one)
     Metric1D met;                   // inv_h of the derivatives (stretched or curvilinear grids)
     W *dQ;
 };

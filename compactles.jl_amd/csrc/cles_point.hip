@@ -101,6 +101,39 @@ __global__ void __launch_bounds__(TPB) k_primitives(PrimArgs<W> a, int64_t N)
 #pragma unroll
     for (int sp = 0; sp < CLES_MAXSP; sp++)
         if (sp < Ns) rho += a.Q[(int64_t)sp * N + I];
+    if (a.sg) {
+        // physics.jl:216-250 _primitives! (StiffenedGas): p = (gamma - 1) rho e - gamma p_inf
+        const double g = a.sg_gamma, pinf = a.sg_pinf, cv = a.sg_cv, R = (g - 1) * cv;
+        if (a.esens) {
+            const double r = d_max(rho, 1e-300);
+            const double m1 = a.Q[(int64_t)Ns * N + I], m2 = a.Q[(int64_t)(Ns + 1) * N + I],
+                         m3 = a.Q[(int64_t)(Ns + 2) * N + I];
+            const double ke = 0.5 * (m1 * m1 + m2 * m2 + m3 * m3) / r;
+            a.esens[I] = (W)((a.Q[(int64_t)(Ns + 3) * N + I] - ke) / r);
+        }
+        a.Y[0][I] = W(1);
+        a.cp[I] = (W)(g * cv);
+        if (rho > 0) {
+            const double ri = 1 / rho;
+            const double u = a.Q[(int64_t)Ns * N + I] * ri;
+            const double v = a.Q[(int64_t)(Ns + 1) * N + I] * ri;
+            const double w = a.Q[(int64_t)(Ns + 2) * N + I] * ri;
+            const double e = a.Q[(int64_t)(Ns + 3) * N + I] * ri - 0.5 * (u * u + v * v + w * w);
+            const double p = (g - 1) * rho * e - g * pinf;
+            const double cc = g * (p + pinf) * ri;
+            a.rho[I] = (W)rho;
+            a.u[I] = (W)u;
+            a.v[I] = (W)v;
+            a.w[I] = (W)w;
+            a.p[I] = (W)p;
+            a.T[I] = (W)d_max((p + pinf) / (rho * R), 1e-300);
+            a.c[I] = (W)sqrt(cc > 0.0 ? cc : 0.0);
+        } else {
+            a.rho[I] = 1; a.u[I] = 0; a.v[I] = 0; a.w[I] = 0;
+            a.p[I] = 1; a.T[I] = 1; a.c[I] = 1;
+        }
+        return;
+    }
     if (a.n9 && rho > 0) {
         // physics.jl:515-552 _primitives! (Nasa9Mixture)
         const N9Table &t = *a.n9;
@@ -545,6 +578,22 @@ __global__ void __launch_bounds__(TPB) k_metric_scale(MetricFields<W> fs, Metric
     }
 }
 
+template <typename W>
+__global__ void __launch_bounds__(TPB) k_body_force(const double *__restrict__ Q, W *__restrict__ dQ, int Ns,
+                                                   int64_t N, double g1, double g2, double g3)
+{
+    const int64_t I = (int64_t)blockIdx.x * TPB + threadIdx.x;
+    if (I >= N) return;
+    double rho = 0.0;
+    for (int sp = 0; sp < Ns; sp++) rho += Q[(int64_t)sp * N + I];
+    const double m1 = Q[(int64_t)Ns * N + I], m2 = Q[(int64_t)(Ns + 1) * N + I], m3 = Q[(int64_t)(Ns + 2) * N + I];
+    W *d = dQ + (int64_t)Ns * N + I;
+    d[0] = (W)((double)d[0] + rho * g1);
+    d[N] = (W)((double)d[N] + rho * g2);
+    d[2 * N] = (W)((double)d[2 * N] + rho * g3);
+    d[3 * N] = (W)((double)d[3 * N] + (m1 * g1 + m2 * g2 + m3 * g3));
+}
+
 // folds.jl:118-151 _pair_index on one rank
 __device__ __forceinline__ int64_t pair_partner(const PairMap &pm, int64_t (&ii)[3])
 {
@@ -843,7 +892,9 @@ __global__ void __launch_bounds__(TPB) k_noslip(PlaneArgs a, double *__restrict_
     *m3 = 0;
     if (a.Twall > 0) {
         double re = 0.0;
-        if (a.n9) {
+        if (a.sg) {   // boundary.jl:213-214
+            re = Q[I] * a.cvk[0] * a.Twall + a.sg_pinf;
+        } else if (a.n9) {
             for (int k = 0; k < a.Ns; k++) re += Q[(int64_t)k * N + I] * n9_e(*a.n9, k, a.Twall);
         } else {
             for (int k = 0; k < a.Ns; k++) re += Q[(int64_t)k * N + I] * a.cvk[k];
@@ -897,9 +948,9 @@ __global__ void __launch_bounds__(TPB) k_nscbc(NscbcArgs<W> a)
         metric_hs(a.met, idx, hs);
         for (int k = 0; k < 3; k++) ih[k] = 1 / hs[k];
     }
-    // physics.jl:142-145 eos_phi for an ideal mixture
+    // physics.jl:142-145 eos_phi for an ideal mixture; :211 for a stiffened gas
     const double Rm = p / (rho * Tp);
-    const double phi = cpm / Rm - 1;
+    const double phi = a.sg_phi > 0 ? a.sg_phi : cpm / Rm - 1;
     const double ke = 0.5 * (uv[0] * uv[0] + uv[1] * uv[1] + uv[2] * uv[2]);
     const double dpn = ih[d] * (double)a.dpn[I];
     const double dun = a.gu[d * 3 + d][I];
@@ -962,7 +1013,7 @@ __global__ void __launch_bounds__(TPB) k_nscbc(NscbcArgs<W> a)
         *q = (W)((double)*q - (Yk * dd1 + rho * dLs));
         // physics.jl:148-152 eos_dphi_dY; a Nasa9Mixture takes cv_k(T) (physics.jl:504-510)
         const double cvk = a.n9 ? n9_cp(*a.n9, k, Tp) - a.Rk[k] : a.cvk[k];
-        sphiY += (cvk * Rm - a.Rk[k] * cvm) / (Rm * Rm) * dLs;
+        if (!(a.sg_phi > 0)) sphiY += (cvk * Rm - a.Rk[k] * cvm) / (Rm * Rm) * dLs;   // :212: 0 for a stiffened gas
     }
 #pragma unroll
     for (int m = 0; m < 3; m++) {
@@ -1496,6 +1547,15 @@ void launch_pair_backward(const T *re, const T *ro, T *dst, const PairMap &pm, d
     hipLaunchKernelGGL((k_pair_backward<T>), dim3(nblocks(N)), dim3(TPB), 0, st, re, ro, dst, pm, sf, epi, gate, N);
     CHECK_LAUNCH();
 }
+template <typename W>
+void launch_body_force(const double *Q, W *dQ, int Ns, int64_t N, const double (&g)[3], hipStream_t st)
+{
+    if (N <= 0) return;
+    hipLaunchKernelGGL((k_body_force<W>), dim3(nblocks(N)), dim3(TPB), 0, st, Q, dQ, Ns, N, g[0], g[1], g[2]);
+    CHECK_LAUNCH();
+}
+template void launch_body_force<double>(const double *, double *, int, int64_t, const double (&)[3], hipStream_t);
+template void launch_body_force<float>(const double *, float *, int, int64_t, const double (&)[3], hipStream_t);
 template void launch_pair_forward<double>(const double *, double *, const PairMap &, double, hipStream_t);
 template void launch_pair_forward<float>(const float *, float *, const PairMap &, double, hipStream_t);
 template void launch_pair_backward<double>(const double *, const double *, double *, const PairMap &, double, int,

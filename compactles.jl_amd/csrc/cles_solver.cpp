@@ -381,6 +381,10 @@ struct Impl {
         a.Ns = s.Ns;
         a.n9 = s.n9;
         for (int k = 0; k < s.Ns; k++) { a.Rk[k] = s.Rk[k]; a.cvk[k] = s.cvk[k]; a.Y[k] = f(s.Y[k]); }
+        a.sg = s.d.eos_kind == CLES_EOS_STIFFENED;
+        a.sg_gamma = s.d.gamma[0];
+        a.sg_pinf = s.d.p_inf;
+        a.sg_cv = s.cvk[0];
         a.rho = f(s.rho); a.u = f(s.u); a.v = f(s.v); a.w = f(s.w);
         a.p = f(s.p); a.T = f(s.T); a.c = f(s.c); a.cp = f(s.cp);
         a.esens = f(s.esens);
@@ -714,7 +718,7 @@ struct Impl {
     // timestep.jl:72-99 step!
     void step(double dt, bool prepared) const
     {
-        const bool fuse = !s.nscbc && !s.geom();
+        const bool fuse = !s.nscbc && !s.geom() && s.d.n_body_forces == 0;
         for (int stg = 0; stg < 5; stg++) {
             s.tstage = s.t + RKC[stg] * dt;
             const bool first_prepared = prepared && stg == 0;
@@ -722,11 +726,22 @@ struct Impl {
             rhs(first_prepared, fuse, RKA[stg], RKB[stg], dt);
             if (!fuse) {
                 nscbc_corrections();
+                sources();
                 lsrk_all(RKA[stg], RKB[stg], dt);
             }
         }
         s.tstage = s.t + dt;
         apply_bcs();
+    }
+
+    // rhs.jl:692 add_sources!: the body forces in order, on the assembled RHS
+    void sources() const
+    {
+        for (int i = 0; i < s.d.n_body_forces; i++) {
+            const double(&g)[3] = s.d.body_force[i];
+            s.timed(PH_OTHER, 8.0 * (s.C + s.Ns) * (double)s.N,
+                    [&] { launch_body_force<W>(s.Q, f(s.dQ), s.Ns, s.N, g, s.st()); });
+        }
     }
 
     // rhs.jl:688-690: boundary corrections after the flux divergence, faces in
@@ -785,6 +800,7 @@ struct Impl {
                 a.eta_t = nd.eta_t;
                 a.eta_Y = nd.eta_Y;
                 a.n9 = s.n9;
+                a.sg_phi = s.d.eos_kind == CLES_EOS_STIFFENED ? 1 / (s.d.gamma[0] - 1) : 0.0;
                 a.met = s.met;
                 a.dQ = f(s.dQ);
                 const double w = sizeof(W);
@@ -813,6 +829,8 @@ struct Impl {
                 a.Twall = s.d.Twall[d][side];
                 for (int k = 0; k < s.Ns; k++) a.cvk[k] = s.cvk[k];
                 a.n9 = s.n9;
+                a.sg = s.d.eos_kind == CLES_EOS_STIFFENED;
+                a.sg_pinf = s.d.p_inf;
                 s.timed(PH_BCS, 8.0 * s.C * (double)a.count, [&] {
                     if (bc == CLES_BC_SLIPWALL)
                         launch_slipwall(a, s.Q, s.st());
@@ -912,8 +930,13 @@ namespace cles {
 
 void validate_eos(const cles_solver_desc &d)
 {
-    if (d.eos_kind != CLES_EOS_IDEAL && d.eos_kind != CLES_EOS_NASA9)
-        throw cles::Error{CLES_ERR_INVALID, "eos_kind must be CLES_EOS_IDEAL or CLES_EOS_NASA9"};
+    if (d.eos_kind != CLES_EOS_IDEAL && d.eos_kind != CLES_EOS_NASA9 && d.eos_kind != CLES_EOS_STIFFENED)
+        throw cles::Error{CLES_ERR_INVALID, "eos_kind must be CLES_EOS_IDEAL, CLES_EOS_NASA9 or CLES_EOS_STIFFENED"};
+    if (d.eos_kind == CLES_EOS_STIFFENED) {   // physics.jl:198-205
+        if (d.n_species != 1) throw cles::Error{CLES_ERR_INVALID, "StiffenedGas is single-component"};
+        if (!(d.gamma[0] > 1) || !(d.R[0] > 0) || !std::isfinite(d.p_inf))
+            throw cles::Error{CLES_ERR_INVALID, "StiffenedGas needs gamma > 1, cv > 0 and a finite p_inf"};
+    }
     if (d.eos_kind == CLES_EOS_NASA9) {   // physics.jl:312-338 _validate_nasa9_intervals
         if (!(d.T_guess > 0)) throw cles::Error{CLES_ERR_INVALID, "Nasa9Mixture: T_guess must be positive"};
         for (int k = 0; k < d.n_species; k++) {
@@ -984,6 +1007,8 @@ void validate_desc(const cles_solver_desc &d)
         throw cles::Error{CLES_ERR_INVALID, "StepControl: cfl_backoff must be positive"};
     if (d.savepoint_interval < 0) throw cles::Error{CLES_ERR_INVALID, "StepControl: savepoint_interval must be >= 0"};
     cles::validate_eos(d);
+    if (d.n_body_forces < 0 || d.n_body_forces > CLES_MAX_FORCES)
+        throw cles::Error{CLES_ERR_INVALID, "n_body_forces must be 0..4"};
     if (d.metric < CLES_METRIC_CARTESIAN || d.metric > CLES_METRIC_SPHERICAL)
         throw cles::Error{CLES_ERR_INVALID, "metric must be CLES_METRIC_CARTESIAN, _CYLINDRICAL or _SPHERICAL"};
     if (d.landing_steps < 0)
@@ -1570,6 +1595,7 @@ int cles_compute_rhs(cles_solver *s, int primitives_current)
         dispatch(s, [&](auto &im) {
             im.rhs(primitives_current != 0, false, 0, 0, 0);
             im.nscbc_corrections();
+            im.sources();
         });
         CLES_HIP(hipStreamSynchronize(s->st()));
     })

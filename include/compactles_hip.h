@@ -200,8 +200,10 @@ typedef struct {
     double cfl_backoff;                    /* CFL multiplier per rollback */
     int savepoint_interval;                /* steps between savepoints */
     int landing_steps;                     /* >= 1; 0 selects the default 2 */
-    /* EOS: CLES_EOS_IDEAL IdealMixture (R, gamma; physics.jl:104-126) or
-     * CLES_EOS_NASA9 Nasa9Mixture (R, nasa9, T_guess; physics.jl:402-571) */
+    /* EOS: CLES_EOS_IDEAL IdealMixture (R, gamma; physics.jl:104-126),
+     * CLES_EOS_NASA9 Nasa9Mixture (R, nasa9, T_guess; physics.jl:402-571) or
+     * CLES_EOS_STIFFENED StiffenedGas (one species: gamma[0], R[0] = (gamma -
+     * 1) cv, p_inf; physics.jl:160-250) */
     int eos_kind;
     double T_guess;
     cles_nasa9_species nasa9[CLES_MAXSP];
@@ -217,9 +219,16 @@ typedef struct {
      * the face's parameters (bc_state, nscbc, Twall) */
     int switchable[3][2];
     int bc_after[3][2];
+    /* Problem.sources (sources.jl): ConstantBodyForce accelerations, applied
+     * in order after the boundary corrections (rhs.jl:692) */
+    int n_body_forces;                     /* 0..CLES_MAX_FORCES */
+    double body_force[4][3];
+    double p_inf;                          /* StiffenedGas cohesive pressure */
 } cles_solver_desc;
 
-enum { CLES_EOS_IDEAL = 0, CLES_EOS_NASA9 = 1 };
+#define CLES_MAX_FORCES 4
+
+enum { CLES_EOS_IDEAL = 0, CLES_EOS_NASA9 = 1, CLES_EOS_STIFFENED = 2 };
 enum { CLES_METRIC_CARTESIAN = 0, CLES_METRIC_CYLINDRICAL = 1, CLES_METRIC_SPHERICAL = 2 };
 
 /* save_checkpoint / load_checkpoint! (src/io.jl:6-75): `prefix`.rNNNN.ckpt

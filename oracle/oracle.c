@@ -1088,7 +1088,27 @@ static void primitives(or_solver *s)
     for (int64_t I = 0; I < s->NP; I++) {
         double rho = 0.0;
         for (int sp = 0; sp < Ns; sp++) rho += Qc(s, sp)[I];
-        if (rho > 0 && s->d.eos_kind == 1) {
+        if (s->d.eos_kind == 2) {
+            /* physics.jl:216-250 _primitives! (StiffenedGas) */
+            const double g = s->d.gamma[0], pinf = s->d.p_inf, cv = s->cvk[0], R = (g - 1) * cv;
+            s->Y[0][I] = 1.0;
+            s->cp[I] = g * cv;
+            if (rho > 0) {
+                double ri = 1 / rho;
+                double u = Qc(s, s->im1)[I] * ri;
+                double v = Qc(s, s->im2)[I] * ri;
+                double w = Qc(s, s->im3)[I] * ri;
+                double e = Qc(s, s->ie)[I] * ri - 0.5 * (u * u + v * v + w * w);
+                double p = (g - 1) * rho * e - g * pinf;
+                s->rho[I] = rho; s->u[I] = u; s->v[I] = v; s->w[I] = w;
+                s->p[I] = p;
+                s->T[I] = jmax((p + pinf) / (rho * R), 1e-300);
+                s->c[I] = sqrt(jmax(g * (p + pinf) * ri, 0.0));
+            } else {
+                s->rho[I] = 1; s->u[I] = 0; s->v[I] = 0; s->w[I] = 0;
+                s->p[I] = 1; s->T[I] = 1; s->c[I] = 1;
+            }
+        } else if (rho > 0 && s->d.eos_kind == 1) {
             primitives_n9(s, I, rho);
         } else if (rho > 0) {
             double ri = 1 / rho;
@@ -1575,7 +1595,7 @@ static void nscbc_outflow(or_solver *s, int d, int side)
         double dd1 = dL / (2 * c * c);
         double dd2 = dL / 2;
         double dd3 = sgn * dL / (2 * rho * c);
-        double phi = eos_phi(rho, p, s->T[I], s->cp[I]);
+        double phi = s->d.eos_kind == 2 ? 1 / (s->d.gamma[0] - 1) : eos_phi(rho, p, s->T[I], s->cp[I]);
         double u1 = s->u[I], u2 = s->v[I], u3 = s->w[I];
         double ke = 0.5 * (u1 * u1 + u2 * u2 + u3 * u3);
         for (int k = 0; k < s->Ns; k++) dQc(s, k)[I] -= s->Y[k][I] * dd1;
@@ -1621,7 +1641,7 @@ static void nscbc_inflow(or_solver *s, int d, int side)
         double dd4 = bc->eta_t * K * (vel[t1][I] - bc->u[t1]) - un * s->gu[d * 3 + t1][I];
         double dd5 = bc->eta_t * K * (vel[t2][I] - bc->u[t2]) - un * s->gu[d * 3 + t2][I];
         double cpm = s->cp[I];
-        double phi = eos_phi(rho, p, Tp, cpm);
+        double phi = s->d.eos_kind == 2 ? 1 / (s->d.gamma[0] - 1) : eos_phi(rho, p, Tp, cpm);
         double u1 = s->u[I], u2 = s->v[I], u3 = s->w[I];
         double ke = 0.5 * (u1 * u1 + u2 * u2 + u3 * u3);
         double uv[3] = {u1, u2, u3};
@@ -1629,7 +1649,7 @@ static void nscbc_inflow(or_solver *s, int d, int side)
         for (int k = 0; k < s->Ns; k++) {
             double dLs = bc->eta_Y * K * (s->Y[k][I] - bc->Y[k]) - un * s->gY[d][k][I];
             dQc(s, k)[I] -= s->Y[k][I] * dd1 + rho * dLs;
-            sphiY += eos_dphi_dY(s, k, rho, p, Tp, cpm) * dLs;
+            if (s->d.eos_kind != 2) sphiY += eos_dphi_dY(s, k, rho, p, Tp, cpm) * dLs;   /* physics.jl:212 */
         }
         for (int a = 0; a < 3; a++) {
             double extra = a == d ? rho * dd3 : a == t1 ? rho * dd4 : rho * dd5;
@@ -1696,6 +1716,19 @@ void or_compute_rhs(or_solver *s, int primitives_current)
             if (bc == OR_BC_NSCBC_OUT) nscbc_outflow(s, d, side);
             else if (bc == OR_BC_NSCBC_IN) nscbc_inflow(s, d, side);
         }
+    /* rhs.jl:692 add_sources!: sources.jl:44-65 ConstantBodyForce, in order */
+    for (int i = 0; i < s->d.n_body_forces; i++) {
+        const double g1 = s->d.body_force[i][0], g2 = s->d.body_force[i][1], g3 = s->d.body_force[i][2];
+        double *d1 = dQc(s, s->im1), *d2 = dQc(s, s->im2), *d3 = dQc(s, s->im3), *de = dQc(s, s->ie);
+        FOR_INTERIOR(s, I) {
+            double rho = 0.0;
+            for (int sp = 0; sp < s->Ns; sp++) rho += Qc(s, sp)[I];
+            d1[I] += rho * g1;
+            d2[I] += rho * g2;
+            d3[I] += rho * g3;
+            de[I] += Qc(s, s->im1)[I] * g1 + Qc(s, s->im2)[I] * g2 + Qc(s, s->im3)[I] * g3;
+        }
+    }
 }
 
 /* boundary.jl:232-243 SlipWallBC, problem.jl:196-207 DirichletBC (constant) */
@@ -1731,7 +1764,9 @@ void or_apply_bcs(or_solver *s)
                     Qc(s, s->im3)[I] = 0;
                     if (Tw > 0) {   /* boundary.jl:203-221 wall_internal_energy */
                         double re = 0.0;
-                        if (s->d.eos_kind == 1) {
+                        if (s->d.eos_kind == 2) {   /* boundary.jl:213-214 */
+                            re = Qc(s, 0)[I] * s->cvk[0] * Tw + s->d.p_inf;
+                        } else if (s->d.eos_kind == 1) {
                             for (int k = 0; k < s->Ns; k++) re += Qc(s, k)[I] * n9_e(s, k, Tw);
                         } else {
                             for (int k = 0; k < s->Ns; k++) re += Qc(s, k)[I] * s->cvk[k];

@@ -112,6 +112,9 @@ typedef struct {
     double Twall[3][2];               /* NoSlipWallBC: > 0 isothermal */
     int switchable[3][2];             /* SwitchableBC: bc until the switch, then bc_after */
     int bc_after[3][2];
+    int n_body_forces;                /* ConstantBodyForce accelerations (sources.jl), in order */
+    double body_force[4][3];
+    double p_inf;                     /* StiffenedGas (eos_kind 2): gamma[0], R[0] = (gamma - 1) cv */
 } or_desc;
 
 /* Step-boundary hooks of run! (src/callbacks.jl): next scheduled instant,

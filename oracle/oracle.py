@@ -62,7 +62,8 @@ class Desc(C.Structure):
                 ("retries", C.c_int), ("cfl_backoff", C.c_double), ("savepoint_interval", C.c_int),
                 ("landing_steps", C.c_int), ("eos_kind", C.c_int), ("T_guess", C.c_double),
                 ("nasa9", Nasa9Sp * MAXSP), ("metric", C.c_int), ("Twall", (C.c_double * 2) * 3),
-                ("switchable", (C.c_int * 2) * 3), ("bc_after", (C.c_int * 2) * 3)]
+                ("switchable", (C.c_int * 2) * 3), ("bc_after", (C.c_int * 2) * 3),
+                ("n_body_forces", C.c_int), ("body_force", (C.c_double * 3) * 4), ("p_inf", C.c_double)]
 
 
 def _cpu_has_avx2() -> bool:
@@ -362,7 +363,8 @@ def make_desc(n_global, L_domain, origin=(0.0, 0.0, 0.0), species=((1.0, 1.4),),
               C_D=0.01, beta_sensor=0, deriv="lele_d1_6", alphaf=0.45, cfl=0.5,
               filter_interval=1, bcs=None, bc_state=None, nscbc=None,
               predict=0.0, max_growth=0.0, dt_min=0.0, dt_min_ratio=1e-8, retries=0, cfl_backoff=0.5,
-              savepoint_interval=25, landing_steps=2, nasa9=None, metric=0, Twall=None, switch=None) -> Desc:
+              savepoint_interval=25, landing_steps=2, nasa9=None, metric=0, Twall=None, switch=None,
+              body_forces=(), stiffened=None) -> Desc:
     """nasa9: (T_guess, [(R, [(Tmin, Tmax, (a1..a7), b1), ...]), ...]) selects
     the Nasa9Mixture (species then ignores gamma)."""
     d = Desc()
@@ -406,6 +408,13 @@ def make_desc(n_global, L_domain, origin=(0.0, 0.0, 0.0), species=((1.0, 1.4),),
     d.metric = {"cartesian": 0, "cylindrical": 1, "spherical": 2}.get(metric, metric)
     for (k, sd), T in (Twall or {}).items():           # NoSlipWallBC faces: {(dim, side): Twall}
         d.Twall[k][sd] = T
+    if stiffened is not None:                          # StiffenedGas (gamma, p_inf, cv), one species
+        g, pinf, cv = stiffened
+        d.eos_kind, d.n_species, d.R[0], d.gamma[0], d.p_inf = 2, 1, (g - 1) * cv, g, pinf
+    d.n_body_forces = len(body_forces)                 # ConstantBodyForce accelerations, in order
+    for i, g in enumerate(body_forces):
+        for k in range(3):
+            d.body_force[i][k] = g[k]
     for (k, sd), after in (switch or {}).items():      # SwitchableBC faces: {(dim, side): bc_after}
         d.switchable[k][sd] = 1
         d.bc_after[k][sd] = after
