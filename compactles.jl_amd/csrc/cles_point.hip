@@ -306,75 +306,134 @@ __global__ void __launch_bounds__(TPB) k_delta4(D4Args<W> a, W *__restrict__ out
     out[I] = acc_out;
 }
 
-// The same sum for MODE 0 with one thread per (x, y) column marching D4_ZB
-// planes in z: the five z values sit in a register window (one new load per
-// plane), the x and y neighbours are direct loads, and no index needs a
-// division.  Blocks of 64 x 4 columns; the block index is renumbered per XCD
-// so that neighbouring y tiles, which share stencil rows, run on one XCD.
-constexpr int D4_ZB = 8;
-
 __device__ __forceinline__ int d4_index(int v, int n, int periodic)
 {
     if (periodic) return v < 0 ? v + n : (v >= n ? v - n : v);
     return v < 0 ? 0 : (v > n - 1 ? n - 1 : v);
 }
 
-template <typename W>
-__global__ void __launch_bounds__(TPB) k_delta4_col(D4Args<W> a, W *__restrict__ out, int gx, int gy)
+// The same sum, 2.5-D: a block owns a 64 x 16 tile of (x, y) columns and
+// marches D4T_Z planes in z.  Each plane's tile with its two-wide halo sits in
+// LDS (every slot holds the value at the wrapped or clamped source index, so
+// edges and collapsed dimensions need no special case), the x and y stencils
+// read LDS, and each thread keeps a five-plane z window in registers for its
+// four points.  HBM reads are the tile plus its halo once per plane.
+constexpr int D4T_X = 64, D4T_Y = 16, D4T_Z = 16, D4T_LX = D4T_X + 4, D4T_LY = D4T_Y + 4;
+
+constexpr int D4T_HALO = D4T_LX * D4T_LY - D4T_X * D4T_Y;   // 336 halo slots per plane
+
+__device__ __forceinline__ int d4_src(int v, int n, int periodic, bool active)
 {
+    return active ? d4_index(v, n, periodic) : 0;
+}
+// a tile slot at source coordinate v is read by some point of the grid only
+// when v lies in [-2, n + 1] (collapsed: v = 0); other slots are zero-filled
+__device__ __forceinline__ bool d4_needed(int v, int n, bool active)
+{
+    return active ? (v >= -2 && v <= n + 1) : v == 0;
+}
+
+template <typename W>
+__global__ void __launch_bounds__(TPB) k_delta4_tile(D4Args<W> a, W *__restrict__ out, int gx, int gy)
+{
+    __shared__ double lds[D4T_LY * D4T_LX];
     const unsigned blk = xcd_block(blockIdx.x, gridDim.x);
     const int bx = (int)(blk % (unsigned)gx);
     const unsigned r = blk / (unsigned)gx;
     const int by = (int)(r % (unsigned)gy), bz = (int)(r / (unsigned)gy);
     const int nx = (int)a.n[0], ny = (int)a.n[1], nz = (int)a.n[2];
-    const int i = bx * 64 + (int)(threadIdx.x & 63);
-    const int j = by * 4 + (int)(threadIdx.x >> 6);
-    if (i >= nx || j >= ny) return;
-    const int sy = nx, sz = nx * ny;
+    const int sy = nx;
+    const int64_t sz = (int64_t)nx * ny;
+    const bool xa = a.active[0], ya = a.active[1], za = a.active[2];
+    const int px = a.periodic[0], py = a.periodic[1], pz = a.periodic[2];
+    const int tx = (int)(threadIdx.x & 63), ty = (int)(threadIdx.x >> 6);
+    const int i0 = bx * D4T_X, j0 = by * D4T_Y, k0 = bz * D4T_Z;
+    const int i = i0 + tx;
+    const W *f = a.f;
     const double D4[5] = {1.0, -4.0, 6.0, -4.0, 1.0};
-    int xo[5], yo[5];
+    // own points: rows j0 + ty + 4q, q = 0..3
+    double w[4][5];
+    bool own[4];
 #pragma unroll
-    for (int m = 0; m < 5; m++) {
-        xo[m] = a.active[0] ? d4_index(i + m - 2, nx, a.periodic[0]) - i : 0;
-        yo[m] = a.active[1] ? (d4_index(j + m - 2, ny, a.periodic[1]) - j) * sy : 0;
+    for (int q = 0; q < 4; q++) {
+        const int j = j0 + ty + 4 * q;
+        own[q] = i < nx && j < ny;
+#pragma unroll
+        for (int m = 0; m < 4; m++) {
+            const int kz = d4_src(k0 - 2 + m, nz, pz, za);
+            w[q][m] = own[q] ? (double)f[(int64_t)kz * sz + (int64_t)j * sy + i] : 0.0;
+        }
     }
-    const W *col = a.f + i + j * sy;
-    const int k0 = bz * D4_ZB;
-    const bool zact = a.active[2];
-    // the z window of every plane of the column, loaded up front
-    double w[D4_ZB + 4];
+    const int kend = k0 + D4T_Z < nz ? k0 + D4T_Z : nz;
+    for (int k = k0; k < kend; k++) {
 #pragma unroll
-    for (int m = 0; m < D4_ZB + 4; m++) {
-        const int kz = k0 - 2 + m;
-        // z inactive: nz = 1, k0 = 0, and only the centre (m = 2, plane 0) is used
-        w[m] = (zact || m == 2) ? (double)col[d4_index(kz, nz, a.periodic[2]) * sz] : 0.0;
-    }
+        for (int q = 0; q < 4; q++) {
+            const int j = j0 + ty + 4 * q;
+            const int kz = d4_src(k + 2, nz, pz, za);
+            w[q][4] = own[q] ? (double)f[(int64_t)kz * sz + (int64_t)j * sy + i] : 0.0;
+        }
+        __syncthreads();   // the previous plane's stencil reads are done
+        const W *plane = f + (int64_t)k * sz;
 #pragma unroll
-    for (int kk = 0; kk < D4_ZB; kk++) {
-        const int k = k0 + kk;
-        if (k < nz) {
-            const double wc = w[kk + 2];
-            const W *base = col + k * sz;
-            W acc_out = 0;
-            if (a.active[0]) {
-                double acc = 0.0;
-#pragma unroll
-                for (int m = 0; m < 5; m++) acc += D4[m] * (m == 2 ? wc : (double)base[xo[m]]);
-                acc_out += (W)(a.wd[0] * fabs(acc));
+        for (int q = 0; q < 4; q++) {
+            const int yl = ty + 4 * q;
+            double v = w[q][2];
+            if (!own[q]) {
+                v = 0.0;
+                if (d4_needed(i, nx, xa) && d4_needed(j0 + yl, ny, ya)) {
+                    const int ii = d4_src(i, nx, px, xa), jj = d4_src(j0 + yl, ny, py, ya);
+                    v = (double)plane[(int64_t)jj * sy + ii];
+                }
             }
-            if (a.active[1]) {
-                double acc = 0.0;
-#pragma unroll
-                for (int m = 0; m < 5; m++) acc += D4[m] * (m == 2 ? wc : (double)base[yo[m]]);
-                acc_out += (W)(a.wd[1] * fabs(acc));
+            lds[(yl + 2) * D4T_LX + tx + 2] = v;
+        }
+        for (int h = (int)threadIdx.x; h < D4T_HALO; h += TPB) {
+            int xl, yl;
+            if (h < 4 * D4T_LX) {   // rows -2, -1, 16, 17 across the full width
+                const int rr = h / D4T_LX;
+                xl = h - rr * D4T_LX;
+                yl = rr < 2 ? rr : D4T_Y + rr;
+            } else {                // columns -2, -1, 64, 65 of the 16 inner rows
+                const int t = h - 4 * D4T_LX;
+                const int c = t / D4T_Y;
+                yl = 2 + (t - c * D4T_Y);
+                xl = c < 2 ? c : D4T_X + c;
             }
-            if (zact) {
-                double acc = 0.0;
-#pragma unroll
-                for (int m = 0; m < 5; m++) acc += D4[m] * w[kk + m];
-                acc_out += (W)(a.wd[2] * fabs(acc));
+            double v = 0.0;
+            if (d4_needed(i0 + xl - 2, nx, xa) && d4_needed(j0 + yl - 2, ny, ya)) {
+                const int ii = d4_src(i0 + xl - 2, nx, px, xa), jj = d4_src(j0 + yl - 2, ny, py, ya);
+                v = (double)plane[(int64_t)jj * sy + ii];
             }
-            out[(int64_t)k * sz + (int64_t)j * sy + i] = acc_out;
+            lds[yl * D4T_LX + xl] = v;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            if (own[q]) {
+                const int xl = tx + 2, yl = ty + 4 * q + 2;
+                W acc_out = 0;
+                if (xa) {
+                    double acc = 0.0;
+#pragma unroll
+                    for (int m = 0; m < 5; m++) acc += D4[m] * lds[yl * D4T_LX + xl + m - 2];
+                    acc_out += (W)(a.wd[0] * fabs(acc));
+                }
+                if (ya) {
+                    double acc = 0.0;
+#pragma unroll
+                    for (int m = 0; m < 5; m++) acc += D4[m] * lds[(yl + m - 2) * D4T_LX + xl];
+                    acc_out += (W)(a.wd[1] * fabs(acc));
+                }
+                if (za) {
+                    double acc = 0.0;
+#pragma unroll
+                    for (int m = 0; m < 5; m++) acc += D4[m] * w[q][m];
+                    acc_out += (W)(a.wd[2] * fabs(acc));
+                }
+                out[(int64_t)k * sz + (int64_t)(j0 + yl - 2) * sy + i] = acc_out;
+            }
+#pragma unroll
+            for (int m = 0; m < 4; m++) w[q][m] = w[q][m + 1];
         }
     }
 }
@@ -1293,10 +1352,10 @@ template <typename W>
 void launch_delta4(const D4Args<W> &a, int mode, W *out, hipStream_t st)
 {
     if (mode == 0) {
-        const int gx = (int)((a.n[0] + 63) / 64), gy = (int)((a.n[1] + 3) / 4);
-        const int gz = (int)((a.n[2] + D4_ZB - 1) / D4_ZB);
+        const int gx = (int)((a.n[0] + D4T_X - 1) / D4T_X), gy = (int)((a.n[1] + D4T_Y - 1) / D4T_Y);
+        const int gz = (int)((a.n[2] + D4T_Z - 1) / D4T_Z);
         const int64_t nb = (int64_t)gx * gy * gz;
-        hipLaunchKernelGGL((k_delta4_col<W>), dim3((unsigned)nb), dim3(TPB), 0, st, a, out, gx, gy);
+        hipLaunchKernelGGL((k_delta4_tile<W>), dim3((unsigned)nb), dim3(TPB), 0, st, a, out, gx, gy);
     } else
         hipLaunchKernelGGL((k_delta4<W, 1>), dim3(nblocks(a.N)), dim3(TPB), 0, st, a, out);
     CHECK_LAUNCH();
