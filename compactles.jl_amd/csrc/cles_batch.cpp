@@ -81,11 +81,20 @@ struct cles_batch {
         return ptr;
     }
     void *wfield() { return alloc((size_t)N * wsize()); }
+    // Creation-time upload of a host table into memory from alloc(): on the
+    // same stream as alloc's zero-fill (a plain hipMemcpy runs on the null
+    // stream, which a non-blocking stream does not wait for, so the fill could
+    // land after it), and complete before the host buffer goes away.
+    void upload(void *dst, const void *src, size_t bytes)
+    {
+        CLES_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, st()));
+        CLES_HIP(hipStreamSynchronize(st()));
+    }
     template <typename X>
     X *upload_vec(const std::vector<X> &v)
     {
         X *d = (X *)alloc(v.size() * sizeof(X));
-        if (!v.empty()) CLES_HIP(hipMemcpyAsync(d, v.data(), v.size() * sizeof(X), hipMemcpyHostToDevice, st()));
+        if (!v.empty()) upload(d, v.data(), v.size() * sizeof(X));
         return d;
     }
 
@@ -518,7 +527,7 @@ int cles_batch_create(cles_ctx *ctx, const cles_solver_desc *desc, int64_t count
             if (d.eos_kind == CLES_EOS_NASA9) {
                 const cles::N9Table t = cles::nasa9_table(d);
                 auto *dt = (cles::N9Table *)b->alloc(sizeof(cles::N9Table));
-                CLES_HIP(hipMemcpy(dt, &t, sizeof t, hipMemcpyHostToDevice));
+                b->upload(dt, &t, sizeof t);
                 b->n9 = dt;
             }
             b->off.resize(count);

@@ -168,6 +168,15 @@ struct cles_solver {
         return ptr;
     }
     void *wfield() { return alloc((size_t)N * wsize()); }
+    // Creation-time upload of a host table into memory from alloc(): on the
+    // same stream as alloc's zero-fill (a plain hipMemcpy runs on the null
+    // stream, which a non-blocking stream does not wait for, so the fill could
+    // land after it), and complete before the host buffer goes away.
+    void upload(void *dst, const void *src, size_t bytes)
+    {
+        CLES_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, st()));
+        CLES_HIP(hipStreamSynchronize(st()));
+    }
 
     ~cles_solver()
     {
@@ -1392,14 +1401,14 @@ int cles_solver_create_geom(cles_ctx *ctx, const cles_solver_desc *desc, const d
             if (desc->eos_kind == CLES_EOS_NASA9) {
                 const cles::N9Table t = cles::nasa9_table(*desc);
                 auto *dt = (cles::N9Table *)s->alloc(sizeof(cles::N9Table));
-                CLES_HIP(hipMemcpy(dt, &t, sizeof t, hipMemcpyHostToDevice));
+                s->upload(dt, &t, sizeof t);
                 s->n9 = dt;
             }
             for (int k = 0; k < 3; k++) {
                 s->met.n[k] = s->n[k];
                 if (!(dxdxi && dxdxi[k])) continue;
                 double *m = (double *)s->alloc((size_t)s->n[k] * sizeof(double));
-                CLES_HIP(hipMemcpy(m, dxdxi[k], (size_t)s->n[k] * sizeof(double), hipMemcpyHostToDevice));
+                s->upload(m, dxdxi[k], (size_t)s->n[k] * sizeof(double));
                 s->met.m[k] = m;
             }
             s->met.kind = kind;
@@ -1417,7 +1426,7 @@ int cles_solver_create_geom(cles_ctx *ctx, const cles_solver_desc *desc, const d
                                  "treated on the GPU backend)");
                 }
                 double *rd = (double *)s->alloc(r.size() * sizeof(double));
-                CLES_HIP(hipMemcpy(rd, r.data(), r.size() * sizeof(double), hipMemcpyHostToDevice));
+                s->upload(rd, r.data(), r.size() * sizeof(double));
                 s->met.r = rd;
                 if (kind == CLES_METRIC_SPHERICAL) {
                     std::vector<double> sc(2 * (size_t)s->n[1]);
@@ -1430,7 +1439,7 @@ int cles_solver_create_geom(cles_ctx *ctx, const cles_solver_desc *desc, const d
                         sc[s->n[1] + j] = std::cos(th);
                     }
                     double *sd = (double *)s->alloc(sc.size() * sizeof(double));
-                    CLES_HIP(hipMemcpy(sd, sc.data(), sc.size() * sizeof(double), hipMemcpyHostToDevice));
+                    s->upload(sd, sc.data(), sc.size() * sizeof(double));
                     s->met.sth = sd;
                     s->met.cth = sd + s->n[1];
                 }
