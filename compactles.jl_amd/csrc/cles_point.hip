@@ -317,7 +317,8 @@ __device__ __forceinline__ int d4_index(int v, int n, int periodic)
 // LDS (every slot holds the value at the wrapped or clamped source index, so
 // edges and collapsed dimensions need no special case), the x and y stencils
 // read LDS, and each thread keeps a five-plane z window in registers for its
-// four points.  HBM reads are the tile plus its halo once per plane.
+// four points.  HBM reads are the tile plus its halo once per plane; the
+// values of the next plane are loaded while the current one is computed.
 constexpr int D4T_X = 64, D4T_Y = 16, D4T_Z = 16, D4T_LX = D4T_X + 4, D4T_LY = D4T_Y + 4;
 
 constexpr int D4T_HALO = D4T_LX * D4T_LY - D4T_X * D4T_Y;   // 336 halo slots per plane
@@ -336,7 +337,7 @@ __device__ __forceinline__ bool d4_needed(int v, int n, bool active)
 template <typename W>
 __global__ void __launch_bounds__(TPB) k_delta4_tile(D4Args<W> a, W *__restrict__ out, int gx, int gy)
 {
-    __shared__ double lds[D4T_LY * D4T_LX];
+    __shared__ W lds[D4T_LY * D4T_LX];   // holds input values, so W is exact
     const unsigned blk = xcd_block(blockIdx.x, gridDim.x);
     const int bx = (int)(blk % (unsigned)gx);
     const unsigned r = blk / (unsigned)gx;
@@ -354,40 +355,32 @@ __global__ void __launch_bounds__(TPB) k_delta4_tile(D4Args<W> a, W *__restrict_
     // own points: rows j0 + ty + 4q, q = 0..3
     double w[4][5];
     bool own[4];
+    // LDS slots this thread fills on every plane: slots 0..3 are its own rows
+    // (from the register window when the point is in the grid), slots 4..5 up
+    // to two halo slots; src is the in-plane source offset, -1 for a slot that
+    // no point reads (zero), and slot -1 marks a halo slot past the 336th
+    int slot[6];
+    int64_t src[6];
 #pragma unroll
     for (int q = 0; q < 4; q++) {
         const int j = j0 + ty + 4 * q;
         own[q] = i < nx && j < ny;
+        slot[q] = (ty + 4 * q + 2) * D4T_LX + tx + 2;
+        src[q] = -1;
+        if (!own[q] && d4_needed(i, nx, xa) && d4_needed(j, ny, ya))
+            src[q] = (int64_t)d4_src(j, ny, py, ya) * sy + d4_src(i, nx, px, xa);
 #pragma unroll
         for (int m = 0; m < 4; m++) {
             const int kz = d4_src(k0 - 2 + m, nz, pz, za);
             w[q][m] = own[q] ? (double)f[(int64_t)kz * sz + (int64_t)j * sy + i] : 0.0;
         }
     }
-    const int kend = k0 + D4T_Z < nz ? k0 + D4T_Z : nz;
-    for (int k = k0; k < kend; k++) {
 #pragma unroll
-        for (int q = 0; q < 4; q++) {
-            const int j = j0 + ty + 4 * q;
-            const int kz = d4_src(k + 2, nz, pz, za);
-            w[q][4] = own[q] ? (double)f[(int64_t)kz * sz + (int64_t)j * sy + i] : 0.0;
-        }
-        __syncthreads();   // the previous plane's stencil reads are done
-        const W *plane = f + (int64_t)k * sz;
-#pragma unroll
-        for (int q = 0; q < 4; q++) {
-            const int yl = ty + 4 * q;
-            double v = w[q][2];
-            if (!own[q]) {
-                v = 0.0;
-                if (d4_needed(i, nx, xa) && d4_needed(j0 + yl, ny, ya)) {
-                    const int ii = d4_src(i, nx, px, xa), jj = d4_src(j0 + yl, ny, py, ya);
-                    v = (double)plane[(int64_t)jj * sy + ii];
-                }
-            }
-            lds[(yl + 2) * D4T_LX + tx + 2] = v;
-        }
-        for (int h = (int)threadIdx.x; h < D4T_HALO; h += TPB) {
+    for (int e = 0; e < 2; e++) {
+        const int h = (int)threadIdx.x + TPB * e;
+        slot[4 + e] = -1;
+        src[4 + e] = -1;
+        if (h < D4T_HALO) {
             int xl, yl;
             if (h < 4 * D4T_LX) {   // rows -2, -1, 16, 17 across the full width
                 const int rr = h / D4T_LX;
@@ -399,13 +392,38 @@ __global__ void __launch_bounds__(TPB) k_delta4_tile(D4Args<W> a, W *__restrict_
                 yl = 2 + (t - c * D4T_Y);
                 xl = c < 2 ? c : D4T_X + c;
             }
-            double v = 0.0;
-            if (d4_needed(i0 + xl - 2, nx, xa) && d4_needed(j0 + yl - 2, ny, ya)) {
-                const int ii = d4_src(i0 + xl - 2, nx, px, xa), jj = d4_src(j0 + yl - 2, ny, py, ya);
-                v = (double)plane[(int64_t)jj * sy + ii];
-            }
-            lds[yl * D4T_LX + xl] = v;
+            slot[4 + e] = yl * D4T_LX + xl;
+            if (d4_needed(i0 + xl - 2, nx, xa) && d4_needed(j0 + yl - 2, ny, ya))
+                src[4 + e] = (int64_t)d4_src(j0 + yl - 2, ny, py, ya) * sy + d4_src(i0 + xl - 2, nx, px, xa);
         }
+    }
+    // the slot values of plane k are loaded one plane ahead, so their latency
+    // overlaps the previous plane's LDS fill and stencils
+    double cur[6], nxt[6];
+    const int kend = k0 + D4T_Z < nz ? k0 + D4T_Z : nz;
+#pragma unroll
+    for (int e = 0; e < 6; e++) {
+        cur[e] = src[e] >= 0 ? (double)f[(int64_t)k0 * sz + src[e]] : 0.0;
+        nxt[e] = 0.0;
+    }
+    for (int k = k0; k < kend; k++) {
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const int j = j0 + ty + 4 * q;
+            const int kz = d4_src(k + 2, nz, pz, za);
+            w[q][4] = own[q] ? (double)f[(int64_t)kz * sz + (int64_t)j * sy + i] : 0.0;
+        }
+        if (k + 1 < kend) {
+            const W *next = f + (int64_t)(k + 1) * sz;
+#pragma unroll
+            for (int e = 0; e < 6; e++) nxt[e] = src[e] >= 0 ? (double)next[src[e]] : 0.0;
+        }
+        __syncthreads();   // the previous plane's stencil reads are done
+#pragma unroll
+        for (int q = 0; q < 4; q++) lds[slot[q]] = (W)(own[q] ? w[q][2] : cur[q]);
+#pragma unroll
+        for (int e = 4; e < 6; e++)
+            if (slot[e] >= 0) lds[slot[e]] = (W)cur[e];
         __syncthreads();
 #pragma unroll
         for (int q = 0; q < 4; q++) {
@@ -415,13 +433,13 @@ __global__ void __launch_bounds__(TPB) k_delta4_tile(D4Args<W> a, W *__restrict_
                 if (xa) {
                     double acc = 0.0;
 #pragma unroll
-                    for (int m = 0; m < 5; m++) acc += D4[m] * lds[yl * D4T_LX + xl + m - 2];
+                    for (int m = 0; m < 5; m++) acc += D4[m] * (double)lds[yl * D4T_LX + xl + m - 2];
                     acc_out += (W)(a.wd[0] * fabs(acc));
                 }
                 if (ya) {
                     double acc = 0.0;
 #pragma unroll
-                    for (int m = 0; m < 5; m++) acc += D4[m] * lds[(yl + m - 2) * D4T_LX + xl];
+                    for (int m = 0; m < 5; m++) acc += D4[m] * (double)lds[(yl + m - 2) * D4T_LX + xl];
                     acc_out += (W)(a.wd[1] * fabs(acc));
                 }
                 if (za) {
@@ -435,6 +453,8 @@ __global__ void __launch_bounds__(TPB) k_delta4_tile(D4Args<W> a, W *__restrict_
 #pragma unroll
             for (int m = 0; m < 4; m++) w[q][m] = w[q][m + 1];
         }
+#pragma unroll
+        for (int e = 0; e < 6; e++) cur[e] = nxt[e];
     }
 }
 
