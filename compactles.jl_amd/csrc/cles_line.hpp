@@ -36,15 +36,20 @@ template <> __device__ __forceinline__ const float *rowp<float>(const LineDev &p
 // ---------------------------------------------------------------------------
 // Epilogues
 // ---------------------------------------------------------------------------
-template <typename T, int EPI>
+// NT: plain stores carry the non-temporal hint (a compile-time choice: a
+// run-time branch between two otherwise identical stores is merged into one
+// plain store by the compiler)
+template <typename T, int EPI, bool NT = false>
 __device__ __forceinline__ void epilogue(const EpiArgs &e, int64_t o, double v)
 {
     if constexpr (EPI == EPI_STORE_GATE || EPI == EPI_LSRK_DEV)
         if (!*e.gate) return;
-    if constexpr (EPI == EPI_STORE || EPI == EPI_STORE_GATE) {
-        ((T *)e.out)[o] = (T)v;
-    } else if constexpr (EPI == EPI_STORE_NEG) {
-        ((T *)e.out)[o] = (T)(-v);
+    if constexpr (EPI == EPI_STORE || EPI == EPI_STORE_GATE || EPI == EPI_STORE_NEG) {
+        const T val = EPI == EPI_STORE_NEG ? (T)(-v) : (T)v;
+        if constexpr (NT)
+            __builtin_nontemporal_store(val, (T *)e.out + o);
+        else
+            ((T *)e.out)[o] = val;
     } else if constexpr (EPI == EPI_SUB) {
         T *out = (T *)e.out;
         out[o] = out[o] - (T)v;
@@ -83,7 +88,7 @@ __device__ __forceinline__ void epilogue(const EpiArgs &e, int64_t o, double v)
 // Batched epilogue: every input of NB points is loaded before any output is
 // stored, so the loads of a batch are in flight together (the compiler cannot
 // reorder them across stores through pointers that may alias).
-template <typename T, int EPI, int NB>
+template <typename T, int EPI, int NB, bool NT = false>
 __device__ __forceinline__ void epilogue_batch(const EpiArgs &e, const int64_t (&o)[NB], const T (&v)[NB],
                                                const bool (&ok)[NB])
 {
@@ -92,7 +97,7 @@ __device__ __forceinline__ void epilogue_batch(const EpiArgs &e, const int64_t (
     if constexpr (EPI == EPI_STORE || EPI == EPI_STORE_NEG || EPI == EPI_STORE_GATE) {
 #pragma unroll
         for (int i = 0; i < NB; i++)
-            if (ok[i]) epilogue<T, EPI>(e, o[i], (double)v[i]);
+            if (ok[i]) epilogue<T, EPI, NT>(e, o[i], (double)v[i]);
     } else if constexpr (EPI == EPI_SUB) {
         T *out = (T *)e.out;
         T a[NB];
@@ -379,7 +384,7 @@ __device__ __forceinline__ double corrected(const LineDev &p, int g, T y, const 
 // chunks.  UNI: Lx == 64, so each wave holds one chunk and every per-row
 // constant is wave-uniform (scalar loads).
 // ---------------------------------------------------------------------------
-template <typename T, int Q, int M, int SYM, int EPI, bool UNI, int CL = MC>
+template <typename T, int Q, int M, int SYM, int EPI, bool UNI, int CL = MC, bool NTS = false>
 __global__ void __launch_bounds__(Q == 1 ? 1024 : 512) k_line_yz(LineDev p, const T *__restrict__ f, int sd, int64_t so,
                                                   int nlane, int nxb, int Lx, EpiArgs e, LineFields mf)
 {
@@ -445,7 +450,7 @@ __global__ void __launch_bounds__(Q == 1 ? 1024 : 512) k_line_yz(LineDev p, cons
             o[i] = off + (int64_t)(s + b + i) * sd;
             v[i] = ch.d[b + i];
         }
-        epilogue_batch<T, EPI, 8>(e, o, v, ok);
+        epilogue_batch<T, EPI, 8, NTS>(e, o, v, ok);
     }
 }
 
@@ -861,6 +866,7 @@ void launch_variant(const HostPlan &hp, int dim, const View3 &v, const T *f, con
 {
     const unsigned nf = (unsigned)mf.nf;
     const LineDev &p = hp.dev;
+
     const int P = p.P;
     const int B = 2 * Q;
     const size_t lim = lds_limit();
@@ -924,10 +930,25 @@ void launch_variant(const HostPlan &hp, int dim, const View3 &v, const T *f, con
             const int Lx = 64, NT = 64 * P;
             const int nxb = (int)((nlane + Lx - 1) / Lx);
             const size_t shm = (size_t)2 * B * NT * sizeof(double);
-            if (p.m + (p.extra ? 1 : 0) <= 16)
+            // The z sweep's plain stores take the non-temporal hint when the launch
+            // writes more than the 256 MB Infinity Cache holds (the outputs are
+            // evicted before they are read again); the x and y sweeps measured
+            // faster without it.
+            const bool narrow = p.m + (p.extra ? 1 : 0) <= 16;
+            bool nts = false;
+            if constexpr (EPI == EPI_STORE || EPI == EPI_STORE_NEG) {
+                nts = dim == 3 && (double)v.n[0] * v.n[1] * v.n[2] * sizeof(T) * nf > 256.0 * 1024 * 1024;
+                if (nts && narrow)
+                    hipLaunchKernelGGL((k_line_yz<T, Q, M, SYM, EPI, true, 16, true>), dim3((unsigned)(nxb * nouter), nf),
+                                       dim3(NT), shm, st, p, f, (int)sd, so, (int)nlane, nxb, Lx, e, mf);
+                else if (nts)
+                    hipLaunchKernelGGL((k_line_yz<T, Q, M, SYM, EPI, true, MC, true>), dim3((unsigned)(nxb * nouter), nf),
+                                       dim3(NT), shm, st, p, f, (int)sd, so, (int)nlane, nxb, Lx, e, mf);
+            }
+            if (!nts && narrow)
                 hipLaunchKernelGGL((k_line_yz<T, Q, M, SYM, EPI, true, 16>), dim3((unsigned)(nxb * nouter), nf), dim3(NT),
                                    shm, st, p, f, (int)sd, so, (int)nlane, nxb, Lx, e, mf);
-            else
+            else if (!nts)
                 hipLaunchKernelGGL((k_line_yz<T, Q, M, SYM, EPI, true>), dim3((unsigned)(nxb * nouter), nf), dim3(NT),
                                    shm, st, p, f, (int)sd, so, (int)nlane, nxb, Lx, e, mf);
         } else {

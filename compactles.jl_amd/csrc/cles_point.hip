@@ -506,16 +506,19 @@ __global__ void __launch_bounds__(TPB) k_dilatation_beta(const W *__restrict__ r
 // ---------------------------------------------------------------------------
 // rhs.jl:495-556 _assemble_fluxes! for the active dimensions
 // ---------------------------------------------------------------------------
-// The flux fields (15 at 256^3 FP64: 2 GB per launch) are consumed by the
-// next line sweeps but far exceed the L2 and Infinity Cache, so they are
-// written with the non-temporal hint.
-template <typename W>
+// The flux fields are read again by the divergence sweeps after the whole
+// launch; when they exceed the 256 MB Infinity Cache (15 fields, 2 GB at
+// 256^3 FP64) they are written with the non-temporal hint.
+template <bool NT, typename W>
 __device__ __forceinline__ void flux_store(W *p, W v)
 {
-    __builtin_nontemporal_store(v, p);
+    if constexpr (NT)
+        __builtin_nontemporal_store(v, p);
+    else
+        *p = v;
 }
 
-template <typename W>
+template <typename W, bool NT>
 __global__ void __launch_bounds__(TPB) k_fluxes(FluxArgs<W> a, int64_t N)
 {
     const int64_t I = (int64_t)blockIdx.x * TPB + threadIdx.x;
@@ -563,18 +566,18 @@ __global__ void __launch_bounds__(TPB) k_fluxes(FluxArgs<W> a, int64_t N)
                 if (sp < Ns) {
                     const W Dk = D0 + a.D_art[sp][I];
                     const W J = rho * (-Dk * a.gY[d][sp][I] + a.Y[sp][I] * Vc);
-                    flux_store(F[sp] + I, rho * a.Y[sp][I] * ud + J);
+                    flux_store<NT>(F[sp] + I, rho * a.Y[sp][I] * ud + J);
                     hdiff += hk[sp] * J;
                 }
         } else {
             // one species: sum_k J_k = 0 forces J_1 = 0 (the reference's
             // rho*D0*dY*(Y-1) with Y == 1 to rounding)
-            flux_store(F[0] + I, rho * a.Y[0][I] * ud);
+            flux_store<NT>(F[0] + I, rho * a.Y[0][I] * ud);
         }
-        flux_store(F[Ns] + I, rho * ud * uv[0] + (d == 0 ? p : W(0)) - tau[d][0]);
-        flux_store(F[Ns + 1] + I, rho * ud * uv[1] + (d == 1 ? p : W(0)) - tau[d][1]);
-        flux_store(F[Ns + 2] + I, rho * ud * uv[2] + (d == 2 ? p : W(0)) - tau[d][2]);
-        flux_store(F[Ns + 3] + I, (E + p) * ud - (uv[0] * tau[d][0] + uv[1] * tau[d][1] + uv[2] * tau[d][2]) -
+        flux_store<NT>(F[Ns] + I, rho * ud * uv[0] + (d == 0 ? p : W(0)) - tau[d][0]);
+        flux_store<NT>(F[Ns + 1] + I, rho * ud * uv[1] + (d == 1 ? p : W(0)) - tau[d][1]);
+        flux_store<NT>(F[Ns + 2] + I, rho * ud * uv[2] + (d == 2 ? p : W(0)) - tau[d][2]);
+        flux_store<NT>(F[Ns + 3] + I, (E + p) * ud - (uv[0] * tau[d][0] + uv[1] * tau[d][1] + uv[2] * tau[d][2]) -
                        kappa * a.gT[d][I] + hdiff);
     }
 }
@@ -1415,7 +1418,11 @@ void launch_dilatation_beta(const W *rho, const W *sensor, const W *sw, W *beta,
 template <typename W>
 void launch_fluxes(const FluxArgs<W> &a, int64_t N, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_fluxes<W>, dim3(nblocks(N)), dim3(TPB), 0, st, a, N);
+    const int nact = (a.active[0] != 0) + (a.active[1] != 0) + (a.active[2] != 0);
+    if ((double)N * sizeof(W) * (a.Ns + 4) * nact > 256.0 * 1024 * 1024)
+        hipLaunchKernelGGL((k_fluxes<W, true>), dim3(nblocks(N)), dim3(TPB), 0, st, a, N);
+    else
+        hipLaunchKernelGGL((k_fluxes<W, false>), dim3(nblocks(N)), dim3(TPB), 0, st, a, N);
     CHECK_LAUNCH();
 }
 
