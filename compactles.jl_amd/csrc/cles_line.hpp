@@ -955,6 +955,9 @@ void launch_variant(const HostPlan &hp, int dim, const View3 &v, const T *f, con
             int Lx = 64;
             while (Lx * P > maxNT && Lx > 1) Lx /= 2;
             while (Lx / 2 >= nlane && Lx > 1) Lx /= 2;   // narrow x extents (2-D / 1-D grids)
+            // few lines (2-D grids): narrower blocks, down to 8 lanes (64-byte
+            // rows), until the launch has two blocks per CU
+            while (Lx > 8 && ((nlane + Lx - 1) / Lx) * nouter * (int64_t)nf < 512) Lx /= 2;
             if (Lx * P > maxNT) throw Error{CLES_ERR_UNSUPPORTED, "line too long for the y/z-sweep kernel"};
             const int NT = Lx * P;
             const int nxb = (int)((nlane + Lx - 1) / Lx);
