@@ -91,7 +91,19 @@ __device__ double n9_temperature(const N9Table &t, double e, const double *Y)
 // ---------------------------------------------------------------------------
 // physics.jl:604-653 _primitives! (IdealMixture), over the interior
 // ---------------------------------------------------------------------------
-template <typename W>
+// The primitives are read again by the gradient sweeps after the whole
+// launch; when they exceed the 256 MB Infinity Cache they are written with the
+// non-temporal hint (a compile-time choice, as for the fluxes).
+template <bool NT, typename W>
+__device__ __forceinline__ void prim_store(W *p, W v)
+{
+    if constexpr (NT)
+        __builtin_nontemporal_store(v, p);
+    else
+        *p = v;
+}
+
+template <typename W, bool NT>
 __global__ void __launch_bounds__(TPB) k_primitives(PrimArgs<W> a, int64_t N)
 {
     const int64_t I = (int64_t)blockIdx.x * TPB + threadIdx.x;
@@ -109,10 +121,10 @@ __global__ void __launch_bounds__(TPB) k_primitives(PrimArgs<W> a, int64_t N)
             const double m1 = a.Q[(int64_t)Ns * N + I], m2 = a.Q[(int64_t)(Ns + 1) * N + I],
                          m3 = a.Q[(int64_t)(Ns + 2) * N + I];
             const double ke = 0.5 * (m1 * m1 + m2 * m2 + m3 * m3) / r;
-            a.esens[I] = (W)((a.Q[(int64_t)(Ns + 3) * N + I] - ke) / r);
+            prim_store<NT>(&a.esens[I], (W)((a.Q[(int64_t)(Ns + 3) * N + I] - ke) / r));
         }
-        a.Y[0][I] = W(1);
-        a.cp[I] = (W)(g * cv);
+        prim_store<NT>(&a.Y[0][I], (W)(W(1)));
+        prim_store<NT>(&a.cp[I], (W)(g * cv));
         if (rho > 0) {
             const double ri = 1 / rho;
             const double u = a.Q[(int64_t)Ns * N + I] * ri;
@@ -121,16 +133,16 @@ __global__ void __launch_bounds__(TPB) k_primitives(PrimArgs<W> a, int64_t N)
             const double e = a.Q[(int64_t)(Ns + 3) * N + I] * ri - 0.5 * (u * u + v * v + w * w);
             const double p = (g - 1) * rho * e - g * pinf;
             const double cc = g * (p + pinf) * ri;
-            a.rho[I] = (W)rho;
-            a.u[I] = (W)u;
-            a.v[I] = (W)v;
-            a.w[I] = (W)w;
-            a.p[I] = (W)p;
-            a.T[I] = (W)d_max((p + pinf) / (rho * R), 1e-300);
-            a.c[I] = (W)sqrt(cc > 0.0 ? cc : 0.0);
+            prim_store<NT>(&a.rho[I], (W)rho);
+            prim_store<NT>(&a.u[I], (W)u);
+            prim_store<NT>(&a.v[I], (W)v);
+            prim_store<NT>(&a.w[I], (W)w);
+            prim_store<NT>(&a.p[I], (W)p);
+            prim_store<NT>(&a.T[I], (W)d_max((p + pinf) / (rho * R), 1e-300));
+            prim_store<NT>(&a.c[I], (W)sqrt(cc > 0.0 ? cc : 0.0));
         } else {
-            a.rho[I] = 1; a.u[I] = 0; a.v[I] = 0; a.w[I] = 0;
-            a.p[I] = 1; a.T[I] = 1; a.c[I] = 1;
+            prim_store<NT>(&a.rho[I], (W)(1)); prim_store<NT>(&a.u[I], (W)(0)); prim_store<NT>(&a.v[I], (W)(0)); prim_store<NT>(&a.w[I], (W)(0));
+            prim_store<NT>(&a.p[I], (W)(1)); prim_store<NT>(&a.T[I], (W)(1)); prim_store<NT>(&a.c[I], (W)(1));
         }
         return;
     }
@@ -142,7 +154,7 @@ __global__ void __launch_bounds__(TPB) k_primitives(PrimArgs<W> a, int64_t N)
         for (int sp = 0; sp < Ns; sp++) {
             const double q = a.Q[(int64_t)sp * N + I];
             Y[sp] = q * ri;
-            a.Y[sp][I] = (W)Y[sp];
+            prim_store<NT>(&a.Y[sp][I], (W)Y[sp]);
             Rm += q * ri * t.R[sp];
         }
         const double u = a.Q[(int64_t)Ns * N + I] * ri;
@@ -158,17 +170,17 @@ __global__ void __launch_bounds__(TPB) k_primitives(PrimArgs<W> a, int64_t N)
             const double m1 = a.Q[(int64_t)Ns * N + I], m2 = a.Q[(int64_t)(Ns + 1) * N + I],
                          m3 = a.Q[(int64_t)(Ns + 2) * N + I];
             const double ke = 0.5 * (m1 * m1 + m2 * m2 + m3 * m3) / r;
-            a.esens[I] = (W)((a.Q[(int64_t)(Ns + 3) * N + I] - ke) / r);
+            prim_store<NT>(&a.esens[I], (W)((a.Q[(int64_t)(Ns + 3) * N + I] - ke) / r));
         }
         const double cc = (cpm / cvm) * Rm * T;
-        a.rho[I] = (W)rho;
-        a.u[I] = (W)u;
-        a.v[I] = (W)v;
-        a.w[I] = (W)w;
-        a.p[I] = (W)(rho * Rm * T);
-        a.T[I] = (W)T;
-        a.c[I] = (W)sqrt(cc > 0.0 ? cc : 0.0);
-        a.cp[I] = (W)cpm;
+        prim_store<NT>(&a.rho[I], (W)rho);
+        prim_store<NT>(&a.u[I], (W)u);
+        prim_store<NT>(&a.v[I], (W)v);
+        prim_store<NT>(&a.w[I], (W)w);
+        prim_store<NT>(&a.p[I], (W)(rho * Rm * T));
+        prim_store<NT>(&a.T[I], (W)T);
+        prim_store<NT>(&a.c[I], (W)sqrt(cc > 0.0 ? cc : 0.0));
+        prim_store<NT>(&a.cp[I], (W)cpm);
     } else if (rho > 0) {
         const double ri = 1 / rho;
         double Rm = 0.0, cvm = 0.0;
@@ -178,7 +190,7 @@ __global__ void __launch_bounds__(TPB) k_primitives(PrimArgs<W> a, int64_t N)
                 const double q = a.Q[(int64_t)sp * N + I];
                 Rm += q * a.Rk[sp];
                 cvm += q * a.cvk[sp];
-                a.Y[sp][I] = (W)(q * ri);
+                prim_store<NT>(&a.Y[sp][I], (W)(q * ri));
             }
         Rm *= ri;
         cvm *= ri;
@@ -195,28 +207,28 @@ __global__ void __launch_bounds__(TPB) k_primitives(PrimArgs<W> a, int64_t N)
             const double m1 = a.Q[(int64_t)Ns * N + I], m2 = a.Q[(int64_t)(Ns + 1) * N + I],
                          m3 = a.Q[(int64_t)(Ns + 2) * N + I];
             const double ke = 0.5 * (m1 * m1 + m2 * m2 + m3 * m3) / r;
-            a.esens[I] = (W)((a.Q[(int64_t)(Ns + 3) * N + I] - ke) / r);
+            prim_store<NT>(&a.esens[I], (W)((a.Q[(int64_t)(Ns + 3) * N + I] - ke) / r));
         }
-        a.rho[I] = (W)rho;
-        a.u[I] = (W)u;
-        a.v[I] = (W)v;
-        a.w[I] = (W)w;
-        a.p[I] = (W)p;
-        a.T[I] = (W)T;
-        a.c[I] = (W)sqrt(gm * Rm * T);
-        a.cp[I] = (W)(cvm + Rm);
+        prim_store<NT>(&a.rho[I], (W)rho);
+        prim_store<NT>(&a.u[I], (W)u);
+        prim_store<NT>(&a.v[I], (W)v);
+        prim_store<NT>(&a.w[I], (W)w);
+        prim_store<NT>(&a.p[I], (W)p);
+        prim_store<NT>(&a.T[I], (W)T);
+        prim_store<NT>(&a.c[I], (W)sqrt(gm * Rm * T));
+        prim_store<NT>(&a.cp[I], (W)(cvm + Rm));
     } else {
         if (a.esens) {
             const double m1 = a.Q[(int64_t)Ns * N + I], m2 = a.Q[(int64_t)(Ns + 1) * N + I],
                          m3 = a.Q[(int64_t)(Ns + 2) * N + I];
             const double ke = 0.5 * (m1 * m1 + m2 * m2 + m3 * m3) / 1.0;
-            a.esens[I] = (W)((a.Q[(int64_t)(Ns + 3) * N + I] - ke) / 1.0);
+            prim_store<NT>(&a.esens[I], (W)((a.Q[(int64_t)(Ns + 3) * N + I] - ke) / 1.0));
         }
-        a.rho[I] = 1; a.u[I] = 0; a.v[I] = 0; a.w[I] = 0;
-        a.p[I] = 1; a.T[I] = 1; a.c[I] = 1; a.cp[I] = 1;
+        prim_store<NT>(&a.rho[I], (W)(1)); prim_store<NT>(&a.u[I], (W)(0)); prim_store<NT>(&a.v[I], (W)(0)); prim_store<NT>(&a.w[I], (W)(0));
+        prim_store<NT>(&a.p[I], (W)(1)); prim_store<NT>(&a.T[I], (W)(1)); prim_store<NT>(&a.c[I], (W)(1)); prim_store<NT>(&a.cp[I], (W)(1));
 #pragma unroll
         for (int sp = 0; sp < CLES_MAXSP; sp++)
-            if (sp < Ns) a.Y[sp][I] = sp == 0 ? W(1) : W(0);
+            if (sp < Ns) prim_store<NT>(&a.Y[sp][I], (W)(sp == 0 ? W(1) : W(0)));
     }
 }
 
@@ -1369,7 +1381,11 @@ __global__ void __launch_bounds__(TPB) k_convert(Dst *__restrict__ d, const Src 
 template <typename W>
 void launch_primitives(const PrimArgs<W> &a, int64_t N, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_primitives<W>, dim3(nblocks(N)), dim3(TPB), 0, st, a, N);
+    // rho, u, v, w, p, T, c, cp, the mass fractions and the e-sensor input
+    if ((double)N * sizeof(W) * (9 + a.Ns) > 256.0 * 1024 * 1024)
+        hipLaunchKernelGGL((k_primitives<W, true>), dim3(nblocks(N)), dim3(TPB), 0, st, a, N);
+    else
+        hipLaunchKernelGGL((k_primitives<W, false>), dim3(nblocks(N)), dim3(TPB), 0, st, a, N);
     CHECK_LAUNCH();
 }
 
