@@ -520,7 +520,17 @@ __global__ void __launch_bounds__(TPB) k_dilatation_beta(const W *__restrict__ r
 // ---------------------------------------------------------------------------
 // The flux fields are read again by the divergence sweeps after the whole
 // launch; when they exceed the 256 MB Infinity Cache (15 fields, 2 GB at
-// 256^3 FP64) they are written with the non-temporal hint.
+// 256^3 FP64) they are written with the non-temporal hint, and the velocity
+// and temperature gradients, dead after this kernel, are read with it.
+template <bool NT, typename W>
+__device__ __forceinline__ W dead_load(const W *p)
+{
+    if constexpr (NT)
+        return __builtin_nontemporal_load(p);
+    else
+        return *p;
+}
+
 template <bool NT, typename W>
 __device__ __forceinline__ void flux_store(W *p, W v)
 {
@@ -547,7 +557,7 @@ __global__ void __launch_bounds__(TPB) k_fluxes(FluxArgs<W> a, int64_t N)
     const W D0 = mu0 / ((W)a.Sc * rho);
     W G[9];
 #pragma unroll
-    for (int m = 0; m < 9; m++) G[m] = a.g.gu[m] ? a.g.gu[m][I] : W(0);
+    for (int m = 0; m < 9; m++) G[m] = a.g.gu[m] ? dead_load<NT>(a.g.gu[m] + I) : W(0);
     const W divu = G[0] + G[4] + G[8];
     const W t23 = W(2.0 / 3.0);
     const W t11 = mu * (2 * G[0] - t23 * divu) + beta * divu;
@@ -590,7 +600,7 @@ __global__ void __launch_bounds__(TPB) k_fluxes(FluxArgs<W> a, int64_t N)
         flux_store<NT>(F[Ns + 1] + I, rho * ud * uv[1] + (d == 1 ? p : W(0)) - tau[d][1]);
         flux_store<NT>(F[Ns + 2] + I, rho * ud * uv[2] + (d == 2 ? p : W(0)) - tau[d][2]);
         flux_store<NT>(F[Ns + 3] + I, (E + p) * ud - (uv[0] * tau[d][0] + uv[1] * tau[d][1] + uv[2] * tau[d][2]) -
-                       kappa * a.gT[d][I] + hdiff);
+                       kappa * dead_load<NT>(a.gT[d] + I) + hdiff);
     }
 }
 
