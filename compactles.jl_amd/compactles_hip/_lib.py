@@ -91,7 +91,7 @@ SIGNATURES = {
     "cles_synchronize": (C.c_int, [P]),
     "cles_memcpy_d2d": (C.c_int, [P, P, P, C.c_size_t]),
     "cles_plan_create": (C.c_int, [P, C.POINTER(SchemeDesc), C.c_int, C.POINTER(C.c_int64), C.c_int,
-                                   C.c_double, C.c_int, PP]),
+                                   C.c_double, C.c_int, C.c_int, C.c_int, PP]),
     "cles_plan_destroy": (C.c_int, [P]),
     "cles_apply_along": (C.c_int, [P, P, P, C.POINTER(C.c_int64), C.POINTER(C.c_int), P]),
     "cles_apply_along_timed": (C.c_int, [P, P, P, C.POINTER(C.c_int64), C.POINTER(C.c_int), C.c_int, DP]),

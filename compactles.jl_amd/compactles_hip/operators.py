@@ -46,17 +46,19 @@ class DirPlan:
     """Device plan for one scheme along one dimension (DirPlan / BandPlan)."""
 
     def __init__(self, ctx: Context, scheme: AbstractCompactScheme, dim: int, n_local, periodic: bool,
-                 h: float, dtype=np.float64):
+                 h: float, dtype=np.float64, lo_fold=None, hi_fold=None):
         self.ctx = ctx
         self.scheme = scheme
         self.dim = dim
         self.n_local = tuple(int(x) for x in n_local)
         self.dtype = np.dtype(dtype)
+        self.lo_fold, self.hi_fold = lo_fold, hi_fold
         desc = scheme.to_desc()
         self._desc = desc
         hp = C.c_void_p()
         check(ctx.lib.cles_plan_create(ctx.handle, C.byref(desc), dim, I64_3(*self.n_local), int(bool(periodic)),
-                                       float(h), F64 if self.dtype == np.float64 else F32, C.byref(hp)))
+                                       float(h), F64 if self.dtype == np.float64 else F32,
+                                       _fold_arg(lo_fold), _fold_arg(hi_fold), C.byref(hp)))
         self.handle = hp
 
     def __del__(self):
@@ -68,10 +70,23 @@ class DirPlan:
             pass
 
 
+def _fold_arg(v) -> int:
+    """The reference's lo_fold / hi_fold keyword (nothing, +1 or -1) as the ABI's int."""
+    if v is None:
+        return 0
+    if v not in (1, -1):
+        raise ValueError(f"a fold sign must be nothing, +1 or -1, got {v!r}")
+    return int(v)
+
+
 def plan_direction(ctx: Context, decomp: Decomp, scheme: AbstractCompactScheme, dim: int, h: float,
-                   dtype=np.float64) -> DirPlan:
-    """operators.jl:44-101 / operators_banded.jl:29-125."""
-    return DirPlan(ctx, scheme, dim, decomp.n_local, decomp.periodic[dim - 1], h, dtype)
+                   dtype=np.float64, lo_fold=None, hi_fold=None) -> DirPlan:
+    """operators.jl:44-101 / operators_banded.jl:29-125, with the parity-fold
+    keywords: lo_fold / hi_fold = sigma_g (the output's sign across a
+    half-offset mirror at that end) or None.  The edge rows read the mirrored
+    interior with the input's sign, sigma_g for a filter and -sigma_g for a
+    derivative (folds.jl:271-274), so no halo needs filling."""
+    return DirPlan(ctx, scheme, dim, decomp.n_local, decomp.periodic[dim - 1], h, dtype, lo_fold, hi_fold)
 
 
 def apply_along_(out: DeviceArray, plan: DirPlan, f: DeviceArray, decomp: Decomp, stream=None):

@@ -552,17 +552,26 @@ class SolverSpec:
         return self.origin[d] + self.coord_shift[d] + np.arange(self.n[d]) * self.h[d]
 
     def _dirichlet_state(self, bc: DirichletBC, d: int, side: int):
+        """The face's one conserved state.  The reference evaluates fun(x, t)
+        at every stage time (problem.jl:199-204); the device holds a constant
+        state, so fun is evaluated over the face at t = 0 and t = 1 and a state
+        that varies over the face or in time is refused."""
         xs = [self.xcoord(k) for k in range(3)]
         xs[d] = xs[d][:1] if side == 0 else xs[d][-1:]
         X, Y, Z = np.meshgrid(*xs, indexing="ij")
-        q = [np.broadcast_to(np.asarray(v, dtype=float), X.shape)
-             for v in conserved_from_prim(self.eos, bc.fun(X, Y, Z, 0.0))]
-        out = []
-        for v in q:
-            if np.ptp(v) != 0.0:
-                raise ValueError("DirichletBC on the GPU backend needs a state that is uniform over the face")
-            out.append(float(v.flat[0]))
-        return out
+        states = []
+        for t in (0.0, 1.0):
+            q = [np.broadcast_to(np.asarray(v, dtype=float), X.shape)
+                 for v in conserved_from_prim(self.eos, bc.fun(X, Y, Z, t))]
+            out = []
+            for v in q:
+                if np.ptp(v) != 0.0:
+                    raise ValueError("DirichletBC on the GPU backend needs a state that is uniform over the face")
+                out.append(float(v.flat[0]))
+            states.append(out)
+        if states[0] != states[1]:
+            raise ValueError("DirichletBC on the GPU backend needs a state that is constant in time")
+        return states[0]
 
     def footprint(self) -> int:
         """Device bytes the solver would allocate (cles_solver_footprint)."""

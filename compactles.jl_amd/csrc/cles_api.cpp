@@ -135,7 +135,7 @@ int cles_synchronize(cles_ctx *ctx)
 }
 
 int cles_plan_create(cles_ctx *ctx, const cles_scheme_desc *scheme, int dim, const int64_t n_local[3],
-                     int periodic, double h, int dtype, cles_plan **out)
+                     int periodic, double h, int dtype, int lo_fold, int hi_fold, cles_plan **out)
 {
     CLES_TRY({
         CLES_REQUIRE(ctx && scheme && n_local && out, "NULL argument");
@@ -145,6 +145,11 @@ int cles_plan_create(cles_ctx *ctx, const cles_scheme_desc *scheme, int dim, con
         for (int d = 0; d < 3; d++) CLES_REQUIRE(n_local[d] >= 1, "extents must be >= 1");
         CLES_REQUIRE(n_local[dim - 1] > 1, "plan along a collapsed dimension");
         CLES_REQUIRE(n_local[dim - 1] <= (1 << 20), "line longer than 2^20 points");
+        CLES_REQUIRE(lo_fold >= -1 && lo_fold <= 1 && hi_fold >= -1 && hi_fold <= 1,
+                     "lo_fold / hi_fold must be 0 (none), +1 or -1");
+        // a periodic dimension has no physical edge, so the keywords do nothing
+        // there (at_lo_edge / at_hi_edge, decomposition.jl:137-140)
+        if (periodic) lo_fold = hi_fold = 0;
         CLES_HIP(hipSetDevice(ctx->device));
         auto *p = new cles_plan;
         p->ctx = ctx;
@@ -152,7 +157,8 @@ int cles_plan_create(cles_ctx *ctx, const cles_scheme_desc *scheme, int dim, con
         p->hp.dtype = dtype;
         for (int d = 0; d < 3; d++) p->hp.nloc[d] = n_local[d];
         try {
-            cles::build_line_plan(p->hp, *scheme, (int)n_local[dim - 1], periodic, h, dim == 1 ? 16 : 32);
+            cles::build_line_plan(p->hp, *scheme, (int)n_local[dim - 1], periodic, h, dim == 1 ? 16 : 32,
+                                  lo_fold, hi_fold);
         } catch (...) {
             delete p;
             throw;

@@ -20,7 +20,7 @@
 extern "C" {
 #endif
 
-#define CLES_ABI_VERSION 4
+#define CLES_ABI_VERSION 5
 
 /* Status codes.  The solver-failure codes mirror SolverFailure reasons
  * (src/stepcontrol.jl:138-164): nonfinite, negative_density, planck, dt_min,
@@ -115,16 +115,27 @@ int cles_memcpy_d2d(cles_ctx *ctx, void *dst, const void *src, size_t bytes);
 /* Operator plugin: plan_direction / apply_along!                           */
 /* ----------------------------------------------------------------------- */
 
-/* plan_direction(decomp, scheme, dim, h) -> DirPlan | BandPlan
- * (src/operators.jl:44-101, src/operators_banded.jl:29-125), one rank:
- * whole-domain lines.  dim is 1-based (1 = x).  n_local are the interior
- * extents; `periodic` selects the cyclic system (otherwise both edges are
- * closed and the closure rows are used).  h is the grid spacing (derivative
- * RHS prescaled by 1/h).  dtype is CLES_F64 or CLES_F32.  Fails with
- * CLES_ERR_TOO_SMALL on the reference's extent check (operators.jl:50-53). */
+/* plan_direction(decomp, scheme, dim, h; lo_fold, hi_fold) -> DirPlan |
+ * BandPlan (src/operators.jl:44-101, src/operators_banded.jl:29-125), one
+ * rank: whole-domain lines.  dim is 1-based (1 = x).  n_local are the
+ * interior extents; `periodic` selects the cyclic system (otherwise both
+ * edges are closed and the closure rows are used).  h is the grid spacing
+ * (derivative RHS prescaled by 1/h).  dtype is CLES_F64 or CLES_F32.
+ * lo_fold / hi_fold: 0 for `nothing`, or the reference's keyword value
+ * sigma_g = +1 / -1, the output's parity across a half-offset mirror at that
+ * end of a closed line (operators.jl:55-70, operators_banded.jl:43-73): the
+ * LHS coupling to the ghost unknowns moves onto the mirrored interior
+ * columns with sign sigma_g.  The reference's RHS reads halos that the caller
+ * mirror-fills with the input's sign sigma_f (fold_fill!, folds.jl:83);
+ * here halos are never read, and the edge rows read the mirrored interior
+ * with sigma_f = sigma_g for a symmetric scheme (filter) and -sigma_g for a
+ * derivative, the pairing fold_apply! uses (folds.jl:271-274).  On a
+ * periodic line the fold arguments are ignored, as the reference's edge test
+ * is false there (decomposition.jl:137-140).  Fails with CLES_ERR_TOO_SMALL on
+ * the reference's extent check (operators.jl:50-53). */
 int cles_plan_create(cles_ctx *ctx, const cles_scheme_desc *scheme, int dim,
                      const int64_t n_local[3], int periodic, double h, int dtype,
-                     cles_plan **plan);
+                     int lo_fold, int hi_fold, cles_plan **plan);
 int cles_plan_destroy(cles_plan *plan);
 
 /* apply_along!(out, plan, f, decomp) (src/operators.jl:199-215): applies the

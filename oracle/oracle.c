@@ -248,6 +248,34 @@ or_plan *or_plan_create(const or_scheme *s, int dim, const int64_t n_local[3],
     return plan_create_fold(s, dim, n_local, halo, periodic, h, 0, 0, msg, msglen);
 }
 
+or_plan *or_plan_create_fold(const or_scheme *s, int dim, const int64_t n_local[3], const int halo[3],
+                             int periodic, double h, int lo_fold, int hi_fold, char *msg, int msglen)
+{
+    return plan_create_fold(s, dim, n_local, halo, periodic, h, lo_fold, hi_fold, msg, msglen);
+}
+
+void or_fold_fill_halos(double *f, const int64_t n_local[3], const int halo[3], int dim, int lo, int hi,
+                        int sigma)
+{
+    const int d = dim - 1, H = halo[d];
+    const int64_t n = n_local[d];
+    int64_t pad[3];
+    for (int k = 0; k < 3; k++) pad[k] = n_local[k] + 2 * halo[k];
+    const int64_t st = d == 0 ? 1 : d == 1 ? pad[0] : pad[0] * pad[1];
+    const int o1 = d == 0 ? 1 : 0, o2 = d == 2 ? 1 : 2;
+    for (int64_t b = 0; b < pad[o2]; b++)
+        for (int64_t a = 0; a < pad[o1]; a++) {
+            int64_t ii[3] = {0, 0, 0};
+            ii[o1] = a;
+            ii[o2] = b;
+            double *base = f + ii[0] + pad[0] * (ii[1] + pad[1] * ii[2]);
+            for (int q = 1; q <= H; q++) {
+                if (lo) base[(H - q) * st] = (double)sigma * base[(H + q - 1) * st];
+                if (hi) base[(H + n + q - 1) * st] = (double)sigma * base[(H + n - q) * st];
+            }
+        }
+}
+
 /* operators.jl:44-101 / operators_banded.jl:29-125 plan_direction with
  * lo_fold = sigma_g: the parity fold of a half-offset low edge moves the LHS
  * coupling to the ghost unknowns onto the mirrored interior columns, and the

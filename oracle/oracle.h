@@ -50,7 +50,18 @@ typedef struct or_plan or_plan;
 or_plan *or_plan_create(const or_scheme *s, int dim, const int64_t n_local[3],
                         const int halo[3], int periodic, double h,
                         char *msg, int msglen);
+/* plan_direction(...; lo_fold, hi_fold) (operators.jl:44-101,
+ * operators_banded.jl:29-125): lo_fold / hi_fold = sigma_g (+1 / -1) or 0 for
+ * `nothing`.  The RHS reads halos mirror-filled by or_fold_fill_halos. */
+or_plan *or_plan_create_fold(const or_scheme *s, int dim, const int64_t n_local[3],
+                             const int halo[3], int periodic, double h, int lo_fold, int hi_fold,
+                             char *msg, int msglen);
 void or_plan_destroy(or_plan *p);
+/* folds.jl:83-110 fold_fill!(f, decomp, d, lo, hi, sigma) on a padded scalar
+ * field, one rank: halo layer q beyond a folded end of dimension dim (1-based)
+ * holds sigma times interior layer q (half-offset mirror). */
+void or_fold_fill_halos(double *f, const int64_t n_local[3], const int halo[3], int dim, int lo, int hi,
+                        int sigma);
 /* apply_along!(out, plan, f, decomp): f, out padded arrays (halos of f current) */
 void or_apply_along(const or_plan *p, double *out, const double *f);
 

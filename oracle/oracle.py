@@ -93,6 +93,9 @@ def _load():
         "or_lele_d1_10": (None, [C.POINTER(Scheme)]),
         "or_compact_filter": (None, [C.POINTER(Scheme), C.c_double]),
         "or_plan_create": (P, [C.POINTER(Scheme), C.c_int, I64, I, C.c_int, C.c_double, C.c_char_p, C.c_int]),
+        "or_plan_create_fold": (P, [C.POINTER(Scheme), C.c_int, I64, I, C.c_int, C.c_double, C.c_int, C.c_int,
+                                    C.c_char_p, C.c_int]),
+        "or_fold_fill_halos": (None, [D, I64, I, C.c_int, C.c_int, C.c_int, C.c_int]),
         "or_plan_destroy": (None, [P]),
         "or_apply_along": (None, [P, D, D]),
         "or_wrap_halos": (None, [D, I64, I, I]),
@@ -182,15 +185,25 @@ def wrap_halos(f: np.ndarray, n_local, halo, periodic):
     return f
 
 
+def fold_fill(f: np.ndarray, n_local, halo, dim: int, lo: bool, hi: bool, sigma: int) -> np.ndarray:
+    """folds.jl:83-110 fold_fill! on a padded scalar field (one rank)."""
+    lib().or_fold_fill_halos(_dptr(f), (C.c_int64 * 3)(*[int(x) for x in n_local]), (C.c_int * 3)(*halo),
+                             int(dim), int(bool(lo)), int(bool(hi)), int(sigma))
+    return f
+
+
 class Plan:
     """apply_along! for one (scheme, dim) on padded arrays (one rank)."""
 
-    def __init__(self, sch: Scheme, dim: int, n_local, periodic: bool, h: float, halo=None):
+    def __init__(self, sch: Scheme, dim: int, n_local, periodic: bool, h: float, halo=None, lo_fold=0,
+                 hi_fold=0):
         self.n_local = tuple(int(x) for x in n_local)
         self.halo = tuple(halo) if halo is not None else halo_for(self.n_local)
+        self.dim, self.lo_fold, self.hi_fold = dim, int(lo_fold), int(hi_fold)
         msg = C.create_string_buffer(256)
-        self._p = lib().or_plan_create(C.byref(sch), dim, (C.c_int64 * 3)(*self.n_local),
-                                       (C.c_int * 3)(*self.halo), int(periodic), float(h), msg, 256)
+        self._p = lib().or_plan_create_fold(C.byref(sch), dim, (C.c_int64 * 3)(*self.n_local),
+                                            (C.c_int * 3)(*self.halo), int(periodic), float(h), self.lo_fold,
+                                            self.hi_fold, msg, 256)
         if not self._p:
             raise ValueError(msg.value.decode())
 

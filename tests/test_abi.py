@@ -34,7 +34,7 @@ def test_library_exports_every_declared_symbol():
 
 def test_abi_version():
     import compactles_hip as ch
-    assert ch.load().cles_abi_version() == 4
+    assert ch.load().cles_abi_version() == 5
 
 
 def test_struct_layout_matches_header():
@@ -123,3 +123,25 @@ def test_python_frontend_validation():
     q = ch.conserved_from_prim(eos, ch.Prim(u=(1.0, 2.0, 3.0), p=0.4, rho=2.0))
     # rho E = p/(gamma-1) + rho |u|^2 / 2
     assert np.allclose(q, [2.0, 2.0, 4.0, 6.0, 0.4 / 0.4 + 0.5 * 2.0 * 14.0], rtol=1e-15)
+
+
+def test_dirichlet_face_state_must_be_constant():
+    """The device holds one Dirichlet state per face: a state that varies over
+    the face or in time is refused at setup (problem.jl:192-207 evaluates
+    fun(x, t) at every stage time)."""
+    import compactles_hip as ch
+    from compactles_hip.problem import SolverSpec
+
+    def prob(fun):
+        bcs = ((ch.DirichletBC(fun), ch.DirichletBC(fun)), (ch.PeriodicBC(), ch.PeriodicBC()),
+               (ch.PeriodicBC(), ch.PeriodicBC()))
+        return ch.Problem(domain=((0.0, 1.0),) * 3, bcs=bcs,
+                          ic=lambda x, y, z: ch.Prim(p=1.0 + 0 * x, rho=1.0 + 0 * x))
+
+    num = ch.Numerics(n_global=(16, 8, 1))
+    ok = SolverSpec(prob(lambda x, y, z, t: ch.Prim(u=(0.5, 0.0, 0.0), p=1.0, rho=1.0)), num)
+    assert ok.desc.bc_state[0][0][1] == 0.5
+    with pytest.raises(ValueError, match="uniform over the face"):
+        SolverSpec(prob(lambda x, y, z, t: ch.Prim(p=1.0 + y, rho=1.0 + 0 * y)), num)
+    with pytest.raises(ValueError, match="constant in time"):
+        SolverSpec(prob(lambda x, y, z, t: ch.Prim(p=1.0 + t, rho=1.0)), num)

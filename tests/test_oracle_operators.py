@@ -260,3 +260,83 @@ def test_extent_validation_messages():
     """operators.jl:50-53: extent too small for the scheme."""
     with pytest.raises(ValueError, match="too small"):
         O.Plan(O.scheme("compact_filter"), 1, (8, 12, 12), False, 1.0)
+
+
+# ---- parity-folded plans (plan_direction's lo_fold / hi_fold) ---------------
+# A fold at a half-offset edge is the same operator as the unfolded one on the
+# line doubled by the mirror: the doubled data has parity sigma_f, so by
+# uniqueness its solution has parity sigma_g (= sigma_f for a filter, -sigma_f
+# for a derivative, folds.jl:271-274), and the rows at the mirror reduce to the
+# folded rows of operators.jl:59-70 / operators_banded.jl:55-73.  One folded
+# end doubles onto a line closed at both ends; two folded ends of the same sign
+# double onto a periodic line of 2n points.
+def fold_sigma_f(name, sigma_g):
+    return sigma_g if name == "compact_filter" else -sigma_g
+
+
+def folded_apply(name, dim, n, h, f_int, lo, hi):
+    """The oracle's folded plan on mirror-filled halos, as fold_apply! runs it."""
+    plan = O.Plan(O.scheme(name), dim, n, False, h, lo_fold=lo, hi_fold=hi)
+    sf = fold_sigma_f(name, lo or hi)
+    F = Hh.pad(f_int, n)
+    O.fold_fill(F, n, Hh.halo_for(n), dim, bool(lo), bool(hi), sf)
+    return Hh.interior(plan.apply(F), n)
+
+
+def doubled_apply(name, dim, n, h, f_int, lo, hi):
+    ax = dim - 1
+    sf = fold_sigma_f(name, lo or hi)
+    mirror = sf * np.flip(f_int, axis=ax)
+    if lo and hi:
+        ext, periodic, keep = np.concatenate([f_int, mirror], axis=ax), True, slice(0, n[ax])
+    elif lo:
+        ext, periodic, keep = np.concatenate([mirror, f_int], axis=ax), False, slice(n[ax], 2 * n[ax])
+    else:
+        ext, periodic, keep = np.concatenate([f_int, mirror], axis=ax), False, slice(0, n[ax])
+    n2 = list(n)
+    n2[ax] *= 2
+    got = apply(name, dim, tuple(n2), periodic, h, np.asfortranarray(ext))
+    idx = [slice(None)] * 3
+    idx[ax] = keep
+    return got[tuple(idx)]
+
+
+@pytest.mark.parametrize("name", ["lele_d1_6", "lele_d1_10", "pade_d1_4", "compact_filter"])
+@pytest.mark.parametrize("ends", [(1, 0), (0, 1), (1, 1)])
+@pytest.mark.parametrize("sigma_g", [1, -1])
+def test_folded_plan_equals_doubled_line(name, ends, sigma_g):
+    n = (18, 13, 11)
+    f = Hh.random_field(n, seed=7)
+    for dim in (1, 2, 3):
+        h = 0.37 / n[dim - 1]
+        lo, hi = sigma_g * ends[0], sigma_g * ends[1]
+        got = folded_apply(name, dim, n, h, f, lo, hi)
+        ref = doubled_apply(name, dim, n, h, f, lo, hi)
+        assert Hh.relerr(got, ref) <= 1e-13, (name, ends, sigma_g, dim, Hh.relerr(got, ref))
+
+
+def test_folded_plan_wrong_parity_is_caught():
+    """The doubled-line check separates the two parities: the plan of the other
+    sign differs at O(1) near the fold."""
+    n = (18, 13, 11)
+    f = Hh.random_field(n, seed=7)
+    h = 0.37 / 18
+    right = doubled_apply("lele_d1_6", 1, n, h, f, 1, 0)
+    plan = O.Plan(O.scheme("lele_d1_6"), 1, n, False, h, lo_fold=-1)
+    F = Hh.pad(f, n)
+    O.fold_fill(F, n, Hh.halo_for(n), 1, True, False, -1)
+    wrong = Hh.interior(plan.apply(F), n)
+    assert Hh.relerr(wrong, right) > 1e-2
+
+
+def test_fold_on_a_periodic_line_is_ignored_by_the_oracle():
+    """One rank of a periodic dimension has no physical edge (at_lo_edge is
+    false), so the keywords do nothing there (operators.jl:57-58)."""
+    n = (16, 9, 9)
+    f = Hh.fourier_field(n, kmax=3, seed=3)
+    h = TWO_PI / 16
+    a = O.Plan(O.scheme("lele_d1_6"), 1, n, True, h, lo_fold=1, hi_fold=1)
+    b = O.Plan(O.scheme("lele_d1_6"), 1, n, True, h)
+    F = Hh.pad(f, n)
+    O.wrap_halos(F, n, Hh.halo_for(n), (True,) * 3)
+    assert np.array_equal(a.apply(F), b.apply(F))
