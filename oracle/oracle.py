@@ -11,6 +11,7 @@ dimensions, 0 on collapsed ones), and the conserved state is
 """
 from __future__ import annotations
 
+import contextlib
 import ctypes as C
 import math
 import os
@@ -78,11 +79,12 @@ def _cpu_has_avx2() -> bool:
     return False
 
 
-def _load():
-    name = "liboracle_v3.so" if _cpu_has_avx2() else "liboracle.so"
-    path = os.path.join(_HERE, name)
-    if not os.path.exists(path):
-        path = os.path.join(_HERE, "liboracle.so")
+def _load(path=None):
+    if path is None:
+        name = "liboracle_v3.so" if _cpu_has_avx2() else "liboracle.so"
+        path = os.path.join(_HERE, name)
+        if not os.path.exists(path):
+            path = os.path.join(_HERE, "liboracle.so")
     if not os.path.exists(path):
         raise RuntimeError("oracle not built: run `make -C oracle`")
     lib = C.CDLL(path)
@@ -138,6 +140,19 @@ def lib():
     if _lib is None:
         _lib = _load()
     return _lib
+
+
+@contextlib.contextmanager
+def library(path):
+    """Within the block the oracle runs from another build of oracle.c (a test
+    plants a mistake in a scratch copy to show that a check catches it).
+    Objects made inside the block must be released inside it."""
+    global _lib
+    prev, _lib = _lib, _load(path)
+    try:
+        yield
+    finally:
+        _lib = prev
 
 
 class OwnedArray(np.ndarray):

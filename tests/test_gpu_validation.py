@@ -1,0 +1,182 @@
+"""The reference's own accuracy anchors on the device path:
+
+  * the one-dimensional shock-capturing battery (test/validation.jl:38-46:
+    Lax, Shu-Osher, Woodward-Colella, planar Noh) at the reference's measured
+    values, and each GPU profile against the oracle's after the same run;
+  * the Taylor-Green dissipation peak at 64^3 with the artificial properties
+    off: eps = 0.01265 at t = 9.13 (README.md:9-13, measured by
+    docs/figures/readme_header.jl:24-100 on one rank);
+  * the Taylor-Green peak -dKE/dt at 128^3 with them on: 1.2065e-2 at
+    t = 8.77 (bench/tgv_energy.jl:26-34, 501-step window).
+
+These pin the 3-D trajectory of the GPU path, and through its 1e-12 parity
+with the oracle over 100 steps (test_gpu_tgv64_golden.py) the oracle's too,
+by numbers the reference measured on itself.  The FP32-mixed form's values
+on the same runs are printed beside them."""
+import os
+import time
+
+import numpy as np
+import pytest
+
+import riemann
+import shock_cases as S
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    import compactles_hip as ch
+    return ch.Context(0)
+
+
+def _measure(name, x, r, u, p):
+    if name == "lax":
+        R, U, P = riemann.profile(x, S.LAX_T, 0.5, S.LAX_L, S.LAX_R, 1.4)
+        return {"rho": S.l1(r, R), "u": S.l1(u, U), "p": S.l1(p, P)}
+    if name == "shu_osher":
+        xr, rr, _, _ = S.read_ref(os.path.join(GOLD, "shu_osher.csv"))
+        ref = S.interp1(xr, rr, x)
+        b = S.so_band(x)
+        return {"L1": S.l1(r, ref), "train": S.l1(r[b], ref[b]), "peak": float(r[b].max())}
+    if name == "woodward":
+        xr, rr, _, _ = S.read_ref(os.path.join(GOLD, "woodward_colella.csv"))
+        i = int(np.argmax(r))
+        return {"L1": S.l1(r, S.interp1(xr, rr, x)), "peak": float(r[i]), "x_peak": float(x[i])}
+    plat, deficit, shock, epre = S.noh_metrics(x, r, 1)
+    return {"plateau": plat, "deficit": deficit, "shock": shock, "pre": epre}
+
+
+# the reference's measured values (test/validation.jl:38-46) and the half-unit
+# of their last quoted digit
+QUOTED = {
+    "lax": {"rho": (5.01e-3, 5e-6), "u": (7.53e-3, 5e-6), "p": (7.57e-3, 5e-6)},
+    "shu_osher": {"L1": (6.84e-3, 5e-6), "train": (2.08e-2, 5e-5), "peak": (4.682, 5e-4)},
+    "woodward": {"L1": (3.15e-2, 5e-5), "peak": (6.595, 1e-3), "x_peak": (0.7785, 5e-5)},
+    "noh1": {"plateau": (3.9968, 5e-5), "deficit": (0.58, 5e-3), "shock": (0.2043, 5e-5)},
+}
+CASES = {"lax": S.lax, "shu_osher": S.shu_osher, "woodward": S.woodward, "noh1": S.noh1}
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_shock_battery_fp64(ctx, name):
+    case = CASES[name]()
+    print(f"case {name}: n={case['n']}, tfinal={case['tfin']}", flush=True)
+    x, r, u, p, ok = S.gpu_run(ctx, case)
+    assert ok and np.all(np.isfinite(r))
+    got = _measure(name, x, r, u, p)
+    for k, (q, tol) in QUOTED[name].items():
+        assert abs(got[k] - q) <= tol * (1 + 1e-9), (name, k, got[k], q)
+    # against the oracle's run of the same case
+    xo, ro, uo, po, oko = S.oracle_run(case)
+    assert oko
+    diff = max(np.abs(r - ro).max() / np.abs(ro).max(), np.abs(p - po).max() / np.abs(po).max())
+    print(f"  {name}: GPU {got}; max relative difference to the oracle's profiles {diff:.2e}")
+    assert diff <= 1e-6, diff
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_shock_battery_fp32_mixed(ctx, name):
+    """The FP32-mixed form on the same cases: finite, complete, and within the
+    reference's own guards (test/validation.jl:114-188); its measurements are
+    printed beside the FP64 ones."""
+    case = CASES[name]()
+    x, r, u, p, ok = S.gpu_run(ctx, case, precision="fp32")
+    assert ok and np.all(np.isfinite(r)) and r.min() > 0
+    got = _measure(name, x, r, u, p)
+    print(f"  {name} FP32-mixed: {got}")
+    guards = {"lax": lambda g: g["rho"] < 1e-2 and g["u"] < 1.5e-2 and g["p"] < 1.5e-2,
+              "shu_osher": lambda g: g["L1"] < 1.5e-2 and g["train"] < 4e-2 and g["peak"] > 4.3,
+              "woodward": lambda g: g["L1"] < 6e-2 and 0.75 < g["x_peak"] < 0.80,
+              "noh1": lambda g: abs(g["plateau"] / 4 - 1) < 0.01 and 0 < g["deficit"] < 0.7 and
+              abs(g["shock"] - 0.2) < 0.025 and g["pre"] < 5e-2}
+    assert guards[name](got), got
+
+
+# ---- Taylor-Green dissipation anchors ----------------------------------------
+def _tgv3d_solver(ctx, n, art, precision="fp64"):
+    """docs/figures/readme_header.jl:24-49 and bench/tgv_energy.jl:171-181:
+    Re 1600, c0 = 10, the 3-D Taylor-Green velocity with its pressure
+    perturbation, cfl 0.6, filter every step."""
+    import compactles_hip as ch
+    g, c0 = 1.4, 10.0
+    p0 = c0 * c0 / g
+    prob = ch.Problem(domain=((0.0, 2 * np.pi),) * 3, bcs=((ch.PeriodicBC(), ch.PeriodicBC()),) * 3,
+                      ic=lambda x, y, z: ch.Prim(u=(np.sin(x) * np.cos(y) * np.cos(z),
+                                                    -np.cos(x) * np.sin(y) * np.cos(z), 0 * x),
+                                                 p=p0 + (np.cos(2 * x) + np.cos(2 * y)) * (np.cos(2 * z) + 2) / 16,
+                                                 rho=1.0 + 0 * x),
+                      eos=ch.single_species(gamma=g), transport=ch.Transport(mu0=1.0 / 1600))
+    num = ch.Numerics(n_global=(n, n, n), art=ch.ArtParams(enabled=art), cfl=0.6, filter_interval=1,
+                      precision=precision)
+    solver, Q = ch.setup(prob, num, ctx)
+    return solver
+
+
+def _readme_peak(ctx, precision="fp64"):
+    """readme_header.jl:51-100: mean KE every 10 steps to t = 12 (nmax
+    10000), centred differences, a 5-point moving mean, the argmax."""
+    s = _tgv3d_solver(ctx, 64, False, precision)
+    vol = (2 * np.pi) ** 3
+    times, energy = [0.0], [s.diagnostics()[0] / vol]
+
+    def record(solver, Q):
+        if solver.step_count % 10 == 0:
+            times.append(solver.t)
+            energy.append(solver.diagnostics()[0] / vol)
+    s.run(12.0, 10_000, callback=record)
+    if times[-1] < s.t:
+        times.append(s.t)
+        energy.append(s.diagnostics()[0] / vol)
+    t, e = np.array(times), np.array(energy)
+    d = np.empty_like(e)
+    d[0] = -(e[1] - e[0]) / (t[1] - t[0])
+    d[1:-1] = -(e[2:] - e[:-2]) / (t[2:] - t[:-2])
+    d[-1] = -(e[-1] - e[-2]) / (t[-1] - t[-2])
+    dm = np.array([d[max(0, i - 2):min(len(d), i + 3)].mean() for i in range(len(d))])
+    i = int(np.argmax(dm))
+    s.close()
+    return dm[i], t[i]
+
+
+def test_tgv64_dissipation_peak(ctx):
+    print("case TGV 64^3, art off, mu0 1/1600, to t = 12", flush=True)
+    t0 = time.time()
+    eps, tp = _readme_peak(ctx)
+    print(f"  FP64: peak eps {eps:.6e} at t = {tp:.4f} ({time.time() - t0:.1f} s); reference 0.01265 at 9.13")
+    assert abs(eps - 0.01265) <= 5e-6 and abs(tp - 9.13) <= 0.03, (eps, tp)
+    eps32, tp32 = _readme_peak(ctx, "fp32")
+    print(f"  FP32-mixed: peak eps {eps32:.6e} at t = {tp32:.4f} (relative to FP64 {abs(eps32 / eps - 1):.2e})")
+    assert abs(eps32 / eps - 1) <= 1e-3
+
+
+def test_tgv128_art_on_dissipation_peak(ctx):
+    """bench/tgv_energy.jl:148-176, 279-293: KE per unit volume after every
+    step to t = 10, -dKE/dt over a window of w = min(250, steps / 8) steps each
+    side, the last (landing) step excluded, the argmax."""
+    print("case TGV 128^3, art on, mu0 1/1600, to t = 10", flush=True)
+    t0 = time.time()
+    s = _tgv3d_solver(ctx, 128, True)
+    vol = (2 * np.pi) ** 3
+    ts, kes = [], []
+
+    def record(solver, Q):
+        ts.append(solver.t)
+        kes.append(solver.diagnostics()[0] / vol)
+    s.run(10.0, 2 ** 62, callback=record)
+    s.close()
+    ts, kes = np.array(ts), np.array(kes)
+    w = max(min(250, len(ts) // 8), 1)
+    last_full = max(len(ts) - 1, 2)
+
+    def rate(i):          # 1-based i as windowed_rate
+        lo, hi = max(i - w, 1), min(i + w, len(ts))
+        return -(kes[hi - 1] - kes[lo - 1]) / (ts[hi - 1] - ts[lo - 1])
+    rates = np.array([rate(i) for i in range(2, last_full + 1)])
+    imax = int(np.argmax(rates)) + 2
+    peak, tpk = rates[imax - 2], ts[imax - 1]
+    print(f"  peak -dKE/dt {peak:.5e} at t = {tpk:.3f} ({len(ts)} steps, {time.time() - t0:.1f} s); "
+          "reference 1.2065e-2 at 8.77")
+    assert abs(peak - 1.2065e-2) <= 5e-6 and abs(tpk - 8.77) <= 0.02, (peak, tpk)
