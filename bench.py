@@ -26,7 +26,7 @@ import sys
 import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
-sys.path[:0] = [os.path.join(ROOT, "compactles.jl_amd"), os.path.join(ROOT, "oracle"), os.path.join(ROOT, "tests")]
+sys.path[:0] = [os.path.join(ROOT, "compactles.jl_amd")]
 
 HBM_PEAK_GBPS = 8000.0
 # the kernel with the largest share of the step's kernel time in each form
@@ -36,15 +36,8 @@ DOMINANT_KERNEL = {"fp64": "cles::k_fluxes<double, true>", "fp32": "cles::k_line
 
 
 def tgv_problem(ch, n, precision):
-    import numpy as np
-
-    def ic(X, Y, Z):
-        return ch.Prim(u=(np.sin(X) * np.cos(Y), -np.cos(X) * np.sin(Y), 0.0 * X), p=71.43, rho=1.0)
-
-    prob = ch.Problem(domain=((0.0, 2 * np.pi),) * 3, bcs=((ch.PeriodicBC(), ch.PeriodicBC()),) * 3, ic=ic,
-                      name="taylor-green")
-    num = ch.Numerics(n_global=(n, n, n), precision=precision, cfl=0.5)
-    return prob, num
+    from compactles_hip import workloads
+    return workloads.tgv_problem(ch, n, precision)
 
 
 def read_traffic(n, precision, kernel):
@@ -57,24 +50,43 @@ def read_traffic(n, precision, kernel):
     return float(m.group(1)) if m else None
 
 
-def cpu_baseline(n, steps=1):
-    """The oracle on the host cores: one run! step of the same workload."""
-    import numpy as np
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
-    import cases
+
+def cpu_baseline(n, steps=3, warmup=1):
+    """The oracle (the C restatement of the reference CPU path, the checker of
+    the tests, loaded only here) on the host cores: `warmup` untimed and
+    `steps` timed run! steps of the same workload."""
+    import numpy as np
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+
+    from compactles_hip import workloads as W
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     L = (2 * np.pi,) * 3
-    X, Y, Z = cases.grid((n, n, n), L)
-    q = cases.conserved(cases.tgv_prim(X, Y, Z))
+    X, Y, Z = W.grid((n, n, n), L)
+    q = W.conserved(W.tgv_prim(X, Y, Z))
     del X, Y, Z
-    s = cases.oracle_solver((n, n, n), L, q, art=True, cfl=0.5)
+    s = O.Solver(O.make_desc((n, n, n), L, art=True, cfl=0.5))
+    for c, qc in enumerate(q):
+        s.Q[s.halo[0]:s.halo[0] + n, s.halo[1]:s.halo[1] + n, s.halo[2]:s.halo[2] + n, c] = qc
     del q
+    s.run(1e9, warmup)
     t0 = time.perf_counter()
     s.run(1e9, steps)
     dt = time.perf_counter() - t0
     return {"value": n ** 3 * steps / dt / 1e6, "unit": "Mpt-steps/s", "cores": threads, "kind": "port",
-            "sample": f"oracle (C restatement of the reference CPU path, OpenMP, FP64) TGV {n}^3, "
-                      f"{steps} full run! step(s), {dt:.2f} s"}
+            "cpu_model": _cpu_model(), "host_cpus": os.cpu_count(),
+            "sample": f"oracle (C restatement of the reference CPU path, OpenMP, {threads} threads, FP64) TGV "
+                      f"{n}^3, {steps} full run! steps after {warmup} untimed, {dt:.2f} s"}
 
 
 def counters_run(ch, ctx, n, steps, precision):
@@ -175,11 +187,11 @@ def measure_tubes(ch, ctx, count, steps, warmup, precision, plain_sample=32):
     tubes (N_i ~ U{128..2048}, p_L ~ U[2, 20], rho_L ~ U[0.1, 10] from one
     seeded generator; He | CO2, Dirichlet ends, art on, cfl 0.4, filter every
     step) advanced together, each with its own dt."""
-    import cases
-    specs = cases.tube_specs(count, seed=20261202)
+    from compactles_hip import workloads as W
+    specs = W.tube_specs(count, seed=20261202)
     npts = sum(sp["n"] for sp in specs)
     t0 = time.perf_counter()
-    b = cases.tube_batch(ctx, specs, precision=precision)
+    b = W.tube_batch(ctx, specs, precision=precision)
     setup_s = time.perf_counter() - t0
     b.run(max_steps=warmup)
     t0 = time.perf_counter()
@@ -194,7 +206,7 @@ def measure_tubes(ch, ctx, count, steps, warmup, precision, plain_sample=32):
     sample = specs[:plain_sample]
     t_plain, pts = 0.0, 0
     for sp in sample:
-        s1 = cases.tube_solver(ctx, sp, precision)
+        s1 = W.tube_solver(ctx, sp, precision)
         s1.run(1e9, warmup)
         ctx.synchronize()
         t0 = time.perf_counter()
@@ -222,22 +234,12 @@ def measure_shock_2d(ch, ctx, precision, steps, warmup, n=(1024, 512, 1), tune=T
     1024 x 512 (two species, NSCBC inflow and outflow in x, periodic y, art on,
     cfl 0.4, filter every step).  With tune, the line tiles are tuned for the
     grid first (outside the timed region)."""
-    import cases
-    q, L, faces = cases.shock_interface(n)
-    eos = ch.IdealMixture([ch.IdealSpecies(f"s{k}", R, g) for k, (R, g) in enumerate(cases.HE_CO2)])
-    bcs = ((cases.nscbc_bc(ch, faces[(0, 0)]), cases.nscbc_bc(ch, faces[(0, 1)])),
-           (ch.PeriodicBC(), ch.PeriodicBC()), (ch.PeriodicBC(), ch.PeriodicBC()))
-    prob = ch.Problem(domain=tuple((0.0, L[k]) for k in range(3)), bcs=bcs, ic=lambda X, Y, Z: None, eos=eos,
-                      transport=ch.Transport(mu0=0.0))
-    num = ch.Numerics(n_global=n, cfl=0.4, precision=precision)
+    from compactles_hip import workloads as W
+    prob, num, Q = W.shock_problem(ch, n, precision)
     if tune:
         ch.tune.tune(ctx, ch.SolverSpec(prob, num), reps=5)
     tiles = ch.tune.lookup(ch.SolverSpec(prob, num))
     s = ch.Solver(ctx, prob, num)
-    Q = s.allocate_state()
-    sl = tuple(slice(s.halo[k], s.halo[k] + n[k]) for k in range(3))
-    for c, qc in enumerate(q):
-        Q[sl + (c,)] = qc
     s.upload(Q)
     s.run(1e9, warmup)
     ctx.synchronize()
@@ -248,7 +250,7 @@ def measure_shock_2d(ch, ctx, precision, steps, warmup, n=(1024, 512, 1), tune=T
     s.close()
     npts = n[0] * n[1] * n[2]
     ms = wall / steps * 1e3
-    # Appendix B: S = 2, D = 2, NSCBC inflow (p, rho) and outflow (p normal and transverse) line ops
+    # byte model with S = 2, D = 2 and the NSCBC inflow (p, rho) and outflow (p normal and transverse) line ops
     model = alg_bytes_per_point_step(2, 2, precision, nscbc_ops=4)
     return {"value": round(npts * steps / wall / 1e6, 3), "unit": "Mpt-steps/s", "ms_per_step": round(ms, 4),
             "step_roofline": {"alg_bytes_per_point_step": model,
@@ -256,6 +258,43 @@ def measure_shock_2d(ch, ctx, precision, steps, warmup, n=(1024, 512, 1), tune=T
                               "frac": round(model * npts / (ms * 1e-3) / 1e9 / HBM_PEAK_GBPS, 4)},
             "line_tiles": {"chunks": list(tiles[0]), "tuned": list(tiles[1])},
             "workload": f"He|CO2 shock / interface {n[0]}x{n[1]}, NSCBC in / out, art on, cfl 0.4"}
+
+
+def measure_operators(ch, ctx, n=256, iters=20, seed=20261202):
+    """The operator form: one compact-operator application (apply_along!) over
+    an n^3 periodic field of seeded Fourier modes (|k| <= 32, amplitude
+    |k|^(-5/3), unit RMS) for the C6 and C10 first derivatives and the C8
+    filter along x, y and z, in FP64 and in FP32 on the same input (cast once).
+    Each is timed with HIP events over `iters` back-to-back launches; the
+    algorithmic bytes are one read and one write of the field (16 B/pt FP64,
+    8 B/pt FP32)."""
+    import numpy as np
+
+    from compactles_hip import workloads as W
+    f64 = W.fourier_field((n, n, n), kmax=32, seed=seed)
+    dec = ch.Decomp((n, n, n), (True, True, True), n_halo=0)
+    h = 2 * np.pi / n
+    out = {}
+    for prec, dt in (("fp64", np.float64), ("fp32", np.float32)):
+        f = ch.DeviceArray.from_numpy(ctx, f64, dt)
+        o = ch.DeviceArray(ctx, (n, n, n), dt)
+        esz = np.dtype(dt).itemsize
+        for name, sch in (("C6", ch.lele_d1_6()), ("C10", ch.lele_d1_10()), ("C8_filter", ch.compact_filter())):
+            row = {}
+            for dim, ax in ((1, "x"), (2, "y"), (3, "z")):
+                plan = ch.plan_direction(ctx, dec, sch, dim, h, dtype=dt)
+                ch.apply_along_timed(o, plan, f, dec, 3)
+                ms = ch.apply_along_timed(o, plan, f, dec, iters)
+                gbps = 2 * esz * n ** 3 / (ms * 1e-3) / 1e9
+                row[ax] = {"us": round(ms * 1e3, 2), "GBps": round(gbps, 1), "frac": round(gbps / HBM_PEAK_GBPS, 4)}
+                del plan
+            out.setdefault(name, {})[prec] = row
+        f.free()
+        o.free()
+    return {"per_operator": out, "alg_bytes_per_point": {"fp64": 16, "fp32": 8},
+            "workload": f"{n}^3 periodic field of seeded Fourier modes (|k| <= 32, |k|^-5/3, seed {seed}), "
+                        "one apply_along! per operator and direction, timed over "
+                        f"{iters} launches with HIP events"}
 
 
 def measure_serving(ch, ctx=None, count=32, rate_per_s=1.5, seed=20261202, budget=64 << 30, runners=4):
@@ -266,7 +305,7 @@ def measure_serving(ch, ctx=None, count=32, rate_per_s=1.5, seed=20261202, budge
     context, the line tiles of every distinct shape are tuned first."""
     import numpy as np
 
-    import cases
+    from compactles_hip import workloads as W
     rng = np.random.default_rng(seed)
     arrivals = np.cumsum(rng.exponential(1.0 / rate_per_s, count))
     kinds = []
@@ -282,18 +321,7 @@ def measure_serving(ch, ctx=None, count=32, rate_per_s=1.5, seed=20261202, budge
             spec = ch.SolverSpec(prob, num)
             reqs.append((prob, num, ch.initialize_(spec, spec.allocate_state(), prob.ic), 0.05))
         else:
-            q, L, faces = cases.shock_interface(n)
-            eos = ch.IdealMixture([ch.IdealSpecies(f"s{k}", R, g) for k, (R, g) in enumerate(cases.HE_CO2)])
-            bcs = ((cases.nscbc_bc(ch, faces[(0, 0)]), cases.nscbc_bc(ch, faces[(0, 1)])),
-                   (ch.PeriodicBC(), ch.PeriodicBC()), (ch.PeriodicBC(), ch.PeriodicBC()))
-            prob = ch.Problem(domain=tuple((0.0, L[k]) for k in range(3)), bcs=bcs, ic=lambda X, Y, Z: None,
-                              eos=eos, transport=ch.Transport(mu0=0.0))
-            num = ch.Numerics(n_global=n, cfl=0.4)
-            spec = ch.SolverSpec(prob, num)
-            Q = spec.allocate_state()
-            sl = tuple(slice(spec.halo[k], spec.halo[k] + n[k]) for k in range(3))
-            for c, qc in enumerate(q):
-                Q[sl + (c,)] = qc
+            prob, num, Q = W.shock_problem(ch, n)
             reqs.append((prob, num, Q, 0.03))
     tuned = 0
     if ctx is not None:
@@ -358,7 +386,7 @@ def main():
     ap.add_argument("--tubes", type=int, default=4096, help="instances of the batched form")
     ap.add_argument("--run-mode", choices=["step_graph", "device_dt", "host_dt"], default="host_dt",
                     help="how run! issues its steps (identical results)")
-    ap.add_argument("--only", choices=["tubes", "serving", "shock2d"], default=None,
+    ap.add_argument("--only", choices=["tubes", "serving", "shock2d", "operators"], default=None,
                     help="measure only the batched-tubes (FP64) or serving form and print its JSON (profiling runs)")
     ap.add_argument("--cpu-n", type=int, default=256)
     ap.add_argument("--profile-steps", type=int, default=2, help="steps of the per-phase event pass (>= 1)")
@@ -370,7 +398,7 @@ def main():
         ap.error("--profile-steps must be at least 1 (the roofline figures come from that pass)")
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
-              "(the multi-GPU sharding is a design in DESIGN.md, not a measured path)", file=sys.stderr)
+              "(sharding over GPUs is designed, not built or measured)", file=sys.stderr)
         sys.exit(2)
 
     import compactles_hip as ch
@@ -387,6 +415,9 @@ def main():
         return
     if args.only == "serving":
         print(json.dumps(measure_serving(ch, ctx)), flush=True)
+        return
+    if args.only == "operators":
+        print(json.dumps(measure_operators(ch, ctx, n)), flush=True)
         return
     primary = measure(ch, ctx, n, args.precision, args.steps, args.warmup, args.profile_steps, args.run_mode)
     out = {
@@ -426,6 +457,7 @@ def main():
         out["forms"]["shock_interface_2d"] = {p: measure_shock_2d(ch, ctx, p, args.steps, args.warmup)
                                               for p in ("fp64", "fp32")}
         out["forms"]["serving"] = measure_serving(ch, ctx)
+        out["forms"]["operators"] = measure_operators(ch, ctx, n)
     ctx.close()
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(args.cpu_n)

@@ -3,7 +3,7 @@ plugin interface, driving the gfx950 kernels of libcompactles_hip.so through
 its C ABI."""
 from ._lib import ClesError, SolverFailure, load  # noqa: F401
 from .device import Context, DeviceArray  # noqa: F401
-from .operators import Decomp, DirPlan, apply_along_, field, filter_field_, plan_direction  # noqa: F401
+from .operators import Decomp, DirPlan, apply_along_, apply_along_timed, field, filter_field_, plan_direction  # noqa: F401
 from .schemes import (BandedClosureRow, BandedCompactScheme, ClosureRow, CompactScheme,  # noqa: F401
                       compact_filter, lele_d1_10, lele_d1_6, pade_d1_4)
 from .problem import (ArtParams, AxisBC, ConstantBodyForce, DirichletBC, ExtrapolationBC, NoSlipWallBC, OriginBC, PoleBC, SwitchableBC,
