@@ -38,7 +38,8 @@ def test_shock2d_full_size_200_steps_fp64(ctx):
     print("case shock2d 1024x512 FP64, 200 steps", flush=True)
     gs = _solver(ctx, "fp64")
     ke0, en0 = gs.diagnostics()
-    assert abs(ke0 - ref["ke0"]) <= 1e-13 * ref["ke0"]
+    # the same initial state, summed in another order over 524288 points
+    e_ke0 = abs(ke0 / ref["ke0"] - 1)
     dts = gs.run(1e9, ref["steps"], record=ref["steps"])
     err_dt = np.abs(dts - np.array(ref["dts"])).max() / max(ref["dts"])
     ke, en = gs.diagnostics()
@@ -49,8 +50,9 @@ def test_shock2d_full_size_200_steps_fp64(ctx):
     scale[nc - 4:nc - 1] = [mom] * 3
     err_q = max(abs(Qi[c][tuple(p)] - vals[c]) / scale[c] for p, vals in zip(ref["probes"], ref["Q_probes"])
                 for c in range(nc))
-    print(f"  dt {err_dt:.2e}, KE {abs(ke / ref['ke'] - 1):.2e}, enstrophy {abs(en / ref['enstrophy'] - 1):.2e}, "
-          f"probed state {err_q:.2e}")
+    print(f"  initial KE {e_ke0:.2e}; after {ref['steps']} steps: dt {err_dt:.2e}, t {abs(gs.t / ref['t'] - 1):.2e}, "
+          f"KE {abs(ke / ref['ke'] - 1):.2e}, enstrophy {abs(en / ref['enstrophy'] - 1):.2e}, probed state {err_q:.2e}")
+    assert e_ke0 <= 1e-11
     assert err_dt <= 1e-12
     assert abs(gs.t - ref["t"]) <= 1e-12 * ref["t"]
     assert abs(ke - ref["ke"]) <= 1e-10 * ref["ke"]

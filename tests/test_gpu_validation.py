@@ -6,8 +6,11 @@
   * the Taylor-Green dissipation peak at 64^3 with the artificial properties
     off: eps = 0.01265 at t = 9.13 (README.md:9-13, measured by
     docs/figures/readme_header.jl:24-100 on one rank);
-  * the Taylor-Green peak -dKE/dt at 128^3 with them on: 1.2065e-2 at
-    t = 8.77 (bench/tgv_energy.jl:26-34, 501-step window).
+  * the Taylor-Green windowed peak -dKE/dt at 128^3 to t = 10 with the
+    artificial properties on (1.2044e-2 at t = 9.06 in 12739 steps) and off
+    (1.2153e-2 at t = 9.00 in 10830 steps), from the windowed table of
+    reference/CALIBRATION.md:216-221 (bench/tgv_energy.jl's estimator; the
+    1.2065e-2 at 8.77 quoted beside it predates the window, :225-230, 323-331).
 
 These pin the 3-D trajectory of the GPU path, and through its 1e-12 parity
 with the oracle over 100 steps (test_gpu_tgv64_golden.py) the oracle's too,
@@ -69,12 +72,23 @@ def test_shock_battery_fp64(ctx, name):
     got = _measure(name, x, r, u, p)
     for k, (q, tol) in QUOTED[name].items():
         assert abs(got[k] - q) <= tol * (1 + 1e-9), (name, k, got[k], q)
-    # against the oracle's run of the same case
+    # against the oracle's run of the same case, measured on the scale of the
+    # case's own conditioning: thousands of steps through the shock sensors
+    # amplify last-bit differences (the reference reproduces itself to four
+    # figures, test/validation.jl:48-52), so the bar is the oracle's response to
+    # its own input changed by one ulp
     xo, ro, uo, po, oko = S.oracle_run(case)
     assert oko
-    diff = max(np.abs(r - ro).max() / np.abs(ro).max(), np.abs(p - po).max() / np.abs(po).max())
-    print(f"  {name}: GPU {got}; max relative difference to the oracle's profiles {diff:.2e}")
-    assert diff <= 1e-6, diff
+    pert = CASES[name]()
+    pert["q"] = [q * (1 + 2.0 ** -52) for q in pert["q"]]
+    _, rs, _, ps, _ = S.oracle_run(pert)
+
+    def l1rel(a, b, c, d):
+        return max(S.l1(a, b) / np.abs(b).mean(), S.l1(c, d) / np.abs(d).mean())
+    dl1, sens = l1rel(r, ro, p, po), l1rel(rs, ro, ps, po)
+    print(f"  {name}: GPU {got}; L1 relative difference to the oracle's profiles {dl1:.2e} "
+          f"(the oracle against itself from a 1-ulp change of its input: {sens:.2e})")
+    assert dl1 <= max(10 * sens, 1e-12), (dl1, sens)
 
 
 @pytest.mark.parametrize("name", list(CASES))
@@ -152,13 +166,17 @@ def test_tgv64_dissipation_peak(ctx):
     assert abs(eps32 / eps - 1) <= 1e-3
 
 
-def test_tgv128_art_on_dissipation_peak(ctx):
+@pytest.mark.parametrize("art,ref_peak,ref_t,ref_steps", [(True, 1.2044e-2, 9.06, 12739),
+                                                           (False, 1.2153e-2, 9.00, 10830)])
+def test_tgv128_windowed_dissipation_peak(ctx, art, ref_peak, ref_t, ref_steps):
     """bench/tgv_energy.jl:148-176, 279-293: KE per unit volume after every
     step to t = 10, -dKE/dt over a window of w = min(250, steps / 8) steps each
-    side, the last (landing) step excluded, the argmax."""
-    print("case TGV 128^3, art on, mu0 1/1600, to t = 10", flush=True)
+    side, the last (landing) step excluded, the argmax.  The step count to
+    t = 10 integrates the whole dt history; the reference states ~0.4 % for
+    the estimator's own uncertainty on the peak (CALIBRATION.md:279-284)."""
+    print(f"case TGV 128^3, art {'on' if art else 'off'}, mu0 1/1600, to t = 10", flush=True)
     t0 = time.time()
-    s = _tgv3d_solver(ctx, 128, True)
+    s = _tgv3d_solver(ctx, 128, art)
     vol = (2 * np.pi) ** 3
     ts, kes = [], []
 
@@ -177,6 +195,12 @@ def test_tgv128_art_on_dissipation_peak(ctx):
     rates = np.array([rate(i) for i in range(2, last_full + 1)])
     imax = int(np.argmax(rates)) + 2
     peak, tpk = rates[imax - 2], ts[imax - 1]
-    print(f"  peak -dKE/dt {peak:.5e} at t = {tpk:.3f} ({len(ts)} steps, {time.time() - t0:.1f} s); "
-          "reference 1.2065e-2 at 8.77")
-    assert abs(peak - 1.2065e-2) <= 5e-6 and abs(tpk - 8.77) <= 0.02, (peak, tpk)
+    # the same rate read at the sampled steps of the split table (every 100
+    # steps, bench/tgv_energy.jl:165-170, 309-320)
+    samp = [(rate(i), ts[i - 1]) for i in range(100, len(ts), 100) if 2 <= i < len(ts)]
+    speak, stpk = max(samp)
+    print(f"  peak -dKE/dt {peak:.5e} at t = {tpk:.3f} over every step, {speak:.5e} at t = {stpk:.3f} over the "
+          f"100-step samples ({len(ts)} steps, {time.time() - t0:.1f} s); reference {ref_peak:.4e} at {ref_t:.2f} "
+          f"in {ref_steps} steps")
+    assert abs(len(ts) - ref_steps) <= 2
+    assert abs(peak / ref_peak - 1) <= 4e-3 and abs(tpk - ref_t) <= 0.15, (peak, tpk)
