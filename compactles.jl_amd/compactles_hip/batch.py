@@ -118,15 +118,17 @@ class Batch:
         return out
 
     # ---- the per-step path ---------------------------------------------------
-    def run(self, max_steps: int = 2 ** 62, record: int = 0):
+    def run(self, max_steps: int = 2 ** 62, record: int = 0, noalloc: bool = False):
         """run! on every instance (each to its own tfinal / nmax).  Returns the
         number of batch iterations, and with record > 0 also the (count x
-        record) dt history of this call."""
+        record) dt history of this call.  noalloc: through the entry point
+        that allocates no device memory (cles_batch_run_noalloc)."""
         cnt = len(self.instances)
         dts = np.zeros((cnt, max(record, 1)))
         taken = C.c_int64()
-        L.check(self.ctx.lib.cles_batch_run(self.handle, int(max_steps), int(record),
-                                            dts.ctypes.data_as(L.DP) if record else None, C.byref(taken)))
+        fn = self.ctx.lib.cles_batch_run_noalloc if noalloc else self.ctx.lib.cles_batch_run
+        L.check(fn(self.handle, int(max_steps), int(record), dts.ctypes.data_as(L.DP) if record else None,
+                   C.byref(taken)))
         return (taken.value, dts[:, :record]) if record else taken.value
 
     def state(self):

@@ -46,6 +46,7 @@ int cles_ctx_create(int device, cles_ctx **out)
         if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
             throw cles::Error{CLES_ERR_UNSUPPORTED,
                               std::string("this library is built for gfx950, device is ") + prop.gcnArchName};
+        cles::noalloc_setup(device);
         auto *c = new cles_ctx;
         c->device = device;
         hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);

@@ -685,12 +685,20 @@ int cles_batch_download(cles_batch *b, int64_t i, double *Qh, const int64_t padd
     })
 }
 
-int cles_batch_run(cles_batch *b, int64_t max_steps, int64_t nrec, double *dts, int64_t *taken)
+namespace {
+
+// run! on every instance (cles_batch_run); with `noalloc` the dt history is
+// staged in the per-device array of cles_noalloc.hip instead of a device
+// allocation of this call.
+void batch_run(cles_batch *b, int64_t max_steps, int64_t nrec, double *dts, int64_t *taken, bool noalloc)
 {
-    CLES_TRY({
+    {
         CLES_REQUIRE(b, "NULL argument");
         CLES_REQUIRE(nrec >= 0 && (nrec == 0 || dts), "dt history needs a host buffer");
         const size_t cnt = (size_t)b->count;
+        if (noalloc && nrec > 0 && (int64_t)cnt > (int64_t)CLES_NOALLOC_DTS_MAX / nrec)
+            throw cles::Error{CLES_ERR_INVALID, "count * nrec exceeds the allocation-free dt history "
+                                                "(CLES_NOALLOC_DTS_MAX doubles)"};
         // the run! entry of every instance (timestep.jl:281-344): fresh
         // workspace, dt_seen = 0, active while t < tfinal and step < nmax
         std::vector<double> t(cnt), tf(cnt);
@@ -713,7 +721,10 @@ int cles_batch_run(cles_batch *b, int64_t max_steps, int64_t nrec, double *dts, 
         CLES_HIP(hipMemsetAsync(b->du, 0, (size_t)b->N * b->C * sizeof(double), b->st()));
         CLES_HIP(hipMemsetAsync(b->dQ, 0, (size_t)b->N * b->C * b->wsize(), b->st()));
         double *ddts = nullptr;
-        if (nrec > 0) {
+        if (nrec > 0 && noalloc) {
+            CLES_HIP(hipSetDevice(b->ctx->device));
+            ddts = cles::noalloc_dts(b->ctx->device, (int64_t)cnt * nrec, b->st());
+        } else if (nrec > 0) {
             CLES_HIP(hipMalloc(&ddts, cnt * (size_t)nrec * sizeof(double)));
             CLES_HIP(hipMemsetAsync(ddts, 0, cnt * (size_t)nrec * sizeof(double), b->st()));
         }
@@ -736,15 +747,27 @@ int cles_batch_run(cles_batch *b, int64_t max_steps, int64_t nrec, double *dts, 
             });
             if (ddts) {
                 CLES_HIP(hipMemcpy(dts, ddts, cnt * (size_t)nrec * sizeof(double), hipMemcpyDeviceToHost));
-                (void)hipFree(ddts);
+                if (!noalloc) (void)hipFree(ddts);
                 ddts = nullptr;
             }
         } catch (...) {
-            if (ddts) (void)hipFree(ddts);
+            if (ddts && !noalloc) (void)hipFree(ddts);
             throw;
         }
         if (taken) *taken = steps;
-    })
+    }
+}
+
+}  // namespace
+
+int cles_batch_run(cles_batch *b, int64_t max_steps, int64_t nrec, double *dts, int64_t *taken)
+{
+    CLES_TRY({ batch_run(b, max_steps, nrec, dts, taken, false); })
+}
+
+int cles_batch_run_noalloc(cles_batch *b, int64_t max_steps, int64_t nrec, double *dts, int64_t *taken)
+{
+    CLES_TRY({ batch_run(b, max_steps, nrec, dts, taken, true); })
 }
 
 int cles_batch_state(cles_batch *b, double *t, int64_t *steps, int *status)

@@ -17,6 +17,13 @@ namespace cles {
 // Errors
 // ---------------------------------------------------------------------------
 void set_error(const std::string &msg);
+
+// The allocation-free entry points' per-device array (cles_noalloc.hip):
+// noalloc_setup looks its address up on `device` (idempotent); noalloc_dts
+// returns it cleared over `doubles` entries on `stream`, or throws
+// CLES_ERR_INVALID beyond CLES_NOALLOC_DTS_MAX.
+void noalloc_setup(int device);
+double *noalloc_dts(int device, int64_t doubles, hipStream_t stream);
 struct Error {
     int code;
     std::string msg;

@@ -442,6 +442,19 @@ int cles_batch_download(cles_batch *batch, int64_t i, double *Q_host,
  * receives each instance's dt sequence of this call when nrec > 0.
  * *steps_taken: batch iterations issued. */
 int cles_batch_run(cles_batch *batch, int64_t max_steps, int64_t nrec, double *dts, int64_t *steps_taken);
+/* cles_batch_run without any device allocation.  No counterpart in the
+ * reference; it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same arguments, kernels and results as cles_batch_run; the dt history
+ * (count x nrec doubles) is staged in a per-device array of
+ * CLES_NOALLOC_DTS_MAX doubles (8 MiB) that each GPU the library runs on holds
+ * until the process ends, cleared at the start of every call.  count * nrec
+ * beyond CLES_NOALLOC_DTS_MAX is CLES_ERR_INVALID.  All calls on one GPU share
+ * that array, so calls that overlap on one GPU (from several threads or
+ * batches at once) race. */
+#define CLES_NOALLOC_DTS_MAX (1 << 20)
+int cles_batch_run_noalloc(cles_batch *batch, int64_t max_steps, int64_t nrec, double *dts,
+                           int64_t *steps_taken);
 /* Per-instance time, completed steps and status (CLES_OK or the check_step
  * code 4..8 it stopped with); NULL arrays are skipped. */
 int cles_batch_state(cles_batch *batch, double *t, int64_t *steps, int *status);
