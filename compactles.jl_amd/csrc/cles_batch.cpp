@@ -663,6 +663,8 @@ int cles_batch_upload(cles_batch *b, int64_t i, const double *Qh, const int64_t 
         CLES_REQUIRE(b && Qh && padded && halo, "NULL argument");
         cles::check_inst(b, i);
         CLES_REQUIRE(halo[0] >= 0 && padded[0] >= b->n[i] + 2 * (int64_t)halo[0], "padded extents smaller than n + 2*halo");
+        CLES_REQUIRE(padded[1] == 1 && padded[2] == 1 && halo[1] == 0 && halo[2] == 0,
+                     "a batch instance is laid out as (n + 2*halo, 1, 1) with halo (H, 0, 0)");
         const int64_t NP = padded[0] * padded[1] * padded[2];
         for (int c = 0; c < b->C; c++)
             CLES_HIP(hipMemcpyAsync(b->Q + (size_t)c * b->N + b->off[i], Qh + c * NP + halo[0],
@@ -677,6 +679,8 @@ int cles_batch_download(cles_batch *b, int64_t i, double *Qh, const int64_t padd
         CLES_REQUIRE(b && Qh && padded && halo, "NULL argument");
         cles::check_inst(b, i);
         CLES_REQUIRE(halo[0] >= 0 && padded[0] >= b->n[i] + 2 * (int64_t)halo[0], "padded extents smaller than n + 2*halo");
+        CLES_REQUIRE(padded[1] == 1 && padded[2] == 1 && halo[1] == 0 && halo[2] == 0,
+                     "a batch instance is laid out as (n + 2*halo, 1, 1) with halo (H, 0, 0)");
         const int64_t NP = padded[0] * padded[1] * padded[2];
         for (int c = 0; c < b->C; c++)
             CLES_HIP(hipMemcpyAsync(Qh + c * NP + halo[0], b->Q + (size_t)c * b->N + b->off[i],

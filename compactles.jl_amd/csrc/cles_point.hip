@@ -117,7 +117,9 @@ __global__ void __launch_bounds__(TPB) k_primitives(PrimArgs<W> a, int64_t N)
         // physics.jl:216-250 _primitives! (StiffenedGas): p = (gamma - 1) rho e - gamma p_inf
         const double g = a.sg_gamma, pinf = a.sg_pinf, cv = a.sg_cv, R = (g - 1) * cv;
         if (a.esens) {
-            const double r = d_max(rho, 1e-300);
+            // a point with rho <= 0 keeps _primitives!' placeholder density 1
+            // (physics.jl:242), which artificial.jl:321 divides by
+            const double r = rho > 0 ? rho : 1.0;
             const double m1 = a.Q[(int64_t)Ns * N + I], m2 = a.Q[(int64_t)(Ns + 1) * N + I],
                          m3 = a.Q[(int64_t)(Ns + 2) * N + I];
             const double ke = 0.5 * (m1 * m1 + m2 * m2 + m3 * m3) / r;

@@ -170,7 +170,14 @@ int cles_server_destroy(cles_server *sv)
         {
             std::lock_guard<std::mutex> lk(sv->mu);
             sv->stop = true;
-            sv->queue.clear();   // queued requests are dropped; running ones finish
+            // queued requests are dropped and end as failed; running ones finish
+            for (int64_t qid : sv->queue) {
+                Request *r = sv->reqs[qid].get();
+                r->state = RQ_FAILED;
+                r->status = CLES_ERR_INVALID;
+                r->msg = "the server stopped before the request was admitted";
+            }
+            sv->queue.clear();
         }
         sv->cv.notify_all();
         for (auto &t : sv->workers) t.join();

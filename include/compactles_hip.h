@@ -484,7 +484,9 @@ typedef struct cles_server cles_server;
 enum { CLES_REQ_QUEUED = 0, CLES_REQ_RUNNING = 1, CLES_REQ_DONE = 2, CLES_REQ_FAILED = 3 };
 
 int cles_server_create(int device, size_t hbm_budget, int runners, cles_server **server);
-/* Drops queued requests, waits for running ones, frees everything. */
+/* Marks queued requests CLES_REQ_FAILED and drops them, waits for running
+ * ones, frees everything.  No other call on the server (cles_server_wait
+ * included) may overlap it. */
 int cles_server_destroy(cles_server *server);
 /* Copies the padded initial state (as cles_solver_upload); CLES_ERR_NOMEM if
  * the request can never fit the budget. */

@@ -139,3 +139,19 @@ def test_batch_fp32_mixed_error(ctx):
     e_q = max(e[1] for e in errs)
     print(f"\nFP32-mixed batched tubes, {nsteps} steps vs FP64 oracle: KE {e_ke:.2e}  Q {e_q:.2e}")
     assert e_ke <= 1e-5 and e_q <= 1e-4
+
+
+def test_batch_upload_refuses_a_transverse_layout(ctx):
+    """An instance is held as (n + 2H, 1, 1) with halo (H, 0, 0): a transverse
+    extent or halo would address another row, so it is refused."""
+    import compactles_hip._lib as L
+    b = cases.tube_batch(ctx, cases.tube_specs(2, seed=3, nmin=64, nmax=96))
+    n = b.instances[0].n
+    for padded, halo in (((n + 8, 2, 1), (4, 0, 0)), ((n + 8, 1, 1), (4, 1, 0)), ((n + 8, 1, 3), (4, 0, 1))):
+        Q = np.zeros(padded + (b.n_cons,), order="F")
+        rc = ctx.lib.cles_batch_upload(b.handle, 0, Q.ctypes.data_as(L.DP), L.I64_3(*padded), L.INT_3(*halo))
+        assert rc == L.ERR_INVALID
+        assert "laid out" in ctx.lib.cles_last_error().decode()
+        rc = ctx.lib.cles_batch_download(b.handle, 0, Q.ctypes.data_as(L.DP), L.I64_3(*padded), L.INT_3(*halo))
+        assert rc != 0
+    b.close()

@@ -102,3 +102,27 @@ def test_water_fp32_mixed(ctx):
     err = _state_err(_q(gs, n), [os_.interior(os_.Q[..., c]) for c in range(5)])
     print(f"\nFP32-mixed water 32x12x12, 10 steps vs FP64 oracle: Q {err:.2e}")
     assert err <= 1e-5
+
+
+def test_nonpositive_density_point_keeps_the_placeholder(ctx):
+    """A point with rho <= 0 takes _primitives!' placeholder state
+    (physics.jl:242: rho = 1); the e-sensor of compute_artificial! divides by
+    that rho (artificial.jl:321), so kappa_art stays finite around the point
+    and equals the oracle's."""
+    import compactles_hip as ch
+    n, L = (32, 12, 12), (1.0, 0.5, 0.5)
+    X, Y, Z = cases.grid(n, L)
+    k = 2 * np.pi / L[1]
+    eos = ch.StiffenedGas(*WATER)
+    prim = ch.Prim(u=(1.0 * np.sin(2 * np.pi * X), 0.5 * np.cos(k * Z), 0 * X),
+                   p=101325.0 * (1 + 0.5 * np.sin(2 * np.pi * X) * np.cos(k * Y)), rho=1000.0 + 0 * X)
+    q = [np.array(c) for c in ch.conserved_from_prim(eos, prim)]
+    q[0][10, 5, 6] = -1.0e-3
+    os_ = cases.oracle_solver(n, L, q, stiffened=WATER, mu0=1e-3)
+    gs = _gpu(ctx, n, L, q, eos, mu0=1e-3)
+    os_.compute_rhs(False)
+    gs.compute_rhs(False)
+    ref = os_.interior(os_.field("kappa_art"))
+    got = gs.field("kappa_art")
+    assert np.all(np.isfinite(got))
+    assert np.abs(got - ref).max() <= 1e-12 * np.abs(ref).max()
