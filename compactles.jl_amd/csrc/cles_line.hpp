@@ -26,6 +26,14 @@ namespace cles {
 constexpr int MC = 32;     // maximum chunk length (points per thread)
 constexpr int NCMAX = 4;   // closure rows per edge handled by the kernels
 
+// Plan tables live in global memory; a pointer read out of memory (the ragged
+// kernel's per-instance plans) is otherwise a flat pointer, whose loads also
+// wait on the LDS counter.
+template <typename T> __device__ __forceinline__ T gload(const T *ptr)
+{
+    return *(const __attribute__((address_space(1))) T *)ptr;
+}
+
 template <typename T> __device__ __forceinline__ const T *coefp(const LineDev &p);
 template <> __device__ __forceinline__ const double *coefp<double>(const LineDev &p) { return p.coef64; }
 template <> __device__ __forceinline__ const float *coefp<float>(const LineDev &p) { return p.coef32; }
@@ -59,7 +67,7 @@ __device__ __forceinline__ void epilogue(const EpiArgs &e, int64_t o, double v)
         const T *acc = (const T *)e.a0;
         const double dq = acc ? (double)(acc[o] - (T)v) : -v;
         const double dt = EPI == EPI_LSRK_DEV ? *e.dtp : e.dt;
-        double nv = e.c0 * e.du[o] + dt * dq;
+        double nv = fma(dt, dq, e.c0 * e.du[o]);
         e.du[o] = nv;
         e.Q[o] += e.c1 * nv;
     } else if constexpr (EPI == EPI_ART_MUBETA) {
@@ -121,7 +129,7 @@ __device__ __forceinline__ void epilogue_batch(const EpiArgs &e, const int64_t (
         for (int i = 0; i < NB; i++) {
             if (!ok[i]) continue;
             const double dq = acc ? (double)(a[i] - v[i]) : -(double)v[i];
-            const double nv = e.c0 * du[i] + dt * dq;
+            const double nv = fma(dt, dq, e.c0 * du[i]);
             e.du[o[i]] = nv;
             e.Q[o[i]] = q[i] + e.c1 * nv;
         }
@@ -293,6 +301,221 @@ struct Chunk {
     }
 };
 
+// Chunk sweeps and spike correction with the row constants read G rows at a
+// time: the loads of group g + 1 are issued once the arithmetic of group g - 1
+// is done and before that of group g, so each group waits for one round trip
+// (instead of every row waiting for its own inside the dependency chain) while
+// only two groups of rows are held in registers.  Rows::fence(v) is an empty
+// asm statement that makes the table pointer depend on v, which keeps the
+// compiler from issuing a group's loads earlier than that.  Rows(field, t)
+// must be in bounds for every t < CL (rows past the chunk's length read
+// padding or a clamped row); rows t >= len are left unchanged through
+// selects, so the arithmetic of rows t < len is that of Chunk::solve /
+// corrected(), operation for operation.
+template <typename T, int Q, int CL, int G, typename Rows>
+__device__ __forceinline__ void solve_grouped(T (&d)[CL], int len, Rows R)
+{
+    static_assert(CL % G == 0, "group size must divide the chunk length");
+    constexpr int NG = CL / G;
+    if constexpr (Q == 1) {
+        T cur[G], nxt[G];
+#pragma unroll
+        for (int i = 0; i < G; i++) cur[i] = R(R1_L, i);
+#pragma unroll
+        for (int g = 0; g < NG; g++) {
+            if (g + 1 < NG) {
+                if (g > 0) R.fence(d[g * G - 1]);
+#pragma unroll
+                for (int i = 0; i < G; i++) nxt[i] = R(R1_L, (g + 1) * G + i);
+            }
+#pragma unroll
+            for (int i = 0; i < G; i++) {
+                const int t = g * G + i;
+                if (t >= 1) {
+                    const T nv = d[t] - cur[i] * d[t - 1];
+                    d[t] = t < len ? nv : d[t];
+                }
+            }
+            if (g + 1 < NG) {
+#pragma unroll
+                for (int i = 0; i < G; i++) cur[i] = nxt[i];
+            }
+        }
+        T cc[G], cd[G], nc[G], nd[G];
+        R.fence(d[CL - 1]);
+#pragma unroll
+        for (int i = 0; i < G; i++) {
+            cc[i] = R(R1_C, (NG - 1) * G + i);
+            cd[i] = R(R1_DINV, (NG - 1) * G + i);
+        }
+#pragma unroll
+        for (int g = NG - 1; g >= 0; g--) {
+            if (g > 0) {
+                if (g < NG - 1) R.fence(d[(g + 1) * G]);
+#pragma unroll
+                for (int i = 0; i < G; i++) {
+                    nc[i] = R(R1_C, (g - 1) * G + i);
+                    nd[i] = R(R1_DINV, (g - 1) * G + i);
+                }
+            }
+#pragma unroll
+            for (int i = G - 1; i >= 0; i--) {
+                const int t = g * G + i;
+                const T nx = d[t + 1 < CL ? t + 1 : CL - 1];
+                const T v = (t == len - 1) ? d[t] * cd[i] : (d[t] - cc[i] * nx) * cd[i];
+                d[t] = t < len ? v : d[t];
+            }
+            if (g > 0) {
+#pragma unroll
+                for (int i = 0; i < G; i++) {
+                    cc[i] = nc[i];
+                    cd[i] = nd[i];
+                }
+            }
+        }
+    } else {
+        T a1[G], a2[G], b1[G], b2[G];
+#pragma unroll
+        for (int i = 0; i < G; i++) {
+            a1[i] = R(R2_L1, i);
+            a2[i] = R(R2_L2, i);
+        }
+#pragma unroll
+        for (int g = 0; g < NG; g++) {
+            if (g + 1 < NG) {
+                if (g > 0) R.fence(d[g * G - 1]);
+#pragma unroll
+                for (int i = 0; i < G; i++) {
+                    b1[i] = R(R2_L1, (g + 1) * G + i);
+                    b2[i] = R(R2_L2, (g + 1) * G + i);
+                }
+            }
+#pragma unroll
+            for (int i = 0; i < G; i++) {
+                const int t = g * G + i;
+                if (t < CL - 1) {
+                    const T xt = d[t];
+                    const T n1 = d[t + 1] - a1[i] * xt;
+                    d[t + 1] = t < len - 1 ? n1 : d[t + 1];
+                    if (t + 2 < CL) {
+                        const T n2 = d[t + 2 < CL ? t + 2 : t] - a2[i] * xt;
+                        d[t + 2 < CL ? t + 2 : t] = t < len - 2 ? n2 : d[t + 2 < CL ? t + 2 : t];
+                    }
+                }
+            }
+            if (g + 1 < NG) {
+#pragma unroll
+                for (int i = 0; i < G; i++) {
+                    a1[i] = b1[i];
+                    a2[i] = b2[i];
+                }
+            }
+        }
+        T u0[G], u1[G], u2[G], w0[G], w1[G], w2[G];
+        R.fence(d[CL - 1]);
+#pragma unroll
+        for (int i = 0; i < G; i++) {
+            u0[i] = R(R2_U0INV, (NG - 1) * G + i);
+            u1[i] = R(R2_U1, (NG - 1) * G + i);
+            u2[i] = R(R2_U2, (NG - 1) * G + i);
+        }
+#pragma unroll
+        for (int g = NG - 1; g >= 0; g--) {
+            if (g > 0) {
+                if (g < NG - 1) R.fence(d[(g + 1) * G]);
+#pragma unroll
+                for (int i = 0; i < G; i++) {
+                    w0[i] = R(R2_U0INV, (g - 1) * G + i);
+                    w1[i] = R(R2_U1, (g - 1) * G + i);
+                    w2[i] = R(R2_U2, (g - 1) * G + i);
+                }
+            }
+#pragma unroll
+            for (int i = G - 1; i >= 0; i--) {
+                const int t = g * G + i;
+                T acc = d[t];
+                if (t + 1 < CL) {
+                    const T x1 = acc - u1[i] * d[t + 1 < CL ? t + 1 : t];
+                    acc = t + 1 < len ? x1 : acc;
+                }
+                if (t + 2 < CL) {
+                    const T x2 = acc - u2[i] * d[t + 2 < CL ? t + 2 : t];
+                    acc = t + 2 < len ? x2 : acc;
+                }
+                d[t] = t < len ? acc * u0[i] : d[t];
+            }
+            if (g > 0) {
+#pragma unroll
+                for (int i = 0; i < G; i++) {
+                    u0[i] = w0[i];
+                    u1[i] = w1[i];
+                    u2[i] = w2[i];
+                }
+            }
+        }
+    }
+}
+
+// x = y - V tail_prev - W head_next for every row of the chunk, the FP64 spike
+// rows read in groups as above (Rows64(field, t) in bounds for t < CL).
+template <typename T, int Q, int CL, int G, typename Rows64>
+__device__ __forceinline__ void correct_grouped(T (&d)[CL], int len, Rows64 R, const double (&tp)[Q],
+                                                const double (&hn)[Q])
+{
+    constexpr int NG = CL / G;
+    constexpr int NF = 2 * Q;
+    const int fld[4] = {Q == 1 ? (int)R1_V : (int)R2_V0, Q == 1 ? (int)R1_W : (int)R2_W0,
+                        Q == 1 ? 0 : (int)R2_V1, Q == 1 ? 0 : (int)R2_W1};
+    double cur[NF][G], nxt[NF][G];
+#pragma unroll
+    for (int f = 0; f < NF; f++)
+#pragma unroll
+        for (int i = 0; i < G; i++) cur[f][i] = R(fld[f], i);
+#pragma unroll
+    for (int g = 0; g < NG; g++) {
+        if (g + 1 < NG) {
+            if (g > 0) R.fence(d[g * G - 1]);
+#pragma unroll
+            for (int f = 0; f < NF; f++)
+#pragma unroll
+                for (int i = 0; i < G; i++) nxt[f][i] = R(fld[f], (g + 1) * G + i);
+        }
+#pragma unroll
+        for (int i = 0; i < G; i++) {
+            const int t = g * G + i;
+            double acc = fma(cur[1][i], hn[0], cur[0][i] * tp[0]);
+            if constexpr (Q == 2) {
+                acc = fma(cur[2][i], tp[1], acc);
+                acc = fma(cur[3][i], hn[1], acc);
+            }
+            const T c = (T)((double)d[t] - acc);
+            d[t] = t < len ? c : d[t];
+        }
+        if (g + 1 < NG) {
+#pragma unroll
+            for (int f = 0; f < NF; f++)
+#pragma unroll
+                for (int i = 0; i < G; i++) cur[f][i] = nxt[f][i];
+        }
+    }
+}
+
+// Row-major plan rows of the chunk starting at s (row index clamped to the line)
+template <typename T>
+struct RowsClamped {
+    const T *base;
+    int n, s;
+    __device__ __forceinline__ T operator()(int f, int t) const
+    {
+        const int g = s + t < n ? s + t : n - 1;
+        return gload(base + (size_t)f * n + g);
+    }
+    template <typename D> __device__ __forceinline__ void fence(const D &dep)
+    {
+        asm volatile("" : "+v"(base) : "v"(dep));
+    }
+};
+
 // Reduced interface system: block PCR across the P chunks of each line
 // (slot = chunk * nl + line-in-block), then fetch the previous chunk's tail
 // and the next chunk's head.  Every thread of the block must call this.
@@ -330,7 +553,10 @@ __device__ __forceinline__ void reduced_solve(const LineDev &p, double *buf, int
         for (int i = 0; i < B; i++) {
             double acc = r[i];
 #pragma unroll
-            for (int j = 0; j < B; j++) acc += al[i * B + j] * rm[j] + be[i * B + j] * rp[j];
+            for (int j = 0; j < B; j++) {
+                acc = fma(al[i * B + j], rm[j], acc);
+                acc = fma(be[i * B + j], rp[j], acc);
+            }
             r[i] = acc;
         }
     }
@@ -370,10 +596,12 @@ __device__ __forceinline__ double corrected(const LineDev &p, int g, T y, const 
     const int n = p.n;
     const double *rw = p.row64;
     if constexpr (Q == 1) {
-        return (double)y - (rw[(size_t)R1_V * n + g] * tp[0] + rw[(size_t)R1_W * n + g] * hn[0]);
+        return (double)y - fma(rw[(size_t)R1_W * n + g], hn[0], rw[(size_t)R1_V * n + g] * tp[0]);
     } else {
-        double acc = rw[(size_t)R2_V0 * n + g] * tp[0] + rw[(size_t)R2_W0 * n + g] * hn[0];
-        acc += rw[(size_t)R2_V1 * n + g] * tp[1] + rw[(size_t)R2_W1 * n + g] * hn[1];
+        double acc = rw[(size_t)R2_V0 * n + g] * tp[0];
+        acc = fma(rw[(size_t)R2_W0 * n + g], hn[0], acc);
+        acc = fma(rw[(size_t)R2_V1 * n + g], tp[1], acc);
+        acc = fma(rw[(size_t)R2_W1 * n + g], hn[1], acc);
         return (double)y - acc;
     }
 }
@@ -385,7 +613,7 @@ __device__ __forceinline__ double corrected(const LineDev &p, int g, T y, const 
 // constant is wave-uniform (scalar loads).
 // ---------------------------------------------------------------------------
 template <typename T, int Q, int M, int SYM, int EPI, bool UNI, int CL = MC, bool NTS = false>
-__global__ void __launch_bounds__(Q == 1 ? 1024 : 512) k_line_yz(LineDev p, const T *__restrict__ f, int sd, int64_t so,
+__global__ void __launch_bounds__(UNI && Q == 1 ? 1024 : 512) k_line_yz(LineDev p, const T *__restrict__ f, int sd, int64_t so,
                                                   int nlane, int nxb, int Lx, EpiArgs e, LineFields mf)
 {
     select_field<T, EPI>(mf, f, e);
@@ -431,14 +659,25 @@ __global__ void __launch_bounds__(Q == 1 ? 1024 : 512) k_line_yz(LineDev p, cons
 
     Chunk<T, Q, M, SYM, CL> ch;
     ch.rhs(p, k, len, fv, fe);
-    ch.solve(p, s, len);
+    // grouped row loads where they fit the 128 registers of a 1024-thread
+    // block (chunks of up to 16 points); wave-uniform rows (UNI) are scalar loads
+    constexpr bool GRP = !UNI && CL <= 16;
+    constexpr int G = CL <= 8 ? CL : 4;
+    if constexpr (GRP)
+        solve_grouped<T, Q, CL, G>(ch.d, len, RowsClamped<T>{rowp<T>(p), n, s});
+    else
+        ch.solve(p, s, len);
     double r[2 * Q], tp[Q], hn[Q];
     ch.ends(len, r);
     reduced_solve<Q>(p, smem, NT, Lx, k, lane, r, tp, hn);
     if (!valid) return;
+    if constexpr (GRP) {
+        correct_grouped<T, Q, CL, G>(ch.d, len, RowsClamped<double>{p.row64, n, s}, tp, hn);
+    } else {
 #pragma unroll
-    for (int t = 0; t < CL; t++)
-        if (t < len) ch.d[t] = (T)corrected<T, Q>(p, s + t, ch.d[t], tp, hn);
+        for (int t = 0; t < CL; t++)
+            if (t < len) ch.d[t] = (T)corrected<T, Q>(p, s + t, ch.d[t], tp, hn);
+    }
 #pragma unroll
     for (int b = 0; b < CL; b += 8) {
         int64_t o[8];
@@ -459,8 +698,8 @@ __global__ void __launch_bounds__(Q == 1 ? 1024 : 512) k_line_yz(LineDev p, cons
 // (line, chunk) pairs, tid = line + Lt * chunk.  Lines are indexed
 // l = j + ny*k over the y-z plane.  UNI: Lt == 64 (one chunk per wave).
 // ---------------------------------------------------------------------------
-template <typename T, int Q, int M, int SYM, int EPI, bool UNI>
-__global__ void __launch_bounds__(Q == 1 ? 1024 : 512) k_line_x(LineDev p, const T *__restrict__ f, int64_t sy, int64_t sz,
+template <typename T, int Q, int M, int SYM, int EPI, bool UNI, int CL = MC>
+__global__ void __launch_bounds__(UNI && Q == 1 ? 1024 : 512) k_line_x(LineDev p, const T *__restrict__ f, int64_t sy, int64_t sz,
                                                  int ny, int64_t nlines, int Lt, EpiArgs e, LineFields mf)
 {
     select_field<T, EPI>(mf, f, e);
@@ -487,9 +726,9 @@ __global__ void __launch_bounds__(Q == 1 ? 1024 : 512) k_line_x(LineDev p, const
     const int s = k * p.m + min(k, p.extra);
     const int len = p.m + (k < p.extra ? 1 : 0);
     const T *trow = tile + li * TS;
-    T fv[MC + 2 * M];
+    T fv[CL + 2 * M];
 #pragma unroll
-    for (int t = -M; t < MC + M; t++) {
+    for (int t = -M; t < CL + M; t++) {
         T val = T(0);
         if (t < len + M) {
             int r = s + t;
@@ -507,17 +746,31 @@ __global__ void __launch_bounds__(Q == 1 ? 1024 : 512) k_line_x(LineDev p, const
 #pragma unroll
     for (int kap = 0; kap < 8; kap++) fe[kap] = (p.closed && kap < n) ? trow[n - 1 - kap] : T(0);
 
-    Chunk<T, Q, M, SYM> ch;
+    Chunk<T, Q, M, SYM, CL> ch;
     ch.rhs(p, k, len, fv, fe);
-    ch.solve(p, s, len);
+    // grouped row loads where they fit the 128 registers of a 1024-thread
+    // block (chunks of up to 16 points); wave-uniform rows (UNI) are scalar loads
+    constexpr bool GRP = !UNI && CL <= 16;
+    constexpr int G = CL <= 8 ? CL : 4;
+    if constexpr (GRP)
+        solve_grouped<T, Q, CL, G>(ch.d, len, RowsClamped<T>{rowp<T>(p), n, s});
+    else
+        ch.solve(p, s, len);
     double r[2 * Q], tp[Q], hn[Q];
     ch.ends(len, r);
     reduced_solve<Q>(p, buf, NT, Lt, k, li, r, tp, hn);
     // every thread has finished reading the tile before the first reduced-solve barrier
     T *wrow = tile + li * TS;
+    if constexpr (GRP) {
+        correct_grouped<T, Q, CL, G>(ch.d, len, RowsClamped<double>{p.row64, n, s}, tp, hn);
 #pragma unroll
-    for (int t = 0; t < MC; t++)
-        if (t < len) wrow[s + t] = (T)corrected<T, Q>(p, s + t, ch.d[t], tp, hn);
+        for (int t = 0; t < CL; t++)
+            if (t < len) wrow[s + t] = ch.d[t];
+    } else {
+#pragma unroll
+        for (int t = 0; t < CL; t++)
+            if (t < len) wrow[s + t] = (T)corrected<T, Q>(p, s + t, ch.d[t], tp, hn);
+    }
     __syncthreads();
 
     for (int lj = 0; lj < Lt; lj++) {
@@ -572,7 +825,10 @@ __device__ __forceinline__ void reduced_solve_wave(const LineDev &p, int k, int 
         for (int i = 0; i < B; i++) {
             double acc = r[i];
 #pragma unroll
-            for (int j = 0; j < B; j++) acc += al[i * B + j] * rm[j] + be[i * B + j] * rp[j];
+            for (int j = 0; j < B; j++) {
+                acc = fma(gload(al + i * B + j), rm[j], acc);
+                acc = fma(gload(be + i * B + j), rp[j], acc);
+            }
             r[i] = acc;
         }
     }
@@ -582,7 +838,7 @@ __device__ __forceinline__ void reduced_solve_wave(const LineDev &p, int k, int 
     for (int i = 0; i < B; i++) {
         double acc = 0.0;
 #pragma unroll
-        for (int j = 0; j < B; j++) acc += mi[i * B + j] * r[j];
+        for (int j = 0; j < B; j++) acc += gload(mi + i * B + j) * r[j];
         z[i] = acc;
     }
     int kprev = k - 1, knext = k + 1;
@@ -694,12 +950,13 @@ __global__ void __launch_bounds__(64) k_line_xw(LineDev p, const T *__restrict__
         for (int kap = 0; kap < 8; kap++) fe[kap] = (p.closed && kap < n) ? trow[xw_slot(n - 1 - kap)] : T(0);
         Chunk<T, Q, M, SYM, XCL> ch;
         ch.rhs(p, k, len, fv, fe);
-        ch.solve(p, rs, len);
+        solve_grouped<T, Q, XCL, 8>(ch.d, len, RowsClamped<T>{rowp<T>(p), n, rs});
         double r[2 * Q], tp[Q], hn[Q];
         ch.ends(len, r);
         reduced_solve_wave<Q>(p, k, kc, base, r, tp, hn);
+        correct_grouped<T, Q, XCL, 8>(ch.d, len, RowsClamped<double>{p.row64, n, rs}, tp, hn);
 #pragma unroll
-        for (int t = 0; t < XCL; t++) res[t] = (T)corrected<T, Q>(p, rs + t, ch.d[t], tp, hn);
+        for (int t = 0; t < XCL; t++) res[t] = ch.d[t];
     }
     __syncthreads();   // all window reads done before the tile is overwritten
     {
@@ -910,15 +1167,23 @@ void launch_variant(const HostPlan &hp, int dim, const View3 &v, const T *f, con
         } else {
             int Lt = 1;
             while (Lt * 2 * P <= 512 && shm_of(Lt * 2) <= 48 * 1024 && Lt < 32) Lt *= 2;
-            if (Lt * P > maxNT || shm_of(Lt) > lim)
+            if (Lt * P > 512 || shm_of(Lt) > lim)
                 throw Error{CLES_ERR_UNSUPPORTED, "line too long for the x-sweep kernel"};
             const int NT = Lt * P;
             const size_t shm = shm_of(Lt);
-            auto kern = k_line_x<T, Q, M, SYM, EPI, false>;
-            allow_lds(kern, shm);
             const int64_t nblk = (nlines + Lt - 1) / Lt;
-            hipLaunchKernelGGL(kern, dim3((unsigned)nblk, nf), dim3(NT), shm, st, p, f, v.s[1], v.s[2], (int)v.n[1],
-                               nlines, Lt, e, mf);
+            const int cl = p.m + (p.extra ? 1 : 0);
+            auto go = [&](auto kern) {
+                allow_lds(kern, shm);
+                hipLaunchKernelGGL(kern, dim3((unsigned)nblk, nf), dim3(NT), shm, st, p, f, v.s[1], v.s[2],
+                                   (int)v.n[1], nlines, Lt, e, mf);
+            };
+            if (cl <= 8)
+                go(k_line_x<T, Q, M, SYM, EPI, false, 8>);
+            else if (cl <= 16)
+                go(k_line_x<T, Q, M, SYM, EPI, false, 16>);
+            else
+                go(k_line_x<T, Q, M, SYM, EPI, false, MC>);
         }
     } else {
         const int64_t nlane = v.n[0];
@@ -953,17 +1218,25 @@ void launch_variant(const HostPlan &hp, int dim, const View3 &v, const T *f, con
                                    shm, st, p, f, (int)sd, so, (int)nlane, nxb, Lx, e, mf);
         } else {
             int Lx = 64;
-            while (Lx * P > maxNT && Lx > 1) Lx /= 2;
+            while (Lx * P > 512 && Lx > 1) Lx /= 2;
             while (Lx / 2 >= nlane && Lx > 1) Lx /= 2;   // narrow x extents (2-D / 1-D grids)
             // few lines (2-D grids): narrower blocks, down to 8 lanes (64-byte
             // rows), until the launch has two blocks per CU
             while (Lx > 8 && ((nlane + Lx - 1) / Lx) * nouter * (int64_t)nf < 512) Lx /= 2;
-            if (Lx * P > maxNT) throw Error{CLES_ERR_UNSUPPORTED, "line too long for the y/z-sweep kernel"};
+            if (Lx * P > 512) throw Error{CLES_ERR_UNSUPPORTED, "line too long for the y/z-sweep kernel"};
             const int NT = Lx * P;
             const int nxb = (int)((nlane + Lx - 1) / Lx);
             const size_t shm = (size_t)2 * B * NT * sizeof(double);
-            hipLaunchKernelGGL((k_line_yz<T, Q, M, SYM, EPI, false>), dim3((unsigned)(nxb * nouter), nf), dim3(NT),
-                               shm, st, p, f, (int)sd, so, (int)nlane, nxb, Lx, e, mf);
+            const int cl = p.m + (p.extra ? 1 : 0);
+            if (cl <= 8)
+                hipLaunchKernelGGL((k_line_yz<T, Q, M, SYM, EPI, false, 8>), dim3((unsigned)(nxb * nouter), nf),
+                                   dim3(NT), shm, st, p, f, (int)sd, so, (int)nlane, nxb, Lx, e, mf);
+            else if (cl <= 16)
+                hipLaunchKernelGGL((k_line_yz<T, Q, M, SYM, EPI, false, 16>), dim3((unsigned)(nxb * nouter), nf),
+                                   dim3(NT), shm, st, p, f, (int)sd, so, (int)nlane, nxb, Lx, e, mf);
+            else
+                hipLaunchKernelGGL((k_line_yz<T, Q, M, SYM, EPI, false>), dim3((unsigned)(nxb * nouter), nf),
+                                   dim3(NT), shm, st, p, f, (int)sd, so, (int)nlane, nxb, Lx, e, mf);
         }
     }
     CLES_HIP(hipGetLastError());

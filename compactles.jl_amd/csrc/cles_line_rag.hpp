@@ -29,65 +29,22 @@ template <typename T> __device__ __forceinline__ const T *rowtp(const LineDev &p
 template <> __device__ __forceinline__ const double *rowtp<double>(const LineDev &p) { return p.rowt64; }
 template <> __device__ __forceinline__ const float *rowtp<float>(const LineDev &p) { return p.rowt32; }
 
-// Chunk::solve with the chunk-major tables: the P lanes of a line read
-// consecutive words for each (field, t).  Same arithmetic, same order.
-template <typename T, int Q, int M, int SYM, int CL>
-__device__ __forceinline__ void rag_solve(Chunk<T, Q, M, SYM, CL> &ch, const LineDev &p, int k, int len)
-{
-    const int P = p.P;
-    const T *rw = rowtp<T>(p) + k;
-    auto R = [&](int f, int t) -> T { return rw[((size_t)f * CL + t) * P]; };
-    T *d = ch.d;
-    if constexpr (Q == 1) {
-#pragma unroll
-        for (int t = 1; t < CL; t++)
-            if (t < len) d[t] -= R(R1_L, t) * d[t - 1];
-#pragma unroll
-        for (int t = CL - 1; t >= 0; t--) {
-            if (t < len) {
-                const T nxt = d[t + 1 < CL ? t + 1 : CL - 1];
-                d[t] = (t == len - 1) ? d[t] * R(R1_DINV, t) : (d[t] - R(R1_C, t) * nxt) * R(R1_DINV, t);
-            }
-        }
-    } else {
-#pragma unroll
-        for (int t = 0; t < CL - 1; t++) {
-            if (t < len - 1) {
-                const T xt = d[t];
-                d[t + 1] -= R(R2_L1, t) * xt;
-                if (t + 2 < CL)
-                    if (t < len - 2) d[t + 2 < CL ? t + 2 : t] -= R(R2_L2, t) * xt;
-            }
-        }
-#pragma unroll
-        for (int t = CL - 1; t >= 0; t--) {
-            if (t < len) {
-                T acc = d[t];
-                if (t + 1 < CL)
-                    if (t + 1 < len) acc -= R(R2_U1, t) * d[t + 1 < CL ? t + 1 : t];
-                if (t + 2 < CL)
-                    if (t + 2 < len) acc -= R(R2_U2, t) * d[t + 2 < CL ? t + 2 : t];
-                d[t] = acc * R(R2_U0INV, t);
-            }
-        }
+// The chunk-major plan rows of chunk k (RAG_MAX_CHUNK zero-padded rows per
+// chunk, so every t < RAG_MAX_CHUNK is in bounds): the P lanes of a line read
+// consecutive words for each (field, t).
+template <typename T>
+struct RowsRag {
+    const T *base;
+    int P;
+    __device__ __forceinline__ T operator()(int f, int t) const
+    {
+        return gload(base + ((size_t)f * RAG_MAX_CHUNK + t) * P);
     }
-}
-
-// corrected() with the chunk-major spikes
-template <typename T, int Q, int CL>
-__device__ __forceinline__ double rag_corrected(const LineDev &p, int k, int t, T y, const double (&tp)[Q],
-                                                const double (&hn)[Q])
-{
-    const double *rw = p.rowt64 + k;
-    auto R = [&](int f) -> double { return rw[((size_t)f * CL + t) * p.P]; };
-    if constexpr (Q == 1) {
-        return (double)y - (R(R1_V) * tp[0] + R(R1_W) * hn[0]);
-    } else {
-        double acc = R(R2_V0) * tp[0] + R(R2_W0) * hn[0];
-        acc += R(R2_V1) * tp[1] + R(R2_W1) * hn[1];
-        return (double)y - acc;
+    template <typename D> __device__ __forceinline__ void fence(const D &dep)
+    {
+        asm volatile("" : "+v"(base) : "v"(dep));
     }
-}
+};
 
 template <typename T, int Q, int M, int SYM, int EPI>
 __global__ void __launch_bounds__(64) k_line_rag(RagLines r, const T *__restrict__ f, EpiArgs e)
@@ -156,13 +113,11 @@ __global__ void __launch_bounds__(64) k_line_rag(RagLines r, const T *__restrict
     for (int kap = 0; kap < 8; kap++) fe[kap] = (p.closed && kap < n) ? trow[rag_slot(n - 1 - kap)] : T(0);
     Chunk<T, Q, M, SYM, CL> ch;
     ch.rhs(p, k, len, fv, fe);
-    rag_solve(ch, p, k, len);
+    solve_grouped<T, Q, CL, 8>(ch.d, len, RowsRag<T>{rowtp<T>(p) + k, p.P});
     double ends[2 * Q], tp[Q], hn[Q];
     ch.ends(len, ends);
     reduced_solve_wave<Q>(p, k, k, li * P, ends, tp, hn);
-#pragma unroll
-    for (int t = 0; t < CL; t++)
-        if (t < len) ch.d[t] = (T)rag_corrected<T, Q, CL>(p, k, t, ch.d[t], tp, hn);
+    correct_grouped<T, Q, CL, 8>(ch.d, len, RowsRag<double>{p.rowt64 + k, p.P}, tp, hn);
     __syncthreads();   // every window read is done before the tile is overwritten
     if (live) {
 #pragma unroll

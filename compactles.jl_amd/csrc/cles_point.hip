@@ -616,7 +616,7 @@ __global__ void __launch_bounds__(TPB) k_lsrk(double *__restrict__ Q, double *__
     if (I >= NC) return;
     if (gate && !*gate) return;
     if (dtp) dt = *dtp;
-    const double v = A * du[I] + dt * (double)dQ[I];
+    const double v = fma(dt, (double)dQ[I], A * du[I]);
     du[I] = v;
     Q[I] += B * v;
 }
