@@ -182,18 +182,21 @@ def measure(ch, ctx, n, precision, steps, warmup, profile_steps, run_mode="host_
     }
 
 
-def measure_tubes(ch, ctx, count, steps, warmup, precision, plain_sample=32):
+def measure_tubes(ch, ctx, count, steps, warmup, precision, plain_sample=32, run_mode="step_graph"):
     """The batched form: `count` independent one-dimensional two-gas shock
     tubes (N_i ~ U{128..2048}, p_L ~ U[2, 20], rho_L ~ U[0.1, 10] from one
     seeded generator; He | CO2, Dirichlet ends, art on, cfl 0.4, filter every
-    step) advanced together, each with its own dt."""
+    step) advanced together, each with its own dt.  In step-graph mode the
+    warm-up covers at least one 8-iteration replay, so the graph is captured
+    before the timed steps."""
     from compactles_hip import workloads as W
     specs = W.tube_specs(count, seed=20261202)
     npts = sum(sp["n"] for sp in specs)
     t0 = time.perf_counter()
     b = W.tube_batch(ctx, specs, precision=precision)
     setup_s = time.perf_counter() - t0
-    b.run(max_steps=warmup)
+    b.set_run_mode(run_mode)
+    b.run(max_steps=max(warmup, 8) if run_mode == "step_graph" else warmup)
     t0 = time.perf_counter()
     b.run(max_steps=steps)
     wall = time.perf_counter() - t0
@@ -221,7 +224,7 @@ def measure_tubes(ch, ctx, count, steps, warmup, precision, plain_sample=32):
             "plain_path_same_instances": {"value": round(plain, 3), "unit": "Mpt-steps/s",
                                           "instances_sampled": len(sample),
                                           "batched_over_plain": round(npts * steps / wall / 1e6 / plain, 1)},
-            "all_ok": bool((status == 0).all()), "setup_s": round(setup_s, 2),
+            "all_ok": bool((status == 0).all()), "setup_s": round(setup_s, 2), "run_mode": run_mode,
             "step_roofline": {"alg_bytes_per_point_step": alg_bytes_per_point_step(2, 1, precision),
                               "achieved_GBps": round(alg / (ms * 1e-3) / 1e9, 1),
                               "frac": round(alg / (ms * 1e-3) / 1e9 / HBM_PEAK_GBPS, 4)},

@@ -141,6 +141,7 @@ SIGNATURES = {
     "cles_batch_download": (C.c_int, [P, C.c_int64, DP, C.POINTER(C.c_int64), C.POINTER(C.c_int)]),
     "cles_batch_run": (C.c_int, [P, C.c_int64, C.c_int64, DP, C.POINTER(C.c_int64)]),
     "cles_batch_run_noalloc": (C.c_int, [P, C.c_int64, C.c_int64, DP, C.POINTER(C.c_int64)]),
+    "cles_batch_set_run_mode": (C.c_int, [P, C.c_int]),
     "cles_batch_state": (C.c_int, [P, DP, C.POINTER(C.c_int64), C.POINTER(C.c_int)]),
     "cles_batch_compute_rhs": (C.c_int, [P, C.c_int]),
     "cles_batch_get_field": (C.c_int, [P, C.c_int64, C.c_char_p, DP]),

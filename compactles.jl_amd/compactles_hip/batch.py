@@ -131,6 +131,16 @@ class Batch:
                    C.byref(taken)))
         return (taken.value, dts[:, :record]) if record else taken.value
 
+    RUN_MODES = {"host_dt": 0, "device_dt": 1, "step_graph": 2}
+
+    def set_run_mode(self, mode: str):
+        """How run() issues its iterations: "host_dt" / "device_dt" launch
+        every kernel from the host, "step_graph" replays a captured graph of 8
+        iterations (bitwise the same results)."""
+        if mode not in self.RUN_MODES:
+            raise ValueError(f"run mode must be one of {sorted(self.RUN_MODES)}")
+        L.check(self.ctx.lib.cles_batch_set_run_mode(self.handle, self.RUN_MODES[mode]))
+
     def state(self):
         """(t, steps, status) arrays over the instances."""
         cnt = len(self.instances)

@@ -69,6 +69,22 @@ struct cles_batch {
     int *status = nullptr, *active = nullptr, *stepped = nullptr, *filt = nullptr, *cnt = nullptr;
     int *host_cnt = nullptr;
 
+    // CLES_RUN_STEP_GRAPH: GRAPH_ITERS run! iterations captured once and
+    // replayed; the captured kernels record dts into gdts (nrec = g_nrec)
+    int run_mode = CLES_RUN_HOST_DT;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    double *gdts = nullptr;
+    size_t gdts_cap = 0;
+    int64_t g_nrec = -1;
+    void drop_graph()
+    {
+        if (gexec) (void)hipGraphExecDestroy(gexec);
+        if (graph) (void)hipGraphDestroy(graph);
+        gexec = nullptr;
+        graph = nullptr;
+    }
+
     hipStream_t st() const { return ctx->stream; }
     size_t wsize() const { return prec == CLES_F64 ? 8 : 4; }
     void *alloc(size_t bytes)
@@ -101,6 +117,8 @@ struct cles_batch {
     ~cles_batch()
     {
         if (ctx && ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+        drop_graph();
+        if (gdts) (void)hipFree(gdts);
         for (void *ptr : allocs) (void)hipFree(ptr);
         if (host_cnt) (void)hipHostFree(host_cnt);
         if (ctx) cles::ctx_release(ctx);
@@ -691,9 +709,42 @@ int cles_batch_download(cles_batch *b, int64_t i, double *Qh, const int64_t padd
 
 namespace {
 
+constexpr int GRAPH_ITERS = 8;   // iterations per replay, and between activity polls
+
+// GRAPH_ITERS iterations of the batch captured on its stream (nothing runs
+// during the capture) and instantiated into b->gexec.
+extern "C++" template <typename Im>
+void capture_iterations(cles_batch *b, Im &im, const cles::BatchCtl &c)
+{
+    b->drop_graph();
+    hipError_t cap = hipStreamBeginCapture(b->st(), hipStreamCaptureModeThreadLocal);
+    if (cap != hipSuccess) throw cles::Error{CLES_ERR_HIP, std::string("hipStreamBeginCapture: ") + hipGetErrorString(cap)};
+    std::string err;
+    try {
+        for (int i = 0; i < GRAPH_ITERS; i++) im.iteration(c);
+    } catch (const cles::Error &e) {
+        err = e.msg;
+    }
+    hipGraph_t g = nullptr;
+    const hipError_t end = hipStreamEndCapture(b->st(), &g);
+    if (!err.empty() || end != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        throw cles::Error{CLES_ERR_HIP, err.empty() ? std::string("hipStreamEndCapture: ") + hipGetErrorString(end) : err};
+    }
+    b->graph = g;
+    const hipError_t inst = hipGraphInstantiate(&b->gexec, g, nullptr, nullptr, 0);
+    if (inst != hipSuccess) {
+        b->drop_graph();
+        throw cles::Error{CLES_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(inst)};
+    }
+}
+
 // run! on every instance (cles_batch_run); with `noalloc` the dt history is
 // staged in the per-device array of cles_noalloc.hip instead of a device
-// allocation of this call.
+// allocation of this call.  In CLES_RUN_STEP_GRAPH mode (not with noalloc)
+// whole groups of GRAPH_ITERS iterations are replays of one captured graph and
+// the rest are issued from the host: the same kernels in the same order, so
+// the results are those of host issue, bit for bit.
 void batch_run(cles_batch *b, int64_t max_steps, int64_t nrec, double *dts, int64_t *taken, bool noalloc)
 {
     {
@@ -703,6 +754,7 @@ void batch_run(cles_batch *b, int64_t max_steps, int64_t nrec, double *dts, int6
         if (noalloc && nrec > 0 && (int64_t)cnt > (int64_t)CLES_NOALLOC_DTS_MAX / nrec)
             throw cles::Error{CLES_ERR_INVALID, "count * nrec exceeds the allocation-free dt history "
                                                 "(CLES_NOALLOC_DTS_MAX doubles)"};
+        const bool graphs = !noalloc && b->run_mode == CLES_RUN_STEP_GRAPH;
         // the run! entry of every instance (timestep.jl:281-344): fresh
         // workspace, dt_seen = 0, active while t < tfinal and step < nmax
         std::vector<double> t(cnt), tf(cnt);
@@ -728,18 +780,43 @@ void batch_run(cles_batch *b, int64_t max_steps, int64_t nrec, double *dts, int6
         if (nrec > 0 && noalloc) {
             CLES_HIP(hipSetDevice(b->ctx->device));
             ddts = cles::noalloc_dts(b->ctx->device, (int64_t)cnt * nrec, b->st());
+        } else if (nrec > 0 && graphs) {
+            // the captured kernels write the history into the batch's own
+            // buffer, which outlives this call
+            const size_t need = cnt * (size_t)nrec;
+            if (need > b->gdts_cap) {
+                b->drop_graph();
+                if (b->gdts) CLES_HIP(hipFree(b->gdts));
+                b->gdts = nullptr;
+                b->gdts_cap = 0;
+                CLES_HIP(hipMalloc(&b->gdts, need * sizeof(double)));
+                b->gdts_cap = need;
+            }
+            ddts = b->gdts;
+            CLES_HIP(hipMemsetAsync(ddts, 0, need * sizeof(double), b->st()));
         } else if (nrec > 0) {
             CLES_HIP(hipMalloc(&ddts, cnt * (size_t)nrec * sizeof(double)));
             CLES_HIP(hipMemsetAsync(ddts, 0, cnt * (size_t)nrec * sizeof(double), b->st()));
         }
+        const bool own_dts = ddts && !noalloc && !graphs;
+        if (graphs && nrec != b->g_nrec) b->drop_graph();
         int64_t steps = 0;
         try {
             cles::bdispatch(b, [&](auto &im) {
                 const cles::BatchCtl c = im.ctl(ddts, nrec);
-                const int64_t poll = 8;
+                const int64_t poll = GRAPH_ITERS;
                 while (nact > 0 && steps < max_steps) {
-                    im.iteration(c);
-                    steps++;
+                    if (graphs && max_steps - steps >= poll && steps % poll == 0) {
+                        if (!b->gexec) {
+                            capture_iterations(b, im, c);
+                            b->g_nrec = nrec;
+                        }
+                        CLES_HIP(hipGraphLaunch(b->gexec, b->st()));
+                        steps += poll;
+                    } else {
+                        im.iteration(c);
+                        steps++;
+                    }
                     if (steps % poll == 0 || steps == max_steps) {
                         cles::launch_batch_count(b->active, b->count, b->cnt, b->st());
                         CLES_HIP(hipMemcpyAsync(b->host_cnt, b->cnt, sizeof(int), hipMemcpyDeviceToHost, b->st()));
@@ -751,11 +828,11 @@ void batch_run(cles_batch *b, int64_t max_steps, int64_t nrec, double *dts, int6
             });
             if (ddts) {
                 CLES_HIP(hipMemcpy(dts, ddts, cnt * (size_t)nrec * sizeof(double), hipMemcpyDeviceToHost));
-                if (!noalloc) (void)hipFree(ddts);
+                if (own_dts) (void)hipFree(ddts);
                 ddts = nullptr;
             }
         } catch (...) {
-            if (ddts && !noalloc) (void)hipFree(ddts);
+            if (own_dts && ddts) (void)hipFree(ddts);
             throw;
         }
         if (taken) *taken = steps;
@@ -772,6 +849,16 @@ int cles_batch_run(cles_batch *b, int64_t max_steps, int64_t nrec, double *dts, 
 int cles_batch_run_noalloc(cles_batch *b, int64_t max_steps, int64_t nrec, double *dts, int64_t *taken)
 {
     CLES_TRY({ batch_run(b, max_steps, nrec, dts, taken, true); })
+}
+
+int cles_batch_set_run_mode(cles_batch *b, int mode)
+{
+    CLES_TRY({
+        CLES_REQUIRE(b, "NULL argument");
+        CLES_REQUIRE(mode >= CLES_RUN_HOST_DT && mode <= CLES_RUN_STEP_GRAPH, "unknown run mode");
+        if (mode != CLES_RUN_STEP_GRAPH) b->drop_graph();
+        b->run_mode = mode;
+    })
 }
 
 int cles_batch_state(cles_batch *b, double *t, int64_t *steps, int *status)

@@ -327,7 +327,8 @@ __device__ __forceinline__ int d4_index(int v, int n, int periodic)
 }
 
 // The same sum, 2.5-D: a block owns a 64 x 16 tile of (x, y) columns and
-// marches D4T_Z planes in z.  Each plane's tile with its two-wide halo sits in
+// marches zl planes in z (at most D4T_Z; fewer on small grids, so that the
+// launch still has a few blocks per CU).  Each plane's tile with its two-wide halo sits in
 // LDS (every slot holds the value at the wrapped or clamped source index, so
 // edges and collapsed dimensions need no special case), the x and y stencils
 // read LDS, and each thread keeps a five-plane z window in registers for its
@@ -349,7 +350,7 @@ __device__ __forceinline__ bool d4_needed(int v, int n, bool active)
 }
 
 template <typename W>
-__global__ void __launch_bounds__(TPB) k_delta4_tile(D4Args<W> a, W *__restrict__ out, int gx, int gy)
+__global__ void __launch_bounds__(TPB) k_delta4_tile(D4Args<W> a, W *__restrict__ out, int gx, int gy, int zl)
 {
     __shared__ W lds[D4T_LY * D4T_LX];   // holds input values, so W is exact
     const unsigned blk = xcd_block(blockIdx.x, gridDim.x);
@@ -362,7 +363,7 @@ __global__ void __launch_bounds__(TPB) k_delta4_tile(D4Args<W> a, W *__restrict_
     const bool xa = a.active[0], ya = a.active[1], za = a.active[2];
     const int px = a.periodic[0], py = a.periodic[1], pz = a.periodic[2];
     const int tx = (int)(threadIdx.x & 63), ty = (int)(threadIdx.x >> 6);
-    const int i0 = bx * D4T_X, j0 = by * D4T_Y, k0 = bz * D4T_Z;
+    const int i0 = bx * D4T_X, j0 = by * D4T_Y, k0 = bz * zl;
     const int i = i0 + tx;
     const W *f = a.f;
     const double D4[5] = {1.0, -4.0, 6.0, -4.0, 1.0};
@@ -414,7 +415,7 @@ __global__ void __launch_bounds__(TPB) k_delta4_tile(D4Args<W> a, W *__restrict_
     // the slot values of plane k are loaded one plane ahead, so their latency
     // overlaps the previous plane's LDS fill and stencils
     double cur[6], nxt[6];
-    const int kend = k0 + D4T_Z < nz ? k0 + D4T_Z : nz;
+    const int kend = k0 + zl < nz ? k0 + zl : nz;
 #pragma unroll
     for (int e = 0; e < 6; e++) {
         cur[e] = src[e] >= 0 ? (double)f[(int64_t)k0 * sz + src[e]] : 0.0;
@@ -1413,9 +1414,11 @@ void launch_delta4(const D4Args<W> &a, int mode, W *out, hipStream_t st)
 {
     if (mode == 0) {
         const int gx = (int)((a.n[0] + D4T_X - 1) / D4T_X), gy = (int)((a.n[1] + D4T_Y - 1) / D4T_Y);
-        const int gz = (int)((a.n[2] + D4T_Z - 1) / D4T_Z);
+        int zl = D4T_Z;
+        while (zl > 1 && (int64_t)gx * gy * ((a.n[2] + zl - 1) / zl) < 1024) zl /= 2;
+        const int gz = (int)((a.n[2] + zl - 1) / zl);
         const int64_t nb = (int64_t)gx * gy * gz;
-        hipLaunchKernelGGL((k_delta4_tile<W>), dim3((unsigned)nb), dim3(TPB), 0, st, a, out, gx, gy);
+        hipLaunchKernelGGL((k_delta4_tile<W>), dim3((unsigned)nb), dim3(TPB), 0, st, a, out, gx, gy, zl);
     } else
         hipLaunchKernelGGL((k_delta4<W, 1>), dim3(nblocks(a.N)), dim3(TPB), 0, st, a, out);
     CHECK_LAUNCH();

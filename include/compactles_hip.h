@@ -455,6 +455,16 @@ int cles_batch_run(cles_batch *batch, int64_t max_steps, int64_t nrec, double *d
 #define CLES_NOALLOC_DTS_MAX (1 << 20)
 int cles_batch_run_noalloc(cles_batch *batch, int64_t max_steps, int64_t nrec, double *dts,
                            int64_t *steps_taken);
+/* How cles_batch_run issues its iterations (the batch always takes every
+ * per-instance decision on the device).  CLES_RUN_HOST_DT and
+ * CLES_RUN_DEVICE_DT: every kernel launched from the host.
+ * CLES_RUN_STEP_GRAPH: groups of 8 iterations captured once as a HIP graph on
+ * the context's stream (at the first run; again after a change of nrec or of
+ * mode) and replayed; results are bitwise those of host issue.  While a
+ * capture is in progress no other thread may issue work to that stream.
+ * cles_batch_run_noalloc always issues from the host.  The reference has no
+ * counterpart (src/timestep.jl:277 loops on the host). */
+int cles_batch_set_run_mode(cles_batch *batch, int mode);
 /* Per-instance time, completed steps and status (CLES_OK or the check_step
  * code 4..8 it stopped with); NULL arrays are skipped. */
 int cles_batch_state(cles_batch *batch, double *t, int64_t *steps, int *status);

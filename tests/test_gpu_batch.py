@@ -64,6 +64,37 @@ def test_batch_run_parity(ctx):
         assert abs(ke[i] - ke_ref) <= 1e-10 * ke_ref, (i, ke[i], ke_ref)
 
 
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_batch_step_graph_equals_host(ctx, precision):
+    """Iterations replayed from the captured 8-iteration graph against the
+    same batch launched from the host: dt histories and states bitwise equal,
+    over a first call with 2 replays and 5 host-issued iterations, a second
+    call that replays the same graph, and a third with another history length
+    (a new capture); one tube stops on its tfinal inside a replay."""
+    specs = cases.tube_specs(7, seed=5, nmin=128, nmax=2048)
+    tf = [1e9] * len(specs)
+    tf[2] = 0.003
+    a = cases.tube_batch(ctx, specs, precision=precision, tfinal=tf)
+    b = cases.tube_batch(ctx, specs, precision=precision, tfinal=tf)
+    a.set_run_mode("step_graph")
+    b.set_run_mode("host_dt")
+    for steps, rec in ((21, 21), (16, 16), (9, 4)):
+        na, da = a.run(steps, record=rec)
+        nb, db = b.run(steps, record=rec)
+        assert na == nb and np.array_equal(da, db), (steps, na, nb)
+        ta, sa, xa = a.state()
+        tb, sb, xb = b.state()
+        assert np.array_equal(ta, tb) and np.array_equal(sa, sb) and np.array_equal(xa, xb)
+        for i in range(len(specs)):
+            for c in range(a.n_cons):
+                assert np.array_equal(a.field(i, f"Q[{c + 1}]"), b.field(i, f"Q[{c + 1}]")), (steps, i, c)
+    assert sa[2] < sa[0]   # the short tube stopped on its tfinal
+    with pytest.raises(ValueError):
+        a.set_run_mode("graph")
+    a.close()
+    b.close()
+
+
 def test_batch_same_alone_and_in_mix(ctx):
     specs = cases.tube_specs(12, seed=11, nmin=128, nmax=2048)
     nsteps = 15
