@@ -116,6 +116,7 @@ SIGNATURES = {
     "cles_solver_profile": (C.c_int, [P, C.c_int]),
     "cles_solver_profile_read": (C.c_int, [P, C.c_int, C.c_char_p, C.c_int, DP, C.POINTER(C.c_int64), DP]),
     "cles_solver_set_run_mode": (C.c_int, [P, C.c_int]),
+    "cles_solver_set_concurrency": (C.c_int, [P, C.c_int]),
     "cles_solver_save_checkpoint": (C.c_int, [P, C.c_char_p, C.c_int]),
     "cles_solver_load_checkpoint": (C.c_int, [P, C.c_char_p, C.c_int]),
     "cles_solver_footprint": (C.c_int, [C.POINTER(SolverDesc), C.POINTER(C.c_size_t)]),

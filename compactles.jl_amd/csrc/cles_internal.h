@@ -13,6 +13,10 @@
 
 namespace cles {
 
+// Grids up to this many points issue the independent chains of the RHS on
+// side streams (their launches are latency-bound and overlap well).
+constexpr double CONC_MAX_POINTS = 4194304.0;
+
 // ---------------------------------------------------------------------------
 // Errors
 // ---------------------------------------------------------------------------
@@ -143,7 +147,7 @@ struct EpiArgs {
     void *out;          // typed by the operator dtype (STORE/NEG/SUB/ART_*)
     void *out2;         // beta (ART_MUBETA)
     const void *a0;     // rho (ART_*), acc dQ (LSRK)
-    const void *a1;     // c (ART_KAPPA, ART_D)
+    const void *a1;     // c (ART_KAPPA, ART_D); LSRK: a second accumulator, a0 - a1 - v
     const void *a2;     // T (ART_KAPPA)
     double *du;         // LSRK
     double *Q;          // LSRK

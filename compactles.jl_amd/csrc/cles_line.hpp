@@ -63,9 +63,16 @@ __device__ __forceinline__ void epilogue(const EpiArgs &e, int64_t o, double v)
         out[o] = out[o] - (T)v;
     } else if constexpr (EPI == EPI_LSRK || EPI == EPI_LSRK_DEV) {
         // rhs.jl:665-667 dQ -= D_z F_z, then timestep.jl:85-94 on the FP64 state
-        // acc == NULL: this is the only active dimension, dQ = 0 - v
-        const T *acc = (const T *)e.a0;
-        const double dq = acc ? (double)(acc[o] - (T)v) : -v;
+        // acc == NULL: this is the only active dimension, dQ = 0 - v; a second
+        // accumulator (a1, the y divergence issued beside the x one) is
+        // subtracted first, as the sequential dQ -= D_y F_y would have
+        const T *acc = (const T *)e.a0, *acc1 = (const T *)e.a1;
+        double dq = -v;
+        if (acc) {
+            T a = acc[o];
+            if (acc1) a = a - acc1[o];
+            dq = (double)(a - (T)v);
+        }
         const double dt = EPI == EPI_LSRK_DEV ? *e.dtp : e.dt;
         double nv = fma(dt, dq, e.c0 * e.du[o]);
         e.du[o] = nv;
@@ -115,14 +122,19 @@ __device__ __forceinline__ void epilogue_batch(const EpiArgs &e, const int64_t (
         for (int i = 0; i < NB; i++)
             if (ok[i]) out[o[i]] = a[i] - v[i];
     } else if constexpr (EPI == EPI_LSRK || EPI == EPI_LSRK_DEV) {
-        const T *acc = (const T *)e.a0;
-        T a[NB];
+        const T *acc = (const T *)e.a0, *acc1 = (const T *)e.a1;
+        T a[NB], a1[NB];
         double du[NB], q[NB];
 #pragma unroll
         for (int i = 0; i < NB; i++) {
             a[i] = (ok[i] && acc) ? acc[o[i]] : T(0);
+            a1[i] = (ok[i] && acc && acc1) ? acc1[o[i]] : T(0);
             du[i] = ok[i] ? e.du[o[i]] : 0.0;
             q[i] = ok[i] ? e.Q[o[i]] : 0.0;
+        }
+        if (acc && acc1) {
+#pragma unroll
+            for (int i = 0; i < NB; i++) a[i] = a[i] - a1[i];
         }
         const double dt = EPI == EPI_LSRK_DEV ? *e.dtp : e.dt;
 #pragma unroll
@@ -183,6 +195,7 @@ __device__ __forceinline__ void select_field(const LineFields &mf, const T *__re
     if constexpr (EPI == EPI_LSRK || EPI == EPI_LSRK_DEV) {
         const int64_t o = mf.eoff[i];
         if (e.a0) e.a0 = (const T *)e.a0 + o;
+        if (e.a1) e.a1 = (const T *)e.a1 + o;
         e.du += o;
         e.Q += o;
     } else {

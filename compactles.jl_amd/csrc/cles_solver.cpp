@@ -98,6 +98,18 @@ struct cles_solver {
     const double *dev_dt = nullptr;
     const int *dev_gate = nullptr, *dev_filt = nullptr;
 
+    // Small grids (at most CONC_MAX_POINTS points, Cartesian, no folds): the
+    // independent chains of the RHS are issued on side streams that fork from
+    // and join the solver's stream through events, so that their launches
+    // overlap; each chain writes its own buffers (sensorK: the kappa* / D*_k
+    // sensors, dQy: the y divergence).  `cur` redirects st() while a chain is
+    // issued.  conc_on: cles_solver_set_concurrency.
+    bool conc = false, conc_on = true;
+    hipStream_t side[3] = {};
+    hipEvent_t fork_ev = nullptr, join_ev[3] = {};
+    hipStream_t cur = nullptr;
+    void *sensorK = nullptr, *dQy = nullptr;
+
     // per-phase event timing (cles_solver_profile)
     struct Phase {
         double ms = 0, bytes = 0;
@@ -154,7 +166,7 @@ struct cles_solver {
         pending.clear();
     }
 
-    hipStream_t st() const { return ctx->stream; }
+    hipStream_t st() const { return cur ? cur : ctx->stream; }
     size_t wsize() const { return prec == CLES_F64 ? 8 : 4; }
 
     void *alloc(size_t bytes)
@@ -181,6 +193,14 @@ struct cles_solver {
     ~cles_solver()
     {
         if (ctx && ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+        for (hipStream_t &sd : side)
+            if (sd) {
+                (void)hipStreamSynchronize(sd);
+                (void)hipStreamDestroy(sd);
+            }
+        if (fork_ev) (void)hipEventDestroy(fork_ev);
+        for (hipEvent_t e : join_ev)
+            if (e) (void)hipEventDestroy(e);
         for (auto &pd : pending) { evpool.push_back(pd.a); evpool.push_back(pd.b); }
         for (hipEvent_t e : evpool) (void)hipEventDestroy(e);
         if (gexec) (void)hipGraphExecDestroy(gexec);
@@ -407,7 +427,16 @@ struct Impl {
     // every field of one direction in one launch
     void velocity_gradients(bool scalars = false) const
     {
-        for (int k = 0; k < s.nact; k++) {
+        for (int k = 0; k < s.nact; k++) gradients_along(k, scalars);
+        if (s.curv) {   // rhs.jl:604 metric_correct_gradients!
+            const CurvArgs<W> a = curv_args();
+            s.timed(PH_OTHER, sizeof(W) * 10.0 * (double)s.N, [&] { launch_metric_grad<W>(a, s.st()); });
+        }
+    }
+    // the gradients along the k-th active dimension
+    void gradients_along(int k, bool scalars) const
+    {
+        {
             const int d = s.adim[k];
             const void *in[LINE_MF] = {s.u, s.v, s.w};
             void *out[LINE_MF] = {s.gu[d * 3], s.gu[d * 3 + 1], s.gu[d * 3 + 2]};
@@ -427,11 +456,27 @@ struct Impl {
             lines_par(DERIV, d, nf, in, out, par, store(out[0]), PH_DERIV);
             if (s.met.m[d] || s.curv) metric(out, nf, 0, d);   // rhs.jl:599,631,634 _scale_grad!
         }
-        if (s.curv) {   // rhs.jl:604 metric_correct_gradients!
-            const CurvArgs<W> a = curv_args();
-            s.timed(PH_OTHER, sizeof(W) * 10.0 * (double)s.N, [&] { launch_metric_grad<W>(a, s.st()); });
-        }
     }
+
+    // Launches of fn on side stream i, ordered after everything issued so far
+    // on the solver's stream; join(i) orders the solver's stream after them.
+    template <typename F>
+    void fork(int i, F &&fn) const
+    {
+        CLES_HIP(hipEventRecord(s.fork_ev, s.ctx->stream));
+        CLES_HIP(hipStreamWaitEvent(s.side[i], s.fork_ev, 0));
+        s.cur = s.side[i];
+        try {
+            fn();
+        } catch (...) {
+            s.cur = nullptr;
+            throw;
+        }
+        s.cur = nullptr;
+        CLES_HIP(hipEventRecord(s.join_ev[i], s.side[i]));
+    }
+    void join(int i) const { CLES_HIP(hipStreamWaitEvent(s.ctx->stream, s.join_ev[i], 0)); }
+    bool concurrent() const { return s.conc && s.conc_on && !s.prof; }
 
     CurvArgs<W> curv_args() const
     {
@@ -505,8 +550,14 @@ struct Impl {
     // artificial.jl:261-359 compute_artificial!
     void artificial() const
     {
+        if (!s.d.art_enabled) return;
+        art_mu_beta();
+        art_kappa_d(s.sensor);
+    }
+    // mu* and beta* (artificial.jl:267-310): needs the velocity gradients
+    void art_mu_beta() const
+    {
         const cles_solver_desc &d = s.d;
-        if (!d.art_enabled) return;
         const double w = sizeof(W);
         s.timed(PH_STRAIN, w * (3 * s.nact + 1) * (double)s.N,
                 [&] { launch_strain<W>(grads(), f(s.strain), s.N, s.st()); });
@@ -535,10 +586,17 @@ struct Impl {
                 launch_dilatation_beta<W>(f(s.rho), f(s.sensor), f(s.tmp2), f(s.beta_art), d.C_beta, s.N, s.st());
             });
         }
+    }
+    // kappa* and D*_k (artificial.jl:312-359): needs the primitives only;
+    // `sensor` is the chain's scratch field
+    void art_kappa_d(void *sensor) const
+    {
+        const cles_solver_desc &d = s.d;
+        const double w = sizeof(W);
         // kappa* from the internal-energy sensor (e stored by the primitives pass)
         D4Args<W> ae = d4(1);
         ae.f = f(s.esens);
-        s.timed(PH_DELTA4, 2 * w * (double)s.N, [&] { launch_delta4<W>(ae, 0, f(s.sensor), s.st()); });
+        s.timed(PH_DELTA4, 2 * w * (double)s.N, [&] { launch_delta4<W>(ae, 0, f(sensor), s.st()); });
         EpiArgs kp{};
         kp.kind = EPI_ART_KAPPA;
         kp.out = s.kappa_art;
@@ -546,18 +604,18 @@ struct Impl {
         kp.a1 = s.c;
         kp.a2 = s.T;
         kp.c0 = d.C_kappa;
-        smooth(s.sensor, kp);
+        smooth(sensor, kp);
         if (s.Ns > 1) {
             for (int sp = 0; sp < s.Ns; sp++) {
                 D4Args<W> b = d4(1);
                 b.f = f(s.Y[sp]);
-                s.timed(PH_DELTA4, 2 * w * (double)s.N, [&] { launch_delta4<W>(b, 0, f(s.sensor), s.st()); });
+                s.timed(PH_DELTA4, 2 * w * (double)s.N, [&] { launch_delta4<W>(b, 0, f(sensor), s.st()); });
                 EpiArgs dd{};
                 dd.kind = EPI_ART_D;
                 dd.out = s.D_art[sp];
                 dd.a1 = s.c;
                 dd.c0 = d.C_D;
-                smooth(s.sensor, dd);
+                smooth(sensor, dd);
             }
         }
     }
@@ -595,16 +653,66 @@ struct Impl {
 
     W *dQc(int c) const { return f(s.dQ) + (size_t)c * s.N; }
 
+    // rhs.jl:624-667 with the x and y divergences issued side by side (dQ =
+    // -D_x F_x and dQy = D_y F_y) and the z pass fused with the LSRK update
+    // reading dQ - dQy: the same operations in the same order as the
+    // sequential dQ -= D_y F_y, so the result is bitwise the same
+    void divergence_split(double A, double B, double dt) const
+    {
+        const void *in[3][LINE_MF];
+        void *out[2][LINE_MF];
+        int64_t eoff[LINE_MF];
+        for (int c = 0; c < s.C; c++) {
+            for (int k = 0; k < 3; k++) in[k][c] = s.flux[s.adim[k]][c];
+            out[0][c] = dQc(c);
+            out[1][c] = f(s.dQy) + (size_t)c * s.N;
+            eoff[c] = (int64_t)c * s.N;
+        }
+        fork(0, [&] { lines(s.dplan[s.adim[1]], s.adim[1], s.C, in[1], out[1], nullptr, store(out[1][0]), PH_DIV); });
+        lines(s.dplan[s.adim[0]], s.adim[0], s.C, in[0], out[0], nullptr, store(out[0][0], EPI_STORE_NEG), PH_DIV);
+        join(0);
+        EpiArgs e{};
+        e.kind = s.dev_dt ? EPI_LSRK_DEV : EPI_LSRK;
+        e.a0 = dQc(0);
+        e.a1 = s.dQy;
+        e.du = s.du;
+        e.Q = s.Q;
+        e.c0 = A;
+        e.c1 = B;
+        e.dt = dt;
+        e.dtp = s.dev_dt;
+        e.gate = s.dev_gate;
+        lines(s.dplan[s.adim[2]], s.adim[2], s.C, in[2], nullptr, eoff, e, PH_DIV_LSRK);
+    }
+
     // rhs.jl:624-694 compute_rhs!; with fuse, the last flux-divergence pass of
     // each component performs the LSRK stage update (A, B, dt).
     void rhs(bool primitives_current, bool fuse, double A, double B, double dt) const
     {
         if (!primitives_current) primitives();
-        // the temperature and species gradients (rhs.jl:608-622) are
-        // independent of the artificial terms and go with the velocity ones
-        velocity_gradients(true);
-        artificial();
+        const bool conc = concurrent();
+        if (conc) {
+            // the y and z gradients and the kappa* / D*_k chain beside the x
+            // gradients, mu* and beta*; every chain joins before the fluxes
+            for (int k = 1; k < s.nact; k++) fork(k - 1, [&] { gradients_along(k, true); });
+            if (s.d.art_enabled) fork(2, [&] { art_kappa_d(s.sensorK); });
+            gradients_along(0, true);
+            for (int k = 1; k < s.nact; k++) join(k - 1);
+            if (s.d.art_enabled) {
+                art_mu_beta();
+                join(2);
+            }
+        } else {
+            // the temperature and species gradients (rhs.jl:608-622) are
+            // independent of the artificial terms and go with the velocity ones
+            velocity_gradients(true);
+            artificial();
+        }
         fluxes();
+        if (conc && fuse && s.nact == 3) {
+            divergence_split(A, B, dt);
+            return;
+        }
         // rhs.jl:624-667 per direction, every component in one launch
         for (int k = 0; k < s.nact; k++) {
             const int d = s.adim[k];
@@ -1146,6 +1254,8 @@ int cles_solver_footprint(const cles_solver_desc *desc, size_t *bytes)
             fields += 3.0 * ((d.n_global[1] == 1) + (d.metric == CLES_METRIC_SPHERICAL && d.n_global[2] == 1));
             if (d.metric == CLES_METRIC_SPHERICAL && d.n_global[1] > 1) fields += 8.0 / w;
         }
+        if (N <= cles::CONC_MAX_POINTS)   // the side chains' buffers (an upper bound: folds and metrics have none)
+            fields += (d.art_enabled ? 1 : 0) + (nact == 3 ? C : 0);
         const double total = N * C * 16.0 + N * C * w + N * w * fields + 1024.0 * 2 * 8 + 16 * 8 + plans;
         *bytes = (size_t)total;
     })
@@ -1453,6 +1563,15 @@ int cles_solver_create_geom(cles_ctx *ctx, const cles_solver_desc *desc, const d
                 double *cotr = (double *)s->alloc((size_t)s->N * sizeof(double));
                 dispatch(s, [&](auto &im) { im.gcl_cotr(cotr); });
                 s->met.cotr = cotr;
+            }
+            if ((double)N <= cles::CONC_MAX_POINTS && !s->geom() && !s->fold[0].on && !s->fold[1].on &&
+                !s->fold[2].on) {
+                for (hipStream_t &sd : s->side) CLES_HIP(hipStreamCreateWithFlags(&sd, hipStreamNonBlocking));
+                CLES_HIP(hipEventCreateWithFlags(&s->fork_ev, hipEventDisableTiming));
+                for (hipEvent_t &e : s->join_ev) CLES_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+                if (desc->art_enabled) s->sensorK = s->wfield();
+                if (s->nact == 3) s->dQy = s->alloc(N * s->C * s->wsize());
+                s->conc = true;
             }
             CLES_HIP(hipStreamSynchronize(ctx->stream));
         } catch (...) {
@@ -1794,6 +1913,15 @@ int run_graph(cles_solver *s, double tfinal, int64_t nmax, double *dts, int64_t 
 }
 
 }  // namespace
+
+int cles_solver_set_concurrency(cles_solver *s, int on)
+{
+    CLES_TRY({
+        CLES_REQUIRE(s, "NULL argument");
+        if (s->conc_on != (on != 0)) drop_graph(s);
+        s->conc_on = on != 0;
+    })
+}
 
 int cles_solver_set_run_mode(cles_solver *s, int mode)
 {

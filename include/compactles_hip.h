@@ -395,6 +395,13 @@ int cles_solver_profile_read(cles_solver *solver, int index, char *name, int nam
  * is CLES_RUN_HOST_DT, which profiling (cles_solver_profile) always uses. */
 enum { CLES_RUN_HOST_DT = 0, CLES_RUN_DEVICE_DT = 1, CLES_RUN_STEP_GRAPH = 2 };
 int cles_solver_set_run_mode(cles_solver *solver, int mode);
+/* Grids of at most 4194304 points without folds or metrics issue the
+ * independent chains of compute_rhs! (the y and z gradients, the kappa* /
+ * D*_k sensors, the y flux divergence) on side streams that fork from and
+ * join the solver's stream, so their launches overlap; results are bitwise
+ * those of one stream.  on = 0 keeps every launch on the solver's stream
+ * (default 1; profiling always does).  No counterpart in the reference. */
+int cles_solver_set_concurrency(cles_solver *solver, int on);
 
 /* Device-resident state geometry: interior extents and per-field element
  * count (the solver stores unpadded column-major fields). */
