@@ -116,7 +116,6 @@ SIGNATURES = {
     "cles_solver_profile": (C.c_int, [P, C.c_int]),
     "cles_solver_profile_read": (C.c_int, [P, C.c_int, C.c_char_p, C.c_int, DP, C.POINTER(C.c_int64), DP]),
     "cles_solver_set_run_mode": (C.c_int, [P, C.c_int]),
-    "cles_solver_set_concurrency": (C.c_int, [P, C.c_int]),
     "cles_solver_save_checkpoint": (C.c_int, [P, C.c_char_p, C.c_int]),
     "cles_solver_load_checkpoint": (C.c_int, [P, C.c_char_p, C.c_int]),
     "cles_solver_footprint": (C.c_int, [C.POINTER(SolverDesc), C.POINTER(C.c_size_t)]),
@@ -126,6 +125,7 @@ SIGNATURES = {
     "cles_solver_bc_switched": (C.c_int, [P, C.c_int, C.c_int, C.POINTER(C.c_int)]),
     "cles_solver_memory": (C.c_int, [P, C.POINTER(C.c_size_t)]),
     "cles_server_create": (C.c_int, [C.c_int, C.c_size_t, C.c_int, PP]),
+    "cles_server_set_run_mode": (C.c_int, [P, C.c_int]),
     "cles_server_destroy": (C.c_int, [P]),
     "cles_server_submit": (C.c_int, [P, C.POINTER(SolverDesc), DP, C.POINTER(C.c_int64), C.POINTER(C.c_int),
                                      C.c_double, C.c_int64, C.POINTER(C.c_int64)]),

@@ -616,11 +616,6 @@ class Solver(SolverSpec):
         """io.jl:49-75 load_checkpoint!: restores Q, t and step."""
         L.check(self.ctx.lib.cles_solver_load_checkpoint(self.handle, prefix.encode(), int(rank)))
 
-    def set_concurrency(self, on: bool):
-        """Side-stream chains of the RHS on small grids (default on; bitwise
-        the same results as one stream)."""
-        L.check(self.ctx.lib.cles_solver_set_concurrency(self.handle, int(bool(on))))
-
     RUN_MODES = {"host_dt": 0, "device_dt": 1, "step_graph": 2}
 
     def set_run_mode(self, mode: str):

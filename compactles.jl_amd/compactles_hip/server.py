@@ -9,18 +9,20 @@ import ctypes as C
 import numpy as np
 
 from . import _lib as L
-from .problem import Numerics, Problem, SolverSpec, initialize_
+from .problem import Numerics, Problem, Solver, SolverSpec, initialize_
 
 QUEUED, RUNNING, DONE, FAILED = range(4)
 
 
 class Server:
-    def __init__(self, device: int = 0, hbm_budget: int = 64 << 30, runners: int = 4):
+    def __init__(self, device: int = 0, hbm_budget: int = 64 << 30, runners: int = 4, run_mode: str = "host_dt"):
+        """run_mode: how each request's solver issues run! (Solver.set_run_mode)."""
         self.lib = L.load()
         h = C.c_void_p()
         L.check(self.lib.cles_server_create(int(device), C.c_size_t(int(hbm_budget)), int(runners), C.byref(h)))
         self.handle = h
         self._specs = {}
+        L.check(self.lib.cles_server_set_run_mode(self.handle, Solver.RUN_MODES[run_mode]))
 
     def submit(self, prob: Problem, num: Numerics, tfinal: float, nmax: int = 2 ** 62, Q: np.ndarray | None = None):
         """Queue setup(prob, num) + run!(...; tfinal, nmax); Q (padded host

@@ -60,6 +60,10 @@ struct cles_batch {
     void *Y[CLES_MAXSP] = {}, *gu[3] = {}, *gT = nullptr, *gY[CLES_MAXSP] = {};
     void *mu_art = nullptr, *beta_art = nullptr, *kappa_art = nullptr, *D_art[CLES_MAXSP] = {};
     void *sensor = nullptr, *strain = nullptr, *esens = nullptr, *tmp1 = nullptr, *tmp2 = nullptr;
+    // the sensors of mu*/beta*, kappa* and D*_k side by side (stride N), and
+    // their coefficient epilogues (device table) for one smoothing launch
+    int nsens = 0;
+    cles::EpiArgs *art_multi = nullptr;
     void *flux[CLES_MAXSP + 4] = {};
 
     // run! control per instance
@@ -287,49 +291,32 @@ struct BImpl {
     }
 
     // artificial.jl:261-359 along the single active dimension: one smoothing
-    // pass, which is also the last and carries the coefficient epilogue
+    // pass, which is also the last and carries the coefficient epilogue.  The
+    // mu*/beta*, kappa* and D*_k sensors are smoothed together in one launch
+    // (each field with its own epilogue); the dilatation sensor's beta* goes
+    // after them, as it overwrites beta*.
     void artificial() const
     {
         const cles_solver_desc &d = b.d;
         if (!d.art_enabled) return;
         const BatchGeom g = geom();
+        const size_t ws = (size_t)b.N * b.wsize();
+        auto sens = [&](int i) { return f((char *)b.sensor + (size_t)i * ws); };
         launch_strain<W>(grads(), f(b.strain), b.N, b.st());
-        launch_delta4_batch<W>(f(b.strain), f(b.sensor), g, 2, b.st());
-        EpiArgs mb{};
-        mb.kind = EPI_ART_MUBETA;
-        mb.out = b.mu_art;
-        mb.out2 = b.beta_art;
-        mb.a0 = b.rho;
-        mb.c0 = d.C_mu;
-        mb.c1 = d.C_beta;
-        line(true, b.sensor, mb);
+        launch_delta4_batch<W>(f(b.strain), sens(0), g, 2, b.st());
+        launch_delta4_batch<W>(f(b.esens), sens(1), g, 1, b.st());
+        for (int sp = 0; sp + 2 < b.nsens; sp++) launch_delta4_batch<W>(f(b.Y[sp]), sens(2 + sp), g, 1, b.st());
+        EpiArgs m{};
+        m.kind = EPI_ART_MULTI;
+        m.multi = b.art_multi;
+        lines(true, b.nsens, b.sensor, m);
         if (d.beta_sensor == 1) {
             launch_gate_beta<W>(grads(), f(b.beta_art), b.N, b.st());
         } else if (d.beta_sensor == 2) {
             launch_dilatation_prep<W>(grads(), f(b.tmp1), f(b.tmp2), b.N, b.st());
-            launch_delta4_batch<W>(f(b.tmp1), f(b.sensor), g, 2, b.st());
+            launch_delta4_batch<W>(f(b.tmp1), sens(0), g, 2, b.st());
             line(true, b.sensor, store(b.sensor));
-            launch_dilatation_beta<W>(f(b.rho), f(b.sensor), f(b.tmp2), f(b.beta_art), d.C_beta, b.N, b.st());
-        }
-        launch_delta4_batch<W>(f(b.esens), f(b.sensor), g, 1, b.st());
-        EpiArgs kp{};
-        kp.kind = EPI_ART_KAPPA;
-        kp.out = b.kappa_art;
-        kp.a0 = b.rho;
-        kp.a1 = b.c;
-        kp.a2 = b.T;
-        kp.c0 = d.C_kappa;
-        line(true, b.sensor, kp);
-        if (b.Ns > 1) {
-            for (int sp = 0; sp < b.Ns; sp++) {
-                launch_delta4_batch<W>(f(b.Y[sp]), f(b.sensor), g, 1, b.st());
-                EpiArgs dd{};
-                dd.kind = EPI_ART_D;
-                dd.out = b.D_art[sp];
-                dd.a1 = b.c;
-                dd.c0 = d.C_D;
-                line(true, b.sensor, dd);
-            }
+            launch_dilatation_beta<W>(f(b.rho), sens(0), f(b.tmp2), f(b.beta_art), d.C_beta, b.N, b.st());
         }
     }
 
@@ -631,13 +618,36 @@ int cles_batch_create(cles_ctx *ctx, const cles_solver_desc *desc, int64_t count
             if (b->Ns > 1)
                 for (int k = 0; k < b->Ns; k++) b->D_art[k] = b->wfield();
             if (d.art_enabled) {
-                b->sensor = b->wfield();
+                b->nsens = 2 + (b->Ns > 1 ? b->Ns : 0);
+                b->sensor = blockf(b->nsens);
                 b->strain = b->wfield();
                 b->esens = b->wfield();
                 if (d.beta_sensor == 2) {
                     b->tmp1 = b->wfield();
                     b->tmp2 = b->wfield();
                 }
+            }
+            if (d.art_enabled) {
+                std::vector<cles::EpiArgs> ep((size_t)b->nsens);
+                ep[0].kind = cles::EPI_ART_MUBETA;
+                ep[0].out = b->mu_art;
+                ep[0].out2 = b->beta_art;
+                ep[0].a0 = b->rho;
+                ep[0].c0 = d.C_mu;
+                ep[0].c1 = d.C_beta;
+                ep[1].kind = cles::EPI_ART_KAPPA;
+                ep[1].out = b->kappa_art;
+                ep[1].a0 = b->rho;
+                ep[1].a1 = b->c;
+                ep[1].a2 = b->T;
+                ep[1].c0 = d.C_kappa;
+                for (int k = 2; k < b->nsens; k++) {
+                    ep[k].kind = cles::EPI_ART_D;
+                    ep[k].out = b->D_art[k - 2];
+                    ep[k].a1 = b->c;
+                    ep[k].c0 = d.C_D;
+                }
+                b->art_multi = b->upload_vec(ep);
             }
             const size_t cnt = (size_t)count;
             for (double **fp : {&b->t, &b->dt, &b->dt_prev, &b->rate_prev, &b->dt_seen, &b->rate, &b->rhomin,

@@ -13,10 +13,6 @@
 
 namespace cles {
 
-// Grids up to this many points issue the independent chains of the RHS on
-// side streams (their launches are latency-bound and overlap well).
-constexpr double CONC_MAX_POINTS = 4194304.0;
-
 // ---------------------------------------------------------------------------
 // Errors
 // ---------------------------------------------------------------------------
@@ -139,6 +135,7 @@ enum EpiKind {
     EPI_ART_D,          // D = C0 c max(v,0)
     EPI_LSRK_DEV,       // EPI_LSRK with dt = *dtp, and no store where *gate == 0
     EPI_STORE_GATE,     // EPI_STORE, and no store where *gate == 0
+    EPI_ART_MULTI,      // field i of the launch applies the ART_* epilogue multi[i]
     EPI_NKIND
 };
 
@@ -147,7 +144,7 @@ struct EpiArgs {
     void *out;          // typed by the operator dtype (STORE/NEG/SUB/ART_*)
     void *out2;         // beta (ART_MUBETA)
     const void *a0;     // rho (ART_*), acc dQ (LSRK)
-    const void *a1;     // c (ART_KAPPA, ART_D); LSRK: a second accumulator, a0 - a1 - v
+    const void *a1;     // c (ART_KAPPA, ART_D)
     const void *a2;     // T (ART_KAPPA)
     double *du;         // LSRK
     double *Q;          // LSRK
@@ -155,13 +152,14 @@ struct EpiArgs {
     double dt;          // LSRK
     const double *dtp;  // EPI_LSRK_DEV: dt in device memory
     const int *gate;    // EPI_LSRK_DEV, EPI_STORE_GATE: store only when *gate != 0
+    const EpiArgs *multi;   // EPI_ART_MULTI: one ART_* epilogue per field (device memory)
 };
 
 // Several independent fields through one line operator in one launch (grid
 // dimension y = field).  Field 0 is the launch's own input and epilogue; field
 // i >= 1 reads in[i] and stores through out[i] (STORE, STORE_NEG, SUB,
 // STORE_GATE), or, for the LSRK kinds, offsets the accumulator, du and Q by
-// eoff[i] elements.  Small grids gain waves per launch and every grid loses
+// eoff[i] elements, or (ART_MULTI) takes the whole epilogue multi[i].  Small grids gain waves per launch and every grid loses
 // the per-launch gaps.
 constexpr int LINE_MF = 12;
 struct LineFields {

@@ -46,13 +46,15 @@ void launch_line(const HostPlan &hp, int dim, const View3 &v, const void *f, con
     if (m.nf > 1) {
         const int k = e.kind;
         if (k != EPI_STORE && k != EPI_STORE_NEG && k != EPI_SUB && k != EPI_STORE_GATE && k != EPI_LSRK &&
-            k != EPI_LSRK_DEV)
-            throw Error{CLES_ERR_INVALID, "several fields per launch need a store, subtract or LSRK epilogue"};
+            k != EPI_LSRK_DEV && k != EPI_ART_MULTI)
+            throw Error{CLES_ERR_INVALID, "several fields per launch need a store, subtract, LSRK or multi epilogue"};
         for (int i = 1; i < m.nf; i++) {
             if (!m.in[i]) throw Error{CLES_ERR_INVALID, "NULL field input"};
-            if (k != EPI_LSRK && k != EPI_LSRK_DEV && !m.out[i]) throw Error{CLES_ERR_INVALID, "NULL field output"};
+            if (k != EPI_LSRK && k != EPI_LSRK_DEV && k != EPI_ART_MULTI && !m.out[i])
+                throw Error{CLES_ERR_INVALID, "NULL field output"};
         }
     }
+    if (e.kind == EPI_ART_MULTI && !e.multi) throw Error{CLES_ERR_INVALID, "the multi epilogue needs its table"};
     if (hp.dtype == CLES_F64)
         launch_typed<double>(hp, dim, v, (const double *)f, e, st, m);
     else

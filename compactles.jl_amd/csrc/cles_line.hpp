@@ -50,6 +50,12 @@ template <> __device__ __forceinline__ const float *rowp<float>(const LineDev &p
 template <typename T, int EPI, bool NT = false>
 __device__ __forceinline__ void epilogue(const EpiArgs &e, int64_t o, double v)
 {
+    if constexpr (EPI == EPI_ART_MULTI) {   // e is the field's own ART_* epilogue (select_field)
+        if (e.kind == EPI_ART_MUBETA) epilogue<T, EPI_ART_MUBETA, NT>(e, o, v);
+        else if (e.kind == EPI_ART_KAPPA) epilogue<T, EPI_ART_KAPPA, NT>(e, o, v);
+        else epilogue<T, EPI_ART_D, NT>(e, o, v);
+        return;
+    }
     if constexpr (EPI == EPI_STORE_GATE || EPI == EPI_LSRK_DEV)
         if (!*e.gate) return;
     if constexpr (EPI == EPI_STORE || EPI == EPI_STORE_GATE || EPI == EPI_STORE_NEG) {
@@ -63,16 +69,9 @@ __device__ __forceinline__ void epilogue(const EpiArgs &e, int64_t o, double v)
         out[o] = out[o] - (T)v;
     } else if constexpr (EPI == EPI_LSRK || EPI == EPI_LSRK_DEV) {
         // rhs.jl:665-667 dQ -= D_z F_z, then timestep.jl:85-94 on the FP64 state
-        // acc == NULL: this is the only active dimension, dQ = 0 - v; a second
-        // accumulator (a1, the y divergence issued beside the x one) is
-        // subtracted first, as the sequential dQ -= D_y F_y would have
-        const T *acc = (const T *)e.a0, *acc1 = (const T *)e.a1;
-        double dq = -v;
-        if (acc) {
-            T a = acc[o];
-            if (acc1) a = a - acc1[o];
-            dq = (double)(a - (T)v);
-        }
+        // acc == NULL: this is the only active dimension, dQ = 0 - v
+        const T *acc = (const T *)e.a0;
+        const double dq = acc ? (double)(acc[o] - (T)v) : -v;
         const double dt = EPI == EPI_LSRK_DEV ? *e.dtp : e.dt;
         double nv = fma(dt, dq, e.c0 * e.du[o]);
         e.du[o] = nv;
@@ -107,6 +106,12 @@ template <typename T, int EPI, int NB, bool NT = false>
 __device__ __forceinline__ void epilogue_batch(const EpiArgs &e, const int64_t (&o)[NB], const T (&v)[NB],
                                                const bool (&ok)[NB])
 {
+    if constexpr (EPI == EPI_ART_MULTI) {
+        if (e.kind == EPI_ART_MUBETA) epilogue_batch<T, EPI_ART_MUBETA, NB, NT>(e, o, v, ok);
+        else if (e.kind == EPI_ART_KAPPA) epilogue_batch<T, EPI_ART_KAPPA, NB, NT>(e, o, v, ok);
+        else epilogue_batch<T, EPI_ART_D, NB, NT>(e, o, v, ok);
+        return;
+    }
     if constexpr (EPI == EPI_STORE_GATE || EPI == EPI_LSRK_DEV)
         if (!*e.gate) return;
     if constexpr (EPI == EPI_STORE || EPI == EPI_STORE_NEG || EPI == EPI_STORE_GATE) {
@@ -122,19 +127,14 @@ __device__ __forceinline__ void epilogue_batch(const EpiArgs &e, const int64_t (
         for (int i = 0; i < NB; i++)
             if (ok[i]) out[o[i]] = a[i] - v[i];
     } else if constexpr (EPI == EPI_LSRK || EPI == EPI_LSRK_DEV) {
-        const T *acc = (const T *)e.a0, *acc1 = (const T *)e.a1;
-        T a[NB], a1[NB];
+        const T *acc = (const T *)e.a0;
+        T a[NB];
         double du[NB], q[NB];
 #pragma unroll
         for (int i = 0; i < NB; i++) {
             a[i] = (ok[i] && acc) ? acc[o[i]] : T(0);
-            a1[i] = (ok[i] && acc && acc1) ? acc1[o[i]] : T(0);
             du[i] = ok[i] ? e.du[o[i]] : 0.0;
             q[i] = ok[i] ? e.Q[o[i]] : 0.0;
-        }
-        if (acc && acc1) {
-#pragma unroll
-            for (int i = 0; i < NB; i++) a[i] = a[i] - a1[i];
         }
         const double dt = EPI == EPI_LSRK_DEV ? *e.dtp : e.dt;
 #pragma unroll
@@ -190,15 +190,15 @@ template <typename T, int EPI>
 __device__ __forceinline__ void select_field(const LineFields &mf, const T *__restrict__ &f, EpiArgs &e)
 {
     const int i = blockIdx.y;
+    if constexpr (EPI == EPI_ART_MULTI) e = e.multi[i];
     if (i == 0) return;
     f = (const T *)mf.in[i];
     if constexpr (EPI == EPI_LSRK || EPI == EPI_LSRK_DEV) {
         const int64_t o = mf.eoff[i];
         if (e.a0) e.a0 = (const T *)e.a0 + o;
-        if (e.a1) e.a1 = (const T *)e.a1 + o;
         e.du += o;
         e.Q += o;
-    } else {
+    } else if constexpr (EPI != EPI_ART_MULTI) {
         e.out = mf.out[i];
     }
 }
@@ -1266,6 +1266,7 @@ void launch_epi(const HostPlan &hp, int dim, const View3 &v, const T *f, const E
         case EPI_ART_MUBETA: return launch_variant<T, Q, M, SYM, EPI_ART_MUBETA>(hp, dim, v, f, e, st, mf);
         case EPI_ART_KAPPA: return launch_variant<T, Q, M, SYM, EPI_ART_KAPPA>(hp, dim, v, f, e, st, mf);
         case EPI_ART_D: return launch_variant<T, Q, M, SYM, EPI_ART_D>(hp, dim, v, f, e, st, mf);
+        case EPI_ART_MULTI: return launch_variant<T, Q, M, SYM, EPI_ART_MULTI>(hp, dim, v, f, e, st, mf);
         default: break;
         }
     } else {

@@ -57,6 +57,7 @@ __global__ void __launch_bounds__(64) k_line_rag(RagLines r, const T *__restrict
     const int item = (int)(logical / (unsigned)r.nf), fld = (int)(logical % (unsigned)r.nf);
     const int64_t fo = (int64_t)fld * r.fstride;
     f += fo;
+    if constexpr (EPI == EPI_ART_MULTI) e = e.multi[fld];   // its own outputs: no field offset
     const int *it = r.items + 3 * item;
     const int P = it[0], start = it[1], count = it[2];
     const int TS = 33 * P;   // rag_slot(32 P - 1) < 33 P
@@ -132,7 +133,7 @@ __global__ void __launch_bounds__(64) k_line_rag(RagLines r, const T *__restrict
         const int nj = r.plans[ij].n;
         const int64_t oj = r.off[ij];
         EpiArgs ej = e;
-        if (fo) {
+        if (EPI != EPI_ART_MULTI && fo) {
             if (ej.out) ej.out = (T *)ej.out + fo;
             if (ej.du) ej.du += fo;
             if (ej.Q) ej.Q += fo;
@@ -159,8 +160,9 @@ template <typename T, int Q, int M, int SYM, int EPI>
 void launch_rag_variant(const RagLines &r, const T *f, const EpiArgs &e, hipStream_t st)
 {
     if (r.nitems <= 0 || r.nf <= 0) return;
-    if (r.nf > 1 && e.kind != EPI_STORE && e.kind != EPI_STORE_NEG && e.kind != EPI_LSRK)
-        throw Error{CLES_ERR_INVALID, "several fields per launch need a store or LSRK epilogue"};
+    if (r.nf > 1 && e.kind != EPI_STORE && e.kind != EPI_STORE_NEG && e.kind != EPI_LSRK && e.kind != EPI_ART_MULTI)
+        throw Error{CLES_ERR_INVALID, "several fields per launch need a store, LSRK or multi epilogue"};
+    if (e.kind == EPI_ART_MULTI && !e.multi) throw Error{CLES_ERR_INVALID, "the multi epilogue needs its table"};
     if (e.kind == EPI_LSRK && e.a0 && r.nf > 1)
         throw Error{CLES_ERR_INVALID, "the ragged LSRK epilogue takes no accumulator with several fields"};
     const size_t shm = (size_t)64 * 33 * sizeof(T);
@@ -178,6 +180,7 @@ void launch_rag_epi(const RagLines &r, const T *f, const EpiArgs &e, hipStream_t
         case EPI_ART_MUBETA: return launch_rag_variant<T, Q, M, SYM, EPI_ART_MUBETA>(r, f, e, st);
         case EPI_ART_KAPPA: return launch_rag_variant<T, Q, M, SYM, EPI_ART_KAPPA>(r, f, e, st);
         case EPI_ART_D: return launch_rag_variant<T, Q, M, SYM, EPI_ART_D>(r, f, e, st);
+        case EPI_ART_MULTI: return launch_rag_variant<T, Q, M, SYM, EPI_ART_MULTI>(r, f, e, st);
         default: break;
         }
     } else {

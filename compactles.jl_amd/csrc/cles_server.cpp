@@ -70,6 +70,7 @@ struct cles_server {
     bool stop = false;
     int64_t next_id = 1;
     int running = 0;
+    int run_mode = CLES_RUN_HOST_DT;   // of every solver it creates
 
     // under mu: the next admissible request (taken off the queue), or nullptr
     Request *admit_locked()
@@ -94,6 +95,7 @@ struct cles_server {
         cles_solver *s = nullptr;
         int rc = cles_ctx_create(device, &ctx);
         if (rc == CLES_OK) rc = cles_solver_create(ctx, &r->desc, &s);
+        if (rc == CLES_OK) rc = cles_solver_set_run_mode(s, run_mode);
         if (rc == CLES_OK) rc = cles_solver_upload(s, r->Q.data(), r->padded, r->halo);
         int run_rc = CLES_OK;
         std::string run_msg;
@@ -182,6 +184,16 @@ int cles_server_destroy(cles_server *sv)
         sv->cv.notify_all();
         for (auto &t : sv->workers) t.join();
         delete sv;
+    })
+}
+
+int cles_server_set_run_mode(cles_server *sv, int run_mode)
+{
+    CLES_TRY({
+        CLES_REQUIRE(sv, "NULL argument");
+        CLES_REQUIRE(run_mode >= CLES_RUN_HOST_DT && run_mode <= CLES_RUN_STEP_GRAPH, "unknown run mode");
+        std::lock_guard<std::mutex> lk(sv->mu);
+        sv->run_mode = run_mode;
     })
 }
 

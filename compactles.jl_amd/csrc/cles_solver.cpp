@@ -98,17 +98,10 @@ struct cles_solver {
     const double *dev_dt = nullptr;
     const int *dev_gate = nullptr, *dev_filt = nullptr;
 
-    // Small grids (at most CONC_MAX_POINTS points, Cartesian, no folds): the
-    // independent chains of the RHS are issued on side streams that fork from
-    // and join the solver's stream through events, so that their launches
-    // overlap; each chain writes its own buffers (sensorK: the kappa* / D*_k
-    // sensors, dQy: the y divergence).  `cur` redirects st() while a chain is
-    // issued.  conc_on: cles_solver_set_concurrency.
-    bool conc = false, conc_on = true;
-    hipStream_t side[3] = {};
-    hipEvent_t fork_ev = nullptr, join_ev[3] = {};
-    hipStream_t cur = nullptr;
-    void *sensorK = nullptr, *dQy = nullptr;
+    // the kappa* and D*_k sensors beside `sensor` (mu*, beta*), smoothed in
+    // the same launches, and the coefficient epilogue of each (device table)
+    void *sensorK = nullptr, *sensD[CLES_MAXSP] = {};
+    cles::EpiArgs *art_multi = nullptr;
 
     // per-phase event timing (cles_solver_profile)
     struct Phase {
@@ -166,7 +159,7 @@ struct cles_solver {
         pending.clear();
     }
 
-    hipStream_t st() const { return cur ? cur : ctx->stream; }
+    hipStream_t st() const { return ctx->stream; }
     size_t wsize() const { return prec == CLES_F64 ? 8 : 4; }
 
     void *alloc(size_t bytes)
@@ -193,14 +186,6 @@ struct cles_solver {
     ~cles_solver()
     {
         if (ctx && ctx->stream) (void)hipStreamSynchronize(ctx->stream);
-        for (hipStream_t &sd : side)
-            if (sd) {
-                (void)hipStreamSynchronize(sd);
-                (void)hipStreamDestroy(sd);
-            }
-        if (fork_ev) (void)hipEventDestroy(fork_ev);
-        for (hipEvent_t e : join_ev)
-            if (e) (void)hipEventDestroy(e);
         for (auto &pd : pending) { evpool.push_back(pd.a); evpool.push_back(pd.b); }
         for (hipEvent_t e : evpool) (void)hipEventDestroy(e);
         if (gexec) (void)hipGraphExecDestroy(gexec);
@@ -283,6 +268,8 @@ struct Impl {
         case EPI_ART_MUBETA: return 4 * w;          // sensor, rho -> mu, beta
         case EPI_ART_KAPPA: return 5 * w;           // sensor, rho, c, T -> kappa
         case EPI_ART_D: return 3 * w;               // sensor, c -> D
+        case EPI_ART_MULTI:                         // the mean over mu*/beta*, kappa*, D*_k
+            return w * (4.0 + 5.0 + 3.0 * (s.Ns > 1 ? s.Ns : 0)) / (2 + (s.Ns > 1 ? s.Ns : 0));
         default: return 2 * w;                      // read f, write out
         }
     }
@@ -458,25 +445,6 @@ struct Impl {
         }
     }
 
-    // Launches of fn on side stream i, ordered after everything issued so far
-    // on the solver's stream; join(i) orders the solver's stream after them.
-    template <typename F>
-    void fork(int i, F &&fn) const
-    {
-        CLES_HIP(hipEventRecord(s.fork_ev, s.ctx->stream));
-        CLES_HIP(hipStreamWaitEvent(s.side[i], s.fork_ev, 0));
-        s.cur = s.side[i];
-        try {
-            fn();
-        } catch (...) {
-            s.cur = nullptr;
-            throw;
-        }
-        s.cur = nullptr;
-        CLES_HIP(hipEventRecord(s.join_ev[i], s.side[i]));
-    }
-    void join(int i) const { CLES_HIP(hipStreamWaitEvent(s.ctx->stream, s.join_ev[i], 0)); }
-    bool concurrent() const { return s.conc && s.conc_on && !s.prof; }
 
     CurvArgs<W> curv_args() const
     {
@@ -550,9 +518,48 @@ struct Impl {
     // artificial.jl:261-359 compute_artificial!
     void artificial() const
     {
-        if (!s.d.art_enabled) return;
-        art_mu_beta();
-        art_kappa_d(s.sensor);
+        const cles_solver_desc &d = s.d;
+        if (!d.art_enabled) return;
+        if (d.beta_sensor == 2) {   // the dilatation sensor reuses `sensor` between the two chains
+            art_mu_beta();
+            art_kappa_d(s.sensor);
+            return;
+        }
+        // the mu*/beta*, kappa* and D*_k sensors, then their smoothing passes
+        // in shared launches (the last with each field's own epilogue)
+        const double w = sizeof(W);
+        s.timed(PH_STRAIN, w * (3 * s.nact + 1) * (double)s.N,
+                [&] { launch_strain<W>(grads(), f(s.strain), s.N, s.st()); });
+        void *sens[LINE_MF] = {s.sensor, s.sensorK};
+        const void *src[LINE_MF] = {s.strain, s.esens};
+        int nf = 2;
+        if (s.Ns > 1)
+            for (int sp = 0; sp < s.Ns; sp++) {
+                sens[nf] = s.sensD[sp];
+                src[nf++] = s.Y[sp];
+            }
+        for (int i = 0; i < nf; i++) {
+            D4Args<W> a = d4(i == 0 ? 2 : 1);
+            a.f = f((void *)src[i]);
+            s.timed(PH_DELTA4, 2 * w * (double)s.N, [&] { launch_delta4<W>(a, 0, f(sens[i]), s.st()); });
+        }
+        for (int k = 0; k < s.nact; k++) {
+            const int dd = s.adim[k];
+            if (k < s.nact - 1) {
+                int par[LINE_MF];
+                for (int i = 0; i < nf; i++) par[i] = 1;
+                lines_par(FILT, dd, nf, sens, sens, par, store(sens[0]), PH_SMOOTH);
+            } else {
+                // a paired fold never folds the last active dimension
+                EpiArgs m{};
+                m.kind = EPI_ART_MULTI;
+                m.multi = s.art_multi;
+                lines(plan(FILT, dd, 1), dd, nf, sens, nullptr, nullptr, m, PH_SMOOTH);
+            }
+        }
+        if (d.beta_sensor == 1)
+            s.timed(PH_OTHER, w * (3 * s.nact + 2) * (double)s.N,
+                    [&] { launch_gate_beta<W>(grads(), f(s.beta_art), s.N, s.st()); });
     }
     // mu* and beta* (artificial.jl:267-310): needs the velocity gradients
     void art_mu_beta() const
@@ -653,66 +660,16 @@ struct Impl {
 
     W *dQc(int c) const { return f(s.dQ) + (size_t)c * s.N; }
 
-    // rhs.jl:624-667 with the x and y divergences issued side by side (dQ =
-    // -D_x F_x and dQy = D_y F_y) and the z pass fused with the LSRK update
-    // reading dQ - dQy: the same operations in the same order as the
-    // sequential dQ -= D_y F_y, so the result is bitwise the same
-    void divergence_split(double A, double B, double dt) const
-    {
-        const void *in[3][LINE_MF];
-        void *out[2][LINE_MF];
-        int64_t eoff[LINE_MF];
-        for (int c = 0; c < s.C; c++) {
-            for (int k = 0; k < 3; k++) in[k][c] = s.flux[s.adim[k]][c];
-            out[0][c] = dQc(c);
-            out[1][c] = f(s.dQy) + (size_t)c * s.N;
-            eoff[c] = (int64_t)c * s.N;
-        }
-        fork(0, [&] { lines(s.dplan[s.adim[1]], s.adim[1], s.C, in[1], out[1], nullptr, store(out[1][0]), PH_DIV); });
-        lines(s.dplan[s.adim[0]], s.adim[0], s.C, in[0], out[0], nullptr, store(out[0][0], EPI_STORE_NEG), PH_DIV);
-        join(0);
-        EpiArgs e{};
-        e.kind = s.dev_dt ? EPI_LSRK_DEV : EPI_LSRK;
-        e.a0 = dQc(0);
-        e.a1 = s.dQy;
-        e.du = s.du;
-        e.Q = s.Q;
-        e.c0 = A;
-        e.c1 = B;
-        e.dt = dt;
-        e.dtp = s.dev_dt;
-        e.gate = s.dev_gate;
-        lines(s.dplan[s.adim[2]], s.adim[2], s.C, in[2], nullptr, eoff, e, PH_DIV_LSRK);
-    }
-
     // rhs.jl:624-694 compute_rhs!; with fuse, the last flux-divergence pass of
     // each component performs the LSRK stage update (A, B, dt).
     void rhs(bool primitives_current, bool fuse, double A, double B, double dt) const
     {
         if (!primitives_current) primitives();
-        const bool conc = concurrent();
-        if (conc) {
-            // the y and z gradients and the kappa* / D*_k chain beside the x
-            // gradients, mu* and beta*; every chain joins before the fluxes
-            for (int k = 1; k < s.nact; k++) fork(k - 1, [&] { gradients_along(k, true); });
-            if (s.d.art_enabled) fork(2, [&] { art_kappa_d(s.sensorK); });
-            gradients_along(0, true);
-            for (int k = 1; k < s.nact; k++) join(k - 1);
-            if (s.d.art_enabled) {
-                art_mu_beta();
-                join(2);
-            }
-        } else {
-            // the temperature and species gradients (rhs.jl:608-622) are
-            // independent of the artificial terms and go with the velocity ones
-            velocity_gradients(true);
-            artificial();
-        }
+        // the temperature and species gradients (rhs.jl:608-622) are
+        // independent of the artificial terms and go with the velocity ones
+        velocity_gradients(true);
+        artificial();
         fluxes();
-        if (conc && fuse && s.nact == 3) {
-            divergence_split(A, B, dt);
-            return;
-        }
         // rhs.jl:624-667 per direction, every component in one launch
         for (int k = 0; k < s.nact; k++) {
             const int d = s.adim[k];
@@ -1238,7 +1195,7 @@ int cles_solver_footprint(const cles_solver_desc *desc, size_t *bytes)
         double fields = 8 + Ns;                                   // primitives, Y
         fields += nact * (3 + 1 + (Ns > 1 ? Ns : 0) + C);         // grad_u, grad_T, grad_Y, fluxes
         fields += 3 + (Ns > 1 ? Ns : 0);                          // mu*, beta*, kappa*, D*
-        if (d.art_enabled) fields += 3 + (d.beta_sensor == 2 ? 2 : 0);
+        if (d.art_enabled) fields += 4 + (Ns > 1 ? Ns : 0) + (d.beta_sensor == 2 ? 2 : 0);
         for (int k = 0; k < 3; k++)   // a SwitchableBC arm that may run NSCBC holds the scratch too
             for (int sd = 0; sd < 2; sd++)
                 nscbc = nscbc || (d.switchable[k][sd] && d.n_global[k] > 1 &&
@@ -1254,8 +1211,6 @@ int cles_solver_footprint(const cles_solver_desc *desc, size_t *bytes)
             fields += 3.0 * ((d.n_global[1] == 1) + (d.metric == CLES_METRIC_SPHERICAL && d.n_global[2] == 1));
             if (d.metric == CLES_METRIC_SPHERICAL && d.n_global[1] > 1) fields += 8.0 / w;
         }
-        if (N <= cles::CONC_MAX_POINTS)   // the side chains' buffers (an upper bound: folds and metrics have none)
-            fields += (d.art_enabled ? 1 : 0) + (nact == 3 ? C : 0);
         const double total = N * C * 16.0 + N * C * w + N * w * fields + 1024.0 * 2 * 8 + 16 * 8 + plans;
         *bytes = (size_t)total;
     })
@@ -1498,6 +1453,9 @@ int cles_solver_create_geom(cles_ctx *ctx, const cles_solver_desc *desc, const d
                 for (int k = 0; k < s->Ns; k++) s->D_art[k] = s->wfield();
             if (desc->art_enabled) {
                 s->sensor = s->wfield();
+                s->sensorK = s->wfield();
+                if (s->Ns > 1)
+                    for (int k = 0; k < s->Ns; k++) s->sensD[k] = s->wfield();
                 s->strain = s->wfield();
                 s->esens = s->wfield();
                 if (desc->beta_sensor == 2) {
@@ -1554,6 +1512,29 @@ int cles_solver_create_geom(cles_ctx *ctx, const cles_solver_desc *desc, const d
                     s->met.cth = sd + s->n[1];
                 }
             }
+            if (desc->art_enabled) {
+                std::vector<cles::EpiArgs> ep(2 + (s->Ns > 1 ? s->Ns : 0));
+                ep[0].kind = cles::EPI_ART_MUBETA;
+                ep[0].out = s->mu_art;
+                ep[0].out2 = s->beta_art;
+                ep[0].a0 = s->rho;
+                ep[0].c0 = desc->C_mu;
+                ep[0].c1 = desc->C_beta;
+                ep[1].kind = cles::EPI_ART_KAPPA;
+                ep[1].out = s->kappa_art;
+                ep[1].a0 = s->rho;
+                ep[1].a1 = s->c;
+                ep[1].a2 = s->T;
+                ep[1].c0 = desc->C_kappa;
+                for (size_t k = 2; k < ep.size(); k++) {
+                    ep[k].kind = cles::EPI_ART_D;
+                    ep[k].out = s->D_art[k - 2];
+                    ep[k].a1 = s->c;
+                    ep[k].c0 = desc->C_D;
+                }
+                s->art_multi = (cles::EpiArgs *)s->alloc(ep.size() * sizeof(cles::EpiArgs));
+                s->upload(s->art_multi, ep.data(), ep.size() * sizeof(cles::EpiArgs));
+            }
             s->part = (double *)s->alloc((size_t)s->nred * 2 * sizeof(double));
             s->scal = (double *)s->alloc(16 * sizeof(double));
             CLES_HIP(hipHostMalloc((void **)&s->host_scal, 16 * sizeof(double), hipHostMallocDefault));
@@ -1563,15 +1544,6 @@ int cles_solver_create_geom(cles_ctx *ctx, const cles_solver_desc *desc, const d
                 double *cotr = (double *)s->alloc((size_t)s->N * sizeof(double));
                 dispatch(s, [&](auto &im) { im.gcl_cotr(cotr); });
                 s->met.cotr = cotr;
-            }
-            if ((double)N <= cles::CONC_MAX_POINTS && !s->geom() && !s->fold[0].on && !s->fold[1].on &&
-                !s->fold[2].on) {
-                for (hipStream_t &sd : s->side) CLES_HIP(hipStreamCreateWithFlags(&sd, hipStreamNonBlocking));
-                CLES_HIP(hipEventCreateWithFlags(&s->fork_ev, hipEventDisableTiming));
-                for (hipEvent_t &e : s->join_ev) CLES_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
-                if (desc->art_enabled) s->sensorK = s->wfield();
-                if (s->nact == 3) s->dQy = s->alloc(N * s->C * s->wsize());
-                s->conc = true;
             }
             CLES_HIP(hipStreamSynchronize(ctx->stream));
         } catch (...) {
@@ -1913,15 +1885,6 @@ int run_graph(cles_solver *s, double tfinal, int64_t nmax, double *dts, int64_t 
 }
 
 }  // namespace
-
-int cles_solver_set_concurrency(cles_solver *s, int on)
-{
-    CLES_TRY({
-        CLES_REQUIRE(s, "NULL argument");
-        if (s->conc_on != (on != 0)) drop_graph(s);
-        s->conc_on = on != 0;
-    })
-}
 
 int cles_solver_set_run_mode(cles_solver *s, int mode)
 {

@@ -395,13 +395,6 @@ int cles_solver_profile_read(cles_solver *solver, int index, char *name, int nam
  * is CLES_RUN_HOST_DT, which profiling (cles_solver_profile) always uses. */
 enum { CLES_RUN_HOST_DT = 0, CLES_RUN_DEVICE_DT = 1, CLES_RUN_STEP_GRAPH = 2 };
 int cles_solver_set_run_mode(cles_solver *solver, int mode);
-/* Grids of at most 4194304 points without folds or metrics issue the
- * independent chains of compute_rhs! (the y and z gradients, the kappa* /
- * D*_k sensors, the y flux divergence) on side streams that fork from and
- * join the solver's stream, so their launches overlap; results are bitwise
- * those of one stream.  on = 0 keeps every launch on the solver's stream
- * (default 1; profiling always does).  No counterpart in the reference. */
-int cles_solver_set_concurrency(cles_solver *solver, int on);
 
 /* Device-resident state geometry: interior extents and per-field element
  * count (the solver stores unpadded column-major fields). */
@@ -501,6 +494,9 @@ typedef struct cles_server cles_server;
 enum { CLES_REQ_QUEUED = 0, CLES_REQ_RUNNING = 1, CLES_REQ_DONE = 2, CLES_REQ_FAILED = 3 };
 
 int cles_server_create(int device, size_t hbm_budget, int runners, cles_server **server);
+/* Run mode (cles_solver_set_run_mode) of the solvers of requests admitted
+ * from now on; default CLES_RUN_HOST_DT. */
+int cles_server_set_run_mode(cles_server *server, int run_mode);
 /* Marks queued requests CLES_REQ_FAILED and drops them, waits for running
  * ones, frees everything.  No other call on the server (cles_server_wait
  * included) may overlap it. */

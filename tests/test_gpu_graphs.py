@@ -118,31 +118,3 @@ def test_run_modes_timing_small_grids(ctx):
         print(f"\nTGV {n}^3 ms/step: " + ", ".join(f"{k} {v:.3f}" for k, v in out.items()))
         assert out["step_graph"] < 1.5 * out["host_dt"]
 
-
-@pytest.mark.parametrize("mode", ["host_dt", "step_graph"])
-@pytest.mark.parametrize("precision", ["fp64", "fp32"])
-def test_side_stream_chains_equal_one_stream(ctx, precision, mode):
-    """The RHS chains issued on side streams (small grids) against every
-    launch on the solver's stream: bitwise equal dt histories and states, in
-    3-D (split x / y divergence) and on the 2-D shock / interface with NSCBC
-    faces."""
-    a, b = _tgv(ctx, precision=precision), _tgv(ctx, precision=precision)
-    b.set_concurrency(False)
-    for s in (a, b):
-        s.set_run_mode(mode)
-    da = a.run(1e9, 20, record=20)
-    db = b.run(1e9, 20, record=20)
-    assert np.array_equal(da, db)
-    assert np.array_equal(_state(a), _state(b))
-    import compactles_hip as ch
-    from compactles_hip import workloads as W
-    pair = []
-    for on in (True, False):
-        prob, num, Q = W.shock_problem(ch, (96, 32, 1), precision)
-        s = ch.Solver(ctx, prob, num)
-        s.upload(Q)
-        s.set_concurrency(on)
-        s.set_run_mode(mode)
-        pair.append((s, s.run(1e9, 12, record=12)))
-    assert np.array_equal(pair[0][1], pair[1][1])
-    assert np.array_equal(_state(pair[0][0]), _state(pair[1][0]))

@@ -26,10 +26,9 @@ def test_footprint_grows_with_grid_and_precision():
     c = ch.SolverSpec(prob, num32).footprint()
     assert c < b
     # the TGV case's allocation table: Q, du, dQ (15), primitives and Y (9), per dimension three velocity
-    # gradients, a temperature gradient and five fluxes (27), mu*, beta*, kappa*, sensor, strain, e (6),
-    # and on a grid this small the side-stream chains' kappa* sensor (1) and y divergence (5):
-    # 63 FP64 fields of 64^3, plus a few per cent of plan tables at most
-    assert 63 * 8 * 64 ** 3 <= b <= 1.05 * 63 * 8 * 64 ** 3
+    # gradients, a temperature gradient and five fluxes (27), mu*, beta*, kappa*, the mu*/beta* and
+    # kappa* sensors, strain, e (7): 58 FP64 fields of 64^3, plus a few per cent of plan tables at most
+    assert 58 * 8 * 64 ** 3 <= b <= 1.05 * 58 * 8 * 64 ** 3
 
 
 def test_server_rejects_request_beyond_budget():
