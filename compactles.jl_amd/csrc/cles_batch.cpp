@@ -303,9 +303,17 @@ struct BImpl {
         const size_t ws = (size_t)b.N * b.wsize();
         auto sens = [&](int i) { return f((char *)b.sensor + (size_t)i * ws); };
         launch_strain<W>(grads(), f(b.strain), b.N, b.st());
-        launch_delta4_batch<W>(f(b.strain), sens(0), g, 2, b.st());
-        launch_delta4_batch<W>(f(b.esens), sens(1), g, 1, b.st());
-        for (int sp = 0; sp + 2 < b.nsens; sp++) launch_delta4_batch<W>(f(b.Y[sp]), sens(2 + sp), g, 1, b.st());
+        for (int i0 = 0; i0 < b.nsens; i0 += D4_MF) {   // D4_MF fields per launch
+            D4Multi<W> m4{};
+            m4.nf = std::min(D4_MF, b.nsens - i0);
+            for (int k = 0; k < m4.nf; k++) {
+                const int i = i0 + k;
+                m4.f[k] = i == 0 ? f(b.strain) : i == 1 ? f(b.esens) : f(b.Y[i - 2]);
+                m4.out[k] = sens(i);
+                m4.wpow[k] = i == 0 ? 2 : 1;
+            }
+            launch_delta4_batch_multi<W>(m4, g, b.st());
+        }
         EpiArgs m{};
         m.kind = EPI_ART_MULTI;
         m.multi = b.art_multi;

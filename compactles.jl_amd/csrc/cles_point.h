@@ -47,6 +47,18 @@ struct D4Args {
     double wd[3];        // h_d^wpow
 };
 
+// Up to D4_MF fields through one delta4 launch (grid y = field): field i
+// reads f[i] and writes out[i] with the weights h_d^wpow[i] (h_d from
+// D4Args::wd with wpow = 1, or the weights as given when wpow[i] == 0).
+constexpr int D4_MF = 4;
+template <typename W>
+struct D4Multi {
+    int nf;
+    const W *f[D4_MF];
+    W *out[D4_MF];
+    int wpow[D4_MF];
+};
+
 template <typename W>
 struct FluxArgs {
     const double *Q;
@@ -226,6 +238,7 @@ void launch_metric_scale(const MetricFields<W> &fs, const Metric1D &g, int mode,
 template <typename W> void launch_primitives(const PrimArgs<W> &a, int64_t N, hipStream_t st);
 template <typename W> void launch_strain(const GradPtrs<W> &g, W *out, int64_t N, hipStream_t st);
 template <typename W> void launch_delta4(const D4Args<W> &a, int mode, W *out, hipStream_t st);
+template <typename W> void launch_delta4_multi(const D4Args<W> &a, const D4Multi<W> &m, hipStream_t st);
 template <typename W> void launch_gate_beta(const GradPtrs<W> &g, W *beta, int64_t N, hipStream_t st);
 template <typename W>
 void launch_dilatation_prep(const GradPtrs<W> &g, W *div, W *sw, int64_t N, hipStream_t st);
@@ -245,6 +258,8 @@ void launch_rate_batch(const RateArgs<W> &a, const BatchGeom &g, const int *mask
                        hipStream_t st);
 template <typename W>
 void launch_delta4_batch(const W *f, W *out, const BatchGeom &g, int wpow, hipStream_t st);
+template <typename W>
+void launch_delta4_batch_multi(const D4Multi<W> &m, const BatchGeom &g, hipStream_t st);
 void launch_batch_dt(const BatchCtl &c, hipStream_t st);
 void launch_batch_post(const BatchCtl &c, hipStream_t st);
 void launch_batch_bcs(const BatchGeom &g, const BatchBC &b, double *Q, hipStream_t st);

@@ -350,8 +350,13 @@ __device__ __forceinline__ bool d4_needed(int v, int n, bool active)
 }
 
 template <typename W>
-__global__ void __launch_bounds__(TPB) k_delta4_tile(D4Args<W> a, W *__restrict__ out, int gx, int gy, int zl)
+__global__ void __launch_bounds__(TPB) k_delta4_tile(D4Args<W> a, D4Multi<W> fm, int gx, int gy, int zl)
 {
+    const int fi = (int)blockIdx.y;
+    W *__restrict__ out = fm.out[fi];
+    double wd[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) wd[k] = fm.wpow[fi] == 2 ? a.wd[k] * a.wd[k] : a.wd[k];
     __shared__ W lds[D4T_LY * D4T_LX];   // holds input values, so W is exact
     const unsigned blk = xcd_block(blockIdx.x, gridDim.x);
     const int bx = (int)(blk % (unsigned)gx);
@@ -365,7 +370,7 @@ __global__ void __launch_bounds__(TPB) k_delta4_tile(D4Args<W> a, W *__restrict_
     const int tx = (int)(threadIdx.x & 63), ty = (int)(threadIdx.x >> 6);
     const int i0 = bx * D4T_X, j0 = by * D4T_Y, k0 = bz * zl;
     const int i = i0 + tx;
-    const W *f = a.f;
+    const W *f = fm.f[fi];
     const double D4[5] = {1.0, -4.0, 6.0, -4.0, 1.0};
     // own points: rows j0 + ty + 4q, q = 0..3
     double w[4][5];
@@ -449,19 +454,19 @@ __global__ void __launch_bounds__(TPB) k_delta4_tile(D4Args<W> a, W *__restrict_
                     double acc = 0.0;
 #pragma unroll
                     for (int m = 0; m < 5; m++) acc += D4[m] * (double)lds[yl * D4T_LX + xl + m - 2];
-                    acc_out += (W)(a.wd[0] * fabs(acc));
+                    acc_out += (W)(wd[0] * fabs(acc));
                 }
                 if (ya) {
                     double acc = 0.0;
 #pragma unroll
                     for (int m = 0; m < 5; m++) acc += D4[m] * (double)lds[(yl + m - 2) * D4T_LX + xl];
-                    acc_out += (W)(a.wd[1] * fabs(acc));
+                    acc_out += (W)(wd[1] * fabs(acc));
                 }
                 if (za) {
                     double acc = 0.0;
 #pragma unroll
                     for (int m = 0; m < 5; m++) acc += D4[m] * w[q][m];
-                    acc_out += (W)(a.wd[2] * fabs(acc));
+                    acc_out += (W)(wd[2] * fabs(acc));
                 }
                 out[(int64_t)k * sz + (int64_t)(j0 + yl - 2) * sy + i] = acc_out;
             }
@@ -1207,9 +1212,11 @@ __global__ void __launch_bounds__(TPB) k_sum_final(const double *__restrict__ pa
 // ---------------------------------------------------------------------------
 // artificial.jl:67-94 along x of each instance (one block per instance)
 template <typename W>
-__global__ void __launch_bounds__(TPB) k_delta4_batch(const W *__restrict__ f, W *__restrict__ out, BatchGeom g,
-                                                      int wpow)
+__global__ void __launch_bounds__(TPB) k_delta4_batch(D4Multi<W> fm, BatchGeom g)
 {
+    const W *__restrict__ f = fm.f[blockIdx.y];
+    W *__restrict__ out = fm.out[blockIdx.y];
+    const int wpow = fm.wpow[blockIdx.y];
     const int64_t i = blockIdx.x;
     const int64_t o = g.off[i];
     const int n = g.n[i];
@@ -1413,14 +1420,29 @@ template <typename W>
 void launch_delta4(const D4Args<W> &a, int mode, W *out, hipStream_t st)
 {
     if (mode == 0) {
-        const int gx = (int)((a.n[0] + D4T_X - 1) / D4T_X), gy = (int)((a.n[1] + D4T_Y - 1) / D4T_Y);
-        int zl = D4T_Z;
-        while (zl > 1 && (int64_t)gx * gy * ((a.n[2] + zl - 1) / zl) < 1024) zl /= 2;
-        const int gz = (int)((a.n[2] + zl - 1) / zl);
-        const int64_t nb = (int64_t)gx * gy * gz;
-        hipLaunchKernelGGL((k_delta4_tile<W>), dim3((unsigned)nb), dim3(TPB), 0, st, a, out, gx, gy, zl);
+        D4Multi<W> m{};
+        m.nf = 1;
+        m.f[0] = a.f;
+        m.out[0] = out;
+        m.wpow[0] = 1;   // a.wd holds the weights
+        launch_delta4_multi<W>(a, m, st);
+        return;
     } else
         hipLaunchKernelGGL((k_delta4<W, 1>), dim3(nblocks(a.N)), dim3(TPB), 0, st, a, out);
+    CHECK_LAUNCH();
+}
+
+// a.wd: h_d (the fields' wpow apply), or the weights themselves with wpow 1
+template <typename W>
+void launch_delta4_multi(const D4Args<W> &a, const D4Multi<W> &m, hipStream_t st)
+{
+    if (m.nf < 1 || m.nf > D4_MF) throw Error{CLES_ERR_INVALID, "delta4 takes 1..4 fields per launch"};
+    const int gx = (int)((a.n[0] + D4T_X - 1) / D4T_X), gy = (int)((a.n[1] + D4T_Y - 1) / D4T_Y);
+    int zl = D4T_Z;
+    while (zl > 1 && (int64_t)gx * gy * ((a.n[2] + zl - 1) / zl) * m.nf < 1024) zl /= 2;
+    const int gz = (int)((a.n[2] + zl - 1) / zl);
+    const int64_t nb = (int64_t)gx * gy * gz;
+    hipLaunchKernelGGL((k_delta4_tile<W>), dim3((unsigned)nb, (unsigned)m.nf), dim3(TPB), 0, st, a, m, gx, gy, zl);
     CHECK_LAUNCH();
 }
 
@@ -1488,14 +1510,27 @@ template void launch_rate_batch<float>(const RateArgs<float> &, const BatchGeom 
                                        hipStream_t);
 
 template <typename W>
-void launch_delta4_batch(const W *f, W *out, const BatchGeom &g, int wpow, hipStream_t st)
+void launch_delta4_batch_multi(const D4Multi<W> &m, const BatchGeom &g, hipStream_t st)
 {
     if (g.count <= 0) return;
-    hipLaunchKernelGGL(k_delta4_batch<W>, dim3((unsigned)g.count), dim3(TPB), 0, st, f, out, g, wpow);
+    if (m.nf < 1 || m.nf > D4_MF) throw Error{CLES_ERR_INVALID, "delta4 takes 1..4 fields per launch"};
+    hipLaunchKernelGGL(k_delta4_batch<W>, dim3((unsigned)g.count, (unsigned)m.nf), dim3(TPB), 0, st, m, g);
     CHECK_LAUNCH();
+}
+template <typename W>
+void launch_delta4_batch(const W *f, W *out, const BatchGeom &g, int wpow, hipStream_t st)
+{
+    D4Multi<W> m{};
+    m.nf = 1;
+    m.f[0] = f;
+    m.out[0] = out;
+    m.wpow[0] = wpow;
+    launch_delta4_batch_multi<W>(m, g, st);
 }
 template void launch_delta4_batch<double>(const double *, double *, const BatchGeom &, int, hipStream_t);
 template void launch_delta4_batch<float>(const float *, float *, const BatchGeom &, int, hipStream_t);
+template void launch_delta4_batch_multi<double>(const D4Multi<double> &, const BatchGeom &, hipStream_t);
+template void launch_delta4_batch_multi<float>(const D4Multi<float> &, const BatchGeom &, hipStream_t);
 
 void launch_batch_dt(const BatchCtl &c, hipStream_t st)
 {
@@ -1612,6 +1647,7 @@ void launch_convert(Dst *d, const Src *s, int64_t N, hipStream_t st)
     template void launch_primitives<W>(const PrimArgs<W> &, int64_t, hipStream_t);                        \
     template void launch_strain<W>(const GradPtrs<W> &, W *, int64_t, hipStream_t);                        \
     template void launch_delta4<W>(const D4Args<W> &, int, W *, hipStream_t);                              \
+    template void launch_delta4_multi<W>(const D4Args<W> &, const D4Multi<W> &, hipStream_t);              \
     template void launch_gate_beta<W>(const GradPtrs<W> &, W *, int64_t, hipStream_t);                     \
     template void launch_dilatation_prep<W>(const GradPtrs<W> &, W *, W *, int64_t, hipStream_t);          \
     template void launch_dilatation_beta<W>(const W *, const W *, const W *, W *, double, int64_t,         \

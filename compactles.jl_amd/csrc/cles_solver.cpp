@@ -538,10 +538,17 @@ struct Impl {
                 sens[nf] = s.sensD[sp];
                 src[nf++] = s.Y[sp];
             }
-        for (int i = 0; i < nf; i++) {
-            D4Args<W> a = d4(i == 0 ? 2 : 1);
-            a.f = f((void *)src[i]);
-            s.timed(PH_DELTA4, 2 * w * (double)s.N, [&] { launch_delta4<W>(a, 0, f(sens[i]), s.st()); });
+        // D4_MF fields per launch: strain with the h^2 weights, the others with h
+        for (int i0 = 0; i0 < nf; i0 += D4_MF) {
+            D4Args<W> a = d4(1);
+            D4Multi<W> m{};
+            m.nf = std::min(D4_MF, nf - i0);
+            for (int i = 0; i < m.nf; i++) {
+                m.f[i] = f((void *)src[i0 + i]);
+                m.out[i] = f(sens[i0 + i]);
+                m.wpow[i] = i0 + i == 0 ? 2 : 1;
+            }
+            s.timed(PH_DELTA4, 2 * w * m.nf * (double)s.N, [&] { launch_delta4_multi<W>(a, m, s.st()); });
         }
         for (int k = 0; k < s.nact; k++) {
             const int dd = s.adim[k];
