@@ -12,6 +12,7 @@ from .problem import (ArtParams, AxisBC, ConstantBodyForce, DirichletBC, Extrapo
                       StiffenedGas, Stretch, sine_cluster, CartesianMetric, CylindricalMetric, SphericalMetric,
                       Transport, conserved_from_prim, initialize_, run_, setup, single_species, tanh_blend)
 from .batch import Batch, Instance  # noqa: F401
+from .stack import Stack  # noqa: F401
 from .server import Server  # noqa: F401
 from . import tune  # noqa: F401
 from .callbacks import AtTime, Callback, EveryStep, EveryTime, ProgressLog, WhenState  # noqa: F401

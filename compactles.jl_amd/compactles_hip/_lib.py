@@ -124,6 +124,12 @@ SIGNATURES = {
     "cles_solver_switch_bc": (C.c_int, [P, C.c_int, C.c_int]),
     "cles_solver_bc_switched": (C.c_int, [P, C.c_int, C.c_int, C.POINTER(C.c_int)]),
     "cles_solver_memory": (C.c_int, [P, C.POINTER(C.c_size_t)]),
+    "cles_stack_create": (C.c_int, [P, C.c_void_p, C.c_int, PP]),
+    "cles_stack_destroy": (C.c_int, [P]),
+    "cles_stack_upload": (C.c_int, [P, C.c_int, DP, C.POINTER(C.c_int64), C.POINTER(C.c_int)]),
+    "cles_stack_download": (C.c_int, [P, C.c_int, DP, C.POINTER(C.c_int64), C.POINTER(C.c_int)]),
+    "cles_stack_run": (C.c_int, [P, DP, C.POINTER(C.c_int64), C.c_int64, C.POINTER(C.c_int64)]),
+    "cles_stack_state": (C.c_int, [P, DP, C.POINTER(C.c_int64), C.POINTER(C.c_int)]),
     "cles_server_create": (C.c_int, [C.c_int, C.c_size_t, C.c_int, PP]),
     "cles_server_set_run_mode": (C.c_int, [P, C.c_int]),
     "cles_server_destroy": (C.c_int, [P]),

@@ -152,6 +152,8 @@ struct EpiArgs {
     double dt;          // LSRK
     const double *dtp;  // EPI_LSRK_DEV: dt in device memory
     const int *gate;    // EPI_LSRK_DEV, EPI_STORE_GATE: store only when *gate != 0
+    unsigned ipts;      //   ... or, when ipts > 0, only where gate[o / ipts] != 0 with dt = dtp[o / ipts]
+                        //   (instances of ipts points side by side)
     const EpiArgs *multi;   // EPI_ART_MULTI: one ART_* epilogue per field (device memory)
 };
 

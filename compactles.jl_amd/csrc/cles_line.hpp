@@ -56,8 +56,13 @@ __device__ __forceinline__ void epilogue(const EpiArgs &e, int64_t o, double v)
         else epilogue<T, EPI_ART_D, NT>(e, o, v);
         return;
     }
-    if constexpr (EPI == EPI_STORE_GATE || EPI == EPI_LSRK_DEV)
-        if (!*e.gate) return;
+    // gated kinds: one flag and dt, or (ipts > 0: instances of ipts points side
+    // by side) the flag and dt of the point's instance
+    [[maybe_unused]] unsigned ii = 0;
+    if constexpr (EPI == EPI_STORE_GATE || EPI == EPI_LSRK_DEV) {
+        if (e.ipts) ii = (unsigned)o / e.ipts;
+        if (!e.gate[ii]) return;
+    }
     if constexpr (EPI == EPI_STORE || EPI == EPI_STORE_GATE || EPI == EPI_STORE_NEG) {
         const T val = EPI == EPI_STORE_NEG ? (T)(-v) : (T)v;
         if constexpr (NT)
@@ -72,7 +77,7 @@ __device__ __forceinline__ void epilogue(const EpiArgs &e, int64_t o, double v)
         // acc == NULL: this is the only active dimension, dQ = 0 - v
         const T *acc = (const T *)e.a0;
         const double dq = acc ? (double)(acc[o] - (T)v) : -v;
-        const double dt = EPI == EPI_LSRK_DEV ? *e.dtp : e.dt;
+        const double dt = EPI == EPI_LSRK_DEV ? e.dtp[ii] : e.dt;
         double nv = fma(dt, dq, e.c0 * e.du[o]);
         e.du[o] = nv;
         e.Q[o] += e.c1 * nv;
@@ -112,12 +117,37 @@ __device__ __forceinline__ void epilogue_batch(const EpiArgs &e, const int64_t (
         else epilogue_batch<T, EPI_ART_D, NB, NT>(e, o, v, ok);
         return;
     }
-    if constexpr (EPI == EPI_STORE_GATE || EPI == EPI_LSRK_DEV)
-        if (!*e.gate) return;
+    // gated kinds: one flag and dt for the launch, or (ipts > 0) those of
+    // each point's instance
+    bool okk[NB];
+    double dtv[NB];
+#pragma unroll
+    for (int i = 0; i < NB; i++) {
+        okk[i] = ok[i];
+        dtv[i] = e.dt;
+    }
+    if constexpr (EPI == EPI_STORE_GATE || EPI == EPI_LSRK_DEV) {
+        if (e.ipts) {
+#pragma unroll
+            for (int i = 0; i < NB; i++) {
+                const unsigned ii = ok[i] ? (unsigned)o[i] / e.ipts : 0u;
+                okk[i] = ok[i] && e.gate[ii] != 0;
+                if constexpr (EPI == EPI_LSRK_DEV) dtv[i] = e.dtp[ii];
+            }
+        } else {
+            if (!*e.gate) return;
+            if constexpr (EPI == EPI_LSRK_DEV) {
+                const double d = *e.dtp;
+#pragma unroll
+                for (int i = 0; i < NB; i++) dtv[i] = d;
+            }
+        }
+    }
     if constexpr (EPI == EPI_STORE || EPI == EPI_STORE_NEG || EPI == EPI_STORE_GATE) {
+        constexpr int E1 = EPI == EPI_STORE_GATE ? (int)EPI_STORE : EPI;   // the gate is applied
 #pragma unroll
         for (int i = 0; i < NB; i++)
-            if (ok[i]) epilogue<T, EPI, NT>(e, o[i], (double)v[i]);
+            if (okk[i]) epilogue<T, E1, NT>(e, o[i], (double)v[i]);
     } else if constexpr (EPI == EPI_SUB) {
         T *out = (T *)e.out;
         T a[NB];
@@ -132,16 +162,15 @@ __device__ __forceinline__ void epilogue_batch(const EpiArgs &e, const int64_t (
         double du[NB], q[NB];
 #pragma unroll
         for (int i = 0; i < NB; i++) {
-            a[i] = (ok[i] && acc) ? acc[o[i]] : T(0);
-            du[i] = ok[i] ? e.du[o[i]] : 0.0;
-            q[i] = ok[i] ? e.Q[o[i]] : 0.0;
+            a[i] = (okk[i] && acc) ? acc[o[i]] : T(0);
+            du[i] = okk[i] ? e.du[o[i]] : 0.0;
+            q[i] = okk[i] ? e.Q[o[i]] : 0.0;
         }
-        const double dt = EPI == EPI_LSRK_DEV ? *e.dtp : e.dt;
 #pragma unroll
         for (int i = 0; i < NB; i++) {
-            if (!ok[i]) continue;
+            if (!okk[i]) continue;
             const double dq = acc ? (double)(a[i] - v[i]) : -(double)v[i];
-            const double nv = fma(dt, dq, e.c0 * du[i]);
+            const double nv = fma(dtv[i], dq, e.c0 * du[i]);
             e.du[o[i]] = nv;
             e.Q[o[i]] = q[i] + e.c1 * nv;
         }

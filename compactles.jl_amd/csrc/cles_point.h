@@ -248,9 +248,12 @@ void launch_dilatation_beta(const W *rho, const W *sensor, const W *sw, W *beta,
 template <typename W> void launch_fluxes(const FluxArgs<W> &a, int64_t N, hipStream_t st);
 template <typename W>
 void launch_lsrk(double *Q, double *du, const W *dQ, double A, double B, double dt, const double *dtp,
-                 const int *gate, int64_t NC, hipStream_t st);
+                 const int *gate, int64_t NC, int64_t N, unsigned ipts, hipStream_t st);
 template <typename W>
 void launch_rate(const RateArgs<W> &a, double *part, int nb, double *out, int64_t N, hipStream_t st);
+template <typename W>
+void launch_rate_inst(const RateArgs<W> &a, double *part, int nb, int ni, double *rate, double *rmin, int64_t N,
+                      hipStream_t st);
 template <typename W>
 void launch_nscbc(const NscbcArgs<W> &a, hipStream_t st);
 template <typename W>

@@ -328,12 +328,14 @@ __device__ __forceinline__ int d4_index(int v, int n, int periodic)
 
 // The same sum, 2.5-D: a block owns a 64 x 16 tile of (x, y) columns and
 // marches zl planes in z (at most D4T_Z; fewer on small grids, so that the
-// launch still has a few blocks per CU).  Each plane's tile with its two-wide halo sits in
-// LDS (every slot holds the value at the wrapped or clamped source index, so
-// edges and collapsed dimensions need no special case), the x and y stencils
-// read LDS, and each thread keeps a five-plane z window in registers for its
-// four points.  HBM reads are the tile plus its halo once per plane; the
-// values of the next plane are loaded while the current one is computed.
+// launch still has a few blocks per CU).  Each plane's tile with its two-wide
+// halo sits in LDS (every slot holds the value at the wrapped or clamped
+// source index, so edges and collapsed dimensions need no special case), the
+// x and y stencils read LDS, and each thread keeps a five-plane z window in
+// registers for its four points (with z inactive, the planes are independent
+// problems side by side and the window's centre is the current plane).  HBM
+// reads are the tile plus its halo once per plane; the values of the next
+// plane are loaded while the current one is computed.
 constexpr int D4T_X = 64, D4T_Y = 16, D4T_Z = 16, D4T_LX = D4T_X + 4, D4T_LY = D4T_Y + 4;
 
 constexpr int D4T_HALO = D4T_LX * D4T_LY - D4T_X * D4T_Y;   // 336 halo slots per plane
@@ -391,7 +393,7 @@ __global__ void __launch_bounds__(TPB) k_delta4_tile(D4Args<W> a, D4Multi<W> fm,
             src[q] = (int64_t)d4_src(j, ny, py, ya) * sy + d4_src(i, nx, px, xa);
 #pragma unroll
         for (int m = 0; m < 4; m++) {
-            const int kz = d4_src(k0 - 2 + m, nz, pz, za);
+            const int kz = za ? d4_src(k0 - 2 + m, nz, pz, za) : d4_index(k0 - 2 + m, nz, 0);
             w[q][m] = own[q] ? (double)f[(int64_t)kz * sz + (int64_t)j * sy + i] : 0.0;
         }
     }
@@ -430,7 +432,7 @@ __global__ void __launch_bounds__(TPB) k_delta4_tile(D4Args<W> a, D4Multi<W> fm,
 #pragma unroll
         for (int q = 0; q < 4; q++) {
             const int j = j0 + ty + 4 * q;
-            const int kz = d4_src(k + 2, nz, pz, za);
+            const int kz = za ? d4_src(k + 2, nz, pz, za) : d4_index(k + 2, nz, 0);
             w[q][4] = own[q] ? (double)f[(int64_t)kz * sz + (int64_t)j * sy + i] : 0.0;
         }
         if (k + 1 < kend) {
@@ -616,12 +618,15 @@ __global__ void __launch_bounds__(TPB) k_fluxes(FluxArgs<W> a, int64_t N)
 template <typename W>
 __global__ void __launch_bounds__(TPB) k_lsrk(double *__restrict__ Q, double *__restrict__ du,
                                               const W *__restrict__ dQ, double A, double B, double dt,
-                                              const double *__restrict__ dtp, const int *__restrict__ gate, int64_t NC)
+                                              const double *__restrict__ dtp, const int *__restrict__ gate, int64_t NC,
+                                              int64_t N, unsigned ipts)
 {
     const int64_t I = (int64_t)blockIdx.x * TPB + threadIdx.x;
     if (I >= NC) return;
-    if (gate && !*gate) return;
-    if (dtp) dt = *dtp;
+    // ipts > 0: instances of ipts points side by side, each with its own gate and dt
+    const unsigned ii = ipts ? (unsigned)(I % N) / ipts : 0u;
+    if (gate && !gate[ii]) return;
+    if (dtp) dt = dtp[ii];
     const double v = fma(dt, (double)dQ[I], A * du[I]);
     du[I] = v;
     Q[I] += B * v;
@@ -888,20 +893,23 @@ __device__ __forceinline__ void block_maxmin(double *smax, double *smin, double 
 }
 
 template <typename W>
-__global__ void __launch_bounds__(TPB) k_rate(RateArgs<W> a, double *__restrict__ part, int64_t N)
+__global__ void __launch_bounds__(TPB) k_rate(RateArgs<W> a, double *__restrict__ part, int64_t N, int64_t Ni)
 {
+    // grid y: instances of Ni points side by side (one instance: Ni = N)
     __shared__ double smax[TPB], smin[TPB];
+    const int64_t base = (int64_t)blockIdx.y * Ni;
     double rate = 0.0, rmin = INFINITY;
-    for (int64_t I = (int64_t)blockIdx.x * TPB + threadIdx.x; I < N; I += (int64_t)gridDim.x * TPB) {
+    for (int64_t I = (int64_t)blockIdx.x * TPB + threadIdx.x; I < Ni; I += (int64_t)gridDim.x * TPB) {
         double rQ;
-        const double acc = rate_at(a, I, N, rQ);
+        const double acc = rate_at(a, base + I, N, rQ);
         rmin = rQ < rmin ? rQ : (rQ != rQ ? rQ : rmin);
         rate = acc > rate ? acc : (acc != acc ? acc : rate);
     }
     block_maxmin(smax, smin, rate, rmin);
     if (threadIdx.x == 0) {
-        part[2 * blockIdx.x] = smax[0];
-        part[2 * blockIdx.x + 1] = smin[0];
+        const int64_t b = (int64_t)blockIdx.y * gridDim.x + blockIdx.x;
+        part[2 * b] = smax[0];
+        part[2 * b + 1] = smin[0];
     }
 }
 
@@ -930,9 +938,12 @@ __global__ void __launch_bounds__(TPB) k_rate_batch(RateArgs<W> a, BatchGeom g, 
     }
 }
 
-__global__ void __launch_bounds__(TPB) k_rate_final(const double *__restrict__ part, int nb, double *__restrict__ out)
+// one block per instance: its nb partials -> rate[i], rmin[i]
+__global__ void __launch_bounds__(TPB) k_rate_final(const double *__restrict__ part, int nb, double *__restrict__ rate,
+                                                    double *__restrict__ rmin)
 {
     __shared__ double smax[TPB], smin[TPB];
+    part += (int64_t)2 * nb * blockIdx.x;
     double mx = 0.0, mn = INFINITY;
     for (int b = threadIdx.x; b < nb; b += TPB) {
         double x = part[2 * b], y = part[2 * b + 1];
@@ -952,8 +963,8 @@ __global__ void __launch_bounds__(TPB) k_rate_final(const double *__restrict__ p
         __syncthreads();
     }
     if (threadIdx.x == 0) {
-        out[0] = smax[0];
-        out[1] = smin[0];
+        rate[blockIdx.x] = smax[0];
+        rmin[blockIdx.x] = smin[0];
     }
 }
 
@@ -1481,18 +1492,31 @@ void launch_fluxes(const FluxArgs<W> &a, int64_t N, hipStream_t st)
 
 template <typename W>
 void launch_lsrk(double *Q, double *du, const W *dQ, double A, double B, double dt, const double *dtp,
-                 const int *gate, int64_t NC, hipStream_t st)
+                 const int *gate, int64_t NC, int64_t N, unsigned ipts, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_lsrk<W>, dim3(nblocks(NC)), dim3(TPB), 0, st, Q, du, dQ, A, B, dt, dtp, gate, NC);
+    hipLaunchKernelGGL(k_lsrk<W>, dim3(nblocks(NC)), dim3(TPB), 0, st, Q, du, dQ, A, B, dt, dtp, gate, NC, N,
+                       ipts);
     CHECK_LAUNCH();
 }
 
 template <typename W>
 void launch_rate(const RateArgs<W> &a, double *part, int nb, double *out, int64_t N, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_rate<W>, dim3(nb), dim3(TPB), 0, st, a, part, N);
+    hipLaunchKernelGGL(k_rate<W>, dim3(nb), dim3(TPB), 0, st, a, part, N, N);
     CHECK_LAUNCH();
-    hipLaunchKernelGGL(k_rate_final, dim3(1), dim3(TPB), 0, st, part, nb, out);
+    hipLaunchKernelGGL(k_rate_final, dim3(1), dim3(TPB), 0, st, part, nb, out, out + 1);
+    CHECK_LAUNCH();
+}
+
+// ni instances of N / ni points side by side: rate[i], rmin[i]; part holds
+// 2 nb ni doubles
+template <typename W>
+void launch_rate_inst(const RateArgs<W> &a, double *part, int nb, int ni, double *rate, double *rmin, int64_t N,
+                      hipStream_t st)
+{
+    hipLaunchKernelGGL(k_rate<W>, dim3(nb, ni), dim3(TPB), 0, st, a, part, N, N / ni);
+    CHECK_LAUNCH();
+    hipLaunchKernelGGL(k_rate_final, dim3(ni), dim3(TPB), 0, st, part, nb, rate, rmin);
     CHECK_LAUNCH();
 }
 
@@ -1654,8 +1678,10 @@ void launch_convert(Dst *d, const Src *s, int64_t N, hipStream_t st)
                                             hipStream_t);                                                  \
     template void launch_fluxes<W>(const FluxArgs<W> &, int64_t, hipStream_t);                             \
     template void launch_lsrk<W>(double *, double *, const W *, double, double, double, const double *,   \
-                                 const int *, int64_t, hipStream_t);                                       \
+                                 const int *, int64_t, int64_t, unsigned, hipStream_t);                    \
     template void launch_rate<W>(const RateArgs<W> &, double *, int, double *, int64_t, hipStream_t);      \
+    template void launch_rate_inst<W>(const RateArgs<W> &, double *, int, int, double *, double *, int64_t,  \
+                                      hipStream_t);                                                        \
     template void launch_integral<W>(const DiagArgs<W> &, int, double *, int, double, double *, hipStream_t);
 CLES_INST(double)
 CLES_INST(float)

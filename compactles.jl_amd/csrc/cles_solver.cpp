@@ -97,6 +97,14 @@ struct cles_solver {
     // set while the iteration is captured: LSRK dt and gates in device memory
     const double *dev_dt = nullptr;
     const int *dev_gate = nullptr, *dev_filt = nullptr;
+    // A stack (cles_stack): `stack` two-dimensional instances of the
+    // descriptor's grid side by side along z (n[2] = stack planes of ipts
+    // points, z inactive).  dev_dt / dev_gate / dev_filt then hold one entry
+    // per instance, and the per-step rates go to srate / srmin.  ipts = 0:
+    // not a stack.
+    int stack = 0;
+    unsigned ipts = 0;
+    double *srate = nullptr, *srmin = nullptr, *spart = nullptr;
 
     // the kappa* and D*_k sensors beside `sensor` (mu*, beta*), smoothed in
     // the same launches, and the coefficient epilogue of each (device table)
@@ -702,6 +710,7 @@ struct Impl {
                 e.dt = dt;
                 e.dtp = s.dev_dt;
                 e.gate = s.dev_gate;
+                e.ipts = s.ipts;
                 lines(s.dplan[d], d, s.C, in, nullptr, eoff, e, PH_DIV_LSRK);
             } else {
                 e.kind = k == 0 ? EPI_STORE_NEG : EPI_SUB;
@@ -755,8 +764,8 @@ struct Impl {
     void lsrk_all(double A, double B, double dt) const
     {
         s.timed(PH_LSRK, (32.0 + sizeof(W)) * s.C * (double)s.N,
-                [&] { launch_lsrk<W>(s.Q, s.du, f(s.dQ), A, B, dt, s.dev_dt, s.dev_gate, (int64_t)s.C * s.N,
-                                     s.st()); });
+                [&] { launch_lsrk<W>(s.Q, s.du, f(s.dQ), A, B, dt, s.dev_dt, s.dev_gate, (int64_t)s.C * s.N, s.N,
+                                     s.ipts, s.st()); });
     }
 
     // max_rate's kernels only: the rate and the minimum density stay in
@@ -781,6 +790,11 @@ struct Impl {
         a.beta_art = f(s.beta_art);
         a.kappa_art = f(s.kappa_art);
         for (int k = 0; k < s.Ns; k++) a.D_art[k] = f(s.D_art[k]);
+        if (s.ipts) {   // a stack: each instance's rate and minimum density
+            const int nb = (int)std::min<int64_t>(s.nred, ((int64_t)s.ipts + 255) / 256);
+            launch_rate_inst<W>(a, s.spart, nb, s.stack, s.srate, s.srmin, s.N, s.st());
+            return;
+        }
         int nb = (int)std::min<int64_t>(s.nred, (s.N + 255) / 256);
         launch_rate<W>(a, s.part, nb, s.scal, s.N, s.st());
     }
@@ -939,6 +953,7 @@ struct Impl {
             EpiArgs e{};
             e.kind = s.dev_filt ? EPI_STORE_GATE : EPI_STORE;
             e.gate = s.dev_filt;
+            e.ipts = s.ipts;
             int par[LINE_MF];
             for (int c = 0; c < s.C; c++) par[c] = cons_parity(d, c);
             lines_par(FILT_Q, d, s.C, in, out, par, e, PH_FILTER_STATE);
@@ -1252,10 +1267,16 @@ int cles_solver_create_ex(cles_ctx *ctx, const cles_solver_desc *desc, const dou
     return cles_solver_create_geom(ctx, desc, nullptr, dxdxi, out);
 }
 
-int cles_solver_create_geom(cles_ctx *ctx, const cles_solver_desc *desc, const double *const x[3],
-                            const double *const dxdxi[3], cles_solver **out)
+}  // extern "C"
+
+namespace {
+
+// cles_solver_create_geom (stack 0), or the solver of a cles_stack of
+// `stack` instances
+void create_solver(cles_ctx *ctx, const cles_solver_desc *desc, const double *const x[3],
+                   const double *const dxdxi[3], int stack, cles_solver **out)
 {
-    CLES_TRY({
+    {
         CLES_REQUIRE(ctx && desc && out, "NULL argument");
         validate_desc(*desc);
         for (int k = 0; k < 3; k++) {
@@ -1364,6 +1385,16 @@ int cles_solver_create_geom(cles_ctx *ctx, const cles_solver_desc *desc, const d
                     s->h[k] = 1.0 / (double)(desc->n_global[k] - 1);
                     s->stretched = true;
                 }
+            }
+            if (stack > 0) {   // the instances as planes along the inactive z
+                CLES_REQUIRE(desc->n_global[2] == 1 && s->nact >= 1, "a stack holds two-dimensional problems");
+                CLES_REQUIRE(desc->metric == CLES_METRIC_CARTESIAN && !(x || dxdxi) && desc->n_body_forces == 0,
+                             "a stack holds Cartesian problems on uniform grids without body forces");
+                CLES_REQUIRE(s->N * stack < (int64_t)1 << 31, "a stack holds fewer than 2^31 points");
+                s->stack = stack;
+                s->ipts = (unsigned)s->N;
+                s->n[2] = stack;
+                s->N *= stack;
             }
             // rhs.jl:256-279 foldspec: folded ends, pairing, sigvel, area parity
             auto set_fold = [&](int k, bool hi, int pdim, int revdim, int s0, int s1, int s2, int area) {
@@ -1552,13 +1583,28 @@ int cles_solver_create_geom(cles_ctx *ctx, const cles_solver_desc *desc, const d
                 dispatch(s, [&](auto &im) { im.gcl_cotr(cotr); });
                 s->met.cotr = cotr;
             }
+            if (stack > 0) {   // per-instance rates and rate partials of a stack
+                s->srate = (double *)s->alloc((size_t)stack * sizeof(double));
+                s->srmin = (double *)s->alloc((size_t)stack * sizeof(double));
+                s->spart = (double *)s->alloc((size_t)stack * s->nred * 2 * sizeof(double));
+            }
             CLES_HIP(hipStreamSynchronize(ctx->stream));
         } catch (...) {
             delete s;
             throw;
         }
         *out = s;
-    })
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int cles_solver_create_geom(cles_ctx *ctx, const cles_solver_desc *desc, const double *const x[3],
+                            const double *const dxdxi[3], cles_solver **out)
+{
+    CLES_TRY({ create_solver(ctx, desc, x, dxdxi, 0, out); })
 }
 
 int cles_solver_destroy(cles_solver *s)
@@ -2099,6 +2145,219 @@ int cles_diagnostics(cles_solver *s, double *ke, double *ens)
     CLES_TRY({
         CLES_REQUIRE(s && ke && ens, "NULL argument");
         dispatch(s, [&](auto &im) { im.diagnostics(ke, ens); });
+    })
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Stacks of same-shape two-dimensional problems (cles_stack_*): one solver
+// whose z planes are the instances, so that every launch covers all of them,
+// and run! per instance through the device control of the batch form
+// (launch_batch_dt / launch_batch_post on per-instance arrays).  An instance's
+// arithmetic is that of the same problem alone: pointwise kernels and line
+// solves act on its own plane, its rate and minimum density are reduced over
+// its own points, and its dt, LSRK gate and filter flag are its own.
+// ---------------------------------------------------------------------------
+struct cles_stack {
+    cles_solver *s = nullptr;
+    int count = 0;
+    double *t = nullptr, *dt = nullptr, *dt_prev = nullptr, *rate_prev = nullptr, *dt_seen = nullptr,
+           *tfinal = nullptr;
+    int64_t *step = nullptr, *nmax = nullptr, *step0 = nullptr;
+    int *status = nullptr, *active = nullptr, *stepped = nullptr, *filt = nullptr, *cnt = nullptr;
+    int *host_cnt = nullptr;
+    ~cles_stack()
+    {
+        if (host_cnt) (void)hipHostFree(host_cnt);
+        if (s) delete s;
+    }
+};
+
+namespace {
+
+void check_plane(const cles_stack *st, int i, const int64_t padded[3], const int halo[3])
+{
+    CLES_REQUIRE(i >= 0 && i < st->count, "instance index out of range");
+    for (int k = 0; k < 2; k++)
+        CLES_REQUIRE(halo[k] >= 0 && padded[k] >= st->s->n[k] + 2 * (int64_t)halo[k],
+                     "padded extents smaller than n + 2*halo");
+    CLES_REQUIRE(halo[2] >= 0 && padded[2] >= 1 + 2 * (int64_t)halo[2], "padded extents smaller than n + 2*halo");
+}
+
+// instance i's plane <-> the padded host layout of one problem (cles_solver_upload)
+void copy_plane(cles_stack *st, int i, double *host, const int64_t padded[3], const int halo[3], bool up)
+{
+    cles_solver *s = st->s;
+    const int64_t nx = s->n[0], ny = s->n[1], Ni = (int64_t)s->ipts;
+    const int64_t P0 = padded[0], P1 = padded[1], NP = padded[0] * padded[1] * padded[2];
+    std::vector<double> buf((size_t)Ni * s->C);
+    if (!up) {
+        for (int c = 0; c < s->C; c++)
+            CLES_HIP(hipMemcpyAsync(&buf[(size_t)c * Ni], s->Q + (size_t)c * s->N + (size_t)i * Ni,
+                                    (size_t)Ni * sizeof(double), hipMemcpyDeviceToHost, s->st()));
+        CLES_HIP(hipStreamSynchronize(s->st()));
+    }
+    for (int c = 0; c < s->C; c++)
+        for (int64_t j = 0; j < ny; j++) {
+            double *h = host + c * NP + halo[0] + P0 * ((j + halo[1]) + P1 * (int64_t)halo[2]);
+            double *b = &buf[(size_t)c * Ni + (size_t)nx * j];
+            if (up) std::memcpy(b, h, sizeof(double) * nx);
+            else std::memcpy(h, b, sizeof(double) * nx);
+        }
+    if (up) {
+        for (int c = 0; c < s->C; c++)
+            CLES_HIP(hipMemcpyAsync(s->Q + (size_t)c * s->N + (size_t)i * Ni, &buf[(size_t)c * Ni],
+                                    (size_t)Ni * sizeof(double), hipMemcpyHostToDevice, s->st()));
+        CLES_HIP(hipStreamSynchronize(s->st()));
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int cles_stack_create(cles_ctx *ctx, const cles_solver_desc *desc, int count, cles_stack **out)
+{
+    CLES_TRY({
+        CLES_REQUIRE(ctx && desc && out, "NULL argument");
+        CLES_REQUIRE(count >= 1 && count <= 4096, "a stack holds 1..4096 instances");
+        auto *st = new cles_stack;
+        try {
+            create_solver(ctx, desc, nullptr, nullptr, count, &st->s);
+            st->count = count;
+            cles_solver *s = st->s;
+            const size_t cnt = (size_t)count;
+            for (double **fp : {&st->t, &st->dt, &st->dt_prev, &st->rate_prev, &st->dt_seen, &st->tfinal})
+                *fp = (double *)s->alloc(cnt * sizeof(double));
+            for (int64_t **fp : {&st->step, &st->nmax, &st->step0}) *fp = (int64_t *)s->alloc(cnt * sizeof(int64_t));
+            for (int **fp : {&st->status, &st->active, &st->stepped, &st->filt})
+                *fp = (int *)s->alloc(cnt * sizeof(int));
+            st->cnt = (int *)s->alloc(sizeof(int));
+            CLES_HIP(hipHostMalloc((void **)&st->host_cnt, sizeof(int), hipHostMallocDefault));
+            CLES_HIP(hipStreamSynchronize(s->st()));
+        } catch (...) {
+            delete st;
+            throw;
+        }
+        *out = st;
+    })
+}
+
+int cles_stack_destroy(cles_stack *st)
+{
+    CLES_TRY({ delete st; })
+}
+
+int cles_stack_upload(cles_stack *st, int i, const double *Q_host, const int64_t padded[3], const int halo[3])
+{
+    CLES_TRY({
+        CLES_REQUIRE(st && Q_host && padded && halo, "NULL argument");
+        check_plane(st, i, padded, halo);
+        copy_plane(st, i, const_cast<double *>(Q_host), padded, halo, true);
+    })
+}
+
+int cles_stack_download(cles_stack *st, int i, double *Q_host, const int64_t padded[3], const int halo[3])
+{
+    CLES_TRY({
+        CLES_REQUIRE(st && Q_host && padded && halo, "NULL argument");
+        check_plane(st, i, padded, halo);
+        copy_plane(st, i, Q_host, padded, halo, false);
+    })
+}
+
+int cles_stack_run(cles_stack *st, const double *tfinal, const int64_t *nmax, int64_t max_steps, int64_t *taken)
+{
+    CLES_TRY({
+        CLES_REQUIRE(st && tfinal && nmax, "NULL argument");
+        cles_solver *s = st->s;
+        const int planes = st->count;
+        const size_t cnt = (size_t)planes;
+        // the run! entry of every instance (timestep.jl:281-344)
+        std::vector<double> t(cnt), tf(tfinal, tfinal + cnt);
+        std::vector<int64_t> stp(cnt), nmx(nmax, nmax + cnt);
+        std::vector<int> status(cnt), active(cnt, 0);
+        CLES_HIP(hipMemcpyAsync(t.data(), st->t, cnt * sizeof(double), hipMemcpyDeviceToHost, s->st()));
+        CLES_HIP(hipMemcpyAsync(stp.data(), st->step, cnt * sizeof(int64_t), hipMemcpyDeviceToHost, s->st()));
+        CLES_HIP(hipMemcpyAsync(status.data(), st->status, cnt * sizeof(int), hipMemcpyDeviceToHost, s->st()));
+        CLES_HIP(hipStreamSynchronize(s->st()));
+        int nact = 0;
+        for (int i = 0; i < st->count; i++) {
+            active[i] = status[i] == CLES_OK && t[i] < tf[i] && stp[i] < nmx[i];
+            nact += active[i];
+        }
+        CLES_HIP(hipMemcpyAsync(st->tfinal, tf.data(), cnt * sizeof(double), hipMemcpyHostToDevice, s->st()));
+        CLES_HIP(hipMemcpyAsync(st->nmax, nmx.data(), cnt * sizeof(int64_t), hipMemcpyHostToDevice, s->st()));
+        CLES_HIP(hipMemcpyAsync(st->active, active.data(), cnt * sizeof(int), hipMemcpyHostToDevice, s->st()));
+        CLES_HIP(hipMemcpyAsync(st->step0, stp.data(), cnt * sizeof(int64_t), hipMemcpyHostToDevice, s->st()));
+        CLES_HIP(hipMemsetAsync(st->dt_seen, 0, cnt * sizeof(double), s->st()));
+        // fresh workspace (timestep.jl:402-405)
+        CLES_HIP(hipMemsetAsync(s->du, 0, (size_t)s->N * s->C * sizeof(double), s->st()));
+        CLES_HIP(hipMemsetAsync(s->dQ, 0, (size_t)s->N * s->C * s->wsize(), s->st()));
+        cles::BatchCtl c{};
+        c.count = planes;
+        c.rate = s->srate;
+        c.rhomin = s->srmin;
+        c.tfinal = st->tfinal;
+        c.t = st->t;
+        c.dt = st->dt;
+        c.dt_prev = st->dt_prev;
+        c.rate_prev = st->rate_prev;
+        c.dt_seen = st->dt_seen;
+        c.step = st->step;
+        c.nmax = st->nmax;
+        c.status = st->status;
+        c.active = st->active;
+        c.stepped = st->stepped;
+        c.filt = st->filt;
+        c.cfl = s->d.cfl;
+        c.predict = s->d.predict;
+        c.max_growth = s->d.max_growth;
+        c.dt_min = s->d.dt_min;
+        c.dt_min_ratio = s->d.dt_min_ratio;
+        c.filter_interval = s->d.filter_interval;
+        c.dts = nullptr;
+        c.step0 = st->step0;
+        c.nrec = 0;
+        int64_t steps = 0;
+        s->dev_dt = st->dt;
+        s->dev_gate = st->stepped;
+        s->dev_filt = st->filt;
+        try {
+            const int64_t poll = 8;
+            while (nact > 0 && steps < max_steps) {
+                dispatch(s, [&](auto &im) { im.iteration_device(c); });
+                steps++;
+                if (steps % poll == 0 || steps == max_steps) {
+                    cles::launch_batch_count(st->active, planes, st->cnt, s->st());
+                    CLES_HIP(hipMemcpyAsync(st->host_cnt, st->cnt, sizeof(int), hipMemcpyDeviceToHost, s->st()));
+                    CLES_HIP(hipStreamSynchronize(s->st()));
+                    nact = *st->host_cnt;
+                }
+            }
+            CLES_HIP(hipStreamSynchronize(s->st()));
+        } catch (...) {
+            s->dev_dt = nullptr;
+            s->dev_gate = s->dev_filt = nullptr;
+            throw;
+        }
+        s->dev_dt = nullptr;
+        s->dev_gate = s->dev_filt = nullptr;
+        if (taken) *taken = steps;
+    })
+}
+
+int cles_stack_state(cles_stack *st, double *t, int64_t *steps, int *status)
+{
+    CLES_TRY({
+        CLES_REQUIRE(st, "NULL argument");
+        cles_solver *s = st->s;
+        const size_t cnt = (size_t)st->count;
+        if (t) CLES_HIP(hipMemcpyAsync(t, st->t, cnt * sizeof(double), hipMemcpyDeviceToHost, s->st()));
+        if (steps) CLES_HIP(hipMemcpyAsync(steps, st->step, cnt * sizeof(int64_t), hipMemcpyDeviceToHost, s->st()));
+        if (status) CLES_HIP(hipMemcpyAsync(status, st->status, cnt * sizeof(int), hipMemcpyDeviceToHost, s->st()));
+        CLES_HIP(hipStreamSynchronize(s->st()));
     })
 }
 

@@ -478,6 +478,32 @@ int cles_batch_get_field(cles_batch *batch, int64_t i, const char *name, double 
 int cles_batch_kinetic_energy(cles_batch *batch, double *ke);
 
 /* ----------------------------------------------------------------------- */
+/* Stacks of same-shape two-dimensional problems                            */
+/* ----------------------------------------------------------------------- */
+/* `count` problems of one cles_solver_desc (n_global = (nx, ny, 1),
+ * Cartesian, uniform grid, no body forces) that differ in their initial
+ * state, end time and step cap, held side by side in one solver so that every
+ * kernel launch covers all of them.  Each follows run! on its own (its own
+ * max_rate, dt, check_step floors, tfinal clip, filter cadence; a failing
+ * instance is frozen with its status); an instance's result does not depend
+ * on the other instances of the stack.  The reference runs one problem per
+ * setup/run! (src/problem.jl:321, src/timestep.jl:277). */
+typedef struct cles_stack cles_stack;
+int cles_stack_create(cles_ctx *ctx, const cles_solver_desc *desc, int count, cles_stack **stack);
+int cles_stack_destroy(cles_stack *stack);
+/* Instance i's state in the padded layout of one problem (as cles_solver_upload). */
+int cles_stack_upload(cles_stack *stack, int i, const double *Q_host, const int64_t padded[3],
+                      const int halo[3]);
+int cles_stack_download(cles_stack *stack, int i, double *Q_host, const int64_t padded[3],
+                        const int halo[3]);
+/* run!(...; tfinal[i], nmax[i]) on every instance, at most max_steps steps of
+ * each; *steps_taken: stack iterations issued. */
+int cles_stack_run(cles_stack *stack, const double *tfinal, const int64_t *nmax, int64_t max_steps,
+                   int64_t *steps_taken);
+/* Per-instance time, completed steps and status; NULL arrays are skipped. */
+int cles_stack_state(cles_stack *stack, double *t, int64_t *steps, int *status);
+
+/* ----------------------------------------------------------------------- */
 /* Serving loop                                                             */
 /* ----------------------------------------------------------------------- */
 /* Independent solve requests (setup + run! of the reference, one problem
