@@ -367,11 +367,30 @@ def measure_serving(ch, ctx=None, count=32, rate_per_s=1.5, seed=20261202, budge
             ctx.synchronize()
             s1.close()
             seq += time.perf_counter() - t1
+    # launch sharing: same-shape 2-D requests as one Stack against one at a time
+    stack = {}
+    if ctx is not None:
+        for n in ((1024, 512, 1), (256, 256, 1)):
+            prob, num, Q = W.shock_problem(ch, n)
+            row = {}
+            for K in (1, 5):
+                st = ch.Stack(ctx, prob, num, K)
+                for i in range(K):
+                    st.upload(i, Q)
+                st.run(1e9, 5)
+                ctx.synchronize()
+                t1 = time.perf_counter()
+                st.run(1e9, 25)
+                ctx.synchronize()
+                row[f"K{K}_Mpt_steps_s"] = round(n[0] * n[1] * K * 20 / (time.perf_counter() - t1) / 1e6, 1)
+                st.close()
+            stack[f"{n[0]}x{n[1]}"] = row
     return {"solves_per_s": round(count / span, 2), "p50_s": round(float(np.percentile(lat, 50)), 4),
             "p99_s": round(float(np.percentile(lat, 99)), 4), "requests": count, "arrival_rate_per_s": rate_per_s,
             "all_done": ok, "hbm_budget_GB": budget / 2 ** 30, "runners": runners, "tuned_shapes": tuned,
             "saturated_solves_per_s": round(count / sat, 2),
             "plain_path_sequential_solves_per_s": round(count / seq, 2) if seq > 0 else None,
+            "stack_limit": 8, "stack_throughput": stack,
             "workload": "Poisson arrivals (seed 20261202): Taylor-Green 32^3..128^3 to t = 0.05 and He|CO2 "
                         "shock / interface 256^2..1024x512 (NSCBC in / out) to t = 0.03, FP64"}
 

@@ -132,6 +132,8 @@ SIGNATURES = {
     "cles_stack_state": (C.c_int, [P, DP, C.POINTER(C.c_int64), C.POINTER(C.c_int)]),
     "cles_server_create": (C.c_int, [C.c_int, C.c_size_t, C.c_int, PP]),
     "cles_server_set_run_mode": (C.c_int, [P, C.c_int]),
+    "cles_server_set_stack_limit": (C.c_int, [P, C.c_int, C.c_double]),
+    "cles_server_group": (C.c_int, [P, C.c_int64, C.POINTER(C.c_int)]),
     "cles_server_destroy": (C.c_int, [P]),
     "cles_server_submit": (C.c_int, [P, C.POINTER(SolverDesc), DP, C.POINTER(C.c_int64), C.POINTER(C.c_int),
                                      C.c_double, C.c_int64, C.POINTER(C.c_int64)]),

@@ -15,14 +15,19 @@ QUEUED, RUNNING, DONE, FAILED = range(4)
 
 
 class Server:
-    def __init__(self, device: int = 0, hbm_budget: int = 64 << 30, runners: int = 4, run_mode: str = "host_dt"):
-        """run_mode: how each request's solver issues run! (Solver.set_run_mode)."""
+    def __init__(self, device: int = 0, hbm_budget: int = 64 << 30, runners: int = 4, run_mode: str = "host_dt",
+                 stack_limit: int = 8, stack_window: float = 0.005):
+        """run_mode: how each request's solver issues run! (Solver.set_run_mode);
+        stack_limit: same-shape two-dimensional requests queued together run
+        as one Stack of up to this many (1: each alone), a request waiting up
+        to stack_window seconds after its submission for the rest."""
         self.lib = L.load()
         h = C.c_void_p()
         L.check(self.lib.cles_server_create(int(device), C.c_size_t(int(hbm_budget)), int(runners), C.byref(h)))
         self.handle = h
         self._specs = {}
         L.check(self.lib.cles_server_set_run_mode(self.handle, Solver.RUN_MODES[run_mode]))
+        L.check(self.lib.cles_server_set_stack_limit(self.handle, int(stack_limit), float(stack_window)))
 
     def submit(self, prob: Problem, num: Numerics, tfinal: float, nmax: int = 2 ** 62, Q: np.ndarray | None = None):
         """Queue setup(prob, num) + run!(...; tfinal, nmax); Q (padded host
@@ -54,6 +59,12 @@ class Server:
         L.check(self.lib.cles_server_result(self.handle, int(rid), Q.ctypes.data_as(L.DP), L.I64_3(*spec.padded),
                                             L.INT_3(*spec.halo), C.byref(status), C.byref(t), C.byref(steps)))
         return Q, status.value, t.value, steps.value
+
+    def group(self, rid: int) -> int:
+        """Requests of the stack a finished request ran in (1: alone)."""
+        g = C.c_int()
+        L.check(self.lib.cles_server_group(self.handle, int(rid), C.byref(g)))
+        return g.value
 
     def timing(self, rid: int):
         """(t_submit, t_admit, t_done, footprint) in seconds since the server started."""

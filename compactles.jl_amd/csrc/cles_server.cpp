@@ -10,8 +10,14 @@
 // passed over kAging times, after which nothing behind it is admitted until
 // it runs (no starvation).
 //
-// Each admitted request owns its device buffers and stream, and the kernels
-// are deterministic, so its result is bitwise the same alone and in any mix.
+// Same-shape two-dimensional requests (one descriptor, byte for byte) that
+// are queued together are admitted together, up to the stack limit and within
+// the budget, and run as one stack (cles_stack_*): every launch covers all of
+// them, each with its own dt, end time and step cap.
+//
+// Each admitted request or stack owns its device buffers and stream, and the
+// kernels are deterministic and act on each instance's own data, so a
+// request's result is bitwise the same alone and in any mix.
 #include <atomic>
 #include <chrono>
 #include <condition_variable>
@@ -48,6 +54,7 @@ struct Request {
     double t_submit = 0, t_admit = 0, t_done = 0;
     double t_sim = 0;
     int64_t steps = 0;
+    int group = 1;   // requests of the stack it ran in
 };
 
 double now_s(std::chrono::steady_clock::time_point t0)
@@ -71,22 +78,96 @@ struct cles_server {
     int64_t next_id = 1;
     int running = 0;
     int run_mode = CLES_RUN_HOST_DT;   // of every solver it creates
+    int stack_limit = 8;               // requests per stack (1: no stacks)
+    double window = 0.005;             // seconds a stackable request waits for company
 
-    // under mu: the next admissible request (taken off the queue), or nullptr
-    Request *admit_locked()
+    static bool stackable(const cles_solver_desc &d)
     {
-        if (running >= runners) return nullptr;
+        return d.n_global[2] == 1 && (d.n_global[0] > 1 || d.n_global[1] > 1) &&
+               d.metric == CLES_METRIC_CARTESIAN && d.n_body_forces == 0;
+    }
+
+    // under mu: the next admissible request with the queued requests of the
+    // same descriptor that fit beside it (taken off the queue); empty if none
+    std::vector<Request *> admit_locked()
+    {
+        std::vector<Request *> g;
+        if (running >= runners) return g;
         auto it = queue.begin();
         for (; it != queue.end(); ++it) {
             const Request *r = reqs[*it].get();
             if (r->footprint <= budget - in_use) break;
-            if (r->passed_over >= kAging) return nullptr;   // capacity is held for it
+            if (r->passed_over >= kAging) return g;   // capacity is held for it
         }
-        if (it == queue.end()) return nullptr;
-        for (auto jt = queue.begin(); jt != it; ++jt) reqs[*jt]->passed_over++;
+        if (it == queue.end()) return g;
         Request *r = reqs[*it].get();
+        if (stack_limit > 1 && stackable(r->desc) && now_s(t0) - r->t_submit < window) {
+            // requests submitted together arrive one call apart: a stackable
+            // request waits up to `window` for the rest of its stack
+            int same = 0;
+            for (int64_t id : queue) same += std::memcmp(&reqs[id]->desc, &r->desc, sizeof(cles_solver_desc)) == 0;
+            if (same < stack_limit) return g;
+        }
+        for (auto jt = queue.begin(); jt != it; ++jt) reqs[*jt]->passed_over++;
         queue.erase(it);
-        return r;
+        g.push_back(r);
+        if (stack_limit > 1 && stackable(r->desc)) {
+            size_t fp = r->footprint;
+            for (auto jt = queue.begin(); jt != queue.end() && (int)g.size() < stack_limit;) {
+                Request *q = reqs[*jt].get();
+                if (std::memcmp(&q->desc, &r->desc, sizeof(cles_solver_desc)) == 0 &&
+                    fp + q->footprint <= budget - in_use) {
+                    fp += q->footprint;
+                    g.push_back(q);
+                    jt = queue.erase(jt);
+                } else {
+                    ++jt;
+                }
+            }
+        }
+        return g;
+    }
+
+    // a stack of same-descriptor requests: cles_stack_* with each request's
+    // own state, end time and step cap
+    void run_stack(const std::vector<Request *> &g)
+    {
+        const int K = (int)g.size();
+        cles_ctx *ctx = nullptr;
+        cles_stack *st = nullptr;
+        std::vector<double> tf(K), t(K, 0.0);
+        std::vector<int64_t> nm(K), steps(K, 0);
+        std::vector<int> status(K, CLES_OK);
+        for (int i = 0; i < K; i++) {
+            tf[i] = g[i]->tfinal;
+            nm[i] = g[i]->nmax;
+        }
+        int rc = cles_ctx_create(device, &ctx);
+        if (rc == CLES_OK) rc = cles_stack_create(ctx, &g[0]->desc, K, &st);
+        for (int i = 0; i < K && rc == CLES_OK; i++) rc = cles_stack_upload(st, i, g[i]->Q.data(), g[i]->padded, g[i]->halo);
+        if (rc == CLES_OK) rc = cles_stack_run(st, tf.data(), nm.data(), INT64_MAX, nullptr);
+        if (rc == CLES_OK) rc = cles_stack_state(st, t.data(), steps.data(), status.data());
+        // the states reached (at a failure: the last completed step)
+        for (int i = 0; i < K && rc == CLES_OK; i++)
+            rc = cles_stack_download(st, i, g[i]->Q.data(), g[i]->padded, g[i]->halo);
+        const std::string msg = rc != CLES_OK ? std::string(cles_last_error()) : std::string();
+        if (st) cles_stack_destroy(st);
+        if (ctx) cles_ctx_destroy(ctx);
+        std::lock_guard<std::mutex> lk(mu);
+        const double done = now_s(t0);
+        for (int i = 0; i < K; i++) {
+            Request *r = g[i];
+            r->group = K;
+            r->status = rc != CLES_OK ? rc : status[i];
+            r->msg = rc != CLES_OK ? msg
+                                   : (status[i] != CLES_OK ? "run! stopped by check_step (status " +
+                                                                 std::to_string(status[i]) + ")"
+                                                           : std::string());
+            r->t_sim = t[i];
+            r->steps = steps[i];
+            r->state = r->status == CLES_OK ? RQ_DONE : RQ_FAILED;
+            r->t_done = done;
+        }
     }
 
     void run_one(Request *r)
@@ -120,17 +201,26 @@ struct cles_server {
     {
         std::unique_lock<std::mutex> lk(mu);
         for (;;) {
-            Request *r = nullptr;
-            cv.wait(lk, [&] { return stop || (r = admit_locked()) != nullptr; });
-            if (!r) return;   // stopping
-            r->state = RQ_RUNNING;
-            r->t_admit = now_s(t0);
-            in_use += r->footprint;
+            std::vector<Request *> g;
+            // a request held for company is admitted when its window ends
+            while (!stop && (g = admit_locked()).empty())
+                cv.wait_for(lk, std::chrono::milliseconds(1));
+            if (g.empty()) return;   // stopping
+            size_t fp = 0;
+            for (Request *r : g) {
+                r->state = RQ_RUNNING;
+                r->t_admit = now_s(t0);
+                fp += r->footprint;
+            }
+            in_use += fp;
             running++;
             lk.unlock();
-            run_one(r);
+            if (g.size() == 1)
+                run_one(g[0]);
+            else
+                run_stack(g);
             lk.lock();
-            in_use -= r->footprint;
+            in_use -= fp;
             running--;
             cv.notify_all();
         }
@@ -184,6 +274,30 @@ int cles_server_destroy(cles_server *sv)
         sv->cv.notify_all();
         for (auto &t : sv->workers) t.join();
         delete sv;
+    })
+}
+
+int cles_server_set_stack_limit(cles_server *sv, int limit, double window_s)
+{
+    CLES_TRY({
+        CLES_REQUIRE(sv, "NULL argument");
+        CLES_REQUIRE(limit >= 1 && limit <= 4096, "the stack limit must be 1..4096");
+        CLES_REQUIRE(window_s >= 0 && window_s <= 10, "the stacking window must be 0..10 s");
+        std::lock_guard<std::mutex> lk(sv->mu);
+        sv->stack_limit = limit;
+        sv->window = window_s;
+    })
+}
+
+int cles_server_group(cles_server *sv, int64_t id, int *size)
+{
+    CLES_TRY({
+        CLES_REQUIRE(sv && size, "NULL argument");
+        std::lock_guard<std::mutex> lk(sv->mu);
+        auto it = sv->reqs.find(id);
+        CLES_REQUIRE(it != sv->reqs.end(), "unknown request id");
+        CLES_REQUIRE(it->second->state == RQ_DONE || it->second->state == RQ_FAILED, "request has not finished");
+        *size = it->second->group;
     })
 }
 

@@ -523,6 +523,14 @@ int cles_server_create(int device, size_t hbm_budget, int runners, cles_server *
 /* Run mode (cles_solver_set_run_mode) of the solvers of requests admitted
  * from now on; default CLES_RUN_HOST_DT. */
 int cles_server_set_run_mode(cles_server *server, int run_mode);
+/* Same-shape two-dimensional requests (identical descriptors) queued
+ * together are admitted together and run as one stack (cles_stack_*) of at
+ * most `limit` requests (default 8; 1: every request alone); a stackable
+ * request waits up to window_s (default 0.005) after its submission for the
+ * rest of its stack.  Results are bitwise those of each request alone. */
+int cles_server_set_stack_limit(cles_server *server, int limit, double window_s);
+/* *size: requests of the stack a finished request ran in (1: alone). */
+int cles_server_group(cles_server *server, int64_t id, int *size);
 /* Marks queued requests CLES_REQ_FAILED and drops them, waits for running
  * ones, frees everything.  No other call on the server (cles_server_wait
  * included) may overlap it. */

@@ -100,3 +100,32 @@ def test_footprint_bounds_allocation(ctx):
         used, fp = s.memory(), s.footprint()
         s.close()
         assert used <= fp <= 1.25 * used + (4 << 20), (used, fp)
+
+
+def test_same_shape_requests_run_as_a_stack(ctx):
+    """Same-descriptor 2-D requests queued together run as one stack (a single
+    runner, so they queue); each result bitwise equal to the request alone,
+    with its own end time and step cap."""
+    import compactles_hip as ch
+    prob, num, Q0 = _shock(ch, (64, 24, 1))
+    reqs = []
+    for k, (scale, tf, nm) in enumerate(((1.0, 0.02, 2 ** 62), (0.9, 0.01, 2 ** 62), (1.1, 1e9, 9), (1.0, 1e9, 4))):
+        Q = Q0.copy()
+        Q[..., 2:5] *= scale   # same internal energy, so every request has its own dt sequence
+        Q[..., 5] += 0.5 * (scale * scale - 1) * (Q[..., 2] ** 2 + Q[..., 3] ** 2) / (scale * scale) / \
+            np.maximum(Q[..., 0] + Q[..., 1], 1e-300)
+        reqs.append((Q, tf, nm))
+    sv = ch.Server(device=0, hbm_budget=8 << 30, runners=1)
+    blocker = sv.submit(*_tgv(ch, 24), tfinal=1e9, nmax=3)   # holds the runner while the others queue
+    ids = [sv.submit(prob, num, tfinal=tf, nmax=nm, Q=Q) for (Q, tf, nm) in reqs]
+    for rid in [blocker] + ids:
+        assert sv.wait(rid, timeout=120.0) == ch.server.DONE
+    sizes = [sv.group(rid) for rid in ids]
+    print(f"\nstack sizes: {sizes}")
+    assert sizes == [4, 4, 4, 4]
+    for rid, (Q, tf, nm) in zip(ids, reqs):
+        got, status, t, steps = sv.result(rid)
+        ref, t_ref, st_ref = _alone(ctx, ch, prob, num, Q, tf, nm)
+        assert status == 0 and steps == st_ref and t == t_ref
+        assert np.array_equal(got, ref), rid
+    sv.close()
