@@ -74,6 +74,10 @@ struct cles_solver {
     void *mu_art = nullptr, *beta_art = nullptr, *kappa_art = nullptr, *D_art[CLES_MAXSP] = {};
     void *sensor = nullptr, *strain = nullptr, *tmp1 = nullptr, *tmp2 = nullptr, *esens = nullptr;
     void *nsd[3] = {};                         // NSCBC pressure / density derivatives
+    // ... or, when they go with the gradient launches (nsd_merged), D_k p and
+    // D_k rho for each active dimension k
+    void *nsdp[3] = {}, *nsdr[3] = {};
+    bool nsd_merged = false;
     void *flux[3][CLES_MAXSP + 4] = {};
     double *part = nullptr, *scal = nullptr;   // reduction partials, device scalars
     int nred = 1024;
@@ -447,6 +451,20 @@ struct Impl {
                         in[nf] = s.Y[sp];
                         out[nf++] = s.gY[d][sp];
                     }
+                if (s.nscbc && s.nsd_merged) {   // the NSCBC faces' full-field derivatives
+                    bool np[3], nr[3];
+                    nscbc_needs(np, nr);
+                    if (np[d]) {
+                        par[nf] = 1;
+                        in[nf] = s.p;
+                        out[nf++] = s.nsdp[d];
+                    }
+                    if (nr[d]) {
+                        par[nf] = 1;
+                        in[nf] = s.rho;
+                        out[nf++] = s.nsdr[d];
+                    }
+                }
             }
             lines_par(DERIV, d, nf, in, out, par, store(out[0]), PH_DERIV);
             if (s.met.m[d] || s.curv) metric(out, nf, 0, d);   // rhs.jl:599,631,634 _scale_grad!
@@ -841,6 +859,28 @@ struct Impl {
 
     // rhs.jl:688-690: boundary corrections after the flux divergence, faces in
     // (dim, low, high) order; nscbc.jl:59-154 outflow, :220-306 inflow.
+    // the full-field derivatives the NSCBC faces read: D_d p on every NSCBC
+    // face's dimension d, D_d rho on inflow faces, the transverse D_t p on
+    // outflow faces (nscbc.jl:69-87, :223-224)
+    void nscbc_needs(bool (&np)[3], bool (&nr)[3]) const
+    {
+        for (int k = 0; k < 3; k++) np[k] = nr[k] = false;
+        for (int d = 0; d < 3; d++)
+            for (int side = 0; side < 2; side++) {
+                if (!s.active[d]) continue;
+                const int bc = s.d.bc[d][side];
+                if (bc != CLES_BC_NSCBC_OUT && bc != CLES_BC_NSCBC_IN) continue;
+                np[d] = true;
+                if (bc == CLES_BC_NSCBC_IN) {
+                    nr[d] = true;
+                } else {
+                    const int t1 = d == 0 ? 1 : 0, t2 = d == 2 ? 1 : 2;
+                    np[t1] = np[t1] || s.active[t1];
+                    np[t2] = np[t2] || s.active[t2];
+                }
+            }
+    }
+
     void nscbc_corrections() const
     {
         if (!s.nscbc) return;
@@ -860,19 +900,28 @@ struct Impl {
                 for (int k = 0; k < 3; k++) a.n[k] = s.n[k];
                 a.N = s.N;
                 a.count = s.N / s.n[d];
-                // full-field derivatives first (nscbc.jl:69-87, :223-224)
-                line_par(DERIV, d, s.p, 1, store(s.nsd[0]), PH_NSCBC);
-                a.dpn = f(s.nsd[0]);
-                if (a.inflow) {
-                    line_par(DERIV, d, s.rho, 1, store(s.nsd[1]), PH_NSCBC);
-                    a.dt1 = f(s.nsd[1]);
-                } else {
+                // full-field derivatives (nscbc.jl:69-87, :223-224): from the
+                // gradient launches of this RHS, or computed here
+                if (!a.inflow) {
                     a.act1 = s.active[t1];
                     a.act2 = s.active[t2];
-                    if (a.act1) line_par(DERIV, t1, s.p, 1, store(s.nsd[1]), PH_NSCBC);
-                    if (a.act2) line_par(DERIV, t2, s.p, 1, store(s.nsd[2]), PH_NSCBC);
-                    a.dt1 = f(s.nsd[1]);
-                    a.dt2 = f(s.nsd[2]);
+                }
+                if (s.nsd_merged) {
+                    a.dpn = f(s.nsdp[d]);
+                    a.dt1 = f(a.inflow ? s.nsdr[d] : s.nsdp[t1]);
+                    a.dt2 = f(a.inflow ? nullptr : s.nsdp[t2]);
+                } else {
+                    line_par(DERIV, d, s.p, 1, store(s.nsd[0]), PH_NSCBC);
+                    a.dpn = f(s.nsd[0]);
+                    if (a.inflow) {
+                        line_par(DERIV, d, s.rho, 1, store(s.nsd[1]), PH_NSCBC);
+                        a.dt1 = f(s.nsd[1]);
+                    } else {
+                        if (a.act1) line_par(DERIV, t1, s.p, 1, store(s.nsd[1]), PH_NSCBC);
+                        if (a.act2) line_par(DERIV, t2, s.p, 1, store(s.nsd[2]), PH_NSCBC);
+                        a.dt1 = f(s.nsd[1]);
+                        a.dt2 = f(s.nsd[2]);
+                    }
                 }
                 a.rho = f(s.rho); a.c = f(s.c); a.p = f(s.p); a.T = f(s.T); a.cp = f(s.cp);
                 for (int k = 0; k < 3; k++) a.vel[k] = f(vel[k]);
@@ -1222,7 +1271,7 @@ int cles_solver_footprint(const cles_solver_desc *desc, size_t *bytes)
             for (int sd = 0; sd < 2; sd++)
                 nscbc = nscbc || (d.switchable[k][sd] && d.n_global[k] > 1 &&
                                   (d.bc_after[k][sd] == CLES_BC_NSCBC_IN || d.bc_after[k][sd] == CLES_BC_NSCBC_OUT));
-        if (nscbc) fields += 3;
+        if (nscbc) fields += std::max(3, 2 * nact);   // NSCBC derivatives
         {   // pairing scratch of a paired fold (folds.jl:276-330): three 8-byte fields
             const bool paired = (d.bc[0][0] == CLES_BC_AXIS && d.n_global[1] > 1) ||
                                 (d.bc[0][0] == CLES_BC_ORIGIN && (d.n_global[1] > 1 || d.n_global[2] > 1)) ||
@@ -1502,8 +1551,20 @@ void create_solver(cles_ctx *ctx, const cles_solver_desc *desc, const double *co
                 }
             }
             s->nscbc_any = s->nscbc_any || s->nscbc;
-            if (s->nscbc_any)
-                for (void *&fp : s->nsd) fp = s->wfield();
+            if (s->nscbc_any) {
+                // Cartesian uniform grids (no metric scaling of the gradients)
+                // compute the NSCBC derivatives in the gradient launches
+                s->nsd_merged = !s->geom() && 4 + (s->Ns > 1 ? s->Ns : 0) + 2 <= cles::LINE_MF;
+                if (s->nsd_merged) {
+                    for (int k = 0; k < 3; k++)
+                        if (s->active[k]) {
+                            s->nsdp[k] = s->wfield();
+                            s->nsdr[k] = s->wfield();
+                        }
+                } else {
+                    for (void *&fp : s->nsd) fp = s->wfield();
+                }
+            }
             if (desc->eos_kind == CLES_EOS_NASA9) {
                 const cles::N9Table t = cles::nasa9_table(*desc);
                 auto *dt = (cles::N9Table *)s->alloc(sizeof(cles::N9Table));
