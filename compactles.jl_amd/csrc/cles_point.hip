@@ -336,7 +336,13 @@ __device__ __forceinline__ int d4_index(int v, int n, int periodic)
 // problems side by side and the window's centre is the current plane).  HBM
 // reads are the tile plus its halo once per plane; the values of the next
 // plane are loaded while the current one is computed.
-constexpr int D4T_X = 64, D4T_Y = 16, D4T_Z = 16, D4T_LX = D4T_X + 4, D4T_LY = D4T_Y + 4;
+//
+// Every value is widened to double once (into LDS, or into the z window as it
+// arrives) and each stencil is v0 - 4 v1 + 6 v2 - 4 v3 + v4 summed left to
+// right.  The products by -4 are exact (and by 6 for float inputs), so they
+// go through fma, which then rounds exactly as the separate product and sum
+// of the pointwise kernel do: the two kernels agree bitwise.
+constexpr int D4T_X = 64, D4T_Y = 16, D4T_Z = 32, D4T_LX = D4T_X + 4, D4T_LY = D4T_Y + 4;
 
 constexpr int D4T_HALO = D4T_LX * D4T_LY - D4T_X * D4T_Y;   // 336 halo slots per plane
 
@@ -351,15 +357,28 @@ __device__ __forceinline__ bool d4_needed(int v, int n, bool active)
     return active ? (v >= -2 && v <= n + 1) : v == 0;
 }
 
+// acc + c v with c v exact: fma rounds once, as the sum alone would; the
+// inexact 6 v (double inputs) keeps its own rounding (no contraction)
+__device__ __forceinline__ double d4_stencil(double v0, double v1, double v2, double v3, double v4, bool six_exact)
+{
+#pragma clang fp contract(off)
+    double acc = v0;
+    acc = fma(-4.0, v1, acc);
+    acc = six_exact ? fma(6.0, v2, acc) : acc + 6.0 * v2;
+    acc = fma(-4.0, v3, acc);
+    return acc + v4;
+}
+
 template <typename W>
 __global__ void __launch_bounds__(TPB) k_delta4_tile(D4Args<W> a, D4Multi<W> fm, int gx, int gy, int zl)
 {
+    constexpr bool SIX = sizeof(W) == 4;   // 6 v exact in double for float v
     const int fi = (int)blockIdx.y;
     W *__restrict__ out = fm.out[fi];
     double wd[3];
 #pragma unroll
     for (int k = 0; k < 3; k++) wd[k] = fm.wpow[fi] == 2 ? a.wd[k] * a.wd[k] : a.wd[k];
-    __shared__ W lds[D4T_LY * D4T_LX];   // holds input values, so W is exact
+    __shared__ double lds[D4T_LY * D4T_LX];   // input values, widened exactly
     const unsigned blk = xcd_block(blockIdx.x, gridDim.x);
     const int bx = (int)(blk % (unsigned)gx);
     const unsigned r = blk / (unsigned)gx;
@@ -373,29 +392,36 @@ __global__ void __launch_bounds__(TPB) k_delta4_tile(D4Args<W> a, D4Multi<W> fm,
     const int i0 = bx * D4T_X, j0 = by * D4T_Y, k0 = bz * zl;
     const int i = i0 + tx;
     const W *f = fm.f[fi];
-    const double D4[5] = {1.0, -4.0, 6.0, -4.0, 1.0};
-    // own points: rows j0 + ty + 4q, q = 0..3
-    double w[4][5];
+    // Every global load below is unconditional, from an in-bounds address
+    // (offset 0 of the plane for a slot nothing reads), and its value selected
+    // afterwards: with no load under a branch, the waits count loads instead
+    // of draining them all, so the loads of the next planes stay in flight
+    // through the current plane's LDS fill and stencils.
+    //
+    // own points: rows j0 + ty + 4q, q = 0..3, with the z window k-2..k+2
+    // widened in wz; plane k+3 is loaded during plane k into wn (stored type)
+    // and widened after plane k's stencils
+    double wz[4][5];
+    W wn[4];
     bool own[4];
+    int ooff[4];
     // LDS slots this thread fills on every plane: slots 0..3 are its own rows
     // (from the register window when the point is in the grid), slots 4..5 up
     // to two halo slots; src is the in-plane source offset, -1 for a slot that
     // no point reads (zero), and slot -1 marks a halo slot past the 336th
     int slot[6];
-    int64_t src[6];
+    int src[6];
+    // z planes are wrapped with a true modulo: the window reaches 3 planes out
+    auto zplane = [&](int v) { return za ? (pz ? ((v % nz) + nz) % nz : d4_index(v, nz, 0)) : d4_index(v, nz, 0); };
 #pragma unroll
     for (int q = 0; q < 4; q++) {
         const int j = j0 + ty + 4 * q;
         own[q] = i < nx && j < ny;
+        ooff[q] = own[q] ? j * sy + i : 0;
         slot[q] = (ty + 4 * q + 2) * D4T_LX + tx + 2;
         src[q] = -1;
         if (!own[q] && d4_needed(i, nx, xa) && d4_needed(j, ny, ya))
-            src[q] = (int64_t)d4_src(j, ny, py, ya) * sy + d4_src(i, nx, px, xa);
-#pragma unroll
-        for (int m = 0; m < 4; m++) {
-            const int kz = za ? d4_src(k0 - 2 + m, nz, pz, za) : d4_index(k0 - 2 + m, nz, 0);
-            w[q][m] = own[q] ? (double)f[(int64_t)kz * sz + (int64_t)j * sy + i] : 0.0;
-        }
+            src[q] = d4_src(j, ny, py, ya) * sy + d4_src(i, nx, px, xa);
     }
 #pragma unroll
     for (int e = 0; e < 2; e++) {
@@ -416,36 +442,43 @@ __global__ void __launch_bounds__(TPB) k_delta4_tile(D4Args<W> a, D4Multi<W> fm,
             }
             slot[4 + e] = yl * D4T_LX + xl;
             if (d4_needed(i0 + xl - 2, nx, xa) && d4_needed(j0 + yl - 2, ny, ya))
-                src[4 + e] = (int64_t)d4_src(j0 + yl - 2, ny, py, ya) * sy + d4_src(i0 + xl - 2, nx, px, xa);
+                src[4 + e] = d4_src(j0 + yl - 2, ny, py, ya) * sy + d4_src(i0 + xl - 2, nx, px, xa);
         }
     }
-    // the slot values of plane k are loaded one plane ahead, so their latency
-    // overlaps the previous plane's LDS fill and stencils
-    double cur[6], nxt[6];
-    const int kend = k0 + zl < nz ? k0 + zl : nz;
+    int soff[6];
 #pragma unroll
-    for (int e = 0; e < 6; e++) {
-        cur[e] = src[e] >= 0 ? (double)f[(int64_t)k0 * sz + src[e]] : 0.0;
-        nxt[e] = 0.0;
+    for (int e = 0; e < 6; e++) soff[e] = src[e] >= 0 ? src[e] : 0;
+#pragma unroll
+    for (int m = 0; m < 5; m++) {
+        const W *pl = f + (int64_t)zplane(k0 - 2 + m) * sz;
+#pragma unroll
+        for (int q = 0; q < 4; q++) wz[q][m] = (double)pl[ooff[q]];
+    }
+    // the slot values of plane k are loaded one plane ahead
+    W cur[6], nxt[6];
+    const int kend = k0 + zl < nz ? k0 + zl : nz;
+    {
+        const W *pl = f + (int64_t)k0 * sz;
+#pragma unroll
+        for (int e = 0; e < 6; e++) cur[e] = pl[soff[e]];
     }
     for (int k = k0; k < kend; k++) {
+        {
+            const W *pl = f + (int64_t)zplane(k + 3) * sz;
 #pragma unroll
-        for (int q = 0; q < 4; q++) {
-            const int j = j0 + ty + 4 * q;
-            const int kz = za ? d4_src(k + 2, nz, pz, za) : d4_index(k + 2, nz, 0);
-            w[q][4] = own[q] ? (double)f[(int64_t)kz * sz + (int64_t)j * sy + i] : 0.0;
+            for (int q = 0; q < 4; q++) wn[q] = pl[ooff[q]];
         }
-        if (k + 1 < kend) {
-            const W *next = f + (int64_t)(k + 1) * sz;
+        {
+            const W *pl = f + (int64_t)(k + 1 < kend ? k + 1 : k) * sz;
 #pragma unroll
-            for (int e = 0; e < 6; e++) nxt[e] = src[e] >= 0 ? (double)next[src[e]] : 0.0;
+            for (int e = 0; e < 6; e++) nxt[e] = pl[soff[e]];
         }
         __syncthreads();   // the previous plane's stencil reads are done
 #pragma unroll
-        for (int q = 0; q < 4; q++) lds[slot[q]] = (W)(own[q] ? w[q][2] : cur[q]);
+        for (int q = 0; q < 4; q++) lds[slot[q]] = own[q] ? wz[q][2] : (src[q] >= 0 ? (double)cur[q] : 0.0);
 #pragma unroll
         for (int e = 4; e < 6; e++)
-            if (slot[e] >= 0) lds[slot[e]] = (W)cur[e];
+            if (slot[e] >= 0) lds[slot[e]] = src[e] >= 0 ? (double)cur[e] : 0.0;
         __syncthreads();
 #pragma unroll
         for (int q = 0; q < 4; q++) {
@@ -453,27 +486,23 @@ __global__ void __launch_bounds__(TPB) k_delta4_tile(D4Args<W> a, D4Multi<W> fm,
                 const int xl = tx + 2, yl = ty + 4 * q + 2;
                 W acc_out = 0;
                 if (xa) {
-                    double acc = 0.0;
-#pragma unroll
-                    for (int m = 0; m < 5; m++) acc += D4[m] * (double)lds[yl * D4T_LX + xl + m - 2];
-                    acc_out += (W)(wd[0] * fabs(acc));
+                    const double *c = lds + yl * D4T_LX + xl;
+                    acc_out += (W)(wd[0] * fabs(d4_stencil(c[-2], c[-1], c[0], c[1], c[2], SIX)));
                 }
                 if (ya) {
-                    double acc = 0.0;
-#pragma unroll
-                    for (int m = 0; m < 5; m++) acc += D4[m] * (double)lds[(yl + m - 2) * D4T_LX + xl];
-                    acc_out += (W)(wd[1] * fabs(acc));
+                    const double *c = lds + yl * D4T_LX + xl;
+                    acc_out += (W)(wd[1] * fabs(d4_stencil(c[-2 * D4T_LX], c[-D4T_LX], c[0], c[D4T_LX], c[2 * D4T_LX], SIX)));
                 }
-                if (za) {
-                    double acc = 0.0;
-#pragma unroll
-                    for (int m = 0; m < 5; m++) acc += D4[m] * w[q][m];
-                    acc_out += (W)(wd[2] * fabs(acc));
-                }
-                out[(int64_t)k * sz + (int64_t)(j0 + yl - 2) * sy + i] = acc_out;
+                if (za)
+                    acc_out += (W)(wd[2] * fabs(d4_stencil(wz[q][0], wz[q][1], wz[q][2], wz[q][3], wz[q][4], SIX)));
+                out[(int64_t)k * sz + ooff[q]] = acc_out;
             }
+        }
 #pragma unroll
-            for (int m = 0; m < 4; m++) w[q][m] = w[q][m + 1];
+        for (int q = 0; q < 4; q++) {
+#pragma unroll
+            for (int m = 0; m < 4; m++) wz[q][m] = wz[q][m + 1];
+            wz[q][4] = (double)wn[q];
         }
 #pragma unroll
         for (int e = 0; e < 6; e++) cur[e] = nxt[e];
