@@ -21,6 +21,12 @@
 
 #include "cles_internal.h"
 
+// Multiply-adds in the line kernels are written as explicit fma where they
+// matter and are otherwise not contracted: the compiler's choice of which
+// product to fuse can differ between instantiations of the same arithmetic
+// (chunk capacities, epilogues), and every kernel must round alike.
+#pragma clang fp contract(off)
+
 namespace cles {
 
 constexpr int MC = 32;     // maximum chunk length (points per thread)
@@ -80,7 +86,7 @@ __device__ __forceinline__ void epilogue(const EpiArgs &e, int64_t o, double v)
         const double dt = EPI == EPI_LSRK_DEV ? e.dtp[ii] : e.dt;
         double nv = fma(dt, dq, e.c0 * e.du[o]);
         e.du[o] = nv;
-        e.Q[o] += e.c1 * nv;
+        e.Q[o] = fma(e.c1, nv, e.Q[o]);
     } else if constexpr (EPI == EPI_ART_MUBETA) {
         // artificial.jl:296-304
         const T *rho = (const T *)e.a0;
@@ -172,7 +178,7 @@ __device__ __forceinline__ void epilogue_batch(const EpiArgs &e, const int64_t (
             const double dq = acc ? (double)(a[i] - v[i]) : -(double)v[i];
             const double nv = fma(dtv[i], dq, e.c0 * du[i]);
             e.du[o[i]] = nv;
-            e.Q[o[i]] = q[i] + e.c1 * nv;
+            e.Q[o[i]] = fma(e.c1, nv, q[i]);
         }
     } else if constexpr (EPI == EPI_ART_MUBETA) {
         const T *rho = (const T *)e.a0;
@@ -235,6 +241,25 @@ __device__ __forceinline__ void select_field(const LineFields &mf, const T *__re
 // ---------------------------------------------------------------------------
 // Per-thread chunk machinery
 // ---------------------------------------------------------------------------
+// Interior RHS of one row from the window fv (centre index c):
+// a0 f_c + sum_m c_m (f_{c+m} + f_{c-m}) (symmetric) or
+// sum_m c_m (f_{c+m} - f_{c-m}), summed left to right, one fma per term.
+template <typename T, int M, int SYM, int W>
+__device__ __forceinline__ T rhs_row(const T (&fv)[W], int c, const T (&ci)[M], T a0)
+{
+    T acc;
+    if constexpr (SYM) {
+        acc = a0 * fv[c];
+#pragma unroll
+        for (int m = 1; m <= M; m++) acc = fma(ci[m - 1], fv[c + m] + fv[c - m], acc);
+    } else {
+        acc = ci[0] * (fv[c + 1] - fv[c - 1]);
+#pragma unroll
+        for (int m = 2; m <= M; m++) acc = fma(ci[m - 1], fv[c + m] - fv[c - m], acc);
+    }
+    return acc;
+}
+
 template <typename T, int Q, int M, int SYM, int CL = MC>
 struct Chunk {
     static constexpr int B = 2 * Q;
@@ -254,29 +279,22 @@ struct Chunk {
         const bool hi = p.closed && k == p.P - 1;
 #pragma unroll
         for (int t = 0; t < CL; t++) {
-            T acc;
-            if constexpr (SYM) {
-                acc = a0 * fv[t + M];
-#pragma unroll
-                for (int m = 1; m <= M; m++) acc += ci[m - 1] * (fv[t + M + m] + fv[t + M - m]);
-            } else {
-                acc = T(0);
-#pragma unroll
-                for (int m = 1; m <= M; m++) acc += ci[m - 1] * (fv[t + M + m] - fv[t + M - m]);
-            }
+            T acc = rhs_row<T, M, SYM>(fv, t + M, ci, a0);
             if (t < NCMAX) {
                 if (lo && t < p.nc) {
-                    acc = T(0);
+                    const T *cr = cf + COEF_CLO + t * 8;
+                    acc = cr[0] * fv[M];
 #pragma unroll
-                    for (int kap = 0; kap < 8; kap++) acc += cf[COEF_CLO + t * 8 + kap] * fv[kap + M];
+                    for (int kap = 1; kap < 8; kap++) acc = fma(cr[kap], fv[kap + M], acc);
                 }
             }
             if (hi) {
                 const int jr = len - 1 - t;
                 if (jr >= 0 && jr < p.nc) {
-                    acc = T(0);
+                    const T *cr = cf + COEF_CHI + jr * 8;
+                    acc = cr[0] * fe[0];
 #pragma unroll
-                    for (int kap = 0; kap < 8; kap++) acc += cf[COEF_CHI + jr * 8 + kap] * fe[kap];
+                    for (int kap = 1; kap < 8; kap++) acc = fma(cr[kap], fe[kap], acc);
                 }
             }
             d[t] = acc;
@@ -294,12 +312,12 @@ struct Chunk {
             const T *rc = rw + (size_t)R1_C * n + rs;
 #pragma unroll
             for (int t = 1; t < CL; t++)
-                if (t < len) d[t] -= rl[t] * d[t - 1];
+                if (t < len) d[t] = fma(-rl[t], d[t - 1], d[t]);
 #pragma unroll
             for (int t = CL - 1; t >= 0; t--) {
                 if (t < len) {
                     const T nxt = d[t + 1 < CL ? t + 1 : CL - 1];
-                    d[t] = (t == len - 1) ? d[t] * rdi[t] : (d[t] - rc[t] * nxt) * rdi[t];
+                    d[t] = (t == len - 1) ? d[t] * rdi[t] : fma(-rc[t], nxt, d[t]) * rdi[t];
                 }
             }
         } else {
@@ -312,9 +330,9 @@ struct Chunk {
             for (int t = 0; t < CL - 1; t++) {
                 if (t < len - 1) {
                     const T xt = d[t];
-                    d[t + 1] -= l1[t] * xt;
+                    d[t + 1] = fma(-l1[t], xt, d[t + 1]);
                     if (t + 2 < CL)
-                        if (t < len - 2) d[t + 2 < CL ? t + 2 : t] -= l2[t] * xt;
+                        if (t < len - 2) d[t + 2 < CL ? t + 2 : t] = fma(-l2[t], xt, d[t + 2 < CL ? t + 2 : t]);
                 }
             }
 #pragma unroll
@@ -322,9 +340,9 @@ struct Chunk {
                 if (t < len) {
                     T acc = d[t];
                     if (t + 1 < CL)
-                        if (t + 1 < len) acc -= u1[t] * d[t + 1 < CL ? t + 1 : t];
+                        if (t + 1 < len) acc = fma(-u1[t], d[t + 1 < CL ? t + 1 : t], acc);
                     if (t + 2 < CL)
-                        if (t + 2 < len) acc -= u2[t] * d[t + 2 < CL ? t + 2 : t];
+                        if (t + 2 < len) acc = fma(-u2[t], d[t + 2 < CL ? t + 2 : t], acc);
                     d[t] = acc * u0[t];
                 }
             }
@@ -374,7 +392,7 @@ __device__ __forceinline__ void solve_grouped(T (&d)[CL], int len, Rows R)
             for (int i = 0; i < G; i++) {
                 const int t = g * G + i;
                 if (t >= 1) {
-                    const T nv = d[t] - cur[i] * d[t - 1];
+                    const T nv = fma(-cur[i], d[t - 1], d[t]);
                     d[t] = t < len ? nv : d[t];
                 }
             }
@@ -404,7 +422,7 @@ __device__ __forceinline__ void solve_grouped(T (&d)[CL], int len, Rows R)
             for (int i = G - 1; i >= 0; i--) {
                 const int t = g * G + i;
                 const T nx = d[t + 1 < CL ? t + 1 : CL - 1];
-                const T v = (t == len - 1) ? d[t] * cd[i] : (d[t] - cc[i] * nx) * cd[i];
+                const T v = (t == len - 1) ? d[t] * cd[i] : fma(-cc[i], nx, d[t]) * cd[i];
                 d[t] = t < len ? v : d[t];
             }
             if (g > 0) {
@@ -437,10 +455,10 @@ __device__ __forceinline__ void solve_grouped(T (&d)[CL], int len, Rows R)
                 const int t = g * G + i;
                 if (t < CL - 1) {
                     const T xt = d[t];
-                    const T n1 = d[t + 1] - a1[i] * xt;
+                    const T n1 = fma(-a1[i], xt, d[t + 1]);
                     d[t + 1] = t < len - 1 ? n1 : d[t + 1];
                     if (t + 2 < CL) {
-                        const T n2 = d[t + 2 < CL ? t + 2 : t] - a2[i] * xt;
+                        const T n2 = fma(-a2[i], xt, d[t + 2 < CL ? t + 2 : t]);
                         d[t + 2 < CL ? t + 2 : t] = t < len - 2 ? n2 : d[t + 2 < CL ? t + 2 : t];
                     }
                 }
@@ -477,11 +495,11 @@ __device__ __forceinline__ void solve_grouped(T (&d)[CL], int len, Rows R)
                 const int t = g * G + i;
                 T acc = d[t];
                 if (t + 1 < CL) {
-                    const T x1 = acc - u1[i] * d[t + 1 < CL ? t + 1 : t];
+                    const T x1 = fma(-u1[i], d[t + 1 < CL ? t + 1 : t], acc);
                     acc = t + 1 < len ? x1 : acc;
                 }
                 if (t + 2 < CL) {
-                    const T x2 = acc - u2[i] * d[t + 2 < CL ? t + 2 : t];
+                    const T x2 = fma(-u2[i], d[t + 2 < CL ? t + 2 : t], acc);
                     acc = t + 2 < len ? x2 : acc;
                 }
                 d[t] = t < len ? acc * u0[i] : d[t];
@@ -608,7 +626,7 @@ __device__ __forceinline__ void reduced_solve(const LineDev &p, double *buf, int
     for (int i = 0; i < B; i++) {
         double acc = 0.0;
 #pragma unroll
-        for (int j = 0; j < B; j++) acc += mi[i * B + j] * r[j];
+        for (int j = 0; j < B; j++) acc = fma(mi[i * B + j], r[j], acc);
         z[i] = acc;
     }
     double *bf = buf + (p.nsteps & 1) * B * NT;
@@ -880,7 +898,7 @@ __device__ __forceinline__ void reduced_solve_wave(const LineDev &p, int k, int 
     for (int i = 0; i < B; i++) {
         double acc = 0.0;
 #pragma unroll
-        for (int j = 0; j < B; j++) acc += gload(mi + i * B + j) * r[j];
+        for (int j = 0; j < B; j++) acc = fma(gload(mi + i * B + j), r[j], acc);
         z[i] = acc;
     }
     int kprev = k - 1, knext = k + 1;
@@ -904,11 +922,14 @@ __device__ __forceinline__ void reduced_solve_wave(const LineDev &p, int k, int 
 }
 
 // One wave per block.  The wave's 64/P lines are staged in LDS (a pad slot
-// after every XCL points keeps the chunk-strided reads conflict-free), so
-// global loads and stores are contiguous 512-byte rows.
-__device__ __forceinline__ int xw_slot(int x) { return x + x / XCL; }
+// after every XC points keeps the chunk-strided reads conflict-free), so
+// global loads and stores are contiguous 512-byte rows.  XC is the chunk
+// capacity (8 or 16): the sweeps run XC steps, so chunks of at most 8 points
+// take the 8-step kernel.
+template <int XC = XCL>
+__device__ __forceinline__ int xw_slot(int x) { return x + x / XC; }
 
-template <typename T, int Q, int M, int SYM, int EPI, bool UNIROW>
+template <typename T, int Q, int M, int SYM, int EPI, bool UNIROW, int XC>
 __global__ void __launch_bounds__(64) k_line_xw(LineDev p, const T *__restrict__ f, int64_t sy, int64_t sz, int ny,
                                                 int64_t nlines, int TS, EpiArgs e, LineFields mf)
 {
@@ -946,19 +967,19 @@ __global__ void __launch_bounds__(64) k_line_xw(LineDev p, const T *__restrict__
         return (int64_t)j * sy + (int64_t)k * sz;
     };
     {
-        T tmp[XCL];
+        T tmp[XC];
 #pragma unroll
-        for (int i = 0; i < XCL; i++) {
+        for (int i = 0; i < XC; i++) {
             tmp[i] = T(0);
             int lj, x;
             line_of(i, lj, x);
             if (i < cnt && lj < nl && l0 + lj < nlines) tmp[i] = f[line_off(lj) + x];
         }
 #pragma unroll
-        for (int i = 0; i < XCL; i++) {
+        for (int i = 0; i < XC; i++) {
             int lj, x;
             line_of(i, lj, x);
-            if (i < cnt && lj < nl) tile[lj * TS + xw_slot(x)] = tmp[i];
+            if (i < cnt && lj < nl) tile[lj * TS + xw_slot<XC>(x)] = tmp[i];
         }
     }
     __syncthreads();
@@ -968,12 +989,12 @@ __global__ void __launch_bounds__(64) k_line_xw(LineDev p, const T *__restrict__
     const int len = p.m + (k < p.extra ? 1 : 0);
     const int rs = UNIROW ? 0 : s;
     const int kc = UNIROW ? 0 : k;
-    T res[XCL];
+    T res[XC];
     {
         const T *trow = tile + (lane / P) * TS;
-        T fv[XCL + 2 * M];
+        T fv[XC + 2 * M];
 #pragma unroll
-        for (int t = -M; t < XCL + M; t++) {
+        for (int t = -M; t < XC + M; t++) {
             T val = T(0);
             if (t < len + M) {
                 int r = s + t;
@@ -983,33 +1004,33 @@ __global__ void __launch_bounds__(64) k_line_xw(LineDev p, const T *__restrict__
                 } else if (r >= n) {
                     if (p.periodic) r -= n; else ok = false;
                 }
-                if (ok) val = trow[xw_slot(r)];
+                if (ok) val = trow[xw_slot<XC>(r)];
             }
             fv[t + M] = val;
         }
         T fe[8];
 #pragma unroll
-        for (int kap = 0; kap < 8; kap++) fe[kap] = (p.closed && kap < n) ? trow[xw_slot(n - 1 - kap)] : T(0);
-        Chunk<T, Q, M, SYM, XCL> ch;
+        for (int kap = 0; kap < 8; kap++) fe[kap] = (p.closed && kap < n) ? trow[xw_slot<XC>(n - 1 - kap)] : T(0);
+        Chunk<T, Q, M, SYM, XC> ch;
         ch.rhs(p, k, len, fv, fe);
-        solve_grouped<T, Q, XCL, 8>(ch.d, len, RowsClamped<T>{rowp<T>(p), n, rs});
+        solve_grouped<T, Q, XC, 8>(ch.d, len, RowsClamped<T>{rowp<T>(p), n, rs});
         double r[2 * Q], tp[Q], hn[Q];
         ch.ends(len, r);
         reduced_solve_wave<Q>(p, k, kc, base, r, tp, hn);
-        correct_grouped<T, Q, XCL, 8>(ch.d, len, RowsClamped<double>{p.row64, n, rs}, tp, hn);
+        correct_grouped<T, Q, XC, 8>(ch.d, len, RowsClamped<double>{p.row64, n, rs}, tp, hn);
 #pragma unroll
-        for (int t = 0; t < XCL; t++) res[t] = ch.d[t];
+        for (int t = 0; t < XC; t++) res[t] = ch.d[t];
     }
     __syncthreads();   // all window reads done before the tile is overwritten
     {
         T *wrow = tile + (lane / P) * TS;
 #pragma unroll
-        for (int t = 0; t < XCL; t++)
-            if (t < len) wrow[xw_slot(s + t)] = res[t];
+        for (int t = 0; t < XC; t++)
+            if (t < len) wrow[xw_slot<XC>(s + t)] = res[t];
     }
     __syncthreads();
 #pragma unroll
-    for (int b = 0; b < XCL; b += 8) {
+    for (int b = 0; b < XC; b += 8) {
         int64_t o[8];
         T v[8];
         bool ok[8];
@@ -1019,7 +1040,7 @@ __global__ void __launch_bounds__(64) k_line_xw(LineDev p, const T *__restrict__
             line_of(b + i, lj, x);
             ok[i] = (b + i) < cnt && lj < nl && l0 + lj < nlines;
             o[i] = ok[i] ? line_off(lj) + x : 0;
-            v[i] = ok[i] ? tile[lj * TS + xw_slot(x)] : T(0);
+            v[i] = ok[i] ? tile[lj * TS + xw_slot<XC>(x)] : T(0);
         }
         epilogue_batch<T, EPI, 8>(e, o, v, ok);
     }
@@ -1086,19 +1107,7 @@ __global__ void __launch_bounds__(64) k_line_xwf(LineDev p, const T *__restrict_
         for (int m = 0; m < M; m++) ci[m] = cf[COEF_CI + m];
         const T a0 = cf[COEF_A0];
 #pragma unroll
-        for (int t = 0; t < XCL; t++) {
-            T acc;
-            if constexpr (SYM) {
-                acc = a0 * fv[t + M];
-#pragma unroll
-                for (int m = 1; m <= M; m++) acc += ci[m - 1] * (fv[t + M + m] + fv[t + M - m]);
-            } else {
-                acc = T(0);
-#pragma unroll
-                for (int m = 1; m <= M; m++) acc += ci[m - 1] * (fv[t + M + m] - fv[t + M - m]);
-            }
-            ch.d[t] = acc;
-        }
+        for (int t = 0; t < XCL; t++) ch.d[t] = rhs_row<T, M, SYM>(fv, t + M, ci, a0);
     } else {
         ch.rhs(p, k, XCL, fv, fe);
     }
@@ -1183,15 +1192,21 @@ void launch_variant(const HostPlan &hp, int dim, const View3 &v, const T *f, con
     } else if (dim == 1 && P <= 64 && p.m + (p.extra ? 1 : 0) <= XCL) {
         const int64_t nlines = v.n[1] * v.n[2];
         const int nl = 64 / P;
-        const int TS = p.n + p.n / XCL + 1;
-        const size_t shm = (size_t)nl * TS * sizeof(T);
         const int64_t nblk = (nlines + nl - 1) / nl;
-        if (p.periodic && p.extra == 0)
-            hipLaunchKernelGGL((k_line_xw<T, Q, M, SYM, EPI, true>), dim3((unsigned)nblk, nf), dim3(64), shm, st, p,
-                               f, v.s[1], v.s[2], (int)v.n[1], nlines, TS, e, mf);
-        else
-            hipLaunchKernelGGL((k_line_xw<T, Q, M, SYM, EPI, false>), dim3((unsigned)nblk, nf), dim3(64), shm, st, p,
-                               f, v.s[1], v.s[2], (int)v.n[1], nlines, TS, e, mf);
+        const bool uni = p.periodic && p.extra == 0;
+        auto go = [&](auto kern, int xc) {
+            const int TS = p.n + p.n / xc + 1;
+            const size_t shm = (size_t)nl * TS * sizeof(T);
+            hipLaunchKernelGGL(kern, dim3((unsigned)nblk, nf), dim3(64), shm, st, p, f, v.s[1], v.s[2], (int)v.n[1],
+                               nlines, TS, e, mf);
+        };
+        if (p.m + (p.extra ? 1 : 0) <= 8) {
+            if (uni) go(k_line_xw<T, Q, M, SYM, EPI, true, 8>, 8);
+            else go(k_line_xw<T, Q, M, SYM, EPI, false, 8>, 8);
+        } else {
+            if (uni) go(k_line_xw<T, Q, M, SYM, EPI, true, XCL>, XCL);
+            else go(k_line_xw<T, Q, M, SYM, EPI, false, XCL>, XCL);
+        }
     } else if (dim == 1) {
         const size_t tile_row = (size_t)(p.n + 1) * sizeof(T);
         const int64_t nlines = v.n[1] * v.n[2];
@@ -1252,7 +1267,10 @@ void launch_variant(const HostPlan &hp, int dim, const View3 &v, const T *f, con
                     hipLaunchKernelGGL((k_line_yz<T, Q, M, SYM, EPI, true, MC, true>), dim3((unsigned)(nxb * nouter), nf),
                                        dim3(NT), shm, st, p, f, (int)sd, so, (int)nlane, nxb, Lx, e, mf);
             }
-            if (!nts && narrow)
+            if (!nts && p.m + (p.extra ? 1 : 0) <= 8)
+                hipLaunchKernelGGL((k_line_yz<T, Q, M, SYM, EPI, true, 8>), dim3((unsigned)(nxb * nouter), nf), dim3(NT),
+                                   shm, st, p, f, (int)sd, so, (int)nlane, nxb, Lx, e, mf);
+            else if (!nts && narrow)
                 hipLaunchKernelGGL((k_line_yz<T, Q, M, SYM, EPI, true, 16>), dim3((unsigned)(nxb * nouter), nf), dim3(NT),
                                    shm, st, p, f, (int)sd, so, (int)nlane, nxb, Lx, e, mf);
             else if (!nts)

@@ -66,20 +66,27 @@ def test_rollback_on_gpu(ctx):
     o = _orc(6.0, **ctl)
     o.run(1e9, 60)
     assert o.retries_used == 1 and o.cfl == 3.0 and o.time[1] == 60
-    # the GPU's own composed run: to the savepoint, on to the failure (which
-    # leaves the same lagged artificial coefficients), restore by hand, lower the CFL
+    # the GPU's own composed run: on to the failure in runs of ten steps,
+    # keeping the state at every savepoint step (the multiples of 10); the
+    # failure leaves the same lagged artificial coefficients; restore the last
+    # savepoint by hand, lower the CFL.  The step the blow-up lands on depends
+    # on the last bits of every step before it, so it is found, not assumed.
     b = _gpu(ctx, 6.0)
-    db1 = b.run(1e9, 20, record=20)
-    Q20, t20 = _state(b), (b.t, b.step_count)
+    saved, db1 = {}, []
     with pytest.raises(ch.SolverFailure) as e:
-        b.run(1e9, 60)
+        while b.step_count < 60:
+            saved[b.step_count] = (_state(b), (b.t, b.step_count))
+            db1.append(b.run(1e9, b.step_count + 10, record=10))
     fail = b.step_count
-    assert e.value.reason == "negative_density" and 20 < fail < 30
-    b.upload(Q20)
-    b.set_time(*t20)
+    sp = fail // 10 * 10
+    assert e.value.reason == "negative_density" and fail > 10
+    Qs, ts = saved[sp]
+    b.upload(Qs)
+    b.set_time(*ts)
     b.cfl = 3.0
-    db2 = b.run(1e9, 60, record=40)
-    assert np.array_equal(da[:20], db1) and np.array_equal(da[fail:fail + 40], db2)
+    db2 = b.run(1e9, 60, record=60 - sp)
+    assert np.array_equal(da[:sp], np.concatenate(db1[:sp // 10]))
+    assert np.array_equal(da[fail:fail + 60 - sp], db2)
     assert np.array_equal(_state(a), _state(b))
 
 
