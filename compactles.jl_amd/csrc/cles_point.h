@@ -257,6 +257,8 @@ void launch_rate_inst(const RateArgs<W> &a, double *part, int nb, int ni, double
 template <typename W>
 void launch_nscbc(const NscbcArgs<W> &a, hipStream_t st);
 template <typename W>
+void launch_nscbc_pair(const NscbcArgs<W> &a0, const NscbcArgs<W> &a1, hipStream_t st);
+template <typename W>
 void launch_rate_batch(const RateArgs<W> &a, const BatchGeom &g, const int *mask, double *rate, double *rhomin,
                        hipStream_t st);
 template <typename W>

@@ -1083,9 +1083,8 @@ __global__ void __launch_bounds__(TPB) k_extrapolate(PlaneArgs a, double *__rest
 // Arithmetic is FP64 in the reference's operation order for every precision.
 // ---------------------------------------------------------------------------
 template <typename W>
-__global__ void __launch_bounds__(TPB) k_nscbc(NscbcArgs<W> a)
+__device__ __forceinline__ void nscbc_point(const NscbcArgs<W> &a, int64_t t)
 {
-    const int64_t t = (int64_t)blockIdx.x * TPB + threadIdx.x;
     if (t >= a.count) return;
     const int d = a.dim;
     const int oa = d == 0 ? 1 : 0, ob = d == 2 ? 1 : 2;
@@ -1184,6 +1183,25 @@ __global__ void __launch_bounds__(TPB) k_nscbc(NscbcArgs<W> a)
     }
     W *qe = a.dQ + (int64_t)(im + 3) * N + I;
     *qe = (W)((double)*qe - (ke * dd1 + phi * dd2 + p * sphiY + rho * (un * dd3 + uv[t1] * dd4 + uv[t2] * dd5)));
+}
+
+template <typename W>
+__global__ void __launch_bounds__(TPB) k_nscbc(NscbcArgs<W> a)
+{
+    nscbc_point(a, (int64_t)blockIdx.x * TPB + threadIdx.x);
+}
+
+// Both faces of one dimension in one launch (grid y = side): their planes are
+// disjoint (n >= 2 along the normal), and each thread updates dQ at its own
+// face point only, so the result is that of the two launches in order.
+template <typename W>
+__global__ void __launch_bounds__(TPB) k_nscbc_pair(NscbcArgs<W> a0, NscbcArgs<W> a1)
+{
+    const int64_t t = (int64_t)blockIdx.x * TPB + threadIdx.x;
+    if (blockIdx.y == 0)
+        nscbc_point(a0, t);
+    else
+        nscbc_point(a1, t);
 }
 
 // ---------------------------------------------------------------------------
@@ -1628,6 +1646,18 @@ void launch_nscbc(const NscbcArgs<W> &a, hipStream_t st)
 }
 template void launch_nscbc<double>(const NscbcArgs<double> &, hipStream_t);
 template void launch_nscbc<float>(const NscbcArgs<float> &, hipStream_t);
+
+template <typename W>
+void launch_nscbc_pair(const NscbcArgs<W> &a0, const NscbcArgs<W> &a1, hipStream_t st)
+{
+    if (a0.dim != a1.dim || a0.side == a1.side || a0.n[a0.dim] < 2 || a0.count != a1.count)
+        throw Error{CLES_ERR_INVALID, "an NSCBC face pair is the two faces of one dimension"};
+    if (a0.count <= 0) return;
+    hipLaunchKernelGGL((k_nscbc_pair<W>), dim3(nblocks(a0.count), 2), dim3(TPB), 0, st, a0, a1);
+    CHECK_LAUNCH();
+}
+template void launch_nscbc_pair<double>(const NscbcArgs<double> &, const NscbcArgs<double> &, hipStream_t);
+template void launch_nscbc_pair<float>(const NscbcArgs<float> &, const NscbcArgs<float> &, hipStream_t);
 
 void launch_slipwall(const PlaneArgs &a, double *Q, hipStream_t st)
 {

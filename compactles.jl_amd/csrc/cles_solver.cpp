@@ -885,7 +885,12 @@ struct Impl {
     {
         if (!s.nscbc) return;
         void *vel[3] = {s.u, s.v, s.w};
-        for (int d = 0; d < 3; d++)
+        const double w = sizeof(W);
+        for (int d = 0; d < 3; d++) {
+            // with the derivatives from the gradient launches, the two faces
+            // of a dimension go through one launch
+            NscbcArgs<W> held[2];
+            bool has[2] = {false, false};
             for (int side = 0; side < 2; side++) {
                 if (!s.active[d]) continue;
                 const int bc = s.d.bc[d][side];
@@ -947,10 +952,24 @@ struct Impl {
                 a.sg_phi = s.d.eos_kind == CLES_EOS_STIFFENED ? 1 / (s.d.gamma[0] - 1) : 0.0;
                 a.met = s.met;
                 a.dQ = f(s.dQ);
-                const double w = sizeof(W);
+                if (s.nsd_merged) {
+                    held[side] = a;
+                    has[side] = true;
+                    continue;
+                }
                 s.timed(PH_NSCBC, (w * (14 + 2 * s.Ns) + 2 * w * s.C) * (double)a.count,
                         [&] { launch_nscbc<W>(a, s.st()); });
             }
+            if (has[0] && has[1]) {
+                s.timed(PH_NSCBC, 2 * (w * (14 + 2 * s.Ns) + 2 * w * s.C) * (double)held[0].count,
+                        [&] { launch_nscbc_pair<W>(held[0], held[1], s.st()); });
+            } else {
+                for (int side = 0; side < 2; side++)
+                    if (has[side])
+                        s.timed(PH_NSCBC, (w * (14 + 2 * s.Ns) + 2 * w * s.C) * (double)held[side].count,
+                                [&] { launch_nscbc<W>(held[side], s.st()); });
+            }
+        }
     }
 
     // boundary.jl:280-286 apply_bcs!
