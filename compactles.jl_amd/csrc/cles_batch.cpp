@@ -44,7 +44,7 @@ struct cles_batch {
     void *tables = nullptr;
     cles::LineDev *dplans = nullptr, *fplans = nullptr;
     int *order_d = nullptr, *items_d = nullptr, *order_f = nullptr, *items_f = nullptr;
-    int nitems_d = 0, nitems_f = 0;
+    int nitems_d = 0, nitems_f = 0, nshort_d = 0, nshort_f = 0;
     int dq = 1, dM = 2, dsym = 0, fq = 1, fM = 4, fsym = 1;
 
     // geometry on the device
@@ -139,6 +139,7 @@ namespace {
 struct PlanSet {
     std::vector<LineDev> dev;                       // per instance, pointers are blob offsets
     std::vector<int> order, items;
+    int nshort = 0;                                 // leading items of the short class
 };
 
 void build_plan_set(PlanSet &ps, std::vector<unsigned char> &blob, const cles_scheme_desc &s,
@@ -153,8 +154,12 @@ void build_plan_set(PlanSet &ps, std::vector<unsigned char> &blob, const cles_sc
         auto key = std::make_pair(n[i], hk);
         auto it = cache.find(key);
         if (it == cache.end()) {
+            // lines of up to 64 x 16 points are split into chunks of at most
+            // 16 (the short class: a kernel with half the registers and LDS
+            // per wave and sweeps half as long); longer ones into up to 32
             LineTables t;
-            build_line_tables(t, s, n[i], periodic, h[i], RAG_MAX_CHUNK, 1, true);
+            const int mc = n[i] <= 64 * RAG_SHORT_CHUNK ? RAG_SHORT_CHUNK : RAG_MAX_CHUNK;
+            build_line_tables(t, s, n[i], periodic, h[i], mc, 1, true);
             if (t.dev.P > 64)
                 throw Error{CLES_ERR_UNSUPPORTED, "batched instances are limited to " +
                                                       std::to_string(64 * RAG_MAX_CHUNK) + " points"};
@@ -174,20 +179,28 @@ void build_plan_set(PlanSet &ps, std::vector<unsigned char> &blob, const cles_sc
         ps.dev[i] = it->second.first;
         P[i] = ps.dev[i].P;
     }
-    // instances grouped by chunk count; one wave holds 64/P of them
+    // instances grouped by class (short first), then chunk count; one wave
+    // holds 64/P of them
+    std::vector<int> cls(count);
+    for (int64_t i = 0; i < count; i++)
+        cls[i] = ps.dev[i].m + (ps.dev[i].extra ? 1 : 0) <= RAG_SHORT_CHUNK ? 0 : 1;
     ps.order.resize(count);
     for (int64_t i = 0; i < count; i++) ps.order[i] = (int)i;
-    std::stable_sort(ps.order.begin(), ps.order.end(), [&](int a, int b) { return P[a] < P[b]; });
+    std::stable_sort(ps.order.begin(), ps.order.end(),
+                     [&](int a, int b) { return cls[a] != cls[b] ? cls[a] < cls[b] : P[a] < P[b]; });
     ps.items.clear();
+    ps.nshort = 0;
     int64_t k = 0;
     while (k < count) {
-        const int Pk = P[ps.order[k]];
+        const int i0 = ps.order[k];
+        const int Pk = P[i0], ck = cls[i0];
         const int per = 64 / Pk;
         int cnt = 0;
-        while (k + cnt < count && cnt < per && P[ps.order[k + cnt]] == Pk) cnt++;
+        while (k + cnt < count && cnt < per && P[ps.order[k + cnt]] == Pk && cls[ps.order[k + cnt]] == ck) cnt++;
         ps.items.push_back(Pk);
         ps.items.push_back((int)k);
         ps.items.push_back(cnt);
+        if (ck == 0) ps.nshort++;
         k += cnt;
     }
 }
@@ -237,6 +250,7 @@ struct BImpl {
         r.order = filter ? b.order_f : b.order_d;
         r.items = filter ? b.items_f : b.items_d;
         r.nitems = filter ? b.nitems_f : b.nitems_d;
+        r.nshort = filter ? b.nshort_f : b.nshort_d;
         r.dt = dtv;
         r.mask = mask;
         r.q = filter ? b.fq : b.dq;
@@ -586,9 +600,11 @@ int cles_batch_create(cles_ctx *ctx, const cles_solver_desc *desc, int64_t count
             b->order_d = b->upload_vec(pd.order);
             b->items_d = b->upload_vec(pd.items);
             b->nitems_d = (int)(pd.items.size() / 3);
+            b->nshort_d = pd.nshort;
             b->order_f = b->upload_vec(pf.order);
             b->items_f = b->upload_vec(pf.items);
             b->nitems_f = (int)(pf.items.size() / 3);
+            b->nshort_f = pf.nshort;
             b->dq = d.deriv.q; b->dM = d.deriv.M; b->dsym = d.deriv.symmetric ? 1 : 0;
             b->fq = d.filt.q; b->fM = d.filt.M; b->fsym = d.filt.symmetric ? 1 : 0;
 

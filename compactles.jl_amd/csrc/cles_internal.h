@@ -76,6 +76,7 @@ struct LineDev {
 };
 
 constexpr int RAG_MAX_CHUNK = 32;   // points per thread in the ragged kernel
+constexpr int RAG_SHORT_CHUNK = 16;   // chunk bound of the short class (lines of at most 64 x 16 points)
 constexpr int COEF_CI = 0, COEF_A0 = 8, COEF_CLO = 16, COEF_CHI = 80, COEF_N = 144;
 
 struct HostPlan {
@@ -96,7 +97,7 @@ struct LineTables {
     size_t o_coef32 = 0, o_row64 = 0, o_row32 = 0, o_pcr = 0, o_rowt64 = 0, o_rowt32 = 0;
 };
 // chunk_major: also store the chunk-major row tables of the ragged kernel
-// (max_chunk must then be RAG_MAX_CHUNK).
+// (RAG_MAX_CHUNK rows per chunk; max_chunk must not exceed it).
 // lo_fold (0: none, +1 / -1: the solution parity sigma_g) replaces the low
 // edge's closure by a parity fold on a half-offset grid (operators.jl:59-69,
 // operators_banded.jl:55-64): the LHS coupling to ghost unknown j < 0 moves to
@@ -202,6 +203,7 @@ struct RagLines {
     const int *order;        // instances grouped by chunk count
     const int *items;        // 3 ints per work item
     int nitems;
+    int nshort;              // items [0, nshort) hold lines of chunks of at most RAG_SHORT_CHUNK points
     const double *dt;        // per-instance dt for the LSRK epilogue (NULL: epi.dt)
     const int *mask;         // per-instance: nonzero applies the epilogue (NULL: all)
     int q, M, sym;           // scheme shape shared by every instance

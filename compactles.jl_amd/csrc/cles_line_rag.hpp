@@ -14,7 +14,10 @@
 
 namespace cles {
 
-__device__ __forceinline__ int rag_slot(int x) { return x + (x >> 5); }
+// a pad slot after every CL points (the chunk capacity) spreads the
+// chunk-strided reads over the banks
+template <int CL>
+__device__ __forceinline__ int rag_slot(int x) { return x + x / CL; }
 
 // Blocks are dispatched round-robin over the 8 XCDs; renumbering them so that
 // each XCD takes one contiguous range keeps the nf blocks of one work item
@@ -46,10 +49,11 @@ struct RowsRag {
     }
 };
 
-template <typename T, int Q, int M, int SYM, int EPI>
+// CL: chunk capacity of the launch's items (RAG_SHORT_CHUNK for the short
+// class, RAG_MAX_CHUNK otherwise); the plan rows keep RAG_MAX_CHUNK per chunk.
+template <typename T, int Q, int M, int SYM, int EPI, int CL>
 __global__ void __launch_bounds__(64) k_line_rag(RagLines r, const T *__restrict__ f, EpiArgs e)
 {
-    constexpr int CL = RAG_MAX_CHUNK;
     extern __shared__ double smem[];
     T *tile = (T *)smem;
     const int lane = threadIdx.x;
@@ -60,7 +64,7 @@ __global__ void __launch_bounds__(64) k_line_rag(RagLines r, const T *__restrict
     if constexpr (EPI == EPI_ART_MULTI) e = e.multi[fld];   // its own outputs: no field offset
     const int *it = r.items + 3 * item;
     const int P = it[0], start = it[1], count = it[2];
-    const int TS = 33 * P;   // rag_slot(32 P - 1) < 33 P
+    const int TS = (CL + 1) * P;   // rag_slot<CL>(CL P - 1) < (CL + 1) P
 
     // stage the item's lines
     for (int lj = 0; lj < count; lj++) {
@@ -78,7 +82,7 @@ __global__ void __launch_bounds__(64) k_line_rag(RagLines r, const T *__restrict
 #pragma unroll
             for (int i = 0; i < 8; i++) {
                 const int x = x0 + lane + 64 * i;
-                if (x < n) row[rag_slot(x)] = v[i];
+                if (x < n) row[rag_slot<CL>(x)] = v[i];
             }
         }
     }
@@ -105,13 +109,13 @@ __global__ void __launch_bounds__(64) k_line_rag(RagLines r, const T *__restrict
             } else if (rr >= n) {
                 if (p.periodic) rr -= n; else ok = false;
             }
-            if (ok) val = trow[rag_slot(rr)];
+            if (ok) val = trow[rag_slot<CL>(rr)];
         }
         fv[t + M] = val;
     }
     T fe[8];
 #pragma unroll
-    for (int kap = 0; kap < 8; kap++) fe[kap] = (p.closed && kap < n) ? trow[rag_slot(n - 1 - kap)] : T(0);
+    for (int kap = 0; kap < 8; kap++) fe[kap] = (p.closed && kap < n) ? trow[rag_slot<CL>(n - 1 - kap)] : T(0);
     Chunk<T, Q, M, SYM, CL> ch;
     ch.rhs(p, k, len, fv, fe);
     solve_grouped<T, Q, CL, 8>(ch.d, len, RowsRag<T>{rowtp<T>(p) + k, p.P});
@@ -123,7 +127,7 @@ __global__ void __launch_bounds__(64) k_line_rag(RagLines r, const T *__restrict
     if (live) {
 #pragma unroll
         for (int t = 0; t < CL; t++)
-            if (t < len) trow[rag_slot(s + t)] = ch.d[t];
+            if (t < len) trow[rag_slot<CL>(s + t)] = ch.d[t];
     }
     __syncthreads();
 
@@ -149,7 +153,7 @@ __global__ void __launch_bounds__(64) k_line_rag(RagLines r, const T *__restrict
                 const int x = x0 + lane + 64 * i;
                 ok[i] = x < nj;
                 o[i] = oj + (ok[i] ? x : 0);
-                v[i] = ok[i] ? row[rag_slot(x)] : T(0);
+                v[i] = ok[i] ? row[rag_slot<CL>(x)] : T(0);
             }
             epilogue_batch<T, EPI, 8>(ej, o, v, ok);
         }
@@ -159,16 +163,32 @@ __global__ void __launch_bounds__(64) k_line_rag(RagLines r, const T *__restrict
 template <typename T, int Q, int M, int SYM, int EPI>
 void launch_rag_variant(const RagLines &r, const T *f, const EpiArgs &e, hipStream_t st)
 {
+    if (r.nshort < 0 || r.nshort > r.nitems) throw Error{CLES_ERR_INVALID, "short items out of range"};
     if (r.nitems <= 0 || r.nf <= 0) return;
     if (r.nf > 1 && e.kind != EPI_STORE && e.kind != EPI_STORE_NEG && e.kind != EPI_LSRK && e.kind != EPI_ART_MULTI)
         throw Error{CLES_ERR_INVALID, "several fields per launch need a store, LSRK or multi epilogue"};
     if (e.kind == EPI_ART_MULTI && !e.multi) throw Error{CLES_ERR_INVALID, "the multi epilogue needs its table"};
     if (e.kind == EPI_LSRK && e.a0 && r.nf > 1)
         throw Error{CLES_ERR_INVALID, "the ragged LSRK epilogue takes no accumulator with several fields"};
-    const size_t shm = (size_t)64 * 33 * sizeof(T);
-    const int64_t nb = (int64_t)r.nitems * r.nf;
-    hipLaunchKernelGGL((k_line_rag<T, Q, M, SYM, EPI>), dim3((unsigned)nb), dim3(64), shm, st, r, f, e);
-    CLES_HIP(hipGetLastError());
+    // the short items, then the rest, each with its own chunk capacity
+    RagLines rs = r, rl = r;
+    rs.nitems = r.nshort;
+    rl.items = r.items + 3 * r.nshort;
+    rl.nitems = r.nitems - r.nshort;
+    if (rs.nitems > 0) {
+        const size_t shm = (size_t)64 * (RAG_SHORT_CHUNK + 1) * sizeof(T);
+        const int64_t nb = (int64_t)rs.nitems * r.nf;
+        hipLaunchKernelGGL((k_line_rag<T, Q, M, SYM, EPI, RAG_SHORT_CHUNK>), dim3((unsigned)nb), dim3(64), shm, st, rs, f,
+                           e);
+        CLES_HIP(hipGetLastError());
+    }
+    if (rl.nitems > 0) {
+        const size_t shm = (size_t)64 * (RAG_MAX_CHUNK + 1) * sizeof(T);
+        const int64_t nb = (int64_t)rl.nitems * r.nf;
+        hipLaunchKernelGGL((k_line_rag<T, Q, M, SYM, EPI, RAG_MAX_CHUNK>), dim3((unsigned)nb), dim3(64), shm, st, rl, f,
+                           e);
+        CLES_HIP(hipGetLastError());
+    }
 }
 
 template <typename T, int Q, int M, int SYM>

@@ -389,7 +389,7 @@ void build_line_tables(LineTables &out, const cles_scheme_desc &s, int n, int pe
     // chunk-major copy of the row constants for the ragged kernel
     std::vector<double> rowt;
     if (chunk_major) {
-        if (max_chunk != RAG_MAX_CHUNK) throw Error{CLES_ERR_INVALID, "chunk-major tables need 32-point chunks"};
+        if (max_chunk > RAG_MAX_CHUNK) throw Error{CLES_ERR_INVALID, "chunk-major tables hold chunks of at most 32 points"};
         rowt.assign((size_t)nrow * RAG_MAX_CHUNK * P, 0.0);
         for (int f = 0; f < nrow; f++)
             for (int k = 0; k < P; k++)
