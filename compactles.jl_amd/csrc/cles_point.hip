@@ -369,8 +369,10 @@ __device__ __forceinline__ double d4_stencil(double v0, double v1, double v2, do
     return acc + v4;
 }
 
+// FP64: four waves per SIMD asked for (126 VGPRs, no spills; the default
+// allocation takes 132 and fits only three)
 template <typename W>
-__global__ void __launch_bounds__(TPB) k_delta4_tile(D4Args<W> a, D4Multi<W> fm, int gx, int gy, int zl)
+__global__ void __launch_bounds__(TPB, sizeof(W) == 8 ? 4 : 1) k_delta4_tile(D4Args<W> a, D4Multi<W> fm, int gx, int gy, int zl)
 {
     constexpr bool SIX = sizeof(W) == 4;   // 6 v exact in double for float v
     const int fi = (int)blockIdx.y;
@@ -398,7 +400,9 @@ __global__ void __launch_bounds__(TPB) k_delta4_tile(D4Args<W> a, D4Multi<W> fm,
     // of draining them all, so the loads of the next planes stay in flight
     // through the current plane's LDS fill and stencils.
     //
-    // own points: rows j0 + ty + 4q, q = 0..3, with the z window k-2..k+2
+    // own points: rows j0 + 4 ty + q, q = 0..3 (four consecutive rows, so
+    // their y stencils share the thread's own values and read only the two
+    // rows above and below from LDS), with the z window k-2..k+2
     // widened in wz; plane k+3 is loaded during plane k into wn (stored type)
     // and widened after plane k's stencils
     double wz[4][5];
@@ -415,10 +419,10 @@ __global__ void __launch_bounds__(TPB) k_delta4_tile(D4Args<W> a, D4Multi<W> fm,
     auto zplane = [&](int v) { return za ? (pz ? ((v % nz) + nz) % nz : d4_index(v, nz, 0)) : d4_index(v, nz, 0); };
 #pragma unroll
     for (int q = 0; q < 4; q++) {
-        const int j = j0 + ty + 4 * q;
+        const int j = j0 + 4 * ty + q;
         own[q] = i < nx && j < ny;
         ooff[q] = own[q] ? j * sy + i : 0;
-        slot[q] = (ty + 4 * q + 2) * D4T_LX + tx + 2;
+        slot[q] = (4 * ty + q + 2) * D4T_LX + tx + 2;
         src[q] = -1;
         if (!own[q] && d4_needed(i, nx, xa) && d4_needed(j, ny, ya))
             src[q] = d4_src(j, ny, py, ya) * sy + d4_src(i, nx, px, xa);
@@ -473,26 +477,38 @@ __global__ void __launch_bounds__(TPB) k_delta4_tile(D4Args<W> a, D4Multi<W> fm,
 #pragma unroll
             for (int e = 0; e < 6; e++) nxt[e] = pl[soff[e]];
         }
+        // the values of the thread's own slots (as written to LDS)
+        double rv[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) rv[q] = own[q] ? wz[q][2] : (src[q] >= 0 ? (double)cur[q] : 0.0);
         __syncthreads();   // the previous plane's stencil reads are done
 #pragma unroll
-        for (int q = 0; q < 4; q++) lds[slot[q]] = own[q] ? wz[q][2] : (src[q] >= 0 ? (double)cur[q] : 0.0);
+        for (int q = 0; q < 4; q++) lds[slot[q]] = rv[q];
 #pragma unroll
         for (int e = 4; e < 6; e++)
             if (slot[e] >= 0) lds[slot[e]] = src[e] >= 0 ? (double)cur[e] : 0.0;
         __syncthreads();
+        // column of rows 4 ty - 2 .. 4 ty + 5 at this thread's x: two rows
+        // above and below from LDS, the four own rows from registers
+        const int xl = tx + 2, yb = 4 * ty + 2;
+        double yv[8];
+        yv[0] = lds[(yb - 2) * D4T_LX + xl];
+        yv[1] = lds[(yb - 1) * D4T_LX + xl];
+#pragma unroll
+        for (int q = 0; q < 4; q++) yv[2 + q] = rv[q];
+        yv[6] = lds[(yb + 4) * D4T_LX + xl];
+        yv[7] = lds[(yb + 5) * D4T_LX + xl];
 #pragma unroll
         for (int q = 0; q < 4; q++) {
             if (own[q]) {
-                const int xl = tx + 2, yl = ty + 4 * q + 2;
+                const int yl = yb + q;
                 W acc_out = 0;
                 if (xa) {
                     const double *c = lds + yl * D4T_LX + xl;
-                    acc_out += (W)(wd[0] * fabs(d4_stencil(c[-2], c[-1], c[0], c[1], c[2], SIX)));
+                    acc_out += (W)(wd[0] * fabs(d4_stencil(c[-2], c[-1], rv[q], c[1], c[2], SIX)));
                 }
-                if (ya) {
-                    const double *c = lds + yl * D4T_LX + xl;
-                    acc_out += (W)(wd[1] * fabs(d4_stencil(c[-2 * D4T_LX], c[-D4T_LX], c[0], c[D4T_LX], c[2 * D4T_LX], SIX)));
-                }
+                if (ya)
+                    acc_out += (W)(wd[1] * fabs(d4_stencil(yv[q], yv[q + 1], yv[q + 2], yv[q + 3], yv[q + 4], SIX)));
                 if (za)
                     acc_out += (W)(wd[2] * fabs(d4_stencil(wz[q][0], wz[q][1], wz[q][2], wz[q][3], wz[q][4], SIX)));
                 out[(int64_t)k * sz + ooff[q]] = acc_out;
