@@ -772,11 +772,37 @@ __global__ void __launch_bounds__(UNI && Q == 1 ? 1024 : 512) k_line_x(LineDev p
     const int tid = threadIdx.x;
     const int64_t l0 = (int64_t)blockIdx.x * Lt;
 
-    for (int li = 0; li < Lt; li++) {
-        const int64_t l = l0 + li;
-        if (l >= nlines) break;
-        const T *src = f + (l % ny) * sy + (l / ny) * sz;
-        for (int i = tid; i < n; i += NT) tile[li * TS + i] = src[i];
+    if constexpr (!UNI && CL <= 8) {
+        // element q = tid + NT i of the block's Lt lines is line q / n, point
+        // q % n; a thread holds at most ceil(n / P) <= CL of them, and all its
+        // loads are issued before the LDS writes
+        const int nel = Lt * n;
+        const int j0 = (int)(l0 % ny), k0 = (int)(l0 / ny);
+        T tmp[CL];
+        int dst[CL];
+#pragma unroll
+        for (int i = 0; i < CL; i++) {
+            const int q = tid + NT * i;
+            const int lj = q / n, x = q - lj * n;
+            int j = j0 + lj, kk = k0;
+            while (j >= ny) {
+                j -= ny;
+                kk++;
+            }
+            const bool ok = q < nel && l0 + lj < nlines;
+            tmp[i] = ok ? f[(int64_t)j * sy + (int64_t)kk * sz + x] : T(0);
+            dst[i] = q < nel ? lj * TS + x : -1;
+        }
+#pragma unroll
+        for (int i = 0; i < CL; i++)
+            if (dst[i] >= 0) tile[dst[i]] = tmp[i];
+    } else {
+        for (int li = 0; li < Lt; li++) {
+            const int64_t l = l0 + li;
+            if (l >= nlines) break;
+            const T *src = f + (l % ny) * sy + (l / ny) * sz;
+            for (int i = tid; i < n; i += NT) tile[li * TS + i] = src[i];
+        }
     }
     __syncthreads();
 
