@@ -693,20 +693,44 @@ __global__ void __launch_bounds__(UNI && Q == 1 ? 1024 : 512) k_line_yz(LineDev 
     const int len = p.m + (k < p.extra ? 1 : 0);
 
     T fv[CL + 2 * M];
+    if constexpr (UNI) {
+        // one chunk per wave: the branches below are uniform
 #pragma unroll
-    for (int t = -M; t < CL + M; t++) {
-        T val = T(0);
-        if (valid && t < len + M) {
-            int r = s + t;
-            bool ok = true;
-            if (r < 0) {
-                if (p.periodic) r += n; else ok = false;
-            } else if (r >= n) {
-                if (p.periodic) r -= n; else ok = false;
+        for (int t = -M; t < CL + M; t++) {
+            T val = T(0);
+            if (valid && t < len + M) {
+                int r = s + t;
+                bool ok = true;
+                if (r < 0) {
+                    if (p.periodic) r += n; else ok = false;
+                } else if (r >= n) {
+                    if (p.periodic) r -= n; else ok = false;
+                }
+                if (ok) val = fl[r * sd];
             }
-            if (ok) val = fl[r * sd];
+            fv[t + M] = val;
         }
-        fv[t + M] = val;
+    } else {
+        // lanes of one wave own different chunks, so branching per load
+        // would diverge: every load is unconditional, from the wrapped row
+        // clamped into the line (and the last lane of the row set for lanes
+        // past it), and its value selected afterwards
+        const T *fa = f + (int64_t)(valid ? xi : nlane - 1) + ob * so;
+#pragma unroll
+        for (int t = -M; t < CL + M; t++) {
+            int r = s + t;
+            bool ok = valid && t < len + M;
+            if (r < 0) {
+                ok = ok && p.periodic;
+                r += n;
+            } else if (r >= n) {
+                ok = ok && p.periodic;
+                r -= n;
+            }
+            const int rc = r < 0 ? 0 : (r > n - 1 ? n - 1 : r);
+            const T val = fa[(int64_t)rc * sd];
+            fv[t + M] = ok ? val : T(0);
+        }
     }
     T fe[8];
 #pragma unroll

@@ -72,15 +72,17 @@ __global__ void __launch_bounds__(64) k_line_rag(RagLines r, const T *__restrict
         const int n = r.plans[inst].n;
         const T *src = f + r.off[inst];
         T *row = tile + lj * TS;
-        for (int x0 = 0; x0 < n; x0 += 512) {
-            T v[8];
+        // a line of up to 64 CL points in one round of CL loads per lane (the
+        // registers of the sweeps are not live yet)
+        for (int x0 = 0; x0 < n; x0 += 64 * CL) {
+            T v[CL];
 #pragma unroll
-            for (int i = 0; i < 8; i++) {
+            for (int i = 0; i < CL; i++) {
                 const int x = x0 + lane + 64 * i;
                 v[i] = x < n ? src[x] : T(0);
             }
 #pragma unroll
-            for (int i = 0; i < 8; i++) {
+            for (int i = 0; i < CL; i++) {
                 const int x = x0 + lane + 64 * i;
                 if (x < n) row[rag_slot<CL>(x)] = v[i];
             }
@@ -144,18 +146,21 @@ __global__ void __launch_bounds__(64) k_line_rag(RagLines r, const T *__restrict
         }
         if (r.dt) ej.dt = r.dt[ij];
         const T *row = tile + lj * TS;
-        for (int x0 = 0; x0 < nj; x0 += 512) {
-            int64_t o[8];
-            T v[8];
-            bool ok[8];
+        // at capacity 32, 16 points per lane per batch (the sweeps' registers
+        // are free here; at capacity 16 that would raise the allocation)
+        constexpr int NB = CL > 16 ? 16 : 8;
+        for (int x0 = 0; x0 < nj; x0 += 64 * NB) {
+            int64_t o[NB];
+            T v[NB];
+            bool ok[NB];
 #pragma unroll
-            for (int i = 0; i < 8; i++) {
+            for (int i = 0; i < NB; i++) {
                 const int x = x0 + lane + 64 * i;
                 ok[i] = x < nj;
                 o[i] = oj + (ok[i] ? x : 0);
                 v[i] = ok[i] ? row[rag_slot<CL>(x)] : T(0);
             }
-            epilogue_batch<T, EPI, 8>(ej, o, v, ok);
+            epilogue_batch<T, EPI, NB>(ej, o, v, ok);
         }
     }
 }
