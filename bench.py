@@ -32,7 +32,7 @@ HBM_PEAK_GBPS = 8000.0
 # the kernel with the largest share of the step's kernel time in each form
 # (rocprofv3 --stats), with the solver phase that launches it
 DOMINANT = {"fp64": "fluxes", "fp32": "line_div_lsrk_z"}
-DOMINANT_KERNEL = {"fp64": "cles::k_fluxes<double, true>", "fp32": "cles::k_line_yz<float, 1, 2, 0, 3, true, 32, false>"}
+DOMINANT_KERNEL = {"fp64": "cles::k_fluxes<double, true, true>", "fp32": "cles::k_line_yz<float, 1, 2, 0, 3, true, 32, false>"}
 
 
 def tgv_problem(ch, n, precision):
