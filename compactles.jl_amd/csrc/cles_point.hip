@@ -598,12 +598,12 @@ __device__ __forceinline__ void flux_store(W *p, W v)
 }
 
 // MS: the species fluxes and enthalpy diffusion are compiled in (used when
-// Ns > 1); the instantiation without them takes half the registers
-template <typename W, bool NT, bool MS>
-__global__ void __launch_bounds__(TPB) k_fluxes(FluxArgs<W> a, int64_t N)
+// Ns > 1); the instantiation without them takes half the registers.  SPM:
+// the bound of the species loops (CLES_MAXSP, or 2 for two-species mixtures:
+// the same operations on fewer registers).  IG: no NASA-9 tables.
+template <typename W, bool NT, bool MS, int SPM, bool IG = false>
+__device__ __forceinline__ void fluxes_point(const FluxArgs<W> &a, int64_t N, int64_t I)
 {
-    const int64_t I = (int64_t)blockIdx.x * TPB + threadIdx.x;
-    if (I >= N) return;
     const int Ns = a.Ns;
     const W rho = a.rho[I];
     const W uv[3] = {a.u[I], a.v[I], a.w[I]};
@@ -630,20 +630,20 @@ __global__ void __launch_bounds__(TPB) k_fluxes(FluxArgs<W> a, int64_t N)
     for (int d = 0; d < 3; d++) {
         if (!a.active[d]) continue;
         const W ud = uv[d];
-        W **F = a.flux[d];
+        W *const *F = a.flux[d];
         W hdiff = 0;
         if (MS && Ns > 1) {
             // rhs.jl:544 species_enthalpy: cp_k T (IdealMixture) or h_k(T) (Nasa9Mixture)
-            W hk[CLES_MAXSP];
+            W hk[SPM];
 #pragma unroll
-            for (int sp = 0; sp < CLES_MAXSP; sp++)
-                if (sp < Ns) hk[sp] = a.n9 ? (W)n9_h(*a.n9, sp, (double)Tp) : (W)a.cpk[sp] * Tp;
+            for (int sp = 0; sp < SPM; sp++)
+                if (sp < Ns) hk[sp] = (!IG && a.n9) ? (W)n9_h(*a.n9, sp, (double)Tp) : (W)a.cpk[sp] * Tp;
             W Vc = 0;
 #pragma unroll
-            for (int sp = 0; sp < CLES_MAXSP; sp++)
+            for (int sp = 0; sp < SPM; sp++)
                 if (sp < Ns) Vc += (D0 + a.D_art[sp][I]) * a.gY[d][sp][I];
 #pragma unroll
-            for (int sp = 0; sp < CLES_MAXSP; sp++)
+            for (int sp = 0; sp < SPM; sp++)
                 if (sp < Ns) {
                     const W Dk = D0 + a.D_art[sp][I];
                     const W J = rho * (-Dk * a.gY[d][sp][I] + a.Y[sp][I] * Vc);
@@ -661,6 +661,23 @@ __global__ void __launch_bounds__(TPB) k_fluxes(FluxArgs<W> a, int64_t N)
         flux_store<NT>(F[Ns + 3] + I, (E + p) * ud - (uv[0] * tau[d][0] + uv[1] * tau[d][1] + uv[2] * tau[d][2]) -
                        kappa * dead_load<NT>(a.gT[d] + I) + hdiff);
     }
+}
+
+template <typename W, bool NT, bool MS>
+__global__ void __launch_bounds__(TPB) k_fluxes(FluxArgs<W> a, int64_t N)
+{
+    const int64_t I = (int64_t)blockIdx.x * TPB + threadIdx.x;
+    if (I >= N) return;
+    fluxes_point<W, NT, MS, CLES_MAXSP>(a, N, I);
+}
+
+// two-species ideal-gas mixtures
+template <typename W, bool NT>
+__global__ void __launch_bounds__(TPB) k_fluxes_two(FluxArgs<W> a, int64_t N)
+{
+    const int64_t I = (int64_t)blockIdx.x * TPB + threadIdx.x;
+    if (I >= N) return;
+    fluxes_point<W, NT, true, 2, true>(a, N, I);
 }
 
 // timestep.jl:85-94 (standalone form, used when the RHS needs boundary corrections)
@@ -1559,7 +1576,12 @@ void launch_fluxes(const FluxArgs<W> &a, int64_t N, hipStream_t st)
     const bool nt = (double)N * sizeof(W) * (a.Ns + 4) * nact > 256.0 * 1024 * 1024;
     // one species: the lean instantiation in FP32 (-4 % at 256^3); in FP64 the
     // full one measured 2 % faster (same results: Ns is checked at run time)
-    if (a.Ns > 1 || sizeof(W) == 8) {
+    if (a.Ns == 2 && !a.n9) {
+        if (nt)
+            hipLaunchKernelGGL((k_fluxes_two<W, true>), dim3(nblocks(N)), dim3(TPB), 0, st, a, N);
+        else
+            hipLaunchKernelGGL((k_fluxes_two<W, false>), dim3(nblocks(N)), dim3(TPB), 0, st, a, N);
+    } else if (a.Ns > 1 || sizeof(W) == 8) {
         if (nt)
             hipLaunchKernelGGL((k_fluxes<W, true, true>), dim3(nblocks(N)), dim3(TPB), 0, st, a, N);
         else
