@@ -895,7 +895,8 @@ __global__ void __launch_bounds__(TPB) k_metric_sources(CurvArgs<W> a)
     }
 }
 
-template <typename W>
+// CART: a Cartesian grid without stretching (the metric code compiled out)
+template <typename W, bool CART = false>
 __device__ __forceinline__ double rate_at(const RateArgs<W> &a, int64_t I, int64_t N, double &rQ)
 {
     rQ = 0.0;
@@ -909,7 +910,7 @@ __device__ __forceinline__ double rate_at(const RateArgs<W> &a, int64_t I, int64
     for (int d = 0; d < 3; d++) {
         if (!a.active[d]) continue;
         double idx = a.invh[d];
-        if (metric_on(a.met, d)) {   // timestep.jl:151 inv_h[d][I] / h[d]
+        if (!CART && metric_on(a.met, d)) {   // timestep.jl:151 inv_h[d][I] / h[d]
             int64_t ii[3];
             interior_ijk(a.met.n, I, ii);
             idx = (1 / metric_h(a.met, d, ii)) / a.h[d];
@@ -917,7 +918,7 @@ __device__ __forceinline__ double rate_at(const RateArgs<W> &a, int64_t I, int64
         acc += (fabs(uv[d]) + c) * idx;
         dsum += idx * idx;
     }
-    if (a.met.kind != CLES_METRIC_CARTESIAN) {   // timestep.jl:186-197 curvature_rate
+    if (!CART && a.met.kind != CLES_METRIC_CARTESIAN) {   // timestep.jl:186-197 curvature_rate
         int64_t ii[3];
         interior_ijk(a.met.n, I, ii);
         const double ir = metric_inv_r(a.met, ii);
@@ -979,7 +980,8 @@ __global__ void __launch_bounds__(TPB) k_rate(RateArgs<W> a, double *__restrict_
     }
 }
 
-// one block per instance of a batch (invh of the instance's own grid)
+// one block per instance of a batch (invh of the instance's own grid; batch
+// instances are Cartesian and unstretched)
 template <typename W>
 __global__ void __launch_bounds__(TPB) k_rate_batch(RateArgs<W> a, BatchGeom g, const int *__restrict__ mask,
                                                     double *__restrict__ rate_out, double *__restrict__ rmin_out)
@@ -993,7 +995,7 @@ __global__ void __launch_bounds__(TPB) k_rate_batch(RateArgs<W> a, BatchGeom g, 
     double rate = 0.0, rmin = INFINITY;
     for (int x = threadIdx.x; x < n; x += TPB) {
         double rQ;
-        const double acc = rate_at(a, o + x, g.N, rQ);
+        const double acc = rate_at<W, true>(a, o + x, g.N, rQ);
         rmin = rQ < rmin ? rQ : (rQ != rQ ? rQ : rmin);
         rate = acc > rate ? acc : (acc != acc ? acc : rate);
     }
