@@ -673,7 +673,7 @@ __device__ __forceinline__ double corrected(const LineDev &p, int g, T y, const 
 // constant is wave-uniform (scalar loads).
 // ---------------------------------------------------------------------------
 template <typename T, int Q, int M, int SYM, int EPI, bool UNI, int CL = MC, bool NTS = false>
-__global__ void __launch_bounds__(UNI && Q == 1 ? 1024 : 512, UNI ? 1 : 3) k_line_yz(LineDev p, const T *__restrict__ f, int sd, int64_t so,
+__global__ void __launch_bounds__(UNI && Q == 1 ? 1024 : 512, UNI ? (Q == 2 ? 4 : 1) : 3) k_line_yz(LineDev p, const T *__restrict__ f, int sd, int64_t so,
                                                   int nlane, int nxb, int Lx, EpiArgs e, LineFields mf)
 {
     select_field<T, EPI>(mf, f, e);
