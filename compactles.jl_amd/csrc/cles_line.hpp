@@ -762,18 +762,21 @@ __global__ void __launch_bounds__(UNI && Q == 1 ? 1024 : 512, UNI ? (Q == 2 ? 4 
         for (int t = 0; t < CL; t++)
             if (t < len) ch.d[t] = (T)corrected<T, Q>(p, s + t, ch.d[t], tp, hn);
     }
+    // the device-dt LSRK epilogue (per-point gate and dt beside three
+    // inputs) goes four points at a time at the full capacity (no spills)
+    constexpr int NB = EPI == EPI_LSRK_DEV && CL > 16 ? 4 : 8;
 #pragma unroll
-    for (int b = 0; b < CL; b += 8) {
-        int64_t o[8];
-        T v[8];
-        bool ok[8];
+    for (int b = 0; b < CL; b += NB) {
+        int64_t o[NB];
+        T v[NB];
+        bool ok[NB];
 #pragma unroll
-        for (int i = 0; i < 8; i++) {
+        for (int i = 0; i < NB; i++) {
             ok[i] = b + i < len;
             o[i] = off + (int64_t)(s + b + i) * sd;
             v[i] = ch.d[b + i];
         }
-        epilogue_batch<T, EPI, 8, NTS>(e, o, v, ok);
+        epilogue_batch<T, EPI, NB, NTS>(e, o, v, ok);
     }
 }
 
@@ -980,7 +983,7 @@ template <int XC = XCL>
 __device__ __forceinline__ int xw_slot(int x) { return x + x / XC; }
 
 template <typename T, int Q, int M, int SYM, int EPI, bool UNIROW, int XC>
-__global__ void __launch_bounds__(64) k_line_xw(LineDev p, const T *__restrict__ f, int64_t sy, int64_t sz, int ny,
+__global__ void __launch_bounds__(64, 4) k_line_xw(LineDev p, const T *__restrict__ f, int64_t sy, int64_t sz, int ny,
                                                 int64_t nlines, int TS, EpiArgs e, LineFields mf)
 {
     select_field<T, EPI>(mf, f, e);
