@@ -579,7 +579,10 @@ struct RowsClamped {
 // Reduced interface system: block PCR across the P chunks of each line
 // (slot = chunk * nl + line-in-block), then fetch the previous chunk's tail
 // and the next chunk's head.  Every thread of the block must call this.
-template <int Q>
+// PF: each step's coefficient blocks are loaded one step ahead, before the
+// exchange through LDS (the barriers wait for LDS traffic only), so their
+// latency overlaps the exchange; for per-lane chunk indices (the x sweep).
+template <int Q, bool PF = false>
 __device__ __forceinline__ void reduced_solve(const LineDev &p, double *buf, int NT, int nl, int k, int li,
                                               double (&r)[2 * Q], double (&tp)[Q], double (&hn)[Q])
 {
@@ -587,37 +590,90 @@ __device__ __forceinline__ void reduced_solve(const LineDev &p, double *buf, int
     const int P = p.P;
     const int slot = k * nl + li;
     const double *pc = p.pcr;
-    for (int st = 0, dd = 1; st < p.nsteps; st++, dd <<= 1) {
-        double *bf = buf + (st & 1) * B * NT;
+    if constexpr (PF) {
+        constexpr int C = 2 * B * B;
+        // one PCR step: multiplies by the blocks in cur and loads the next
+        // step's into nxt (this step's again on the last step: in bounds, unused)
+        auto step = [&](int st, double (&cur)[C], double (&nxt)[C]) {
+            const int dd = 1 << st;
+            const int sn = st + 1 < p.nsteps ? st + 1 : st;
+            const double *an = pc + ((size_t)(sn * P + k) * 2) * B * B;
 #pragma unroll
-        for (int c = 0; c < B; c++) bf[c * NT + slot] = r[c];
-        __syncthreads();
-        int km = k - dd, kp = k + dd;
-        bool hm = true, hp = true;
-        if (p.periodic) {
-            km &= (P - 1);
-            kp &= (P - 1);
-        } else {
-            hm = km >= 0;
-            hp = kp < P;
-        }
-        double rm[B], rp[B];
+            for (int c = 0; c < C; c++) nxt[c] = gload(an + c);
+            double *bf = buf + (st & 1) * B * NT;
 #pragma unroll
-        for (int c = 0; c < B; c++) {
-            rm[c] = hm ? bf[c * NT + km * nl + li] : 0.0;
-            rp[c] = hp ? bf[c * NT + kp * nl + li] : 0.0;
-        }
-        const double *al = pc + ((size_t)(st * P + k) * 2) * B * B;
-        const double *be = al + B * B;
-#pragma unroll
-        for (int i = 0; i < B; i++) {
-            double acc = r[i];
-#pragma unroll
-            for (int j = 0; j < B; j++) {
-                acc = fma(al[i * B + j], rm[j], acc);
-                acc = fma(be[i * B + j], rp[j], acc);
+            for (int c = 0; c < B; c++) bf[c * NT + slot] = r[c];
+            __syncthreads();
+            int km = k - dd, kp = k + dd;
+            bool hm = true, hp = true;
+            if (p.periodic) {
+                km &= (P - 1);
+                kp &= (P - 1);
+            } else {
+                hm = km >= 0;
+                hp = kp < P;
             }
-            r[i] = acc;
+            double rm[B], rp[B];
+#pragma unroll
+            for (int c = 0; c < B; c++) {
+                rm[c] = hm ? bf[c * NT + km * nl + li] : 0.0;
+                rp[c] = hp ? bf[c * NT + kp * nl + li] : 0.0;
+            }
+#pragma unroll
+            for (int i = 0; i < B; i++) {
+                double acc = r[i];
+#pragma unroll
+                for (int j = 0; j < B; j++) {
+                    acc = fma(cur[i * B + j], rm[j], acc);
+                    acc = fma(cur[B * B + i * B + j], rp[j], acc);
+                }
+                r[i] = acc;
+            }
+        };
+        double ca[C], cb[C];
+        if (p.nsteps > 0) {
+            const double *a0 = pc + (size_t)k * 2 * B * B;
+#pragma unroll
+            for (int c = 0; c < C; c++) ca[c] = gload(a0 + c);
+        }
+        // two steps per trip, so the prefetch buffers alternate without copies
+        for (int st = 0; st < p.nsteps; st += 2) {
+            step(st, ca, cb);
+            if (st + 1 < p.nsteps) step(st + 1, cb, ca);
+        }
+    } else {
+        for (int st = 0, dd = 1; st < p.nsteps; st++, dd <<= 1) {
+            double *bf = buf + (st & 1) * B * NT;
+#pragma unroll
+            for (int c = 0; c < B; c++) bf[c * NT + slot] = r[c];
+            __syncthreads();
+            int km = k - dd, kp = k + dd;
+            bool hm = true, hp = true;
+            if (p.periodic) {
+                km &= (P - 1);
+                kp &= (P - 1);
+            } else {
+                hm = km >= 0;
+                hp = kp < P;
+            }
+            double rm[B], rp[B];
+#pragma unroll
+            for (int c = 0; c < B; c++) {
+                rm[c] = hm ? bf[c * NT + km * nl + li] : 0.0;
+                rp[c] = hp ? bf[c * NT + kp * nl + li] : 0.0;
+            }
+            const double *al = pc + ((size_t)(st * P + k) * 2) * B * B;
+            const double *be = al + B * B;
+#pragma unroll
+            for (int i = 0; i < B; i++) {
+                double acc = r[i];
+#pragma unroll
+                for (int j = 0; j < B; j++) {
+                    acc = fma(al[i * B + j], rm[j], acc);
+                    acc = fma(be[i * B + j], rp[j], acc);
+                }
+                r[i] = acc;
+            }
         }
     }
     const double *mi = pc + (size_t)p.nsteps * P * 2 * B * B + (size_t)k * B * B;
@@ -753,7 +809,7 @@ __global__ void __launch_bounds__(UNI && Q == 1 ? 1024 : 512, UNI ? (Q == 2 ? 4 
         ch.solve(p, s, len);
     double r[2 * Q], tp[Q], hn[Q];
     ch.ends(len, r);
-    reduced_solve<Q>(p, smem, NT, Lx, k, lane, r, tp, hn);
+    reduced_solve<Q, !UNI && Q == 1 && sizeof(T) == 8>(p, smem, NT, Lx, k, lane, r, tp, hn);
     if (!valid) return;
     if constexpr (GRP) {
         correct_grouped<T, Q, CL, G>(ch.d, len, RowsClamped<double>{p.row64, n, s}, tp, hn);
@@ -786,7 +842,7 @@ __global__ void __launch_bounds__(UNI && Q == 1 ? 1024 : 512, UNI ? (Q == 2 ? 4 
 // l = j + ny*k over the y-z plane.  UNI: Lt == 64 (one chunk per wave).
 // ---------------------------------------------------------------------------
 template <typename T, int Q, int M, int SYM, int EPI, bool UNI, int CL = MC>
-__global__ void __launch_bounds__(UNI && Q == 1 ? 1024 : 512) k_line_x(LineDev p, const T *__restrict__ f, int64_t sy, int64_t sz,
+__global__ void __launch_bounds__(UNI && Q == 1 ? 1024 : 512, !UNI && CL <= 8 && Q == 1 ? 5 : 1) k_line_x(LineDev p, const T *__restrict__ f, int64_t sy, int64_t sz,
                                                  int ny, int64_t nlines, int Lt, EpiArgs e, LineFields mf)
 {
     select_field<T, EPI>(mf, f, e);
@@ -871,7 +927,7 @@ __global__ void __launch_bounds__(UNI && Q == 1 ? 1024 : 512) k_line_x(LineDev p
         ch.solve(p, s, len);
     double r[2 * Q], tp[Q], hn[Q];
     ch.ends(len, r);
-    reduced_solve<Q>(p, buf, NT, Lt, k, li, r, tp, hn);
+    reduced_solve<Q, !UNI && Q == 1>(p, buf, NT, Lt, k, li, r, tp, hn);
     // every thread has finished reading the tile before the first reduced-solve barrier
     T *wrow = tile + li * TS;
     if constexpr (GRP) {
