@@ -961,44 +961,100 @@ __global__ void __launch_bounds__(UNI && Q == 1 ? 1024 : 512, !UNI && CL <= 8 &&
 // ---------------------------------------------------------------------------
 constexpr int XCL = 16;
 
-template <int Q>
+// PF: each step's coefficient blocks are loaded one step ahead (per-lane
+// chunk indices kc, whose blocks are vector loads), so their latency overlaps
+// the previous step's shuffles and products.
+template <int Q, bool PF = false>
 __device__ __forceinline__ void reduced_solve_wave(const LineDev &p, int k, int kc, int base, double (&r)[2 * Q],
                                                    double (&tp)[Q], double (&hn)[Q])
 {
     constexpr int B = 2 * Q;
     const int P = p.P;
     const double *pc = p.pcr;
-    for (int st = 0, dd = 1; st < p.nsteps; st++, dd <<= 1) {
-        int km = k - dd, kp = k + dd;
-        bool hm = true, hp = true;
-        if (p.periodic) {
-            km &= (P - 1);
-            kp &= (P - 1);
-        } else {
-            hm = km >= 0;
-            hp = kp < P;
-            km = hm ? km : k;
-            kp = hp ? kp : k;
-        }
-        double rm[B], rp[B];
+    if constexpr (PF) {
+        constexpr int C = 2 * B * B;
+        // one PCR step: multiplies by the blocks in cur and loads the next
+        // step's into nxt (this step's again on the last step: in bounds, unused)
+        auto step = [&](int st, double (&cur)[C], double (&nxt)[C]) {
+            const int dd = 1 << st;
+            const int sn = st + 1 < p.nsteps ? st + 1 : st;
+            const double *an = pc + ((size_t)(sn * P + kc) * 2) * B * B;
 #pragma unroll
-        for (int c = 0; c < B; c++) {
-            rm[c] = __shfl(r[c], base + km, 64);
-            rp[c] = __shfl(r[c], base + kp, 64);
-            if (!hm) rm[c] = 0.0;
-            if (!hp) rp[c] = 0.0;
-        }
-        const double *al = pc + ((size_t)(st * P + kc) * 2) * B * B;
-        const double *be = al + B * B;
-#pragma unroll
-        for (int i = 0; i < B; i++) {
-            double acc = r[i];
-#pragma unroll
-            for (int j = 0; j < B; j++) {
-                acc = fma(gload(al + i * B + j), rm[j], acc);
-                acc = fma(gload(be + i * B + j), rp[j], acc);
+            for (int c = 0; c < C; c++) nxt[c] = gload(an + c);
+            int km = k - dd, kp = k + dd;
+            bool hm = true, hp = true;
+            if (p.periodic) {
+                km &= (P - 1);
+                kp &= (P - 1);
+            } else {
+                hm = km >= 0;
+                hp = kp < P;
+                km = hm ? km : k;
+                kp = hp ? kp : k;
             }
-            r[i] = acc;
+            double rm[B], rp[B];
+#pragma unroll
+            for (int c = 0; c < B; c++) {
+                rm[c] = __shfl(r[c], base + km, 64);
+                rp[c] = __shfl(r[c], base + kp, 64);
+                if (!hm) rm[c] = 0.0;
+                if (!hp) rp[c] = 0.0;
+            }
+#pragma unroll
+            for (int i = 0; i < B; i++) {
+                double acc = r[i];
+#pragma unroll
+                for (int j = 0; j < B; j++) {
+                    acc = fma(cur[i * B + j], rm[j], acc);
+                    acc = fma(cur[B * B + i * B + j], rp[j], acc);
+                }
+                r[i] = acc;
+            }
+        };
+        double ca[C], cb[C];
+        if (p.nsteps > 0) {
+            const double *a0 = pc + (size_t)kc * 2 * B * B;
+#pragma unroll
+            for (int c = 0; c < C; c++) ca[c] = gload(a0 + c);
+        }
+        // two steps per trip, so the prefetch buffers alternate without copies
+        for (int st = 0; st < p.nsteps; st += 2) {
+            step(st, ca, cb);
+            if (st + 1 < p.nsteps) step(st + 1, cb, ca);
+        }
+    } else {
+        for (int st = 0, dd = 1; st < p.nsteps; st++, dd <<= 1) {
+            int km = k - dd, kp = k + dd;
+            bool hm = true, hp = true;
+            if (p.periodic) {
+                km &= (P - 1);
+                kp &= (P - 1);
+            } else {
+                hm = km >= 0;
+                hp = kp < P;
+                km = hm ? km : k;
+                kp = hp ? kp : k;
+            }
+            double rm[B], rp[B];
+#pragma unroll
+            for (int c = 0; c < B; c++) {
+                rm[c] = __shfl(r[c], base + km, 64);
+                rp[c] = __shfl(r[c], base + kp, 64);
+                if (!hm) rm[c] = 0.0;
+                if (!hp) rp[c] = 0.0;
+            }
+            const double *al = pc + ((size_t)(st * P + kc) * 2) * B * B;
+            const double *be = al + B * B;
+#pragma unroll
+            for (int i = 0; i < B; i++) {
+                double acc = r[i];
+#pragma unroll
+                for (int j = 0; j < B; j++) {
+                    acc = fma(gload(al + i * B + j), rm[j], acc);
+                    acc = fma(gload(be + i * B + j), rp[j], acc);
+                }
+                r[i] = acc;
+            }
         }
     }
     const double *mi = pc + (size_t)p.nsteps * P * 2 * B * B + (size_t)kc * B * B;
@@ -1125,7 +1181,7 @@ __global__ void __launch_bounds__(64, 4) k_line_xw(LineDev p, const T *__restric
         solve_grouped<T, Q, XC, 8>(ch.d, len, RowsClamped<T>{rowp<T>(p), n, rs});
         double r[2 * Q], tp[Q], hn[Q];
         ch.ends(len, r);
-        reduced_solve_wave<Q>(p, k, kc, base, r, tp, hn);
+        reduced_solve_wave<Q, !UNIROW && Q == 1 && sizeof(T) == 8>(p, k, kc, base, r, tp, hn);
         correct_grouped<T, Q, XC, 8>(ch.d, len, RowsClamped<double>{p.row64, n, rs}, tp, hn);
 #pragma unroll
         for (int t = 0; t < XC; t++) res[t] = ch.d[t];

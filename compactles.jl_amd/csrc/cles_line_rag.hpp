@@ -123,7 +123,7 @@ __global__ void __launch_bounds__(64) k_line_rag(RagLines r, const T *__restrict
     solve_grouped<T, Q, CL, 8>(ch.d, len, RowsRag<T>{rowtp<T>(p) + k, p.P});
     double ends[2 * Q], tp[Q], hn[Q];
     ch.ends(len, ends);
-    reduced_solve_wave<Q>(p, k, k, li * P, ends, tp, hn);
+    reduced_solve_wave<Q, Q == 1>(p, k, k, li * P, ends, tp, hn);
     correct_grouped<T, Q, CL, 8>(ch.d, len, RowsRag<double>{p.rowt64 + k, p.P}, tp, hn);
     __syncthreads();   // every window read is done before the tile is overwritten
     if (live) {
