@@ -30,8 +30,8 @@ struct PrimArgs {
     const N9Table *n9;   // Nasa9Mixture (NULL: IdealMixture)
     int sg;              // StiffenedGas (one species): gamma, p_inf, cv
     double sg_gamma, sg_pinf, sg_cv;
-    W *rho, *u, *v, *w, *p, *T, *c, *cp;
-    W *Y[CLES_MAXSP];
+    W *rho, *u, *v, *w, *p, *T, *c, *cp;   // ideal gas: cp may be NULL (not stored)
+    W *Y[CLES_MAXSP];                       // ideal gas: Y[k] may be NULL (not stored)
     W *esens;   // optional: internal energy of the kappa* sensor (artificial.jl:319-326)
 };
 
@@ -66,6 +66,7 @@ struct FluxArgs {
     int active[3];
     double mu0, Pr, Sc;
     double cpk[CLES_MAXSP];
+    double R1, cv1;      // one-species ideal gas: cp and Y_1 formed from rho where NULL
     const N9Table *n9;   // Nasa9Mixture: species enthalpies h_k(T) (NULL: cp_k T)
     const W *rho, *u, *v, *w, *p, *T, *cp;
     const W *mu_art, *beta_art, *kappa_art;
@@ -139,6 +140,7 @@ struct RateArgs {
     double h[3];             // spacing of each dimension (stretched: computational)
     Metric1D met;            // stretched dimensions use inv_h[d] / h[d] (timestep.jl:151)
     double mu0, Pr, Sc;
+    double R1, cv1;          // one-species ideal gas: cp formed from rho where cp is NULL
     const W *rho, *u, *v, *w, *c, *cp;
     const W *mu_art, *beta_art, *kappa_art;
     const W *D_art[CLES_MAXSP];

@@ -103,6 +103,20 @@ __device__ __forceinline__ void prim_store(W *p, W v)
         *p = v;
 }
 
+// One-species ideal gas: the cp and Y_1 that k_primitives stores at a point
+// of density rho > 0, formed by the same operations in the same order (with
+// one species its sums over species start from 0 and take one term), so the
+// kernels that read them get the same bits when they are not stored.
+__device__ __forceinline__ void one_species_cp_y(double rho, double R1, double cv1, double &cp, double &Y)
+{
+    const double ri = 1 / rho;
+    double Rm = rho * R1, cvm = rho * cv1;
+    Rm *= ri;
+    cvm *= ri;
+    cp = cvm + Rm;
+    Y = rho * ri;
+}
+
 // IG: an ideal-gas mixture (no stiffened gas, no NASA-9 tables); that
 // instantiation compiles the other branches out, which frees registers
 template <typename W, bool NT, bool IG>
@@ -194,7 +208,7 @@ __global__ void __launch_bounds__(TPB) k_primitives(PrimArgs<W> a, int64_t N)
                 const double q = a.Q[(int64_t)sp * N + I];
                 Rm += q * a.Rk[sp];
                 cvm += q * a.cvk[sp];
-                prim_store<NT>(&a.Y[sp][I], (W)(q * ri));
+                if (a.Y[sp]) prim_store<NT>(&a.Y[sp][I], (W)(q * ri));
             }
         Rm *= ri;
         cvm *= ri;
@@ -220,7 +234,7 @@ __global__ void __launch_bounds__(TPB) k_primitives(PrimArgs<W> a, int64_t N)
         prim_store<NT>(&a.p[I], (W)p);
         prim_store<NT>(&a.T[I], (W)T);
         prim_store<NT>(&a.c[I], (W)sqrt(gm * Rm * T));
-        prim_store<NT>(&a.cp[I], (W)(cvm + Rm));
+        if (a.cp) prim_store<NT>(&a.cp[I], (W)(cvm + Rm));
     } else {
         if (a.esens) {
             const double m1 = a.Q[(int64_t)Ns * N + I], m2 = a.Q[(int64_t)(Ns + 1) * N + I],
@@ -229,10 +243,11 @@ __global__ void __launch_bounds__(TPB) k_primitives(PrimArgs<W> a, int64_t N)
             prim_store<NT>(&a.esens[I], (W)((a.Q[(int64_t)(Ns + 3) * N + I] - ke) / 1.0));
         }
         prim_store<NT>(&a.rho[I], (W)(1)); prim_store<NT>(&a.u[I], (W)(0)); prim_store<NT>(&a.v[I], (W)(0)); prim_store<NT>(&a.w[I], (W)(0));
-        prim_store<NT>(&a.p[I], (W)(1)); prim_store<NT>(&a.T[I], (W)(1)); prim_store<NT>(&a.c[I], (W)(1)); prim_store<NT>(&a.cp[I], (W)(1));
+        prim_store<NT>(&a.p[I], (W)(1)); prim_store<NT>(&a.T[I], (W)(1)); prim_store<NT>(&a.c[I], (W)(1));
+        if (a.cp) prim_store<NT>(&a.cp[I], (W)(1));
 #pragma unroll
         for (int sp = 0; sp < CLES_MAXSP; sp++)
-            if (sp < Ns) prim_store<NT>(&a.Y[sp][I], (W)(sp == 0 ? W(1) : W(0)));
+            if (sp < Ns && a.Y[sp]) prim_store<NT>(&a.Y[sp][I], (W)(sp == 0 ? W(1) : W(0)));
     }
 }
 
@@ -612,7 +627,17 @@ __device__ __forceinline__ void fluxes_point(const FluxArgs<W> &a, int64_t N, in
     const W mu0 = (W)a.mu0;
     const W mu = mu0 + (a.mu_art ? a.mu_art[I] : W(0));
     const W beta = a.beta_art ? a.beta_art[I] : W(0);
-    const W kappa = mu0 * a.cp[I] / (W)a.Pr + (a.kappa_art ? a.kappa_art[I] : W(0));
+    // cp and Y_1 are NULL together (one-species ideal gas: formed from rho)
+    W cpm, Y1c = W(1);
+    if (a.cp) {
+        cpm = a.cp[I];
+    } else {
+        double c, y;
+        one_species_cp_y((double)rho, a.R1, a.cv1, c, y);
+        cpm = (W)c;
+        Y1c = (W)y;
+    }
+    const W kappa = mu0 * cpm / (W)a.Pr + (a.kappa_art ? a.kappa_art[I] : W(0));
     const W D0 = mu0 / ((W)a.Sc * rho);
     W G[9];
 #pragma unroll
@@ -653,7 +678,7 @@ __device__ __forceinline__ void fluxes_point(const FluxArgs<W> &a, int64_t N, in
         } else {
             // one species: sum_k J_k = 0 forces J_1 = 0 (the reference's
             // rho*D0*dY*(Y-1) with Y == 1 to rounding)
-            flux_store<NT>(F[0] + I, rho * a.Y[0][I] * ud);
+            flux_store<NT>(F[0] + I, rho * (a.Y[0] ? a.Y[0][I] : Y1c) * ud);
         }
         flux_store<NT>(F[Ns] + I, rho * ud * uv[0] + (d == 0 ? p : W(0)) - tau[d][0]);
         flux_store<NT>(F[Ns + 1] + I, rho * ud * uv[1] + (d == 1 ? p : W(0)) - tau[d][1]);
@@ -903,7 +928,14 @@ __device__ __forceinline__ double rate_at(const RateArgs<W> &a, int64_t I, int64
     for (int sp = 0; sp < a.Ns; sp++) rQ += a.Q[(int64_t)sp * N + I];
     const double rho = a.rho[I];
     const double ri = 1 / rho;
-    const double c = a.c[I], cp = a.cp[I];
+    const double c = a.c[I];
+    double cp, y;
+    if (CART || a.cp) {   // the Cartesian (batch) form always has cp stored
+        cp = a.cp[I];
+    } else {
+        one_species_cp_y(rho, a.R1, a.cv1, cp, y);
+        cp = (double)(W)cp;
+    }
     const double uv[3] = {(double)a.u[I], (double)a.v[I], (double)a.w[I]};
     double acc = 0.0, dsum = 0.0;
 #pragma unroll

@@ -11,6 +11,7 @@
 //    and, when no boundary correction follows, the last direction feeds the
 //    LSRK update (timestep.jl:85-94) without storing the final dQ.
 // Periodic halos are index wraps inside the kernels (halo.jl:27-36).
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -173,6 +174,10 @@ struct cles_solver {
 
     hipStream_t st() const { return ctx->stream; }
     size_t wsize() const { return prec == CLES_F64 ? 8 : 4; }
+    // one-species ideal gas that never runs NSCBC: the primitives do not
+    // store Y_1 and cp (physics.jl:604-653 stores both per point); the fluxes
+    // and the rate form them from rho by the same operations, same bits
+    bool const_cp() const { return Ns == 1 && d.eos_kind == CLES_EOS_IDEAL && !nscbc_any; }
 
     void *alloc(size_t bytes)
     {
@@ -416,8 +421,10 @@ struct Impl {
         a.rho = f(s.rho); a.u = f(s.u); a.v = f(s.v); a.w = f(s.w);
         a.p = f(s.p); a.T = f(s.T); a.c = f(s.c); a.cp = f(s.cp);
         a.esens = f(s.esens);
+        const bool cc = s.const_cp();
+        if (cc) a.cp = a.Y[0] = nullptr;
         const double w = sizeof(W);
-        s.timed(PH_PRIMITIVES, (8.0 * s.C + w * (8 + s.Ns + (s.esens ? 1 : 0))) * (double)s.N,
+        s.timed(PH_PRIMITIVES, (8.0 * s.C + w * ((cc ? 7 : 8 + s.Ns) + (s.esens ? 1 : 0))) * (double)s.N,
                 [&] { launch_primitives<W>(a, s.N, s.st()); });
     }
 
@@ -686,8 +693,12 @@ struct Impl {
             for (int k = 0; k < s.Ns; k++) a.gY[d][k] = f(s.gY[d][k]);
             for (int c = 0; c < s.C; c++) a.flux[d][c] = f(s.flux[d][c]);
         }
+        a.R1 = s.Rk[0];
+        a.cv1 = s.cvk[0];
+        const bool cc = s.const_cp();
+        if (cc) a.cp = a.Y[0] = nullptr;
         const double w = sizeof(W);
-        const double reads = 8.0 + w * (10 + 3 * s.nact + s.nact + s.Ns * (s.Ns > 1 ? 2 + s.nact : 1));
+        const double reads = 8.0 + w * ((cc ? 8 : 10 + s.Ns * (s.Ns > 1 ? 2 + s.nact : 1)) + 3 * s.nact + s.nact);
         s.timed(PH_FLUXES, (reads + w * s.nact * s.C) * (double)s.N, [&] { launch_fluxes<W>(a, s.N, s.st()); });
     }
 
@@ -765,13 +776,16 @@ struct Impl {
         a.Pr = s.d.Pr;
         a.Sc = s.d.Sc;
         a.rho = f(s.rho); a.u = f(s.u); a.v = f(s.v); a.w = f(s.w); a.c = f(s.c); a.cp = f(s.cp);
+        a.R1 = s.Rk[0];
+        a.cv1 = s.cvk[0];
+        if (s.const_cp()) a.cp = nullptr;
         a.mu_art = f(s.mu_art);
         a.beta_art = f(s.beta_art);
         a.kappa_art = f(s.kappa_art);
         for (int k = 0; k < s.Ns; k++) a.D_art[k] = f(s.D_art[k]);
         int nb = (int)std::min<int64_t>(s.nred, (s.N + 255) / 256);
         const double w = sizeof(W);
-        s.timed(PH_RATE, (8.0 * s.Ns + w * (9 + (s.Ns > 1 ? s.Ns : 0))) * (double)s.N,
+        s.timed(PH_RATE, (8.0 * s.Ns + w * ((s.const_cp() ? 8 : 9) + (s.Ns > 1 ? s.Ns : 0))) * (double)s.N,
                 [&] { launch_rate<W>(a, s.part, nb, s.scal, s.N, s.st()); });
         CLES_HIP(hipMemcpyAsync(s.host_scal, s.scal, 2 * sizeof(double), hipMemcpyDeviceToHost, s.st()));
         CLES_HIP(hipStreamSynchronize(s.st()));
@@ -804,6 +818,9 @@ struct Impl {
         a.Pr = s.d.Pr;
         a.Sc = s.d.Sc;
         a.rho = f(s.rho); a.u = f(s.u); a.v = f(s.v); a.w = f(s.w); a.c = f(s.c); a.cp = f(s.cp);
+        a.R1 = s.Rk[0];
+        a.cv1 = s.cvk[0];
+        if (s.const_cp()) a.cp = nullptr;
         a.mu_art = f(s.mu_art);
         a.beta_art = f(s.beta_art);
         a.kappa_art = f(s.kappa_art);
@@ -1780,6 +1797,26 @@ int cles_solver_get_field(cles_solver *s, const char *name, double *host)
         else if (std::sscanf(name, "D_art[%d]", &a) == 1 && a >= 1 && a <= s->Ns)
             src = s->D_art[a - 1];
         if (!src) throw cles::Error{CLES_ERR_INVALID, "unknown or unallocated field '" + nm + "'"};
+        if (s->const_cp() && (nm == "cp" || nm == "Y[1]")) {
+            // not stored for a one-species ideal gas: formed from rho as the
+            // primitives would, in the working precision
+            std::vector<double> r((size_t)s->N);
+            CLES_REQUIRE(cles_solver_get_field(s, "rho", r.data()) == CLES_OK, "rho download failed");
+            for (size_t i = 0; i < r.size(); i++) {
+                double cp = 1.0, y = 1.0;
+                if (r[i] > 0) {
+                    const double ri = 1 / r[i];
+                    double Rm = r[i] * s->Rk[0], cvm = r[i] * s->cvk[0];
+                    Rm *= ri;
+                    cvm *= ri;
+                    cp = cvm + Rm;
+                    y = r[i] * ri;
+                }
+                const double v = nm == "cp" ? cp : y;
+                host[i] = s->prec == CLES_F64 ? v : (double)(float)v;
+            }
+            return CLES_OK;
+        }
         std::vector<double> buf((size_t)s->N);
         if (is_double || s->prec == CLES_F64) {
             CLES_HIP(hipMemcpyAsync(buf.data(), src, s->N * sizeof(double), hipMemcpyDeviceToHost, s->st()));
