@@ -46,7 +46,7 @@ def read_traffic(n, precision, kernel):
     path = os.path.join(ROOT, "profiles", f"tgv{n}_{precision}_hbm_counters.md")
     if not os.path.exists(path):
         return None
-    m = re.search(r"traffic_bytes_per_launch\[" + re.escape(kernel) + r"\]:\s*([0-9.eE+]+)", open(path).read())
+    m = re.search(r"(?m)^traffic_bytes_per_launch\[" + re.escape(kernel) + r"\]:\s*([0-9.eE+]+)", open(path).read())
     return float(m.group(1)) if m else None
 
 

@@ -698,7 +698,7 @@ struct Impl {
         const bool cc = s.const_cp();
         if (cc) a.cp = a.Y[0] = nullptr;
         const double w = sizeof(W);
-        const double reads = 8.0 + w * ((cc ? 8 : 10 + s.Ns * (s.Ns > 1 ? 2 + s.nact : 1)) + 3 * s.nact + s.nact);
+        const double reads = 8.0 + w * ((cc ? 9 : 10 + s.Ns * (s.Ns > 1 ? 2 + s.nact : 1)) + 3 * s.nact + s.nact);
         s.timed(PH_FLUXES, (reads + w * s.nact * s.C) * (double)s.N, [&] { launch_fluxes<W>(a, s.N, s.st()); });
     }
 
