@@ -312,7 +312,9 @@ int cles_solver_download(cles_solver *solver, double *Q_host,
 /* Download a named internal field's interior (as double) into an unpadded
  * host array: "rho","u","v","w","p","T","c","cp","mu_art","beta_art",
  * "kappa_art","Q[c]","dQ[c]","du[c]","grad_u[dj]","grad_T[d]","Y[k]","D_art[k]"
- * (1-based). */
+ * (1-based).  For a one-species ideal gas without NSCBC faces, "cp" and "Y[1]"
+ * are not stored on the device; they are formed on the host from "rho" by the
+ * operations the primitives use (same values). */
 int cles_solver_get_field(cles_solver *solver, const char *name, double *host);
 
 int cles_solver_get_time(cles_solver *solver, double *t, int64_t *step);
