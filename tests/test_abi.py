@@ -145,3 +145,13 @@ def test_dirichlet_face_state_must_be_constant():
         SolverSpec(prob(lambda x, y, z, t: ch.Prim(p=1.0 + y, rho=1.0 + 0 * y)), num)
     with pytest.raises(ValueError, match="constant in time"):
         SolverSpec(prob(lambda x, y, z, t: ch.Prim(p=1.0 + t, rho=1.0)), num)
+
+
+def test_bench_traffic_lookup_skips_earlier_build_entries():
+    # counter lines kept from an earlier build sit under another key and must
+    # not be read as the current kernel's traffic
+    sys.path.insert(0, ROOT)
+    import bench
+    assert bench.read_traffic(256, "fp64", "cles::k_fluxes<double, true, true>") is None
+    assert bench.read_traffic(256, "fp64", "cles::k_strain<double>") > 0
+    assert bench.read_traffic(256, "fp32", bench.DOMINANT_KERNEL["fp32"]) > 0
